@@ -1,0 +1,538 @@
+// SIPG operator apply, block diagonal and block-Jacobi smoother kernels (gfx950).
+//
+// Data layout in HBM (element e, nele elements):
+//   DOF vectors        x[e*nloc + n]                       (reference layout, element-major)
+//   geometry (SoA)     G[c*nele + e], cf[(f*D+d)*nele + e], wf/pen[f*nele + e]
+//   face records       nbface[e*nface + f] = N*nface + f' (or -1), fcode = type | alnmt<<4
+//   face traces (SoA)  trace[(f*sngi + g)*nele + e] = (u, grad u . n_f) at face point g
+//   smoother (SoA)     S[(n*nloc + m)*nele + e]
+//
+// The apply is two launches:
+//   k_trace     per element: u and du/dn at the element's own face points (4 faces x 12
+//               points in 3D) -> trace buffer;
+//   k_residual  per element: volume term, then for every face the SIPG flux from the own
+//               trace and the neighbour's trace at the aligned points, lifted to the test
+//               functions; writes r = b - A x, or applies the smoother x += S r.
+// One thread owns one element; the reference tables are indexed with wave-uniform
+// addresses, so they are read through the scalar cache and broadcast.
+#include "dgmi_internal.h"
+
+namespace dgmi {
+
+template <int D> struct Dm;
+template <> struct Dm<3> { static constexpr int NL = 20, NGI = 24, SG = 12, NF = 4, NG = 6; };
+template <> struct Dm<2> { static constexpr int NL = 10, NGI = 12, SG = 4, NF = 3, NG = 3; };
+
+// Vector loads/stores of one element's row: 16-byte accesses when the row length
+// allows (3D P3 in both precisions, 2D in FP64), 8-byte otherwise (2D P3 in FP32).
+// DOF vectors are required to be 16-byte aligned.
+template <typename T, int NL> struct RowVec {
+  static constexpr int W = (NL % (16 / sizeof(T)) == 0) ? 16 / sizeof(T) : 8 / sizeof(T);
+  using V = typename std::conditional<
+      sizeof(T) * W == 16, typename std::conditional<sizeof(T) == 8, double2, float4>::type,
+      typename std::conditional<sizeof(T) == 8, double, float2>::type>::type;
+  static_assert(NL % W == 0, "row length not a multiple of the vector width");
+};
+
+template <typename T, int NL>
+__device__ __forceinline__ void load_row(const T* __restrict__ p, T* v) {
+  using RV = RowVec<T, NL>;
+  constexpr int W = RV::W;
+  const typename RV::V* q = reinterpret_cast<const typename RV::V*>(p);
+#pragma unroll
+  for (int i = 0; i < NL / W; ++i) {
+    typename RV::V t = q[i];
+    const T* tt = reinterpret_cast<const T*>(&t);
+#pragma unroll
+    for (int k = 0; k < W; ++k) v[i * W + k] = tt[k];
+  }
+}
+template <typename T, int NL>
+__device__ __forceinline__ void store_row(T* __restrict__ p, const T* v) {
+  using RV = RowVec<T, NL>;
+  constexpr int W = RV::W;
+  typename RV::V* q = reinterpret_cast<typename RV::V*>(p);
+#pragma unroll
+  for (int i = 0; i < NL / W; ++i) {
+    typename RV::V t;
+    T* tt = reinterpret_cast<T*>(&t);
+#pragma unroll
+    for (int k = 0; k < W; ++k) tt[k] = v[i * W + k];
+    q[i] = t;
+  }
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_trace(int nele, const T* __restrict__ x,
+                                               const T* __restrict__ ftab,
+                                               const T* __restrict__ cf,
+                                               typename Vec2<T>::type* __restrict__ trace) {
+  using M = Dm<D>;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nele) return;
+  T xv[M::NL];
+  load_row<T, M::NL>(x + (size_t)e * M::NL, xv);
+  for (int f = 0; f < M::NF; ++f) {
+    T c[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
+    const T* tf = ftab + f * M::SG * (1 + D) * M::NL;
+#pragma unroll 2
+    for (int g = 0; g < M::SG; ++g) {
+      const T* t = tf + g * (1 + D) * M::NL;
+      T u = 0, gd[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) gd[d] = 0;
+#pragma unroll
+      for (int n = 0; n < M::NL; ++n) {
+        u += t[n] * xv[n];
+#pragma unroll
+        for (int d = 0; d < D; ++d) gd[d] += t[(1 + d) * M::NL + n] * xv[n];
+      }
+      T dn = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) dn += c[d] * gd[d];
+      typename Vec2<T>::type tv;
+      tv.x = u;
+      tv.y = dn;
+      trace[(f * M::SG + g) * nele + e] = tv;
+    }
+  }
+}
+
+// MODE 0: r = -A x; 1: r = b - A x; 2: x += S (b - A x); 3: as 2 and also store r;
+// 4: r = +A x
+template <typename T, int D, int MODE>
+__global__ __launch_bounds__(256) void k_residual(
+    int nele, T* __restrict__ x, const T* __restrict__ b, T* __restrict__ r,
+    const T* __restrict__ vtab, const T* __restrict__ wgt, const T* __restrict__ ftab,
+    const T* __restrict__ sw, const int32_t* __restrict__ aln, const T* __restrict__ G,
+    const T* __restrict__ cf, const T* __restrict__ wf, const T* __restrict__ pen,
+    const int32_t* __restrict__ nbface, const int32_t* __restrict__ fcode,
+    const typename Vec2<T>::type* __restrict__ trace, const T* __restrict__ S) {
+  using M = Dm<D>;
+  constexpr int NL = M::NL;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nele) return;
+  T xv[NL], acc[NL];
+  load_row<T, NL>(x + (size_t)e * NL, xv);
+#pragma unroll
+  for (int n = 0; n < NL; ++n) acc[n] = 0;
+
+  // ---- volume: acc_n += sum_g w_g dphi_n . (kappa detJ invJ^T invJ) grad_xi u
+  T gm[M::NG];
+#pragma unroll
+  for (int k = 0; k < M::NG; ++k) gm[k] = G[k * nele + e];
+#pragma unroll 2
+  for (int g = 0; g < M::NGI; ++g) {
+    const T* t = vtab + g * D * NL;
+    T gd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gd[d] = 0;
+#pragma unroll
+    for (int n = 0; n < NL; ++n)
+#pragma unroll
+      for (int d = 0; d < D; ++d) gd[d] += t[d * NL + n] * xv[n];
+    const T w = wgt[g];
+    T y[D];
+    if constexpr (D == 3) {
+      y[0] = w * (gm[0] * gd[0] + gm[3] * gd[1] + gm[4] * gd[2]);
+      y[1] = w * (gm[3] * gd[0] + gm[1] * gd[1] + gm[5] * gd[2]);
+      y[2] = w * (gm[4] * gd[0] + gm[5] * gd[1] + gm[2] * gd[2]);
+    } else {
+      y[0] = w * (gm[0] * gd[0] + gm[2] * gd[1]);
+      y[1] = w * (gm[2] * gd[0] + gm[1] * gd[1]);
+    }
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      T s = acc[n];
+#pragma unroll
+      for (int d = 0; d < D; ++d) s += t[d * NL + n] * y[d];
+      acc[n] = s;
+    }
+  }
+
+  // ---- faces
+  for (int f = 0; f < M::NF; ++f) {
+    const int z = e * M::NF + f;
+    const int code = fcode[z];
+    const int type = code & 15;
+    if (type >= 2) continue;  // faces without an operator term
+    const int a = code >> 4;
+    T c[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
+    const T wfe = wf[f * nele + e];
+    const T pe = pen[f * nele + e];
+    int nbe = 0, nbf = 0;
+    if (type == 0) {
+      const int nbz = nbface[z];
+      nbe = nbz / M::NF;
+      nbf = nbz - nbe * M::NF;
+    }
+    const T* tf = ftab + f * M::SG * (1 + D) * NL;
+#pragma unroll 2
+    for (int g = 0; g < M::SG; ++g) {
+      const typename Vec2<T>::type tr = trace[(f * M::SG + g) * nele + e];
+      const T ws = wfe * sw[g];
+      T qsn, qdn;
+      if (type == 0) {
+        // interior: neighbour point aligned with g; grad u_N . n_E = -(grad u_N . n_N)
+        const int ga = aln[a * M::SG + g];
+        const typename Vec2<T>::type tn = trace[(nbf * M::SG + ga) * nele + nbe];
+        const T jump = tr.x - tn.x;
+        qsn = ws * (T(-0.5) * (tr.y - tn.y) + pe * jump);
+        qdn = ws * (T(-0.5) * jump);
+      } else {
+        // Dirichlet
+        qsn = ws * (pe * tr.x - tr.y);
+        qdn = -ws * tr.x;
+      }
+      T qd[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) qd[d] = c[d] * qdn;
+      const T* t = tf + g * (1 + D) * NL;
+#pragma unroll
+      for (int n = 0; n < NL; ++n) {
+        T s = acc[n] + t[n] * qsn;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s += t[(1 + d) * NL + n] * qd[d];
+        acc[n] = s;
+      }
+    }
+  }
+
+  // ---- epilogue
+  T rv[NL];
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int n = 0; n < NL; ++n) rv[n] = -acc[n];
+  } else if constexpr (MODE == 4) {
+#pragma unroll
+    for (int n = 0; n < NL; ++n) rv[n] = acc[n];
+  } else {
+    T bv[NL];
+    load_row<T, NL>(b + (size_t)e * NL, bv);
+#pragma unroll
+    for (int n = 0; n < NL; ++n) rv[n] = bv[n] - acc[n];
+  }
+  if constexpr (MODE <= 1 || MODE == 4) {
+    store_row<T, NL>(r + (size_t)e * NL, rv);
+  } else {
+    if constexpr (MODE == 3) store_row<T, NL>(r + (size_t)e * NL, rv);
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      T s = xv[n];
+#pragma unroll
+      for (int m = 0; m < NL; ++m) s += S[(n * NL + m) * nele + e] * rv[m];
+      xv[n] = s;
+    }
+    store_row<T, NL>(x + (size_t)e * NL, xv);
+  }
+}
+
+// x = S b  (a smoothing step from x = 0, where the residual is b itself)
+template <typename T, int NL>
+__global__ __launch_bounds__(256) void k_smooth_zero(int nele, T* __restrict__ x,
+                                                     const T* __restrict__ b,
+                                                     const T* __restrict__ S) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nele) return;
+  T bv[NL], xv[NL];
+  load_row<T, NL>(b + (size_t)e * NL, bv);
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    T s = 0;
+#pragma unroll
+    for (int m = 0; m < NL; ++m) s += S[(n * NL + m) * nele + e] * bv[m];
+    xv[n] = s;
+  }
+  store_row<T, NL>(x + (size_t)e * NL, xv);
+}
+
+// bdiagK[e][n][m] = sum_{j,k} G_jk Sref_jk[n][m]
+//   + sum_f wf_f (pen_f M_f[n][m] - alpha_f sum_d c_fd (C_fd[n][m] + C_fd[m][n]))
+// alpha = 1/2 on interior faces, 1 on Dirichlet faces (z02/get_bdiag_diag.py:105-284).
+// refmats: Sref (D*D*NL*NL) | M (NF*NL*NL) | C (NF*D*NL*NL), FP64.
+template <int D>
+__global__ __launch_bounds__(256) void k_bdiag(int nele, const double* __restrict__ refm,
+                                               const double* __restrict__ G,
+                                               const double* __restrict__ cf,
+                                               const double* __restrict__ wf,
+                                               const double* __restrict__ pen,
+                                               const int32_t* __restrict__ fcode,
+                                               double* __restrict__ bdiag) {
+  using M = Dm<D>;
+  constexpr int NL = M::NL, NN = NL * NL;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (int64_t)nele * NN) return;
+  const int e = (int)(tid / NN);
+  const int nm = (int)(tid - (int64_t)e * NN);
+  const int n = nm / NL, m = nm - (nm / NL) * NL;
+  const double* Sref = refm;
+  const double* Mf = refm + D * D * NN;
+  const double* Cf = Mf + M::NF * NN;
+  double s = 0.0;
+  if constexpr (D == 3) {
+    const int jj[6] = {0, 1, 2, 0, 0, 1}, kk[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const double gq = G[q * nele + e];
+      const int j = jj[q], k = kk[q];
+      double v = Sref[(j * D + k) * NN + nm];
+      if (j != k) v += Sref[(k * D + j) * NN + nm];
+      s += gq * v;
+    }
+  } else {
+    const int jj[3] = {0, 1, 0}, kk[3] = {0, 1, 1};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double gq = G[q * nele + e];
+      const int j = jj[q], k = kk[q];
+      double v = Sref[(j * D + k) * NN + nm];
+      if (j != k) v += Sref[(k * D + j) * NN + nm];
+      s += gq * v;
+    }
+  }
+  for (int f = 0; f < M::NF; ++f) {
+    const int type = fcode[e * M::NF + f] & 15;
+    if (type >= 2) continue;
+    const double alpha = type == 0 ? 0.5 : 1.0;
+    double t = pen[f * nele + e] * Mf[f * NN + nm];
+    double cs = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      cs += cf[(f * D + d) * nele + e] *
+            (Cf[(f * D + d) * NN + n * NL + m] + Cf[(f * D + d) * NN + m * NL + n]);
+    t -= alpha * cs;
+    s += wf[f * nele + e] * t;
+  }
+  bdiag[(size_t)e * NN + nm] = s;
+}
+
+// Right-hand side (z02/volume_mf_diff.py:195-239 volume, :298-357 Dirichlet faces, with the
+// owner's kappa as z05/volume_mf_um_diff_varprop.py:320-342):
+//   b_n = detJ sum_m Mref[n][m] f_m
+//       + sum_{Dirichlet f} wf_f (pen_f sum_m M_f[n][m] u_m - sum_d c_fd sum_m C_fd[m][n] u_m)
+template <int D>
+__global__ __launch_bounds__(256) void k_rhs(int nele, const double* __restrict__ refm,
+                                             const double* __restrict__ detJ,
+                                             const double* __restrict__ cf,
+                                             const double* __restrict__ wf,
+                                             const double* __restrict__ pen,
+                                             const int32_t* __restrict__ fcode,
+                                             const double* __restrict__ fsrc,
+                                             const double* __restrict__ ubc,
+                                             double* __restrict__ b) {
+  using M = Dm<D>;
+  constexpr int NL = M::NL, NN = NL * NL;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (int64_t)nele * NL) return;
+  const int e = (int)(tid / NL);
+  const int n = (int)(tid - (int64_t)e * NL);
+  const double* Mf = refm + D * D * NN;
+  const double* Cf = Mf + M::NF * NN;
+  const double* Mref = Cf + M::NF * D * NN;
+  const double* fe = fsrc + (size_t)e * NL;
+  const double* ue = ubc + (size_t)e * NL;
+  double s = 0.0;
+  for (int m = 0; m < NL; ++m) s += Mref[n * NL + m] * fe[m];
+  s *= detJ[e];
+  for (int f = 0; f < M::NF; ++f) {
+    if ((fcode[e * M::NF + f] & 15) != 1) continue;
+    double pm = 0.0, cm = 0.0;
+    for (int m = 0; m < NL; ++m) {
+      pm += Mf[f * NN + n * NL + m] * ue[m];
+      double t = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) t += cf[(f * D + d) * nele + e] * Cf[(f * D + d) * NN + m * NL + n];
+      cm += t * ue[m];
+    }
+    s += wf[f * nele + e] * (pen[f * nele + e] * pm - cm);
+  }
+  b[tid] = s;
+}
+
+int launch_rhs(dgmi_ctx* c, const double* refm, const double* detJ, const double* f,
+               const double* ubc, double* b, hipStream_t s) {
+  const int nele = (int)c->nele;
+  const int64_t nthreads = (int64_t)nele * c->nloc;
+  const dim3 blk(256), grd((unsigned)((nthreads + 255) / 256));
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_rhs<3>, grd, blk, 0, s, nele, refm, detJ, c->g64.cf, c->g64.wf,
+                       c->g64.pen, c->fcode, f, ubc, b);
+  else
+    hipLaunchKernelGGL(k_rhs<2>, grd, blk, 0, s, nele, refm, detJ, c->g64.cf, c->g64.wf,
+                       c->g64.pen, c->fcode, f, ubc, b);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+// Column m of the 3-sweep damped point-Jacobi operator on one block, times omega:
+// c = 0; 3x c += w (e_m - B c)/diag(B); S[:, m] = w c.
+template <typename T, int NL>
+__global__ __launch_bounds__(256) void k_build_S_jacobi(int nele, const double* __restrict__ B,
+                                                        double omega, T* __restrict__ S) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (int64_t)nele * NL) return;
+  const int e = (int)(tid / NL);
+  const int m = (int)(tid - (int64_t)e * NL);
+  const double* Be = B + (size_t)e * NL * NL;
+  double c[NL];
+#pragma unroll
+  for (int n = 0; n < NL; ++n) c[n] = 0.0;
+  for (int it = 0; it < 3; ++it) {
+    double cn[NL];
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      double bc = 0.0;
+#pragma unroll
+      for (int k = 0; k < NL; ++k) bc += Be[n * NL + k] * c[k];
+      const double rhs = (n == m ? 1.0 : 0.0) - bc;
+      cn[n] = c[n] + omega * rhs / Be[n * NL + n];
+    }
+#pragma unroll
+    for (int n = 0; n < NL; ++n) c[n] = cn[n];
+  }
+#pragma unroll
+  for (int n = 0; n < NL; ++n) S[(n * NL + m) * nele + e] = (T)(omega * c[n]);
+}
+
+__global__ void k_extract_diag(int64_t n, int nl, const double* __restrict__ B,
+                               double* __restrict__ d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t e = i / nl;
+  const int k = (int)(i - e * nl);
+  d[i] = B[(e * nl + k) * nl + k];
+}
+
+template <typename T>
+__global__ void k_convert(const double* __restrict__ in, T* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (T)in[i];
+}
+
+// ---------------------------------------------------------------------------------------
+template <typename T, int D>
+static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mode,
+                      hipStream_t s) {
+  const int nele = (int)c->nele;
+  const dim3 blk(256), grd((nele + 255) / 256);
+  hipLaunchKernelGGL((k_trace<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf, g.trace);
+  const T* vt = g.tab + g.off_vtab;
+  const T* wg = g.tab + g.off_wgt;
+  const T* ft = g.tab + g.off_ftab;
+  const T* sw = g.tab + g.off_sw;
+#define DGMI_RES(MODE)                                                                     \
+  hipLaunchKernelGGL((k_residual<T, D, MODE>), grd, blk, 0, s, nele, x, b, r, vt, wg, ft, sw, \
+                     c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, g.trace, g.S)
+  switch (mode) {
+    case 0: DGMI_RES(0); break;
+    case 1: DGMI_RES(1); break;
+    case 2: DGMI_RES(2); break;
+    case 3: DGMI_RES(3); break;
+    default: DGMI_RES(4); break;
+  }
+#undef DGMI_RES
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_apply(dgmi_ctx* c, DevGeom<T>& g, const T* x, const T* b, T* r, hipStream_t s) {
+  T* xm = const_cast<T*>(x);  // not written in modes 0/1
+  if (c->D == 3) return apply_impl<T, 3>(c, g, xm, b, r, b ? 1 : 0, s);
+  return apply_impl<T, 2>(c, g, xm, b, r, b ? 1 : 0, s);
+}
+
+template <typename T>
+int launch_apply_pos(dgmi_ctx* c, DevGeom<T>& g, const T* x, T* r, hipStream_t s) {
+  T* xm = const_cast<T*>(x);
+  if (c->D == 3) return apply_impl<T, 3>(c, g, xm, nullptr, r, 4, s);
+  return apply_impl<T, 2>(c, g, xm, nullptr, r, 4, s);
+}
+
+template <typename T>
+int launch_smooth(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r_out, hipStream_t s) {
+  const int mode = r_out ? 3 : 2;
+  if (c->D == 3) return apply_impl<T, 3>(c, g, x, b, r_out, mode, s);
+  return apply_impl<T, 2>(c, g, x, b, r_out, mode, s);
+}
+
+template <typename T>
+int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t s) {
+  const int nele = (int)c->nele;
+  const dim3 blk(256), grd((nele + 255) / 256);
+  if (c->D == 3)
+    hipLaunchKernelGGL((k_smooth_zero<T, 20>), grd, blk, 0, s, nele, x, b, g.S);
+  else
+    hipLaunchKernelGGL((k_smooth_zero<T, 10>), grd, blk, 0, s, nele, x, b, g.S);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_bdiag(dgmi_ctx* c, const double* refm, double* bdiag, hipStream_t s) {
+  const int nele = (int)c->nele;
+  const int64_t nthreads = (int64_t)nele * c->nloc * c->nloc;
+  const dim3 blk(256), grd((unsigned)((nthreads + 255) / 256));
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_bdiag<3>, grd, blk, 0, s, nele, refm, c->g64.G, c->g64.cf, c->g64.wf,
+                       c->g64.pen, c->fcode, bdiag);
+  else
+    hipLaunchKernelGGL(k_bdiag<2>, grd, blk, 0, s, nele, refm, c->g64.G, c->g64.cf, c->g64.wf,
+                       c->g64.pen, c->fcode, bdiag);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_build_S_jacobi(dgmi_ctx* c, const double* bdiag, double omega, T* S, hipStream_t s) {
+  const int nele = (int)c->nele;
+  const int64_t nthreads = (int64_t)nele * c->nloc;
+  const dim3 blk(256), grd((unsigned)((nthreads + 255) / 256));
+  if (c->D == 3)
+    hipLaunchKernelGGL((k_build_S_jacobi<T, 20>), grd, blk, 0, s, nele, bdiag, omega, S);
+  else
+    hipLaunchKernelGGL((k_build_S_jacobi<T, 10>), grd, blk, 0, s, nele, bdiag, omega, S);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_extract_diag(dgmi_ctx* c, const double* B, double* d, hipStream_t s) {
+  const int64_t n = c->nele * c->nloc;
+  const dim3 blk(256), grd((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL(k_extract_diag, grd, blk, 0, s, n, c->nloc, B, d);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_convert(const double* in, T* out, int64_t n, hipStream_t s) {
+  const dim3 blk(256), grd((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL((k_convert<T>), grd, blk, 0, s, in, out, n);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template int launch_apply<double>(dgmi_ctx*, DevGeom<double>&, const double*, const double*,
+                                  double*, hipStream_t);
+template int launch_apply<float>(dgmi_ctx*, DevGeom<float>&, const float*, const float*, float*,
+                                 hipStream_t);
+template int launch_apply_pos<double>(dgmi_ctx*, DevGeom<double>&, const double*, double*,
+                                      hipStream_t);
+template int launch_apply_pos<float>(dgmi_ctx*, DevGeom<float>&, const float*, float*, hipStream_t);
+template int launch_smooth<double>(dgmi_ctx*, DevGeom<double>&, double*, const double*, double*,
+                                   hipStream_t);
+template int launch_smooth<float>(dgmi_ctx*, DevGeom<float>&, float*, const float*, float*,
+                                  hipStream_t);
+template int launch_smooth_zero<double>(dgmi_ctx*, DevGeom<double>&, double*, const double*,
+                                        hipStream_t);
+template int launch_smooth_zero<float>(dgmi_ctx*, DevGeom<float>&, float*, const float*,
+                                       hipStream_t);
+template int launch_build_S_jacobi<double>(dgmi_ctx*, const double*, double, double*, hipStream_t);
+template int launch_build_S_jacobi<float>(dgmi_ctx*, const double*, double, float*, hipStream_t);
+template int launch_convert<double>(const double*, double*, int64_t, hipStream_t);
+template int launch_convert<float>(const double*, float*, int64_t, hipStream_t);
+
+}  // namespace dgmi

@@ -1,0 +1,77 @@
+// Host-side set-up for the DG diffusion context: reference-element tables, mesh
+// topology, affine element geometry, P1CG graph, SFC ordering and SFC level operators.
+// Pure C++ (no HIP); compiled with -ffp-contract=off so that the SFC quantisation and
+// the tables evaluate in the same IEEE operation order as the Python oracle.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dgmi {
+
+// Reference P3 element (the reference's SHATRInew outputs for p = 3, quadrature
+// degree 6: z02/shape_function.py:459-1317, sizes z02/function_space.py:32-66).
+struct ElementTables {
+  int D = 0, nloc = 0, ngi = 0, sngi = 0, nface = 0, snloc = 0;
+  std::vector<double> n;        // (nloc, ngi)
+  std::vector<double> nlx;      // (D, nloc, ngi)
+  std::vector<double> weight;   // (ngi)
+  std::vector<double> sn;       // (nface, nloc, sngi)
+  std::vector<double> snlx;     // (nface, D, nloc, sngi)
+  std::vector<double> sweight;  // (sngi)
+  std::vector<int> gi_align;    // (D, sngi)
+  std::vector<double> p1_prolong;  // (nloc, D+1)
+  std::vector<int> index;       // (nloc, D+1) barycentric multi-indices
+};
+ElementTables make_element(int D);
+
+// nbf / alnmt / bctype as z02/mesh_init.py:293-390 (3D) and :40-100 (2D).
+struct Topology {
+  std::vector<int64_t> nbf, alnmt, bctype;  // (nele*nface)
+};
+Topology build_topology(const int64_t* cells, int64_t nele, int D);
+
+// Affine geometry per element.  J[i][j] = x_i[j] - x_D[j] (rows are edge vectors
+// from vertex D), which equals the reference's iso-parametric Jacobian for
+// straight-sided P3 elements.  All arrays are structure-of-arrays, length nele each
+// component.
+struct Geometry {
+  // metric  kappa*detJ*(invJ^T invJ): D==3 -> 6 comps (00,11,22,01,02,12); D==2 -> 3 (00,11,01)
+  std::vector<double> G;
+  // c_f = invJ^T n_f : (nface*D) comps
+  std::vector<double> cf;
+  // wf = kappa_f * sdet_f (0 on faces that carry no operator term), pen = eta/h : nface comps
+  std::vector<double> wf, pen;
+  // face record: neighbour global face (N*nface + f') or -1; alignment; type
+  std::vector<int32_t> nbface;  // (nele*nface) element-major
+  std::vector<int32_t> fcode;   // (nele*nface): (type+1) | (alnmt << 4); type -1 int, 0 Dirichlet
+  std::vector<double> detJ;     // (nele)
+  double max_affine_dev = 0.0;  // max |x_node - affine position| / h
+};
+bool build_geometry(const ElementTables& el, const double* x_ref_in, const double* kappa,
+                    const Topology& topo, int64_t nele, double eta_e, Geometry& g,
+                    std::string& err);
+
+// P1CG CSR graph (z02/sparsity.py:102-122): all vertex pairs per element, sorted.
+void p1cg_sparsity(const int64_t* cells, int64_t nele, int D, int64_t nverts,
+                   std::vector<int64_t>& fina, std::vector<int64_t>& cola);
+// greedy distance-2 colouring, colours 1..ncolor (z02/sparsity.py:235-269)
+int color2(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
+           std::vector<int32_t>& whichc);
+// Hilbert SFC numbering: sfc[node] = 1-based position (ties broken by node id).
+void hilbert_order(const double* verts, int64_t nverts, int D, std::vector<int64_t>& sfc);
+int nlevel_sfc(int64_t n);
+
+// SFC level operators in CSR (SFC order), level l aggregates positions p >> l
+// (z00-fortran-src/compact_fcode.f90:7-225).
+struct SfcLevelsHost {
+  int nlevel = 0;
+  std::vector<int64_t> n;                 // nodes per level
+  std::vector<std::vector<int32_t>> rowptr, col;
+  std::vector<std::vector<double>> val, diag;
+};
+void build_sfc_levels(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
+                      const std::vector<double>& rar, const std::vector<int64_t>& sfc,
+                      SfcLevelsHost& lv);
+
+}  // namespace dgmi

@@ -1,0 +1,382 @@
+"""Reference-compatible host interface of the DG diffusion solve on MI355X.
+
+The reference exposes module-level functions over torch tensors that read a global
+state bag (z02/cmmn_data.py:19-64, ``config.sf_nd_nb``).  ``DGDiffusion`` holds that state
+explicitly -- one library context per mesh -- and its methods keep the reference's
+names, argument meaning, in-place mutation and return values:
+
+  get_residual_or_smooth(r0, x_i, x_rhs, do_smooth=False) -> (r0, x_i)
+                                          z02/volume_mf_diff.py:403-483
+  get_bdiag_diag() -> (diagK, bdiagK)      z02/get_bdiag_diag.py:30-62
+  _get_RAR_and_sfc_data_Um()               z02/volume_mf_diff.py:1642-1693
+  _um_left_precond(x_i, x_rhs) -> x_i      z02/volume_mf_diff.py:1934-2029
+  mg_on_P1CG(r0_sfc) -> e_sfc              z02/multigrid_linearelastic.py:112-172
+  _gmres_mg_solver(x_i, x_rhs, tol) -> (x_i, its)   z02/volume_mf_diff.py:1757-1841
+  _mg_solver / _cg_mg_solver               z02/volume_mf_diff.py:1844-1931
+  _get_rhs(x_rhs, f, u_bc) -> x_rhs        z02/volume_mf_diff.py:153-192
+  solve_for_diff(x_i, f, u_bc, alpha_x_i, t=None) -> x_i   z02/volume_mf_diff.py:33-55
+  get_l2_error(x_num, x_ana) -> (l2, linf) z02/volume_mf_diff.py:2032-2056
+
+Module-level functions of the same names act on the space installed with
+``set_space`` (the analogue of ``config.sf_nd_nb``).  All arithmetic runs in libdgmi.so;
+torch tensors are device buffers and streams only.  Errors raise ``ValueError`` for bad
+arguments and ``DgmiError`` for failures inside the library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+ETA_E = 36.0        # z02/config.py:245
+JAC_WEI = 2.0 / 3.0  # z02/config.py:108
+GMRES_M = 30        # restart length of the workloads (the reference config uses 10, :149)
+GMRES_ITS = 400     # z02/config.py:150
+
+_DTYPES = {torch.float64: _lib.DGMI_F64, torch.float32: _lib.DGMI_F32}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DGDiffusion:
+    """One P3 DG mesh, its SIPG operator and SFC multigrid hierarchy on one GPU."""
+
+    def __init__(self, x_ref_in, cells, verts, kappa=None, bctype=None, eta_e: float = ETA_E,
+                 blk_solver: str = "jacobian", jac_wei: float = JAC_WEI, device="cuda:0"):
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("DGDiffusion runs on a GPU device")
+        torch.cuda.set_device(self.device)
+        x = np.ascontiguousarray(np.asarray(x_ref_in, dtype=np.float64))
+        cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int64))
+        verts = np.ascontiguousarray(np.asarray(verts, dtype=np.float64))
+        if x.ndim != 3 or x.shape[1] not in (2, 3):
+            raise ValueError("x_ref_in must be (nele, D, nloc) with D in (2, 3)")
+        D = x.shape[1]
+        nele = x.shape[0]
+        if cells.shape != (nele, D + 1) or verts.ndim != 2 or verts.shape[1] != D:
+            raise ValueError("cells must be (nele, D+1) and verts (nverts, D)")
+        kappa = np.ones(nele) if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
+        if kappa.shape != (nele,):
+            raise ValueError("kappa must have one value per element")
+        self._keep = [x, cells, verts, kappa]
+        bc = None
+        if bctype is not None:
+            bc = np.ascontiguousarray(np.asarray(bctype, dtype=np.int64).reshape(-1))
+            if bc.size != nele * (D + 1):
+                raise ValueError("bctype must have nele*nface entries")
+            self._keep.append(bc)
+        desc = _lib.MeshDesc(
+            D=D, nele=nele, nverts=verts.shape[0],
+            x_ref_in=x.ctypes.data_as(_lib._DP), cells=cells.ctypes.data_as(_lib._I64P),
+            verts=verts.ctypes.data_as(_lib._DP), kappa=kappa.ctypes.data_as(_lib._DP),
+            bctype=bc.ctypes.data_as(_lib._I64P) if bc is not None else None, eta_e=eta_e)
+        ctx = C.c_void_p()
+        check(self.lib.dgmi_ctx_create(C.byref(desc), C.byref(ctx)), "dgmi_ctx_create")
+        self.ctx = ctx
+        info = (C.c_int64 * 8)()
+        check(self.lib.dgmi_ctx_info(ctx, info), "dgmi_ctx_info")
+        self.D = D
+        self.nele, self.nloc, self.ndof, self.nface = info[0], info[1], info[2], info[3]
+        self.ngi, self.sngi, self.nverts = info[4], info[5], info[6]
+        self.x_ref_in = x
+        self.cells = cells
+        self.verts = verts
+        self.kappa = kappa
+        self.blk_solver = blk_solver
+        self.jac_wei = jac_wei
+        self._smoother_ready = False
+        self.diagK = None
+        self.bdiagK = None
+
+    @classmethod
+    def from_mesh(cls, mesh, kappa=None, **kw):
+        """Build from a ``meshgen.Mesh``."""
+        return cls(mesh.x_ref_in, mesh.cells, mesh.verts, kappa, **kw)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.dgmi_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- helpers
+    def _vec(self, t, name, dtypes=(torch.float64, torch.float32)):
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{name} must be a torch tensor")
+        if t.device != self.device and not (t.device.type == "cuda" and self.device.index is None):
+            raise ValueError(f"{name} must live on {self.device}")
+        if t.dtype not in dtypes:
+            raise ValueError(f"{name} has dtype {t.dtype}; expected one of {dtypes}")
+        if t.numel() != self.ndof or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous tensor of {self.ndof} values")
+        return t
+
+    def new_vector(self, dtype=torch.float64):
+        return torch.zeros(self.ndof, dtype=dtype, device=self.device)
+
+    def topology(self):
+        n = self.nele * self.nface
+        nbf, aln, bct = (np.empty(n, np.int64) for _ in range(3))
+        check(self.lib.dgmi_topology(self.ctx, nbf.ctypes.data_as(_lib._I64P),
+                                     aln.ctypes.data_as(_lib._I64P),
+                                     bct.ctypes.data_as(_lib._I64P)), "dgmi_topology")
+        return nbf, aln, bct
+
+    def element_tables(self):
+        D, nl, ng, sg, nf = self.D, self.nloc, self.ngi, self.sngi, self.nface
+        arrs = dict(n=np.empty((nl, ng)), nlx=np.empty((D, nl, ng)), weight=np.empty(ng),
+                    sn=np.empty((nf, nl, sg)), snlx=np.empty((nf, D, nl, sg)),
+                    sweight=np.empty(sg), gi_align=np.empty((D, sg), np.int64))
+        check(self.lib.dgmi_element_tables(
+            self.ctx, *[arrs[k].ctypes.data_as(_lib._DP) for k in
+                        ("n", "nlx", "weight", "sn", "snlx", "sweight")],
+            arrs["gi_align"].ctypes.data_as(_lib._I64P)), "dgmi_element_tables")
+        return arrs
+
+    # ---------------------------------------------------------------- operator
+    def apply(self, x, b=None, out=None):
+        """out = b - A x (b None: out = -A x) in the precision of x."""
+        self._vec(x, "x")
+        dt = _DTYPES[x.dtype]
+        if b is not None:
+            self._vec(b, "b", (x.dtype,))
+        out = torch.empty_like(x) if out is None else self._vec(out, "out", (x.dtype,))
+        check(self.lib.dgmi_apply(self.ctx, dt, _ptr(x), _ptr(b), _ptr(out), _stream()),
+              "dgmi_apply")
+        return out
+
+    def get_residual_or_smooth(self, r0, x_i, x_rhs, do_smooth=False):
+        """r0 += x_rhs - A x_i; with do_smooth also x_i += omega c, c from the block
+        smoother applied to r0.  Both tensors are mutated and returned flattened
+        (z02/volume_mf_diff.py:403-483)."""
+        r0 = r0.view(-1)
+        x_i = x_i.view(-1)
+        self._vec(r0, "r0")
+        self._vec(x_i, "x_i", (r0.dtype,))
+        dt = _DTYPES[r0.dtype]
+        if not (isinstance(x_rhs, (int, float)) and x_rhs == 0):
+            x_rhs = self._vec(x_rhs.view(-1), "x_rhs", (r0.dtype,))
+            check(self.lib.dgmi_axpy(dt, self.ndof, 1.0, _ptr(x_rhs), _ptr(r0), _stream()),
+                  "dgmi_axpy")
+        if do_smooth:
+            self._ensure_smoother()
+            check(self.lib.dgmi_smooth(self.ctx, dt, _ptr(x_i), _ptr(r0), _ptr(r0), _stream()),
+                  "dgmi_smooth")
+        else:
+            check(self.lib.dgmi_apply(self.ctx, dt, _ptr(x_i), _ptr(r0), _ptr(r0), _stream()),
+                  "dgmi_apply")
+        return r0, x_i
+
+    def get_bdiag_diag(self):
+        """Diagonal blocks and diagonal of A in FP64 (z02/get_bdiag_diag.py:30-62)."""
+        bd = torch.empty((self.nele, self.nloc, self.nloc), dtype=torch.float64, device=self.device)
+        dg = torch.empty((self.nele, self.nloc), dtype=torch.float64, device=self.device)
+        check(self.lib.dgmi_bdiag(self.ctx, _ptr(bd), _ptr(dg), _stream()), "dgmi_bdiag")
+        self.bdiagK, self.diagK = bd, dg
+        return dg, bd
+
+    def _ensure_smoother(self):
+        if not self._smoother_ready:
+            v = {"jacobian": _lib.DGMI_SMOOTH_JACOBI3, "direct": _lib.DGMI_SMOOTH_DIRECT}[self.blk_solver]
+            check(self.lib.dgmi_smoother_setup(self.ctx, v, self.jac_wei, _stream()),
+                  "dgmi_smoother_setup")
+            self._smoother_ready = True
+
+    # ---------------------------------------------------------------- multigrid
+    def _get_RAR_and_sfc_data_Um(self):
+        """P1CG operator RAR by coloured probing + SFC level operators."""
+        self._ensure_smoother()
+        check(self.lib.dgmi_mg_setup(self.ctx, _stream()), "dgmi_mg_setup")
+        info = (C.c_int64 * 128)()
+        check(self.lib.dgmi_mg_info(self.ctx, info, 128), "dgmi_mg_info")
+        self.cg_nonods, self.ncola, self.ncolor, self.nlevel = info[0], info[1], info[2], info[3]
+        self.nodes_per_level = [info[4 + i] for i in range(self.nlevel)]
+        self._mg_ready = True
+        return 0
+
+    def mg_export(self):
+        """Host copies of (fina, cola, RAR values, sfc numbering)."""
+        fina = np.empty(self.cg_nonods + 1, np.int64)
+        cola = np.empty(self.ncola, np.int64)
+        rar = np.empty(self.ncola)
+        sfc = np.empty(self.cg_nonods, np.int64)
+        check(self.lib.dgmi_mg_export(self.ctx, fina.ctypes.data_as(_lib._I64P),
+                                      cola.ctypes.data_as(_lib._I64P), rar.ctypes.data_as(_lib._DP),
+                                      sfc.ctypes.data_as(_lib._I64P)), "dgmi_mg_export")
+        return fina, cola, rar, sfc
+
+    def _need_mg(self):
+        if not getattr(self, "_mg_ready", False):
+            self._get_RAR_and_sfc_data_Um()
+
+    def restrict_to_p1cg(self, r):
+        """P3DG -> P1CG in SFC order (I_cf P^T, permuted)."""
+        self._need_mg()
+        self._vec(r, "r")
+        out = torch.empty(self.cg_nonods, dtype=r.dtype, device=self.device)
+        check(self.lib.dgmi_restrict(self.ctx, _DTYPES[r.dtype], _ptr(r), _ptr(out), _stream()),
+              "dgmi_restrict")
+        return out
+
+    def prolong_add_from_p1cg(self, e_sfc, x):
+        """x += P I_fc e (e in SFC order)."""
+        self._need_mg()
+        self._vec(x, "x")
+        check(self.lib.dgmi_prolong_add(self.ctx, _DTYPES[x.dtype], _ptr(e_sfc), _ptr(x),
+                                        _stream()), "dgmi_prolong_add")
+        return x
+
+    def mg_on_P1CG(self, r0_sfc):
+        """One SFC V-cycle, r and e in SFC order (z02/multigrid_linearelastic.py:112-172)."""
+        self._need_mg()
+        if r0_sfc.numel() != self.cg_nonods or r0_sfc.dtype not in _DTYPES:
+            raise ValueError("r0 must hold cg_nonods values of float64/float32")
+        e = torch.empty_like(r0_sfc)
+        check(self.lib.dgmi_sfc_vcycle(self.ctx, _DTYPES[r0_sfc.dtype], _ptr(r0_sfc), _ptr(e),
+                                       _stream()), "dgmi_sfc_vcycle")
+        return e
+
+    def _um_left_precond(self, x_i, x_rhs):
+        """x_i = M^-1 x_rhs, one V-cycle from x_i = 0 (every caller in the reference passes a
+        zeroed x_i, z02/volume_mf_diff.py:1785-1786, 1807-1808)."""
+        self._need_mg()
+        x_i = self._vec(x_i.view(-1), "x_i")
+        x_rhs = self._vec(x_rhs.view(-1), "x_rhs", (x_i.dtype,))
+        check(self.lib.dgmi_precond(self.ctx, _DTYPES[x_i.dtype], _ptr(x_rhs), _ptr(x_i),
+                                    _stream()), "dgmi_precond")
+        return x_i
+
+    # ---------------------------------------------------------------- solvers
+    def _gmres_mg_solver(self, x_i, x_rhs, tol=None, rtol=None, m=GMRES_M, max_its=GMRES_ITS,
+                         inner=torch.float64):
+        """Left-preconditioned GMRES(m).  The reference stops on the absolute residual
+        ``tol``; pass ``rtol`` for ||b - A x|| <= rtol ||b||.  ``inner`` float32 runs the
+        Krylov loop in FP32 inside FP64 iterative refinement.  Returns (x_i, its) and
+        keeps the statistics in ``self.last_stats``."""
+        self._need_mg()
+        x_i = self._vec(x_i.view(-1), "x_i", (torch.float64,))
+        b = self._vec(x_rhs.view(-1), "x_rhs", (torch.float64,))
+        if rtol is None:
+            if tol is None:
+                raise ValueError("give tol (absolute) or rtol (relative)")
+            bn = float(torch.linalg.vector_norm(b))
+            rtol = tol / bn if bn > 0 else 0.0
+        stats = (C.c_double * 4)()
+        check(self.lib.dgmi_gmres_solve(self.ctx, _DTYPES[inner], _ptr(b), _ptr(x_i), rtol, m,
+                                        max_its, stats, _stream()), "dgmi_gmres_solve")
+        self.last_stats = dict(its=int(stats[0]), relres=stats[1], outer=int(stats[2]),
+                               cycles=int(stats[3]))
+        return x_i, int(stats[0])
+
+    def _mg_solver(self, x_i, x_rhs, tol=None, rtol=None, max_its=GMRES_ITS):
+        """Stand-alone V-cycle iteration (z02/volume_mf_diff.py:1844-1873)."""
+        self._need_mg()
+        x_i = self._vec(x_i.view(-1), "x_i", (torch.float64,))
+        b = self._vec(x_rhs.view(-1), "x_rhs", (torch.float64,))
+        bn = float(torch.linalg.vector_norm(b))
+        thr = tol if rtol is None else rtol * bn
+        r = self.apply(x_i, b)
+        e = torch.zeros_like(x_i)
+        its = 0
+        while float(torch.linalg.vector_norm(r)) > thr and its < max_its:
+            self._um_left_precond(e, r)
+            check(self.lib.dgmi_axpy(_lib.DGMI_F64, self.ndof, 1.0, _ptr(e), _ptr(x_i), _stream()),
+                  "dgmi_axpy")
+            self.apply(x_i, b, out=r)
+            its += 1
+        return x_i, its
+
+    def _get_rhs(self, x_rhs, f, u_bc, alpha_x_i=None):
+        """x_rhs = M f + Dirichlet terms (z02/volume_mf_diff.py:153-192)."""
+        x_rhs = self._vec(x_rhs.view(-1), "x_rhs", (torch.float64,))
+        f = self._vec(torch.as_tensor(f, dtype=torch.float64, device=self.device).reshape(-1).contiguous(), "f", (torch.float64,))
+        u = self._vec(torch.as_tensor(u_bc, dtype=torch.float64, device=self.device).reshape(-1).contiguous(), "u_bc", (torch.float64,))
+        check(self.lib.dgmi_rhs(self.ctx, _ptr(f), _ptr(u), _ptr(x_rhs), _stream()), "dgmi_rhs")
+        return x_rhs
+
+    def solve_for_diff(self, x_i, f, u_bc, alpha_x_i=None, t=None, rtol=1e-10,
+                       inner=torch.float64, m=GMRES_M, max_its=GMRES_ITS):
+        """RHS, multigrid set-up and GMRES solve (z02/volume_mf_diff.py:33-148)."""
+        self._need_mg()
+        shape = x_i.shape
+        b = self._get_rhs(self.new_vector(), f, u_bc)
+        x_i, _ = self._gmres_mg_solver(x_i.view(-1), b, rtol=rtol, inner=inner, m=m,
+                                       max_its=max_its)
+        return x_i.view(shape)
+
+    def get_l2_error(self, x_num, x_ana):
+        """(l2, linf) with the reference's definition (z02/volume_mf_diff.py:2032-2056):
+        l2^2 = sum_e sum_n sum_g (n[n,g] (x_num - x_ana)[e,n])^2 detwei[e,g]."""
+        tabs = self.element_tables()
+        d = (x_num.view(-1) - x_ana.view(-1)).double().cpu().numpy().reshape(self.nele, self.nloc)
+        D = self.D
+        v = self.x_ref_in[:, :, :D + 1]                     # vertices (nele, D, D+1)
+        J = v[:, :, :D] - v[:, :, D:D + 1]                  # columns x_k - x_D
+        det = np.linalg.det(np.transpose(J, (0, 2, 1)))
+        l2 = np.sqrt(np.einsum("ng,g,en,e->", tabs["n"] ** 2, tabs["weight"], d ** 2, det))
+        return l2, float(np.abs(d).max())
+
+
+# ---------------------------------------------------------------- module-level API
+_space: DGDiffusion | None = None
+
+
+def set_space(space: DGDiffusion):
+    """Install the active space (the analogue of ``config.sf_nd_nb``)."""
+    global _space
+    _space = space
+
+
+def _active():
+    if _space is None:
+        raise ValueError("no active space: call set_space(DGDiffusion(...)) first")
+    return _space
+
+
+def get_residual_or_smooth(r0, x_i, x_rhs, do_smooth=False):
+    return _active().get_residual_or_smooth(r0, x_i, x_rhs, do_smooth)
+
+
+def get_bdiag_diag():
+    return _active().get_bdiag_diag()
+
+
+def _get_RAR_and_sfc_data_Um():
+    return _active()._get_RAR_and_sfc_data_Um()
+
+
+def _um_left_precond(x_i, x_rhs):
+    return _active()._um_left_precond(x_i, x_rhs)
+
+
+def _gmres_mg_solver(x_i, x_rhs, tol):
+    return _active()._gmres_mg_solver(x_i, x_rhs, tol)
+
+
+def _mg_solver(x_i, x_rhs, tol):
+    return _active()._mg_solver(x_i, x_rhs, tol)
+
+
+def solve_for_diff(x_i, f, u_bc, alpha_x_i=None, t=None):
+    return _active().solve_for_diff(x_i, f, u_bc, alpha_x_i, t)
+
+
+def get_l2_error(x_num, x_ana):
+    return _active().get_l2_error(x_num, x_ana)

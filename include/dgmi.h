@@ -1,0 +1,157 @@
+/* dgmi.h -- C ABI of libdgmi.so, the MI355X-native interior-penalty DG diffusion solve.
+ *
+ * The reference (linfeng810/dg_nn_public) has no FFI: its operator API is a set of
+ * module-level Python functions over torch tensors that read global state
+ * (z02/cmmn_data.py:19-64).  Each entry point below replaces one of those functions;
+ * the Python layer in dg_nn_public_amd/ keeps the reference's names and signatures on
+ * top of it.  Citations are relative to the reference repository root,
+ * z02/ = z02-implicit/z06-DGtoCG/, z05/ = z05-ns/.
+ *
+ * Conventions
+ *   - every entry point returns 0 on success and a negative code on error; the message
+ *     is available from dgmi_last_error() (thread-local);
+ *   - vectors are caller-owned device pointers in the reference's layout: flat
+ *     (nele*nloc), element-major; the context owns geometry, tables, multigrid data and
+ *     scratch;
+ *   - `dtype` selects the arithmetic of an operator apply: DGMI_F64, DGMI_F32, or
+ *     DGMI_BF16 (bf16 inputs, fp32 accumulation; vectors are then fp32);
+ *   - `stream` is a hipStream_t passed as void* (0 = the null stream); no entry point
+ *     synchronises the device unless it says so.
+ *   - a context is not safe for concurrent use from several threads.
+ */
+#ifndef DGMI_H
+#define DGMI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dgmi_ctx dgmi_ctx;
+
+enum { DGMI_F64 = 0, DGMI_F32 = 1, DGMI_BF16 = 2 };
+enum { DGMI_SMOOTH_JACOBI3 = 0, DGMI_SMOOTH_DIRECT = 1 };
+
+typedef struct {
+  int32_t D;               /* 2 (P3 triangles) or 3 (P3 tetrahedra)                      */
+  int64_t nele;            /* number of elements                                        */
+  int64_t nverts;          /* number of P1 vertices (P1CG nodes)                         */
+  const double* x_ref_in;  /* host (nele, D, nloc) P3 node coordinates                   */
+  const int64_t* cells;    /* host (nele, D+1) vertex ids of each element                */
+  const double* verts;     /* host (nverts, D) vertex coordinates                        */
+  const double* kappa;     /* host (nele) diffusivity per element                        */
+  const int64_t* bctype;   /* host (nele*(D+1)) face types, or NULL: boundary faces are
+                              Dirichlet (0), matched faces interior (-1); other types
+                              carry no operator term (z02/mesh_init.py:697-741)          */
+  double eta_e;            /* penalty coefficient (z02/config.py:245: 36)                 */
+} dgmi_mesh_desc;
+
+/* Context: topology (z02/mesh_init.py:260-658), reference-element tables
+ * (z02/shape_function.py:459-1317), affine geometry, uploaded to the current device. */
+int dgmi_ctx_create(const dgmi_mesh_desc* desc, dgmi_ctx** out);
+int dgmi_ctx_destroy(dgmi_ctx* ctx);
+/* info[0..7] = nele, nloc, ndof, nface, ngi, sngi, nverts, device bytes held */
+int dgmi_ctx_info(const dgmi_ctx* ctx, int64_t* info);
+/* host copies of nbf, alnmt, glb_bcface_type (each nele*nface) */
+int dgmi_topology(const dgmi_ctx* ctx, int64_t* nbf, int64_t* alnmt, int64_t* bctype);
+/* host copy of the element tables: n (nloc*ngi), nlx (D*nloc*ngi), weight (ngi),
+ * sn (nface*nloc*sngi), snlx (nface*D*nloc*sngi), sweight (sngi), gi_align (D*sngi) */
+int dgmi_element_tables(const dgmi_ctx* ctx, double* n, double* nlx, double* weight,
+                        double* sn, double* snlx, double* sweight, int64_t* gi_align);
+
+/* r = b - A x  (b may be NULL: r = -A x).  Replaces get_residual_or_smooth with
+ * do_smooth=False (z02/volume_mf_diff.py:403-483; kappa as
+ * z05/volume_mf_um_diff_varprop.py:347-436, 509-724). */
+int dgmi_apply(dgmi_ctx* ctx, int dtype, const void* x, const void* b, void* r, void* stream);
+
+/* b = sum_e M_e f_e + Dirichlet-face terms with data u_bc, scaled by the owner's kappa --
+ * _get_rhs (z02/volume_mf_diff.py:153-357; kappa as z05/volume_mf_um_diff_varprop.py:320-342).
+ * f, u_bc, b: FP64 device vectors in DOF layout. */
+int dgmi_rhs(dgmi_ctx* ctx, const double* f, const double* u_bc, double* b, void* stream);
+
+/* bdiagK (nele, nloc, nloc) and diagK (nele, nloc) in FP64 -- get_bdiag_diag
+ * (z02/get_bdiag_diag.py:30-62).  Either output may be NULL. */
+int dgmi_bdiag(dgmi_ctx* ctx, double* bdiag, double* diag, void* stream);
+
+/* Build the per-element smoother matrices S (c = S r) for `variant`:
+ * DGMI_SMOOTH_JACOBI3: three damped point-Jacobi sweeps on bdiagK from c = 0
+ *   (z02/volume_mf_diff.py:474-480, blk_solver='jacobian');
+ * DGMI_SMOOTH_DIRECT: S = bdiagK^-1 (z05/volume_mf_um_diff_varprop.py:384-392). */
+int dgmi_smoother_setup(dgmi_ctx* ctx, int variant, double omega, void* stream);
+
+/* One smoothing step: r = b - A x; x += omega * S r (get_residual_or_smooth with
+ * do_smooth=True).  r_out may be NULL (the residual is then not stored). */
+int dgmi_smooth(dgmi_ctx* ctx, int dtype, void* x, const void* b, void* r_out, void* stream);
+
+/* Multigrid set-up (z02/volume_mf_diff.py:1642-1754, z02/multigrid_linearelastic.py:175-283):
+ * P1CG sparsity, distance-2 colouring, RAR by coloured probing with the FP64 apply,
+ * SFC ordering, SFC level operators.  Synchronises the stream. */
+int dgmi_mg_setup(dgmi_ctx* ctx, void* stream);
+/* info[0] = cg_nonods, info[1] = ncola, info[2] = ncolor, info[3] = nlevel,
+ * info[4..4+nlevel) = nodes per level */
+int dgmi_mg_info(const dgmi_ctx* ctx, int64_t* info, int cap);
+/* host copies: fina (cg+1), cola (ncola), rar values (ncola), sfc (cg, 1-based) */
+int dgmi_mg_export(const dgmi_ctx* ctx, int64_t* fina, int64_t* cola, double* rar,
+                   int64_t* sfc);
+
+/* P3DG -> P1CG restriction in SFC order: r1_sfc = perm(I_cf P^T r)
+ * (z02/volume_mf_diff.py:1959-1996). */
+int dgmi_restrict(dgmi_ctx* ctx, int dtype, const void* r_dg, void* r_cg_sfc, void* stream);
+/* x += P I_fc e, with e in SFC order (z02/volume_mf_diff.py:2007-2018). */
+int dgmi_prolong_add(dgmi_ctx* ctx, int dtype, const void* e_cg_sfc, void* x_dg, void* stream);
+/* One SFC V-cycle on the P1CG levels, r and e in SFC order -- mg_on_P1CG
+ * (z02/multigrid_linearelastic.py:112-172). */
+int dgmi_sfc_vcycle(dgmi_ctx* ctx, int dtype, const void* r_sfc, void* e_sfc, void* stream);
+/* One V-cycle from x = 0: x = M^-1 b -- _um_left_precond (z02/volume_mf_diff.py:1934-2029). */
+int dgmi_precond(dgmi_ctx* ctx, int dtype, const void* b, void* x, void* stream);
+
+/* Left-preconditioned restarted GMRES(m) for A x = b on device vectors (FP64 x, b)
+ * -- _gmres_mg_solver (z02/volume_mf_diff.py:1757-1841).  inner_dtype selects the
+ * arithmetic of the operator applies and V-cycles inside the Krylov loop; with
+ * DGMI_F32/DGMI_BF16 the solve runs as FP64 iterative refinement around the inner
+ * GMRES.  Stops when ||b - A x|| <= rtol ||b|| (FP64 true residual) or after max_its
+ * inner iterations.  Synchronises the stream.  stats (may be NULL): [0] iterations,
+ * [1] final ||r||/||b||, [2] outer refinement steps, [3] restarts. */
+int dgmi_gmres_solve(dgmi_ctx* ctx, int inner_dtype, const double* b, double* x, double rtol,
+                     int m, int max_its, double* stats, void* stream);
+
+/* Krylov vector primitives on device vectors of n FP64 or FP32 values. */
+int dgmi_dot(int dtype, int64_t n, const void* x, const void* y, double* out_dev, void* stream);
+int dgmi_axpy(int dtype, int64_t n, double a, const void* x, void* y, void* stream);
+
+/* ---- host-only set-up (no device needed) ------------------------------------------
+ * Replacements of the reference's f2py Fortran modules and Python set-up loops; also
+ * used internally by dgmi_ctx_create / dgmi_mg_setup. */
+/* element tables as dgmi_element_tables, without a context (SHATRInew,
+ * z02/shape_function.py:459-1317) */
+int dgmi_host_element_tables(int D, double* n, double* nlx, double* weight, double* sn,
+                             double* snlx, double* sweight, int64_t* gi_align);
+/* nbf / alnmt / glb_bcface_type (z02/mesh_init.py:293-390 with get_nb.getfinele,
+ * z00-fortran-src/get_neighbouring.f90:1-130) */
+int dgmi_host_topology(const int64_t* cells, int64_t nele, int D, int64_t* nbf, int64_t* alnmt,
+                       int64_t* bctype);
+/* P1CG CSR pattern (get_nb.getfin_p1cg, get_neighbouring.f90:133-156; z02/sparsity.py:102-122) */
+int dgmi_host_p1cg_sparsity(const int64_t* cells, int64_t nele, int D, int64_t nverts,
+                            int64_t cap, int64_t* fina, int64_t* cola, int64_t* ncola);
+/* greedy distance-2 colouring, colours 1..ncolor (_color2, z02/sparsity.py:235-269) */
+int dgmi_host_color2(const int64_t* fina, const int64_t* cola, int64_t n, int32_t* whichc,
+                     int32_t* ncolor);
+/* SFC numbering sfc[node] = 1-based curve position; replaces
+ * sfc.ncurve_python_subdomain_space_filling_curve (combined_fortran_source.f90:10575)
+ * with a Hilbert order of the vertex coordinates */
+int dgmi_host_sfc_order(const double* verts, int64_t nverts, int D, int64_t* sfc);
+/* calculate_nlevel_sfc (z00-fortran-src/compact_fcode.f90:229-250) */
+int dgmi_host_nlevel_sfc(int64_t n);
+/* SFC level operators, concatenated per level (best_sfc_mapping_to_sfc_matrix_unstructured,
+ * compact_fcode.f90:7-225): level_n[l], rowptr_all (sum(level_n+1)), col_all/val_all */
+int dgmi_host_sfc_levels(const int64_t* fina, const int64_t* cola, const double* a,
+                         const int64_t* sfc, int64_t n, int64_t cap_rows, int64_t cap_nnz,
+                         int64_t* nlevel, int64_t* level_n, int64_t* rowptr_all,
+                         int64_t* col_all, double* val_all);
+
+const char* dgmi_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DGMI_H */

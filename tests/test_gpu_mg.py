@@ -1,0 +1,65 @@
+"""HIP multigrid (RAR by coloured probing, transfers, SFC V-cycle, preconditioner)
+against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_common import build, rel, to_dev, to_np
+from oracle import dg, mg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=[(2, 8, 0.5), (3, 4, 1.0)], ids=["2d", "3d"])
+def prob(request):
+    D, n, sig = request.param
+    print(f"[case] D={D} n={n} sigma={sig}", flush=True)
+    m, pb, g, sp = build(D, n, sigma=sig)
+    g._get_RAR_and_sfc_data_Um()
+    mgd = mg.build_mg(sp, m.verts)
+    return m, pb, g, sp, mgd
+
+
+def test_rar_and_sfc_match_oracle(prob):
+    m, pb, g, sp, mgd = prob
+    fina, cola, rar, sfc = g.mg_export()
+    assert np.array_equal(fina, mgd.fina) and np.array_equal(cola, mgd.cola)
+    assert np.array_equal(sfc, mgd.levels.sfc)
+    assert rel(rar, mgd.rar) < 1e-12
+    assert g.nodes_per_level == mgd.levels.nodes_per_level
+
+
+def test_restrict_and_prolong_match_oracle(prob):
+    m, pb, g, sp, mgd = prob
+    P = sp.el.p1_prolong
+    r = np.random.default_rng(1).standard_normal(sp.ndof)
+    ref = (mgd.I_cf @ mg.restrict_p3_to_p1(P, r))[np.argsort(mgd.levels.sfc)]
+    assert rel(to_np(g.restrict_to_p1cg(to_dev(r))), ref) < 1e-13
+    e = np.random.default_rng(2).standard_normal(m.nverts)  # natural order
+    x0 = np.random.default_rng(3).standard_normal(sp.ndof)
+    ref = x0 + mg.prolong_p1_to_p3(P, mgd.I_fc @ e)
+    x = to_dev(x0)
+    g.prolong_add_from_p1cg(to_dev(e[np.argsort(mgd.levels.sfc)]), x)
+    assert rel(to_np(x), ref) < 1e-13
+
+
+def test_sfc_vcycle_matches_oracle(prob):
+    m, pb, g, sp, mgd = prob
+    r = np.random.default_rng(4).standard_normal(m.nverts)
+    ref = mg.mg_on_P1CG(r, mgd.levels)
+    out = to_np(g.mg_on_P1CG(to_dev(r)))
+    print("vcycle rel err", rel(out, ref))
+    assert rel(out, ref) < 1e-11
+    out32 = to_np(g.mg_on_P1CG(to_dev(r, torch.float32)))
+    assert rel(out32, ref) < 1e-4
+
+
+def test_preconditioner_matches_oracle(prob):
+    m, pb, g, sp, mgd = prob
+    b = np.random.default_rng(5).standard_normal(sp.ndof)
+    ref = mg.um_left_precond(sp, mgd, b)
+    out = to_np(g._um_left_precond(g.new_vector(), to_dev(b)))
+    print("precond rel err", rel(out, ref))
+    assert rel(out, ref) < 1e-10
+    out32 = to_np(g._um_left_precond(g.new_vector(torch.float32), to_dev(b, torch.float32)))
+    assert rel(out32, ref) < 1e-4

@@ -1,0 +1,135 @@
+"""HIP operator apply, RHS, block diagonal and smoother against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_common import build, rel, to_dev, to_np
+from oracle import dg
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, 6, 1.0), (3, 3, 1.0)]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=["2d", "3d"])
+def prob(request):
+    D, n, sig = request.param
+    print(f"[case] D={D} n={n} sigma={sig}", flush=True)
+    return build(D, n, sigma=sig)
+
+
+def test_topology_matches_oracle(prob):
+    m, pb, g, sp = prob
+    nbf, aln, bct = g.topology()
+    assert np.array_equal(nbf, sp.nbf) and np.array_equal(aln, sp.alnmt)
+    assert np.array_equal(bct, sp.bctype)
+
+
+def test_apply_f64_matches_oracle(prob):
+    m, pb, g, sp = prob
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(sp.ndof)
+    b = rng.standard_normal(sp.ndof)
+    ref = b - dg.apply_A(sp, x)
+    out = g.apply(to_dev(x), to_dev(b))
+    err = rel(to_np(out), ref)
+    print("f64 apply rel err", err)
+    assert err < 1e-12
+    out2 = g.apply(to_dev(x))  # b = None: r = -A x
+    assert rel(to_np(out2), -dg.apply_A(sp, x)) < 1e-12
+
+
+def test_apply_f32_within_tolerance(prob):
+    m, pb, g, sp = prob
+    x = np.random.default_rng(8).standard_normal(sp.ndof)
+    ref = dg.apply_A(sp, x)
+    out = -to_np(g.apply(to_dev(x, torch.float32)))
+    err = rel(out, ref)
+    print("f32 apply rel err", err)
+    assert err < 1e-6
+
+
+def test_get_residual_or_smooth_accumulates_like_the_reference(prob):
+    m, pb, g, sp = prob
+    rng = np.random.default_rng(9)
+    r0_init, x, b = rng.standard_normal((3, sp.ndof))
+    r0 = to_dev(r0_init)
+    xi = to_dev(x)
+    r0, xi = g.get_residual_or_smooth(r0, xi, to_dev(b), do_smooth=False)
+    assert rel(to_np(r0), r0_init + b - dg.apply_A(sp, x)) < 1e-12
+    r0 = to_dev(np.zeros(sp.ndof))
+    r0, _ = g.get_residual_or_smooth(r0, to_dev(x), 0, do_smooth=False)
+    assert rel(to_np(r0), -dg.apply_A(sp, x)) < 1e-12
+
+
+def test_rhs_matches_oracle(prob):
+    m, pb, g, sp = prob
+    ref = dg.get_rhs(sp, pb.f, pb.u_bc)
+    rng = np.random.default_rng(10)
+    ub = rng.standard_normal(sp.ndof)  # non-zero Dirichlet data exercises the face terms
+    ref2 = dg.get_rhs(sp, pb.f, ub)
+    out = g._get_rhs(g.new_vector(), pb.f, pb.u_bc)
+    out2 = g._get_rhs(g.new_vector(), pb.f, ub)
+    assert rel(to_np(out), ref) < 1e-12 and rel(to_np(out2), ref2) < 1e-12
+
+
+def test_bdiag_matches_oracle(prob):
+    m, pb, g, sp = prob
+    d_ref, B_ref = dg.get_bdiag_diag(sp)
+    d, B = g.get_bdiag_diag()
+    assert rel(to_np(B), B_ref) < 1e-12 and rel(to_np(d), d_ref) < 1e-12
+
+
+@pytest.mark.parametrize("blk", ["jacobian", "direct"])
+def test_smooth_step_matches_oracle(prob, blk):
+    m, pb, g0, sp = prob
+    from dg_nn_public_amd.diffusion import DGDiffusion
+    g = DGDiffusion.from_mesh(m, pb.kappa, blk_solver=blk)
+    _, B = dg.get_bdiag_diag(sp)
+    rng = np.random.default_rng(11)
+    x, b = rng.standard_normal((2, sp.ndof))
+    r_ref = np.zeros(sp.ndof)
+    x_ref = x.copy()
+    dg.get_residual_or_smooth(sp, r_ref, x_ref, b, True, B, blk)
+    r0 = g.new_vector()
+    xi = to_dev(x)
+    r0, xi = g.get_residual_or_smooth(r0, xi, to_dev(b), do_smooth=True)
+    assert rel(to_np(r0), r_ref) < 1e-12
+    assert rel(to_np(xi), x_ref) < 1e-11
+    # FP32 smoothing step within single-precision tolerance
+    r32 = g.new_vector(torch.float32)
+    x32 = to_dev(x, torch.float32)
+    g.get_residual_or_smooth(r32, x32, to_dev(b, torch.float32), do_smooth=True)
+    assert rel(to_np(x32), x_ref) < 1e-5
+
+
+def test_bad_arguments_raise(prob):
+    m, pb, g, sp = prob
+    with pytest.raises(ValueError):
+        g.apply(torch.zeros(sp.ndof + 1, dtype=torch.float64, device="cuda:0"))
+    with pytest.raises(ValueError):
+        g.apply(torch.zeros(sp.ndof, dtype=torch.float16, device="cuda:0"))
+    with pytest.raises(ValueError):
+        g.apply(torch.zeros(sp.ndof, dtype=torch.float64))
+
+
+def test_full_size_3d_apply_properties():
+    """The 3D workload size (24^3 x 6 tets, 1.66M DOFs): FP64 apply against the oracle,
+    exactness on linear fields off the boundary, FP32 against FP64."""
+    m, pb, g, sp = build(3, 24, sigma=1.0)
+    print("[case] full-size 3D n=24", flush=True)
+    x = np.random.default_rng(12).standard_normal(sp.ndof)
+    ref = dg.apply_A(sp, x)
+    y64 = -to_np(g.apply(to_dev(x)))
+    err64 = rel(y64, ref)
+    y32 = -to_np(g.apply(to_dev(x, torch.float32)))
+    err32 = rel(y32, ref)
+    print("full-size rel err f64", err64, "f32", err32)
+    assert err64 < 1e-12 and err32 < 1e-6
+    from gpu_common import build as _b
+    m1, pb1, g1, sp1 = _b(3, 24, sigma=0.0)
+    c = np.array([0.3, -1.2, 0.7])
+    u = (np.einsum("d,edn->en", c, m1.x_ref_in) + 0.5).reshape(-1)
+    r = to_np(g1.apply(to_dev(u))).reshape(m1.nele, -1)
+    hasb = (sp1.bctype.reshape(m1.nele, -1) == 0).any(1)
+    assert np.abs(r[~hasb]).max() < 1e-10

@@ -1,0 +1,50 @@
+"""End-to-end solves on the GPU: FP64 GMRES and FP32-inner / FP64-refinement GMRES reach
+||r||/||b|| <= 1e-10 and agree with the oracle's solve."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_common import build, rel, to_dev, to_np
+from oracle import dg, krylov, mg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=[(2, 8, 0.0), (3, 4, 1.0)], ids=["2d", "3d"])
+def prob(request):
+    D, n, sig = request.param
+    print(f"[case] D={D} n={n} sigma={sig}", flush=True)
+    m, pb, g, sp = build(D, n, sigma=sig)
+    b = dg.get_rhs(sp, pb.f, pb.u_bc)
+    mgd = mg.build_mg(sp, m.verts)
+    x_ref, _ = krylov.gmres_mg_solver(sp, mgd, np.zeros(sp.ndof), b, rtol=1e-12, m=30)
+    return m, pb, g, sp, b, x_ref
+
+
+@pytest.mark.parametrize("inner", [torch.float64, torch.float32], ids=["f64", "f32-ir"])
+def test_gmres_reaches_tolerance_and_matches_oracle(prob, inner):
+    m, pb, g, sp, b, x_ref = prob
+    x = g.new_vector()
+    x, its = g._gmres_mg_solver(x, to_dev(b), rtol=1e-10, inner=inner)
+    xs = to_np(x)
+    relres = np.linalg.norm(b - dg.apply_A(sp, xs)) / np.linalg.norm(b)
+    print(inner, "its", its, g.last_stats, "oracle relres", relres, "vs oracle", rel(xs, x_ref))
+    assert g.last_stats["relres"] <= 1e-10
+    assert relres <= 2e-10
+    assert rel(xs, x_ref) < 1e-7
+
+
+def test_solve_for_diff_matches_manufactured_solution():
+    m, pb, g, sp = build(2, 8, sigma=0.0)
+    x = g.solve_for_diff(g.new_vector(), pb.f, pb.u_bc)
+    l2, linf = g.get_l2_error(x, to_dev(pb.u_exact))
+    l2o, _ = dg.get_l2_error(sp, to_np(x), pb.u_exact)
+    assert abs(l2 - l2o) < 1e-12 and l2 < 1e-4
+
+
+def test_mg_solver_converges():
+    m, pb, g, sp = build(2, 6, sigma=0.0)
+    b = to_dev(dg.get_rhs(sp, pb.f, pb.u_bc))
+    x, its = g._mg_solver(g.new_vector(), b, rtol=1e-8)
+    r = g.apply(x, b)
+    assert float(torch.linalg.vector_norm(r)) <= 1e-8 * float(torch.linalg.vector_norm(b))
