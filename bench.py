@@ -47,6 +47,9 @@ def main():
     ap.add_argument("--n", type=int, default=24, help="hex cells per direction")
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--applies", type=int, default=50)
+    ap.add_argument("--face-kappa", default="harmonic", choices=["harmonic", "mean"],
+                    help="interior-face diffusivity; 'mean' is the reference's arithmetic "
+                         "mean, which is not coercive at sigma=1 and stagnates")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
     args = ap.parse_args()
@@ -65,7 +68,7 @@ def main():
     t0 = time.time()
     mesh = meshgen.cube_mesh(args.n, jitter=0.15, seed=0)
     pb = meshgen.sine_problem(mesh, sigma=args.sigma, kappa_seed=1)
-    g = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev)
+    g = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa=args.face_kappa)
     g._get_RAR_and_sfc_data_Um()
     torch.cuda.synchronize()
     setup_s = time.time() - t0
@@ -121,6 +124,19 @@ def main():
           file=sys.stderr, flush=True)
     total64 = sum(lat64)
 
+    # the reference's arithmetic-mean face diffusivity on the same problem, 60 iterations
+    ref_form = None
+    if args.face_kappa != "mean" and args.sigma > 0:
+        gm = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa="mean")
+        gm._get_RAR_and_sfc_data_Um()
+        bm = gm._get_rhs(gm.new_vector(), pb.f, pb.u_bc)
+        gm._gmres_mg_solver(gm.new_vector(), bm, rtol=1e-10, max_its=60)
+        ref_form = {"face_kappa": "mean", "iterations": gm.last_stats["its"],
+                    "relres": gm.last_stats["relres"]}
+        gm.close()
+        print(f"[bench] reference mean-kappa form after 60 its: {ref_form}", file=sys.stderr,
+              flush=True)
+
     nele, nloc = g.nele, g.nloc
     F = flops_per_elem(3, nloc, g.ngi, g.sngi, g.nface)
     achieved = F * nele / t_apply64
@@ -138,7 +154,7 @@ def main():
         ctx = threadpool_limits(1) if threadpool_limits else None
         if ctx:
             ctx.__enter__()
-        sp = odg.make_space(mesh, pb.kappa)
+        sp = odg.make_space(mesh, pb.kappa, face_kappa=args.face_kappa)
         xs = np.random.default_rng(0).standard_normal(sp.ndof)
         odg.apply_A(sp, xs)  # builds the per-space face cache
         tc = time.perf_counter()
@@ -172,7 +188,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3D IP-DG P3 tets, {args.n}^3x6 = {nele} elements, "
                                f"{nele*nloc} DOFs, lognormal kappa sigma={args.sigma}, "
-                               f"GMRES(30)+SFC-MG, FP64, rtol 1e-10",
+                               f"{args.face_kappa}-mean face kappa, GMRES(30)+SFC-MG, FP64, "
+                               f"rtol 1e-10",
                    "global_batch": 1, "parallelism": "single GPU"},
         "latency_s": {"p50": float(np.percentile(lat64, 50)), "p99": float(np.percentile(lat64, 99))},
         "mixed": {"solves_per_s": len(lat32) / sum(lat32),
@@ -180,6 +197,7 @@ def main():
                   "speedup_vs_fp64": (len(lat32) / sum(lat32)) / (len(lat64) / total64),
                   "iterations": [s["its"] for s in st32], "outer": [s["outer"] for s in st32],
                   "relres": max(s["relres"] for s in st32)},
+        "reference_face_kappa_check": ref_form,
         "fp64_iterations": [s["its"] for s in st64],
         "fp64_relres": max(s["relres"] for s in st64),
         "apply_gdofs": {"f64": gdofs64, "f32": gdofs32},

@@ -31,6 +31,7 @@ class MeshDesc(C.Structure):
         ("kappa", _DP),
         ("bctype", _I64P),
         ("eta_e", C.c_double),
+        ("face_kappa", C.c_int32),
     ]
 
 

@@ -8,13 +8,13 @@
 //   smoother (SoA)     S[(n*nloc + m)*nele + e]
 //
 // The apply is two launches:
-//   k_trace     per element: u and du/dn at the element's own face points (4 faces x 12
-//               points in 3D) -> trace buffer;
-//   k_residual  per element: volume term, then for every face the SIPG flux from the own
-//               trace and the neighbour's trace at the aligned points, lifted to the test
-//               functions; writes r = b - A x, or applies the smoother x += S r.
-// One thread owns one element; the reference tables are indexed with wave-uniform
-// addresses, so they are read through the scalar cache and broadcast.
+//   k_trace_w     u and du/dn at every element's own face points (4 faces x 12 points in
+//                 3D) -> trace buffer;
+//   k_residual_w  volume term, then for every face the SIPG flux from the own trace and
+//                 the neighbour's trace at the aligned points, lifted to the test
+//                 functions; writes r = b - A x, or applies the smoother x += S r.
+// MODE 0: r = -A x; 1: r = b - A x; 2: x += S (b - A x); 3: as 2 and also store r;
+// 4: r = +A x.
 #include "dgmi_internal.h"
 
 namespace dgmi {
@@ -62,172 +62,200 @@ __device__ __forceinline__ void store_row(T* __restrict__ p, const T* v) {
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Wave-split variants.  A block of nface waves owns 64 elements (one per lane); wave w
+// handles face w and the volume points [w*ngi/nface, (w+1)*ngi/nface) of all 64
+// elements, so the tables stay wave-uniform (scalar loads) while each element gets nface
+// lanes of work.  Partial residuals are summed through LDS.
 template <typename T, int D>
-__global__ __launch_bounds__(256) void k_trace(int nele, const T* __restrict__ x,
-                                               const T* __restrict__ ftab,
-                                               const T* __restrict__ cf,
-                                               typename Vec2<T>::type* __restrict__ trace) {
+__global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __restrict__ x,
+                                                          const T* __restrict__ ftab,
+                                                          const T* __restrict__ cf,
+                                                          typename Vec2<T>::type* __restrict__ trace) {
   using M = Dm<D>;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int e = blockIdx.x * 64 + lane;
   if (e >= nele) return;
   T xv[M::NL];
   load_row<T, M::NL>(x + (size_t)e * M::NL, xv);
-  for (int f = 0; f < M::NF; ++f) {
-    T c[D];
+  T c[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
-    const T* tf = ftab + f * M::SG * (1 + D) * M::NL;
+  for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
+  const T* tf = ftab + f * M::SG * (1 + D) * M::NL;
 #pragma unroll 2
-    for (int g = 0; g < M::SG; ++g) {
-      const T* t = tf + g * (1 + D) * M::NL;
-      T u = 0, gd[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) gd[d] = 0;
-#pragma unroll
-      for (int n = 0; n < M::NL; ++n) {
-        u += t[n] * xv[n];
-#pragma unroll
-        for (int d = 0; d < D; ++d) gd[d] += t[(1 + d) * M::NL + n] * xv[n];
-      }
-      T dn = 0;
-#pragma unroll
-      for (int d = 0; d < D; ++d) dn += c[d] * gd[d];
-      typename Vec2<T>::type tv;
-      tv.x = u;
-      tv.y = dn;
-      trace[(f * M::SG + g) * nele + e] = tv;
-    }
-  }
-}
-
-// MODE 0: r = -A x; 1: r = b - A x; 2: x += S (b - A x); 3: as 2 and also store r;
-// 4: r = +A x
-template <typename T, int D, int MODE>
-__global__ __launch_bounds__(256) void k_residual(
-    int nele, T* __restrict__ x, const T* __restrict__ b, T* __restrict__ r,
-    const T* __restrict__ vtab, const T* __restrict__ wgt, const T* __restrict__ ftab,
-    const T* __restrict__ sw, const int32_t* __restrict__ aln, const T* __restrict__ G,
-    const T* __restrict__ cf, const T* __restrict__ wf, const T* __restrict__ pen,
-    const int32_t* __restrict__ nbface, const int32_t* __restrict__ fcode,
-    const typename Vec2<T>::type* __restrict__ trace, const T* __restrict__ S) {
-  using M = Dm<D>;
-  constexpr int NL = M::NL;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nele) return;
-  T xv[NL], acc[NL];
-  load_row<T, NL>(x + (size_t)e * NL, xv);
-#pragma unroll
-  for (int n = 0; n < NL; ++n) acc[n] = 0;
-
-  // ---- volume: acc_n += sum_g w_g dphi_n . (kappa detJ invJ^T invJ) grad_xi u
-  T gm[M::NG];
-#pragma unroll
-  for (int k = 0; k < M::NG; ++k) gm[k] = G[k * nele + e];
-#pragma unroll 2
-  for (int g = 0; g < M::NGI; ++g) {
-    const T* t = vtab + g * D * NL;
-    T gd[D];
+  for (int g = 0; g < M::SG; ++g) {
+    const T* t = tf + g * (1 + D) * M::NL;
+    T u = 0, gd[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) gd[d] = 0;
 #pragma unroll
-    for (int n = 0; n < NL; ++n)
+    for (int n = 0; n < M::NL; ++n) {
+      u += t[n] * xv[n];
 #pragma unroll
-      for (int d = 0; d < D; ++d) gd[d] += t[d * NL + n] * xv[n];
-    const T w = wgt[g];
-    T y[D];
-    if constexpr (D == 3) {
-      y[0] = w * (gm[0] * gd[0] + gm[3] * gd[1] + gm[4] * gd[2]);
-      y[1] = w * (gm[3] * gd[0] + gm[1] * gd[1] + gm[5] * gd[2]);
-      y[2] = w * (gm[4] * gd[0] + gm[5] * gd[1] + gm[2] * gd[2]);
-    } else {
-      y[0] = w * (gm[0] * gd[0] + gm[2] * gd[1]);
-      y[1] = w * (gm[2] * gd[0] + gm[1] * gd[1]);
+      for (int d = 0; d < D; ++d) gd[d] += t[(1 + d) * M::NL + n] * xv[n];
     }
+    T dn = 0;
 #pragma unroll
-    for (int n = 0; n < NL; ++n) {
-      T s = acc[n];
-#pragma unroll
-      for (int d = 0; d < D; ++d) s += t[d * NL + n] * y[d];
-      acc[n] = s;
-    }
+    for (int d = 0; d < D; ++d) dn += c[d] * gd[d];
+    typename Vec2<T>::type tv;
+    tv.x = u;
+    tv.y = dn;
+    trace[(f * M::SG + g) * nele + e] = tv;
   }
+}
 
-  // ---- faces
-  for (int f = 0; f < M::NF; ++f) {
-    const int z = e * M::NF + f;
-    const int code = fcode[z];
-    const int type = code & 15;
-    if (type >= 2) continue;  // faces without an operator term
-    const int a = code >> 4;
-    T c[D];
+template <typename T, int D, int MODE>
+__global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
+    int nele, T* __restrict__ x, const T* b, T* r, const T* __restrict__ vtab,
+    const T* __restrict__ wgt, const T* __restrict__ ftab, const T* __restrict__ sw,
+    const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
+    const T* __restrict__ wf, const T* __restrict__ pen, const int32_t* __restrict__ nbface,
+    const int32_t* __restrict__ fcode, const typename Vec2<T>::type* __restrict__ trace,
+    const T* __restrict__ S) {
+  using M = Dm<D>;
+  constexpr int NL = M::NL, NF = M::NF;
+  constexpr int GPW = M::NGI / NF;          // volume points per wave
+  constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
+  static_assert(M::NGI % NF == 0, "volume points must split evenly over the waves");
+  __shared__ T red[NF][NL][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int e = blockIdx.x * 64 + lane;
+  const bool valid = e < nele;
+  T xv[NL], acc[NL];
 #pragma unroll
-    for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
-    const T wfe = wf[f * nele + e];
-    const T pe = pen[f * nele + e];
-    int nbe = 0, nbf = 0;
-    if (type == 0) {
-      const int nbz = nbface[z];
-      nbe = nbz / M::NF;
-      nbf = nbz - nbe * M::NF;
-    }
-    const T* tf = ftab + f * M::SG * (1 + D) * NL;
+  for (int n = 0; n < NL; ++n) acc[n] = 0;
+  if (valid) {
+    load_row<T, NL>(x + (size_t)e * NL, xv);
+    // ---- volume points of this wave
+    T gm[M::NG];
+#pragma unroll
+    for (int k = 0; k < M::NG; ++k) gm[k] = G[k * nele + e];
 #pragma unroll 2
-    for (int g = 0; g < M::SG; ++g) {
-      const typename Vec2<T>::type tr = trace[(f * M::SG + g) * nele + e];
-      const T ws = wfe * sw[g];
-      T qsn, qdn;
-      if (type == 0) {
-        // interior: neighbour point aligned with g; grad u_N . n_E = -(grad u_N . n_N)
-        const int ga = aln[a * M::SG + g];
-        const typename Vec2<T>::type tn = trace[(nbf * M::SG + ga) * nele + nbe];
-        const T jump = tr.x - tn.x;
-        qsn = ws * (T(-0.5) * (tr.y - tn.y) + pe * jump);
-        qdn = ws * (T(-0.5) * jump);
-      } else {
-        // Dirichlet
-        qsn = ws * (pe * tr.x - tr.y);
-        qdn = -ws * tr.x;
-      }
-      T qd[D];
+    for (int gg = 0; gg < GPW; ++gg) {
+      const int g = w * GPW + gg;
+      const T* t = vtab + g * D * NL;
+      T gd[D];
 #pragma unroll
-      for (int d = 0; d < D; ++d) qd[d] = c[d] * qdn;
-      const T* t = tf + g * (1 + D) * NL;
+      for (int d = 0; d < D; ++d) gd[d] = 0;
+#pragma unroll
+      for (int n = 0; n < NL; ++n)
+#pragma unroll
+        for (int d = 0; d < D; ++d) gd[d] += t[d * NL + n] * xv[n];
+      const T wq = wgt[g];
+      T y[D];
+      if constexpr (D == 3) {
+        y[0] = wq * (gm[0] * gd[0] + gm[3] * gd[1] + gm[4] * gd[2]);
+        y[1] = wq * (gm[3] * gd[0] + gm[1] * gd[1] + gm[5] * gd[2]);
+        y[2] = wq * (gm[4] * gd[0] + gm[5] * gd[1] + gm[2] * gd[2]);
+      } else {
+        y[0] = wq * (gm[0] * gd[0] + gm[2] * gd[1]);
+        y[1] = wq * (gm[2] * gd[0] + gm[1] * gd[1]);
+      }
 #pragma unroll
       for (int n = 0; n < NL; ++n) {
-        T s = acc[n] + t[n] * qsn;
+        T s = acc[n];
 #pragma unroll
-        for (int d = 0; d < D; ++d) s += t[(1 + d) * NL + n] * qd[d];
+        for (int d = 0; d < D; ++d) s += t[d * NL + n] * y[d];
         acc[n] = s;
       }
     }
+    // ---- face w
+    const int f = w;
+    const int z = e * NF + f;
+    const int code = fcode[z];
+    const int type = code & 15;
+    if (type < 2) {
+      const int a = code >> 4;
+      T c[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
+      const T wfe = wf[f * nele + e];
+      const T pe = pen[f * nele + e];
+      int nbe = 0, nbf = 0;
+      if (type == 0) {
+        const int nbz = nbface[z];
+        nbe = nbz / NF;
+        nbf = nbz - nbe * NF;
+      }
+      const T* tf = ftab + f * M::SG * (1 + D) * NL;
+#pragma unroll 2
+      for (int g = 0; g < M::SG; ++g) {
+        const typename Vec2<T>::type tr = trace[(f * M::SG + g) * nele + e];
+        const T ws = wfe * sw[g];
+        T qsn, qdn;
+        if (type == 0) {
+          const int ga = aln[a * M::SG + g];
+          const typename Vec2<T>::type tn = trace[(nbf * M::SG + ga) * nele + nbe];
+          const T jump = tr.x - tn.x;
+          qsn = ws * (T(-0.5) * (tr.y - tn.y) + pe * jump);
+          qdn = ws * (T(-0.5) * jump);
+        } else {
+          qsn = ws * (pe * tr.x - tr.y);
+          qdn = -ws * tr.x;
+        }
+        T qd[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) qd[d] = c[d] * qdn;
+        const T* t = tf + g * (1 + D) * NL;
+#pragma unroll
+        for (int n = 0; n < NL; ++n) {
+          T s = acc[n] + t[n] * qsn;
+#pragma unroll
+          for (int d = 0; d < D; ++d) s += t[(1 + d) * NL + n] * qd[d];
+          acc[n] = s;
+        }
+      }
+    }
   }
-
-  // ---- epilogue
-  T rv[NL];
-  if constexpr (MODE == 0) {
 #pragma unroll
-    for (int n = 0; n < NL; ++n) rv[n] = -acc[n];
-  } else if constexpr (MODE == 4) {
+  for (int n = 0; n < NL; ++n) red[w][n][lane] = acc[n];
+  __syncthreads();
+  // rows n of this wave: sum the partials, form the residual
+  T rv[NPW];
 #pragma unroll
-    for (int n = 0; n < NL; ++n) rv[n] = acc[n];
-  } else {
-    T bv[NL];
-    load_row<T, NL>(b + (size_t)e * NL, bv);
+  for (int k = 0; k < NPW; ++k) {
+    const int n = w * NPW + k;
+    if (n < NL) {
+      T s = 0;
 #pragma unroll
-    for (int n = 0; n < NL; ++n) rv[n] = bv[n] - acc[n];
+      for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
+      if constexpr (MODE == 0) rv[k] = -s;
+      else if constexpr (MODE == 4) rv[k] = s;
+      else rv[k] = valid ? b[(size_t)e * NL + n] - s : T(0);
+    }
   }
   if constexpr (MODE <= 1 || MODE == 4) {
-    store_row<T, NL>(r + (size_t)e * NL, rv);
-  } else {
-    if constexpr (MODE == 3) store_row<T, NL>(r + (size_t)e * NL, rv);
+    if (valid) {
 #pragma unroll
-    for (int n = 0; n < NL; ++n) {
-      T s = xv[n];
-#pragma unroll
-      for (int m = 0; m < NL; ++m) s += S[(n * NL + m) * nele + e] * rv[m];
-      xv[n] = s;
+      for (int k = 0; k < NPW; ++k) {
+        const int n = w * NPW + k;
+        if (n < NL) r[(size_t)e * NL + n] = rv[k];
+      }
     }
-    store_row<T, NL>(x + (size_t)e * NL, xv);
+  } else {
+    // smoothing: every wave needs the whole residual row; exchange it through red[0]
+    // (each wave only read red[*][its rows], so overwriting red[0][its rows] is safe)
+#pragma unroll
+    for (int k = 0; k < NPW; ++k) {
+      const int n = w * NPW + k;
+      if (n < NL) red[0][n][lane] = rv[k];
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < NPW; ++k) {
+        const int n = w * NPW + k;
+        if (n < NL) {
+          T s = xv[n];
+#pragma unroll
+          for (int m = 0; m < NL; ++m) s += S[(n * NL + m) * nele + e] * red[0][m][lane];
+          x[(size_t)e * NL + n] = s;
+          if constexpr (MODE == 3) r[(size_t)e * NL + n] = rv[k];
+        }
+      }
+    }
   }
 }
 
@@ -418,14 +446,14 @@ template <typename T, int D>
 static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mode,
                       hipStream_t s) {
   const int nele = (int)c->nele;
-  const dim3 blk(256), grd((nele + 255) / 256);
-  hipLaunchKernelGGL((k_trace<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf, g.trace);
+  const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
+  hipLaunchKernelGGL((k_trace_w<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf, g.trace);
   const T* vt = g.tab + g.off_vtab;
   const T* wg = g.tab + g.off_wgt;
   const T* ft = g.tab + g.off_ftab;
   const T* sw = g.tab + g.off_sw;
-#define DGMI_RES(MODE)                                                                     \
-  hipLaunchKernelGGL((k_residual<T, D, MODE>), grd, blk, 0, s, nele, x, b, r, vt, wg, ft, sw, \
+#define DGMI_RES(MODE)                                                                       \
+  hipLaunchKernelGGL((k_residual_w<T, D, MODE>), grd, blk, 0, s, nele, x, b, r, vt, wg, ft, sw, \
                      c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, g.trace, g.S)
   switch (mode) {
     case 0: DGMI_RES(0); break;

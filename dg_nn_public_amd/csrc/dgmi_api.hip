@@ -2,6 +2,7 @@
 // multigrid set-up, the V-cycle preconditioner and the GMRES driver.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <numeric>
@@ -385,7 +386,9 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
   }
   Geometry geo;
   std::string err;
-  if (!build_geometry(c->el, d->x_ref_in, c->kappa.data(), c->topo, c->nele, c->eta_e, geo, err)) {
+  DGMI_CHECK(d->face_kappa == 0 || d->face_kappa == 1, "face_kappa must be 0 or 1");
+  if (!build_geometry(c->el, d->x_ref_in, c->kappa.data(), c->topo, c->nele, c->eta_e,
+                      d->face_kappa, geo, err)) {
     delete c;
     set_error(err);
     return -2;
@@ -668,7 +671,10 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
                "device allocation failed");
   }
   // levels with at most SMALL_N nodes run in the single-work-group kernel
-  const int64_t SMALL_N = 4096;
+  // (DGMI_SMALL_LEVEL_N overrides the threshold, e.g. to exercise the grid kernels on
+  // small meshes in tests)
+  int64_t SMALL_N = 4096;
+  if (const char* ev = std::getenv("DGMI_SMALL_LEVEL_N")) SMALL_N = std::max<int64_t>(1, std::atoll(ev));
   int l0 = L - 1;
   for (int l = 0; l < L; ++l)
     if (lh.n[l] <= SMALL_N) {

@@ -279,7 +279,7 @@ Topology build_topology(const int64_t* cells, int64_t nele, int D) {
 }
 
 bool build_geometry(const ElementTables& el, const double* x, const double* kappa,
-                    const Topology& topo, int64_t nele, double eta_e, Geometry& g,
+                    const Topology& topo, int64_t nele, double eta_e, int face_kappa, Geometry& g,
                     std::string& err) {
   const int D = el.D, nloc = el.nloc, nf = el.nface, nb = D + 1;
   const int ng = D == 3 ? 6 : 3;
@@ -391,7 +391,8 @@ bool build_geometry(const ElementTables& el, const double* x, const double* kapp
       double kf = 0.0;
       if (type < 0 && topo.nbf[z] >= 0) {
         const int64_t nbe = topo.nbf[z] / nf;
-        kf = 0.5 * (ke + kappa[nbe]);
+        const double kn = kappa[nbe];
+        kf = face_kappa == 1 ? 2.0 * ke * kn / (ke + kn) : 0.5 * (ke + kn);
         g.nbface[z] = (int32_t)topo.nbf[z];
         g.fcode[z] = 0 | ((int32_t)topo.alnmt[z] << 4);
       } else if (type == 0) {

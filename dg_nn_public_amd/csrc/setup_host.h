@@ -49,7 +49,7 @@ struct Geometry {
   double max_affine_dev = 0.0;  // max |x_node - affine position| / h
 };
 bool build_geometry(const ElementTables& el, const double* x_ref_in, const double* kappa,
-                    const Topology& topo, int64_t nele, double eta_e, Geometry& g,
+                    const Topology& topo, int64_t nele, double eta_e, int face_kappa, Geometry& g,
                     std::string& err);
 
 // P1CG CSR graph (z02/sparsity.py:102-122): all vertex pairs per element, sorted.

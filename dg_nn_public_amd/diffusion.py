@@ -52,7 +52,11 @@ class DGDiffusion:
     """One P3 DG mesh, its SIPG operator and SFC multigrid hierarchy on one GPU."""
 
     def __init__(self, x_ref_in, cells, verts, kappa=None, bctype=None, eta_e: float = ETA_E,
-                 blk_solver: str = "jacobian", jac_wei: float = JAC_WEI, device="cuda:0"):
+                 blk_solver: str = "jacobian", jac_wei: float = JAC_WEI, device="cuda:0",
+                 face_kappa: str = "mean"):
+        """face_kappa: "mean" is the reference's interior-face diffusivity (kE + kN)/2;
+        "harmonic" uses 2 kE kN/(kE + kN) (symmetric weighted interior penalty), which
+        keeps the operator positive definite at high contrast."""
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -81,7 +85,8 @@ class DGDiffusion:
             D=D, nele=nele, nverts=verts.shape[0],
             x_ref_in=x.ctypes.data_as(_lib._DP), cells=cells.ctypes.data_as(_lib._I64P),
             verts=verts.ctypes.data_as(_lib._DP), kappa=kappa.ctypes.data_as(_lib._DP),
-            bctype=bc.ctypes.data_as(_lib._I64P) if bc is not None else None, eta_e=eta_e)
+            bctype=bc.ctypes.data_as(_lib._I64P) if bc is not None else None, eta_e=eta_e,
+            face_kappa={"mean": 0, "harmonic": 1}[face_kappa])
         ctx = C.c_void_p()
         check(self.lib.dgmi_ctx_create(C.byref(desc), C.byref(ctx)), "dgmi_ctx_create")
         self.ctx = ctx
@@ -95,6 +100,7 @@ class DGDiffusion:
         self.verts = verts
         self.kappa = kappa
         self.blk_solver = blk_solver
+        self.face_kappa = face_kappa
         self.jac_wei = jac_wei
         self._smoother_ready = False
         self.diagK = None

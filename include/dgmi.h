@@ -44,6 +44,10 @@ typedef struct {
                               Dirichlet (0), matched faces interior (-1); other types
                               carry no operator term (z02/mesh_init.py:697-741)          */
   double eta_e;            /* penalty coefficient (z02/config.py:245: 36)                 */
+  int32_t face_kappa;      /* interior-face diffusivity: 0 = arithmetic mean, the
+                              reference's (z05/volume_mf_um_diff_varprop.py:567);
+                              1 = harmonic mean (symmetric weighted interior penalty,
+                              coercive at any contrast)                                  */
 } dgmi_mesh_desc;
 
 /* Context: topology (z02/mesh_init.py:260-658), reference-element tables

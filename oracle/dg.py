@@ -95,6 +95,7 @@ class Space:
     sdetwei: np.ndarray = None   # (nele, nface, sngi)
     gamma: np.ndarray = None     # (nele, nface) eta/h
     eta_e: float = ETA_E
+    face_kappa: str = "mean"
 
     @property
     def D(self):
@@ -117,7 +118,20 @@ def _inv_det(J):
     return np.linalg.inv(J), np.linalg.det(J)
 
 
-def make_space(mesh, kappa=None, eta_e: float = ETA_E) -> Space:
+def face_kappa_value(kE, kN, mode="mean"):
+    """Interior-face diffusivity.  "mean": (kE + kN)/2, the reference's choice
+    (z05/volume_mf_um_diff_varprop.py:567).  "harmonic": 2 kE kN/(kE + kN), which is the
+    symmetric weighted interior penalty of Ern, Stephansen & Zunino (IMA J. Numer. Anal.
+    29, 2009) -- its diffusivity-weighted average {kappa grad u}_w equals the harmonic
+    mean times the plain average -- and keeps the form coercive at any contrast."""
+    if mode == "mean":
+        return 0.5 * (kE + kN)
+    if mode == "harmonic":
+        return 2.0 * kE * kN / (kE + kN)
+    raise ValueError("face_kappa must be 'mean' or 'harmonic'")
+
+
+def make_space(mesh, kappa=None, eta_e: float = ETA_E, face_kappa: str = "mean") -> Space:
     D = mesh.D
     el = tables.make_element(D)
     nbf, alnmt, bctype = topology(mesh.cells, D)
@@ -126,7 +140,8 @@ def make_space(mesh, kappa=None, eta_e: float = ETA_E) -> Space:
     if kappa is None:
         kappa = np.ones(nele)
     sp = Space(el=el, nele=nele, x_ref_in=x, kappa=np.asarray(kappa, dtype=np.float64),
-               nbf=nbf, alnmt=alnmt, bctype=bctype, cells=mesh.cells, eta_e=eta_e)
+               nbf=nbf, alnmt=alnmt, bctype=bctype, cells=mesh.cells, eta_e=eta_e,
+               face_kappa=face_kappa)
     # volume: J[i][j] = sum_n nlx[i,n,g] x[j,n]  (z02/volume_mf_diff.py:536-545)
     Jv = np.einsum("ing,ejn->egij", el.nlx, x)
     invJ, det = _inv_det(Jv)
@@ -183,7 +198,7 @@ def _face_cache(sp: Space):
     dnv_nb = np.einsum("zgin,zgi->zng", snx_nb, nrm[zi])
     E = gf // nface
     F = gf % nface
-    kap_i = 0.5 * (sp.kappa[E[zi]] + sp.kappa[nb // nface])
+    kap_i = face_kappa_value(sp.kappa[E[zi]], sp.kappa[nb // nface], sp.face_kappa)
     sp._fc = dict(dnv=dnv, zi=zi, zb=zb, nb_e=nb // nface, sn_nb=sn_nb, dnv_nb=dnv_nb,
                   E=E, F=F, kap_i=kap_i,
                   sdw=sp.sdetwei.reshape(-1, el.sngi), gam=sp.gamma.reshape(-1))
@@ -292,7 +307,7 @@ def get_bdiag_diag(sp: Space):
         interior = sp.bctype[gfi] < 0
         dirich = sp.bctype[gfi] == 0
         nbe = np.where(interior, sp.nbf[gfi] // nface, 0)
-        kap_i = 0.5 * (sp.kappa + sp.kappa[nbe])
+        kap_i = face_kappa_value(sp.kappa, sp.kappa[nbe], sp.face_kappa)
         Ki = (-0.5 * sd - 0.5 * sd.transpose(0, 2, 1)
               + sp.gamma[:, f][:, None, None] * ss) * kap_i[:, None, None]
         Kb = (-sd - sd.transpose(0, 2, 1)

@@ -9,11 +9,12 @@ from oracle import dg, mg
 DEV = "cuda:0"
 
 
-def build(D, n, sigma=0.0, seed=0, blk_solver="jacobian"):
+def build(D, n, sigma=0.0, seed=0, blk_solver="jacobian", face_kappa="mean"):
     m = meshgen.square_mesh(n, seed=seed) if D == 2 else meshgen.cube_mesh(n, seed=seed)
     pb = meshgen.sine_problem(m, sigma=sigma)
-    g = DGDiffusion.from_mesh(m, pb.kappa, blk_solver=blk_solver, device=DEV)
-    sp = dg.make_space(m, pb.kappa)
+    g = DGDiffusion.from_mesh(m, pb.kappa, blk_solver=blk_solver, device=DEV,
+                              face_kappa=face_kappa)
+    sp = dg.make_space(m, pb.kappa, face_kappa=face_kappa)
     return m, pb, g, sp
 
 

@@ -63,3 +63,20 @@ def test_preconditioner_matches_oracle(prob):
     assert rel(out, ref) < 1e-10
     out32 = to_np(g._um_left_precond(g.new_vector(torch.float32), to_dev(b, torch.float32)))
     assert rel(out32, ref) < 1e-4
+
+
+def test_grid_level_kernels_match_oracle(monkeypatch):
+    """Force the grid-sized SFC-level kernels (k_sweep / k_sweep_restrict) on a small mesh
+    by lowering the single-work-group threshold, and compare V-cycle and preconditioner."""
+    monkeypatch.setenv("DGMI_SMALL_LEVEL_N", "8")
+    m, pb, g, sp = build(3, 4, sigma=1.0)
+    g._get_RAR_and_sfc_data_Um()
+    mgd = mg.build_mg(sp, m.verts)
+    r = np.random.default_rng(6).standard_normal(m.nverts)
+    ref = mg.mg_on_P1CG(r, mgd.levels)
+    out = to_np(g.mg_on_P1CG(to_dev(r)))
+    print("grid-level vcycle rel err", rel(out, ref))
+    assert rel(out, ref) < 1e-11
+    b = np.random.default_rng(7).standard_normal(sp.ndof)
+    ref = mg.um_left_precond(sp, mgd, b)
+    assert rel(to_np(g._um_left_precond(g.new_vector(), to_dev(b))), ref) < 1e-10

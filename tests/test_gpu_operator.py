@@ -113,6 +113,15 @@ def test_bad_arguments_raise(prob):
         g.apply(torch.zeros(sp.ndof, dtype=torch.float64))
 
 
+@pytest.mark.parametrize("D,n", [(2, 6), (3, 3)])
+def test_harmonic_face_kappa_apply_and_bdiag_match_oracle(D, n):
+    m, pb, g, sp = build(D, n, sigma=2.0, face_kappa="harmonic")
+    x = np.random.default_rng(13).standard_normal(sp.ndof)
+    assert rel(-to_np(g.apply(to_dev(x))), dg.apply_A(sp, x)) < 1e-12
+    d, B = g.get_bdiag_diag()
+    assert rel(to_np(B), dg.get_bdiag_diag(sp)[1]) < 1e-12
+
+
 def test_full_size_3d_apply_properties():
     """The 3D workload size (24^3 x 6 tets, 1.66M DOFs): FP64 apply against the oracle,
     exactness on linear fields off the boundary, FP32 against FP64."""

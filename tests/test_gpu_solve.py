@@ -34,6 +34,22 @@ def test_gmres_reaches_tolerance_and_matches_oracle(prob, inner):
     assert rel(xs, x_ref) < 1e-7
 
 
+@pytest.mark.parametrize("inner", [torch.float64, torch.float32], ids=["f64", "f32-ir"])
+def test_high_contrast_harmonic_solve_converges(inner):
+    """sigma = 2 lognormal diffusivity with the harmonic face coefficient: both solvers
+    reach 1e-10 and agree with the oracle's solve."""
+    m, pb, g, sp = build(3, 6, sigma=2.0, face_kappa="harmonic")
+    b = dg.get_rhs(sp, pb.f, pb.u_bc)
+    x, its = g._gmres_mg_solver(g.new_vector(), to_dev(b), rtol=1e-10, inner=inner)
+    xs = to_np(x)
+    assert np.linalg.norm(b - dg.apply_A(sp, xs)) <= 2e-10 * np.linalg.norm(b)
+    mgd = mg.build_mg(sp, m.verts)
+    x_ref, _ = krylov.gmres_mg_solver(sp, mgd, np.zeros(sp.ndof), b, rtol=1e-12, m=30,
+                                      max_its=600)
+    print(inner, "its", its, g.last_stats, "vs oracle", rel(xs, x_ref))
+    assert rel(xs, x_ref) < 1e-7
+
+
 def test_solve_for_diff_matches_manufactured_solution():
     m, pb, g, sp = build(2, 8, sigma=0.0)
     x = g.solve_for_diff(g.new_vector(), pb.f, pb.u_bc)

@@ -33,6 +33,27 @@ def test_spd_for_moderate_contrast(small):
     assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() > 0
 
 
+def test_reference_mean_loses_definiteness_harmonic_keeps_it():
+    """At high contrast the reference's arithmetic-mean face diffusivity gives an
+    indefinite operator; the harmonic mean (weighted interior penalty) stays SPD and
+    equals the mean where kappa is continuous."""
+    m = _mesh(3, 3)
+    kap = np.exp(2.0 * np.random.default_rng(1).standard_normal(m.nele))
+    A_mean = dg.assemble_dense(dg.make_space(m, kap, face_kappa="mean"))
+    sp_h = dg.make_space(m, kap, face_kappa="harmonic")
+    A_h = dg.assemble_dense(sp_h)
+    assert np.linalg.eigvalsh(0.5 * (A_mean + A_mean.T)).min() < 0
+    assert np.linalg.eigvalsh(0.5 * (A_h + A_h.T)).min() > 0
+    assert np.abs(A_h - A_h.T).max() <= 1e-13 * np.abs(A_h).max()
+    d, B = dg.get_bdiag_diag(sp_h)
+    nl = sp_h.nloc
+    assert np.abs(np.stack([A_h[e * nl:(e + 1) * nl, e * nl:(e + 1) * nl]
+                            for e in range(m.nele)]) - B).max() <= 1e-12 * np.abs(B).max()
+    one = np.ones(m.nele)
+    assert np.allclose(dg.assemble_dense(dg.make_space(m, one, face_kappa="harmonic")),
+                       dg.assemble_dense(dg.make_space(m, one)), rtol=0, atol=1e-12)
+
+
 def test_bdiag_is_diagonal_block_of_probed_matrix(small):
     _, sp, A = small
     d, B = dg.get_bdiag_diag(sp)
