@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--face-kappa", default="harmonic", choices=["harmonic", "mean"],
                     help="interior-face diffusivity; 'mean' is the reference's arithmetic "
                          "mean, which is not coercive at sigma=1 and stagnates")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="only operator applies (FP64, FP32) and a 256 MiB elementwise copy "
+                         "that calibrates HBM byte counters; for rocprofv3 --pmc passes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
     args = ap.parse_args()
@@ -97,6 +100,16 @@ def main():
     t_apply32 = time_applies(torch.float32, args.applies)
     print(f"[bench] apply f64 {t_apply64*1e6:.1f} us, f32 {t_apply32*1e6:.1f} us",
           file=sys.stderr, flush=True)
+    if args.apply_only:
+        # calibration: elementwise copy of 256 MiB (reads 256 MiB, writes 256 MiB)
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.add(src, 0.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"apply_us": {"f64": t_apply64 * 1e6, "f32": t_apply32 * 1e6},
+                          "calibration_copy_bytes": 2 * src.numel() * 8}))
+        return
 
     # ---- solves
     def run_solves(inner, k, w):

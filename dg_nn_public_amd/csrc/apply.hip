@@ -62,6 +62,30 @@ __device__ __forceinline__ void store_row(T* __restrict__ p, const T* v) {
   }
 }
 
+// Smoother matrices are stored as structure-of-vectors: entry S[n][m] of element e lives in
+// vector ((n*(NL/W) + m/W)*nele + e), component m%W, with W the row vector width above, so
+// one wave-wide load moves 64 x 16 bytes.
+template <typename T, int NL>
+__host__ __device__ inline size_t s_index(int n, int m, int e, int nele) {
+  constexpr int W = RowVec<T, NL>::W;
+  return ((size_t)(n * (NL / W) + m / W) * nele + e) * W + (m % W);
+}
+
+// row n of S for element e
+template <typename T, int NL>
+__device__ __forceinline__ void load_S_row(const T* __restrict__ S, int n, int e, int nele, T* v) {
+  using RV = RowVec<T, NL>;
+  constexpr int W = RV::W;
+  const typename RV::V* q = reinterpret_cast<const typename RV::V*>(S);
+#pragma unroll
+  for (int i = 0; i < NL / W; ++i) {
+    typename RV::V t = q[(size_t)(n * (NL / W) + i) * nele + e];
+    const T* tt = reinterpret_cast<const T*>(&t);
+#pragma unroll
+    for (int k = 0; k < W; ++k) v[i * W + k] = tt[k];
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Wave-split variants.  A block of nface waves owns 64 elements (one per lane); wave w
 // handles face w and the volume points [w*ngi/nface, (w+1)*ngi/nface) of all 64
@@ -248,9 +272,11 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
       for (int k = 0; k < NPW; ++k) {
         const int n = w * NPW + k;
         if (n < NL) {
+          T srow[NL];
+          load_S_row<T, NL>(S, n, e, nele, srow);
           T s = xv[n];
 #pragma unroll
-          for (int m = 0; m < NL; ++m) s += S[(n * NL + m) * nele + e] * red[0][m][lane];
+          for (int m = 0; m < NL; ++m) s += srow[m] * red[0][m][lane];
           x[(size_t)e * NL + n] = s;
           if constexpr (MODE == 3) r[(size_t)e * NL + n] = rv[k];
         }
@@ -270,9 +296,11 @@ __global__ __launch_bounds__(256) void k_smooth_zero(int nele, T* __restrict__ x
   load_row<T, NL>(b + (size_t)e * NL, bv);
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
+    T srow[NL];
+    load_S_row<T, NL>(S, n, e, nele, srow);
     T s = 0;
 #pragma unroll
-    for (int m = 0; m < NL; ++m) s += S[(n * NL + m) * nele + e] * bv[m];
+    for (int m = 0; m < NL; ++m) s += srow[m] * bv[m];
     xv[n] = s;
   }
   store_row<T, NL>(x + (size_t)e * NL, xv);
@@ -423,7 +451,7 @@ __global__ __launch_bounds__(256) void k_build_S_jacobi(int nele, const double* 
     for (int n = 0; n < NL; ++n) c[n] = cn[n];
   }
 #pragma unroll
-  for (int n = 0; n < NL; ++n) S[(n * NL + m) * nele + e] = (T)(omega * c[n]);
+  for (int n = 0; n < NL; ++n) S[s_index<T, NL>(n, m, e, nele)] = (T)(omega * c[n]);
 }
 
 __global__ void k_extract_diag(int64_t n, int nl, const double* __restrict__ B,

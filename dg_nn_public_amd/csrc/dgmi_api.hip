@@ -313,9 +313,17 @@ static int gmres_cycle(dgmi_ctx* c, const T* r0, T* d, int m, double eps, int* i
     DGMI_TRY(launch_apply_pos<T>(c, geom<T>(c), Vj(j), t, s));  // t = A v_j
     DGMI_TRY(precond<T>(c, t, w, s));                           // w = M^-1 A v_j
     double* hcol = k->H + (size_t)j * ldh;
-    for (int i = 0; i <= j; ++i) {
-      DGMI_TRY(launch_dot<T>(n, w, Vj(i), hcol + i, 0, s));
-      DGMI_TRY(kry::axpy_dev<T>(n, hcol + i, -1.0, Vj(i), w, s));
+    if (c->gmres_orth == 0) {
+      // modified Gram-Schmidt, as the reference (z02/volume_mf_diff.py:1813-1816)
+      for (int i = 0; i <= j; ++i) {
+        DGMI_TRY(launch_dot<T>(n, w, Vj(i), hcol + i, 0, s));
+        DGMI_TRY(kry::axpy_dev<T>(n, hcol + i, -1.0, Vj(i), w, s));
+      }
+    } else {
+      // classical Gram-Schmidt with one re-orthogonalisation (CGS2): two fused passes,
+      // the Hessenberg column is the sum of both passes' coefficients
+      DGMI_TRY(kry::cgs_pass<T>(n, j + 1, V, n, w, hcol, 0, s));
+      DGMI_TRY(kry::cgs_pass<T>(n, j + 1, V, n, w, hcol, 1, s));
     }
     DGMI_TRY(launch_dot<T>(n, w, w, k->red, 1, s));
     DGMI_TRY(kry::store_h(k->red + 1, hcol + j + 1, 0, s));
@@ -536,15 +544,25 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
   } else {
     const int NL = c->nloc, NN = NL * NL;
     std::vector<double> B((size_t)nS), St((size_t)nS);
+    std::vector<float> Sf((size_t)nS);
     DGMI_HIP(hipMemcpyAsync(B.data(), sc.bdiag, sizeof(double) * nS, hipMemcpyDeviceToHost, s));
     DGMI_HIP(hipStreamSynchronize(s));
-    for (int64_t e = 0; e < c->nele; ++e) {
+    // structure-of-vectors layout of the kernels (width W: 16 bytes, or 8 when the row
+    // length is not a multiple of 16 bytes)
+    auto width = [&](size_t bytes) { return (NL % (16 / bytes) == 0) ? 16 / bytes : 8 / bytes; };
+    const int W64 = (int)width(8), W32 = (int)width(4);
+    const int64_t ne = c->nele;
+    for (int64_t e = 0; e < ne; ++e) {
       double* blk = B.data() + e * NN;
       DGMI_CHECK(invert_block(blk, NL), "singular diagonal block " + std::to_string(e));
-      for (int k = 0; k < NN; ++k) St[(size_t)k * c->nele + e] = omega * blk[k];
+      for (int n = 0; n < NL; ++n)
+        for (int m = 0; m < NL; ++m) {
+          const double v = omega * blk[n * NL + m];
+          St[((size_t)(n * (NL / W64) + m / W64) * ne + e) * W64 + m % W64] = v;
+          Sf[((size_t)(n * (NL / W32) + m / W32) * ne + e) * W32 + m % W32] = (float)v;
+        }
     }
     DGMI_HIP(hipMemcpy(c->g64.S, St.data(), sizeof(double) * nS, hipMemcpyHostToDevice));
-    std::vector<float> Sf(St.begin(), St.end());
     DGMI_HIP(hipMemcpy(c->g32.S, Sf.data(), sizeof(float) * nS, hipMemcpyHostToDevice));
   }
   c->smoother_variant = variant;
@@ -673,15 +691,23 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   // levels with at most SMALL_N nodes run in the single-work-group kernel
   // (DGMI_SMALL_LEVEL_N overrides the threshold, e.g. to exercise the grid kernels on
   // small meshes in tests)
-  int64_t SMALL_N = 4096;
-  if (const char* ev = std::getenv("DGMI_SMALL_LEVEL_N")) SMALL_N = std::max<int64_t>(1, std::atoll(ev));
-  int l0 = L - 1;
-  for (int l = 0; l < L; ++l)
-    if (lh.n[l] <= SMALL_N) {
-      l0 = std::min(l, L - 1);
-      break;
-    }
+  // The coarse end of the hierarchy that fits in LDS (120 KiB of the 160 KiB per work-group)
+  // runs in one work-group; larger levels run as grid kernels.
+  lv.nnz.resize(L);
+  for (int l = 0; l < L; ++l) lv.nnz[l] = (int64_t)lh.col[l].size();
+  int l0 = choose_small_level(lv, lv.nnz, 120 * 1024);
+  DGMI_CHECK(l0 >= 0, "too many SFC levels");
+  if (const char* ev = std::getenv("DGMI_SMALL_LEVEL_N")) {
+    const int64_t small_n = std::max<int64_t>(1, std::atoll(ev));
+    for (int l = 0; l < L; ++l)
+      if (lh.n[l] <= small_n) {
+        l0 = std::max(l0, std::min(l, L - 1));
+        break;
+      }
+  }
   mg.small_level = l0;
+  DGMI_TRY(set_small_lds_attr<double>(lds_bytes_from<double>(lv, lv.nnz, l0)));
+  DGMI_TRY(set_small_lds_attr<float>(lds_bytes_from<float>(lv, lv.nnz, l0)));
   DGMI_TRY(upload_level_descs(c));
   mg.ready = true;
   return 0;
@@ -810,6 +836,18 @@ int dgmi_gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, double 
     stats[3] = cycles;
   }
   return 0;
+}
+
+int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
+  DGMI_CHECK(c && name, "null argument");
+  const std::string k(name);
+  if (k == "gmres_orth") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "gmres_orth must be 0 (MGS) or 1 (CGS2)");
+    c->gmres_orth = (int)value;
+    return 0;
+  }
+  set_error("unknown parameter " + k);
+  return -2;
 }
 
 int dgmi_dot(int dtype, int64_t n, const void* x, const void* y, double* out, void* stream) {

@@ -44,7 +44,7 @@ template <typename T> struct DevGeom {
 
 struct SfcLevelsDev {
   int nlevel = 0;
-  std::vector<int64_t> n;
+  std::vector<int64_t> n, nnz;
   std::vector<int32_t*> rowptr, col;
   std::vector<double*> val64, diag64;
   std::vector<float*> val32, diag32;
@@ -71,6 +71,7 @@ struct MGDev {
   int32_t* cellpos = nullptr;
   SfcLevelsDev lv;
   int small_level = 0;  // first level handled by the single-workgroup kernel
+  bool small_lds = true;  // stage the small levels in LDS
   void* r_sfc = nullptr;  // (cg_nonods) fp64-sized scratch
   void* e_sfc = nullptr;
 };
@@ -93,6 +94,9 @@ int givens(int j, int ldh, double* H, double* cs, double* sn, double* g, double*
 int backsolve(int k, int ldh, const double* H, const double* g, double* y, hipStream_t s);
 int store_h(const double* src, double* dst, int mode, hipStream_t s);
 int set_g(double* g, int len, const double* beta, hipStream_t s);
+template <typename T>
+int cgs_pass(int64_t n, int k, const T* V, int64_t ldv, T* w, double* h, int accumulate,
+             hipStream_t s);
 }  // namespace kry
 
 }  // namespace dgmi
@@ -112,6 +116,7 @@ struct dgmi_ctx {
   dgmi::DevGeom<double> g64;
   dgmi::DevGeom<float> g32;
   int smoother_variant = -1;
+  int gmres_orth = 1;  // 0: modified Gram-Schmidt, 1: CGS2 (fused passes)
   double omega = 2.0 / 3.0;
   dgmi::MGDev mg;
   // solver scratch (FP64 and FP32 vectors of ndof), created on demand
@@ -146,6 +151,11 @@ int launch_prolong_add(dgmi_ctx* c, const T* e_sfc, T* x_dg, hipStream_t s);
 template <typename T>
 int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s);
 int upload_level_descs(dgmi_ctx* c);
+int choose_small_level(const SfcLevelsDev& lv, const std::vector<int64_t>& nnz, size_t budget);
+template <typename T>
+size_t lds_bytes_from(const SfcLevelsDev& lv, const std::vector<int64_t>& nnz, int l0);
+template <typename T>
+int set_small_lds_attr(size_t bytes);
 int launch_probe_mask(dgmi_ctx* c, const int32_t* whichc_dev, int color, double* x_dg,
                       hipStream_t s);
 int launch_probe_collect(dgmi_ctx* c, const double* r_sfc, const int32_t* spidx,

@@ -154,7 +154,76 @@ __global__ void k_set_g(double* __restrict__ g, int len, const double* __restric
 }
 
 // ---------------------------------------------------------------------------------------
+// Fused classical Gram-Schmidt pass over up to MDOT_K basis vectors:
+//   h[j] = <w, V_j> (one read of w for all j), then w -= sum_j h[j] V_j.
+constexpr int MDOT_K = 32;
+constexpr int MDOT_BLOCKS = 512;
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_mdot_partial(int64_t n, int k,
+                                                              const T* __restrict__ V,
+                                                              int64_t ldv,
+                                                              const T* __restrict__ w,
+                                                              double* __restrict__ partial) {
+  double acc[MDOT_K];
+#pragma unroll
+  for (int j = 0; j < MDOT_K; ++j) acc[j] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double wi = (double)w[i];
+#pragma unroll
+    for (int j = 0; j < MDOT_K; ++j)
+      if (j < k) acc[j] += wi * (double)V[(int64_t)j * ldv + i];
+  }
+  __shared__ double sh[RED_THREADS / 64][MDOT_K];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < MDOT_K; ++j) {
+    if (j < k) {
+      const double v = wave_sum(acc[j]);
+      if (lane == 0) sh[wid][j] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double s = 0.0;
+    for (int ww = 0; ww < RED_THREADS / 64; ++ww) s += sh[ww][threadIdx.x];
+    partial[(int64_t)blockIdx.x * MDOT_K + threadIdx.x] = s;
+  }
+}
+
+// h_out[j] (+)= s_j and tmp[j] = s_j, s_j = sum over blocks of partial[b][j]
+__global__ __launch_bounds__(RED_THREADS) void k_mdot_final(const double* __restrict__ partial,
+                                                            int nb, int k,
+                                                            double* __restrict__ h_out,
+                                                            double* __restrict__ tmp,
+                                                            int accumulate) {
+  for (int j = 0; j < k; ++j) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(int64_t)b * MDOT_K + j];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+      tmp[j] = s;
+      h_out[j] = accumulate ? h_out[j] + s : s;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void k_maxpy(int64_t n, int k, const T* __restrict__ V, int64_t ldv,
+                        const double* __restrict__ h, T* __restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = (double)w[i];
+  for (int j = 0; j < k; ++j) s -= h[j] * (double)V[(int64_t)j * ldv + i];
+  w[i] = (T)s;
+}
+
+// ---------------------------------------------------------------------------------------
 static double* g_partial = nullptr;
+static double* g_mpartial = nullptr;
+static double* g_mtmp = nullptr;
 
 static int ensure_partial() {
   if (!g_partial) DGMI_HIP(hipMalloc(&g_partial, sizeof(double) * RED_BLOCKS));
@@ -186,6 +255,34 @@ template int launch_axpy<float>(int64_t, double, const float*, float*, hipStream
 
 // exported to the solver translation unit
 namespace kry {
+// One classical Gram-Schmidt pass of w against V_0..V_{k-1}; writes (or, with
+// accumulate, adds) the coefficients into h.  Chunks of MDOT_K vectors.
+template <typename T>
+int cgs_pass(int64_t n, int k, const T* V, int64_t ldv, T* w, double* h, int accumulate,
+             hipStream_t s) {
+  if (!g_mpartial) DGMI_HIP(hipMalloc(&g_mpartial, sizeof(double) * MDOT_BLOCKS * MDOT_K));
+  if (!g_mtmp) DGMI_HIP(hipMalloc(&g_mtmp, sizeof(double) * MDOT_K));
+  const int nb = (int)std::min<int64_t>(MDOT_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
+  const dim3 grd((unsigned)((n + 255) / 256));
+  // within a chunk all dots use the same w (classical GS); successive chunks see the
+  // updated w (block-modified GS), which only matters for restart lengths above MDOT_K
+  for (int c0 = 0; c0 < k; c0 += MDOT_K) {
+    const int kc = std::min(MDOT_K, k - c0);
+    hipLaunchKernelGGL((k_mdot_partial<T>), dim3(nb), dim3(RED_THREADS), 0, s, n, kc,
+                       V + (int64_t)c0 * ldv, ldv, w, g_mpartial);
+    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(RED_THREADS), 0, s, g_mpartial, nb, kc,
+                       h + c0, g_mtmp, accumulate);
+    hipLaunchKernelGGL((k_maxpy<T>), grd, dim3(256), 0, s, n, kc, V + (int64_t)c0 * ldv, ldv,
+                       (const double*)g_mtmp, w);
+  }
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+template int cgs_pass<double>(int64_t, int, const double*, int64_t, double*, double*, int,
+                              hipStream_t);
+template int cgs_pass<float>(int64_t, int, const float*, int64_t, float*, double*, int,
+                             hipStream_t);
+
 template <typename T>
 int axpy_dev(int64_t n, const double* a, double sgn, const T* x, T* y, hipStream_t s) {
   const dim3 blk(256), grd((unsigned)((n + 255) / 256));

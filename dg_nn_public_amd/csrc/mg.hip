@@ -261,6 +261,190 @@ __global__ __launch_bounds__(1024) void k_small_levels(const LevelDesc<T>* __res
   }
 }
 
+// Same cycle over levels l0..L-1 with the whole coarse hierarchy staged in LDS: CSR
+// (int32 rowptr/col, T values), diagonal and the r/A/B vectors of every level.  The
+// level table itself is built in LDS by thread 0.  Reads r and A of level l0 from global
+// memory and writes B of level l0 back.
+template <typename T> struct LdsLevel {
+  int n;
+  int* rp;
+  int* col;
+  T* val;
+  T* diag;
+  T* r;
+  T* A;
+  T* B;
+};
+
+constexpr int MAX_LEVELS = 40;
+
+__host__ __device__ inline size_t lds_align(size_t x) { return (x + 15) & ~size_t(15); }
+
+template <typename T>
+__host__ __device__ inline size_t lds_level_bytes(int n, int nnz) {
+  return lds_align(sizeof(int) * (n + 1)) + lds_align(sizeof(int) * nnz) +
+         lds_align(sizeof(T) * nnz) + 4 * lds_align(sizeof(T) * n);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* __restrict__ lvs,
+                                                           int l0, int L, T omega, int pre,
+                                                           int post) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  LdsLevel<T>* tab = reinterpret_cast<LdsLevel<T>*>(smem);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  if (tid == 0) {
+    size_t off = lds_align(sizeof(LdsLevel<T>) * MAX_LEVELS);
+    for (int l = l0; l < L; ++l) {
+      const int n = lvs[l].n, nnz = lvs[l].rp[n];
+      LdsLevel<T>& t = tab[l - l0];
+      t.n = n;
+      t.rp = reinterpret_cast<int*>(smem + off);
+      off += lds_align(sizeof(int) * (n + 1));
+      t.col = reinterpret_cast<int*>(smem + off);
+      off += lds_align(sizeof(int) * nnz);
+      t.val = reinterpret_cast<T*>(smem + off);
+      off += lds_align(sizeof(T) * nnz);
+      t.diag = reinterpret_cast<T*>(smem + off);
+      off += lds_align(sizeof(T) * n);
+      t.r = reinterpret_cast<T*>(smem + off);
+      off += lds_align(sizeof(T) * n);
+      t.A = reinterpret_cast<T*>(smem + off);
+      off += lds_align(sizeof(T) * n);
+      t.B = reinterpret_cast<T*>(smem + off);
+      off += lds_align(sizeof(T) * n);
+    }
+  }
+  __syncthreads();
+  for (int l = l0; l < L; ++l) {
+    const LevelDesc<T>& g = lvs[l];
+    const LdsLevel<T>& t = tab[l - l0];
+    const int n = t.n, nnz = g.rp[n];
+    for (int i = tid; i <= n; i += nth) t.rp[i] = g.rp[i];
+    for (int q = tid; q < nnz; q += nth) {
+      t.col[q] = g.col[q];
+      t.val[q] = g.val[q];
+    }
+    for (int i = tid; i < n; i += nth) t.diag[i] = g.diag[i];
+    if (l == l0)
+      for (int i = tid; i < n; i += nth) {
+        t.r[i] = g.r[i];
+        t.A[i] = g.A[i];
+      }
+  }
+  __syncthreads();
+  auto row = [&](const LdsLevel<T>& t, int i, const T* ein, const T* ec) {
+    T s = 0;
+    for (int q = t.rp[i]; q < t.rp[i + 1]; ++q) {
+      const int j = t.col[q];
+      T ej = ein[j];
+      if (ec) ej += ec[j >> 1];
+      s += t.val[q] * ej;
+    }
+    return s;
+  };
+  auto sweep = [&](const LdsLevel<T>& t, const T* ein, const T* ec, T* eout) {
+    for (int i = tid; i < t.n; i += nth) {
+      const T ax = row(t, i, ein, ec);
+      T ei = ein[i];
+      if (ec) ei += ec[i >> 1];
+      eout[i] = ei + omega * (t.r[i] - ax) / t.diag[i];
+    }
+    __syncthreads();
+  };
+  for (int l = l0; l < L - 1; ++l) {
+    const LdsLevel<T>& t = tab[l - l0];
+    const LdsLevel<T>& tc = tab[l + 1 - l0];
+    T* cur = t.A;
+    T* nxt = t.B;
+    for (int it = 0; it < pre; ++it) {
+      sweep(t, cur, (const T*)nullptr, nxt);
+      T* tmp = cur;
+      cur = nxt;
+      nxt = tmp;
+    }
+    if (cur == t.A) {
+      for (int i = tid; i < t.n; i += nth) t.B[i] = t.A[i];
+      __syncthreads();
+      cur = t.B;
+    }
+    for (int I = tid; I < tc.n; I += nth) {
+      T rs = 0, es = 0;
+      for (int h = 0; h < 2; ++h) {
+        const int i = 2 * I + h;
+        if (i < t.n) {
+          const T rr = t.r[i] - row(t, i, cur, (const T*)nullptr);
+          const T en = cur[i] + omega * rr / t.diag[i];
+          t.A[i] = en;
+          rs += rr;
+          es += en;
+        }
+      }
+      tc.r[I] = rs;
+      tc.A[I] = es;
+    }
+    __syncthreads();
+  }
+  {
+    const LdsLevel<T>& t = tab[L - 1 - l0];
+    for (int i = tid; i < t.n; i += nth) t.B[i] = t.r[i] / t.diag[i];
+    __syncthreads();
+  }
+  for (int l = L - 2; l >= l0; --l) {
+    const LdsLevel<T>& t = tab[l - l0];
+    const T* ec = tab[l + 1 - l0].B;
+    sweep(t, t.A, ec, t.B);
+    const T* cur = t.B;
+    T* nxt = t.A;
+    for (int it = 1; it < post; ++it) {
+      sweep(t, cur, (const T*)nullptr, nxt);
+      T* tmp = const_cast<T*>(cur);
+      cur = nxt;
+      nxt = tmp;
+    }
+    if (cur != t.B) {
+      for (int i = tid; i < t.n; i += nth) t.B[i] = cur[i];
+      __syncthreads();
+    }
+  }
+  {
+    const LdsLevel<T>& t = tab[0];
+    T* out = lvs[l0].B;
+    for (int i = tid; i < t.n; i += nth) out[i] = t.B[i];
+  }
+}
+
+// LDS bytes needed to stage levels l0..L-1 (level table included)
+template <typename T>
+size_t lds_bytes_from(const SfcLevelsDev& lv, const std::vector<int64_t>& nnz, int l0) {
+  size_t b = lds_align(sizeof(LdsLevel<T>) * MAX_LEVELS);
+  for (int l = l0; l < lv.nlevel; ++l) b += lds_level_bytes<T>((int)lv.n[l], (int)nnz[l]);
+  return b;
+}
+
+// first level index from which the remaining hierarchy fits in `budget` bytes of LDS
+int choose_small_level(const SfcLevelsDev& lv, const std::vector<int64_t>& nnz, size_t budget) {
+  const int L = lv.nlevel;
+  if (L > MAX_LEVELS) return -1;
+  int l0 = L - 1;
+  for (int l = L - 1; l >= 0; --l) {
+    if (lds_bytes_from<double>(lv, nnz, l) <= budget) l0 = l;
+    else break;
+  }
+  return l0;
+}
+
+template <typename T>
+int set_small_lds_attr(size_t bytes) {
+  DGMI_HIP(hipFuncSetAttribute((const void*)k_small_levels_lds<T>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return 0;
+}
+template int set_small_lds_attr<double>(size_t);
+template int set_small_lds_attr<float>(size_t);
+template size_t lds_bytes_from<double>(const SfcLevelsDev&, const std::vector<int64_t>&, int);
+template size_t lds_bytes_from<float>(const SfcLevelsDev&, const std::vector<int64_t>&, int);
+
 // ---------------------------------------------------------------------------------------
 template <typename T> static T* vec(void* p) { return reinterpret_cast<T*>(p); }
 
@@ -356,8 +540,14 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   if (l0 == 0) {
     DGMI_HIP(hipMemcpyAsync(lv.r[0], r_sfc, sizeof(T) * lv.n[0], hipMemcpyDeviceToDevice, s));
   }
-  hipLaunchKernelGGL((k_small_levels<T>), dim3(1), dim3(1024), 0, s,
-                     (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32), l0, L, omega, 3, 3);
+  if (mg.small_lds) {
+    const size_t bytes = lds_bytes_from<T>(lv, lv.nnz, l0);
+    hipLaunchKernelGGL((k_small_levels_lds<T>), dim3(1), dim3(1024), bytes, s,
+                       (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32), l0, L, omega, 3, 3);
+  } else {
+    hipLaunchKernelGGL((k_small_levels<T>), dim3(1), dim3(1024), 0, s,
+                       (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32), l0, L, omega, 3, 3);
+  }
   if (l0 == 0) {
     DGMI_HIP(hipMemcpyAsync(e_sfc, lv.e2[0], sizeof(T) * lv.n[0], hipMemcpyDeviceToDevice, s));
   }

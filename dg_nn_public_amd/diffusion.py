@@ -271,12 +271,18 @@ class DGDiffusion:
 
     # ---------------------------------------------------------------- solvers
     def _gmres_mg_solver(self, x_i, x_rhs, tol=None, rtol=None, m=GMRES_M, max_its=GMRES_ITS,
-                         inner=torch.float64):
+                         inner=torch.float64, orth="cgs2"):
         """Left-preconditioned GMRES(m).  The reference stops on the absolute residual
         ``tol``; pass ``rtol`` for ||b - A x|| <= rtol ||b||.  ``inner`` float32 runs the
-        Krylov loop in FP32 inside FP64 iterative refinement.  Returns (x_i, its) and
-        keeps the statistics in ``self.last_stats``."""
+        Krylov loop in FP32 inside FP64 iterative refinement.  ``orth`` "mgs" is the
+        reference's modified Gram-Schmidt, "cgs2" fused classical Gram-Schmidt with
+        re-orthogonalisation.  Returns (x_i, its) and keeps the statistics in
+        ``self.last_stats``."""
         self._need_mg()
+        if orth not in ("mgs", "cgs2"):
+            raise ValueError("orth must be 'mgs' or 'cgs2'")
+        check(self.lib.dgmi_set_param(self.ctx, b"gmres_orth", 0.0 if orth == "mgs" else 1.0),
+              "dgmi_set_param")
         x_i = self._vec(x_i.view(-1), "x_i", (torch.float64,))
         b = self._vec(x_rhs.view(-1), "x_rhs", (torch.float64,))
         if rtol is None:

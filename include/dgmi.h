@@ -119,6 +119,11 @@ int dgmi_precond(dgmi_ctx* ctx, int dtype, const void* b, void* x, void* stream)
 int dgmi_gmres_solve(dgmi_ctx* ctx, int inner_dtype, const double* b, double* x, double rtol,
                      int m, int max_its, double* stats, void* stream);
 
+/* Solver parameters by name:
+ *   "gmres_orth"  0 = modified Gram-Schmidt, as the reference (z02/volume_mf_diff.py:1813-1816);
+ *                 1 = classical Gram-Schmidt with one re-orthogonalisation (default). */
+int dgmi_set_param(dgmi_ctx* ctx, const char* name, double value);
+
 /* Krylov vector primitives on device vectors of n FP64 or FP32 values. */
 int dgmi_dot(int dtype, int64_t n, const void* x, const void* y, double* out_dev, void* stream);
 int dgmi_axpy(int dtype, int64_t n, double a, const void* x, void* y, void* stream);

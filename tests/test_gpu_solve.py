@@ -21,11 +21,12 @@ def prob(request):
     return m, pb, g, sp, b, x_ref
 
 
+@pytest.mark.parametrize("orth", ["cgs2", "mgs"])
 @pytest.mark.parametrize("inner", [torch.float64, torch.float32], ids=["f64", "f32-ir"])
-def test_gmres_reaches_tolerance_and_matches_oracle(prob, inner):
+def test_gmres_reaches_tolerance_and_matches_oracle(prob, inner, orth):
     m, pb, g, sp, b, x_ref = prob
     x = g.new_vector()
-    x, its = g._gmres_mg_solver(x, to_dev(b), rtol=1e-10, inner=inner)
+    x, its = g._gmres_mg_solver(x, to_dev(b), rtol=1e-10, inner=inner, orth=orth)
     xs = to_np(x)
     relres = np.linalg.norm(b - dg.apply_A(sp, xs)) / np.linalg.norm(b)
     print(inner, "its", its, g.last_stats, "oracle relres", relres, "vs oracle", rel(xs, x_ref))
