@@ -157,7 +157,7 @@ __global__ void k_set_g(double* __restrict__ g, int len, const double* __restric
 // Fused classical Gram-Schmidt pass over up to MDOT_K basis vectors:
 //   h[j] = <w, V_j> (one read of w for all j), then w -= sum_j h[j] V_j.
 constexpr int MDOT_K = 32;
-constexpr int MDOT_BLOCKS = 512;
+constexpr int MDOT_BLOCKS = 2048;
 
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_mdot_partial(int64_t n, int k,
