@@ -10,6 +10,12 @@ resident in HBM.
 Prints one JSON line: solves/s (FP64 solver) as `value`, latency p50/p99, the
 mixed-precision solver's solves/s, operator-apply GDoF/s in FP64 and FP32, the roofline
 of the apply, and the CPU oracle baseline.
+
+Other workloads (``--workload``), each printing its own JSON line:
+  precision  48^3 x 6 tets (663,552 elements), sigma 2: operator error of the FP32 and
+             bf16-input applies against FP64, iteration counts of FP64 / FP32 / bf16 solves
+  batch      256 independent solves (half 2D, half 3D, 2e3..2e5 elements log-uniform,
+             sigma 0.5..2) packed per dimension into one context each
 """
 from __future__ import annotations
 
@@ -55,11 +61,18 @@ def main():
                          "that calibrates HBM byte counters; for rocprofv3 --pmc passes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
+    ap.add_argument("--workload", default="solve", choices=["solve", "precision", "batch"])
+    ap.add_argument("--nsolves", type=int, default=256, help="batch workload size")
+    ap.add_argument("--emax", type=float, default=2e5, help="batch: largest element count")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not "
                                    "supported (multi-GPU sharding is design-only)"}))
         sys.exit(2)
+    if args.workload == "precision":
+        return run_precision(args)
+    if args.workload == "batch":
+        return run_batch(args)
 
     import numpy as np
     import torch
@@ -82,23 +95,29 @@ def main():
     stream = torch.cuda.current_stream()
 
     # ---- operator apply throughput (HIP events on the stream the kernels run on)
-    def time_applies(dtype, k):
+    def time_applies(dtype, k, kernel="mfma", bf16=False):
+        if dtype == torch.float32:
+            g.set_f32_kernel(kernel)
         x = torch.randn(g.ndof, dtype=dtype, device=dev)
         bb = torch.randn(g.ndof, dtype=dtype, device=dev)
         r = torch.empty_like(x)
         for _ in range(3):
-            g.apply(x, bb, out=r)
+            g.apply(x, bb, out=r, bf16=bf16)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(k):
-            g.apply(x, bb, out=r)
+            g.apply(x, bb, out=r, bf16=bf16)
         e1.record(stream)
         torch.cuda.synchronize()
+        g.set_f32_kernel("mfma")
         return e0.elapsed_time(e1) / 1e3 / k
 
     t_apply64 = time_applies(torch.float64, args.applies)
-    t_apply32 = time_applies(torch.float32, args.applies)
-    print(f"[bench] apply f64 {t_apply64*1e6:.1f} us, f32 {t_apply32*1e6:.1f} us",
+    t_apply32v = time_applies(torch.float32, args.applies, kernel="vector")
+    t_apply32 = time_applies(torch.float32, args.applies, kernel="mfma")
+    t_apply16 = time_applies(torch.float32, args.applies, bf16=True)
+    print(f"[bench] apply f64 {t_apply64*1e6:.1f} us, f32 vector {t_apply32v*1e6:.1f} us, "
+          f"f32 mfma {t_apply32*1e6:.1f} us, bf16 mfma {t_apply16*1e6:.1f} us",
           file=sys.stderr, flush=True)
     if args.apply_only:
         # calibration: elementwise copy of 256 MiB (reads 256 MiB, writes 256 MiB)
@@ -107,7 +126,8 @@ def main():
         for _ in range(5):
             torch.add(src, 0.0, out=dst)
         torch.cuda.synchronize()
-        print(json.dumps({"apply_us": {"f64": t_apply64 * 1e6, "f32": t_apply32 * 1e6},
+        print(json.dumps({"apply_us": {"f64": t_apply64 * 1e6, "f32_vector": t_apply32v * 1e6,
+                                       "f32_mfma": t_apply32 * 1e6, "bf16_mfma": t_apply16 * 1e6},
                           "calibration_copy_bytes": 2 * src.numel() * 8}))
         return
 
@@ -134,6 +154,9 @@ def main():
           file=sys.stderr, flush=True)
     lat32, st32 = run_solves(torch.float32, args.steps, args.warmup)
     print(f"[bench] mixed solves: {[round(v, 3) for v in lat32]} s, stats {st32[-1]}",
+          file=sys.stderr, flush=True)
+    lat16, st16 = run_solves(torch.bfloat16, args.steps, args.warmup)
+    print(f"[bench] bf16-inner solves: {[round(v, 3) for v in lat16]} s, stats {st16[-1]}",
           file=sys.stderr, flush=True)
     total64 = sum(lat64)
 
@@ -210,10 +233,16 @@ def main():
                   "speedup_vs_fp64": (len(lat32) / sum(lat32)) / (len(lat64) / total64),
                   "iterations": [s["its"] for s in st32], "outer": [s["outer"] for s in st32],
                   "relres": max(s["relres"] for s in st32)},
+        "mixed_bf16": {"solves_per_s": len(lat16) / sum(lat16),
+                       "p50": float(np.percentile(lat16, 50)), "p99": float(np.percentile(lat16, 99)),
+                       "speedup_vs_fp64": (len(lat16) / sum(lat16)) / (len(lat64) / total64),
+                       "iterations": [s["its"] for s in st16], "outer": [s["outer"] for s in st16],
+                       "relres": max(s["relres"] for s in st16)},
         "reference_face_kappa_check": ref_form,
         "fp64_iterations": [s["its"] for s in st64],
         "fp64_relres": max(s["relres"] for s in st64),
-        "apply_gdofs": {"f64": gdofs64, "f32": gdofs32},
+        "apply_gdofs": {"f64": gdofs64, "f32_vector": nele * nloc / t_apply32v / 1e9,
+                        "f32_mfma": gdofs32, "bf16_mfma": nele * nloc / t_apply16 / 1e9},
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
@@ -223,6 +252,170 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _time_apply(g, dtype, k, kernel="mfma", bf16=False):
+    import torch
+    if dtype == torch.float32:
+        g.set_f32_kernel(kernel)
+    x = torch.randn(g.ndof, dtype=dtype, device="cuda:0")
+    bb = torch.randn(g.ndof, dtype=dtype, device="cuda:0")
+    r = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        g.apply(x, bb, out=r, bf16=bf16)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        g.apply(x, bb, out=r, bf16=bf16)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    g.set_f32_kernel("mfma")
+    return e0.elapsed_time(e1) / 1e3 / k
+
+
+def run_precision(args):
+    """48^3 x 6 tets, lognormal sigma 2: operator error of the FP32 / bf16-input applies
+    against the FP64 apply, and FP64 / FP32-IR / bf16-IR solves to 1e-10."""
+    import numpy as np
+    import torch
+
+    from dg_nn_public_amd import meshgen
+    from dg_nn_public_amd.diffusion import DGDiffusion
+
+    n = 48 if args.n == 24 else args.n
+    sigma = 2.0
+    t0 = time.time()
+    mesh = meshgen.cube_mesh(n, jitter=0.15, seed=0)
+    pb = meshgen.sine_problem(mesh, sigma=sigma, kappa_seed=1)
+    g = DGDiffusion.from_mesh(mesh, pb.kappa, face_kappa="harmonic")
+    g._get_RAR_and_sfc_data_Um()
+    b = g._get_rhs(g.new_vector(), pb.f, pb.u_bc)
+    torch.cuda.synchronize()
+    setup = time.time() - t0
+    print(f"[precision] setup {setup:.1f}s nele={g.nele}", file=sys.stderr, flush=True)
+    solves = {}
+    xh = None
+    for name, inner in (("f64", torch.float64), ("f32_ir", torch.float32),
+                        ("bf16_ir", torch.bfloat16)):
+        x = g.new_vector()
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        g._gmres_mg_solver(x, b, rtol=1e-10, inner=inner)
+        torch.cuda.synchronize()
+        solves[name] = dict(g.last_stats, seconds=time.perf_counter() - ts)
+        if name == "f64":
+            xh = x
+        print(f"[precision] {name}: {solves[name]}", file=sys.stderr, flush=True)
+
+    def op_err(x64):
+        ref = g.apply(x64)
+        out = {}
+        for kname, kern, bf in (("f32_vector", "vector", False), ("f32_mfma", "mfma", False),
+                                ("bf16_mfma", "mfma", True)):
+            g.set_f32_kernel(kern)
+            y = g.apply(x64.float(), bf16=bf).double()
+            out[kname] = float(torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref))
+        g.set_f32_kernel("mfma")
+        return out
+
+    gen = torch.Generator(device="cpu").manual_seed(2)
+    xr = (torch.rand(g.ndof, generator=gen, dtype=torch.float64) * 2 - 1).to("cuda:0")
+    errs = {"x_uniform": op_err(xr), "x_solution": op_err(xh)}
+    ap = {"f64": _time_apply(g, torch.float64, args.applies),
+          "f32_vector": _time_apply(g, torch.float32, args.applies, "vector"),
+          "f32_mfma": _time_apply(g, torch.float32, args.applies, "mfma"),
+          "bf16_mfma": _time_apply(g, torch.float32, args.applies, bf16=True)}
+    nd = g.nele * g.nloc
+    print(json.dumps({
+        "metric": "operator rel. L2 error vs FP64 and iteration counts (precision workload)",
+        "config": {"workload": f"3D IP-DG P3 tets, {n}^3x6 = {g.nele} elements, {nd} DOFs, "
+                               f"lognormal kappa sigma={sigma}, harmonic face kappa, "
+                               "GMRES(30)+SFC-MG, rtol 1e-10"},
+        "operator_rel_err": errs,
+        "solves": solves,
+        "iteration_change_vs_f64": {k: v["its"] - solves["f64"]["its"] for k, v in solves.items()},
+        "apply_us": {k: v * 1e6 for k, v in ap.items()},
+        "apply_gdofs": {k: nd / v / 1e9 for k, v in ap.items()},
+        "setup_s": setup,
+    }))
+
+
+def _batch_requests(nsolves, emax, seed=0):
+    """The batch mix: half 2D, half 3D; element counts log-uniform in [2e3, emax];
+    sigma U[0.5, 2]; grid n from the element count."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(nsolves):
+        D = 2 if i < nsolves // 2 else 3
+        E = float(np.exp(rng.uniform(np.log(2e3), np.log(emax))))
+        n = max(2, int(round((E / 2) ** 0.5))) if D == 2 else max(2, int(round((E / 6) ** (1 / 3))))
+        reqs.append(dict(D=D, n=n, mesh_seed=i, kappa_seed=1000 + i,
+                         sigma=float(rng.uniform(0.5, 2.0))))
+    return reqs
+
+
+def run_batch(args):
+    """256 independent solves, packed per dimension into one context each (one launch per
+    kernel covers every solve of that dimension); FP32-IR and FP64 solves."""
+    import numpy as np
+    import torch
+
+    from dg_nn_public_amd import meshgen
+    from dg_nn_public_amd.batch import DGBatch
+
+    reqs = _batch_requests(args.nsolves, args.emax)
+    out = {"groups": {}}
+    total_solve = {"f32_ir": 0.0, "f64": 0.0}
+    nsol = 0
+    t_setup = 0.0
+    for D in (2, 3):
+        rs = [r for r in reqs if r["D"] == D]
+        if not rs:
+            continue
+        t0 = time.time()
+        pbs = []
+        for r in rs:
+            m = meshgen.square_mesh(r["n"], seed=r["mesh_seed"]) if D == 2 else \
+                meshgen.cube_mesh(r["n"], seed=r["mesh_seed"])
+            pbs.append(meshgen.sine_problem(m, sigma=r["sigma"], kappa_seed=r["kappa_seed"]))
+        tm = time.time() - t0
+        bt = DGBatch([p.mesh for p in pbs], [p.kappa for p in pbs])
+        b = bt.rhs([p.f for p in pbs], [p.u_bc for p in pbs])
+        torch.cuda.synchronize()
+        ts = time.time() - t0 - tm
+        t_setup += ts
+        print(f"[batch] D={D}: {len(rs)} solves, {bt.space.nele} elements, mesh gen {tm:.1f}s, "
+              f"context+multigrid setup {ts:.1f}s", file=sys.stderr, flush=True)
+        grp = {"solves": len(rs), "elements": int(bt.space.nele), "setup_s": ts}
+        for name, inner in (("f32_ir", torch.float32), ("f64", torch.float64)):
+            lat = []
+            for rep in range(args.warmup + args.steps):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                x, st = bt.solve(b, rtol=1e-10, inner=inner)
+                torch.cuda.synchronize()
+                if rep >= args.warmup:
+                    lat.append(time.perf_counter() - t1)
+            its = [s["its"] for s in st]
+            grp[name] = {"seconds": float(np.median(lat)), "max_relres": max(s["relres"] for s in st),
+                         "its_min": min(its), "its_max": max(its), "its_mean": float(np.mean(its))}
+            total_solve[name] += float(np.median(lat))
+            print(f"[batch] D={D} {name}: {grp[name]}", file=sys.stderr, flush=True)
+        nsol += len(rs)
+        out["groups"][D] = grp
+        bt.close()
+    sps = {k: nsol / v for k, v in total_solve.items()}
+    print(json.dumps({
+        "metric": "solves/s at rtol 1e-10, packed batch (batch workload)",
+        "value": sps["f32_ir"], "unit": "solves/s", "higher_is_better": True,
+        "config": {"workload": f"{nsol} independent solves, half 2D P3 tris / half 3D P3 tets, "
+                               f"2e3..{args.emax:.0e} elements log-uniform, sigma U[0.5,2], "
+                               "packed per dimension, FP32 inner + FP64 refinement"},
+        "fp64_solves_per_s": sps["f64"], "speedup_vs_fp64": sps["f32_ir"] / sps["f64"],
+        "setup_s_total": t_setup, "groups": out["groups"],
+    }))
 
 
 if __name__ == "__main__":

@@ -38,6 +38,8 @@ class MeshDesc(C.Structure):
 SIGNATURES = {
     "dgmi_ctx_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(_P)]),
     "dgmi_ctx_destroy": (C.c_int, [_P]),
+    "dgmi_ctx_create_batch": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, _I64P, _I64P,
+                                        C.POINTER(_P)]),
     "dgmi_ctx_info": (C.c_int, [_P, _I64P]),
     "dgmi_topology": (C.c_int, [_P, _I64P, _I64P, _I64P]),
     "dgmi_element_tables": (C.c_int, [_P, _DP, _DP, _DP, _DP, _DP, _DP, _I64P]),

@@ -1,0 +1,87 @@
+"""Packed solves: many independent diffusion problems in one library context.
+
+The reference solves one mesh per process (z02/main.py:202-227).  For batches and for
+serving, the meshes of several requests of the same dimension are packed into one union
+mesh whose components are the requests: every operator apply, smoothing step, DG<->P1CG
+transfer and V-cycle level is one kernel launch over all of them, while each component
+keeps its own SFC hierarchy (built from its own vertices), its own Krylov space and its
+own convergence test.  A component's result is the one it gets when solved alone, up to
+the solver tolerance.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .diffusion import GMRES_ITS, GMRES_M, DGDiffusion
+
+
+@dataclass
+class Packed:
+    """Union of meshes of one dimension; component c owns elements
+    [elem_off[c], elem_off[c+1]) and vertices [vert_off[c], vert_off[c+1])."""
+    D: int
+    x_ref_in: np.ndarray
+    cells: np.ndarray
+    verts: np.ndarray
+    elem_off: np.ndarray
+    vert_off: np.ndarray
+
+
+def pack(meshes) -> Packed:
+    """Concatenate meshes (``meshgen.Mesh``-like: D, x_ref_in, cells, verts) of one
+    dimension, renumbering each mesh's vertices after the previous ones."""
+    if not meshes:
+        raise ValueError("no meshes to pack")
+    D = meshes[0].D
+    if any(m.D != D for m in meshes):
+        raise ValueError("packed meshes must share one dimension")
+    eo = np.zeros(len(meshes) + 1, dtype=np.int64)
+    vo = np.zeros(len(meshes) + 1, dtype=np.int64)
+    for i, m in enumerate(meshes):
+        eo[i + 1] = eo[i] + m.cells.shape[0]
+        vo[i + 1] = vo[i] + m.verts.shape[0]
+    cells = np.concatenate([np.asarray(m.cells, dtype=np.int64) + vo[i]
+                            for i, m in enumerate(meshes)])
+    return Packed(D, np.concatenate([m.x_ref_in for m in meshes]), cells,
+                  np.concatenate([m.verts for m in meshes]), eo, vo)
+
+
+class DGBatch:
+    """Independent P3 DG diffusion solves of one dimension packed into one context."""
+
+    def __init__(self, meshes, kappas, device="cuda:0", face_kappa="harmonic",
+                 blk_solver="jacobian"):
+        p = pack(meshes)
+        kappa = np.concatenate([np.asarray(k, dtype=np.float64).reshape(-1) for k in kappas])
+        self.space = DGDiffusion(p.x_ref_in, p.cells, p.verts, kappa, device=device,
+                                 face_kappa=face_kappa, blk_solver=blk_solver,
+                                 components=(p.elem_off, p.vert_off))
+        self.packed = p
+        self.ncomp = len(meshes)
+        self.nloc = self.space.nloc
+        self.dof_off = p.elem_off * self.nloc
+        self.space._get_RAR_and_sfc_data_Um()
+
+    def rhs(self, fs, u_bcs):
+        """b = M f + Dirichlet terms of every component (nodal f and u_bc per mesh)."""
+        f = np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1) for a in fs])
+        u = np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1) for a in u_bcs])
+        return self.space._get_rhs(self.space.new_vector(), f, u)
+
+    def solve(self, b, rtol=1e-10, inner=torch.float32, m=GMRES_M, max_its=GMRES_ITS, x=None):
+        """Solve every component to ||r_c|| <= rtol ||b_c||.  Returns (x, stats) with one
+        stats dict per component."""
+        x = self.space.new_vector() if x is None else x
+        self.space._gmres_mg_solver(x, b, rtol=rtol, inner=inner, m=m, max_its=max_its)
+        st = self.space.last_stats
+        return x, (st if isinstance(st, list) else [st])
+
+    def split(self, v):
+        """Per-component views of a packed DOF vector."""
+        return [v[self.dof_off[c]:self.dof_off[c + 1]] for c in range(self.ncomp)]
+
+    def close(self):
+        self.space.close()

@@ -16,75 +16,13 @@
 // MODE 0: r = -A x; 1: r = b - A x; 2: x += S (b - A x); 3: as 2 and also store r;
 // 4: r = +A x.
 #include "dgmi_internal.h"
+#include "rowvec.h"
 
 namespace dgmi {
 
 template <int D> struct Dm;
 template <> struct Dm<3> { static constexpr int NL = 20, NGI = 24, SG = 12, NF = 4, NG = 6; };
 template <> struct Dm<2> { static constexpr int NL = 10, NGI = 12, SG = 4, NF = 3, NG = 3; };
-
-// Vector loads/stores of one element's row: 16-byte accesses when the row length
-// allows (3D P3 in both precisions, 2D in FP64), 8-byte otherwise (2D P3 in FP32).
-// DOF vectors are required to be 16-byte aligned.
-template <typename T, int NL> struct RowVec {
-  static constexpr int W = (NL % (16 / sizeof(T)) == 0) ? 16 / sizeof(T) : 8 / sizeof(T);
-  using V = typename std::conditional<
-      sizeof(T) * W == 16, typename std::conditional<sizeof(T) == 8, double2, float4>::type,
-      typename std::conditional<sizeof(T) == 8, double, float2>::type>::type;
-  static_assert(NL % W == 0, "row length not a multiple of the vector width");
-};
-
-template <typename T, int NL>
-__device__ __forceinline__ void load_row(const T* __restrict__ p, T* v) {
-  using RV = RowVec<T, NL>;
-  constexpr int W = RV::W;
-  const typename RV::V* q = reinterpret_cast<const typename RV::V*>(p);
-#pragma unroll
-  for (int i = 0; i < NL / W; ++i) {
-    typename RV::V t = q[i];
-    const T* tt = reinterpret_cast<const T*>(&t);
-#pragma unroll
-    for (int k = 0; k < W; ++k) v[i * W + k] = tt[k];
-  }
-}
-template <typename T, int NL>
-__device__ __forceinline__ void store_row(T* __restrict__ p, const T* v) {
-  using RV = RowVec<T, NL>;
-  constexpr int W = RV::W;
-  typename RV::V* q = reinterpret_cast<typename RV::V*>(p);
-#pragma unroll
-  for (int i = 0; i < NL / W; ++i) {
-    typename RV::V t;
-    T* tt = reinterpret_cast<T*>(&t);
-#pragma unroll
-    for (int k = 0; k < W; ++k) tt[k] = v[i * W + k];
-    q[i] = t;
-  }
-}
-
-// Smoother matrices are stored as structure-of-vectors: entry S[n][m] of element e lives in
-// vector ((n*(NL/W) + m/W)*nele + e), component m%W, with W the row vector width above, so
-// one wave-wide load moves 64 x 16 bytes.
-template <typename T, int NL>
-__host__ __device__ inline size_t s_index(int n, int m, int e, int nele) {
-  constexpr int W = RowVec<T, NL>::W;
-  return ((size_t)(n * (NL / W) + m / W) * nele + e) * W + (m % W);
-}
-
-// row n of S for element e
-template <typename T, int NL>
-__device__ __forceinline__ void load_S_row(const T* __restrict__ S, int n, int e, int nele, T* v) {
-  using RV = RowVec<T, NL>;
-  constexpr int W = RV::W;
-  const typename RV::V* q = reinterpret_cast<const typename RV::V*>(S);
-#pragma unroll
-  for (int i = 0; i < NL / W; ++i) {
-    typename RV::V t = q[(size_t)(n * (NL / W) + i) * nele + e];
-    const T* tt = reinterpret_cast<const T*>(&t);
-#pragma unroll
-    for (int k = 0; k < W; ++k) v[i * W + k] = tt[k];
-  }
-}
 
 // ---------------------------------------------------------------------------------------
 // Wave-split variants.  A block of nface waves owns 64 elements (one per lane); wave w
@@ -495,25 +433,30 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   return 0;
 }
 
+// FP32 vectors go to the matrix-core kernels unless the context selects the vector ones
+template <typename T>
+static int apply_dispatch(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mode,
+                          hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value)
+    if (c->f32_kernel != 0) return launch_apply_mf(c, c->f32_kernel == 2, x, b, r, mode, s);
+  if (c->D == 3) return apply_impl<T, 3>(c, g, x, b, r, mode, s);
+  return apply_impl<T, 2>(c, g, x, b, r, mode, s);
+}
+
 template <typename T>
 int launch_apply(dgmi_ctx* c, DevGeom<T>& g, const T* x, const T* b, T* r, hipStream_t s) {
   T* xm = const_cast<T*>(x);  // not written in modes 0/1
-  if (c->D == 3) return apply_impl<T, 3>(c, g, xm, b, r, b ? 1 : 0, s);
-  return apply_impl<T, 2>(c, g, xm, b, r, b ? 1 : 0, s);
+  return apply_dispatch<T>(c, g, xm, b, r, b ? 1 : 0, s);
 }
 
 template <typename T>
 int launch_apply_pos(dgmi_ctx* c, DevGeom<T>& g, const T* x, T* r, hipStream_t s) {
-  T* xm = const_cast<T*>(x);
-  if (c->D == 3) return apply_impl<T, 3>(c, g, xm, nullptr, r, 4, s);
-  return apply_impl<T, 2>(c, g, xm, nullptr, r, 4, s);
+  return apply_dispatch<T>(c, g, const_cast<T*>(x), nullptr, r, 4, s);
 }
 
 template <typename T>
 int launch_smooth(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r_out, hipStream_t s) {
-  const int mode = r_out ? 3 : 2;
-  if (c->D == 3) return apply_impl<T, 3>(c, g, x, b, r_out, mode, s);
-  return apply_impl<T, 2>(c, g, x, b, r_out, mode, s);
+  return apply_dispatch<T>(c, g, x, b, r_out, r_out ? 3 : 2, s);
 }
 
 template <typename T>

@@ -177,12 +177,11 @@ void* dgmi_ctx::alloc(size_t bytes) {
 }
 
 namespace dgmi {
-static std::list<std::pair<dgmi_ctx*, Scratch>> g_scratch;  // stable references
+// per-context scratch, owned by the context (no shared state between contexts, so
+// different contexts may be driven from different host threads)
 static Scratch& scratch(dgmi_ctx* c) {
-  for (auto& p : g_scratch)
-    if (p.first == c) return p.second;
-  g_scratch.emplace_back(c, Scratch{});
-  return g_scratch.back().second;
+  if (!c->scratch) c->scratch = new Scratch{};
+  return *static_cast<Scratch*>(c->scratch);
 }
 template <typename T> static T* vbuf(dgmi_ctx* c, int i);
 template <> double* vbuf<double>(dgmi_ctx* c, int i) {
@@ -234,6 +233,13 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s) {
   T* r = vbuf<T>(c, 3);
   T* rc = cgbuf<T>(c, 0);
   T* ec = cgbuf<T>(c, 1);
+  // bf16-input applies inside the preconditioner only (the Krylov operator stays fp32)
+  struct KernelScope {
+    dgmi_ctx* c;
+    int saved;
+    ~KernelScope() { c->f32_kernel = saved; }
+  } scope{c, c->f32_kernel};
+  if (std::is_same<T, float>::value && c->pc_bf16) c->f32_kernel = 2;
   // pre-smoothing: the first step starts from x = 0, where the residual is b
   DGMI_TRY(launch_smooth_zero<T>(c, g, x, b, s));
   for (int it = 1; it < 3; ++it) DGMI_TRY(launch_smooth<T>(c, g, x, b, nullptr, s));
@@ -246,103 +252,228 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s) {
   return 0;
 }
 
-// Solver workspace: Krylov basis, Hessenberg, rotations (device), pinned host scalars.
+// Solver workspace: Krylov basis (and, for the flexible variant, the preconditioned basis),
+// per-component Hessenberg matrices, rotations, right-hand sides and scalars on the device,
+// pinned host mirrors of the per-component scalars the host reads back.
 struct Krylov {
-  int m = 0;
+  int m = 0, ncomp = 0;
+  size_t vbytes = 0, zbytes = 0;
   void* V = nullptr;  // (m+1) * ndof of T
-  size_t vbytes = 0;
-  double* H = nullptr;   // (m+1) x m column-major, ldh = m+1
-  double* cs = nullptr;
-  double* sn = nullptr;
-  double* g = nullptr;
-  double* y = nullptr;
-  double* red = nullptr;  // reduction scalars
-  double* host = nullptr;  // pinned
+  void* Z = nullptr;  // m * ndof of T (flexible GMRES)
+  int ldh = 0;
+  int64_t hs = 0, rs = 0, gs = 0, ys = 0;  // per-component strides of H, cs/sn, g, y
+  double *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
+  double *beta = nullptr, *hn = nullptr, *res = nullptr, *thr = nullptr, *eps = nullptr;
+  double *rnorm = nullptr, *bnorm = nullptr;
+  int32_t *want = nullptr, *active = nullptr, *kk = nullptr, *nact = nullptr;
+  double* hd = nullptr;   // pinned: 2 * ncomp doubles
+  int32_t* hi = nullptr;  // pinned: 2 * ncomp + 1 ints
 };
-static std::list<std::pair<dgmi_ctx*, Krylov>> g_kry;
-static Krylov* krylov(dgmi_ctx* c, int m, size_t elem) {
-  Krylov* k = nullptr;
-  for (auto& p : g_kry)
-    if (p.first == c) k = &p.second;
-  if (!k) {
-    g_kry.emplace_back(c, Krylov{});
-    k = &g_kry.back().second;
+static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
+  if (!c->krylov) c->krylov = new Krylov{};
+  Krylov* k = static_cast<Krylov*>(c->krylov);
+  const int nc = c->ncomp;
+  if (!k->beta) {
+    k->ncomp = nc;
+    auto dv = [&](size_t n) { return static_cast<double*>(c->alloc(sizeof(double) * n)); };
+    auto iv = [&](size_t n) { return static_cast<int32_t*>(c->alloc(sizeof(int32_t) * n)); };
+    k->beta = dv(nc);
+    k->hn = dv(nc);
+    k->res = dv(nc);
+    k->thr = dv(nc);
+    k->eps = dv(nc);
+    k->rnorm = dv(nc);
+    k->bnorm = dv(nc);
+    k->want = iv(nc);
+    k->active = iv(nc);
+    k->kk = iv(nc);
+    k->nact = iv(1);
+    if (hipHostMalloc((void**)&k->hd, sizeof(double) * 2 * nc) != hipSuccess) k->hd = nullptr;
+    if (hipHostMalloc((void**)&k->hi, sizeof(int32_t) * (2 * nc + 1)) != hipSuccess) k->hi = nullptr;
+    if (!k->beta || !k->nact || !k->hd || !k->hi) return nullptr;
   }
   const size_t need = (size_t)(m + 1) * c->ndof * elem;
   if (k->m < m || k->vbytes < need) {
     k->V = c->alloc(need);
     k->vbytes = need;
-    const int ldh = m + 1;
-    k->H = static_cast<double*>(c->alloc(sizeof(double) * ldh * m));
-    k->cs = static_cast<double*>(c->alloc(sizeof(double) * (m + 1)));
-    k->sn = static_cast<double*>(c->alloc(sizeof(double) * (m + 1)));
-    k->g = static_cast<double*>(c->alloc(sizeof(double) * (m + 2)));
-    k->y = static_cast<double*>(c->alloc(sizeof(double) * (m + 1)));
+    k->Z = nullptr;
+    k->zbytes = 0;
+    k->ldh = m + 1;
+    k->hs = (int64_t)k->ldh * m;
+    k->rs = m + 1;
+    k->gs = m + 2;
+    k->ys = m + 1;
+    k->H = static_cast<double*>(c->alloc(sizeof(double) * k->hs * nc));
+    k->cs = static_cast<double*>(c->alloc(sizeof(double) * k->rs * nc));
+    k->sn = static_cast<double*>(c->alloc(sizeof(double) * k->rs * nc));
+    k->g = static_cast<double*>(c->alloc(sizeof(double) * k->gs * nc));
+    k->y = static_cast<double*>(c->alloc(sizeof(double) * k->ys * nc));
     k->m = m;
+    if (!k->V || !k->H || !k->cs || !k->sn || !k->g || !k->y) return nullptr;
   }
-  if (!k->red) k->red = static_cast<double*>(c->alloc(sizeof(double) * 8));
-  if (!k->host && hipHostMalloc((void**)&k->host, sizeof(double) * 8) != hipSuccess) return nullptr;
-  if (!k->V || !k->H || !k->g || !k->red || !k->host) return nullptr;
+  const size_t zneed = (size_t)m * c->ndof * elem;
+  if (flexible && k->zbytes < zneed) {
+    k->Z = c->alloc(zneed);
+    k->zbytes = zneed;
+    if (!k->Z) return nullptr;
+  }
   return k;
 }
 
-// One GMRES cycle on A d = r0 from d = 0, left-preconditioned, modified Gram-Schmidt,
-// Givens rotations (Saad, Alg. 6.9).  Stops at m iterations or when the preconditioned
-// residual estimate |g_{j+1}| <= eps * beta.  Returns iterations used in *its.
+// One GMRES cycle on A d = r0 from d = 0 for every wanted component (want[c] on the
+// device), Givens rotations (Saad, Alg. 6.9).  Left-preconditioned (the reference's
+// z02/volume_mf_diff.py:1757-1841): Krylov space of M^-1 A, estimate of the preconditioned
+// residual.  Flexible (Saad, Alg. 9.6, right-preconditioned with the preconditioned basis
+// kept): the estimate is of the true residual and the preconditioner may vary between
+// steps.  A component stops at m iterations or when its estimate falls to eps[c] times its
+// initial norm (eps on the device).  Iterations per component land in k->kk.
 template <typename T>
-static int gmres_cycle(dgmi_ctx* c, const T* r0, T* d, int m, double eps, int* its,
+static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool flexible,
                        hipStream_t s) {
-  Krylov* k = krylov(c, m, sizeof(T));
-  DGMI_CHECK(k, "device allocation failed (Krylov workspace)");
   const int64_t n = c->ndof;
-  const int ldh = m + 1;
+  const int nc = c->ncomp;
+  const SegDev& sg = c->seg;
   T* V = static_cast<T*>(k->V);
+  T* Z = static_cast<T*>(k->Z);
   T* w = vbuf<T>(c, 2);
   T* t = vbuf<T>(c, 1);
+  DGMI_CHECK(w && t, "device allocation failed");
   auto Vj = [&](int j) { return V + (size_t)j * n; };
-  DGMI_TRY(precond<T>(c, r0, Vj(0), s));
-  DGMI_TRY(launch_dot<T>(n, Vj(0), Vj(0), k->red, 1, s));  // red[1] = beta
-  DGMI_TRY(kry::scale_dev<T>(n, k->red + 1, Vj(0), Vj(0), s));
-  DGMI_TRY(kry::set_g(k->g, m + 2, k->red, s));
-  DGMI_HIP(hipMemcpyAsync(k->host, k->red + 1, sizeof(double), hipMemcpyDeviceToHost, s));
-  DGMI_HIP(hipStreamSynchronize(s));
-  const double beta = k->host[0];
-  DGMI_CHECK(std::isfinite(beta), "non-finite preconditioned residual");
-  int j = 0;
-  for (; j < m; ++j) {
-    DGMI_TRY(launch_apply_pos<T>(c, geom<T>(c), Vj(j), t, s));  // t = A v_j
-    DGMI_TRY(precond<T>(c, t, w, s));                           // w = M^-1 A v_j
-    double* hcol = k->H + (size_t)j * ldh;
+  auto Zj = [&](int j) { return Z + (size_t)j * n; };
+  const T* start = r0;
+  if (!flexible) {
+    DGMI_TRY(precond<T>(c, r0, Vj(0), s));
+    start = Vj(0);
+  }
+  DGMI_TRY(kry::sdot<T>(sg, start, start, nullptr, k->beta, s));
+  DGMI_TRY(kry::scycle_start(nc, k->beta, k->eps, k->want, k->active, k->thr, k->g, k->gs,
+                             m + 2, k->kk, s));
+  DGMI_TRY((kry::sscale<T, T>(sg, k->beta, 0, k->active, start, Vj(0), s)));
+  for (int j = 0; j < m; ++j) {
+    if (flexible) {
+      DGMI_TRY(precond<T>(c, Vj(j), Zj(j), s));                  // z_j = M^-1 v_j
+      DGMI_TRY(launch_apply_pos<T>(c, geom<T>(c), Zj(j), w, s));  // w = A z_j
+    } else {
+      DGMI_TRY(launch_apply_pos<T>(c, geom<T>(c), Vj(j), t, s));  // t = A v_j
+      DGMI_TRY(precond<T>(c, t, w, s));                           // w = M^-1 A v_j
+    }
+    double* hcol = k->H + (size_t)j * k->ldh;
     if (c->gmres_orth == 0) {
       // modified Gram-Schmidt, as the reference (z02/volume_mf_diff.py:1813-1816)
-      for (int i = 0; i <= j; ++i) {
-        DGMI_TRY(launch_dot<T>(n, w, Vj(i), hcol + i, 0, s));
-        DGMI_TRY(kry::axpy_dev<T>(n, hcol + i, -1.0, Vj(i), w, s));
-      }
+      for (int i = 0; i <= j; ++i)
+        DGMI_TRY(kry::sorth_pass<T>(sg, 1, Vj(i), n, w, hcol + i, k->hs, 0, k->active, s));
     } else {
       // classical Gram-Schmidt with one re-orthogonalisation (CGS2): two fused passes,
       // the Hessenberg column is the sum of both passes' coefficients
-      DGMI_TRY(kry::cgs_pass<T>(n, j + 1, V, n, w, hcol, 0, s));
-      DGMI_TRY(kry::cgs_pass<T>(n, j + 1, V, n, w, hcol, 1, s));
+      DGMI_TRY(kry::sorth_pass<T>(sg, j + 1, V, n, w, hcol, k->hs, 0, k->active, s));
+      DGMI_TRY(kry::sorth_pass<T>(sg, j + 1, V, n, w, hcol, k->hs, 1, k->active, s));
     }
-    DGMI_TRY(launch_dot<T>(n, w, w, k->red, 1, s));
-    DGMI_TRY(kry::store_h(k->red + 1, hcol + j + 1, 0, s));
-    DGMI_TRY(kry::scale_dev<T>(n, k->red + 1, w, Vj(j + 1), s));
-    DGMI_TRY(kry::givens(j, ldh, k->H, k->cs, k->sn, k->g, k->red + 2, s));
-    DGMI_HIP(hipMemcpyAsync(k->host + 1, k->red + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+    DGMI_TRY(kry::sdot<T>(sg, w, w, nullptr, k->hn, s));
+    DGMI_TRY((kry::sscale<T, T>(sg, k->hn, 0, k->active, w, Vj(j + 1), s)));
+    DGMI_TRY(kry::sgivens(nc, j, k->ldh, k->hs, k->H, k->hn, k->cs, k->sn, k->rs, k->g, k->gs,
+                          k->res, k->thr, k->active, k->kk, k->nact, s));
+    DGMI_HIP(hipMemcpyAsync(k->hi + 2 * nc, k->nact, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     DGMI_HIP(hipStreamSynchronize(s));
-    const double est = k->host[1];
-    DGMI_CHECK(std::isfinite(est), "non-finite Arnoldi residual estimate");
-    if (est <= eps * beta) {
-      ++j;
-      break;
-    }
+    if (k->hi[2 * nc] == 0) break;
   }
-  const int kk = std::min(j, m);
-  DGMI_TRY(kry::backsolve(kk, ldh, k->H, k->g, k->y, s));
-  DGMI_TRY(kry::fill<T>(n, d, T(0), s));
-  DGMI_TRY(kry::combine<T>(n, kk, V, n, k->y, d, s));
-  *its = kk;
+  DGMI_TRY(kry::sbacksolve(nc, k->ldh, k->hs, k->H, k->g, k->gs, k->kk, k->y, k->ys, s));
+  DGMI_TRY(kry::scombine<T>(sg, flexible ? Z : V, n, k->y, k->ys, k->kk, d, s));
+  return 0;
+}
+
+// Per-component GMRES with an FP64 true-residual outer loop (iterative refinement when
+// the cycles run in FP32).  stats: 4 doubles per component (its, relres, outer, cycles).
+static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, double rtol, int m,
+                       int max_its, double* stats, hipStream_t s) {
+  const int nc = c->ncomp;
+  const SegDev& sg = c->seg;
+  const bool flexible = inner != DGMI_F64 && c->inner_flexible;
+  Krylov* k = krylov(c, m, inner == DGMI_F64 ? sizeof(double) : sizeof(float), flexible);
+  DGMI_CHECK(k, "device allocation failed (Krylov workspace)");
+  double* r64 = vbuf<double>(c, 0);
+  DGMI_CHECK(r64, "device allocation failed");
+  DGMI_TRY(kry::sdot<double>(sg, b, b, nullptr, k->bnorm, s));
+  std::vector<double> bnorm(nc), rnorm(nc), eps(nc);
+  std::vector<int> its(nc, 0), outer(nc, 0);
+  DGMI_HIP(hipMemcpyAsync(bnorm.data(), k->bnorm, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  DGMI_HIP(hipStreamSynchronize(s));
+  for (double v : bnorm) DGMI_CHECK(std::isfinite(v), "non-finite right-hand side");
+  int cycles = 0;
+  for (;;) {
+    DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r64, s));  // FP64 true residual
+    DGMI_TRY(kry::sdot<double>(sg, r64, r64, nullptr, k->rnorm, s));
+    DGMI_HIP(hipMemcpyAsync(k->hd, k->rnorm, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    DGMI_HIP(hipStreamSynchronize(s));
+    int nwant = 0;
+    for (int q = 0; q < nc; ++q) {
+      rnorm[q] = k->hd[q];
+      DGMI_CHECK(std::isfinite(rnorm[q]), "non-finite residual in component " + std::to_string(q));
+      const double tol = rtol * bnorm[q];
+      const bool want = !(rnorm[q] <= tol || its[q] >= max_its || bnorm[q] == 0.0);
+      k->hi[q] = want ? 1 : 0;
+      nwant += want;
+      double e = want ? std::min(0.5, 0.5 * tol / rnorm[q]) : 0.5;
+      if (inner != DGMI_F64) e = std::max(e, 1e-6);  // FP32 cycles stop near fp32 accuracy
+      eps[q] = e;
+    }
+    if (nwant == 0) break;
+    std::memcpy(k->hd + nc, eps.data(), sizeof(double) * nc);
+    DGMI_HIP(hipMemcpyAsync(k->want, k->hi, sizeof(int32_t) * nc, hipMemcpyHostToDevice, s));
+    DGMI_HIP(hipMemcpyAsync(k->eps, k->hd + nc, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    if (inner == DGMI_F64) {
+      // the cycle uses scratch slots 1..3; the correction has its own slot
+      double* dv = corr_buf<double>(c);
+      DGMI_CHECK(dv, "device allocation failed");
+      DGMI_TRY(gmres_cycle<double>(c, k, r64, dv, m, false, s));
+      DGMI_TRY(kry::sadd<double>(sg, nullptr, dv, x, s));
+    } else {  // FP32 cycle (bf16-input preconditioner applies when inner == DGMI_BF16)
+      float* r32 = vbuf<float>(c, 0);
+      float* dv = corr_buf<float>(c);
+      DGMI_CHECK(r32 && dv, "device allocation failed");
+      // residual scaled to unit norm per component in FP32; correction scaled back in FP64
+      DGMI_TRY((kry::sscale<double, float>(sg, k->rnorm, 0, k->want, r64, r32, s)));
+      DGMI_TRY(gmres_cycle<float>(c, k, r32, dv, m, flexible, s));
+      DGMI_TRY(kry::sadd<float>(sg, k->rnorm, dv, x, s));
+    }
+    DGMI_HIP(hipMemcpyAsync(k->hi + nc, k->kk, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, s));
+    DGMI_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < nc; ++q)
+      if (k->hi[q]) {
+        its[q] += k->hi[nc + q];
+        ++outer[q];
+      }
+    ++cycles;
+  }
+  if (stats)
+    for (int q = 0; q < nc; ++q) {
+      stats[4 * q + 0] = its[q];
+      stats[4 * q + 1] = bnorm[q] > 0 ? rnorm[q] / bnorm[q] : 0.0;
+      stats[4 * q + 2] = outer[q];
+      stats[4 * q + 3] = outer[q];
+    }
+  return 0;
+}
+
+// Segmented blocking of DOF vectors by component (chunks of SEG_CHUNK DOFs).
+static int build_seg(dgmi_ctx* c) {
+  SegDev& sg = c->seg;
+  sg.ncomp = c->ncomp;
+  std::vector<SegBlk> blk;
+  std::vector<int32_t> cb(c->ncomp + 1, 0);
+  for (int q = 0; q < c->ncomp; ++q) {
+    cb[q] = (int32_t)blk.size();
+    const int64_t d0 = c->comp_eoff[q] * c->nloc, d1 = c->comp_eoff[q + 1] * c->nloc;
+    for (int64_t a = d0; a < d1; a += SEG_CHUNK) blk.push_back({a, std::min(d1, a + SEG_CHUNK), q, 0});
+  }
+  cb[c->ncomp] = (int32_t)blk.size();
+  sg.nblk = (int)blk.size();
+  sg.blk = static_cast<SegBlk*>(c->alloc(sizeof(SegBlk) * std::max<size_t>(blk.size(), 1)));
+  sg.comp_blk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * cb.size()));
+  sg.partial = static_cast<double*>(c->alloc(sizeof(double) * MDOT_K * std::max(sg.nblk, 1)));
+  sg.coef = static_cast<double*>(c->alloc(sizeof(double) * MDOT_K * c->ncomp));
+  DGMI_CHECK(sg.blk && sg.comp_blk && sg.partial && sg.coef, "device allocation failed");
+  DGMI_HIP(hipMemcpy(sg.blk, blk.data(), sizeof(SegBlk) * blk.size(), hipMemcpyHostToDevice));
+  DGMI_HIP(hipMemcpy(sg.comp_blk, cb.data(), sizeof(int32_t) * cb.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -358,7 +489,8 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
   DGMI_CHECK(d->D == 2 || d->D == 3, "D must be 2 or 3");
   DGMI_CHECK(d->nele > 0 && d->nverts > 0, "empty mesh");
   DGMI_CHECK(d->x_ref_in && d->cells && d->verts && d->kappa, "null mesh array");
-  DGMI_CHECK(d->nele <= 4000000, "nele above the 32-bit index range of the kernels (4e6)");
+  // the largest 32-bit kernel index is (nface*sngi)*nele (face traces): 48 nele in 3D
+  DGMI_CHECK(d->nele <= 40000000, "nele above the 32-bit index range of the kernels (4e7)");
   auto* c = new dgmi_ctx();
   c->D = d->D;
   c->nele = d->nele;
@@ -422,6 +554,41 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
       rc = -3;
     }
   }
+  if (!rc) rc = build_mfma_tables(c);
+  if (!rc) {
+    c->ncomp = 1;
+    c->comp_eoff = {0, c->nele};
+    c->comp_voff = {0, c->nverts};
+    rc = build_seg(c);
+  }
+  if (rc) {
+    dgmi_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int dgmi_ctx_create_batch(const dgmi_mesh_desc* d, int32_t ncomp, const int64_t* eoff,
+                          const int64_t* voff, dgmi_ctx** out) {
+  DGMI_CHECK(d && eoff && voff && out, "null argument");
+  DGMI_CHECK(ncomp >= 1, "ncomp must be at least 1");
+  DGMI_CHECK(eoff[0] == 0 && eoff[ncomp] == d->nele && voff[0] == 0 && voff[ncomp] == d->nverts,
+             "component offsets must start at 0 and end at nele / nverts");
+  for (int q = 0; q < ncomp; ++q)
+    DGMI_CHECK(eoff[q + 1] > eoff[q] && voff[q + 1] > voff[q], "empty component");
+  for (int q = 0; q < ncomp; ++q)
+    for (int64_t e = eoff[q]; e < eoff[q + 1]; ++e)
+      for (int k = 0; k <= d->D; ++k) {
+        const int64_t v = d->cells[e * (d->D + 1) + k];
+        DGMI_CHECK(v >= voff[q] && v < voff[q + 1], "a cell uses a vertex of another component");
+      }
+  dgmi_ctx* c = nullptr;
+  DGMI_TRY(dgmi_ctx_create(d, &c));
+  c->ncomp = ncomp;
+  c->comp_eoff.assign(eoff, eoff + ncomp + 1);
+  c->comp_voff.assign(voff, voff + ncomp + 1);
+  const int rc = build_seg(c);
   if (rc) {
     dgmi_ctx_destroy(c);
     return rc;
@@ -434,17 +601,12 @@ int dgmi_ctx_destroy(dgmi_ctx* c) {
   if (!c) return 0;
   (void)hipDeviceSynchronize();
   for (void* p : c->allocs) (void)hipFree(p);
-  for (auto it = g_scratch.begin(); it != g_scratch.end(); ++it)
-    if (it->first == c) {
-      g_scratch.erase(it);
-      break;
-    }
-  for (auto it = g_kry.begin(); it != g_kry.end(); ++it)
-    if (it->first == c) {
-      if (it->second.host) (void)hipHostFree(it->second.host);
-      g_kry.erase(it);
-      break;
-    }
+  delete static_cast<Scratch*>(c->scratch);
+  if (Krylov* k = static_cast<Krylov*>(c->krylov)) {
+    if (k->hd) (void)hipHostFree(k->hd);
+    if (k->hi) (void)hipHostFree(k->hi);
+    delete k;
+  }
   delete c;
   return 0;
 }
@@ -485,6 +647,16 @@ int dgmi_element_tables(const dgmi_ctx* c, double* n, double* nlx, double* weigh
   return 0;
 }
 
+// FP32 preconditioner applies inside this scope use the bf16-input matrix-core kernels
+struct Bf16Precond {
+  dgmi_ctx* c;
+  bool saved;
+  explicit Bf16Precond(dgmi_ctx* ctx, bool on = true) : c(ctx), saved(ctx->pc_bf16) {
+    if (on) c->pc_bf16 = true;
+  }
+  ~Bf16Precond() { c->pc_bf16 = saved; }
+};
+
 static int check_align(const void* p) {
   DGMI_CHECK(((uintptr_t)p & 15) == 0, "DOF vectors must be 16-byte aligned");
   return 0;
@@ -500,6 +672,9 @@ int dgmi_apply(dgmi_ctx* c, int dtype, const void* x, const void* b, void* r, vo
     return launch_apply<double>(c, c->g64, (const double*)x, (const double*)b, (double*)r, s);
   if (dtype == DGMI_F32)
     return launch_apply<float>(c, c->g32, (const float*)x, (const float*)b, (float*)r, s);
+  if (dtype == DGMI_BF16)
+    return launch_apply_mf(c, true, const_cast<float*>((const float*)x), (const float*)b,
+                           (float*)r, b ? 1 : 0, s);
   set_error("dtype not supported by dgmi_apply");
   return -2;
 }
@@ -580,6 +755,8 @@ int dgmi_smooth(dgmi_ctx* c, int dtype, void* x, const void* b, void* r_out, voi
     return launch_smooth<double>(c, c->g64, (double*)x, (const double*)b, (double*)r_out, s);
   if (dtype == DGMI_F32)
     return launch_smooth<float>(c, c->g32, (float*)x, (const float*)b, (float*)r_out, s);
+  if (dtype == DGMI_BF16)
+    return launch_apply_mf(c, true, (float*)x, (const float*)b, (float*)r_out, r_out ? 3 : 2, s);
   set_error("dtype not supported by dgmi_smooth");
   return -2;
 }
@@ -588,41 +765,62 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   DGMI_CHECK(c, "null context");
   hipStream_t s = (hipStream_t)stream;
   MGDev& mg = c->mg;
-  const int D = c->D, NB = D + 1;
+  const int D = c->D;
+  const int nc = c->ncomp;
   p1cg_sparsity(c->cells.data(), c->nele, D, c->nverts, mg.fina, mg.cola);
-  mg.cg_nonods = c->nverts;
   mg.ncola = (int64_t)mg.cola.size();
   std::vector<int32_t> whichc;
   mg.ncolor = color2(mg.fina, mg.cola, whichc);
-  hilbert_order(c->verts.data(), c->nverts, D, mg.sfc);
-  const int64_t ncg = mg.cg_nonods;
-  // restriction lists by SFC position; multiplicities
-  std::vector<int32_t> mult(ncg, 0);
-  for (int64_t v : c->cells) mult[v]++;
-  std::vector<int32_t> rptr(ncg + 1, 0), rent(c->cells.size()), cellpos(c->cells.size());
-  for (int64_t i = 0; i < ncg; ++i) rptr[mg.sfc[i]] = mult[i];  // position sfc[i]-1, +1 shift
-  for (int64_t p = 0; p < ncg; ++p) rptr[p + 1] += rptr[p];
-  std::vector<int32_t> fillp(rptr.begin(), rptr.end() - 1);
-  for (int64_t q = 0; q < (int64_t)c->cells.size(); ++q) {
-    const int64_t p = mg.sfc[c->cells[q]] - 1;
-    rent[fillp[p]++] = (int32_t)q;
-    cellpos[q] = (int32_t)p;
+  // components own contiguous vertex ranges and share no P1CG coupling
+  for (int q = 0; q < nc; ++q)
+    for (int64_t i = c->comp_voff[q]; i < c->comp_voff[q + 1]; ++i)
+      for (int64_t k = mg.fina[i]; k < mg.fina[i + 1]; ++k)
+        DGMI_CHECK(mg.cola[k] >= c->comp_voff[q] && mg.cola[k] < c->comp_voff[q + 1],
+                   "components share P1 vertices");
+  // Hilbert order of each component's own vertices (local 1-based positions)
+  std::vector<std::vector<int64_t>> lsfc(nc);
+  for (int q = 0; q < nc; ++q) {
+    const int64_t v0 = c->comp_voff[q], nv = c->comp_voff[q + 1] - v0;
+    hilbert_order(c->verts.data() + v0 * D, nv, D, lsfc[q]);
   }
-  std::vector<double> rw(ncg);
-  std::vector<int32_t> whichc_sfc(ncg);
-  for (int64_t i = 0; i < ncg; ++i) {
-    DGMI_CHECK(mult[i] > 0, "P1 vertex not used by any element");
-    rw[mg.sfc[i] - 1] = 1.0 / mult[i];
-    whichc_sfc[mg.sfc[i] - 1] = whichc[i];
-  }
-  mg.rptr = upload<int32_t>(c, rptr);
-  mg.rent = upload<int32_t>(c, rent);
-  mg.cellpos = upload<int32_t>(c, cellpos);
-  mg.rw64 = upload<double>(c, rw);
-  mg.rw32 = upload<float>(c, rw);
+  // restriction lists for a layout: SFC position of every vertex (0-based)
+  auto build_transfers = [&](const std::vector<int64_t>& pos, int64_t ncg) -> int {
+    const int64_t nv = c->nverts;
+    std::vector<int32_t> mult(nv, 0);
+    for (int64_t v : c->cells) mult[v]++;
+    std::vector<int32_t> rptr(ncg + 1, 0), rent(c->cells.size()), cellpos(c->cells.size());
+    for (int64_t i = 0; i < nv; ++i) rptr[pos[i] + 1] = mult[i];
+    for (int64_t p = 0; p < ncg; ++p) rptr[p + 1] += rptr[p];
+    std::vector<int32_t> fillp(rptr.begin(), rptr.end() - 1);
+    for (int64_t q = 0; q < (int64_t)c->cells.size(); ++q) {
+      const int64_t p = pos[c->cells[q]];
+      rent[fillp[p]++] = (int32_t)q;
+      cellpos[q] = (int32_t)p;
+    }
+    std::vector<double> rw(ncg, 0.0);
+    for (int64_t i = 0; i < nv; ++i) {
+      DGMI_CHECK(mult[i] > 0, "P1 vertex not used by any element");
+      rw[pos[i]] = 1.0 / mult[i];
+    }
+    mg.rptr = upload<int32_t>(c, rptr);
+    mg.rent = upload<int32_t>(c, rent);
+    mg.cellpos = upload<int32_t>(c, cellpos);
+    mg.rw64 = upload<double>(c, rw);
+    mg.rw32 = upload<float>(c, rw);
+    DGMI_CHECK(mg.rptr && mg.rent && mg.cellpos && mg.rw64 && mg.rw32, "device allocation failed");
+    mg.cg_nonods = ncg;
+    return 0;
+  };
+  // provisional layout for the probing: components back to back, unpadded
+  std::vector<int64_t> pos(c->nverts);
+  for (int q = 0; q < nc; ++q)
+    for (int64_t i = 0; i < c->comp_voff[q + 1] - c->comp_voff[q]; ++i)
+      pos[c->comp_voff[q] + i] = c->comp_voff[q] + lsfc[q][i] - 1;
+  DGMI_TRY(build_transfers(pos, c->nverts));
+  std::vector<int32_t> whichc_sfc(c->nverts);
+  for (int64_t i = 0; i < c->nverts; ++i) whichc_sfc[pos[i]] = whichc[i];
   int32_t* whichc_d = upload<int32_t>(c, whichc_sfc);
-  DGMI_CHECK(mg.rptr && mg.rent && mg.cellpos && mg.rw64 && mg.rw32 && whichc_d,
-             "device allocation failed");
+  DGMI_CHECK(whichc_d, "device allocation failed");
   // coloured probing of RAR (z02/volume_mf_diff.py:1696-1754): entries grouped by the
   // colour of their column; each row holds at most one entry per colour
   std::vector<int64_t> cnt(mg.ncolor + 1, 0);
@@ -632,11 +830,11 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   std::vector<int32_t> spidx(mg.ncola), rowpos(mg.ncola);
   {
     std::vector<int64_t> f2(off.begin(), off.end());
-    for (int64_t i = 0; i < ncg; ++i)
+    for (int64_t i = 0; i < c->nverts; ++i)
       for (int64_t q = mg.fina[i]; q < mg.fina[i + 1]; ++q) {
         const int cc = whichc[mg.cola[q]];
         spidx[f2[cc]] = (int32_t)q;
-        rowpos[f2[cc]] = (int32_t)(mg.sfc[i] - 1);
+        rowpos[f2[cc]] = (int32_t)pos[i];
         f2[cc]++;
       }
   }
@@ -645,7 +843,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   double* vals_d = static_cast<double*>(c->alloc(sizeof(double) * mg.ncola));
   double* probe = vbuf<double>(c, 0);
   double* aprobe = vbuf<double>(c, 1);
-  double* rsfc = cgbuf<double>(c, 0);
+  double* rsfc = static_cast<double*>(c->alloc(sizeof(double) * c->nverts));
   DGMI_CHECK(spidx_d && rowpos_d && vals_d && probe && aprobe && rsfc, "device allocation failed");
   for (int cc = 1; cc <= mg.ncolor; ++cc) {
     DGMI_TRY(launch_probe_mask(c, whichc_d, cc, probe, s));
@@ -658,57 +856,66 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   DGMI_HIP(hipMemcpyAsync(mg.rar.data(), vals_d, sizeof(double) * mg.ncola,
                           hipMemcpyDeviceToHost, s));
   DGMI_HIP(hipStreamSynchronize(s));
-  // SFC levels (z00-fortran-src/compact_fcode.f90:7-225)
-  SfcLevelsHost lh;
-  build_sfc_levels(mg.fina, mg.cola, mg.rar, mg.sfc, lh);
-  SfcLevelsDev& lv = mg.lv;
-  lv.nlevel = lh.nlevel;
-  lv.n = lh.n;
-  const int L = lh.nlevel;
-  lv.rowptr.resize(L);
-  lv.col.resize(L);
-  lv.val64.resize(L);
-  lv.diag64.resize(L);
-  lv.val32.resize(L);
-  lv.diag32.resize(L);
-  lv.r.resize(L);
-  lv.e.resize(L);
-  lv.e2.resize(L);
-  for (int l = 0; l < L; ++l) {
-    lv.rowptr[l] = upload<int32_t>(c, lh.rowptr[l]);
-    lv.col[l] = upload<int32_t>(c, lh.col[l]);
-    lv.val64[l] = upload<double>(c, lh.val[l]);
-    lv.diag64[l] = upload<double>(c, lh.diag[l]);
-    lv.val32[l] = upload<float>(c, lh.val[l]);
-    lv.diag32[l] = upload<float>(c, lh.diag[l]);
-    lv.r[l] = c->alloc(sizeof(double) * lh.n[l]);
-    lv.e[l] = c->alloc(sizeof(double) * lh.n[l]);
-    lv.e2[l] = c->alloc(sizeof(double) * lh.n[l]);
-    DGMI_CHECK(lv.rowptr[l] && lv.col[l] && lv.val64[l] && lv.diag64[l] && lv.val32[l] &&
-                   lv.diag32[l] && lv.r[l] && lv.e[l] && lv.e2[l],
-               "device allocation failed");
+  // SFC levels of every component (z00-fortran-src/compact_fcode.f90:7-225)
+  SfcLayout lay;
+  lay.lh.resize(nc);
+  lay.start.resize(nc);
+  const char* ev = std::getenv("DGMI_SMALL_LEVEL_N");  // test hook: force grid levels
+  const int64_t small_n = ev ? std::max<int64_t>(1, std::atoll(ev)) : 0;
+  mg.comp_levels.assign(nc, {});
+  for (int q = 0; q < nc; ++q) {
+    const int64_t v0 = c->comp_voff[q], v1 = c->comp_voff[q + 1];
+    std::vector<int64_t> fq(v1 - v0 + 1), cq(mg.fina[v1] - mg.fina[v0]);
+    std::vector<double> aq(cq.size());
+    for (int64_t i = v0; i <= v1; ++i) fq[i - v0] = mg.fina[i] - mg.fina[v0];
+    for (int64_t k = mg.fina[v0]; k < mg.fina[v1]; ++k) {
+      cq[k - mg.fina[v0]] = mg.cola[k] - v0;
+      aq[k - mg.fina[v0]] = mg.rar[k];
+    }
+    build_sfc_levels(fq, cq, aq, lsfc[q], lay.lh[q]);
+    const SfcLevelsHost& h = lay.lh[q];
+    std::vector<int64_t> nnz(h.nlevel);
+    for (int l = 0; l < h.nlevel; ++l) nnz[l] = (int64_t)h.col[l].size();
+    // the coarse end that fits in LDS (120 KiB of the 160 KiB per work-group) runs in
+    // one work-group; larger levels run as grid kernels
+    int l0 = lds_start_level(h.n, nnz, 120 * 1024);
+    DGMI_CHECK(l0 >= 0, "too many SFC levels");
+    if (small_n)
+      for (int l = 0; l < h.nlevel; ++l)
+        if (h.n[l] <= small_n) {
+          l0 = std::max(l0, std::min(l, h.nlevel - 1));
+          break;
+        }
+    lay.start[q] = l0;
+    mg.comp_levels[q] = h.n;
   }
-  // levels with at most SMALL_N nodes run in the single-work-group kernel
-  // (DGMI_SMALL_LEVEL_N overrides the threshold, e.g. to exercise the grid kernels on
-  // small meshes in tests)
-  // The coarse end of the hierarchy that fits in LDS (120 KiB of the 160 KiB per work-group)
-  // runs in one work-group; larger levels run as grid kernels.
-  lv.nnz.resize(L);
-  for (int l = 0; l < L; ++l) lv.nnz[l] = (int64_t)lh.col[l].size();
-  int l0 = choose_small_level(lv, lv.nnz, 120 * 1024);
-  DGMI_CHECK(l0 >= 0, "too many SFC levels");
-  if (const char* ev = std::getenv("DGMI_SMALL_LEVEL_N")) {
-    const int64_t small_n = std::max<int64_t>(1, std::atoll(ev));
-    for (int l = 0; l < L; ++l)
-      if (lh.n[l] <= small_n) {
-        l0 = std::max(l0, std::min(l, L - 1));
-        break;
-      }
+  // final layout: components by start level, descending; segments padded to a multiple
+  // of 2^start so that every component's pairs stay inside it (the last one needs none)
+  lay.order.resize(nc);
+  std::iota(lay.order.begin(), lay.order.end(), 0);
+  std::stable_sort(lay.order.begin(), lay.order.end(),
+                   [&](int a, int b) { return lay.start[a] > lay.start[b]; });
+  lay.off0.assign(nc, 0);
+  lay.len0.assign(nc, 0);
+  lay.smax = lay.start[lay.order[0]];
+  int64_t total = 0;
+  for (int k = 0; k < nc; ++k) {
+    const int q = lay.order[k];
+    const int64_t n0 = lay.lh[q].n[0];
+    const int64_t a = int64_t(1) << lay.start[q];
+    lay.off0[q] = total;
+    lay.len0[q] = (k + 1 == nc) ? n0 : (n0 + a - 1) / a * a;
+    total += lay.len0[q];
   }
-  mg.small_level = l0;
-  DGMI_TRY(set_small_lds_attr<double>(lds_bytes_from<double>(lv, lv.nnz, l0)));
-  DGMI_TRY(set_small_lds_attr<float>(lds_bytes_from<float>(lv, lv.nnz, l0)));
-  DGMI_TRY(upload_level_descs(c));
+  mg.sfc.assign(c->nverts, 0);
+  for (int q = 0; q < nc; ++q)
+    for (int64_t i = 0; i < c->comp_voff[q + 1] - c->comp_voff[q]; ++i) {
+      pos[c->comp_voff[q] + i] = lay.off0[q] + lsfc[q][i] - 1;
+      mg.sfc[c->comp_voff[q] + i] = pos[c->comp_voff[q] + i] + 1;
+    }
+  if (nc > 1) DGMI_TRY(build_transfers(pos, total));
+  DGMI_TRY(upload_sfc_levels(c, lay));
+  mg.small_level = lay.smax;
   mg.ready = true;
   return 0;
 }
@@ -716,13 +923,16 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
 int dgmi_mg_info(const dgmi_ctx* c, int64_t* info, int cap) {
   DGMI_CHECK(c && info, "null argument");
   DGMI_CHECK(c->mg.ready, "multigrid not set up");
-  const int L = c->mg.lv.nlevel;
+  // one component: its level sizes; several: the padded level-0 length and the sizes of
+  // the first component
+  const std::vector<int64_t>& n = c->mg.comp_levels[0];
+  const int L = (int)n.size();
   DGMI_CHECK(cap >= 4 + L, "info buffer too small");
   info[0] = c->mg.cg_nonods;
   info[1] = c->mg.ncola;
   info[2] = c->mg.ncolor;
   info[3] = L;
-  for (int l = 0; l < L; ++l) info[4 + l] = c->mg.lv.n[l];
+  for (int l = 0; l < L; ++l) info[4 + l] = n[l];
   return 0;
 }
 
@@ -774,6 +984,10 @@ int dgmi_precond(dgmi_ctx* c, int dtype, const void* b, void* x, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   if (dtype == DGMI_F64) return precond<double>(c, (const double*)b, (double*)x, s);
   if (dtype == DGMI_F32) return precond<float>(c, (const float*)b, (float*)x, s);
+  if (dtype == DGMI_BF16) {
+    Bf16Precond scope(c);
+    return precond<float>(c, (const float*)b, (float*)x, s);
+  }
   set_error("dtype not supported");
   return -2;
 }
@@ -782,60 +996,12 @@ int dgmi_gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, double 
                      int max_its, double* stats, void* stream) {
   DGMI_CHECK(c && b && x, "null argument");
   DGMI_CHECK(m >= 1 && m <= 200, "restart length m must be in [1, 200]");
-  DGMI_CHECK(inner == DGMI_F64 || inner == DGMI_F32, "inner dtype not supported");
+  DGMI_CHECK(inner == DGMI_F64 || inner == DGMI_F32 || inner == DGMI_BF16,
+             "inner dtype not supported");
   DGMI_TRY(check_align(b));
   DGMI_TRY(check_align(x));
-  hipStream_t s = (hipStream_t)stream;
-  const int64_t n = c->ndof;
-  Krylov* k = krylov(c, m, inner == DGMI_F64 ? sizeof(double) : sizeof(float));
-  DGMI_CHECK(k, "device allocation failed (Krylov workspace)");
-  double* r64 = vbuf<double>(c, 0);
-  DGMI_CHECK(r64, "device allocation failed");
-  // ||b||
-  DGMI_TRY(launch_dot<double>(n, b, b, k->red + 4, 1, s));
-  DGMI_HIP(hipMemcpyAsync(k->host + 4, k->red + 5, sizeof(double), hipMemcpyDeviceToHost, s));
-  DGMI_HIP(hipStreamSynchronize(s));
-  const double bnorm = k->host[4];
-  DGMI_CHECK(std::isfinite(bnorm), "non-finite right-hand side");
-  const double tol = rtol * bnorm;
-  int its = 0, outer = 0, cycles = 0;
-  double rnorm = 0.0;
-  for (;;) {
-    DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r64, s));  // FP64 true residual
-    DGMI_TRY(launch_dot<double>(n, r64, r64, k->red + 4, 1, s));
-    DGMI_HIP(hipMemcpyAsync(k->host + 4, k->red + 5, sizeof(double), hipMemcpyDeviceToHost, s));
-    DGMI_HIP(hipStreamSynchronize(s));
-    rnorm = k->host[4];
-    DGMI_CHECK(std::isfinite(rnorm), "non-finite residual");
-    if (rnorm <= tol || its >= max_its || bnorm == 0.0) break;
-    const double eps = std::min(0.5, 0.5 * tol / rnorm);
-    int used = 0;
-    if (inner == DGMI_F64) {
-      // the cycle uses scratch slots 1..3; the correction has its own slot
-      double* dv = corr_buf<double>(c);
-      DGMI_CHECK(dv, "device allocation failed");
-      DGMI_TRY(gmres_cycle<double>(c, r64, dv, m, eps, &used, s));
-      DGMI_TRY(kry::add_cast<double>(n, dv, x, nullptr, s));
-    } else {
-      float* r32 = vbuf<float>(c, 0);
-      float* dv = corr_buf<float>(c);
-      DGMI_CHECK(r32 && dv, "device allocation failed");
-      // scaled residual in FP32; the correction is scaled back in FP64
-      DGMI_TRY((kry::cast<double, float>(n, r64, r32, 1.0 / rnorm, s)));
-      DGMI_TRY(gmres_cycle<float>(c, r32, dv, m, std::max(eps * 1.0, 1e-6), &used, s));
-      DGMI_TRY(kry::add_cast<float>(n, dv, x, k->red + 5, s));  // red[5] = ||r||
-    }
-    its += used;
-    ++outer;
-    ++cycles;
-  }
-  if (stats) {
-    stats[0] = its;
-    stats[1] = bnorm > 0 ? rnorm / bnorm : 0.0;
-    stats[2] = outer;
-    stats[3] = cycles;
-  }
-  return 0;
+  Bf16Precond scope(c, inner == DGMI_BF16);
+  return gmres_solve(c, inner, b, x, rtol, m, max_its, stats, (hipStream_t)stream);
 }
 
 int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
@@ -844,6 +1010,17 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "gmres_orth") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "gmres_orth must be 0 (MGS) or 1 (CGS2)");
     c->gmres_orth = (int)value;
+    return 0;
+  }
+  if (k == "inner_flexible") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "inner_flexible must be 0 or 1");
+    c->inner_flexible = (int)value;
+    return 0;
+  }
+  if (k == "f32_kernel") {
+    DGMI_CHECK(value == 0.0 || value == 1.0 || value == 2.0,
+               "f32_kernel must be 0 (vector), 1 (MFMA fp32) or 2 (MFMA bf16 inputs)");
+    c->f32_kernel = (int)value;
     return 0;
   }
   set_error("unknown parameter " + k);

@@ -42,17 +42,61 @@ template <typename T> struct DevGeom {
   typename Vec2<T>::type* trace = nullptr;  // (nface*sngi, nele)
 };
 
+// SFC levels on the device.  "Grid" levels 0..small_level hold every component whose
+// single-work-group start level is at least l, concatenated in component order (largest
+// start level first); a component's segment at level l starts at its level-0 offset / 2^l,
+// so the pairwise restriction (2I, 2I+1) -> I and the nearest upsample i >> 1 act within
+// the component.  Segments are padded with empty rows (diagonal 1) where needed.  Levels
+// past a component's start level live in a packed per-component pool.
 struct SfcLevelsDev {
-  int nlevel = 0;
-  std::vector<int64_t> n, nnz;
+  int nlevel = 0;                 // grid-layout levels held (small_level + 1)
+  std::vector<int64_t> n, nnz;    // rows and entries per grid level
+  std::vector<int64_t> grid_rows;  // rows swept by the grid kernels at level l < small_level
   std::vector<int32_t*> rowptr, col;
   std::vector<double*> val64, diag64;
   std::vector<float*> val32, diag32;
   // per-level work vectors (both precisions share the byte buffers, sized for FP64)
   std::vector<void*> r, e, e2;
-  // device arrays of LevelDesc<double> / LevelDesc<float> for the single-work-group kernel
+  // packed deeper levels of every component
+  int32_t* pool_rp = nullptr;
+  int32_t* pool_col = nullptr;
+  double* pool_val64 = nullptr;
+  double* pool_diag64 = nullptr;
+  float* pool_val32 = nullptr;
+  float* pool_diag32 = nullptr;
+  void* pool_vec = nullptr;  // r, A, B of every pooled level (FP64-sized)
+  // device tables for the single-work-group kernel: LevelDesc<T>[ncomp][MAX_LEVELS] and
+  // (start level, level count) per component
   void* desc64 = nullptr;
   void* desc32 = nullptr;
+  int32_t* rng = nullptr;
+  bool any_start0 = false;  // some component starts at level 0
+  size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the single-work-group kernel
+};
+
+// Blocks of DOF vectors that never cross a component (an independent solve packed into
+// the context).  Reductions sum a component's blocks in a fixed order.
+struct SegBlk {
+  int64_t beg, end;
+  int32_t comp, pad;
+};
+constexpr int MDOT_K = 32;     // basis vectors per fused Gram-Schmidt pass
+constexpr int SEG_CHUNK = 1024;  // DOFs per segmented block
+struct SegDev {
+  int ncomp = 1;
+  int nblk = 0;
+  SegBlk* blk = nullptr;        // (nblk)
+  int32_t* comp_blk = nullptr;  // (ncomp + 1) first block of each component
+  double* partial = nullptr;    // (nblk * MDOT_K)
+  double* coef = nullptr;       // (ncomp * MDOT_K)
+};
+
+// Operand tables of the matrix-core apply (apply_mfma.hip), in MFMA lane order.
+struct MfTabs {
+  void* b1 = nullptr;   // bf16 A operands of Y = T Xe, lane order
+  void* b2 = nullptr;   // bf16 A operands of r = T^T Z, lane order
+  float* wv = nullptr;  // quadrature weights: volume points, then face points
+  float* tT = nullptr;  // fp32 T, row-major (rows x nloc) plus 16 zero words
 };
 
 struct MGDev {
@@ -70,33 +114,33 @@ struct MGDev {
   // prolongation: per p1dg index e*(D+1)+k, the SFC position of its vertex
   int32_t* cellpos = nullptr;
   SfcLevelsDev lv;
-  int small_level = 0;  // first level handled by the single-workgroup kernel
-  bool small_lds = true;  // stage the small levels in LDS
-  void* r_sfc = nullptr;  // (cg_nonods) fp64-sized scratch
-  void* e_sfc = nullptr;
+  int small_level = 0;  // grid kernels sweep levels below this one
+  // level sizes of every component (host), components in context order
+  std::vector<std::vector<int64_t>> comp_levels;
 };
 
 namespace kry {
 template <typename T>
-int axpy_dev(int64_t n, const double* a, double sgn, const T* x, T* y, hipStream_t s);
+int sdot(const SegDev& sg, const T* x, const T* y, double* out, double* sq, hipStream_t s);
 template <typename T>
-int scale_dev(int64_t n, const double* a, const T* x, T* y, hipStream_t s);
+int sorth_pass(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h, int64_t hs,
+               int accumulate, const int32_t* active, hipStream_t s);
 template <typename Tin, typename Tout>
-int cast(int64_t n, const Tin* x, Tout* y, double scale, hipStream_t s);
+int sscale(const SegDev& sg, const double* a, int mode, const int32_t* active, const Tin* x,
+           Tout* y, hipStream_t s);
 template <typename Tin>
-int add_cast(int64_t n, const Tin* x, double* y, const double* scale, hipStream_t s);
+int sadd(const SegDev& sg, const double* a, const Tin* x, double* y, hipStream_t s);
 template <typename T>
-int fill(int64_t n, T* y, T v, hipStream_t s);
-template <typename T>
-int combine(int64_t n, int k, const T* V, int64_t ldv, const double* y, T* x, hipStream_t s);
-int givens(int j, int ldh, double* H, double* cs, double* sn, double* g, double* res,
-           hipStream_t s);
-int backsolve(int k, int ldh, const double* H, const double* g, double* y, hipStream_t s);
-int store_h(const double* src, double* dst, int mode, hipStream_t s);
-int set_g(double* g, int len, const double* beta, hipStream_t s);
-template <typename T>
-int cgs_pass(int64_t n, int k, const T* V, int64_t ldv, T* w, double* h, int accumulate,
-             hipStream_t s);
+int scombine(const SegDev& sg, const T* V, int64_t ldv, const double* yv, int64_t ys,
+             const int32_t* kk, T* x, hipStream_t s);
+int scycle_start(int ncomp, const double* beta, const double* eps, const int32_t* want,
+                 int32_t* active, double* thr, double* g, int64_t gs, int glen, int32_t* kk,
+                 hipStream_t s);
+int sgivens(int ncomp, int j, int ldh, int64_t hs, double* H, const double* hn, double* cs,
+            double* sn, int64_t rs, double* g, int64_t gs, double* res, const double* thr,
+            int32_t* active, int32_t* kk, int32_t* nact, hipStream_t s);
+int sbacksolve(int ncomp, int ldh, int64_t hs, const double* H, const double* g, int64_t gs,
+               const int32_t* kk, double* yv, int64_t ys, hipStream_t s);
 }  // namespace kry
 
 }  // namespace dgmi
@@ -117,6 +161,20 @@ struct dgmi_ctx {
   dgmi::DevGeom<float> g32;
   int smoother_variant = -1;
   int gmres_orth = 1;  // 0: modified Gram-Schmidt, 1: CGS2 (fused passes)
+  // kernel family of every FP32-vector apply/smooth: 0 vector FMA, 1 MFMA fp32,
+  // 2 MFMA with bf16 inputs
+  int f32_kernel = 1;
+  bool pc_bf16 = false;  // FP32 preconditioner applies with bf16 inputs
+  // FP32 refinement cycles: 1 flexible (right-preconditioned) GMRES, 0 left-preconditioned
+  int inner_flexible = 1;
+  // independent solves packed in this context: component c owns elements
+  // [comp_eoff[c], comp_eoff[c+1]) and vertices [comp_voff[c], comp_voff[c+1])
+  int ncomp = 1;
+  std::vector<int64_t> comp_eoff, comp_voff;
+  dgmi::SegDev seg;
+  void* scratch = nullptr;  // solver scratch vectors (dgmi_api.hip)
+  void* krylov = nullptr;   // Krylov workspace (dgmi_api.hip)
+  dgmi::MfTabs mf;
   double omega = 2.0 / 3.0;
   dgmi::MGDev mg;
   // solver scratch (FP64 and FP32 vectors of ndof), created on demand
@@ -134,6 +192,10 @@ int launch_apply_pos(dgmi_ctx* c, DevGeom<T>& g, const T* x, T* r, hipStream_t s
 template <typename T>
 int launch_smooth(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r_out, hipStream_t s);
 int launch_extract_diag(dgmi_ctx* c, const double* B, double* d, hipStream_t s);
+// matrix-core apply (apply_mfma.hip); mode 0 r=-Ax, 1 r=b-Ax, 2 x+=S(b-Ax), 3 as 2 + r, 4 r=Ax
+int build_mfma_tables(dgmi_ctx* c);
+int launch_apply_mf(dgmi_ctx* c, bool bf16, float* x, const float* b, float* r, int mode,
+                    hipStream_t s);
 int launch_rhs(dgmi_ctx* c, const double* refm, const double* detJ, const double* f,
                const double* ubc, double* b, hipStream_t s);
 template <typename T>
@@ -150,10 +212,21 @@ template <typename T>
 int launch_prolong_add(dgmi_ctx* c, const T* e_sfc, T* x_dg, hipStream_t s);
 template <typename T>
 int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s);
-int upload_level_descs(dgmi_ctx* c);
-int choose_small_level(const SfcLevelsDev& lv, const std::vector<int64_t>& nnz, size_t budget);
+// single-work-group part of the V-cycle: LDS bytes of levels l0.. of one component, the
+// first level from which a component fits in `budget`, and the device level tables
 template <typename T>
-size_t lds_bytes_from(const SfcLevelsDev& lv, const std::vector<int64_t>& nnz, int l0);
+size_t lds_bytes_from(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, int l0);
+int lds_start_level(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, size_t budget);
+// Host description of the SFC hierarchy of every component and of its device layout.
+struct SfcLayout {
+  std::vector<SfcLevelsHost> lh;  // per component (context order), local indices
+  std::vector<int> start;         // single-work-group start level per component
+  std::vector<int> order;         // components in layout order (start level descending)
+  std::vector<int64_t> off0;      // level-0 offset of each component (context order)
+  std::vector<int64_t> len0;      // level-0 segment length (padded except the last)
+  int smax = 0;
+};
+int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay);
 template <typename T>
 int set_small_lds_attr(size_t bytes);
 int launch_probe_mask(dgmi_ctx* c, const int32_t* whichc_dev, int color, double* x_dg,

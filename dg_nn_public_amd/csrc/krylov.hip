@@ -58,118 +58,54 @@ __global__ void k_axpy(int64_t n, double a, const T* __restrict__ x, T* __restri
   if (i < n) y[i] = (T)((double)y[i] + a * (double)x[i]);
 }
 
-// y += sgn * (*a) x  (a on the device)
-template <typename T>
-__global__ void k_axpy_dev(int64_t n, const double* __restrict__ a, double sgn,
-                           const T* __restrict__ x, T* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double av = sgn * a[0];
-  if (i < n) y[i] = (T)((double)y[i] + av * (double)x[i]);
-}
+// Fused classical Gram-Schmidt passes run over up to MDOT_K basis vectors at a time:
+//   h[j] = <w, V_j> (one read of w for all j), then w -= sum_j h[j] V_j.
 
-// y = x / (*a)
-template <typename T>
-__global__ void k_scale_dev(int64_t n, const double* __restrict__ a, const T* __restrict__ x,
-                            T* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double inv = 1.0 / a[0];
-  if (i < n) y[i] = (T)((double)x[i] * inv);
-}
-
-template <typename Tin, typename Tout>
-__global__ void k_cast(int64_t n, const Tin* __restrict__ x, Tout* __restrict__ y, double scale) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = (Tout)((double)x[i] * scale);
-}
-
-// y += scale * x (x of another precision): the refinement update x64 += d32
-template <typename Tin>
-__global__ void k_add_cast(int64_t n, const Tin* __restrict__ x, double* __restrict__ y,
-                           const double* __restrict__ scale) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double s = scale ? scale[0] : 1.0;
-  if (i < n) y[i] += s * (double)x[i];
-}
-
-template <typename T>
-__global__ void k_fill(int64_t n, T* __restrict__ y, T v) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = v;
-}
-
-// x += sum_j y[j] V[j]  (V rows of length n, k rows)
-template <typename T>
-__global__ void k_combine(int64_t n, int k, const T* __restrict__ V, int64_t ldv,
-                          const double* __restrict__ yv, T* __restrict__ x) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = (double)x[i];
-  for (int j = 0; j < k; ++j) s += yv[j] * (double)V[(int64_t)j * ldv + i];
-  x[i] = (T)s;
-}
-
-// Givens step for column j of the Hessenberg matrix (Saad, Alg. 6.9 with rotations):
-// H column-major (m+1) x m in `H`; cs/sn rotations; g right-hand side.  Applies the j
-// previous rotations to column j, forms rotation j, updates g, stores |g[j+1]| in res.
-__global__ void k_givens(int j, int ldh, double* __restrict__ H, double* __restrict__ cs,
-                         double* __restrict__ sn, double* __restrict__ g,
-                         double* __restrict__ res) {
-  if (threadIdx.x != 0) return;
-  double* h = H + (int64_t)j * ldh;
-  for (int i = 0; i < j; ++i) {
-    const double t = cs[i] * h[i] + sn[i] * h[i + 1];
-    h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-    h[i] = t;
-  }
-  const double a = h[j], b = h[j + 1];
-  const double r = hypot(a, b);
-  const double c = r > 0 ? a / r : 1.0, s = r > 0 ? b / r : 0.0;
-  cs[j] = c;
-  sn[j] = s;
-  h[j] = r;
-  h[j + 1] = 0.0;
-  g[j + 1] = -s * g[j];
-  g[j] = c * g[j];
-  res[0] = fabs(g[j + 1]);
-}
-
-// back substitution R y = g for the leading k x k block; y into yv
-__global__ void k_backsolve(int k, int ldh, const double* __restrict__ H,
-                            const double* __restrict__ g, double* __restrict__ yv) {
-  if (threadIdx.x != 0) return;
-  for (int i = k - 1; i >= 0; --i) {
-    double s = g[i];
-    for (int c = i + 1; c < k; ++c) s -= H[(int64_t)c * ldh + i] * yv[c];
-    yv[i] = s / H[(int64_t)i * ldh + i];
-  }
-}
-
-// H[i, j] = hh[0] ; used to move a reduced dot into the Hessenberg column
-__global__ void k_store_h(const double* __restrict__ src, double* __restrict__ dst, int mode) {
-  if (threadIdx.x == 0) dst[0] = mode == 1 ? sqrt(src[0]) : src[0];
-}
-
-__global__ void k_set_g(double* __restrict__ g, int len, const double* __restrict__ beta) {
-  for (int i = threadIdx.x; i < len; i += blockDim.x) g[i] = i == 0 ? beta[1] : 0.0;
-}
 
 // ---------------------------------------------------------------------------------------
-// Fused classical Gram-Schmidt pass over up to MDOT_K basis vectors:
-//   h[j] = <w, V_j> (one read of w for all j), then w -= sum_j h[j] V_j.
-constexpr int MDOT_K = 32;
-constexpr int MDOT_BLOCKS = 2048;
+// Segmented (per-component) kernels.  A context may hold several independent solves
+// ("components", contiguous DOF ranges).  Vectors are cut into blocks that never cross a
+// component (SegBlk); every reduction sums a component's blocks in a fixed order, so a
+// component's dot products do not depend on what else is in the batch.  Per-component
+// scalars live in device arrays indexed by component.
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_sdot_partial(const SegBlk* __restrict__ sb,
+                                                              const T* __restrict__ x,
+                                                              const T* __restrict__ y,
+                                                              double* __restrict__ partial) {
+  const SegBlk b = sb[blockIdx.x];
+  double acc = 0.0;
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) acc += (double)x[i] * (double)y[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// one work-group per component: out[c] = sum of its partials; sq[c] = sqrt(out[c])
+__global__ __launch_bounds__(RED_THREADS) void k_sdot_final(const int32_t* __restrict__ comp_blk,
+                                                            const double* __restrict__ partial,
+                                                            double* __restrict__ out,
+                                                            double* __restrict__ sq) {
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int b = comp_blk[c] + threadIdx.x; b < comp_blk[c + 1]; b += blockDim.x) acc += partial[b];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) {
+    if (out) out[c] = acc;
+    if (sq) sq[c] = sqrt(acc);
+  }
+}
 
 template <typename T>
-__global__ __launch_bounds__(RED_THREADS) void k_mdot_partial(int64_t n, int k,
-                                                              const T* __restrict__ V,
-                                                              int64_t ldv,
-                                                              const T* __restrict__ w,
-                                                              double* __restrict__ partial) {
+__global__ __launch_bounds__(RED_THREADS) void k_smdot_partial(const SegBlk* __restrict__ sb,
+                                                               int k, const T* __restrict__ V,
+                                                               int64_t ldv,
+                                                               const T* __restrict__ w,
+                                                               double* __restrict__ partial) {
+  const SegBlk b = sb[blockIdx.x];
   double acc[MDOT_K];
 #pragma unroll
   for (int j = 0; j < MDOT_K; ++j) acc[j] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) {
     const double wi = (double)w[i];
 #pragma unroll
     for (int j = 0; j < MDOT_K; ++j)
@@ -186,44 +122,175 @@ __global__ __launch_bounds__(RED_THREADS) void k_mdot_partial(int64_t n, int k,
   }
   __syncthreads();
   if (threadIdx.x < k) {
-    double s = 0.0;
-    for (int ww = 0; ww < RED_THREADS / 64; ++ww) s += sh[ww][threadIdx.x];
-    partial[(int64_t)blockIdx.x * MDOT_K + threadIdx.x] = s;
+    double t = 0.0;
+    for (int ww = 0; ww < RED_THREADS / 64; ++ww) t += sh[ww][threadIdx.x];
+    partial[(int64_t)blockIdx.x * MDOT_K + threadIdx.x] = t;
   }
 }
 
-// h_out[j] (+)= s_j and tmp[j] = s_j, s_j = sum over blocks of partial[b][j]
-__global__ __launch_bounds__(RED_THREADS) void k_mdot_final(const double* __restrict__ partial,
-                                                            int nb, int k,
-                                                            double* __restrict__ h_out,
-                                                            double* __restrict__ tmp,
-                                                            int accumulate) {
+// per component c: coef[c][j] = sum of partials (0 when inactive); h[c*hs + j] (+)= coef
+__global__ __launch_bounds__(RED_THREADS) void k_smdot_final(const int32_t* __restrict__ comp_blk,
+                                                             const double* __restrict__ partial,
+                                                             int k, double* __restrict__ h,
+                                                             int64_t hs, double* __restrict__ coef,
+                                                             int accumulate,
+                                                             const int32_t* __restrict__ active) {
+  const int c = blockIdx.x;
+  const bool on = active[c] != 0;
   for (int j = 0; j < k; ++j) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(int64_t)b * MDOT_K + j];
-    s = block_sum(s);
+    double acc = 0.0;
+    for (int b = comp_blk[c] + threadIdx.x; b < comp_blk[c + 1]; b += blockDim.x)
+      acc += partial[(int64_t)b * MDOT_K + j];
+    acc = block_sum(acc);
     if (threadIdx.x == 0) {
-      tmp[j] = s;
-      h_out[j] = accumulate ? h_out[j] + s : s;
+      const double v = on ? acc : 0.0;
+      coef[c * MDOT_K + j] = v;
+      if (on) h[c * hs + j] = accumulate ? h[c * hs + j] + v : v;
     }
     __syncthreads();
   }
 }
 
+// w -= sum_j coef[comp][j] V_j
 template <typename T>
-__global__ void k_maxpy(int64_t n, int k, const T* __restrict__ V, int64_t ldv,
-                        const double* __restrict__ h, T* __restrict__ w) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = (double)w[i];
-  for (int j = 0; j < k; ++j) s -= h[j] * (double)V[(int64_t)j * ldv + i];
-  w[i] = (T)s;
+__global__ __launch_bounds__(RED_THREADS) void k_smaxpy(const SegBlk* __restrict__ sb, int k,
+                                                        const T* __restrict__ V, int64_t ldv,
+                                                        const double* __restrict__ coef,
+                                                        T* __restrict__ w) {
+  const SegBlk b = sb[blockIdx.x];
+  const double* cf = coef + b.comp * MDOT_K;
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) {
+    double s = (double)w[i];
+    for (int j = 0; j < k; ++j) s -= cf[j] * (double)V[(int64_t)j * ldv + i];
+    w[i] = (T)s;
+  }
+}
+
+// y = scale * x per component: mode 0 scale = 1/a[c]; 1 scale = a[c].  Inactive
+// components (or a[c] = 0 in mode 0) get y = 0.
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(RED_THREADS) void k_sscale(const SegBlk* __restrict__ sb,
+                                                        const double* __restrict__ a, int mode,
+                                                        const int32_t* __restrict__ active,
+                                                        const Tin* __restrict__ x,
+                                                        Tout* __restrict__ y) {
+  const SegBlk b = sb[blockIdx.x];
+  const double av = a[b.comp];
+  const bool on = (!active || active[b.comp]) && (mode == 1 || av != 0.0);
+  const double f = on ? (mode == 0 ? 1.0 / av : av) : 0.0;
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) y[i] = (Tout)((double)x[i] * f);
+}
+
+// y64 += a[c] * x
+template <typename Tin>
+__global__ __launch_bounds__(RED_THREADS) void k_sadd(const SegBlk* __restrict__ sb,
+                                                      const double* __restrict__ a,
+                                                      const Tin* __restrict__ x,
+                                                      double* __restrict__ y) {
+  const SegBlk b = sb[blockIdx.x];
+  const double av = a ? a[b.comp] : 1.0;
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) y[i] += av * (double)x[i];
+}
+
+// x = sum_{j < kk[c]} yv[c][j] V_j
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_scombine(const SegBlk* __restrict__ sb,
+                                                          const T* __restrict__ V, int64_t ldv,
+                                                          const double* __restrict__ yv,
+                                                          int64_t ys, const int32_t* __restrict__ kk,
+                                                          T* __restrict__ x) {
+  const SegBlk b = sb[blockIdx.x];
+  const int k = kk[b.comp];
+  const double* yc = yv + b.comp * ys;
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += yc[j] * (double)V[(int64_t)j * ldv + i];
+    x[i] = (T)s;
+  }
+}
+
+// Cycle start, one thread per component: active[c] = want[c] and beta[c] finite > 0;
+// thr[c] = eps[c] beta[c]; g[c] = (beta[c], 0, ...); kk[c] = 0.
+__global__ void k_scycle_start(int ncomp, const double* __restrict__ beta,
+                               const double* __restrict__ eps, const int32_t* __restrict__ want,
+                               int32_t* __restrict__ active, double* __restrict__ thr,
+                               double* __restrict__ g, int64_t gs, int glen,
+                               int32_t* __restrict__ kk) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncomp) return;
+  const double b = beta[c];
+  const bool on = want[c] != 0 && b > 0.0 && isfinite(b);
+  active[c] = on ? 1 : 0;
+  thr[c] = eps[c] * b;
+  for (int i = 0; i < glen; ++i) g[c * gs + i] = (i == 0 && on) ? b : 0.0;
+  kk[c] = 0;
+}
+
+// Givens step j for every active component (Saad, Alg. 6.9 with rotations); the
+// subdiagonal entry comes from hn[c].  A component whose estimate |g[j+1]| falls to
+// thr[c] (or whose basis breaks down) turns inactive.  nact[0] counts the still-active ones.
+__global__ void k_sgivens(int ncomp, int j, int ldh, int64_t hs, double* __restrict__ H,
+                          const double* __restrict__ hn, double* __restrict__ cs,
+                          double* __restrict__ sn, int64_t rs, double* __restrict__ g,
+                          int64_t gs, double* __restrict__ res, const double* __restrict__ thr,
+                          int32_t* __restrict__ active, int32_t* __restrict__ kk,
+                          int32_t* __restrict__ nact) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncomp) return;
+  if (!active[c]) return;
+  double* h = H + c * hs + (int64_t)j * ldh;
+  double* csc = cs + c * rs;
+  double* snc = sn + c * rs;
+  double* gc = g + c * gs;
+  h[j + 1] = hn[c];
+  for (int i = 0; i < j; ++i) {
+    const double t = csc[i] * h[i] + snc[i] * h[i + 1];
+    h[i + 1] = -snc[i] * h[i] + csc[i] * h[i + 1];
+    h[i] = t;
+  }
+  const double a = h[j], bb = h[j + 1];
+  const double r = hypot(a, bb);
+  const double cc = r > 0 ? a / r : 1.0, ss = r > 0 ? bb / r : 0.0;
+  csc[j] = cc;
+  snc[j] = ss;
+  h[j] = r;
+  h[j + 1] = 0.0;
+  gc[j + 1] = -ss * gc[j];
+  gc[j] = cc * gc[j];
+  res[c] = fabs(gc[j + 1]);
+  kk[c] = j + 1;
+  const bool stop = !(res[c] > thr[c]) || !(hn[c] > 0.0) || r == 0.0;
+  if (stop) active[c] = 0;
+  else atomicAdd(nact, 1);
+}
+
+// back substitution R y = g per component over its kk[c] columns
+__global__ void k_sbacksolve(int ncomp, int ldh, int64_t hs, const double* __restrict__ H,
+                             const double* __restrict__ g, int64_t gs,
+                             const int32_t* __restrict__ kk, double* __restrict__ yv,
+                             int64_t ys) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncomp) return;
+  const double* Hc = H + c * hs;
+  const double* gc = g + c * gs;
+  double* yc = yv + c * ys;
+  const int k = kk[c];
+  for (int i = k - 1; i >= 0; --i) {
+    double s = gc[i];
+    for (int q = i + 1; q < k; ++q) s -= Hc[(int64_t)q * ldh + i] * yc[q];
+    yc[i] = s / Hc[(int64_t)i * ldh + i];
+  }
+}
+
+__global__ void k_izero(int n, int32_t* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------------------
-static double* g_partial = nullptr;
-static double* g_mpartial = nullptr;
-static double* g_mtmp = nullptr;
+// partial sums of the context-free dot (one buffer per host thread, so calls from
+// different threads on different streams never share it)
+static thread_local double* g_partial = nullptr;
 
 static int ensure_partial() {
   if (!g_partial) DGMI_HIP(hipMalloc(&g_partial, sizeof(double) * RED_BLOCKS));
@@ -255,112 +322,107 @@ template int launch_axpy<float>(int64_t, double, const float*, float*, hipStream
 
 // exported to the solver translation unit
 namespace kry {
-// One classical Gram-Schmidt pass of w against V_0..V_{k-1}; writes (or, with
-// accumulate, adds) the coefficients into h.  Chunks of MDOT_K vectors.
+// ---- segmented launchers
 template <typename T>
-int cgs_pass(int64_t n, int k, const T* V, int64_t ldv, T* w, double* h, int accumulate,
-             hipStream_t s) {
-  if (!g_mpartial) DGMI_HIP(hipMalloc(&g_mpartial, sizeof(double) * MDOT_BLOCKS * MDOT_K));
-  if (!g_mtmp) DGMI_HIP(hipMalloc(&g_mtmp, sizeof(double) * MDOT_K));
-  const int nb = (int)std::min<int64_t>(MDOT_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
-  const dim3 grd((unsigned)((n + 255) / 256));
-  // within a chunk all dots use the same w (classical GS); successive chunks see the
-  // updated w (block-modified GS), which only matters for restart lengths above MDOT_K
+int sdot(const SegDev& sg, const T* x, const T* y, double* out, double* sq, hipStream_t s) {
+  hipLaunchKernelGGL((k_sdot_partial<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, x, y,
+                     sg.partial);
+  hipLaunchKernelGGL(k_sdot_final, dim3(sg.ncomp), dim3(RED_THREADS), 0, s, sg.comp_blk,
+                     (const double*)sg.partial, out, sq);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+// One classical Gram-Schmidt pass of w against V_0..V_{k-1} per component; writes (or, with
+// accumulate, adds) the coefficients into h[c*hs + j].  Chunks of MDOT_K vectors: within
+// a chunk all dots use the same w (classical GS); successive chunks see the updated w.
+template <typename T>
+int sorth_pass(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h, int64_t hs,
+               int accumulate, const int32_t* active, hipStream_t s) {
   for (int c0 = 0; c0 < k; c0 += MDOT_K) {
     const int kc = std::min(MDOT_K, k - c0);
-    hipLaunchKernelGGL((k_mdot_partial<T>), dim3(nb), dim3(RED_THREADS), 0, s, n, kc,
-                       V + (int64_t)c0 * ldv, ldv, w, g_mpartial);
-    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(RED_THREADS), 0, s, g_mpartial, nb, kc,
-                       h + c0, g_mtmp, accumulate);
-    hipLaunchKernelGGL((k_maxpy<T>), grd, dim3(256), 0, s, n, kc, V + (int64_t)c0 * ldv, ldv,
-                       (const double*)g_mtmp, w);
+    hipLaunchKernelGGL((k_smdot_partial<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, kc,
+                       V + (int64_t)c0 * ldv, ldv, (const T*)w, sg.partial);
+    hipLaunchKernelGGL(k_smdot_final, dim3(sg.ncomp), dim3(RED_THREADS), 0, s, sg.comp_blk,
+                       (const double*)sg.partial, kc, h + c0, hs, sg.coef, accumulate, active);
+    hipLaunchKernelGGL((k_smaxpy<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, kc,
+                       V + (int64_t)c0 * ldv, ldv, (const double*)sg.coef, w);
   }
   DGMI_HIP(hipGetLastError());
   return 0;
 }
-template int cgs_pass<double>(int64_t, int, const double*, int64_t, double*, double*, int,
-                              hipStream_t);
-template int cgs_pass<float>(int64_t, int, const float*, int64_t, float*, double*, int,
-                             hipStream_t);
 
-template <typename T>
-int axpy_dev(int64_t n, const double* a, double sgn, const T* x, T* y, hipStream_t s) {
-  const dim3 blk(256), grd((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL((k_axpy_dev<T>), grd, blk, 0, s, n, a, sgn, x, y);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-template <typename T>
-int scale_dev(int64_t n, const double* a, const T* x, T* y, hipStream_t s) {
-  const dim3 blk(256), grd((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL((k_scale_dev<T>), grd, blk, 0, s, n, a, x, y);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
 template <typename Tin, typename Tout>
-int cast(int64_t n, const Tin* x, Tout* y, double scale, hipStream_t s) {
-  const dim3 blk(256), grd((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL((k_cast<Tin, Tout>), grd, blk, 0, s, n, x, y, scale);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-template <typename Tin>
-int add_cast(int64_t n, const Tin* x, double* y, const double* scale, hipStream_t s) {
-  const dim3 blk(256), grd((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL((k_add_cast<Tin>), grd, blk, 0, s, n, x, y, scale);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-template <typename T>
-int fill(int64_t n, T* y, T v, hipStream_t s) {
-  const dim3 blk(256), grd((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL((k_fill<T>), grd, blk, 0, s, n, y, v);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-template <typename T>
-int combine(int64_t n, int k, const T* V, int64_t ldv, const double* y, T* x, hipStream_t s) {
-  const dim3 blk(256), grd((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL((k_combine<T>), grd, blk, 0, s, n, k, V, ldv, y, x);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-int givens(int j, int ldh, double* H, double* cs, double* sn, double* g, double* res,
-           hipStream_t s) {
-  hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, s, j, ldh, H, cs, sn, g, res);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-int backsolve(int k, int ldh, const double* H, const double* g, double* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), 0, s, k, ldh, H, g, y);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-int store_h(const double* src, double* dst, int mode, hipStream_t s) {
-  hipLaunchKernelGGL(k_store_h, dim3(1), dim3(64), 0, s, src, dst, mode);
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-int set_g(double* g, int len, const double* beta, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_g, dim3(1), dim3(128), 0, s, g, len, beta);
+int sscale(const SegDev& sg, const double* a, int mode, const int32_t* active, const Tin* x,
+           Tout* y, hipStream_t s) {
+  hipLaunchKernelGGL((k_sscale<Tin, Tout>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, a,
+                     mode, active, x, y);
   DGMI_HIP(hipGetLastError());
   return 0;
 }
 
-template int axpy_dev<double>(int64_t, const double*, double, const double*, double*, hipStream_t);
-template int axpy_dev<float>(int64_t, const double*, double, const float*, float*, hipStream_t);
-template int scale_dev<double>(int64_t, const double*, const double*, double*, hipStream_t);
-template int scale_dev<float>(int64_t, const double*, const float*, float*, hipStream_t);
-template int cast<double, float>(int64_t, const double*, float*, double, hipStream_t);
-template int cast<float, double>(int64_t, const float*, double*, double, hipStream_t);
-template int cast<double, double>(int64_t, const double*, double*, double, hipStream_t);
-template int cast<float, float>(int64_t, const float*, float*, double, hipStream_t);
-template int add_cast<float>(int64_t, const float*, double*, const double*, hipStream_t);
-template int add_cast<double>(int64_t, const double*, double*, const double*, hipStream_t);
-template int fill<double>(int64_t, double*, double, hipStream_t);
-template int fill<float>(int64_t, float*, float, hipStream_t);
-template int combine<double>(int64_t, int, const double*, int64_t, const double*, double*, hipStream_t);
-template int combine<float>(int64_t, int, const float*, int64_t, const double*, float*, hipStream_t);
+template <typename Tin>
+int sadd(const SegDev& sg, const double* a, const Tin* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL((k_sadd<Tin>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, a, x, y);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int scombine(const SegDev& sg, const T* V, int64_t ldv, const double* yv, int64_t ys,
+             const int32_t* kk, T* x, hipStream_t s) {
+  hipLaunchKernelGGL((k_scombine<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, V, ldv, yv,
+                     ys, kk, x);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+int scycle_start(int ncomp, const double* beta, const double* eps, const int32_t* want,
+                 int32_t* active, double* thr, double* g, int64_t gs, int glen, int32_t* kk,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_scycle_start, dim3((ncomp + 63) / 64), dim3(64), 0, s, ncomp, beta, eps,
+                     want, active, thr, g, gs, glen, kk);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+int sgivens(int ncomp, int j, int ldh, int64_t hs, double* H, const double* hn, double* cs,
+            double* sn, int64_t rs, double* g, int64_t gs, double* res, const double* thr,
+            int32_t* active, int32_t* kk, int32_t* nact, hipStream_t s) {
+  hipLaunchKernelGGL(k_izero, dim3(1), dim3(64), 0, s, 1, nact);
+  hipLaunchKernelGGL(k_sgivens, dim3((ncomp + 63) / 64), dim3(64), 0, s, ncomp, j, ldh, hs, H,
+                     hn, cs, sn, rs, g, gs, res, thr, active, kk, nact);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+int sbacksolve(int ncomp, int ldh, int64_t hs, const double* H, const double* g, int64_t gs,
+               const int32_t* kk, double* yv, int64_t ys, hipStream_t s) {
+  hipLaunchKernelGGL(k_sbacksolve, dim3((ncomp + 63) / 64), dim3(64), 0, s, ncomp, ldh, hs, H, g,
+                     gs, kk, yv, ys);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template int sdot<double>(const SegDev&, const double*, const double*, double*, double*, hipStream_t);
+template int sdot<float>(const SegDev&, const float*, const float*, double*, double*, hipStream_t);
+template int sorth_pass<double>(const SegDev&, int, const double*, int64_t, double*, double*,
+                                int64_t, int, const int32_t*, hipStream_t);
+template int sorth_pass<float>(const SegDev&, int, const float*, int64_t, float*, double*,
+                               int64_t, int, const int32_t*, hipStream_t);
+template int sscale<double, double>(const SegDev&, const double*, int, const int32_t*,
+                                    const double*, double*, hipStream_t);
+template int sscale<float, float>(const SegDev&, const double*, int, const int32_t*,
+                                  const float*, float*, hipStream_t);
+template int sscale<double, float>(const SegDev&, const double*, int, const int32_t*,
+                                   const double*, float*, hipStream_t);
+template int sadd<double>(const SegDev&, const double*, const double*, double*, hipStream_t);
+template int sadd<float>(const SegDev&, const double*, const float*, double*, hipStream_t);
+template int scombine<double>(const SegDev&, const double*, int64_t, const double*, int64_t,
+                              const int32_t*, double*, hipStream_t);
+template int scombine<float>(const SegDev&, const float*, int64_t, const double*, int64_t,
+                             const int32_t*, float*, hipStream_t);
+
 }  // namespace kry
 
 }  // namespace dgmi

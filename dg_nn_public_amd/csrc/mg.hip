@@ -4,6 +4,8 @@
 // per-V-cycle argsort/permutation (z02/volume_mf_diff.py:1992-2007) is folded into the
 // transfer kernels' index lists.  The 1-D SFC levels are CSR matrices; level l+1
 // aggregates positions (2I, 2I+1) of level l.
+#include <mutex>
+
 #include "dgmi_internal.h"
 
 namespace dgmi {
@@ -173,8 +175,11 @@ __global__ __launch_bounds__(256) void k_sweep_restrict(int n, const int32_t* __
   ecn[I] = es;
 }
 
+// One level of one component for the single-work-group kernel.  Row i (0 <= i < n) has
+// entries [rp[i], rp[i+1]) of col/val; col - cbase is the row index within r/A/B.
 template <typename T> struct LevelDesc {
   int n;
+  int cbase;
   const int32_t* rp;
   const int32_t* col;
   const T* val;
@@ -184,87 +189,12 @@ template <typename T> struct LevelDesc {
   T* B;  // pong; the level's final correction after the up-cycle
 };
 
-// All levels l0..L-1 in one work-group (levels small enough that a grid launch per sweep
-// would be pure launch latency).  Same operations as the grid kernels above.
-template <typename T>
-__global__ __launch_bounds__(1024) void k_small_levels(const LevelDesc<T>* __restrict__ lvs, int l0,
-                                                       int L, T omega, int pre, int post) {
-  const int tid = threadIdx.x, nth = blockDim.x;
-  auto sweep = [&](const LevelDesc<T>& lv, const T* ein, const T* ec, T* eout) {
-    for (int i = tid; i < lv.n; i += nth) {
-      const T ax = level_row<T>(i, lv.rp, lv.col, lv.val, ein, ec);
-      T ei = ein[i];
-      if (ec) ei += ec[i >> 1];
-      eout[i] = ei + omega * (lv.r[i] - ax) / lv.diag[i];
-    }
-    __syncthreads();
-  };
-  for (int l = l0; l < L - 1; ++l) {
-    const LevelDesc<T>& lv = lvs[l];
-    const T* cur = lv.A;
-    T* nxt = lv.B;
-    for (int it = 0; it < pre; ++it) {
-      sweep(lv, cur, (const T*)nullptr, nxt);
-      T* t = const_cast<T*>(cur);
-      cur = nxt;
-      nxt = t;
-    }
-    // fused residual+sweep+restrict; result of the 4th sweep lands in lv.A
-    const LevelDesc<T>& lc = lvs[l + 1];
-    const int nc = lc.n;
-    T* fin = lv.A;
-    if (cur == lv.A) {  // need a distinct output buffer: copy cur -> B first
-      for (int i = tid; i < lv.n; i += nth) lv.B[i] = lv.A[i];
-      __syncthreads();
-      cur = lv.B;
-    }
-    for (int I = tid; I < nc; I += nth) {
-      T rs = 0, es = 0;
-      for (int h = 0; h < 2; ++h) {
-        const int i = 2 * I + h;
-        if (i < lv.n) {
-          const T rr = lv.r[i] - level_row<T>(i, lv.rp, lv.col, lv.val, cur, (const T*)nullptr);
-          const T en = cur[i] + omega * rr / lv.diag[i];
-          fin[i] = en;
-          rs += rr;
-          es += en;
-        }
-      }
-      lc.r[I] = rs;
-      lc.A[I] = es;
-    }
-    __syncthreads();
-  }
-  // coarsest level: one node, direct solve (z02/multigrid_linearelastic.py:156-160)
-  {
-    const LevelDesc<T>& lv = lvs[L - 1];
-    for (int i = tid; i < lv.n; i += nth) lv.B[i] = lv.r[i] / lv.diag[i];
-    __syncthreads();
-  }
-  for (int l = L - 2; l >= l0; --l) {
-    const LevelDesc<T>& lv = lvs[l];
-    const T* ec = lvs[l + 1].B;
-    // post sweeps: (A + P ec) -> B, B -> A, A -> B, ...
-    sweep(lv, lv.A, ec, lv.B);
-    const T* cur = lv.B;
-    T* nxt = lv.A;
-    for (int it = 1; it < post; ++it) {
-      sweep(lv, cur, (const T*)nullptr, nxt);
-      T* t = const_cast<T*>(cur);
-      cur = nxt;
-      nxt = t;
-    }
-    if (cur != lv.B) {
-      for (int i = tid; i < lv.n; i += nth) lv.B[i] = cur[i];
-      __syncthreads();
-    }
-  }
-}
-
-// Same cycle over levels l0..L-1 with the whole coarse hierarchy staged in LDS: CSR
-// (int32 rowptr/col, T values), diagonal and the r/A/B vectors of every level.  The
-// level table itself is built in LDS by thread 0.  Reads r and A of level l0 from global
-// memory and writes B of level l0 back.
+// The coarse end of the cycle (levels l0..L-1 of one component per work-group, where a
+// grid launch per sweep would be pure launch latency), with the component's coarse
+// hierarchy staged in LDS: CSR (int32 rowptr/col, T values), diagonal and the r/A/B
+// vectors of every level.  The level table itself is built in LDS by thread 0.  Reads r
+// and A of level l0 from global memory and writes B of level l0 back.  Same operations as
+// the grid kernels above.
 template <typename T> struct LdsLevel {
   int n;
   int* rp;
@@ -287,16 +217,18 @@ __host__ __device__ inline size_t lds_level_bytes(int n, int nnz) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* __restrict__ lvs,
-                                                           int l0, int L, T omega, int pre,
-                                                           int post) {
+__global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* __restrict__ descs,
+                                                           const int32_t* __restrict__ rng,
+                                                           T omega, int pre, int post) {
   extern __shared__ __align__(16) unsigned char smem[];
   LdsLevel<T>* tab = reinterpret_cast<LdsLevel<T>*>(smem);
   const int tid = threadIdx.x, nth = blockDim.x;
+  const LevelDesc<T>* lvs = descs + (size_t)blockIdx.x * MAX_LEVELS;
+  const int l0 = rng[2 * blockIdx.x], L = rng[2 * blockIdx.x + 1];
   if (tid == 0) {
     size_t off = lds_align(sizeof(LdsLevel<T>) * MAX_LEVELS);
     for (int l = l0; l < L; ++l) {
-      const int n = lvs[l].n, nnz = lvs[l].rp[n];
+      const int n = lvs[l].n, nnz = lvs[l].rp[n] - lvs[l].rp[0];
       LdsLevel<T>& t = tab[l - l0];
       t.n = n;
       t.rp = reinterpret_cast<int*>(smem + off);
@@ -319,11 +251,12 @@ __global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* _
   for (int l = l0; l < L; ++l) {
     const LevelDesc<T>& g = lvs[l];
     const LdsLevel<T>& t = tab[l - l0];
-    const int n = t.n, nnz = g.rp[n];
-    for (int i = tid; i <= n; i += nth) t.rp[i] = g.rp[i];
+    const int base = g.rp[0];
+    const int n = t.n, nnz = g.rp[n] - base;
+    for (int i = tid; i <= n; i += nth) t.rp[i] = g.rp[i] - base;
     for (int q = tid; q < nnz; q += nth) {
-      t.col[q] = g.col[q];
-      t.val[q] = g.val[q];
+      t.col[q] = g.col[base + q] - g.cbase;
+      t.val[q] = g.val[base + q];
     }
     for (int i = tid; i < n; i += nth) t.diag[i] = g.diag[i];
     if (l == l0)
@@ -414,39 +347,57 @@ __global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* _
   }
 }
 
-// LDS bytes needed to stage levels l0..L-1 (level table included)
+// LDS bytes needed to stage levels l0.. of one component (level table included)
 template <typename T>
-size_t lds_bytes_from(const SfcLevelsDev& lv, const std::vector<int64_t>& nnz, int l0) {
+size_t lds_bytes_from(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, int l0) {
   size_t b = lds_align(sizeof(LdsLevel<T>) * MAX_LEVELS);
-  for (int l = l0; l < lv.nlevel; ++l) b += lds_level_bytes<T>((int)lv.n[l], (int)nnz[l]);
+  for (size_t l = l0; l < n.size(); ++l) b += lds_level_bytes<T>((int)n[l], (int)nnz[l]);
   return b;
 }
 
-// first level index from which the remaining hierarchy fits in `budget` bytes of LDS
-int choose_small_level(const SfcLevelsDev& lv, const std::vector<int64_t>& nnz, size_t budget) {
-  const int L = lv.nlevel;
+// first level from which the remaining hierarchy fits in `budget` bytes of LDS (FP64)
+int lds_start_level(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, size_t budget) {
+  const int L = (int)n.size();
   if (L > MAX_LEVELS) return -1;
   int l0 = L - 1;
   for (int l = L - 1; l >= 0; --l) {
-    if (lds_bytes_from<double>(lv, nnz, l) <= budget) l0 = l;
+    if (lds_bytes_from<double>(n, nnz, l) <= budget) l0 = l;
     else break;
   }
   return l0;
 }
 
+// The dynamic-LDS cap is a property of the kernel, shared by every context: it only
+// ever grows, so a context set up later never lowers what another one launches with.
 template <typename T>
 int set_small_lds_attr(size_t bytes) {
+  static std::mutex mu;
+  static size_t cap = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (bytes <= cap) return 0;
   DGMI_HIP(hipFuncSetAttribute((const void*)k_small_levels_lds<T>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  cap = bytes;
   return 0;
 }
 template int set_small_lds_attr<double>(size_t);
 template int set_small_lds_attr<float>(size_t);
-template size_t lds_bytes_from<double>(const SfcLevelsDev&, const std::vector<int64_t>&, int);
-template size_t lds_bytes_from<float>(const SfcLevelsDev&, const std::vector<int64_t>&, int);
+template size_t lds_bytes_from<double>(const std::vector<int64_t>&, const std::vector<int64_t>&, int);
+template size_t lds_bytes_from<float>(const std::vector<int64_t>&, const std::vector<int64_t>&, int);
 
 // ---------------------------------------------------------------------------------------
 template <typename T> static T* vec(void* p) { return reinterpret_cast<T*>(p); }
+
+// device copy of a host vector (converted to T); nullptr on failure
+template <typename T, typename U>
+static T* upload_vec(dgmi_ctx* c, const std::vector<U>& h) {
+  std::vector<T> t(h.begin(), h.end());
+  T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(t.size(), 1)));
+  if (!d) return nullptr;
+  if (!t.empty() && hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return nullptr;
+  return d;
+}
 
 template <typename T>
 int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s) {
@@ -494,13 +445,14 @@ int launch_probe_collect(dgmi_ctx*, const double* r_sfc, const int32_t* spidx,
   return 0;
 }
 
-// One V-cycle (z02/multigrid_linearelastic.py:112-172).  Level buffers: A_l/B_l ping-pong,
-// level 0's B is the caller's output e_sfc and level 0's rhs is the caller's r_sfc.
+// One V-cycle (z02/multigrid_linearelastic.py:112-172) of every component.  Level buffers:
+// A_l/B_l ping-pong, level 0's B is the caller's output e_sfc and level 0's rhs is the
+// caller's r_sfc.  Grid levels 0..small_level-1 sweep the rows of the components that
+// start deeper; one work-group per component then runs the rest.
 template <typename T>
 int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   auto& mg = c->mg;
   auto& lv = mg.lv;
-  const int L = lv.nlevel;
   const bool f64 = std::is_same<T, double>::value;
   const T omega = (T)c->omega;
   const int l0 = mg.small_level;
@@ -510,16 +462,10 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   auto val = [&](int l) -> const T* { return f64 ? (const T*)lv.val64[l] : (const T*)lv.val32[l]; };
   auto dia = [&](int l) -> const T* { return f64 ? (const T*)lv.diag64[l] : (const T*)lv.diag32[l]; };
   const dim3 blk(256);
-  if (L == 1) {
-    hipLaunchKernelGGL((k_sweep<T>), dim3(1), blk, 0, s, 1, lv.rowptr[0], lv.col[0], val(0),
-                       dia(0), R(0), A(0), (const T*)nullptr, B(0), (T)1);  // e = r/a from e=0
-    DGMI_HIP(hipGetLastError());
-    return 0;
-  }
   DGMI_HIP(hipMemsetAsync(A(0), 0, sizeof(T) * lv.n[0], s));
   // down-cycle on grid-sized levels
-  for (int l = 0; l < l0 && l < L - 1; ++l) {
-    const int n = (int)lv.n[l];
+  for (int l = 0; l < l0; ++l) {
+    const int n = (int)lv.grid_rows[l];
     const dim3 grd((n + 255) / 256);
     T* cur = A(l);
     T* nxt = B(l);
@@ -534,26 +480,18 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
     hipLaunchKernelGGL((k_sweep_restrict<T>), grc, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l),
                        dia(l), R(l), cur, nxt, R(l + 1), A(l + 1), omega);
   }
-  // small levels in one work-group (writes B(l0)).  The device level table points level
-  // 0 at internal buffers, so when level 0 itself is small the caller's vectors are
-  // staged through them.
-  if (l0 == 0) {
+  // the level tables point level 0 at internal buffers, so components that start at
+  // level 0 see the caller's vectors through them
+  if (lv.any_start0)
     DGMI_HIP(hipMemcpyAsync(lv.r[0], r_sfc, sizeof(T) * lv.n[0], hipMemcpyDeviceToDevice, s));
-  }
-  if (mg.small_lds) {
-    const size_t bytes = lds_bytes_from<T>(lv, lv.nnz, l0);
-    hipLaunchKernelGGL((k_small_levels_lds<T>), dim3(1), dim3(1024), bytes, s,
-                       (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32), l0, L, omega, 3, 3);
-  } else {
-    hipLaunchKernelGGL((k_small_levels<T>), dim3(1), dim3(1024), 0, s,
-                       (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32), l0, L, omega, 3, 3);
-  }
-  if (l0 == 0) {
+  hipLaunchKernelGGL((k_small_levels_lds<T>), dim3(c->ncomp), dim3(1024), f64 ? lv.lds64 : lv.lds32,
+                     s, (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32), (const int32_t*)lv.rng,
+                     omega, 3, 3);
+  if (lv.any_start0)
     DGMI_HIP(hipMemcpyAsync(e_sfc, lv.e2[0], sizeof(T) * lv.n[0], hipMemcpyDeviceToDevice, s));
-  }
   // up-cycle on grid-sized levels
   for (int l = l0 - 1; l >= 0; --l) {
-    const int n = (int)lv.n[l];
+    const int n = (int)lv.grid_rows[l];
     const dim3 grd((n + 255) / 256);
     // (A + P B_{l+1}) -> B, B -> A, A -> B
     hipLaunchKernelGGL((k_sweep<T>), grd, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l), dia(l),
@@ -567,28 +505,170 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   return 0;
 }
 
-// device tables of LevelDesc for the single-work-group kernel; level 0 points at internal
-// buffers (the caller's vectors are staged through them when level 0 is small)
-int upload_level_descs(dgmi_ctx* c) {
-  auto& lv = c->mg.lv;
-  const int L = lv.nlevel;
-  std::vector<LevelDesc<double>> d64(L);
-  std::vector<LevelDesc<float>> d32(L);
-  for (int l = 0; l < L; ++l) {
-    d64[l] = {(int)lv.n[l], lv.rowptr[l], lv.col[l], lv.val64[l], lv.diag64[l],
-              (double*)lv.r[l], (double*)lv.e[l], (double*)lv.e2[l]};
-    d32[l] = {(int)lv.n[l], lv.rowptr[l], lv.col[l], lv.val32[l], lv.diag32[l],
-              (float*)lv.r[l], (float*)lv.e[l], (float*)lv.e2[l]};
-  }
-  lv.desc64 = c->alloc(sizeof(LevelDesc<double>) * L);
-  lv.desc32 = c->alloc(sizeof(LevelDesc<float>) * L);
-  if (!lv.desc64 || !lv.desc32) {
+// Device layout of every component's SFC hierarchy (see SfcLevelsDev) and the level
+// tables of the single-work-group kernel.
+int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
+  auto& mg = c->mg;
+  auto& lv = mg.lv;
+  const int nc = (int)lay.lh.size();
+  const int S = lay.smax;
+  auto fail = [&]() {
     set_error("device allocation failed");
     return -3;
+  };
+  // ---- grid levels 0..S
+  lv.nlevel = S + 1;
+  lv.n.assign(S + 1, 0);
+  lv.nnz.assign(S + 1, 0);
+  lv.grid_rows.assign(S, 0);
+  lv.rowptr.assign(S + 1, nullptr);
+  lv.col.assign(S + 1, nullptr);
+  lv.val64.assign(S + 1, nullptr);
+  lv.diag64.assign(S + 1, nullptr);
+  lv.val32.assign(S + 1, nullptr);
+  lv.diag32.assign(S + 1, nullptr);
+  lv.r.assign(S + 1, nullptr);
+  lv.e.assign(S + 1, nullptr);
+  lv.e2.assign(S + 1, nullptr);
+  std::vector<std::vector<int64_t>> segoff(S + 1, std::vector<int64_t>(nc, -1));
+  std::vector<std::vector<int64_t>> rpoff(S + 1, std::vector<int64_t>(nc, 0));
+  for (int l = 0; l <= S; ++l) {
+    std::vector<int32_t> rp(1, 0), col;
+    std::vector<double> val, diag;
+    int64_t rows = 0;
+    for (int q : lay.order) {
+      if (lay.start[q] < l) continue;
+      const SfcLevelsHost& h = lay.lh[q];
+      const int64_t seg = (q == lay.order.back()) ? h.n[l] : (lay.len0[q] >> l);
+      segoff[l][q] = rows;
+      rpoff[l][q] = (int64_t)rp.size() - 1;
+      if (l < S && lay.start[q] > l) lv.grid_rows[l] = rows + seg;
+      for (int64_t i = 0; i < seg; ++i) {
+        if (i < h.n[l]) {
+          for (int32_t k = h.rowptr[l][i]; k < h.rowptr[l][i + 1]; ++k) {
+            col.push_back((int32_t)(h.col[l][k] + rows));
+            val.push_back(h.val[l][k]);
+          }
+          diag.push_back(h.diag[l][i]);
+        } else {
+          diag.push_back(1.0);  // padding row: no entries, stays zero
+        }
+        rp.push_back((int32_t)col.size());
+      }
+      rows += seg;
+    }
+    lv.n[l] = rows;
+    lv.nnz[l] = (int64_t)col.size();
+    lv.rowptr[l] = upload_vec<int32_t>(c, rp);
+    lv.col[l] = upload_vec<int32_t>(c, col);
+    lv.val64[l] = upload_vec<double>(c, val);
+    lv.diag64[l] = upload_vec<double>(c, diag);
+    lv.val32[l] = upload_vec<float>(c, val);
+    lv.diag32[l] = upload_vec<float>(c, diag);
+    lv.r[l] = c->alloc(sizeof(double) * std::max<int64_t>(rows, 1));
+    lv.e[l] = c->alloc(sizeof(double) * std::max<int64_t>(rows, 1));
+    lv.e2[l] = c->alloc(sizeof(double) * std::max<int64_t>(rows, 1));
+    if (!lv.rowptr[l] || !lv.col[l] || !lv.val64[l] || !lv.diag64[l] || !lv.val32[l] ||
+        !lv.diag32[l] || !lv.r[l] || !lv.e[l] || !lv.e2[l])
+      return fail();
   }
-  DGMI_HIP(hipMemcpy(lv.desc64, d64.data(), sizeof(LevelDesc<double>) * L, hipMemcpyHostToDevice));
-  DGMI_HIP(hipMemcpy(lv.desc32, d32.data(), sizeof(LevelDesc<float>) * L, hipMemcpyHostToDevice));
-  return 0;
+  // ---- pooled deeper levels
+  std::vector<int32_t> prp, pcol;
+  std::vector<double> pval, pdiag;
+  struct PoolRef { int64_t rp, diag, vec; };
+  std::vector<std::vector<PoolRef>> pref(nc);
+  int64_t pvec = 0;
+  for (int q = 0; q < nc; ++q) {
+    const SfcLevelsHost& h = lay.lh[q];
+    pref[q].resize(h.nlevel);
+    for (int l = lay.start[q] + 1; l < h.nlevel; ++l) {
+      pref[q][l] = {(int64_t)prp.size(), (int64_t)pdiag.size(), pvec};
+      const int32_t base = (int32_t)pcol.size();
+      for (int64_t i = 0; i <= h.n[l]; ++i) prp.push_back(base + h.rowptr[l][i]);
+      pcol.insert(pcol.end(), h.col[l].begin(), h.col[l].end());
+      pval.insert(pval.end(), h.val[l].begin(), h.val[l].end());
+      pdiag.insert(pdiag.end(), h.diag[l].begin(), h.diag[l].end());
+      pvec += 3 * h.n[l];
+    }
+  }
+  lv.pool_rp = upload_vec<int32_t>(c, prp);
+  lv.pool_col = upload_vec<int32_t>(c, pcol);
+  lv.pool_val64 = upload_vec<double>(c, pval);
+  lv.pool_diag64 = upload_vec<double>(c, pdiag);
+  lv.pool_val32 = upload_vec<float>(c, pval);
+  lv.pool_diag32 = upload_vec<float>(c, pdiag);
+  lv.pool_vec = c->alloc(sizeof(double) * std::max<int64_t>(pvec, 1));
+  if (!lv.pool_rp || !lv.pool_col || !lv.pool_val64 || !lv.pool_diag64 || !lv.pool_val32 ||
+      !lv.pool_diag32 || !lv.pool_vec)
+    return fail();
+  // ---- level tables, components in layout order (one work-group each)
+  std::vector<LevelDesc<double>> d64((size_t)nc * MAX_LEVELS);
+  std::vector<LevelDesc<float>> d32((size_t)nc * MAX_LEVELS);
+  std::vector<int32_t> rng(2 * nc);
+  size_t lds64 = 0, lds32 = 0;
+  lv.any_start0 = false;
+  for (int b = 0; b < nc; ++b) {
+    const int q = lay.order[b];
+    const SfcLevelsHost& h = lay.lh[q];
+    const int s0 = lay.start[q];
+    if (h.nlevel > MAX_LEVELS) {
+      set_error("too many SFC levels");
+      return -2;
+    }
+    rng[2 * b] = s0;
+    rng[2 * b + 1] = h.nlevel;
+    lv.any_start0 |= s0 == 0;
+    std::vector<int64_t> nnzq(h.nlevel);
+    for (int l = 0; l < h.nlevel; ++l) nnzq[l] = (int64_t)h.col[l].size();
+    lds64 = std::max(lds64, lds_bytes_from<double>(h.n, nnzq, s0));
+    lds32 = std::max(lds32, lds_bytes_from<float>(h.n, nnzq, s0));
+    for (int l = s0; l < h.nlevel; ++l) {
+      LevelDesc<double>& a = d64[(size_t)b * MAX_LEVELS + l];
+      LevelDesc<float>& f = d32[(size_t)b * MAX_LEVELS + l];
+      a.n = f.n = (int)h.n[l];
+      if (l == s0) {
+        const int64_t so = segoff[l][q];
+        a.cbase = f.cbase = (int)so;
+        a.rp = f.rp = lv.rowptr[l] + rpoff[l][q];
+        a.col = f.col = lv.col[l];
+        a.val = lv.val64[l];
+        f.val = lv.val32[l];
+        a.diag = lv.diag64[l] + so;
+        f.diag = lv.diag32[l] + so;
+        a.r = (double*)lv.r[l] + so;
+        a.A = (double*)lv.e[l] + so;
+        a.B = (double*)lv.e2[l] + so;
+        f.r = (float*)lv.r[l] + so;
+        f.A = (float*)lv.e[l] + so;
+        f.B = (float*)lv.e2[l] + so;
+      } else {
+        const PoolRef& p = pref[q][l];
+        a.cbase = f.cbase = 0;
+        a.rp = f.rp = lv.pool_rp + p.rp;
+        a.col = f.col = lv.pool_col;
+        a.val = lv.pool_val64;
+        f.val = lv.pool_val32;
+        a.diag = lv.pool_diag64 + p.diag;
+        f.diag = lv.pool_diag32 + p.diag;
+        double* v64 = (double*)lv.pool_vec + p.vec;
+        float* v32 = (float*)lv.pool_vec + p.vec;  // FP32 views use the first half
+        a.r = v64;
+        a.A = v64 + h.n[l];
+        a.B = v64 + 2 * h.n[l];
+        f.r = v32;
+        f.A = v32 + h.n[l];
+        f.B = v32 + 2 * h.n[l];
+      }
+    }
+  }
+  lv.desc64 = upload_vec<LevelDesc<double>>(c, d64);
+  lv.desc32 = upload_vec<LevelDesc<float>>(c, d32);
+  lv.rng = upload_vec<int32_t>(c, rng);
+  if (!lv.desc64 || !lv.desc32 || !lv.rng) return fail();
+  lv.lds64 = lds64;
+  lv.lds32 = lds32;
+  if (int rc = set_small_lds_attr<double>(lds64)) return rc;
+  return set_small_lds_attr<float>(lds32);
 }
 
 template int launch_restrict<double>(dgmi_ctx*, const double*, double*, hipStream_t);

@@ -1,0 +1,74 @@
+// Row-vector helpers shared by the apply kernels: 16- or 8-byte loads/stores of one
+// element's DOF row and of rows of the per-element smoother matrices.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <type_traits>
+
+namespace dgmi {
+
+// Vector loads/stores of one element's row: 16-byte accesses when the row length
+// allows (3D P3 in both precisions, 2D in FP64), 8-byte otherwise (2D P3 in FP32).
+// DOF vectors are required to be 16-byte aligned.
+template <typename T, int NL> struct RowVec {
+  static constexpr int W = (NL % (16 / sizeof(T)) == 0) ? 16 / sizeof(T) : 8 / sizeof(T);
+  using V = typename std::conditional<
+      sizeof(T) * W == 16, typename std::conditional<sizeof(T) == 8, double2, float4>::type,
+      typename std::conditional<sizeof(T) == 8, double, float2>::type>::type;
+  static_assert(NL % W == 0, "row length not a multiple of the vector width");
+};
+
+template <typename T, int NL>
+__device__ __forceinline__ void load_row(const T* __restrict__ p, T* v) {
+  using RV = RowVec<T, NL>;
+  constexpr int W = RV::W;
+  const typename RV::V* q = reinterpret_cast<const typename RV::V*>(p);
+#pragma unroll
+  for (int i = 0; i < NL / W; ++i) {
+    typename RV::V t = q[i];
+    const T* tt = reinterpret_cast<const T*>(&t);
+#pragma unroll
+    for (int k = 0; k < W; ++k) v[i * W + k] = tt[k];
+  }
+}
+template <typename T, int NL>
+__device__ __forceinline__ void store_row(T* __restrict__ p, const T* v) {
+  using RV = RowVec<T, NL>;
+  constexpr int W = RV::W;
+  typename RV::V* q = reinterpret_cast<typename RV::V*>(p);
+#pragma unroll
+  for (int i = 0; i < NL / W; ++i) {
+    typename RV::V t;
+    T* tt = reinterpret_cast<T*>(&t);
+#pragma unroll
+    for (int k = 0; k < W; ++k) tt[k] = v[i * W + k];
+    q[i] = t;
+  }
+}
+
+// Smoother matrices are stored as structure-of-vectors: entry S[n][m] of element e lives in
+// vector ((n*(NL/W) + m/W)*nele + e), component m%W, with W the row vector width above, so
+// one wave-wide load moves 64 x 16 bytes.
+template <typename T, int NL>
+__host__ __device__ inline size_t s_index(int n, int m, int e, int nele) {
+  constexpr int W = RowVec<T, NL>::W;
+  return ((size_t)(n * (NL / W) + m / W) * nele + e) * W + (m % W);
+}
+
+// row n of S for element e
+template <typename T, int NL>
+__device__ __forceinline__ void load_S_row(const T* __restrict__ S, int n, int e, int nele, T* v) {
+  using RV = RowVec<T, NL>;
+  constexpr int W = RV::W;
+  const typename RV::V* q = reinterpret_cast<const typename RV::V*>(S);
+#pragma unroll
+  for (int i = 0; i < NL / W; ++i) {
+    typename RV::V t = q[(size_t)(n * (NL / W) + i) * nele + e];
+    const T* tt = reinterpret_cast<const T*>(&t);
+#pragma unroll
+    for (int k = 0; k < W; ++k) v[i * W + k] = tt[k];
+  }
+}
+
+}  // namespace dgmi

@@ -38,6 +38,11 @@ GMRES_M = 30        # restart length of the workloads (the reference config uses
 GMRES_ITS = 400     # z02/config.py:150
 
 _DTYPES = {torch.float64: _lib.DGMI_F64, torch.float32: _lib.DGMI_F32}
+# inner precision of the iterative-refinement solve; bfloat16 means FP32 Krylov vectors with
+# bf16-input / fp32-accumulate matrix-core applies
+_INNER = {torch.float64: _lib.DGMI_F64, torch.float32: _lib.DGMI_F32,
+          torch.bfloat16: _lib.DGMI_BF16}
+F32_KERNELS = {"vector": 0, "mfma": 1, "mfma_bf16": 2}
 
 
 def _ptr(t):
@@ -53,10 +58,13 @@ class DGDiffusion:
 
     def __init__(self, x_ref_in, cells, verts, kappa=None, bctype=None, eta_e: float = ETA_E,
                  blk_solver: str = "jacobian", jac_wei: float = JAC_WEI, device="cuda:0",
-                 face_kappa: str = "mean"):
+                 face_kappa: str = "mean", components=None):
         """face_kappa: "mean" is the reference's interior-face diffusivity (kE + kN)/2;
         "harmonic" uses 2 kE kN/(kE + kN) (symmetric weighted interior penalty), which
-        keeps the operator positive definite at high contrast."""
+        keeps the operator positive definite at high contrast.  ``components`` =
+        (elem_off, vert_off) packs independent solves into this one mesh (see
+        ``dg_nn_public_amd.batch``): each keeps its own multigrid hierarchy, Krylov space and
+        convergence test, while every kernel launch covers all of them."""
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -88,7 +96,22 @@ class DGDiffusion:
             bctype=bc.ctypes.data_as(_lib._I64P) if bc is not None else None, eta_e=eta_e,
             face_kappa={"mean": 0, "harmonic": 1}[face_kappa])
         ctx = C.c_void_p()
-        check(self.lib.dgmi_ctx_create(C.byref(desc), C.byref(ctx)), "dgmi_ctx_create")
+        self.ncomp = 1
+        if components is None:
+            check(self.lib.dgmi_ctx_create(C.byref(desc), C.byref(ctx)), "dgmi_ctx_create")
+            self.elem_off = np.array([0, nele], dtype=np.int64)
+            self.vert_off = np.array([0, verts.shape[0]], dtype=np.int64)
+        else:
+            eo = np.ascontiguousarray(components[0], dtype=np.int64)
+            vo = np.ascontiguousarray(components[1], dtype=np.int64)
+            if eo.ndim != 1 or eo.shape != vo.shape or eo.size < 2:
+                raise ValueError("components must be two offset arrays of equal length >= 2")
+            self.ncomp = eo.size - 1
+            self.elem_off, self.vert_off = eo, vo
+            check(self.lib.dgmi_ctx_create_batch(C.byref(desc), self.ncomp,
+                                                 eo.ctypes.data_as(_lib._I64P),
+                                                 vo.ctypes.data_as(_lib._I64P), C.byref(ctx)),
+                  "dgmi_ctx_create_batch")
         self.ctx = ctx
         info = (C.c_int64 * 8)()
         check(self.lib.dgmi_ctx_info(ctx, info), "dgmi_ctx_info")
@@ -157,10 +180,23 @@ class DGDiffusion:
         return arrs
 
     # ---------------------------------------------------------------- operator
-    def apply(self, x, b=None, out=None):
-        """out = b - A x (b None: out = -A x) in the precision of x."""
+    def set_f32_kernel(self, name):
+        """Kernel family of FP32 applies and smooths: "vector" (FMA), "mfma" (fp32 matrix
+        cores, the default) or "mfma_bf16" (bf16 inputs, fp32 accumulation)."""
+        if name not in F32_KERNELS:
+            raise ValueError(f"f32 kernel must be one of {sorted(F32_KERNELS)}")
+        check(self.lib.dgmi_set_param(self.ctx, b"f32_kernel", float(F32_KERNELS[name])),
+              "dgmi_set_param")
+
+    def apply(self, x, b=None, out=None, bf16=False):
+        """out = b - A x (b None: out = -A x) in the precision of x.  ``bf16`` (FP32 x only)
+        rounds the contraction inputs to bf16 and accumulates in fp32."""
         self._vec(x, "x")
         dt = _DTYPES[x.dtype]
+        if bf16:
+            if x.dtype != torch.float32:
+                raise ValueError("the bf16-input apply takes float32 vectors")
+            dt = _lib.DGMI_BF16
         if b is not None:
             self._vec(b, "b", (x.dtype,))
         out = torch.empty_like(x) if out is None else self._vec(out, "out", (x.dtype,))
@@ -274,10 +310,12 @@ class DGDiffusion:
                          inner=torch.float64, orth="cgs2"):
         """Left-preconditioned GMRES(m).  The reference stops on the absolute residual
         ``tol``; pass ``rtol`` for ||b - A x|| <= rtol ||b||.  ``inner`` float32 runs the
-        Krylov loop in FP32 inside FP64 iterative refinement.  ``orth`` "mgs" is the
+        Krylov loop in FP32 inside FP64 iterative refinement; bfloat16 does the same with
+        bf16-input matrix-core applies.  ``orth`` "mgs" is the
         reference's modified Gram-Schmidt, "cgs2" fused classical Gram-Schmidt with
         re-orthogonalisation.  Returns (x_i, its) and keeps the statistics in
-        ``self.last_stats``."""
+        ``self.last_stats`` (for a packed batch a list of per-component dicts, and ``its``
+        is the largest count)."""
         self._need_mg()
         if orth not in ("mgs", "cgs2"):
             raise ValueError("orth must be 'mgs' or 'cgs2'")
@@ -290,12 +328,13 @@ class DGDiffusion:
                 raise ValueError("give tol (absolute) or rtol (relative)")
             bn = float(torch.linalg.vector_norm(b))
             rtol = tol / bn if bn > 0 else 0.0
-        stats = (C.c_double * 4)()
-        check(self.lib.dgmi_gmres_solve(self.ctx, _DTYPES[inner], _ptr(b), _ptr(x_i), rtol, m,
+        stats = (C.c_double * (4 * self.ncomp))()
+        check(self.lib.dgmi_gmres_solve(self.ctx, _INNER[inner], _ptr(b), _ptr(x_i), rtol, m,
                                         max_its, stats, _stream()), "dgmi_gmres_solve")
-        self.last_stats = dict(its=int(stats[0]), relres=stats[1], outer=int(stats[2]),
-                               cycles=int(stats[3]))
-        return x_i, int(stats[0])
+        per = [dict(its=int(stats[4 * q]), relres=stats[4 * q + 1], outer=int(stats[4 * q + 2]),
+                    cycles=int(stats[4 * q + 3])) for q in range(self.ncomp)]
+        self.last_stats = per[0] if self.ncomp == 1 else per
+        return x_i, max(p["its"] for p in per)
 
     def _mg_solver(self, x_i, x_rhs, tol=None, rtol=None, max_its=GMRES_ITS):
         """Stand-alone V-cycle iteration (z02/volume_mf_diff.py:1844-1873)."""

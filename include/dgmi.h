@@ -54,6 +54,16 @@ typedef struct {
  * (z02/shape_function.py:459-1317), affine geometry, uploaded to the current device. */
 int dgmi_ctx_create(const dgmi_mesh_desc* desc, dgmi_ctx** out);
 int dgmi_ctx_destroy(dgmi_ctx* ctx);
+/* A context holding ncomp independent solves ("components") packed into one mesh: the
+ * union of their cells and vertices, component c owning elements [elem_off[c],
+ * elem_off[c+1]) and vertices [vert_off[c], vert_off[c+1]).  Operator applies, smoothing
+ * and every V-cycle level run as one launch over all components; each component keeps its
+ * own SFC hierarchy and Krylov space, and dgmi_gmres_solve converges each to its own
+ * tolerance (stats then holds 4 values per component).  No counterpart in the reference,
+ * which solves one mesh per process; it is the packed-launch entry of the batch and
+ * serving workloads. */
+int dgmi_ctx_create_batch(const dgmi_mesh_desc* desc, int32_t ncomp, const int64_t* elem_off,
+                          const int64_t* vert_off, dgmi_ctx** out);
 /* info[0..7] = nele, nloc, ndof, nface, ngi, sngi, nverts, device bytes held */
 int dgmi_ctx_info(const dgmi_ctx* ctx, int64_t* info);
 /* host copies of nbf, alnmt, glb_bcface_type (each nele*nface) */
@@ -112,16 +122,23 @@ int dgmi_precond(dgmi_ctx* ctx, int dtype, const void* b, void* x, void* stream)
 /* Left-preconditioned restarted GMRES(m) for A x = b on device vectors (FP64 x, b)
  * -- _gmres_mg_solver (z02/volume_mf_diff.py:1757-1841).  inner_dtype selects the
  * arithmetic of the operator applies and V-cycles inside the Krylov loop; with
- * DGMI_F32/DGMI_BF16 the solve runs as FP64 iterative refinement around the inner
- * GMRES.  Stops when ||b - A x|| <= rtol ||b|| (FP64 true residual) or after max_its
- * inner iterations.  Synchronises the stream.  stats (may be NULL): [0] iterations,
- * [1] final ||r||/||b||, [2] outer refinement steps, [3] restarts. */
+ * DGMI_F32 the solve runs as FP64 iterative refinement around FP32 cycles (flexible
+ * GMRES unless "inner_flexible" is 0), with DGMI_BF16 likewise with bf16-input applies
+ * inside the preconditioner.  Stops when ||b - A x|| <= rtol ||b|| (FP64 true residual)
+ * or after max_its inner iterations, per component.  Synchronises the stream.  stats
+ * (may be NULL), 4 values per component: iterations, final ||r||/||b||, outer
+ * refinement steps, restarts. */
 int dgmi_gmres_solve(dgmi_ctx* ctx, int inner_dtype, const double* b, double* x, double rtol,
                      int m, int max_its, double* stats, void* stream);
 
 /* Solver parameters by name:
- *   "gmres_orth"  0 = modified Gram-Schmidt, as the reference (z02/volume_mf_diff.py:1813-1816);
- *                 1 = classical Gram-Schmidt with one re-orthogonalisation (default). */
+ *   "gmres_orth"      0 = modified Gram-Schmidt, as the reference
+ *                     (z02/volume_mf_diff.py:1813-1816); 1 = classical Gram-Schmidt with
+ *                     one re-orthogonalisation (default).
+ *   "inner_flexible"  FP32/BF16 refinement cycles: 1 = flexible (right-preconditioned)
+ *                     GMRES (default), 0 = left-preconditioned like the FP64 solve.
+ *   "f32_kernel"      kernels of FP32 applies: 0 = vector FMA, 1 = MFMA fp32 (default),
+ *                     2 = MFMA with bf16 inputs. */
 int dgmi_set_param(dgmi_ctx* ctx, const char* name, double value);
 
 /* Krylov vector primitives on device vectors of n FP64 or FP32 values. */

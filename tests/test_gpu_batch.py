@@ -1,0 +1,93 @@
+"""Packed batches of independent solves: every component's preconditioner and solve
+equal the ones it gets alone (same inputs, same kernels), and each converges to its own
+tolerance."""
+import numpy as np
+import pytest
+import torch
+
+from dg_nn_public_amd import meshgen
+from dg_nn_public_amd.batch import DGBatch, pack
+from dg_nn_public_amd.diffusion import DGDiffusion
+from gpu_common import rel, to_dev, to_np
+from oracle import dg
+
+pytestmark = pytest.mark.gpu
+
+# (D, n, mesh seed, sigma, kappa seed): uneven sizes, different diffusivities
+MIX3 = [(3, 3, 0, 1.0, 1), (3, 5, 1, 0.5, 2), (3, 2, 2, 2.0, 3), (3, 4, 3, 1.5, 4)]
+MIX2 = [(2, 6, 0, 0.0, 1), (2, 11, 1, 1.0, 2), (2, 4, 2, 2.0, 3)]
+
+
+def _problems(mix):
+    out = []
+    for D, n, ms, sig, ks in mix:
+        m = meshgen.square_mesh(n, seed=ms) if D == 2 else meshgen.cube_mesh(n, seed=ms)
+        out.append(meshgen.sine_problem(m, sigma=sig, kappa_seed=ks))
+    return out
+
+
+@pytest.fixture(scope="module", params=[MIX3, MIX2], ids=["3d", "2d"])
+def batch(request):
+    pbs = _problems(request.param)
+    print(f"[case] batch of {len(pbs)} D={pbs[0].mesh.D}", flush=True)
+    bt = DGBatch([p.mesh for p in pbs], [p.kappa for p in pbs])
+    alone = []
+    for p in pbs:
+        g = DGDiffusion.from_mesh(p.mesh, p.kappa, face_kappa="harmonic")
+        g._get_RAR_and_sfc_data_Um()
+        alone.append(g)
+    return pbs, bt, alone
+
+
+def test_pack_offsets_and_renumbering():
+    pbs = _problems(MIX2)
+    p = pack([q.mesh for q in pbs])
+    assert p.elem_off[-1] == sum(q.mesh.nele for q in pbs)
+    for c, q in enumerate(pbs):
+        cells = p.cells[p.elem_off[c]:p.elem_off[c + 1]]
+        assert cells.min() >= p.vert_off[c] and cells.max() < p.vert_off[c + 1]
+        assert np.array_equal(cells - p.vert_off[c], q.mesh.cells)
+
+
+def test_batch_apply_equals_components(batch):
+    pbs, bt, alone = batch
+    rng = np.random.default_rng(5)
+    xs = [rng.standard_normal(p.mesh.nele * p.mesh.nloc) for p in pbs]
+    y = bt.space.apply(to_dev(np.concatenate(xs)))
+    for c, (p, part) in enumerate(zip(pbs, bt.split(y))):
+        assert np.array_equal(to_np(part), to_np(alone[c].apply(to_dev(xs[c]))))
+
+
+def test_batch_preconditioner_equals_components(batch):
+    pbs, bt, alone = batch
+    rng = np.random.default_rng(6)
+    rs = [rng.standard_normal(p.mesh.nele * p.mesh.nloc) for p in pbs]
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        x = torch.zeros(bt.space.ndof, dtype=dt, device="cuda:0")
+        bt.space._um_left_precond(x, to_dev(np.concatenate(rs), dt))
+        for c, part in enumerate(bt.split(x)):
+            xa = torch.zeros(alone[c].ndof, dtype=dt, device="cuda:0")
+            alone[c]._um_left_precond(xa, to_dev(rs[c], dt))
+            err = rel(to_np(part), to_np(xa))
+            print(dt, c, "precond batch vs alone", err, flush=True)
+            assert err < tol
+
+
+@pytest.mark.parametrize("inner", [torch.float64, torch.float32], ids=["f64", "f32-ir"])
+def test_batch_solve_equals_alone_and_converges(batch, inner):
+    pbs, bt, alone = batch
+    b = bt.rhs([p.f for p in pbs], [p.u_bc for p in pbs])
+    x, st = bt.solve(b, rtol=1e-10, inner=inner)
+    for c, (p, part) in enumerate(zip(pbs, bt.split(x))):
+        sp = dg.make_space(p.mesh, p.kappa, face_kappa="harmonic")
+        bc = dg.get_rhs(sp, p.f, p.u_bc)
+        xs = to_np(part)
+        relres = np.linalg.norm(bc - dg.apply_A(sp, xs)) / np.linalg.norm(bc)
+        xa, _ = alone[c]._gmres_mg_solver(alone[c].new_vector(), to_dev(bc), rtol=1e-10,
+                                          inner=inner)
+        sa = alone[c].last_stats
+        print(inner, c, st[c], "alone", sa, "oracle relres", relres,
+              "vs alone", rel(xs, to_np(xa)), flush=True)
+        assert st[c]["relres"] <= 1e-10 and relres <= 2e-10
+        assert st[c]["its"] == sa["its"]
+        assert rel(xs, to_np(xa)) < 1e-9

@@ -142,3 +142,62 @@ def test_full_size_3d_apply_properties():
     r = to_np(g1.apply(to_dev(u))).reshape(m1.nele, -1)
     hasb = (sp1.bctype.reshape(m1.nele, -1) == 0).any(1)
     assert np.abs(r[~hasb]).max() < 1e-10
+
+
+# ---- matrix-core kernels (FP32 and bf16-input) ------------------------------------------
+def test_f32_vector_and_mfma_kernels_agree_with_oracle(prob):
+    m, pb, g, sp = prob
+    rng = np.random.default_rng(21)
+    x, b = rng.standard_normal((2, sp.ndof))
+    ref = b - dg.apply_A(sp, x)
+    errs = {}
+    for k in ("vector", "mfma"):
+        g.set_f32_kernel(k)
+        errs[k] = rel(to_np(g.apply(to_dev(x, torch.float32), to_dev(b, torch.float32))), ref)
+        pos = to_np(g.apply(to_dev(x, torch.float32)))
+        assert rel(-pos, dg.apply_A(sp, x)) < 1e-6
+    g.set_f32_kernel("mfma")
+    print("f32 rel err", errs, flush=True)
+    assert errs["vector"] < 1e-6 and errs["mfma"] < 1e-6
+
+
+def test_bf16_input_apply_error_is_bf16_sized(prob):
+    """bf16 inputs (8-bit significand) with fp32 accumulation: the relative error of A x
+    for random x sits near the bf16 unit roundoff 2^-8, far above fp32's."""
+    m, pb, g, sp = prob
+    x = np.random.default_rng(22).standard_normal(sp.ndof)
+    ref = dg.apply_A(sp, x)
+    y = -to_np(g.apply(to_dev(x, torch.float32), bf16=True))
+    err = rel(y, ref)
+    print("bf16 apply rel err", err, flush=True)
+    assert 1e-5 < err < 2e-2
+
+
+@pytest.mark.parametrize("kernel", ["vector", "mfma", "mfma_bf16"])
+def test_f32_smooth_kernels_match_oracle(prob, kernel):
+    m, pb, g, sp = prob
+    _, B = dg.get_bdiag_diag(sp)
+    rng = np.random.default_rng(23)
+    x, b = rng.standard_normal((2, sp.ndof))
+    r_ref = np.zeros(sp.ndof)
+    x_ref = x.copy()
+    dg.get_residual_or_smooth(sp, r_ref, x_ref, b, True, B, "jacobian")
+    g.set_f32_kernel(kernel)
+    try:
+        r32 = g.new_vector(torch.float32)
+        x32 = to_dev(x, torch.float32)
+        g.get_residual_or_smooth(r32, x32, to_dev(b, torch.float32), do_smooth=True)
+        tol = 2e-2 if kernel == "mfma_bf16" else 1e-5
+        ex, er = rel(to_np(x32), x_ref), rel(to_np(r32), r_ref)
+        print(kernel, "smooth rel err x", ex, "r", er, flush=True)
+        assert ex < tol and er < tol
+    finally:
+        g.set_f32_kernel("mfma")
+
+
+def test_unknown_f32_kernel_raises(prob):
+    m, pb, g, sp = prob
+    with pytest.raises(ValueError):
+        g.set_f32_kernel("tensor")
+    with pytest.raises(ValueError):
+        g.apply(to_dev(np.zeros(sp.ndof)), bf16=True)  # bf16 inputs need float32 vectors

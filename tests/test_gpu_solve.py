@@ -22,7 +22,8 @@ def prob(request):
 
 
 @pytest.mark.parametrize("orth", ["cgs2", "mgs"])
-@pytest.mark.parametrize("inner", [torch.float64, torch.float32], ids=["f64", "f32-ir"])
+@pytest.mark.parametrize("inner", [torch.float64, torch.float32, torch.bfloat16],
+                         ids=["f64", "f32-ir", "bf16-ir"])
 def test_gmres_reaches_tolerance_and_matches_oracle(prob, inner, orth):
     m, pb, g, sp, b, x_ref = prob
     x = g.new_vector()
@@ -35,7 +36,8 @@ def test_gmres_reaches_tolerance_and_matches_oracle(prob, inner, orth):
     assert rel(xs, x_ref) < 1e-7
 
 
-@pytest.mark.parametrize("inner", [torch.float64, torch.float32], ids=["f64", "f32-ir"])
+@pytest.mark.parametrize("inner", [torch.float64, torch.float32, torch.bfloat16],
+                         ids=["f64", "f32-ir", "bf16-ir"])
 def test_high_contrast_harmonic_solve_converges(inner):
     """sigma = 2 lognormal diffusivity with the harmonic face coefficient: both solvers
     reach 1e-10 and agree with the oracle's solve."""
