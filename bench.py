@@ -16,6 +16,9 @@ Other workloads (``--workload``), each printing its own JSON line:
              bf16-input applies against FP64, iteration counts of FP64 / FP32 / bf16 solves
   batch      256 independent solves (half 2D, half 3D, 2e3..2e5 elements log-uniform,
              sigma 0.5..2) packed per dimension into one context each
+  serve      Poisson arrivals of solve requests (the batch size mix plus 5 % 3D jobs of
+             55^3 x 6 = 998,250 tets) at several rates, continuous batching under an
+             HBM budget; p50/p99 latency and solves/s, FP32-IR and FP64
 """
 from __future__ import annotations
 
@@ -59,9 +62,16 @@ def main():
     ap.add_argument("--apply-only", action="store_true",
                     help="only operator applies (FP64, FP32) and a 256 MiB elementwise copy "
                          "that calibrates HBM byte counters; for rocprofv3 --pmc passes")
+    ap.add_argument("--smoother", default="direct", choices=["direct", "jacobian"],
+                    help="block smoother: exact block inverse (the reference's variable-"
+                         "diffusivity solver) or 3 Jacobi sweeps (its constant-diffusivity one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
-    ap.add_argument("--workload", default="solve", choices=["solve", "precision", "batch"])
+    ap.add_argument("--workload", default="solve",
+                    choices=["solve", "precision", "batch", "serve"])
+    ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates, req/s")
+    ap.add_argument("--nreq", type=int, default=48, help="serve: requests per rate")
+    ap.add_argument("--large-n", type=int, default=55, help="serve: hex cells of large jobs")
     ap.add_argument("--nsolves", type=int, default=256, help="batch workload size")
     ap.add_argument("--emax", type=float, default=2e5, help="batch: largest element count")
     args = ap.parse_args()
@@ -73,6 +83,8 @@ def main():
         return run_precision(args)
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "serve":
+        return run_serve(args)
 
     import numpy as np
     import torch
@@ -84,7 +96,8 @@ def main():
     t0 = time.time()
     mesh = meshgen.cube_mesh(args.n, jitter=0.15, seed=0)
     pb = meshgen.sine_problem(mesh, sigma=args.sigma, kappa_seed=1)
-    g = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa=args.face_kappa)
+    g = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa=args.face_kappa,
+                              blk_solver=args.smoother)
     g._get_RAR_and_sfc_data_Um()
     torch.cuda.synchronize()
     setup_s = time.time() - t0
@@ -163,7 +176,8 @@ def main():
     # the reference's arithmetic-mean face diffusivity on the same problem, 60 iterations
     ref_form = None
     if args.face_kappa != "mean" and args.sigma > 0:
-        gm = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa="mean")
+        gm = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa="mean",
+                                   blk_solver=args.smoother)
         gm._get_RAR_and_sfc_data_Um()
         bm = gm._get_rhs(gm.new_vector(), pb.f, pb.u_bc)
         gm._gmres_mg_solver(gm.new_vector(), bm, rtol=1e-10, max_its=60)
@@ -224,7 +238,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3D IP-DG P3 tets, {args.n}^3x6 = {nele} elements, "
                                f"{nele*nloc} DOFs, lognormal kappa sigma={args.sigma}, "
-                               f"{args.face_kappa}-mean face kappa, GMRES(30)+SFC-MG, FP64, "
+                               f"{args.face_kappa}-mean face kappa, {args.smoother} block "
+                               "smoother, GMRES(30)+SFC-MG, FP64, "
                                f"rtol 1e-10",
                    "global_batch": 1, "parallelism": "single GPU"},
         "latency_s": {"p50": float(np.percentile(lat64, 50)), "p99": float(np.percentile(lat64, 99))},
@@ -288,7 +303,7 @@ def run_precision(args):
     t0 = time.time()
     mesh = meshgen.cube_mesh(n, jitter=0.15, seed=0)
     pb = meshgen.sine_problem(mesh, sigma=sigma, kappa_seed=1)
-    g = DGDiffusion.from_mesh(mesh, pb.kappa, face_kappa="harmonic")
+    g = DGDiffusion.from_mesh(mesh, pb.kappa, face_kappa="harmonic", blk_solver=args.smoother)
     g._get_RAR_and_sfc_data_Um()
     b = g._get_rhs(g.new_vector(), pb.f, pb.u_bc)
     torch.cuda.synchronize()
@@ -331,7 +346,7 @@ def run_precision(args):
         "metric": "operator rel. L2 error vs FP64 and iteration counts (precision workload)",
         "config": {"workload": f"3D IP-DG P3 tets, {n}^3x6 = {g.nele} elements, {nd} DOFs, "
                                f"lognormal kappa sigma={sigma}, harmonic face kappa, "
-                               "GMRES(30)+SFC-MG, rtol 1e-10"},
+                               f"{args.smoother} block smoother, GMRES(30)+SFC-MG, rtol 1e-10"},
         "operator_rel_err": errs,
         "solves": solves,
         "iteration_change_vs_f64": {k: v["its"] - solves["f64"]["its"] for k, v in solves.items()},
@@ -381,7 +396,7 @@ def run_batch(args):
                 meshgen.cube_mesh(r["n"], seed=r["mesh_seed"])
             pbs.append(meshgen.sine_problem(m, sigma=r["sigma"], kappa_seed=r["kappa_seed"]))
         tm = time.time() - t0
-        bt = DGBatch([p.mesh for p in pbs], [p.kappa for p in pbs])
+        bt = DGBatch([p.mesh for p in pbs], [p.kappa for p in pbs], blk_solver=args.smoother)
         b = bt.rhs([p.f for p in pbs], [p.u_bc for p in pbs])
         torch.cuda.synchronize()
         ts = time.time() - t0 - tm
@@ -415,6 +430,60 @@ def run_batch(args):
                                "packed per dimension, FP32 inner + FP64 refinement"},
         "fp64_solves_per_s": sps["f64"], "speedup_vs_fp64": sps["f32_ir"] / sps["f64"],
         "setup_s_total": t_setup, "groups": out["groups"],
+    }))
+
+
+def run_serve(args):
+    """Poisson request streams at several rates, served by the continuous-batching loop;
+    FP32-IR and FP64 inner solves on the same streams."""
+    import numpy as np
+    import torch
+
+    from dg_nn_public_amd import meshgen
+    from dg_nn_public_amd.serve import Request, Server, summarize
+
+    rng = np.random.default_rng(7)
+    specs = _batch_requests(args.nreq, args.emax, seed=11)
+    rng.shuffle(specs)
+    nlarge = max(1, int(round(0.05 * args.nreq)))
+    for i in range(nlarge):  # 5 % large 3D jobs
+        specs[i * (len(specs) // nlarge)] = dict(D=3, n=args.large_n, mesh_seed=5000 + i,
+                                                 kappa_seed=6000 + i, sigma=1.0)
+    t0 = time.time()
+    payload = []
+    for i, r in enumerate(specs):
+        m = meshgen.square_mesh(r["n"], seed=r["mesh_seed"]) if r["D"] == 2 else \
+            meshgen.cube_mesh(r["n"], seed=r["mesh_seed"])
+        sig = float(rng.uniform(0.5, 2.0))
+        amp = float(rng.uniform(0.5, 2.0))
+        p = meshgen.sine_problem(m, sigma=sig, kappa_seed=r["kappa_seed"], amplitude=amp)
+        payload.append((m, p))
+        if i % 8 == 7:
+            print(f"[serve] generated {i + 1}/{len(specs)} request meshes", file=sys.stderr, flush=True)
+    print(f"[serve] {len(payload)} requests, {sum(m.nele for m, _ in payload)} elements, "
+          f"meshes in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    rates = [float(v) for v in args.rates.split(",")]
+    out = {}
+    for lam in rates:
+        ar = np.cumsum(np.random.default_rng(int(lam)).exponential(1.0 / lam, size=len(payload)))
+        reqs = [Request(i, m, p.kappa, p.f, p.u_bc, float(a))
+                for i, ((m, p), a) in enumerate(zip(payload, ar))]
+        row = {}
+        for name, inner in (("f32_ir", torch.float32), ("f64", torch.float64)):
+            res = Server(hbm_budget=200e9, max_jobs=4, max_batch=64, inner=inner).run(reqs)
+            row[name] = summarize(res)
+            print(f"[serve] rate {lam:g}/s {name}: {row[name]}", file=sys.stderr, flush=True)
+        row["speedup_vs_fp64"] = row["f32_ir"]["solves_per_s"] / row["f64"]["solves_per_s"]
+        out[f"{lam:g}"] = row
+    best = max(out.values(), key=lambda r: r["f32_ir"]["solves_per_s"])
+    print(json.dumps({
+        "metric": "solves/s + p50/p99 latency at rtol 1e-10 (serving workload)",
+        "value": best["f32_ir"]["solves_per_s"], "unit": "solves/s", "higher_is_better": True,
+        "config": {"workload": f"Poisson arrivals, {len(payload)} requests per rate: half 2D "
+                               f"P3 tris / half 3D P3 tets 2e3..{args.emax:.0e} elements, "
+                               f"{nlarge} jobs of {args.large_n}^3x6 tets, sigma and source "
+                               "amplitude U[0.5,2], 200 GB HBM budget, 4 job slots"},
+        "rates": out,
     }))
 
 

@@ -57,6 +57,7 @@ SIGNATURES = {
     "dgmi_precond": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "dgmi_gmres_solve": (C.c_int, [_P, C.c_int, _P, _P, C.c_double, C.c_int, C.c_int, _DP, _P]),
     "dgmi_set_param": (C.c_int, [_P, C.c_char_p, C.c_double]),
+    "dgmi_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32), _DP, _P]),
     "dgmi_dot": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, _P]),
     "dgmi_axpy": (C.c_int, [C.c_int, C.c_int64, C.c_double, _P, _P, _P]),
     "dgmi_last_error": (C.c_char_p, []),

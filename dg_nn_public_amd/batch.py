@@ -50,10 +50,12 @@ def pack(meshes) -> Packed:
 
 
 class DGBatch:
-    """Independent P3 DG diffusion solves of one dimension packed into one context."""
+    """Independent P3 DG diffusion solves of one dimension packed into one context.
+    The default smoother is the exact block inverse that the reference's
+    variable-diffusivity solver uses (z05/volume_mf_um_diff_varprop.py:384-392)."""
 
     def __init__(self, meshes, kappas, device="cuda:0", face_kappa="harmonic",
-                 blk_solver="jacobian"):
+                 blk_solver="direct", graph_vcycle=True):
         p = pack(meshes)
         kappa = np.concatenate([np.asarray(k, dtype=np.float64).reshape(-1) for k in kappas])
         self.space = DGDiffusion(p.x_ref_in, p.cells, p.verts, kappa, device=device,
@@ -63,6 +65,8 @@ class DGBatch:
         self.ncomp = len(meshes)
         self.nloc = self.space.nloc
         self.dof_off = p.elem_off * self.nloc
+        if not graph_vcycle:  # launch the V-cycle kernels one by one (no stream capture)
+            self.space.lib.dgmi_set_param(self.space.ctx, b"graph_vcycle", 0.0)
         self.space._get_RAR_and_sfc_data_Um()
 
     def rhs(self, fs, u_bcs):

@@ -5,6 +5,8 @@
 //   geometry (SoA)     G[c*nele + e], cf[(f*D+d)*nele + e], wf/pen[f*nele + e]
 //   face records       nbface[e*nface + f] = N*nface + f' (or -1), fcode = type | alnmt<<4
 //   face traces (SoA)  trace[(f*sngi + g)*nele + e] = (u, grad u . n_f) at face point g
+//                      (lanes are elements here; the matrix-core kernels, whose lanes
+//                      hold points, use the element-major layout instead)
 //   smoother (SoA)     S[(n*nloc + m)*nele + e]
 //
 // The apply is two launches:
@@ -33,8 +35,10 @@ template <typename T, int D>
 __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __restrict__ x,
                                                           const T* __restrict__ ftab,
                                                           const T* __restrict__ cf,
-                                                          typename Vec2<T>::type* __restrict__ trace) {
+                                                          typename Vec2<T>::type* __restrict__ trace,
+                                                          const int32_t* __restrict__ act) {
   using M = Dm<D>;
+  if (act && !act[blockIdx.x]) return;  // 64 elements of finished solves
   const int lane = threadIdx.x & 63;
   const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int e = blockIdx.x * 64 + lane;
@@ -74,8 +78,9 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
     const T* __restrict__ wf, const T* __restrict__ pen, const int32_t* __restrict__ nbface,
     const int32_t* __restrict__ fcode, const typename Vec2<T>::type* __restrict__ trace,
-    const T* __restrict__ S) {
+    const T* __restrict__ S, const int32_t* __restrict__ act) {
   using M = Dm<D>;
+  if (act && !act[blockIdx.x]) return;  // whole block: no barrier is reached
   constexpr int NL = M::NL, NF = M::NF;
   constexpr int GPW = M::NGI / NF;          // volume points per wave
   constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
@@ -227,9 +232,10 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
 template <typename T, int NL>
 __global__ __launch_bounds__(256) void k_smooth_zero(int nele, T* __restrict__ x,
                                                      const T* __restrict__ b,
-                                                     const T* __restrict__ S) {
+                                                     const T* __restrict__ S,
+                                                     const int32_t* __restrict__ act) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nele) return;
+  if (e >= nele || (act && !act[e >> 6])) return;
   T bv[NL], xv[NL];
   load_row<T, NL>(b + (size_t)e * NL, bv);
 #pragma unroll
@@ -392,6 +398,88 @@ __global__ __launch_bounds__(256) void k_build_S_jacobi(int nele, const double* 
   for (int n = 0; n < NL; ++n) S[s_index<T, NL>(n, m, e, nele)] = (T)(omega * c[n]);
 }
 
+// omega * B^-1 for every element (the exact block smoother of
+// z05/volume_mf_um_diff_varprop.py:384-392), written in both precisions.  One wave per
+// element: Gauss-Jordan elimination with partial pivoting on [B | I] in LDS.  A singular
+// block sets *fail to 1 + its element index.
+template <int NL>
+__global__ __launch_bounds__(256) void k_build_S_direct(int nele, const double* __restrict__ B,
+                                                        double omega, double* __restrict__ S64,
+                                                        float* __restrict__ S32,
+                                                        int* __restrict__ fail) {
+  constexpr int W2 = 2 * NL + 1;  // [B | I] row, padded
+  __shared__ double a[4][NL][W2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int e = blockIdx.x * 4 + w;
+  const bool valid = e < nele;  // every wave runs the same steps (block barriers)
+  const double* Be = B + (size_t)(valid ? e : 0) * NL * NL;
+  for (int k = lane; k < NL * 2 * NL; k += 64) {
+    const int r = k / (2 * NL), cc = k % (2 * NL);
+    a[w][r][cc] = cc < NL ? Be[r * NL + cc] : (cc - NL == r ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  bool singular = false;
+  for (int c = 0; c < NL; ++c) {
+    // pivot: largest |a[r][c]|, r >= c (lanes r - c), first index on ties
+    double best = lane + c < NL ? fabs(a[w][lane + c][c]) : -1.0;
+    int arg = lane + c;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int oa = __shfl_xor(arg, off, 64);
+      if (ob > best || (ob == best && oa < arg)) {
+        best = ob;
+        arg = oa;
+      }
+    }
+    const int p = arg;
+    singular |= !(best > 0.0);
+    __syncthreads();
+    if (p != c)
+      for (int cc = lane; cc < 2 * NL; cc += 64) {
+        const double t = a[w][c][cc];
+        a[w][c][cc] = a[w][p][cc];
+        a[w][p][cc] = t;
+      }
+    __syncthreads();
+    const double d = singular ? 0.0 : 1.0 / a[w][c][c];
+    double f[NL];  // multipliers of every row, read before the update
+#pragma unroll
+    for (int r = 0; r < NL; ++r) f[r] = a[w][r][c];
+    __syncthreads();
+    for (int cc = lane; cc < 2 * NL; cc += 64) {
+      const double pv = a[w][c][cc] * d;
+#pragma unroll
+      for (int r = 0; r < NL; ++r)
+        if (r != c) a[w][r][cc] -= f[r] * pv;
+      a[w][c][cc] = pv;
+    }
+    __syncthreads();
+  }
+  if (valid) {
+    if (singular && lane == 0) atomicCAS(fail, 0, 1 + e);
+    for (int k = lane; k < NL * NL; k += 64) {
+      const int n = k / NL, m = k % NL;
+      const double v = omega * a[w][n][NL + m];
+      S64[s_index<double, NL>(n, m, e, nele)] = v;
+      S32[s_index<float, NL>(n, m, e, nele)] = (float)v;
+    }
+  }
+}
+
+int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* fail,
+                          hipStream_t s) {
+  const int nele = (int)c->nele;
+  const dim3 blk(256), grd((nele + 3) / 4);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_build_S_direct<20>, grd, blk, 0, s, nele, bdiag, omega, c->g64.S,
+                       c->g32.S, fail);
+  else
+    hipLaunchKernelGGL(k_build_S_direct<10>, grd, blk, 0, s, nele, bdiag, omega, c->g64.S,
+                       c->g32.S, fail);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
 __global__ void k_extract_diag(int64_t n, int nl, const double* __restrict__ B,
                                double* __restrict__ d) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -413,14 +501,16 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
                       hipStream_t s) {
   const int nele = (int)c->nele;
   const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
-  hipLaunchKernelGGL((k_trace_w<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf, g.trace);
+  const int32_t* act = c->mask_on ? c->act64 : nullptr;
+  hipLaunchKernelGGL((k_trace_w<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf, g.trace,
+                     act);
   const T* vt = g.tab + g.off_vtab;
   const T* wg = g.tab + g.off_wgt;
   const T* ft = g.tab + g.off_ftab;
   const T* sw = g.tab + g.off_sw;
 #define DGMI_RES(MODE)                                                                       \
   hipLaunchKernelGGL((k_residual_w<T, D, MODE>), grd, blk, 0, s, nele, x, b, r, vt, wg, ft, sw, \
-                     c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, g.trace, g.S)
+                     c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, g.trace, g.S, act)
   switch (mode) {
     case 0: DGMI_RES(0); break;
     case 1: DGMI_RES(1); break;
@@ -464,9 +554,11 @@ int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t
   const int nele = (int)c->nele;
   const dim3 blk(256), grd((nele + 255) / 256);
   if (c->D == 3)
-    hipLaunchKernelGGL((k_smooth_zero<T, 20>), grd, blk, 0, s, nele, x, b, g.S);
+    hipLaunchKernelGGL((k_smooth_zero<T, 20>), grd, blk, 0, s, nele, x, b, g.S,
+                       c->mask_on ? (const int32_t*)c->act64 : nullptr);
   else
-    hipLaunchKernelGGL((k_smooth_zero<T, 10>), grd, blk, 0, s, nele, x, b, g.S);
+    hipLaunchKernelGGL((k_smooth_zero<T, 10>), grd, blk, 0, s, nele, x, b, g.S,
+                       c->mask_on ? (const int32_t*)c->act64 : nullptr);
   DGMI_HIP(hipGetLastError());
   return 0;
 }

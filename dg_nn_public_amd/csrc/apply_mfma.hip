@@ -203,18 +203,32 @@ __device__ __forceinline__ void mf_lift(Lift<TIN, D>& lf, const MfLds<TIN, D>& L
     lf.add(L, t, lane, z);
 }
 
+// Does the work-group touch an element of an unfinished solve?  act holds one flag per 64
+// elements; a work-group covers 16*WPB elements.
+template <int WPB>
+__device__ __forceinline__ bool mf_block_active(const int32_t* __restrict__ act, int nele) {
+  const int e0 = blockIdx.x * 16 * WPB;
+  const int e1 = min(e0 + 16 * WPB, nele) - 1;
+  bool on = false;
+  for (int b = e0 >> 6; b <= (e1 >> 6); ++b) on |= act[b] != 0;
+  return on;
+}
+
 // ---------------------------------------------------------------------------------------
-// Trace pass: u and du/dn at every own face point -> trace[(f*sngi + g)*nele + e].
-// Block: 4 waves, 16 elements per wave.
-template <typename TIN, int D>
-__global__ __launch_bounds__(256) void k_trace_mf(int nele, const float* __restrict__ x,
+// Trace pass: u and du/dn at every own face point -> trace[(e*nface + f)*sngi + g]
+// (element-major: a neighbour's face is one or two cache lines for the gathers).
+// Block: WPB waves (tile shape chosen by the autotuner), 16 elements per wave.
+template <typename TIN, int D, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_trace_mf(int nele, const float* __restrict__ x,
                                                   MfArgs ta, const float* __restrict__ cf,
-                                                  float2* __restrict__ trace) {
+                                                  float2* __restrict__ trace,
+                                                  const int32_t* __restrict__ act) {
   using M = Mq<D>;
   using S = MfShape<D>;
   __shared__ MfLds<TIN, D> L;
+  if (act && !mf_block_active<WPB>(act, nele)) return;  // before any barrier
   const int lane = threadIdx.x & 63, q = lane >> 4;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = blockIdx.x * WPB + (threadIdx.x >> 6);
   const int el = wave * 16 + (lane & 15);
   const bool valid = el < nele;
   const int ec = valid ? el : nele - 1;
@@ -235,7 +249,7 @@ __global__ __launch_bounds__(256) void k_trace_mf(int nele, const float* __restr
 #pragma unroll
       for (int d = 0; d < D; ++d) dn += c[f][d] * y[1 + d];
       const int g = 4 * tt + q;
-      if (valid) trace[(f * M::SG + g) * nele + el] = make_float2(y[0], dn);
+      if (valid) trace[(el * M::NF + f) * M::SG + g] = make_float2(y[0], dn);
     }
   }
 }
@@ -244,20 +258,22 @@ __global__ __launch_bounds__(256) void k_trace_mf(int nele, const float* __restr
 // store r; 4: r = +A x.  b and r may alias.  Every per-element operand and the
 // neighbour traces are requested in the prologue, so they are in flight during the
 // volume contractions.
-template <typename TIN, int D, int MODE>
-__global__ __launch_bounds__(256) void k_residual_mf(
+template <typename TIN, int D, int MODE, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_residual_mf(
     int nele, float* __restrict__ x, const float* b, float* r, MfArgs ta,
     const float* __restrict__ wv, const float* __restrict__ G, const float* __restrict__ cf,
     const float* __restrict__ wf, const float* __restrict__ pen,
     const int32_t* __restrict__ nbface, const int32_t* __restrict__ fcode,
-    const float2* __restrict__ trace, const float* __restrict__ Sm) {
+    const float2* __restrict__ trace, const float* __restrict__ Sm,
+    const int32_t* __restrict__ act) {
   using M = Mq<D>;
   using S = MfShape<D>;
   constexpr int NL = M::NL, NF = M::NF, FT = S::FT;
   __shared__ MfLds<TIN, D> L;
-  __shared__ float rx[(MODE == 2 || MODE == 3) ? 4 : 1][16][NL + 1];
+  if (act && !mf_block_active<WPB>(act, nele)) return;  // before any barrier
+  __shared__ float rx[(MODE == 2 || MODE == 3) ? WPB : 1][16][NL + 1];
   const int lane = threadIdx.x & 63, q = lane >> 4, wl = threadIdx.x >> 6;
-  const int wave = blockIdx.x * 4 + wl;
+  const int wave = blockIdx.x * WPB + wl;
   const int el = wave * 16 + (lane & 15);
   const bool valid = el < nele;
   const int ec = valid ? el : nele - 1;
@@ -284,11 +300,10 @@ __global__ __launch_bounds__(256) void k_residual_mf(
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     const int type = code[f] & 15, a = code[f] >> 4;
-    const int nbe = nbz[f] / NF, nbf = nbz[f] - (nbz[f] / NF) * NF;
 #pragma unroll
     for (int tt = 0; tt < FT; ++tt) {
       tn[f][tt] = make_float2(0.f, 0.f);
-      if (type == 0) tn[f][tt] = trace[(nbf * M::SG + L.aln[a * M::SG + 4 * tt + q]) * nele + nbe];
+      if (type == 0) tn[f][tt] = trace[nbz[f] * M::SG + L.aln[a * M::SG + 4 * tt + q]];
     }
   }
   Lift<TIN, D> lift;
@@ -443,7 +458,12 @@ int build_mfma_tables(dgmi_ctx* c) {
   for (int g = 0; g < SG; ++g) wv[NGI + g] = (float)el.sweight[g];
   auto up = [&](const void* h, size_t bytes) -> void* {
     void* d = c->alloc(bytes);
-    if (d && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    if (!d) return nullptr;
+    const hipError_t e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      note_alloc_error(std::string("hipMemcpy to device: ") + hipGetErrorString(e));
+      return nullptr;
+    }
     return d;
   };
   c->mf.b1 = up(b1.data(), b1.size() * 2);
@@ -451,24 +471,26 @@ int build_mfma_tables(dgmi_ctx* c) {
   c->mf.wv = (float*)up(wv.data(), wv.size() * 4);
   c->mf.tT = (float*)up(tT.data(), tT.size() * 4);
   if (!c->mf.b1 || !c->mf.b2 || !c->mf.wv || !c->mf.tT) {
-    set_error("device allocation failed");
+    set_error(alloc_failure());
     return -3;
   }
   return 0;
 }
 
-template <typename TIN, int D>
+template <typename TIN, int D, int WPB>
 static int apply_mf_impl(dgmi_ctx* c, float* x, const float* b, float* r, int mode,
                          hipStream_t s) {
   const int nele = (int)c->nele;
   DevGeom<float>& g = c->g32;
   const MfArgs ta{c->mf.tT, c->mf.b1, c->mf.b2, c->aln};
-  const dim3 blk(256), grd((nele + 63) / 64);
-  hipLaunchKernelGGL((k_trace_mf<TIN, D>), grd, blk, 0, s, nele, x, ta, g.cf, (float2*)g.trace);
+  const dim3 blk(64 * WPB), grd((nele + 16 * WPB - 1) / (16 * WPB));
+  const int32_t* act = c->mask_on ? c->act64 : nullptr;
+  hipLaunchKernelGGL((k_trace_mf<TIN, D, WPB>), grd, blk, 0, s, nele, x, ta, g.cf,
+                     (float2*)g.trace, act);
 #define DGMI_RES_MF(MODE)                                                                      \
-  hipLaunchKernelGGL((k_residual_mf<TIN, D, MODE>), grd, blk, 0, s, nele, x, b, r, ta,         \
+  hipLaunchKernelGGL((k_residual_mf<TIN, D, MODE, WPB>), grd, blk, 0, s, nele, x, b, r, ta,    \
                      c->mf.wv, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,                     \
-                     (const float2*)g.trace, g.S)
+                     (const float2*)g.trace, g.S, act)
   switch (mode) {
     case 0: DGMI_RES_MF(0); break;
     case 1: DGMI_RES_MF(1); break;
@@ -481,16 +503,25 @@ static int apply_mf_impl(dgmi_ctx* c, float* x, const float* b, float* r, int mo
   return 0;
 }
 
+template <typename TIN, int D>
+static int apply_mf_wpb(dgmi_ctx* c, float* x, const float* b, float* r, int mode,
+                        hipStream_t s) {
+  const int wpb = std::is_same<TIN, float>::value ? c->mf_wpb : c->bf_wpb;
+  if (wpb == 8) return apply_mf_impl<TIN, D, 8>(c, x, b, r, mode, s);
+  if (wpb == 2) return apply_mf_impl<TIN, D, 2>(c, x, b, r, mode, s);
+  return apply_mf_impl<TIN, D, 4>(c, x, b, r, mode, s);
+}
+
 // mode as k_residual_mf; bf16 selects the BF16-input variant
 int launch_apply_mf(dgmi_ctx* c, bool bf16, float* x, const float* b, float* r, int mode,
                     hipStream_t s) {
   if (c->nele == 0) return 0;
   if (bf16) {
-    if (c->D == 3) return apply_mf_impl<__bf16, 3>(c, x, b, r, mode, s);
-    return apply_mf_impl<__bf16, 2>(c, x, b, r, mode, s);
+    if (c->D == 3) return apply_mf_wpb<__bf16, 3>(c, x, b, r, mode, s);
+    return apply_mf_wpb<__bf16, 2>(c, x, b, r, mode, s);
   }
-  if (c->D == 3) return apply_mf_impl<float, 3>(c, x, b, r, mode, s);
-  return apply_mf_impl<float, 2>(c, x, b, r, mode, s);
+  if (c->D == 3) return apply_mf_wpb<float, 3>(c, x, b, r, mode, s);
+  return apply_mf_wpb<float, 2>(c, x, b, r, mode, s);
 }
 
 }  // namespace dgmi

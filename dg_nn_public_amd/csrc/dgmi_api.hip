@@ -13,7 +13,13 @@
 namespace dgmi {
 
 static thread_local std::string g_err;
+static thread_local std::string g_alloc_err;
 void set_error(const std::string& msg) { g_err = msg; }
+void note_alloc_error(const std::string& msg) { g_alloc_err = msg; }
+std::string alloc_failure(const char* what) {
+  return std::string("device allocation failed") + (what ? std::string(" (") + what + ")" : "") +
+         (g_alloc_err.empty() ? "" : ": " + g_alloc_err);
+}
 
 #define DGMI_CHECK(cond, msg)   \
   do {                          \
@@ -33,8 +39,13 @@ static T* upload(dgmi_ctx* c, const std::vector<U>& h) {
   std::vector<T> t(h.begin(), h.end());
   T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(t.size(), 1)));
   if (!d) return nullptr;
-  if (!t.empty() && hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice) != hipSuccess)
-    return nullptr;
+  if (!t.empty()) {
+    const hipError_t e = hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      g_alloc_err = std::string("hipMemcpy to device: ") + hipGetErrorString(e);
+      return nullptr;
+    }
+  }
   return d;
 }
 
@@ -76,7 +87,7 @@ static int upload_geom(dgmi_ctx* c, DevGeom<T>& g, const std::vector<double>& ta
   g.pen = upload<T>(c, geo.pen);
   g.trace = static_cast<typename Vec2<T>::type*>(
       c->alloc(sizeof(typename Vec2<T>::type) * c->nface * c->sngi * c->nele));
-  DGMI_CHECK(g.tab && g.G && g.cf && g.wf && g.pen && g.trace, "device allocation failed");
+  DGMI_CHECK(g.tab && g.G && g.cf && g.wf && g.pen && g.trace, alloc_failure());
   return 0;
 }
 
@@ -121,39 +132,6 @@ static std::vector<double> bdiag_refmats(const ElementTables& el) {
   return r;
 }
 
-// Gauss-Jordan inverse with partial pivoting of one n x n block (row-major), in place
-static bool invert_block(double* a, int n) {
-  std::vector<double> inv(n * n, 0.0);
-  for (int i = 0; i < n; ++i) inv[i * n + i] = 1.0;
-  for (int c = 0; c < n; ++c) {
-    int p = c;
-    for (int r = c + 1; r < n; ++r)
-      if (std::fabs(a[r * n + c]) > std::fabs(a[p * n + c])) p = r;
-    if (a[p * n + c] == 0.0) return false;
-    if (p != c)
-      for (int k = 0; k < n; ++k) {
-        std::swap(a[p * n + k], a[c * n + k]);
-        std::swap(inv[p * n + k], inv[c * n + k]);
-      }
-    const double d = 1.0 / a[c * n + c];
-    for (int k = 0; k < n; ++k) {
-      a[c * n + k] *= d;
-      inv[c * n + k] *= d;
-    }
-    for (int r = 0; r < n; ++r)
-      if (r != c) {
-        const double f = a[r * n + c];
-        if (f != 0.0)
-          for (int k = 0; k < n; ++k) {
-            a[r * n + k] -= f * a[c * n + k];
-            inv[r * n + k] -= f * inv[c * n + k];
-          }
-      }
-  }
-  std::copy(inv.begin(), inv.end(), a);
-  return true;
-}
-
 struct Scratch {
   double* bdiag = nullptr;
   double* refm = nullptr;
@@ -170,7 +148,11 @@ using namespace dgmi;
 
 void* dgmi_ctx::alloc(size_t bytes) {
   void* p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess || !p) {
+    g_alloc_err = "hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e);
+    return nullptr;
+  }
   allocs.push_back(p);
   dev_bytes += bytes;
   return p;
@@ -215,11 +197,11 @@ static int compute_bdiag(dgmi_ctx* c, hipStream_t s) {
   Scratch& sc = scratch(c);
   if (!sc.refm) {
     sc.refm = upload<double>(c, bdiag_refmats(c->el));
-    DGMI_CHECK(sc.refm, "device allocation failed");
+    DGMI_CHECK(sc.refm, alloc_failure());
   }
   if (!sc.bdiag) {
     sc.bdiag = static_cast<double*>(c->alloc(sizeof(double) * c->nele * c->nloc * c->nloc));
-    DGMI_CHECK(sc.bdiag, "device allocation failed");
+    DGMI_CHECK(sc.bdiag, alloc_failure());
   }
   return launch_bdiag(c, sc.refm, sc.bdiag, s);
 }
@@ -337,7 +319,7 @@ static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool fl
   T* Z = static_cast<T*>(k->Z);
   T* w = vbuf<T>(c, 2);
   T* t = vbuf<T>(c, 1);
-  DGMI_CHECK(w && t, "device allocation failed");
+  DGMI_CHECK(w && t, alloc_failure());
   auto Vj = [&](int j) { return V + (size_t)j * n; };
   auto Zj = [&](int j) { return Z + (size_t)j * n; };
   const T* start = r0;
@@ -345,9 +327,10 @@ static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool fl
     DGMI_TRY(precond<T>(c, r0, Vj(0), s));
     start = Vj(0);
   }
-  DGMI_TRY(kry::sdot<T>(sg, start, start, nullptr, k->beta, s));
+  DGMI_TRY(kry::sdot<T>(sg, start, start, nullptr, k->beta, s, k->want));
   DGMI_TRY(kry::scycle_start(nc, k->beta, k->eps, k->want, k->active, k->thr, k->g, k->gs,
                              m + 2, k->kk, s));
+  if (c->mask_on) DGMI_TRY(kry::expand_active(sg, c->nele, k->active, c->act64, s));
   DGMI_TRY((kry::sscale<T, T>(sg, k->beta, 0, k->active, start, Vj(0), s)));
   for (int j = 0; j < m; ++j) {
     if (flexible) {
@@ -368,10 +351,11 @@ static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool fl
       DGMI_TRY(kry::sorth_pass<T>(sg, j + 1, V, n, w, hcol, k->hs, 0, k->active, s));
       DGMI_TRY(kry::sorth_pass<T>(sg, j + 1, V, n, w, hcol, k->hs, 1, k->active, s));
     }
-    DGMI_TRY(kry::sdot<T>(sg, w, w, nullptr, k->hn, s));
+    DGMI_TRY(kry::sdot<T>(sg, w, w, nullptr, k->hn, s, k->active));
     DGMI_TRY((kry::sscale<T, T>(sg, k->hn, 0, k->active, w, Vj(j + 1), s)));
     DGMI_TRY(kry::sgivens(nc, j, k->ldh, k->hs, k->H, k->hn, k->cs, k->sn, k->rs, k->g, k->gs,
                           k->res, k->thr, k->active, k->kk, k->nact, s));
+    if (c->mask_on) DGMI_TRY(kry::expand_active(sg, c->nele, k->active, c->act64, s));
     DGMI_HIP(hipMemcpyAsync(k->hi + 2 * nc, k->nact, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     DGMI_HIP(hipStreamSynchronize(s));
     if (k->hi[2 * nc] == 0) break;
@@ -389,27 +373,43 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
   const SegDev& sg = c->seg;
   const bool flexible = inner != DGMI_F64 && c->inner_flexible;
   Krylov* k = krylov(c, m, inner == DGMI_F64 ? sizeof(double) : sizeof(float), flexible);
-  DGMI_CHECK(k, "device allocation failed (Krylov workspace)");
+  DGMI_CHECK(k, alloc_failure("Krylov workspace"));
   double* r64 = vbuf<double>(c, 0);
-  DGMI_CHECK(r64, "device allocation failed");
+  DGMI_CHECK(r64, alloc_failure());
   DGMI_TRY(kry::sdot<double>(sg, b, b, nullptr, k->bnorm, s));
   std::vector<double> bnorm(nc), rnorm(nc), eps(nc);
   std::vector<int> its(nc, 0), outer(nc, 0);
+  std::vector<char> done(nc, 0);
   DGMI_HIP(hipMemcpyAsync(bnorm.data(), k->bnorm, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
   DGMI_HIP(hipStreamSynchronize(s));
   for (double v : bnorm) DGMI_CHECK(std::isfinite(v), "non-finite right-hand side");
   int cycles = 0;
+  // element kernels skip the solves that have finished (several components only)
+  struct MaskScope {
+    dgmi_ctx* c;
+    ~MaskScope() { c->mask_on = false; }
+  } mask_scope{c};
+  c->mask_on = false;
   for (;;) {
-    DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r64, s));  // FP64 true residual
+    // FP64 true residual of every component (a component that converged inside the last
+    // cycle still needs its new residual)
+    c->mask_on = false;
+    DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r64, s));
     DGMI_TRY(kry::sdot<double>(sg, r64, r64, nullptr, k->rnorm, s));
     DGMI_HIP(hipMemcpyAsync(k->hd, k->rnorm, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
     DGMI_HIP(hipStreamSynchronize(s));
     int nwant = 0;
     for (int q = 0; q < nc; ++q) {
+      if (done[q]) {  // finished earlier: its residual was not recomputed, keep the final one
+        k->hi[q] = 0;
+        eps[q] = 0.5;
+        continue;
+      }
       rnorm[q] = k->hd[q];
       DGMI_CHECK(std::isfinite(rnorm[q]), "non-finite residual in component " + std::to_string(q));
       const double tol = rtol * bnorm[q];
       const bool want = !(rnorm[q] <= tol || its[q] >= max_its || bnorm[q] == 0.0);
+      done[q] = !want;
       k->hi[q] = want ? 1 : 0;
       nwant += want;
       double e = want ? std::min(0.5, 0.5 * tol / rnorm[q]) : 0.5;
@@ -420,16 +420,20 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
     std::memcpy(k->hd + nc, eps.data(), sizeof(double) * nc);
     DGMI_HIP(hipMemcpyAsync(k->want, k->hi, sizeof(int32_t) * nc, hipMemcpyHostToDevice, s));
     DGMI_HIP(hipMemcpyAsync(k->eps, k->hd + nc, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    if (nc > 1) {
+      c->mask_on = true;
+      DGMI_TRY(kry::expand_active(sg, c->nele, k->want, c->act64, s));
+    }
     if (inner == DGMI_F64) {
       // the cycle uses scratch slots 1..3; the correction has its own slot
       double* dv = corr_buf<double>(c);
-      DGMI_CHECK(dv, "device allocation failed");
+      DGMI_CHECK(dv, alloc_failure());
       DGMI_TRY(gmres_cycle<double>(c, k, r64, dv, m, false, s));
       DGMI_TRY(kry::sadd<double>(sg, nullptr, dv, x, s));
     } else {  // FP32 cycle (bf16-input preconditioner applies when inner == DGMI_BF16)
       float* r32 = vbuf<float>(c, 0);
       float* dv = corr_buf<float>(c);
-      DGMI_CHECK(r32 && dv, "device allocation failed");
+      DGMI_CHECK(r32 && dv, alloc_failure());
       // residual scaled to unit norm per component in FP32; correction scaled back in FP64
       DGMI_TRY((kry::sscale<double, float>(sg, k->rnorm, 0, k->want, r64, r32, s)));
       DGMI_TRY(gmres_cycle<float>(c, k, r32, dv, m, flexible, s));
@@ -471,7 +475,12 @@ static int build_seg(dgmi_ctx* c) {
   sg.comp_blk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * cb.size()));
   sg.partial = static_cast<double*>(c->alloc(sizeof(double) * MDOT_K * std::max(sg.nblk, 1)));
   sg.coef = static_cast<double*>(c->alloc(sizeof(double) * MDOT_K * c->ncomp));
-  DGMI_CHECK(sg.blk && sg.comp_blk && sg.partial && sg.coef, "device allocation failed");
+  sg.eoff = static_cast<int64_t*>(c->alloc(sizeof(int64_t) * (c->ncomp + 1)));
+  c->act64 = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * ((c->nele + 63) / 64)));
+  DGMI_CHECK(sg.blk && sg.comp_blk && sg.partial && sg.coef && sg.eoff && c->act64,
+             alloc_failure());
+  DGMI_HIP(hipMemcpy(sg.eoff, c->comp_eoff.data(), sizeof(int64_t) * (c->ncomp + 1),
+                     hipMemcpyHostToDevice));
   DGMI_HIP(hipMemcpy(sg.blk, blk.data(), sizeof(SegBlk) * blk.size(), hipMemcpyHostToDevice));
   DGMI_HIP(hipMemcpy(sg.comp_blk, cb.data(), sizeof(int32_t) * cb.size(), hipMemcpyHostToDevice));
   return 0;
@@ -550,7 +559,7 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
     c->aln = upload<int32_t>(c, al);
     c->detJ = upload<double>(c, geo.detJ);
     if (!c->nbface || !c->fcode || !c->aln || !c->detJ) {
-      set_error("device allocation failed");
+      set_error(alloc_failure());
       rc = -3;
     }
   }
@@ -600,6 +609,9 @@ int dgmi_ctx_create_batch(const dgmi_mesh_desc* d, int32_t ncomp, const int64_t*
 int dgmi_ctx_destroy(dgmi_ctx* c) {
   if (!c) return 0;
   (void)hipDeviceSynchronize();
+  if (c->mg.graph64.exec) (void)hipGraphExecDestroy(c->mg.graph64.exec);
+  if (c->mg.graph32.exec) (void)hipGraphExecDestroy(c->mg.graph32.exec);
+  if (c->mg.capture_stream) (void)hipStreamDestroy(c->mg.capture_stream);
   for (void* p : c->allocs) (void)hipFree(p);
   delete static_cast<Scratch*>(c->scratch);
   if (Krylov* k = static_cast<Krylov*>(c->krylov)) {
@@ -685,7 +697,7 @@ int dgmi_rhs(dgmi_ctx* c, const double* f, const double* u_bc, double* b, void* 
   Scratch& sc = scratch(c);
   if (!sc.refm) {
     sc.refm = upload<double>(c, bdiag_refmats(c->el));
-    DGMI_CHECK(sc.refm, "device allocation failed");
+    DGMI_CHECK(sc.refm, alloc_failure());
   }
   return launch_rhs(c, sc.refm, c->detJ, f, u_bc, b, s);
 }
@@ -711,34 +723,20 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
   const int64_t nS = c->nele * c->nloc * c->nloc;
   if (!c->g64.S) c->g64.S = static_cast<double*>(c->alloc(sizeof(double) * nS));
   if (!c->g32.S) c->g32.S = static_cast<float*>(c->alloc(sizeof(float) * nS));
-  DGMI_CHECK(c->g64.S && c->g32.S, "device allocation failed");
+  DGMI_CHECK(c->g64.S && c->g32.S, alloc_failure());
   Scratch& sc = scratch(c);
   if (variant == DGMI_SMOOTH_JACOBI3) {
     DGMI_TRY(launch_build_S_jacobi<double>(c, sc.bdiag, omega, c->g64.S, s));
     DGMI_TRY(launch_build_S_jacobi<float>(c, sc.bdiag, omega, c->g32.S, s));
   } else {
-    const int NL = c->nloc, NN = NL * NL;
-    std::vector<double> B((size_t)nS), St((size_t)nS);
-    std::vector<float> Sf((size_t)nS);
-    DGMI_HIP(hipMemcpyAsync(B.data(), sc.bdiag, sizeof(double) * nS, hipMemcpyDeviceToHost, s));
+    int* fail = static_cast<int*>(c->alloc(sizeof(int)));
+    DGMI_CHECK(fail, alloc_failure());
+    DGMI_HIP(hipMemsetAsync(fail, 0, sizeof(int), s));
+    DGMI_TRY(launch_build_S_direct(c, sc.bdiag, omega, fail, s));
+    int hf = 0;
+    DGMI_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, s));
     DGMI_HIP(hipStreamSynchronize(s));
-    // structure-of-vectors layout of the kernels (width W: 16 bytes, or 8 when the row
-    // length is not a multiple of 16 bytes)
-    auto width = [&](size_t bytes) { return (NL % (16 / bytes) == 0) ? 16 / bytes : 8 / bytes; };
-    const int W64 = (int)width(8), W32 = (int)width(4);
-    const int64_t ne = c->nele;
-    for (int64_t e = 0; e < ne; ++e) {
-      double* blk = B.data() + e * NN;
-      DGMI_CHECK(invert_block(blk, NL), "singular diagonal block " + std::to_string(e));
-      for (int n = 0; n < NL; ++n)
-        for (int m = 0; m < NL; ++m) {
-          const double v = omega * blk[n * NL + m];
-          St[((size_t)(n * (NL / W64) + m / W64) * ne + e) * W64 + m % W64] = v;
-          Sf[((size_t)(n * (NL / W32) + m / W32) * ne + e) * W32 + m % W32] = (float)v;
-        }
-    }
-    DGMI_HIP(hipMemcpy(c->g64.S, St.data(), sizeof(double) * nS, hipMemcpyHostToDevice));
-    DGMI_HIP(hipMemcpy(c->g32.S, Sf.data(), sizeof(float) * nS, hipMemcpyHostToDevice));
+    DGMI_CHECK(hf == 0, "singular diagonal block " + std::to_string(hf - 1));
   }
   c->smoother_variant = variant;
   c->omega = omega;
@@ -807,7 +805,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     mg.cellpos = upload<int32_t>(c, cellpos);
     mg.rw64 = upload<double>(c, rw);
     mg.rw32 = upload<float>(c, rw);
-    DGMI_CHECK(mg.rptr && mg.rent && mg.cellpos && mg.rw64 && mg.rw32, "device allocation failed");
+    DGMI_CHECK(mg.rptr && mg.rent && mg.cellpos && mg.rw64 && mg.rw32, alloc_failure());
     mg.cg_nonods = ncg;
     return 0;
   };
@@ -820,7 +818,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   std::vector<int32_t> whichc_sfc(c->nverts);
   for (int64_t i = 0; i < c->nverts; ++i) whichc_sfc[pos[i]] = whichc[i];
   int32_t* whichc_d = upload<int32_t>(c, whichc_sfc);
-  DGMI_CHECK(whichc_d, "device allocation failed");
+  DGMI_CHECK(whichc_d, alloc_failure());
   // coloured probing of RAR (z02/volume_mf_diff.py:1696-1754): entries grouped by the
   // colour of their column; each row holds at most one entry per colour
   std::vector<int64_t> cnt(mg.ncolor + 1, 0);
@@ -844,7 +842,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   double* probe = vbuf<double>(c, 0);
   double* aprobe = vbuf<double>(c, 1);
   double* rsfc = static_cast<double*>(c->alloc(sizeof(double) * c->nverts));
-  DGMI_CHECK(spidx_d && rowpos_d && vals_d && probe && aprobe && rsfc, "device allocation failed");
+  DGMI_CHECK(spidx_d && rowpos_d && vals_d && probe && aprobe && rsfc, alloc_failure());
   for (int cc = 1; cc <= mg.ncolor; ++cc) {
     DGMI_TRY(launch_probe_mask(c, whichc_d, cc, probe, s));
     DGMI_TRY(launch_apply<double>(c, c->g64, probe, nullptr, aprobe, s));  // -A R m
@@ -1004,12 +1002,81 @@ int dgmi_gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, double 
   return gmres_solve(c, inner, b, x, rtol, m, max_its, stats, (hipStream_t)stream);
 }
 
+int dgmi_autotune(dgmi_ctx* c, int reps, int32_t* choice, double* times_us, void* stream) {
+  DGMI_CHECK(c, "null context");
+  DGMI_CHECK(reps >= 1, "reps must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  float* x = vbuf<float>(c, 1);
+  float* b = vbuf<float>(c, 2);
+  float* r = vbuf<float>(c, 3);
+  DGMI_CHECK(x && b && r, alloc_failure());
+  DGMI_HIP(hipMemsetAsync(x, 0, sizeof(float) * c->ndof, s));
+  DGMI_HIP(hipMemsetAsync(b, 0, sizeof(float) * c->ndof, s));
+  hipEvent_t e0, e1;
+  DGMI_HIP(hipEventCreate(&e0));
+  DGMI_HIP(hipEventCreate(&e1));
+  const int saved_k = c->f32_kernel, saved_m = c->mf_wpb, saved_b = c->bf_wpb;
+  // candidates: FP32 vector; FP32 MFMA with 2/4/8 waves per work-group; bf16 likewise
+  const int cand[7][2] = {{0, 4}, {1, 2}, {1, 4}, {1, 8}, {2, 2}, {2, 4}, {2, 8}};
+  double t[7];
+  int rc = 0;
+  for (int k = 0; k < 7 && !rc; ++k) {
+    c->f32_kernel = cand[k][0] == 2 ? 1 : cand[k][0];
+    c->mf_wpb = cand[k][1];
+    c->bf_wpb = cand[k][1];
+    auto run = [&]() {
+      return cand[k][0] == 2 ? launch_apply_mf(c, true, x, b, r, 1, s)
+                             : launch_apply<float>(c, c->g32, x, b, r, s);
+    };
+    rc = run();  // warm-up
+    if (rc) break;
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < reps && !rc; ++i) rc = run();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    t[k] = 1e3 * ms / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->f32_kernel = saved_k;
+  c->mf_wpb = saved_m;
+  c->bf_wpb = saved_b;
+  if (rc) return rc;
+  int bf = 0, bb = 4;
+  for (int k = 1; k < 4; ++k)
+    if (t[k] < t[bf]) bf = k;
+  for (int k = 5; k < 7; ++k)
+    if (t[k] < t[bb]) bb = k;
+  c->f32_kernel = cand[bf][0];
+  if (bf > 0) c->mf_wpb = cand[bf][1];
+  c->bf_wpb = cand[bb][1];
+  if (choice) {
+    choice[0] = c->f32_kernel;
+    choice[1] = c->mf_wpb;
+    choice[2] = c->bf_wpb;
+  }
+  if (times_us) std::copy(t, t + 7, times_us);
+  return 0;
+}
+
 int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   DGMI_CHECK(c && name, "null argument");
   const std::string k(name);
   if (k == "gmres_orth") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "gmres_orth must be 0 (MGS) or 1 (CGS2)");
     c->gmres_orth = (int)value;
+    return 0;
+  }
+  if (k == "mfma_waves" || k == "bf16_waves") {
+    DGMI_CHECK(value == 2.0 || value == 4.0 || value == 8.0, k + " must be 2, 4 or 8");
+    (k == "mfma_waves" ? c->mf_wpb : c->bf_wpb) = (int)value;
+    return 0;
+  }
+  if (k == "graph_vcycle") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "graph_vcycle must be 0 or 1");
+    c->mg.use_graph = value != 0.0;
     return 0;
   }
   if (k == "inner_flexible") {

@@ -11,6 +11,10 @@
 namespace dgmi {
 
 void set_error(const std::string& msg);
+// message for a failed device allocation, with the HIP error of the last failure on this
+// thread
+std::string alloc_failure(const char* what = nullptr);
+void note_alloc_error(const std::string& msg);
 
 #define DGMI_HIP(call)                                                          \
   do {                                                                          \
@@ -39,7 +43,9 @@ template <typename T> struct DevGeom {
   T* wf = nullptr;     // (nface, nele)
   T* pen = nullptr;    // (nface, nele)
   T* S = nullptr;      // smoother (nloc*nloc, nele), row n col m at (n*nloc+m)*nele + e
-  typename Vec2<T>::type* trace = nullptr;  // (nface*sngi, nele)
+  // face traces: (nface*sngi, nele) for the vector kernels, (nele, nface, sngi) for the
+  // matrix-core kernels
+  typename Vec2<T>::type* trace = nullptr;
 };
 
 // SFC levels on the device.  "Grid" levels 0..small_level hold every component whose
@@ -87,6 +93,7 @@ struct SegDev {
   int nblk = 0;
   SegBlk* blk = nullptr;        // (nblk)
   int32_t* comp_blk = nullptr;  // (ncomp + 1) first block of each component
+  int64_t* eoff = nullptr;      // (ncomp + 1) element offsets of the components
   double* partial = nullptr;    // (nblk * MDOT_K)
   double* coef = nullptr;       // (ncomp * MDOT_K)
 };
@@ -97,6 +104,14 @@ struct MfTabs {
   void* b2 = nullptr;   // bf16 A operands of r = T^T Z, lane order
   float* wv = nullptr;  // quadrature weights: volume points, then face points
   float* tT = nullptr;  // fp32 T, row-major (rows x nloc) plus 16 zero words
+};
+
+// a captured V-cycle and the arguments it was captured with
+struct VGraph {
+  hipGraphExec_t exec = nullptr;
+  const void* r = nullptr;
+  void* e = nullptr;
+  double omega = 0.0;
 };
 
 struct MGDev {
@@ -117,11 +132,17 @@ struct MGDev {
   int small_level = 0;  // grid kernels sweep levels below this one
   // level sizes of every component (host), components in context order
   std::vector<std::vector<int64_t>> comp_levels;
+  // V-cycle replayed from a captured graph (set_param "graph_vcycle")
+  bool use_graph = true;
+  VGraph graph64, graph32;
+  hipStream_t capture_stream = nullptr;
 };
 
 namespace kry {
+// active (may be null): components with a zero flag are skipped (their result is 0)
 template <typename T>
-int sdot(const SegDev& sg, const T* x, const T* y, double* out, double* sq, hipStream_t s);
+int sdot(const SegDev& sg, const T* x, const T* y, double* out, double* sq, hipStream_t s,
+         const int32_t* active = nullptr);
 template <typename T>
 int sorth_pass(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h, int64_t hs,
                int accumulate, const int32_t* active, hipStream_t s);
@@ -141,6 +162,9 @@ int sgivens(int ncomp, int j, int ldh, int64_t hs, double* H, const double* hn, 
             int32_t* active, int32_t* kk, int32_t* nact, hipStream_t s);
 int sbacksolve(int ncomp, int ldh, int64_t hs, const double* H, const double* g, int64_t gs,
                const int32_t* kk, double* yv, int64_t ys, hipStream_t s);
+// act64[b] = any component overlapping elements [64b, 64b+64) has flag[c] != 0
+int expand_active(const SegDev& sg, int64_t nele, const int32_t* flag, int32_t* act64,
+                  hipStream_t s);
 }  // namespace kry
 
 }  // namespace dgmi
@@ -164,6 +188,8 @@ struct dgmi_ctx {
   // kernel family of every FP32-vector apply/smooth: 0 vector FMA, 1 MFMA fp32,
   // 2 MFMA with bf16 inputs
   int f32_kernel = 1;
+  // waves (16 elements each) per work-group of the fp32 / bf16 matrix-core applies
+  int mf_wpb = 4, bf_wpb = 4;
   bool pc_bf16 = false;  // FP32 preconditioner applies with bf16 inputs
   // FP32 refinement cycles: 1 flexible (right-preconditioned) GMRES, 0 left-preconditioned
   int inner_flexible = 1;
@@ -172,6 +198,10 @@ struct dgmi_ctx {
   int ncomp = 1;
   std::vector<int64_t> comp_eoff, comp_voff;
   dgmi::SegDev seg;
+  // per 64 elements: 1 while the element's solve is unfinished; element kernels skip
+  // finished solves while mask_on (set only inside the GMRES driver)
+  int32_t* act64 = nullptr;
+  bool mask_on = false;
   void* scratch = nullptr;  // solver scratch vectors (dgmi_api.hip)
   void* krylov = nullptr;   // Krylov workspace (dgmi_api.hip)
   dgmi::MfTabs mf;
@@ -203,6 +233,10 @@ int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t
 int launch_bdiag(dgmi_ctx* c, const double* refmats, double* bdiag, hipStream_t s);
 template <typename T>
 int launch_build_S_jacobi(dgmi_ctx* c, const double* bdiag, double omega, T* S, hipStream_t s);
+// omega * inverse of every diagonal block into both smoother copies; *fail = 1 + element
+// index of a singular block
+int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* fail,
+                          hipStream_t s);
 template <typename T>
 int launch_convert(const double* in, T* out, int64_t n, hipStream_t s);
 // transfers + SFC levels (mg.hip)

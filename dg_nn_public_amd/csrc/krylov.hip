@@ -72,8 +72,13 @@ template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_sdot_partial(const SegBlk* __restrict__ sb,
                                                               const T* __restrict__ x,
                                                               const T* __restrict__ y,
-                                                              double* __restrict__ partial) {
+                                                              double* __restrict__ partial,
+                                                              const int32_t* __restrict__ active) {
   const SegBlk b = sb[blockIdx.x];
+  if (active && !active[b.comp]) {  // finished component (uniform over the block)
+    if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
   double acc = 0.0;
   for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) acc += (double)x[i] * (double)y[i];
   acc = block_sum(acc);
@@ -100,8 +105,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_smdot_partial(const SegBlk* __r
                                                                int k, const T* __restrict__ V,
                                                                int64_t ldv,
                                                                const T* __restrict__ w,
-                                                               double* __restrict__ partial) {
+                                                               double* __restrict__ partial,
+                                                               const int32_t* __restrict__ active) {
   const SegBlk b = sb[blockIdx.x];
+  if (active && !active[b.comp]) {  // finished component (uniform over the block)
+    if (threadIdx.x < k) partial[(int64_t)blockIdx.x * MDOT_K + threadIdx.x] = 0.0;
+    return;
+  }
   double acc[MDOT_K];
 #pragma unroll
   for (int j = 0; j < MDOT_K; ++j) acc[j] = 0.0;
@@ -156,8 +166,10 @@ template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_smaxpy(const SegBlk* __restrict__ sb, int k,
                                                         const T* __restrict__ V, int64_t ldv,
                                                         const double* __restrict__ coef,
-                                                        T* __restrict__ w) {
+                                                        T* __restrict__ w,
+                                                        const int32_t* __restrict__ active) {
   const SegBlk b = sb[blockIdx.x];
+  if (active && !active[b.comp]) return;
   const double* cf = coef + b.comp * MDOT_K;
   for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) {
     double s = (double)w[i];
@@ -282,6 +294,22 @@ __global__ void k_sbacksolve(int ncomp, int ldh, int64_t hs, const double* __res
   }
 }
 
+__global__ void k_expand_active(int64_t nele, int ncomp, const int64_t* __restrict__ eoff,
+                                const int32_t* __restrict__ flag, int32_t* __restrict__ act64) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t lo = 64 * b, hi = min(lo + 64, nele);
+  if (lo >= nele) return;
+  int a = 0, z = ncomp;  // first component whose range ends after lo
+  while (a < z) {
+    const int m = (a + z) / 2;
+    if (eoff[m + 1] <= lo) a = m + 1;
+    else z = m;
+  }
+  int on = 0;
+  for (int c = a; c < ncomp && eoff[c] < hi; ++c) on |= flag[c] != 0;
+  act64[b] = on;
+}
+
 __global__ void k_izero(int n, int32_t* __restrict__ p) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -324,9 +352,10 @@ template int launch_axpy<float>(int64_t, double, const float*, float*, hipStream
 namespace kry {
 // ---- segmented launchers
 template <typename T>
-int sdot(const SegDev& sg, const T* x, const T* y, double* out, double* sq, hipStream_t s) {
+int sdot(const SegDev& sg, const T* x, const T* y, double* out, double* sq, hipStream_t s,
+         const int32_t* active) {
   hipLaunchKernelGGL((k_sdot_partial<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, x, y,
-                     sg.partial);
+                     sg.partial, active);
   hipLaunchKernelGGL(k_sdot_final, dim3(sg.ncomp), dim3(RED_THREADS), 0, s, sg.comp_blk,
                      (const double*)sg.partial, out, sq);
   DGMI_HIP(hipGetLastError());
@@ -342,11 +371,11 @@ int sorth_pass(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h
   for (int c0 = 0; c0 < k; c0 += MDOT_K) {
     const int kc = std::min(MDOT_K, k - c0);
     hipLaunchKernelGGL((k_smdot_partial<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, kc,
-                       V + (int64_t)c0 * ldv, ldv, (const T*)w, sg.partial);
+                       V + (int64_t)c0 * ldv, ldv, (const T*)w, sg.partial, active);
     hipLaunchKernelGGL(k_smdot_final, dim3(sg.ncomp), dim3(RED_THREADS), 0, s, sg.comp_blk,
                        (const double*)sg.partial, kc, h + c0, hs, sg.coef, accumulate, active);
     hipLaunchKernelGGL((k_smaxpy<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, kc,
-                       V + (int64_t)c0 * ldv, ldv, (const double*)sg.coef, w);
+                       V + (int64_t)c0 * ldv, ldv, (const double*)sg.coef, w, active);
   }
   DGMI_HIP(hipGetLastError());
   return 0;
@@ -404,8 +433,19 @@ int sbacksolve(int ncomp, int ldh, int64_t hs, const double* H, const double* g,
   return 0;
 }
 
-template int sdot<double>(const SegDev&, const double*, const double*, double*, double*, hipStream_t);
-template int sdot<float>(const SegDev&, const float*, const float*, double*, double*, hipStream_t);
+int expand_active(const SegDev& sg, int64_t nele, const int32_t* flag, int32_t* act64,
+                  hipStream_t s) {
+  const int64_t nb = (nele + 63) / 64;
+  hipLaunchKernelGGL(k_expand_active, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, nele,
+                     sg.ncomp, sg.eoff, flag, act64);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template int sdot<double>(const SegDev&, const double*, const double*, double*, double*,
+                          hipStream_t, const int32_t*);
+template int sdot<float>(const SegDev&, const float*, const float*, double*, double*, hipStream_t,
+                         const int32_t*);
 template int sorth_pass<double>(const SegDev&, int, const double*, int64_t, double*, double*,
                                 int64_t, int, const int32_t*, hipStream_t);
 template int sorth_pass<float>(const SegDev&, int, const float*, int64_t, float*, double*,

@@ -113,18 +113,27 @@ __global__ void k_probe_collect(int64_t nent, const int32_t* __restrict__ spidx,
 //   eout = ein + omega (b - A ein) / diag
 // With `ec` set, the input is ein + P ec (nearest prolongation, coarse index i>>1), which
 // fuses the up-cycle's e_l += Upsample(e_{l+1}) (:163-164) into the first post-sweep.
+// Grid kernels give each row a group of ROW_LANES lanes: the lanes load the row's entries
+// side by side (rows hold 14-32 entries), so a sweep costs one round of dependent loads
+// instead of one per entry, and the group sums its partial products with shuffles.
+constexpr int ROW_LANES = 16;
+
 template <typename T>
-__device__ __forceinline__ T level_row(int i, const int32_t* __restrict__ rp,
-                                       const int32_t* __restrict__ col,
-                                       const T* __restrict__ val, const T* __restrict__ ein,
-                                       const T* __restrict__ ec) {
+__device__ __forceinline__ T level_row_group(int i, int sub, const int32_t* __restrict__ rp,
+                                             const int32_t* __restrict__ col,
+                                             const T* __restrict__ val,
+                                             const T* __restrict__ ein,
+                                             const T* __restrict__ ec) {
   T s = 0;
-  for (int q = rp[i]; q < rp[i + 1]; ++q) {
+  const int q1 = rp[i + 1];
+  for (int q = rp[i] + sub; q < q1; q += ROW_LANES) {
     const int j = col[q];
     T ej = ein[j];
     if (ec) ej += ec[j >> 1];
     s += val[q] * ej;
   }
+#pragma unroll
+  for (int off = ROW_LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   return s;
 }
 
@@ -136,17 +145,21 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int32_t* __restrict_
                                                const T* __restrict__ b, const T* __restrict__ ein,
                                                const T* __restrict__ ec, T* __restrict__ eout,
                                                T omega) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T ax = level_row<T>(i, rp, col, val, ein, ec);
-  T ei = ein[i];
-  if (ec) ei += ec[i >> 1];
-  eout[i] = ei + omega * (b[i] - ax) / diag[i];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / ROW_LANES, sub = t % ROW_LANES;
+  if (i >= n) return;  // whole groups leave together
+  const T ax = level_row_group<T>(i, sub, rp, col, val, ein, ec);
+  if (sub == 0) {
+    T ei = ein[i];
+    if (ec) ei += ec[i >> 1];
+    eout[i] = ei + omega * (b[i] - ax) / diag[i];
+  }
 }
 
 // Last pre-sweep of level l, fused with the restriction to level l+1: the reference
 // evaluates rr = b - A e and applies e += omega rr/diag in the same call (:144-146), then
-// restricts rr and e by pairwise sums (:147-153).  One thread per coarse node.
+// restricts rr and e by pairwise sums (:147-153).  Two lane groups per coarse node, one
+// per fine row of its pair.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep_restrict(int n, const int32_t* __restrict__ rp,
                                                         const int32_t* __restrict__ col,
@@ -156,23 +169,25 @@ __global__ __launch_bounds__(256) void k_sweep_restrict(int n, const int32_t* __
                                                         const T* __restrict__ ein,
                                                         T* __restrict__ eout, T* __restrict__ rc,
                                                         T* __restrict__ ecn, T omega) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = t / (2 * ROW_LANES);
+  const int h = (t / ROW_LANES) & 1, sub = t % ROW_LANES;
   const int nc = (n - 1) / 2 + 1;
-  if (I >= nc) return;
-  T rs = 0, es = 0;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int i = 2 * I + h;
-    if (i < n) {
-      const T rr = b[i] - level_row<T>(i, rp, col, val, ein, (const T*)nullptr);
-      const T en = ein[i] + omega * rr / diag[i];
-      eout[i] = en;
-      rs += rr;
-      es += en;
-    }
+  if (I >= nc) return;  // both groups of a pair leave together
+  const int i = 2 * I + h;
+  T rr = 0, en = 0;
+  if (i < n) {  // uniform over the group
+    rr = b[i] - level_row_group<T>(i, sub, rp, col, val, ein, (const T*)nullptr);
+    en = ein[i] + omega * rr / diag[i];
+    if (sub == 0) eout[i] = en;
   }
-  rc[I] = rs;
-  ecn[I] = es;
+  // pair sum: lane l of group 0 with lane l of group 1
+  const T rs = rr + __shfl_xor(rr, ROW_LANES, 64);
+  const T es = en + __shfl_xor(en, ROW_LANES, 64);
+  if (h == 0 && sub == 0) {
+    rc[I] = rs;
+    ecn[I] = es;
+  }
 }
 
 // One level of one component for the single-work-group kernel.  Row i (0 <= i < n) has
@@ -394,8 +409,13 @@ static T* upload_vec(dgmi_ctx* c, const std::vector<U>& h) {
   std::vector<T> t(h.begin(), h.end());
   T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(t.size(), 1)));
   if (!d) return nullptr;
-  if (!t.empty() && hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice) != hipSuccess)
-    return nullptr;
+  if (!t.empty()) {
+    const hipError_t e = hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      note_alloc_error(std::string("hipMemcpy to device: ") + hipGetErrorString(e));
+      return nullptr;
+    }
+  }
   return d;
 }
 
@@ -449,8 +469,16 @@ int launch_probe_collect(dgmi_ctx*, const double* r_sfc, const int32_t* spidx,
 // A_l/B_l ping-pong, level 0's B is the caller's output e_sfc and level 0's rhs is the
 // caller's r_sfc.  Grid levels 0..small_level-1 sweep the rows of the components that
 // start deeper; one work-group per component then runs the rest.
+// y[i] = x[i] (or 0 with x null): replaces memset/memcpy nodes so the cycle can be
+// captured in a graph
 template <typename T>
-int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
+__global__ void k_copy(int64_t n, const T* __restrict__ x, T* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x ? x[i] : T(0);
+}
+
+template <typename T>
+static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   auto& mg = c->mg;
   auto& lv = mg.lv;
   const bool f64 = std::is_same<T, double>::value;
@@ -462,11 +490,12 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   auto val = [&](int l) -> const T* { return f64 ? (const T*)lv.val64[l] : (const T*)lv.val32[l]; };
   auto dia = [&](int l) -> const T* { return f64 ? (const T*)lv.diag64[l] : (const T*)lv.diag32[l]; };
   const dim3 blk(256);
-  DGMI_HIP(hipMemsetAsync(A(0), 0, sizeof(T) * lv.n[0], s));
+  const dim3 gcp((unsigned)((lv.n[0] + 255) / 256));
+  hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)nullptr, A(0));
   // down-cycle on grid-sized levels
   for (int l = 0; l < l0; ++l) {
     const int n = (int)lv.grid_rows[l];
-    const dim3 grd((n + 255) / 256);
+    const dim3 grd((unsigned)(((int64_t)n * ROW_LANES + 255) / 256));
     T* cur = A(l);
     T* nxt = B(l);
     for (int it = 0; it < 3; ++it) {
@@ -476,23 +505,23 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
     }
     // after 3 sweeps cur == B(l); 4th sweep writes A(l) and the restricted r, e
     const int nc = (n - 1) / 2 + 1;
-    const dim3 grc((nc + 255) / 256);
+    const dim3 grc((unsigned)(((int64_t)nc * 2 * ROW_LANES + 255) / 256));
     hipLaunchKernelGGL((k_sweep_restrict<T>), grc, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l),
                        dia(l), R(l), cur, nxt, R(l + 1), A(l + 1), omega);
   }
   // the level tables point level 0 at internal buffers, so components that start at
   // level 0 see the caller's vectors through them
   if (lv.any_start0)
-    DGMI_HIP(hipMemcpyAsync(lv.r[0], r_sfc, sizeof(T) * lv.n[0], hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], r_sfc, vec<T>(lv.r[0]));
   hipLaunchKernelGGL((k_small_levels_lds<T>), dim3(c->ncomp), dim3(1024), f64 ? lv.lds64 : lv.lds32,
                      s, (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32), (const int32_t*)lv.rng,
                      omega, 3, 3);
   if (lv.any_start0)
-    DGMI_HIP(hipMemcpyAsync(e_sfc, lv.e2[0], sizeof(T) * lv.n[0], hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)vec<T>(lv.e2[0]), e_sfc);
   // up-cycle on grid-sized levels
   for (int l = l0 - 1; l >= 0; --l) {
     const int n = (int)lv.grid_rows[l];
-    const dim3 grd((n + 255) / 256);
+    const dim3 grd((unsigned)(((int64_t)n * ROW_LANES + 255) / 256));
     // (A + P B_{l+1}) -> B, B -> A, A -> B
     hipLaunchKernelGGL((k_sweep<T>), grd, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l), dia(l),
                        R(l), A(l), (const T*)B(l + 1), B(l), omega);
@@ -505,6 +534,41 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   return 0;
 }
 
+// The cycle is a fixed sequence of ~7 launches per grid level plus one; it is captured
+// once per (precision, r, e, omega) into a graph on a private stream and replayed on the
+// caller's stream.
+template <typename T>
+int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
+  auto& mg = c->mg;
+  if (!mg.use_graph) return vcycle_body<T>(c, r_sfc, e_sfc, s);
+  VGraph& vg = std::is_same<T, double>::value ? mg.graph64 : mg.graph32;
+  if (!vg.exec || vg.r != (const void*)r_sfc || vg.e != (void*)e_sfc || vg.omega != c->omega) {
+    if (vg.exec) {
+      DGMI_HIP(hipGraphExecDestroy(vg.exec));
+      vg.exec = nullptr;
+    }
+    if (!mg.capture_stream)
+      DGMI_HIP(hipStreamCreateWithFlags(&mg.capture_stream, hipStreamNonBlocking));
+    DGMI_HIP(hipStreamBeginCapture(mg.capture_stream, hipStreamCaptureModeRelaxed));
+    const int rc = vcycle_body<T>(c, r_sfc, e_sfc, mg.capture_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(mg.capture_stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    DGMI_HIP(ec);
+    const hipError_t ei = hipGraphInstantiate(&vg.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    DGMI_HIP(ei);
+    vg.r = r_sfc;
+    vg.e = e_sfc;
+    vg.omega = c->omega;
+  }
+  DGMI_HIP(hipGraphLaunch(vg.exec, s));
+  return 0;
+}
+
 // Device layout of every component's SFC hierarchy (see SfcLevelsDev) and the level
 // tables of the single-work-group kernel.
 int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
@@ -513,7 +577,7 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
   const int nc = (int)lay.lh.size();
   const int S = lay.smax;
   auto fail = [&]() {
-    set_error("device allocation failed");
+    set_error(alloc_failure());
     return -3;
   };
   // ---- grid levels 0..S

@@ -188,6 +188,17 @@ class DGDiffusion:
         check(self.lib.dgmi_set_param(self.ctx, b"f32_kernel", float(F32_KERNELS[name])),
               "dgmi_set_param")
 
+    def autotune(self, reps=5):
+        """Time the FP32 apply kernels (vector; matrix cores with 32/64/128 elements per
+        work-group, fp32 and bf16 inputs) on this mesh and keep the fastest.  Returns the
+        choice and the timings in microseconds."""
+        ch = (C.c_int32 * 3)()
+        tm = (C.c_double * 7)()
+        check(self.lib.dgmi_autotune(self.ctx, reps, ch, tm, _stream()), "dgmi_autotune")
+        names = ["vector", "mfma_w2", "mfma_w4", "mfma_w8", "bf16_w2", "bf16_w4", "bf16_w8"]
+        return {"f32_kernel": ["vector", "mfma"][ch[0]], "mfma_waves": ch[1],
+                "bf16_waves": ch[2], "us": dict(zip(names, list(tm)))}
+
     def apply(self, x, b=None, out=None, bf16=False):
         """out = b - A x (b None: out = -A x) in the precision of x.  ``bf16`` (FP32 x only)
         rounds the contraction inputs to bf16 and accumulates in fp32."""

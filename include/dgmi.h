@@ -131,6 +131,13 @@ int dgmi_precond(dgmi_ctx* ctx, int dtype, const void* b, void* x, void* stream)
 int dgmi_gmres_solve(dgmi_ctx* ctx, int inner_dtype, const double* b, double* x, double rtol,
                      int m, int max_its, double* stats, void* stream);
 
+/* Autotuner of the FP32 apply: times (reps launches each, on this context's mesh) the
+ * vector kernels and the matrix-core kernels with 2, 4 and 8 waves (32, 64, 128 elements)
+ * per work-group, fp32 and bf16-input, and keeps the fastest for this context.
+ * choice[3] = f32 kernel family (0 vector, 1 MFMA), fp32 and bf16 waves per work-group;
+ * times_us[7] = vector, fp32 x {2,4,8}, bf16 x {2,4,8}.  No counterpart in the reference. */
+int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, void* stream);
+
 /* Solver parameters by name:
  *   "gmres_orth"      0 = modified Gram-Schmidt, as the reference
  *                     (z02/volume_mf_diff.py:1813-1816); 1 = classical Gram-Schmidt with
@@ -138,7 +145,11 @@ int dgmi_gmres_solve(dgmi_ctx* ctx, int inner_dtype, const double* b, double* x,
  *   "inner_flexible"  FP32/BF16 refinement cycles: 1 = flexible (right-preconditioned)
  *                     GMRES (default), 0 = left-preconditioned like the FP64 solve.
  *   "f32_kernel"      kernels of FP32 applies: 0 = vector FMA, 1 = MFMA fp32 (default),
- *                     2 = MFMA with bf16 inputs. */
+ *                     2 = MFMA with bf16 inputs.
+ *   "graph_vcycle"    1 = replay the V-cycle from a captured HIP graph (default), 0 = launch
+ *                     its kernels one by one.
+ *   "mfma_waves", "bf16_waves"  waves (16 elements each) per work-group of the fp32 / bf16
+ *                     matrix-core applies: 2, 4 (default) or 8. */
 int dgmi_set_param(dgmi_ctx* ctx, const char* name, double value);
 
 /* Krylov vector primitives on device vectors of n FP64 or FP32 values. */

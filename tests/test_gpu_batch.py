@@ -33,7 +33,7 @@ def batch(request):
     bt = DGBatch([p.mesh for p in pbs], [p.kappa for p in pbs])
     alone = []
     for p in pbs:
-        g = DGDiffusion.from_mesh(p.mesh, p.kappa, face_kappa="harmonic")
+        g = DGDiffusion.from_mesh(p.mesh, p.kappa, face_kappa="harmonic", blk_solver="direct")
         g._get_RAR_and_sfc_data_Um()
         alone.append(g)
     return pbs, bt, alone

@@ -80,3 +80,26 @@ def test_grid_level_kernels_match_oracle(monkeypatch):
     b = np.random.default_rng(7).standard_normal(sp.ndof)
     ref = mg.um_left_precond(sp, mgd, b)
     assert rel(to_np(g._um_left_precond(g.new_vector(), to_dev(b))), ref) < 1e-10
+
+
+def test_graph_vcycle_replays_equal_eager(prob):
+    """The preconditioner's V-cycle is replayed from a captured graph: several replays, with
+    eager work and synchronisations between them, equal the eagerly launched cycle."""
+    m, pb, g, sp, mgd = prob
+    rng = np.random.default_rng(31)
+    for dt in (torch.float64, torch.float32):
+        outs = {}
+        for mode in (0.0, 1.0):
+            g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", mode)
+            res = []
+            for k in range(3):
+                b = to_dev(rng.standard_normal(sp.ndof), dt) if mode == 0.0 else outs[0.0][k][0]
+                x = torch.zeros(sp.ndof, dtype=dt, device="cuda:0")
+                g._um_left_precond(x, b)
+                torch.cuda.synchronize()
+                junk = torch.randn(1000, device="cuda:0").sum()  # eager work between replays
+                res.append((b, x.clone(), float(junk)))
+            outs[mode] = res
+        for (b0, x0, _), (b1, x1, _) in zip(outs[0.0], outs[1.0]):
+            assert torch.equal(x0, x1), dt
+    g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", 1.0)

@@ -201,3 +201,31 @@ def test_unknown_f32_kernel_raises(prob):
         g.set_f32_kernel("tensor")
     with pytest.raises(ValueError):
         g.apply(to_dev(np.zeros(sp.ndof)), bf16=True)  # bf16 inputs need float32 vectors
+
+
+@pytest.mark.parametrize("waves", [2, 4, 8])
+def test_mfma_tile_shapes_match_oracle(prob, waves):
+    m, pb, g, sp = prob
+    x = np.random.default_rng(24).standard_normal(sp.ndof)
+    ref = dg.apply_A(sp, x)
+    g.lib.dgmi_set_param(g.ctx, b"mfma_waves", float(waves))
+    g.lib.dgmi_set_param(g.ctx, b"bf16_waves", float(waves))
+    try:
+        e32 = rel(-to_np(g.apply(to_dev(x, torch.float32))), ref)
+        e16 = rel(-to_np(g.apply(to_dev(x, torch.float32), bf16=True)), ref)
+    finally:
+        g.lib.dgmi_set_param(g.ctx, b"mfma_waves", 4.0)
+        g.lib.dgmi_set_param(g.ctx, b"bf16_waves", 4.0)
+    print("waves", waves, "f32", e32, "bf16", e16, flush=True)
+    assert e32 < 1e-6 and e16 < 2e-2
+
+
+def test_autotune_keeps_a_correct_kernel(prob):
+    m, pb, g, sp = prob
+    choice = g.autotune(reps=3)
+    print("autotune", choice, flush=True)
+    assert choice["f32_kernel"] in ("vector", "mfma") and len(choice["us"]) == 7
+    assert all(v > 0 for v in choice["us"].values())
+    x = np.random.default_rng(25).standard_normal(sp.ndof)
+    assert rel(-to_np(g.apply(to_dev(x, torch.float32))), dg.apply_A(sp, x)) < 1e-6
+    g.set_f32_kernel("mfma")
