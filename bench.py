@@ -7,9 +7,11 @@ interior vertices jittered +-0.15h (seed 0), lognormal diffusivity per element
 full solve ||b - A x|| <= 1e-10 ||b|| from x = 0 with GMRES(30) + SFC multigrid, inputs
 resident in HBM.
 
-Prints one JSON line: solves/s (FP64 solver) as `value`, latency p50/p99, the
-mixed-precision solver's solves/s, operator-apply GDoF/s in FP64 and FP32, the roofline
-of the apply, and the CPU oracle baseline.
+Prints one JSON line: solves/s of the mixed-precision solver (FP32 flexible GMRES cycles
+inside FP64 iterative refinement to ||r||/||b|| <= 1e-10) as `value`, its latency p50/p99,
+the plain FP64 solver and the bf16-preconditioned variant beside it, operator-apply GDoF/s
+per kernel family, the roofline of the FP32 apply (and of the FP64 one), and the CPU oracle
+baseline.
 
 Other workloads (``--workload``), each printing its own JSON line:
   precision  48^3 x 6 tets (663,552 elements), sigma 2: operator error of the FP32 and
@@ -53,7 +55,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=24, help="hex cells per direction")
+    ap.add_argument("--n", type=int, default=24, help="cells per direction")
+    ap.add_argument("--dim", type=int, default=3, choices=[2, 3],
+                    help="2: P3 triangles on the unit square (n x n x 2, jitter 0.2h)")
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--applies", type=int, default=50)
     ap.add_argument("--face-kappa", default="harmonic", choices=["harmonic", "mean"],
@@ -94,13 +98,16 @@ def main():
 
     dev = "cuda:0"
     t0 = time.time()
-    mesh = meshgen.cube_mesh(args.n, jitter=0.15, seed=0)
+    mesh = meshgen.cube_mesh(args.n, jitter=0.15, seed=0) if args.dim == 3 else \
+        meshgen.square_mesh(args.n, jitter=0.2, seed=0)
     pb = meshgen.sine_problem(mesh, sigma=args.sigma, kappa_seed=1)
     g = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa=args.face_kappa,
                               blk_solver=args.smoother)
     g._get_RAR_and_sfc_data_Um()
+    tuned = g.autotune(reps=5)  # FP32 kernel family and tile shape for this mesh
     torch.cuda.synchronize()
     setup_s = time.time() - t0
+    print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     print(f"[bench] setup done in {setup_s:.1f}s: nele={g.nele} ndof={g.ndof} "
           f"levels={g.nlevel} colours={g.ncolor}", file=sys.stderr, flush=True)
 
@@ -122,7 +129,7 @@ def main():
             g.apply(x, bb, out=r, bf16=bf16)
         e1.record(stream)
         torch.cuda.synchronize()
-        g.set_f32_kernel("mfma")
+        g.set_f32_kernel(tuned["f32_kernel"])  # back to the autotuner's choice
         return e0.elapsed_time(e1) / 1e3 / k
 
     t_apply64 = time_applies(torch.float64, args.applies)
@@ -188,7 +195,7 @@ def main():
               flush=True)
 
     nele, nloc = g.nele, g.nloc
-    F = flops_per_elem(3, nloc, g.ngi, g.sngi, g.nface)
+    F = flops_per_elem(args.dim, nloc, g.ngi, g.sngi, g.nface)
     achieved = F * nele / t_apply64
     gdofs64 = nele * nloc / t_apply64 / 1e9
     gdofs32 = nele * nloc / t_apply32 / 1e9
@@ -223,47 +230,65 @@ def main():
                          f"GDoF/s), scaled by the {applies_per_solve:.0f} applies one GPU "
                          f"solve performs"}
 
+    # headline: the mixed-precision solve (FP32 flexible GMRES cycles inside FP64 iterative
+    # refinement, FP64 true residual to 1e-10); the plain FP64 solver is reported beside it
+    sps32 = len(lat32) / sum(lat32)
+    sps64 = len(lat64) / total64
+    vec32 = tuned["f32_kernel"] == "vector"
+    t32 = t_apply32v if vec32 else t_apply32
+    ach32 = F * nele / t32
+    # calibrated HBM bytes per apply of this workload (FETCH_SIZE x 2 + WRITE_SIZE, two
+    # rocprofv3 --pmc passes, profiles/pmc_bytes_apply_tet24.md)
+    pmc = {"f64": 323e6, "f32_vector": 166e6, "f32_mfma": 125e6}
+    on_profiled = args.dim == 3 and args.n == 24
     out = {
         "metric": "solves/s + p50/p99 latency at rtol 1e-10; DG apply GDoF/s, mixed vs FP64",
-        "value": len(lat64) / total64,
+        "value": sps32,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * total64 / len(lat64),
+        "ms_per_step": 1e3 * sum(lat32) / len(lat32),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"3D IP-DG P3 tets, {args.n}^3x6 = {nele} elements, "
+        "config": {"workload": (f"3D IP-DG P3 tets, {args.n}^3x6" if args.dim == 3 else
+                                f"2D IP-DG P3 triangles, {args.n}^2x2") + f" = {nele} elements, "
                                f"{nele*nloc} DOFs, lognormal kappa sigma={args.sigma}, "
                                f"{args.face_kappa}-mean face kappa, {args.smoother} block "
-                               "smoother, GMRES(30)+SFC-MG, FP64, "
-                               f"rtol 1e-10",
+                               "smoother, GMRES(30)+SFC-MG, FP32 flexible cycles in FP64 "
+                               "iterative refinement, rtol 1e-10",
                    "global_batch": 1, "parallelism": "single GPU"},
-        "latency_s": {"p50": float(np.percentile(lat64, 50)), "p99": float(np.percentile(lat64, 99))},
-        "mixed": {"solves_per_s": len(lat32) / sum(lat32),
-                  "p50": float(np.percentile(lat32, 50)), "p99": float(np.percentile(lat32, 99)),
-                  "speedup_vs_fp64": (len(lat32) / sum(lat32)) / (len(lat64) / total64),
-                  "iterations": [s["its"] for s in st32], "outer": [s["outer"] for s in st32],
-                  "relres": max(s["relres"] for s in st32)},
+        "latency_s": {"p50": float(np.percentile(lat32, 50)), "p99": float(np.percentile(lat32, 99))},
+        "iterations": [s["its"] for s in st32], "outer": [s["outer"] for s in st32],
+        "relres": max(s["relres"] for s in st32),
+        "fp64": {"solves_per_s": sps64,
+                 "p50": float(np.percentile(lat64, 50)), "p99": float(np.percentile(lat64, 99)),
+                 "iterations": [s["its"] for s in st64], "relres": max(s["relres"] for s in st64)},
+        "speedup_vs_fp64": sps32 / sps64,
         "mixed_bf16": {"solves_per_s": len(lat16) / sum(lat16),
                        "p50": float(np.percentile(lat16, 50)), "p99": float(np.percentile(lat16, 99)),
-                       "speedup_vs_fp64": (len(lat16) / sum(lat16)) / (len(lat64) / total64),
+                       "speedup_vs_fp64": (len(lat16) / sum(lat16)) / sps64,
                        "iterations": [s["its"] for s in st16], "outer": [s["outer"] for s in st16],
                        "relres": max(s["relres"] for s in st16)},
         "reference_face_kappa_check": ref_form,
-        "fp64_iterations": [s["its"] for s in st64],
-        "fp64_relres": max(s["relres"] for s in st64),
+        "autotune": tuned,
         "apply_gdofs": {"f64": gdofs64, "f32_vector": nele * nloc / t_apply32v / 1e9,
                         "f32_mfma": gdofs32, "bf16_mfma": nele * nloc / t_apply16 / 1e9},
         "setup_s": setup_s,
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
-                     "kernel": "operator apply (k_trace + k_residual), FP64",
-                     "flop_per_elem": F, "elements": nele, "avg_launch_us": t_apply64 * 1e6,
-                     "achieved_GBps_min_traffic": bytes_per_elem(3, nloc, g.nface, 8) * nele / t_apply64 / 1e9},
+        "roofline": {"bound": "mfma", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32,
+                     "traffic": (pmc["f32_vector" if vec32 else "f32_mfma"] if on_profiled else None),
+                     "kernel": "FP32 operator apply (trace + residual, "
+                               + ("vector" if vec32 else "MFMA") + " kernels, autotuned)",
+                     "flop_per_elem": F, "elements": nele, "avg_launch_us": t32 * 1e6},
+        "roofline_fp64": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                          "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                          "traffic": pmc["f64"] if on_profiled else None,
+                          "avg_launch_us": t_apply64 * 1e6,
+                          "achieved_GBps_min_traffic": bytes_per_elem(args.dim, nloc, g.nface, 8) * nele / t_apply64 / 1e9},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

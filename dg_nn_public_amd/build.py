@@ -29,7 +29,8 @@ HIP_FLAGS = [
     "-ffp-contract=fast",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{INCLUDE}", f"-I{CSRC}"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", f"-I{INCLUDE}",
+              f"-I{CSRC}"]
 
 
 def _run(cmd):
@@ -58,7 +59,7 @@ def build(verbose: bool = False) -> str:
             if o.strip():
                 print(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

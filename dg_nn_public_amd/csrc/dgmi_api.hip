@@ -777,10 +777,12 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
                    "components share P1 vertices");
   // Hilbert order of each component's own vertices (local 1-based positions)
   std::vector<std::vector<int64_t>> lsfc(nc);
-  for (int q = 0; q < nc; ++q) {
-    const int64_t v0 = c->comp_voff[q], nv = c->comp_voff[q + 1] - v0;
-    hilbert_order(c->verts.data() + v0 * D, nv, D, lsfc[q]);
-  }
+  parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
+    for (int64_t q = q0; q < q1; ++q) {
+      const int64_t v0 = c->comp_voff[q], nv = c->comp_voff[q + 1] - v0;
+      hilbert_order(c->verts.data() + v0 * D, nv, D, lsfc[q]);
+    }
+  }, 1);
   // restriction lists for a layout: SFC position of every vertex (0-based)
   auto build_transfers = [&](const std::vector<int64_t>& pos, int64_t ncg) -> int {
     const int64_t nv = c->nverts;
@@ -861,7 +863,10 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   const char* ev = std::getenv("DGMI_SMALL_LEVEL_N");  // test hook: force grid levels
   const int64_t small_n = ev ? std::max<int64_t>(1, std::atoll(ev)) : 0;
   mg.comp_levels.assign(nc, {});
-  for (int q = 0; q < nc; ++q) {
+  std::vector<char> too_deep(nc, 0);
+  // components are independent: their levels are built on the host threads
+  parallel_chunks(nc, [&](int, int64_t qa, int64_t qb) {
+  for (int64_t q = qa; q < qb; ++q) {
     const int64_t v0 = c->comp_voff[q], v1 = c->comp_voff[q + 1];
     std::vector<int64_t> fq(v1 - v0 + 1), cq(mg.fina[v1] - mg.fina[v0]);
     std::vector<double> aq(cq.size());
@@ -877,7 +882,10 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     // the coarse end that fits in LDS (120 KiB of the 160 KiB per work-group) runs in
     // one work-group; larger levels run as grid kernels
     int l0 = lds_start_level(h.n, nnz, 120 * 1024);
-    DGMI_CHECK(l0 >= 0, "too many SFC levels");
+    if (l0 < 0) {
+      too_deep[q] = 1;
+      continue;
+    }
     if (small_n)
       for (int l = 0; l < h.nlevel; ++l)
         if (h.n[l] <= small_n) {
@@ -887,6 +895,8 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     lay.start[q] = l0;
     mg.comp_levels[q] = h.n;
   }
+  }, 1);
+  for (int q = 0; q < nc; ++q) DGMI_CHECK(!too_deep[q], "too many SFC levels");
   // final layout: components by start level, descending; segments padded to a multiple
   // of 2^start so that every component's pairs stay inside it (the last one needs none)
   lay.order.resize(nc);
@@ -1024,9 +1034,13 @@ int dgmi_autotune(dgmi_ctx* c, int reps, int32_t* choice, double* times_us, void
     c->f32_kernel = cand[k][0] == 2 ? 1 : cand[k][0];
     c->mf_wpb = cand[k][1];
     c->bf_wpb = cand[k][1];
+    // the smoothing step (apply fused with x += S r) once the smoother exists, else the
+    // plain apply; x = b = 0 keeps every run identical
+    const bool sm = c->smoother_variant >= 0;
     auto run = [&]() {
-      return cand[k][0] == 2 ? launch_apply_mf(c, true, x, b, r, 1, s)
-                             : launch_apply<float>(c, c->g32, x, b, r, s);
+      if (cand[k][0] == 2) return launch_apply_mf(c, true, x, b, r, sm ? 2 : 1, s);
+      return sm ? launch_smooth<float>(c, c->g32, x, b, nullptr, s)
+                : launch_apply<float>(c, c->g32, x, b, r, s);
     };
     rc = run();  // warm-up
     if (rc) break;

@@ -145,19 +145,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_smdot_final(const int32_t* __re
                                                              int64_t hs, double* __restrict__ coef,
                                                              int accumulate,
                                                              const int32_t* __restrict__ active) {
+  // all k sums at once: lane group j (8 lanes) sums column j of the component's partials
+  // in a fixed strided order, then a fixed shuffle tree
+  constexpr int G = RED_THREADS / MDOT_K;  // lanes per column
+  static_assert(G * MDOT_K == RED_THREADS && (G & (G - 1)) == 0, "lane groups");
   const int c = blockIdx.x;
-  const bool on = active[c] != 0;
-  for (int j = 0; j < k; ++j) {
-    double acc = 0.0;
-    for (int b = comp_blk[c] + threadIdx.x; b < comp_blk[c + 1]; b += blockDim.x)
+  const int j = threadIdx.x / G, sub = threadIdx.x % G;
+  double acc = 0.0;
+  if (j < k)
+    for (int b = comp_blk[c] + sub; b < comp_blk[c + 1]; b += G)
       acc += partial[(int64_t)b * MDOT_K + j];
-    acc = block_sum(acc);
-    if (threadIdx.x == 0) {
-      const double v = on ? acc : 0.0;
-      coef[c * MDOT_K + j] = v;
-      if (on) h[c * hs + j] = accumulate ? h[c * hs + j] + v : v;
-    }
-    __syncthreads();
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (j < k && sub == 0) {
+    const bool on = active[c] != 0;
+    const double v = on ? acc : 0.0;
+    coef[c * MDOT_K + j] = v;
+    if (on) h[c * hs + j] = accumulate ? h[c * hs + j] + v : v;
   }
 }
 

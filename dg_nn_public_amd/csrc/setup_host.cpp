@@ -238,42 +238,67 @@ ElementTables make_element(int D) {
   return el;
 }
 
+// elements of every vertex, in increasing element order (CSR)
+static void vertex_elements(const int64_t* cells, int64_t nele, int nb, int64_t nverts,
+                            std::vector<int64_t>& vptr, std::vector<int64_t>& velem) {
+  vptr.assign(nverts + 1, 0);
+  for (int64_t q = 0; q < nele * nb; ++q) vptr[cells[q] + 1]++;
+  for (int64_t v = 0; v < nverts; ++v) vptr[v + 1] += vptr[v];
+  velem.resize(nele * nb);
+  std::vector<int64_t> fill(vptr.begin(), vptr.end() - 1);
+  for (int64_t e = 0; e < nele; ++e)
+    for (int k = 0; k < nb; ++k) velem[fill[cells[e * nb + k]]++] = e;
+}
+
 Topology build_topology(const int64_t* cells, int64_t nele, int D) {
-  const int nf = D + 1;
+  // The neighbour across face (e, f) is the other element holding the face's vertices:
+  // scan the elements of the face's smallest vertex.  The alignment of a matched pair is
+  // read from the pair (lower global face id first) and stored on both sides.
+  const int nf = D + 1, nb = D + 1;
   const int64_t nfaces = nele * nf;
   Topology t;
   t.nbf.assign(nfaces, -1);
   t.alnmt.assign(nfaces, -1);
   t.bctype.assign(nfaces, 0);
-  std::vector<std::array<int64_t, 3>> key(nfaces);
-  std::vector<std::array<int64_t, 3>> verts(nfaces);
-  for (int64_t e = 0; e < nele; ++e)
-    for (int f = 0; f < nf; ++f) {
-      std::array<int64_t, 3> v = {0, 0, 0};
-      for (int i = 0; i < D; ++i)
-        v[i] = cells[e * (D + 1) + (D == 3 ? TET_FACE_VERTS[f][i] : TRI_FACE_VERTS[f][i])];
-      verts[e * nf + f] = v;
-      std::array<int64_t, 3> s = v;
-      std::sort(s.begin(), s.begin() + D);
-      key[e * nf + f] = s;
-    }
-  std::vector<int64_t> order(nfaces);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int64_t a, int64_t b) { return key[a] < key[b]; });
-  for (int64_t i = 0; i + 1 < nfaces; ++i) {
-    int64_t a = order[i], b = order[i + 1];
-    if (key[a] == key[b]) {
-      t.nbf[a] = b;
-      t.nbf[b] = a;
-      int al;
-      if (verts[a][0] == verts[b][0]) al = 0;
-      else if (verts[a][1] == verts[b][0]) al = 1;
-      else al = 2;
-      t.alnmt[a] = t.alnmt[b] = al;
-      ++i;
-    }
-  }
+  int64_t nverts = 0;
+  for (int64_t q = 0; q < nele * nb; ++q) nverts = std::max(nverts, cells[q] + 1);
+  std::vector<int64_t> vptr, velem;
+  vertex_elements(cells, nele, nb, nverts, vptr, velem);
+  auto face_verts = [&](int64_t e, int f, int64_t* v) {
+    for (int i = 0; i < D; ++i)
+      v[i] = cells[e * nb + (D == 3 ? TET_FACE_VERTS[f][i] : TRI_FACE_VERTS[f][i])];
+  };
+  parallel_chunks(nele, [&](int, int64_t e0, int64_t e1) {
+    for (int64_t e = e0; e < e1; ++e)
+      for (int f = 0; f < nf; ++f) {
+        int64_t v[3] = {0, 0, 0}, s3[3] = {0, 0, 0};
+        face_verts(e, f, v);
+        std::copy(v, v + D, s3);
+        std::sort(s3, s3 + D);
+        for (int64_t q = vptr[s3[0]]; q < vptr[s3[0] + 1]; ++q) {
+          const int64_t e2 = velem[q];
+          if (e2 == e) continue;
+          int hit = 0;
+          for (int k = 0; k < nb; ++k)
+            for (int i = 1; i < D; ++i) hit += cells[e2 * nb + k] == s3[i];
+          if (hit != D - 1) continue;
+          for (int f2 = 0; f2 < nf; ++f2) {
+            int64_t w[3] = {0, 0, 0}, ws[3] = {0, 0, 0};
+            face_verts(e2, f2, w);
+            std::copy(w, w + D, ws);
+            std::sort(ws, ws + D);
+            if (!std::equal(ws, ws + D, s3)) continue;
+            const int64_t z = e * nf + f, z2 = e2 * nf + f2;
+            const int64_t* va = z < z2 ? v : w;
+            const int64_t* vb = z < z2 ? w : v;
+            t.nbf[z] = z2;
+            t.alnmt[z] = va[0] == vb[0] ? 0 : (va[1] == vb[0] ? 1 : 2);
+            break;
+          }
+          break;
+        }
+      }
+  });
   for (int64_t z = 0; z < nfaces; ++z) t.bctype[z] = t.nbf[z] >= 0 ? -1 : 0;
   return t;
 }
@@ -293,7 +318,12 @@ bool build_geometry(const ElementTables& el, const double* x, const double* kapp
   g.max_affine_dev = 0.0;
   double swsum = 0.0;
   for (double w : el.sweight) swsum += w;
-  for (int64_t e = 0; e < nele; ++e) {
+  // elements are independent: chunks per host thread, per-thread deviation and error
+  const int nth = host_threads();
+  std::vector<double> tdev(nth, 0.0);
+  std::vector<int64_t> tbad(nth, -1);
+  parallel_chunks(nele, [&](int tid, int64_t e0, int64_t e1) {
+  for (int64_t e = e0; e < e1; ++e) {
     const double* xe = x + e * D * nloc;  // (D, nloc)
     double v[4][3] = {{0}};
     for (int k = 0; k < nb; ++k)
@@ -310,7 +340,7 @@ bool build_geometry(const ElementTables& el, const double* x, const double* kapp
         double exp_x = 0.0;
         for (int k = 0; k < nb; ++k) exp_x += el.p1_prolong[n * nb + k] * v[k][j];
         double dev = std::fabs(xe[j * nloc + n] - exp_x) / (hscale > 0 ? hscale : 1.0);
-        g.max_affine_dev = std::max(g.max_affine_dev, dev);
+        tdev[tid] = std::max(tdev[tid], dev);
       }
     double det, inv[3][3];
     if (D == 3) {
@@ -334,8 +364,8 @@ bool build_geometry(const ElementTables& el, const double* x, const double* kapp
       inv[1][1] = J[0][0] / det;
     }
     if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
-      err = "degenerate element " + std::to_string(e);
-      return false;
+      if (tbad[tid] < 0) tbad[tid] = e;
+      continue;
     }
     g.detJ[e] = det;
     const double ke = kappa[e];
@@ -405,26 +435,47 @@ bool build_geometry(const ElementTables& el, const double* x, const double* kapp
       g.wf[f * nele + e] = kf * sdet;
     }
   }
+  });
+  for (int t = 0; t < nth; ++t) {
+    g.max_affine_dev = std::max(g.max_affine_dev, tdev[t]);
+    if (tbad[t] >= 0) {
+      err = "degenerate element " + std::to_string(tbad[t]);
+      return false;
+    }
+  }
   return true;
 }
 
 void p1cg_sparsity(const int64_t* cells, int64_t nele, int D, int64_t nverts,
                    std::vector<int64_t>& fina, std::vector<int64_t>& cola) {
+  // row i = sorted distinct vertices of the elements around vertex i
   const int nb = D + 1;
-  std::vector<std::pair<int64_t, int64_t>> pr;
-  pr.reserve((size_t)nele * nb * nb);
-  for (int64_t e = 0; e < nele; ++e)
-    for (int a = 0; a < nb; ++a)
-      for (int b = 0; b < nb; ++b) pr.emplace_back(cells[e * nb + a], cells[e * nb + b]);
-  std::sort(pr.begin(), pr.end());
-  pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+  std::vector<int64_t> vptr, velem;
+  vertex_elements(cells, nele, nb, nverts, vptr, velem);
+  auto row = [&](int64_t v, std::vector<int64_t>& buf) {
+    buf.clear();
+    for (int64_t q = vptr[v]; q < vptr[v + 1]; ++q)
+      for (int k = 0; k < nb; ++k) buf.push_back(cells[velem[q] * nb + k]);
+    std::sort(buf.begin(), buf.end());
+    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+  };
   fina.assign(nverts + 1, 0);
-  cola.resize(pr.size());
-  for (size_t i = 0; i < pr.size(); ++i) {
-    fina[pr[i].first + 1]++;
-    cola[i] = pr[i].second;
-  }
-  for (int64_t i = 0; i < nverts; ++i) fina[i + 1] += fina[i];
+  parallel_chunks(nverts, [&](int, int64_t a, int64_t b) {
+    std::vector<int64_t> buf;
+    for (int64_t v = a; v < b; ++v) {
+      row(v, buf);
+      fina[v + 1] = (int64_t)buf.size();
+    }
+  });
+  for (int64_t v = 0; v < nverts; ++v) fina[v + 1] += fina[v];
+  cola.resize(fina[nverts]);
+  parallel_chunks(nverts, [&](int, int64_t a, int64_t b) {
+    std::vector<int64_t> buf;
+    for (int64_t v = a; v < b; ++v) {
+      row(v, buf);
+      std::copy(buf.begin(), buf.end(), cola.begin() + fina[v]);
+    }
+  });
 }
 
 int color2(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,

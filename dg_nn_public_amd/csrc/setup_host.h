@@ -3,11 +3,42 @@
 // Pure C++ (no HIP); compiled with -ffp-contract=off so that the SFC quantisation and
 // the tables evaluate in the same IEEE operation order as the Python oracle.
 #pragma once
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace dgmi {
+
+// Host threads used by the set-up (DGMI_HOST_THREADS overrides; at most 16).
+inline int host_threads() {
+  static const int n = [] {
+    if (const char* ev = std::getenv("DGMI_HOST_THREADS")) return std::max(1, std::atoi(ev));
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, hc));
+  }();
+  return n;
+}
+
+// fn(t, begin, end) over contiguous chunks of [0, n), one per thread; small ranges run
+// inline.
+template <typename F>
+void parallel_chunks(int64_t n, F fn, int64_t min_chunk = 4096) {
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), n / min_chunk));
+  if (nt <= 1) {
+    fn(0, (int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; ++t) {
+    const int64_t a = n * t / nt, b = n * (t + 1) / nt;
+    th.emplace_back([=, &fn] { fn(t, a, b); });
+  }
+  for (auto& x : th) x.join();
+}
 
 // Reference P3 element (the reference's SHATRInew outputs for p = 3, quadrature
 // degree 6: z02/shape_function.py:459-1317, sizes z02/function_space.py:32-66).

@@ -133,7 +133,8 @@ int dgmi_gmres_solve(dgmi_ctx* ctx, int inner_dtype, const double* b, double* x,
 
 /* Autotuner of the FP32 apply: times (reps launches each, on this context's mesh) the
  * vector kernels and the matrix-core kernels with 2, 4 and 8 waves (32, 64, 128 elements)
- * per work-group, fp32 and bf16-input, and keeps the fastest for this context.
+ * per work-group, fp32 and bf16-input, and keeps the fastest for this context.  Times the
+ * smoothing step once the smoother is set up, the plain apply before.
  * choice[3] = f32 kernel family (0 vector, 1 MFMA), fp32 and bf16 waves per work-group;
  * times_us[7] = vector, fp32 x {2,4,8}, bf16 x {2,4,8}.  No counterpart in the reference. */
 int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, void* stream);
