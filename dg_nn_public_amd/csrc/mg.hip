@@ -281,22 +281,37 @@ __global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* _
       }
   }
   __syncthreads();
-  auto row = [&](const LdsLevel<T>& t, int i, const T* ein, const T* ec) {
+  // a level of n rows gets G lanes per row (a power of two, at most 16, as many as the
+  // work-group can give every row in one pass): the entries of a row are loaded side by
+  // side and summed with shuffles, as in the grid kernels
+  auto lanes_for = [&](int n) {
+    int g = 1;
+    while (g < 16 && 2 * g * n <= nth) g *= 2;
+    return g;
+  };
+  auto row = [&](const LdsLevel<T>& t, int i, int sub, int G, const T* ein, const T* ec) {
     T s = 0;
-    for (int q = t.rp[i]; q < t.rp[i + 1]; ++q) {
+    for (int q = t.rp[i] + sub; q < t.rp[i + 1]; q += G) {
       const int j = t.col[q];
       T ej = ein[j];
       if (ec) ej += ec[j >> 1];
       s += t.val[q] * ej;
     }
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
   };
   auto sweep = [&](const LdsLevel<T>& t, const T* ein, const T* ec, T* eout) {
-    for (int i = tid; i < t.n; i += nth) {
-      const T ax = row(t, i, ein, ec);
-      T ei = ein[i];
-      if (ec) ei += ec[i >> 1];
-      eout[i] = ei + omega * (t.r[i] - ax) / t.diag[i];
+    const int G = lanes_for(t.n), R = nth / G;
+    for (int base = 0; base < t.n; base += R) {
+      const int i = base + tid / G, sub = tid % G;
+      if (i < t.n) {  // uniform over a lane group
+        const T ax = row(t, i, sub, G, ein, ec);
+        if (sub == 0) {
+          T ei = ein[i];
+          if (ec) ei += ec[i >> 1];
+          eout[i] = ei + omega * (t.r[i] - ax) / t.diag[i];
+        }
+      }
     }
     __syncthreads();
   };
@@ -316,20 +331,29 @@ __global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* _
       __syncthreads();
       cur = t.B;
     }
-    for (int I = tid; I < tc.n; I += nth) {
-      T rs = 0, es = 0;
-      for (int h = 0; h < 2; ++h) {
-        const int i = 2 * I + h;
-        if (i < t.n) {
-          const T rr = t.r[i] - row(t, i, cur, (const T*)nullptr);
-          const T en = cur[i] + omega * rr / t.diag[i];
-          t.A[i] = en;
-          rs += rr;
-          es += en;
+    // fused residual + sweep + pairwise restriction: 2G lanes per coarse row, one group
+    // of G per fine row of its pair
+    {
+      const int G = lanes_for(t.n), R = nth / (2 * G);
+      for (int base = 0; base < tc.n; base += R) {
+        const int I = base + tid / (2 * G);
+        const int h = (tid / G) & 1, sub = tid % G;
+        if (I < tc.n) {  // uniform over the 2G lanes of the pair
+          const int i = 2 * I + h;
+          T rr = 0, en = 0;
+          if (i < t.n) {  // uniform over a lane group
+            rr = t.r[i] - row(t, i, sub, G, cur, (const T*)nullptr);
+            en = cur[i] + omega * rr / t.diag[i];
+            if (sub == 0) t.A[i] = en;
+          }
+          const T rs = rr + __shfl_xor(rr, G, 64);
+          const T es = en + __shfl_xor(en, G, 64);
+          if (h == 0 && sub == 0) {
+            tc.r[I] = rs;
+            tc.A[I] = es;
+          }
         }
       }
-      tc.r[I] = rs;
-      tc.A[I] = es;
     }
     __syncthreads();
   }
