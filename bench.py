@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--smoother", default="direct", choices=["direct", "jacobian"],
                     help="block smoother: exact block inverse (the reference's variable-"
                          "diffusivity solver) or 3 Jacobi sweeps (its constant-diffusivity one)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the SFC V-cycle kernels one by one instead of replaying a "
+                         "captured graph")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
     ap.add_argument("--workload", default="solve",
@@ -104,6 +107,8 @@ def main():
     g = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa=args.face_kappa,
                               blk_solver=args.smoother)
     g._get_RAR_and_sfc_data_Um()
+    if args.no_graph:
+        g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", 0.0)
     tuned = g.autotune(reps=5)  # FP32 kernel family and tile shape for this mesh
     torch.cuda.synchronize()
     setup_s = time.time() - t0
@@ -140,6 +145,15 @@ def main():
           f"f32 mfma {t_apply32*1e6:.1f} us, bf16 mfma {t_apply16*1e6:.1f} us",
           file=sys.stderr, flush=True)
     if args.apply_only:
+        # FP32 preconditioner V-cycles with each kernel family (smoothing kernels for the
+        # counter passes)
+        b32 = b.float()
+        for kern in ("vector", "mfma", "mfma_bf16"):
+            g.set_f32_kernel(kern)
+            for _ in range(3):
+                g._um_left_precond(g.new_vector(torch.float32), b32)
+        g.set_f32_kernel(tuned["f32_kernel"])
+        torch.cuda.synchronize()
         # calibration: elementwise copy of 256 MiB (reads 256 MiB, writes 256 MiB)
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)

@@ -16,7 +16,10 @@
 //                 the neighbour's trace at the aligned points, lifted to the test
 //                 functions; writes r = b - A x, or applies the smoother x += S r.
 // MODE 0: r = -A x; 1: r = b - A x; 2: x += S (b - A x); 3: as 2 and also store r;
-// 4: r = +A x.
+// 4: r = +A x; 5: x = (1 - omega) x + S (b - A_off x), where A_off is A without its
+// diagonal blocks.  Mode 5 equals mode 2 when S = omega A_ee^-1 (the exact block
+// smoother): it needs no volume term and only the neighbour half of the face terms.
+#include "dgmi.h"
 #include "dgmi_internal.h"
 #include "rowvec.h"
 
@@ -78,9 +81,10 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
     const T* __restrict__ wf, const T* __restrict__ pen, const int32_t* __restrict__ nbface,
     const int32_t* __restrict__ fcode, const typename Vec2<T>::type* __restrict__ trace,
-    const T* __restrict__ S, const int32_t* __restrict__ act) {
+    const T* __restrict__ S, const int32_t* __restrict__ act, T omega) {
   using M = Dm<D>;
   if (act && !act[blockIdx.x]) return;  // whole block: no barrier is reached
+  constexpr bool OFFD = MODE == 5;
   constexpr int NL = M::NL, NF = M::NF;
   constexpr int GPW = M::NGI / NF;          // volume points per wave
   constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
@@ -95,6 +99,8 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   for (int n = 0; n < NL; ++n) acc[n] = 0;
   if (valid) {
     load_row<T, NL>(x + (size_t)e * NL, xv);
+  }
+  if (valid && !OFFD) {
     // ---- volume points of this wave
     T gm[M::NG];
 #pragma unroll
@@ -128,12 +134,14 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
         acc[n] = s;
       }
     }
-    // ---- face w
+  }
+  if (valid) {
+    // ---- face w (mode 5: interior faces, neighbour terms only)
     const int f = w;
     const int z = e * NF + f;
     const int code = fcode[z];
     const int type = code & 15;
-    if (type < 2) {
+    if (OFFD ? type == 0 : type < 2) {
       const int a = code >> 4;
       T c[D];
 #pragma unroll
@@ -149,16 +157,22 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
       const T* tf = ftab + f * M::SG * (1 + D) * NL;
 #pragma unroll 2
       for (int g = 0; g < M::SG; ++g) {
-        const typename Vec2<T>::type tr = trace[(f * M::SG + g) * nele + e];
         const T ws = wfe * sw[g];
         T qsn, qdn;
-        if (type == 0) {
+        if constexpr (OFFD) {
+          const int ga = aln[a * M::SG + g];
+          const typename Vec2<T>::type tn = trace[(nbf * M::SG + ga) * nele + nbe];
+          qsn = ws * (T(0.5) * tn.y - pe * tn.x);
+          qdn = ws * (T(0.5) * tn.x);
+        } else if (type == 0) {
+          const typename Vec2<T>::type tr = trace[(f * M::SG + g) * nele + e];
           const int ga = aln[a * M::SG + g];
           const typename Vec2<T>::type tn = trace[(nbf * M::SG + ga) * nele + nbe];
           const T jump = tr.x - tn.x;
           qsn = ws * (T(-0.5) * (tr.y - tn.y) + pe * jump);
           qdn = ws * (T(-0.5) * jump);
         } else {
+          const typename Vec2<T>::type tr = trace[(f * M::SG + g) * nele + e];
           qsn = ws * (pe * tr.x - tr.y);
           qdn = -ws * tr.x;
         }
@@ -216,8 +230,8 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
         const int n = w * NPW + k;
         if (n < NL) {
           T srow[NL];
-          load_S_row<T, NL>(S, n, e, nele, srow);
-          T s = xv[n];
+          load_smoother_row<T, NL>(S, n, e, nele, srow);
+          T s = OFFD ? (T(1) - omega) * xv[n] : xv[n];
 #pragma unroll
           for (int m = 0; m < NL; ++m) s += srow[m] * red[0][m][lane];
           x[(size_t)e * NL + n] = s;
@@ -241,7 +255,7 @@ __global__ __launch_bounds__(256) void k_smooth_zero(int nele, T* __restrict__ x
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
     T srow[NL];
-    load_S_row<T, NL>(S, n, e, nele, srow);
+    load_smoother_row<T, NL>(S, n, e, nele, srow);
     T s = 0;
 #pragma unroll
     for (int m = 0; m < NL; ++m) s += srow[m] * bv[m];
@@ -399,13 +413,12 @@ __global__ __launch_bounds__(256) void k_build_S_jacobi(int nele, const double* 
 }
 
 // omega * B^-1 for every element (the exact block smoother of
-// z05/volume_mf_um_diff_varprop.py:384-392), written in both precisions.  One wave per
-// element: Gauss-Jordan elimination with partial pivoting on [B | I] in LDS.  A singular
-// block sets *fail to 1 + its element index.
+// z05/volume_mf_um_diff_varprop.py:384-392), FP64.  One wave per element: Gauss-Jordan
+// elimination with partial pivoting on [B | I] in LDS.  A singular block sets *fail to
+// 1 + its element index.
 template <int NL>
 __global__ __launch_bounds__(256) void k_build_S_direct(int nele, const double* __restrict__ B,
                                                         double omega, double* __restrict__ S64,
-                                                        float* __restrict__ S32,
                                                         int* __restrict__ fail) {
   constexpr int W2 = 2 * NL + 1;  // [B | I] row, padded
   __shared__ double a[4][NL][W2];
@@ -459,11 +472,23 @@ __global__ __launch_bounds__(256) void k_build_S_direct(int nele, const double* 
     if (singular && lane == 0) atomicCAS(fail, 0, 1 + e);
     for (int k = lane; k < NL * NL; k += 64) {
       const int n = k / NL, m = k % NL;
-      const double v = omega * a[w][n][NL + m];
-      S64[s_index<double, NL>(n, m, e, nele)] = v;
-      S32[s_index<float, NL>(n, m, e, nele)] = (float)v;
+      S64[s_index<double, NL>(n, m, e, nele)] = omega * a[w][n][NL + m];
     }
   }
+}
+
+// FP32 packed copy of the FP64 smoother: S32[(a,b)] = (S[a][b] + S[b][a]) / 2, a <= b
+template <int NL>
+__global__ __launch_bounds__(256) void k_pack_S_sym(int nele, const double* __restrict__ S64,
+                                                    float* __restrict__ S32) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nele) return;
+  for (int a = 0; a < NL; ++a)
+    for (int b = a; b < NL; ++b) {
+      const double v = 0.5 * (S64[s_index<double, NL>(a, b, e, nele)] +
+                              S64[s_index<double, NL>(b, a, e, nele)]);
+      S32[(size_t)sym_index<NL>(a, b) * nele + e] = (float)v;
+    }
 }
 
 int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* fail,
@@ -472,10 +497,21 @@ int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* f
   const dim3 blk(256), grd((nele + 3) / 4);
   if (c->D == 3)
     hipLaunchKernelGGL(k_build_S_direct<20>, grd, blk, 0, s, nele, bdiag, omega, c->g64.S,
-                       c->g32.S, fail);
+                       fail);
   else
     hipLaunchKernelGGL(k_build_S_direct<10>, grd, blk, 0, s, nele, bdiag, omega, c->g64.S,
-                       c->g32.S, fail);
+                       fail);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s) {
+  const int nele = (int)c->nele;
+  const dim3 blk(256), grd((nele + 255) / 256);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_pack_S_sym<20>, grd, blk, 0, s, nele, c->g64.S, c->g32.S);
+  else
+    hipLaunchKernelGGL(k_pack_S_sym<10>, grd, blk, 0, s, nele, c->g64.S, c->g32.S);
   DGMI_HIP(hipGetLastError());
   return 0;
 }
@@ -510,12 +546,14 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   const T* sw = g.tab + g.off_sw;
 #define DGMI_RES(MODE)                                                                       \
   hipLaunchKernelGGL((k_residual_w<T, D, MODE>), grd, blk, 0, s, nele, x, b, r, vt, wg, ft, sw, \
-                     c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, g.trace, g.S, act)
+                     c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, g.trace, g.S, act,   \
+                     (T)c->omega)
   switch (mode) {
     case 0: DGMI_RES(0); break;
     case 1: DGMI_RES(1); break;
     case 2: DGMI_RES(2); break;
     case 3: DGMI_RES(3); break;
+    case 5: DGMI_RES(5); break;
     default: DGMI_RES(4); break;
   }
 #undef DGMI_RES
@@ -544,9 +582,13 @@ int launch_apply_pos(dgmi_ctx* c, DevGeom<T>& g, const T* x, T* r, hipStream_t s
   return apply_dispatch<T>(c, g, const_cast<T*>(x), nullptr, r, 4, s);
 }
 
+// FP32 sweeps of the exact block smoother (S = omega A_ee^-1) use the neighbour-only form
+// (mode 5) unless the context turns it off; FP64 keeps the reference's x += S (b - A x).
 template <typename T>
 int launch_smooth(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r_out, hipStream_t s) {
-  return apply_dispatch<T>(c, g, x, b, r_out, r_out ? 3 : 2, s);
+  const bool offd = std::is_same<T, float>::value && !r_out && c->smooth_offdiag &&
+                    c->smoother_variant == DGMI_SMOOTH_DIRECT;
+  return apply_dispatch<T>(c, g, x, b, r_out, offd ? 5 : (r_out ? 3 : 2), s);
 }
 
 template <typename T>
@@ -622,7 +664,6 @@ template int launch_smooth_zero<double>(dgmi_ctx*, DevGeom<double>&, double*, co
 template int launch_smooth_zero<float>(dgmi_ctx*, DevGeom<float>&, float*, const float*,
                                        hipStream_t);
 template int launch_build_S_jacobi<double>(dgmi_ctx*, const double*, double, double*, hipStream_t);
-template int launch_build_S_jacobi<float>(dgmi_ctx*, const double*, double, float*, hipStream_t);
 template int launch_convert<double>(const double*, double*, int64_t, hipStream_t);
 template int launch_convert<float>(const double*, float*, int64_t, hipStream_t);
 

@@ -54,16 +54,27 @@ template <int D> struct MfShape {
 // BF16: the two operand tables in MFMA lane order (16 bytes per lane, conflict free).
 template <typename TIN, int D> struct MfLds;
 
+// Copy 16-byte units [FIRST, LAST) from global memory to LDS with NTH threads.
+template <int NTH, int FIRST, int LAST, typename V>
+__device__ __forceinline__ void stage16(V* __restrict__ dst, const V* __restrict__ src, int tid) {
+  for (int i = FIRST + tid; i < LAST; i += NTH) dst[i] = src[i];
+}
+
 template <int D> struct MfLds<float, D> {
   using M = Mq<D>;
   using S = MfShape<D>;
   static constexpr int R = 16 * S::NT, WORDS = R * M::NL + 16;
-  float T[WORDS];
+  static_assert(WORDS % 4 == 0 && (16 * S::NTV * M::NL) % 4 == 0, "16-byte staging");
+  __align__(16) float T[WORDS];
   int aln[D * M::SG];
+  // FACES: only the face rows (the trace pass uses no volume rows)
+  template <int NTH, bool FACES>
   __device__ __forceinline__ void load(const float* __restrict__ tT, const int32_t* __restrict__ al,
-                                       int tid, int nth) {
-    for (int i = tid; i < WORDS; i += nth) T[i] = tT[i];
-    for (int i = tid; i < D * M::SG; i += nth) aln[i] = al[i];
+                                       int tid) {
+    constexpr int F4 = FACES ? 16 * S::NTV * M::NL / 4 : 0;
+    stage16<NTH, F4, WORDS / 4>(reinterpret_cast<float4*>(T),
+                                reinterpret_cast<const float4*>(tT), tid);
+    for (int i = tid; i < D * M::SG; i += NTH) aln[i] = al[i];
   }
   struct XOp { float v[S::KS]; };
   __device__ __forceinline__ static XOp load_x(const float* __restrict__ x, int e, int q) {
@@ -95,11 +106,16 @@ template <int D> struct MfLds<__bf16, D> {
   bf16x8 t1[N1];
   bf16x8 t2[N2];
   int aln[D * M::SG];
+  // FACES: only the face tiles of t1 (the trace pass lifts nothing)
+  template <int NTH, bool FACES>
   __device__ __forceinline__ void load(const bf16x8* __restrict__ b1, const bf16x8* __restrict__ b2,
-                                       const int32_t* __restrict__ al, int tid, int nth) {
-    for (int i = tid; i < N1; i += nth) t1[i] = b1[i];
-    for (int i = tid; i < N2; i += nth) t2[i] = b2[i];
-    for (int i = tid; i < D * M::SG; i += nth) aln[i] = al[i];
+                                       const int32_t* __restrict__ al, int tid) {
+    using U = uint4;
+    stage16<NTH, FACES ? S::NTV * 64 : 0, N1>(reinterpret_cast<U*>(t1),
+                                              reinterpret_cast<const U*>(b1), tid);
+    if constexpr (!FACES)
+      stage16<NTH, 0, N2>(reinterpret_cast<U*>(t2), reinterpret_cast<const U*>(b2), tid);
+    for (int i = tid; i < D * M::SG; i += NTH) aln[i] = al[i];
   }
   struct XOp { bf16x8 v; };
   __device__ __forceinline__ static XOp load_x(const float* __restrict__ x, int e, int q) {
@@ -177,12 +193,12 @@ struct MfArgs {
   const int32_t* aln;
 };
 
-template <typename TIN, int D>
+template <typename TIN, int D, int NTH, bool FACES>
 __device__ __forceinline__ void mf_load_tables(MfLds<TIN, D>& L, const MfArgs& a) {
   if constexpr (std::is_same<TIN, float>::value)
-    L.load(a.tT, a.aln, threadIdx.x, blockDim.x);
+    L.template load<NTH, FACES>(a.tT, a.aln, threadIdx.x);
   else
-    L.load((const bf16x8*)a.b1, (const bf16x8*)a.b2, a.aln, threadIdx.x, blockDim.x);
+    L.template load<NTH, FACES>((const bf16x8*)a.b1, (const bf16x8*)a.b2, a.aln, threadIdx.x);
 }
 
 template <typename TIN, int D>
@@ -238,7 +254,7 @@ __global__ __launch_bounds__(64 * WPB) void k_trace_mf(int nele, const float* __
   for (int f = 0; f < M::NF; ++f)
 #pragma unroll
     for (int d = 0; d < D; ++d) c[f][d] = cf[(f * D + d) * nele + ec];
-  mf_load_tables<TIN, D>(L, ta);
+  mf_load_tables<TIN, D, 64 * WPB, true>(L, ta);
   __syncthreads();
 #pragma unroll
   for (int f = 0; f < M::NF; ++f) {
@@ -255,7 +271,8 @@ __global__ __launch_bounds__(64 * WPB) void k_trace_mf(int nele, const float* __
 }
 
 // Residual pass.  MODE 0: r = -A x; 1: r = b - A x; 2: x += S (b - A x); 3: as 2 and
-// store r; 4: r = +A x.  b and r may alias.  Every per-element operand and the
+// store r; 4: r = +A x; 5: x = (1 - omega) x + S (b - A_off x) (exact block smoother,
+// neighbour terms only: no volume or own-trace contractions).  b and r may alias.  Every per-element operand and the
 // neighbour traces are requested in the prologue, so they are in flight during the
 // volume contractions.
 template <typename TIN, int D, int MODE, int WPB>
@@ -265,23 +282,30 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
     const float* __restrict__ wf, const float* __restrict__ pen,
     const int32_t* __restrict__ nbface, const int32_t* __restrict__ fcode,
     const float2* __restrict__ trace, const float* __restrict__ Sm,
-    const int32_t* __restrict__ act) {
+    const int32_t* __restrict__ act, float omega) {
   using M = Mq<D>;
   using S = MfShape<D>;
   constexpr int NL = M::NL, NF = M::NF, FT = S::FT;
+  constexpr bool OFFD = MODE == 5;
+  // mode 5 lifts no volume tile; the bf16 lift pairs tiles, so an odd volume count keeps
+  // zero volume tiles for the pairing
+  constexpr bool SKIP_VOL = OFFD && (std::is_same<TIN, float>::value || S::NTV % 2 == 0);
   __shared__ MfLds<TIN, D> L;
   if (act && !mf_block_active<WPB>(act, nele)) return;  // before any barrier
-  __shared__ float rx[(MODE == 2 || MODE == 3) ? WPB : 1][16][NL + 1];
+  __shared__ float rx[(MODE == 2 || MODE == 3 || OFFD) ? WPB : 1][16][NL + 1];
   const int lane = threadIdx.x & 63, q = lane >> 4, wl = threadIdx.x >> 6;
   const int wave = blockIdx.x * WPB + wl;
   const int el = wave * 16 + (lane & 15);
   const bool valid = el < nele;
   const int ec = valid ? el : nele - 1;
   // ---- prologue: element operands
-  const auto xo = MfLds<TIN, D>::load_x(x, ec, q);
-  float gm[M::NG];
+  typename MfLds<TIN, D>::XOp xo{};
+  float gm[M::NG] = {};
+  if constexpr (!OFFD) {
+    xo = MfLds<TIN, D>::load_x(x, ec, q);
 #pragma unroll
-  for (int k = 0; k < M::NG; ++k) gm[k] = G[k * nele + ec];
+    for (int k = 0; k < M::NG; ++k) gm[k] = G[k * nele + ec];
+  }
   int code[NF], nbz[NF];
   float c[NF][D], wfe[NF], pe[NF];
 #pragma unroll
@@ -293,7 +317,7 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
     wfe[f] = wf[f * nele + ec];
     pe[f] = pen[f * nele + ec];
   }
-  mf_load_tables<TIN, D>(L, ta);
+  mf_load_tables<TIN, D, 64 * WPB, false>(L, ta);
   __syncthreads();
   // neighbour traces at the aligned points of this lane's face points
   float2 tn[NF][FT];
@@ -310,10 +334,14 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
   lift.init();
   // ---- volume points
 #pragma unroll
-  for (int t = 0; t < S::NTV; ++t) {
+  for (int t = 0; t < (SKIP_VOL ? 0 : S::NTV); ++t) {
+    v4f z = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (OFFD) {
+      mf_lift<TIN, D>(lift, L, t, lane, z);
+      continue;
+    }
     const v4f y = mf_tile<TIN, D>(L, t, lane, xo);
     const float wq = wv[4 * t + q];
-    v4f z;
     if constexpr (D == 3) {
       z[0] = wq * (gm[0] * y[0] + gm[3] * y[1] + gm[4] * y[2]);
       z[1] = wq * (gm[3] * y[0] + gm[1] * y[1] + gm[5] * y[2]);
@@ -333,13 +361,22 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
 #pragma unroll
     for (int tt = 0; tt < FT; ++tt) {
       const int t = S::NTV + f * FT + tt;
-      const v4f y = mf_tile<TIN, D>(L, t, lane, xo);
-      float dn = 0.f;
-#pragma unroll
-      for (int d = 0; d < D; ++d) dn += c[f][d] * y[1 + d];
       const float ws = wfe[f] * wv[M::NGI + 4 * tt + q];
       float qsn = 0.f, qdn = 0.f;
-      if (type == 0) {
+      v4f y = {0.f, 0.f, 0.f, 0.f};
+      float dn = 0.f;
+      if constexpr (!OFFD) {
+        y = mf_tile<TIN, D>(L, t, lane, xo);
+#pragma unroll
+        for (int d = 0; d < D; ++d) dn += c[f][d] * y[1 + d];
+      }
+      if (OFFD) {
+        if (type == 0) {
+          const float2 nb = tn[f][tt];
+          qsn = ws * (0.5f * nb.y - pe[f] * nb.x);
+          qdn = ws * (0.5f * nb.x);
+        }
+      } else if (type == 0) {
         const float2 nb = tn[f][tt];
         const float jump = y[0] - nb.x;
         qsn = ws * (-0.5f * (dn - nb.y) + pe[f] * jump);
@@ -397,9 +434,9 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
           const int n = 16 * mt + 4 * q + i;
           if (n < NL) {
             float srow[NL];
-            load_S_row<float, NL>(Sm, n, el, nele, srow);
+            load_S_row_sym<NL>(Sm, n, el, nele, srow);
             const size_t k = (size_t)el * NL + n;
-            float s = x[k];
+            float s = OFFD ? (1.0f - omega) * x[k] : x[k];
 #pragma unroll
             for (int m = 0; m < NL; ++m) s += srow[m] * rx[wl][le][m];
             x[k] = s;
@@ -490,12 +527,13 @@ static int apply_mf_impl(dgmi_ctx* c, float* x, const float* b, float* r, int mo
 #define DGMI_RES_MF(MODE)                                                                      \
   hipLaunchKernelGGL((k_residual_mf<TIN, D, MODE, WPB>), grd, blk, 0, s, nele, x, b, r, ta,    \
                      c->mf.wv, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,                     \
-                     (const float2*)g.trace, g.S, act)
+                     (const float2*)g.trace, g.S, act, (float)c->omega)
   switch (mode) {
     case 0: DGMI_RES_MF(0); break;
     case 1: DGMI_RES_MF(1); break;
     case 2: DGMI_RES_MF(2); break;
     case 3: DGMI_RES_MF(3); break;
+    case 5: DGMI_RES_MF(5); break;
     default: DGMI_RES_MF(4); break;
   }
 #undef DGMI_RES_MF

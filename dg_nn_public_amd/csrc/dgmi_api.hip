@@ -250,6 +250,12 @@ struct Krylov {
   int32_t *want = nullptr, *active = nullptr, *kk = nullptr, *nact = nullptr;
   double* hd = nullptr;   // pinned: 2 * ncomp doubles
   int32_t* hi = nullptr;  // pinned: 2 * ncomp + 1 ints
+  // per-iteration progress: status = (active count, res/thr of each active component) on
+  // the device, copied into ring slot j of pinned memory at the end of iteration j
+  double* status = nullptr;  // (1 + ncomp)
+  double* ring = nullptr;    // pinned: (m + 1) * (1 + ncomp)
+  int nring = 0;
+  std::vector<hipEvent_t> ev;  // (m + 1) completion events of the ring slots
 };
 static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
   if (!c->krylov) c->krylov = new Krylov{};
@@ -270,9 +276,10 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     k->active = iv(nc);
     k->kk = iv(nc);
     k->nact = iv(1);
+    k->status = dv(nc + 1);
     if (hipHostMalloc((void**)&k->hd, sizeof(double) * 2 * nc) != hipSuccess) k->hd = nullptr;
     if (hipHostMalloc((void**)&k->hi, sizeof(int32_t) * (2 * nc + 1)) != hipSuccess) k->hi = nullptr;
-    if (!k->beta || !k->nact || !k->hd || !k->hi) return nullptr;
+    if (!k->beta || !k->nact || !k->status || !k->hd || !k->hi) return nullptr;
   }
   const size_t need = (size_t)(m + 1) * c->ndof * elem;
   if (k->m < m || k->vbytes < need) {
@@ -292,6 +299,16 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     k->y = static_cast<double*>(c->alloc(sizeof(double) * k->ys * nc));
     k->m = m;
     if (!k->V || !k->H || !k->cs || !k->sn || !k->g || !k->y) return nullptr;
+    if (k->ring) (void)hipHostFree(k->ring);
+    k->ring = nullptr;
+    if (hipHostMalloc((void**)&k->ring, sizeof(double) * (m + 1) * (nc + 1)) != hipSuccess)
+      return nullptr;
+    while ((int)k->ev.size() < m + 1) {
+      hipEvent_t e;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+      k->ev.push_back(e);
+    }
+    k->nring = m + 1;
   }
   const size_t zneed = (size_t)m * c->ndof * elem;
   if (flexible && k->zbytes < zneed) {
@@ -332,6 +349,17 @@ static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool fl
                              m + 2, k->kk, s));
   if (c->mask_on) DGMI_TRY(kry::expand_active(sg, c->nele, k->active, c->act64, s));
   DGMI_TRY((kry::sscale<T, T>(sg, k->beta, 0, k->active, start, Vj(0), s)));
+  DGMI_CHECK(m + 1 <= k->nring, "progress ring smaller than the restart length");
+  int known = -1;  // last iteration whose progress the host has read
+  double kn_nact = 1.0, kn_ratio = INFINITY;
+  auto read_slot = [&](int q) {
+    const double* st = k->ring + (size_t)q * (nc + 1);
+    kn_nact = st[0];
+    kn_ratio = 0.0;
+    for (int i = 0; i < nc; ++i) kn_ratio = std::max(kn_ratio, st[1 + i]);
+    if (kn_nact == 0) kn_ratio = 0.0;
+    known = q;
+  };
   for (int j = 0; j < m; ++j) {
     if (flexible) {
       DGMI_TRY(precond<T>(c, Vj(j), Zj(j), s));                  // z_j = M^-1 v_j
@@ -345,20 +373,48 @@ static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool fl
       // modified Gram-Schmidt, as the reference (z02/volume_mf_diff.py:1813-1816)
       for (int i = 0; i <= j; ++i)
         DGMI_TRY(kry::sorth_pass<T>(sg, 1, Vj(i), n, w, hcol + i, k->hs, 0, k->active, s));
-    } else {
-      // classical Gram-Schmidt with one re-orthogonalisation (CGS2): two fused passes,
+      DGMI_TRY(kry::sdot<T>(sg, w, w, nullptr, k->hn, s, k->active));
+    } else if (j + 1 <= MDOT_K) {
+      // classical Gram-Schmidt with one re-orthogonalisation (CGS2) and the norm, fused:
       // the Hessenberg column is the sum of both passes' coefficients
+      DGMI_TRY(kry::sorth_cgs2<T>(sg, j + 1, V, n, w, hcol, k->hs, k->hn, k->active, s));
+    } else {
       DGMI_TRY(kry::sorth_pass<T>(sg, j + 1, V, n, w, hcol, k->hs, 0, k->active, s));
       DGMI_TRY(kry::sorth_pass<T>(sg, j + 1, V, n, w, hcol, k->hs, 1, k->active, s));
+      DGMI_TRY(kry::sdot<T>(sg, w, w, nullptr, k->hn, s, k->active));
     }
-    DGMI_TRY(kry::sdot<T>(sg, w, w, nullptr, k->hn, s, k->active));
     DGMI_TRY((kry::sscale<T, T>(sg, k->hn, 0, k->active, w, Vj(j + 1), s)));
     DGMI_TRY(kry::sgivens(nc, j, k->ldh, k->hs, k->H, k->hn, k->cs, k->sn, k->rs, k->g, k->gs,
-                          k->res, k->thr, k->active, k->kk, k->nact, s));
+                          k->res, k->thr, k->active, k->kk, k->nact, k->status, s));
     if (c->mask_on) DGMI_TRY(kry::expand_active(sg, c->nele, k->active, c->act64, s));
-    DGMI_HIP(hipMemcpyAsync(k->hi + 2 * nc, k->nact, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    DGMI_HIP(hipStreamSynchronize(s));
-    if (k->hi[2 * nc] == 0) break;
+    // Progress is read back late: the host queues iteration j+1 while the device runs
+    // iteration j and waits for iteration j itself only when some component could have
+    // reached its threshold in it.  An iteration queued after every component stopped
+    // changes nothing that is used (inactive components keep H, g, kk; kernels skip them
+    // or write slots beyond kk).
+    DGMI_HIP(hipMemcpyAsync(k->ring + (size_t)j * (nc + 1), k->status, sizeof(double) * (nc + 1),
+                            hipMemcpyDeviceToHost, s));
+    DGMI_HIP(hipEventRecord(k->ev[j], s));
+    while (known < j) {
+      const hipError_t q = hipEventQuery(k->ev[known + 1]);
+      if (q == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: clear it for the launch checks
+        break;
+      }
+      DGMI_HIP(q);
+      read_slot(known + 1);
+    }
+    if (known < j - 1) {
+      DGMI_HIP(hipEventSynchronize(k->ev[j - 1]));
+      read_slot(j - 1);
+    }
+    if (known >= 0 && kn_nact == 0) break;
+    // the largest drop of an estimate in one iteration taken as possible: 20x
+    if (known < j && kn_ratio <= 20.0) {
+      DGMI_HIP(hipEventSynchronize(k->ev[j]));
+      read_slot(j);
+      if (kn_nact == 0) break;
+    }
   }
   DGMI_TRY(kry::sbacksolve(nc, k->ldh, k->hs, k->H, k->g, k->gs, k->kk, k->y, k->ys, s));
   DGMI_TRY(kry::scombine<T>(sg, flexible ? Z : V, n, k->y, k->ys, k->kk, d, s));
@@ -470,6 +526,25 @@ static int build_seg(dgmi_ctx* c) {
     for (int64_t a = d0; a < d1; a += SEG_CHUNK) blk.push_back({a, std::min(d1, a + SEG_CHUNK), q, 0});
   }
   cb[c->ncomp] = (int32_t)blk.size();
+  // wide blocks of the Gram-Schmidt kernels
+  std::vector<SegBlk> wb;
+  std::vector<int32_t> wcb(c->ncomp + 1, 0);
+  sg.wvec4 = true;
+  for (int q = 0; q < c->ncomp; ++q) {
+    wcb[q] = (int32_t)wb.size();
+    const int64_t d0 = c->comp_eoff[q] * c->nloc, d1 = c->comp_eoff[q + 1] * c->nloc;
+    for (int64_t a = d0; a < d1; a += GS_CHUNK) {
+      wb.push_back({a, std::min(d1, a + GS_CHUNK), q, 0});
+      sg.wvec4 = sg.wvec4 && a % 4 == 0;
+    }
+  }
+  wcb[c->ncomp] = (int32_t)wb.size();
+  sg.nwblk = (int)wb.size();
+  sg.wblk = static_cast<SegBlk*>(c->alloc(sizeof(SegBlk) * std::max<size_t>(wb.size(), 1)));
+  sg.comp_wblk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * wcb.size()));
+  DGMI_CHECK(sg.wblk && sg.comp_wblk, alloc_failure());
+  DGMI_HIP(hipMemcpy(sg.wblk, wb.data(), sizeof(SegBlk) * wb.size(), hipMemcpyHostToDevice));
+  DGMI_HIP(hipMemcpy(sg.comp_wblk, wcb.data(), sizeof(int32_t) * wcb.size(), hipMemcpyHostToDevice));
   sg.nblk = (int)blk.size();
   sg.blk = static_cast<SegBlk*>(c->alloc(sizeof(SegBlk) * std::max<size_t>(blk.size(), 1)));
   sg.comp_blk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * cb.size()));
@@ -617,6 +692,8 @@ int dgmi_ctx_destroy(dgmi_ctx* c) {
   if (Krylov* k = static_cast<Krylov*>(c->krylov)) {
     if (k->hd) (void)hipHostFree(k->hd);
     if (k->hi) (void)hipHostFree(k->hi);
+    if (k->ring) (void)hipHostFree(k->ring);
+    for (hipEvent_t e : k->ev) (void)hipEventDestroy(e);
     delete k;
   }
   delete c;
@@ -720,14 +797,15 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
              "unknown smoother variant");
   hipStream_t s = (hipStream_t)stream;
   DGMI_TRY(compute_bdiag(c, s));
+  // FP64 full blocks; FP32 the packed symmetric form (NL(NL+1)/2 entries per element)
   const int64_t nS = c->nele * c->nloc * c->nloc;
+  const int64_t nS32 = c->nele * (c->nloc * (c->nloc + 1) / 2);
   if (!c->g64.S) c->g64.S = static_cast<double*>(c->alloc(sizeof(double) * nS));
-  if (!c->g32.S) c->g32.S = static_cast<float*>(c->alloc(sizeof(float) * nS));
+  if (!c->g32.S) c->g32.S = static_cast<float*>(c->alloc(sizeof(float) * nS32));
   DGMI_CHECK(c->g64.S && c->g32.S, alloc_failure());
   Scratch& sc = scratch(c);
   if (variant == DGMI_SMOOTH_JACOBI3) {
     DGMI_TRY(launch_build_S_jacobi<double>(c, sc.bdiag, omega, c->g64.S, s));
-    DGMI_TRY(launch_build_S_jacobi<float>(c, sc.bdiag, omega, c->g32.S, s));
   } else {
     int* fail = static_cast<int*>(c->alloc(sizeof(int)));
     DGMI_CHECK(fail, alloc_failure());
@@ -738,6 +816,7 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
     DGMI_HIP(hipStreamSynchronize(s));
     DGMI_CHECK(hf == 0, "singular diagonal block " + std::to_string(hf - 1));
   }
+  DGMI_TRY(launch_pack_S_sym(c, s));
   c->smoother_variant = variant;
   c->omega = omega;
   return 0;
@@ -860,6 +939,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   SfcLayout lay;
   lay.lh.resize(nc);
   lay.start.resize(nc);
+  lay.start32.resize(nc);
   const char* ev = std::getenv("DGMI_SMALL_LEVEL_N");  // test hook: force grid levels
   const int64_t small_n = ev ? std::max<int64_t>(1, std::atoll(ev)) : 0;
   mg.comp_levels.assign(nc, {});
@@ -879,9 +959,9 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     const SfcLevelsHost& h = lay.lh[q];
     std::vector<int64_t> nnz(h.nlevel);
     for (int l = 0; l < h.nlevel; ++l) nnz[l] = (int64_t)h.col[l].size();
-    // the coarse end that fits in LDS (120 KiB of the 160 KiB per work-group) runs in
-    // one work-group; larger levels run as grid kernels
-    int l0 = lds_start_level(h.n, nnz, 120 * 1024);
+    // the coarse end that fits in LDS (120 KiB of the 160 KiB per work-group in FP64,
+    // 150 KiB in FP32) runs in one work-group; larger levels run as grid kernels
+    int l0 = lds_start_level<double>(h.n, nnz, 120 * 1024);
     if (l0 < 0) {
       too_deep[q] = 1;
       continue;
@@ -893,6 +973,10 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
           break;
         }
     lay.start[q] = l0;
+    // FP32 may start lower: one level at most, so the padding of the layout (to the
+    // FP64 start) keeps every pair inside its component
+    const int l32 = lds_start_level<float>(h.n, nnz, 150 * 1024);
+    lay.start32[q] = small_n ? l0 : std::max(l0 - 1, std::min(l0, l32));
     mg.comp_levels[q] = h.n;
   }
   }, 1);
@@ -901,8 +985,12 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   // of 2^start so that every component's pairs stay inside it (the last one needs none)
   lay.order.resize(nc);
   std::iota(lay.order.begin(), lay.order.end(), 0);
-  std::stable_sort(lay.order.begin(), lay.order.end(),
-                   [&](int a, int b) { return lay.start[a] > lay.start[b]; });
+  // (start, start32) descending: the components swept by a grid level are a prefix in
+  // both precisions
+  std::stable_sort(lay.order.begin(), lay.order.end(), [&](int a, int b) {
+    return lay.start[a] != lay.start[b] ? lay.start[a] > lay.start[b]
+                                        : lay.start32[a] > lay.start32[b];
+  });
   lay.off0.assign(nc, 0);
   lay.len0.assign(nc, 0);
   lay.smax = lay.start[lay.order[0]];
@@ -924,6 +1012,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   if (nc > 1) DGMI_TRY(build_transfers(pos, total));
   DGMI_TRY(upload_sfc_levels(c, lay));
   mg.small_level = lay.smax;
+  mg.small_level32 = lay.start32[lay.order[0]];
   mg.ready = true;
   return 0;
 }
@@ -1086,6 +1175,11 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "mfma_waves" || k == "bf16_waves") {
     DGMI_CHECK(value == 2.0 || value == 4.0 || value == 8.0, k + " must be 2, 4 or 8");
     (k == "mfma_waves" ? c->mf_wpb : c->bf_wpb) = (int)value;
+    return 0;
+  }
+  if (k == "smooth_offdiag") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "smooth_offdiag must be 0 or 1");
+    c->smooth_offdiag = (int)value;
     return 0;
   }
   if (k == "graph_vcycle") {

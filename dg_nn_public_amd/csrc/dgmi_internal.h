@@ -57,7 +57,8 @@ template <typename T> struct DevGeom {
 struct SfcLevelsDev {
   int nlevel = 0;                 // grid-layout levels held (small_level + 1)
   std::vector<int64_t> n, nnz;    // rows and entries per grid level
-  std::vector<int64_t> grid_rows;  // rows swept by the grid kernels at level l < small_level
+  std::vector<int64_t> grid_rows;    // rows swept by the FP64 grid kernels at level l
+  std::vector<int64_t> grid_rows32;  // the same for FP32 (its LDS kernel may start lower)
   std::vector<int32_t*> rowptr, col;
   std::vector<double*> val64, diag64;
   std::vector<float*> val32, diag32;
@@ -75,8 +76,16 @@ struct SfcLevelsDev {
   // (start level, level count) per component
   void* desc64 = nullptr;
   void* desc32 = nullptr;
-  int32_t* rng = nullptr;
-  bool any_start0 = false;  // some component starts at level 0
+  int32_t* rng64 = nullptr;  // FP64 (start, count); FP32 starts one level lower where its
+  int32_t* rng32 = nullptr;  // half-size hierarchy fits in LDS
+  // LDS images of every component's levels (16-byte aligned pieces) and per component
+  // (first 16-byte word, words to copy)
+  void* img64 = nullptr;
+  void* img32 = nullptr;
+  int64_t* imginfo64 = nullptr;
+  int64_t* imginfo32 = nullptr;
+  bool any_start0 = false;    // some component's FP64 kernel starts at level 0
+  bool any_start0_32 = false;
   size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the single-work-group kernel
 };
 
@@ -88,6 +97,7 @@ struct SegBlk {
 };
 constexpr int MDOT_K = 32;     // basis vectors per fused Gram-Schmidt pass
 constexpr int SEG_CHUNK = 1024;  // DOFs per segmented block
+constexpr int GS_CHUNK = 4096;   // DOFs per wide block of the Gram-Schmidt kernels
 struct SegDev {
   int ncomp = 1;
   int nblk = 0;
@@ -96,6 +106,11 @@ struct SegDev {
   int64_t* eoff = nullptr;      // (ncomp + 1) element offsets of the components
   double* partial = nullptr;    // (nblk * MDOT_K)
   double* coef = nullptr;       // (ncomp * MDOT_K)
+  // wide blocks (GS_CHUNK DOFs) of the fused Gram-Schmidt kernels
+  int nwblk = 0;
+  SegBlk* wblk = nullptr;        // (nwblk)
+  int32_t* comp_wblk = nullptr;  // (ncomp + 1)
+  bool wvec4 = false;            // every wide block starts at a multiple of 4 DOFs
 };
 
 // Operand tables of the matrix-core apply (apply_mfma.hip), in MFMA lane order.
@@ -129,7 +144,8 @@ struct MGDev {
   // prolongation: per p1dg index e*(D+1)+k, the SFC position of its vertex
   int32_t* cellpos = nullptr;
   SfcLevelsDev lv;
-  int small_level = 0;  // grid kernels sweep levels below this one
+  int small_level = 0;    // FP64 grid kernels sweep levels below this one
+  int small_level32 = 0;  // FP32 grid kernels sweep levels below this one
   // level sizes of every component (host), components in context order
   std::vector<std::vector<int64_t>> comp_levels;
   // V-cycle replayed from a captured graph (set_param "graph_vcycle")
@@ -146,6 +162,11 @@ int sdot(const SegDev& sg, const T* x, const T* y, double* out, double* sq, hipS
 template <typename T>
 int sorth_pass(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h, int64_t hs,
                int accumulate, const int32_t* active, hipStream_t s);
+// CGS2 of w against V_0..V_{k-1}, 1 <= k <= MDOT_K, per component: h[c*hs + j] = the sum
+// of both passes' coefficients, hn[c] = ||w|| afterwards (0 for inactive components)
+template <typename T>
+int sorth_cgs2(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h, int64_t hs,
+               double* hn, const int32_t* active, hipStream_t s);
 template <typename Tin, typename Tout>
 int sscale(const SegDev& sg, const double* a, int mode, const int32_t* active, const Tin* x,
            Tout* y, hipStream_t s);
@@ -157,9 +178,11 @@ int scombine(const SegDev& sg, const T* V, int64_t ldv, const double* yv, int64_
 int scycle_start(int ncomp, const double* beta, const double* eps, const int32_t* want,
                  int32_t* active, double* thr, double* g, int64_t gs, int glen, int32_t* kk,
                  hipStream_t s);
+// status[0] = active components after the step, status[1 + c] = res[c] / thr[c] for the
+// still-active components (0 for the others)
 int sgivens(int ncomp, int j, int ldh, int64_t hs, double* H, const double* hn, double* cs,
             double* sn, int64_t rs, double* g, int64_t gs, double* res, const double* thr,
-            int32_t* active, int32_t* kk, int32_t* nact, hipStream_t s);
+            int32_t* active, int32_t* kk, int32_t* nact, double* status, hipStream_t s);
 int sbacksolve(int ncomp, int ldh, int64_t hs, const double* H, const double* g, int64_t gs,
                const int32_t* kk, double* yv, int64_t ys, hipStream_t s);
 // act64[b] = any component overlapping elements [64b, 64b+64) has flag[c] != 0
@@ -188,6 +211,9 @@ struct dgmi_ctx {
   // kernel family of every FP32-vector apply/smooth: 0 vector FMA, 1 MFMA fp32,
   // 2 MFMA with bf16 inputs
   int f32_kernel = 1;
+  // FP32 smoothing sweeps with the exact block smoother in the neighbour-only form
+  // x = (1 - omega) x + S (b - A_off x) (no volume term, half the face terms)
+  int smooth_offdiag = 1;
   // waves (16 elements each) per work-group of the fp32 / bf16 matrix-core applies
   int mf_wpb = 4, bf_wpb = 4;
   bool pc_bf16 = false;  // FP32 preconditioner applies with bf16 inputs
@@ -237,6 +263,8 @@ int launch_build_S_jacobi(dgmi_ctx* c, const double* bdiag, double omega, T* S, 
 // index of a singular block
 int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* fail,
                           hipStream_t s);
+// g32.S = packed symmetric FP32 copy of g64.S
+int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s);
 template <typename T>
 int launch_convert(const double* in, T* out, int64_t n, hipStream_t s);
 // transfers + SFC levels (mg.hip)
@@ -250,11 +278,13 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s);
 // first level from which a component fits in `budget`, and the device level tables
 template <typename T>
 size_t lds_bytes_from(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, int l0);
+template <typename T>
 int lds_start_level(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, size_t budget);
 // Host description of the SFC hierarchy of every component and of its device layout.
 struct SfcLayout {
   std::vector<SfcLevelsHost> lh;  // per component (context order), local indices
-  std::vector<int> start;         // single-work-group start level per component
+  std::vector<int> start;         // single-work-group start level per component (FP64)
+  std::vector<int> start32;       // the same for FP32: start or start - 1
   std::vector<int> order;         // components in layout order (start level descending)
   std::vector<int64_t> off0;      // level-0 offset of each component (context order)
   std::vector<int64_t> len0;      // level-0 segment length (padded except the last)

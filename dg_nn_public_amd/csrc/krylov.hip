@@ -58,10 +58,6 @@ __global__ void k_axpy(int64_t n, double a, const T* __restrict__ x, T* __restri
   if (i < n) y[i] = (T)((double)y[i] + a * (double)x[i]);
 }
 
-// Fused classical Gram-Schmidt passes run over up to MDOT_K basis vectors at a time:
-//   h[j] = <w, V_j> (one read of w for all j), then w -= sum_j h[j] V_j.
-
-
 // ---------------------------------------------------------------------------------------
 // Segmented (per-component) kernels.  A context may hold several independent solves
 // ("components", contiguous DOF ranges).  Vectors are cut into blocks that never cross a
@@ -100,71 +96,6 @@ __global__ __launch_bounds__(RED_THREADS) void k_sdot_final(const int32_t* __res
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(RED_THREADS) void k_smdot_partial(const SegBlk* __restrict__ sb,
-                                                               int k, const T* __restrict__ V,
-                                                               int64_t ldv,
-                                                               const T* __restrict__ w,
-                                                               double* __restrict__ partial,
-                                                               const int32_t* __restrict__ active) {
-  const SegBlk b = sb[blockIdx.x];
-  if (active && !active[b.comp]) {  // finished component (uniform over the block)
-    if (threadIdx.x < k) partial[(int64_t)blockIdx.x * MDOT_K + threadIdx.x] = 0.0;
-    return;
-  }
-  double acc[MDOT_K];
-#pragma unroll
-  for (int j = 0; j < MDOT_K; ++j) acc[j] = 0.0;
-  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) {
-    const double wi = (double)w[i];
-#pragma unroll
-    for (int j = 0; j < MDOT_K; ++j)
-      if (j < k) acc[j] += wi * (double)V[(int64_t)j * ldv + i];
-  }
-  __shared__ double sh[RED_THREADS / 64][MDOT_K];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < MDOT_K; ++j) {
-    if (j < k) {
-      const double v = wave_sum(acc[j]);
-      if (lane == 0) sh[wid][j] = v;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < k) {
-    double t = 0.0;
-    for (int ww = 0; ww < RED_THREADS / 64; ++ww) t += sh[ww][threadIdx.x];
-    partial[(int64_t)blockIdx.x * MDOT_K + threadIdx.x] = t;
-  }
-}
-
-// per component c: coef[c][j] = sum of partials (0 when inactive); h[c*hs + j] (+)= coef
-__global__ __launch_bounds__(RED_THREADS) void k_smdot_final(const int32_t* __restrict__ comp_blk,
-                                                             const double* __restrict__ partial,
-                                                             int k, double* __restrict__ h,
-                                                             int64_t hs, double* __restrict__ coef,
-                                                             int accumulate,
-                                                             const int32_t* __restrict__ active) {
-  // all k sums at once: lane group j (8 lanes) sums column j of the component's partials
-  // in a fixed strided order, then a fixed shuffle tree
-  constexpr int G = RED_THREADS / MDOT_K;  // lanes per column
-  static_assert(G * MDOT_K == RED_THREADS && (G & (G - 1)) == 0, "lane groups");
-  const int c = blockIdx.x;
-  const int j = threadIdx.x / G, sub = threadIdx.x % G;
-  double acc = 0.0;
-  if (j < k)
-    for (int b = comp_blk[c] + sub; b < comp_blk[c + 1]; b += G)
-      acc += partial[(int64_t)b * MDOT_K + j];
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if (j < k && sub == 0) {
-    const bool on = active[c] != 0;
-    const double v = on ? acc : 0.0;
-    coef[c * MDOT_K + j] = v;
-    if (on) h[c * hs + j] = accumulate ? h[c * hs + j] + v : v;
-  }
-}
-
 // w -= sum_j coef[comp][j] V_j
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_smaxpy(const SegBlk* __restrict__ sb, int k,
@@ -179,6 +110,260 @@ __global__ __launch_bounds__(RED_THREADS) void k_smaxpy(const SegBlk* __restrict
     double s = (double)w[i];
     for (int j = 0; j < k; ++j) s -= cf[j] * (double)V[(int64_t)j * ldv + i];
     w[i] = (T)s;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused CGS2 (classical Gram-Schmidt with one re-orthogonalisation) and the norm of the
+// result, over wide blocks of GS_CHUNK DOFs that never cross a component:
+//   k_gs_dots            p1 = V^T w
+//   k_gs_update<1>       w -= V c1, then p2 = V^T w      (c1: pass-1 coefficients)
+//   k_gs_update<2>       w -= V c2, then p3 = w^T w      (c2: pass-2 coefficients)
+// V is read three times per Arnoldi step instead of four (two passes of dots and updates)
+// and w twice less.  Each thread owns 16-byte row groups; the k column sums of a block are
+// reduced across the wave by a butterfly that halves the columns held per lane at every
+// step (k-1 exchanges in place of 6 per column), then across the block's waves in LDS.
+constexpr int GS_THREADS = 256;
+constexpr int GS_FINAL = 1024;  // threads of the per-component final reduction
+
+template <typename T>
+struct Row16;  // 16 bytes of T
+template <>
+struct Row16<float> {
+  using V = float4;
+  static constexpr int N = 4;
+};
+template <>
+struct Row16<double> {
+  using V = double2;
+  static constexpr int N = 2;
+};
+
+// N consecutive values at p: one 16-byte access when N fills it, else N scalar accesses
+template <typename T, int N>
+__device__ __forceinline__ void load_n(const T* __restrict__ p, T (&o)[N]) {
+  if constexpr (N == Row16<T>::N) {
+    const typename Row16<T>::V q = *reinterpret_cast<const typename Row16<T>::V*>(p);
+    const T* qs = reinterpret_cast<const T*>(&q);
+#pragma unroll
+    for (int u = 0; u < N; ++u) o[u] = qs[u];
+  } else {
+#pragma unroll
+    for (int u = 0; u < N; ++u) o[u] = p[u];
+  }
+}
+template <typename T, int N>
+__device__ __forceinline__ void store_n(T* __restrict__ p, const T (&o)[N]) {
+  if constexpr (N == Row16<T>::N) {
+    typename Row16<T>::V q;
+    T* qs = reinterpret_cast<T*>(&q);
+#pragma unroll
+    for (int u = 0; u < N; ++u) qs[u] = o[u];
+    *reinterpret_cast<typename Row16<T>::V*>(p) = q;
+  } else {
+#pragma unroll
+    for (int u = 0; u < N; ++u) p[u] = o[u];
+  }
+}
+
+// Sums a[0..KC) over the 64 lanes of the wave, KC a power of two.  On return a[0] of
+// lane l is the sum of column l / (64 / KC).  Fixed exchange pattern, so the result does
+// not depend on timing.
+template <int KC>
+__device__ __forceinline__ void wave_reduce_cols(double (&a)[KC]) {
+  static_assert(KC >= 1 && KC <= 64 && (KC & (KC - 1)) == 0, "power-of-two column count");
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int S = KC, o = 32; S > 1; S >>= 1, o >>= 1) {
+    const bool up = (lane & o) != 0;  // this lane keeps the upper half of the columns
+#pragma unroll
+    for (int i = 0; i < S / 2; ++i) {
+      const double send = up ? a[i] : a[i + S / 2];
+      const double keep = up ? a[i + S / 2] : a[i];
+      a[i] = keep + __shfl_xor(send, o, 64);
+    }
+  }
+#pragma unroll
+  for (int o = 32 / KC; o > 0; o >>= 1) a[0] += __shfl_xor(a[0], o, 64);
+}
+
+constexpr int pow2_ceil(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
+  return p;
+}
+
+// out[j] = block sum of column j, j < k.  The butterfly runs over the next power of two
+// of columns (the extra ones are zero).
+template <int KC>
+__device__ __forceinline__ void block_store_cols(const double (&a)[KC], int k,
+                                                 double* __restrict__ out) {
+  constexpr int P = pow2_ceil(KC);
+  static_assert(P <= 64, "at most 64 columns");
+  __shared__ double sh[GS_THREADS / 64][P];
+  double ap[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) ap[j] = j < KC ? a[j] : 0.0;
+  wave_reduce_cols<P>(ap);
+  constexpr int L = 64 / P;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane % L == 0) sh[wid][lane / L] = ap[0];
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double t = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < GS_THREADS / 64; ++ww) t += sh[ww][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+// Column j < KC of the block's rows reads V_min(j, k-1): the loads of a row group are
+// issued together with no branch, and the columns past k (zero coefficients, unused sums)
+// re-read a column that is already in flight.
+
+// acc[j] += <V_j, w> over rows [i, i + N) of the lane
+template <typename T, int KC, int N>
+__device__ __forceinline__ void gs_dots_rows(int k, const T* __restrict__ V, int64_t ldv,
+                                             const T* __restrict__ w, int64_t i,
+                                             double (&acc)[KC]) {
+  T wv[N];
+  load_n<T, N>(w + i, wv);
+#pragma unroll
+  for (int j = 0; j < KC; ++j) {
+    T vv[N];
+    load_n<T, N>(V + (int64_t)min(j, k - 1) * ldv + i, vv);
+#pragma unroll
+    for (int u = 0; u < N; ++u) acc[j] += (double)vv[u] * (double)wv[u];
+  }
+}
+
+// w -= sum_j coef_j V_j over rows [i, i + N), then MODE 1: acc[j] += <V_j, w>; MODE 2:
+// acc[0] += <w, w> (with w as stored)
+// The update runs in T (the precision w is stored in), the dot sums in FP64.
+template <typename T, int KC, int MODE, int N, int NO>
+__device__ __forceinline__ void gs_update_rows(int k, const T* __restrict__ V, int64_t ldv,
+                                               const T* __restrict__ cf, T* __restrict__ w,
+                                               int64_t i, double (&acc)[NO]) {
+  T wv[N];
+  load_n<T, N>(w + i, wv);
+  T vv[KC][N];
+#pragma unroll
+  for (int j = 0; j < KC; ++j) {
+    load_n<T, N>(V + (int64_t)min(j, k - 1) * ldv + i, vv[j]);
+    const T cj = cf[j];
+#pragma unroll
+    for (int u = 0; u < N; ++u) wv[u] -= cj * vv[j][u];
+  }
+  store_n<T, N>(w + i, wv);
+  if constexpr (MODE == 1) {
+#pragma unroll
+    for (int j = 0; j < KC; ++j)
+#pragma unroll
+      for (int u = 0; u < N; ++u) acc[j] += (double)(vv[j][u] * wv[u]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < N; ++u) acc[0] += (double)wv[u] * (double)wv[u];
+  }
+}
+
+// partial[blk][j] = <V_j, w> over the block, j < k <= KC.  VEC: 16-byte row groups (the
+// block start and V's leading dimension are multiples of the group), else single rows.
+template <typename T, int KC, bool VEC>
+__global__ __launch_bounds__(GS_THREADS) void k_gs_dots(const SegBlk* __restrict__ sb, int k,
+                                                       const T* __restrict__ V, int64_t ldv,
+                                                       const T* __restrict__ w,
+                                                       double* __restrict__ partial,
+                                                       const int32_t* __restrict__ active) {
+  constexpr int N = VEC ? Row16<T>::N : 1;
+  const SegBlk b = sb[blockIdx.x];
+  double* out = partial + (int64_t)blockIdx.x * MDOT_K;
+  if (active && !active[b.comp]) {  // finished component (uniform over the block)
+    if (threadIdx.x < k) out[threadIdx.x] = 0.0;
+    return;
+  }
+  double acc[KC];
+#pragma unroll
+  for (int j = 0; j < KC; ++j) acc[j] = 0.0;
+  int64_t i = b.beg + N * threadIdx.x;
+#pragma unroll 1
+  for (; i + N <= b.end; i += N * GS_THREADS) gs_dots_rows<T, KC, N>(k, V, ldv, w, i, acc);
+  if constexpr (N > 1)
+    for (; i < b.end; ++i) gs_dots_rows<T, KC, 1>(k, V, ldv, w, i, acc);  // ragged end
+  block_store_cols<KC>(acc, k, out);
+}
+
+// w -= sum_j coef[comp][j] V_j over the block, then MODE 1: partial[blk][j] = <V_j, w>;
+// MODE 2: partial[blk][0] = <w, w>.
+template <typename T, int KC, int MODE, bool VEC>
+__global__ __launch_bounds__(GS_THREADS) void k_gs_update(const SegBlk* __restrict__ sb, int k,
+                                                         const T* __restrict__ V, int64_t ldv,
+                                                         const double* __restrict__ coef,
+                                                         T* __restrict__ w,
+                                                         double* __restrict__ partial,
+                                                         const int32_t* __restrict__ active) {
+  constexpr int N = VEC ? Row16<T>::N : 1;
+  constexpr int NO = MODE == 1 ? KC : 1;
+  const int kout = MODE == 1 ? k : 1;
+  const SegBlk b = sb[blockIdx.x];
+  double* out = partial + (int64_t)blockIdx.x * MDOT_K;
+  if (active && !active[b.comp]) {
+    if (threadIdx.x < kout) out[threadIdx.x] = 0.0;
+    return;
+  }
+  __shared__ T cf[KC];
+  if (threadIdx.x < KC) cf[threadIdx.x] = threadIdx.x < k ? (T)coef[b.comp * MDOT_K + threadIdx.x] : T(0);
+  __syncthreads();
+  double acc[NO];
+#pragma unroll
+  for (int j = 0; j < NO; ++j) acc[j] = 0.0;
+  int64_t i = b.beg + N * threadIdx.x;
+#pragma unroll 1
+  for (; i + N <= b.end; i += N * GS_THREADS)
+    gs_update_rows<T, KC, MODE, N, NO>(k, V, ldv, cf, w, i, acc);
+  if constexpr (N > 1)
+    for (; i < b.end; ++i) gs_update_rows<T, KC, MODE, 1, NO>(k, V, ldv, cf, w, i, acc);
+  block_store_cols<NO>(acc, kout, out);
+}
+
+// One work-group per component sums its wide-block partials, column j by 32 lane groups
+// in a fixed order and then across the groups.  mode 0: h = coef = sums; 1: h += sums,
+// coef = sums; 2: hn[c] = sqrt(sum of column 0).  Inactive components: coef 0 (h kept),
+// hn 0.
+__global__ __launch_bounds__(GS_FINAL) void k_gs_final(const int32_t* __restrict__ comp_wblk,
+                                                       const double* __restrict__ partial,
+                                                       int k, int mode, double* __restrict__ h,
+                                                       int64_t hs, double* __restrict__ coef,
+                                                       double* __restrict__ hn,
+                                                       const int32_t* __restrict__ active) {
+  constexpr int G = GS_FINAL / MDOT_K;  // lane groups
+  const int c = blockIdx.x;
+  const int j = threadIdx.x % MDOT_K, grp = threadIdx.x / MDOT_K;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (j < k) {
+    int bb = comp_wblk[c] + grp;
+    const int e = comp_wblk[c + 1];
+    for (; bb + 3 * G < e; bb += 4 * G) {
+      a0 += partial[(int64_t)bb * MDOT_K + j];
+      a1 += partial[(int64_t)(bb + G) * MDOT_K + j];
+      a2 += partial[(int64_t)(bb + 2 * G) * MDOT_K + j];
+      a3 += partial[(int64_t)(bb + 3 * G) * MDOT_K + j];
+    }
+    for (; bb < e; bb += G) a0 += partial[(int64_t)bb * MDOT_K + j];
+  }
+  __shared__ double sh[G][MDOT_K + 1];
+  sh[grp][j] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double t = 0.0;
+    for (int q = 0; q < G; ++q) t += sh[q][threadIdx.x];
+    const bool on = !active || active[c] != 0;
+    if (mode == 2) {
+      hn[c] = on ? sqrt(t) : 0.0;
+    } else {
+      const double v = on ? t : 0.0;
+      coef[c * MDOT_K + threadIdx.x] = v;
+      if (on) h[c * hs + threadIdx.x] = mode == 1 ? h[c * hs + threadIdx.x] + v : v;
+    }
   }
 }
 
@@ -244,40 +429,60 @@ __global__ void k_scycle_start(int ncomp, const double* __restrict__ beta,
 
 // Givens step j for every active component (Saad, Alg. 6.9 with rotations); the
 // subdiagonal entry comes from hn[c].  A component whose estimate |g[j+1]| falls to
-// thr[c] (or whose basis breaks down) turns inactive.  nact[0] counts the still-active ones.
-__global__ void k_sgivens(int ncomp, int j, int ldh, int64_t hs, double* __restrict__ H,
-                          const double* __restrict__ hn, double* __restrict__ cs,
-                          double* __restrict__ sn, int64_t rs, double* __restrict__ g,
-                          int64_t gs, double* __restrict__ res, const double* __restrict__ thr,
-                          int32_t* __restrict__ active, int32_t* __restrict__ kk,
-                          int32_t* __restrict__ nact) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncomp) return;
-  if (!active[c]) return;
-  double* h = H + c * hs + (int64_t)j * ldh;
-  double* csc = cs + c * rs;
-  double* snc = sn + c * rs;
-  double* gc = g + c * gs;
-  h[j + 1] = hn[c];
-  for (int i = 0; i < j; ++i) {
-    const double t = csc[i] * h[i] + snc[i] * h[i + 1];
-    h[i + 1] = -snc[i] * h[i] + csc[i] * h[i + 1];
-    h[i] = t;
+// thr[c] (or whose basis breaks down) turns inactive.  One work-group: nact[0] and
+// status[0] count the still-active components, status[1 + c] = res[c] / thr[c] for them.
+constexpr int GIV_THREADS = 256;
+__global__ __launch_bounds__(GIV_THREADS) void k_sgivens(
+    int ncomp, int j, int ldh, int64_t hs, double* __restrict__ H, const double* __restrict__ hn,
+    double* __restrict__ cs, double* __restrict__ sn, int64_t rs, double* __restrict__ g,
+    int64_t gs, double* __restrict__ res, const double* __restrict__ thr,
+    int32_t* __restrict__ active, int32_t* __restrict__ kk, int32_t* __restrict__ nact,
+    double* __restrict__ status) {
+  int cnt = 0;
+  for (int c = threadIdx.x; c < ncomp; c += GIV_THREADS) {
+    double ratio = 0.0;
+    if (active[c]) {
+      double* h = H + c * hs + (int64_t)j * ldh;
+      double* csc = cs + c * rs;
+      double* snc = sn + c * rs;
+      double* gc = g + c * gs;
+      h[j + 1] = hn[c];
+      for (int i = 0; i < j; ++i) {
+        const double t = csc[i] * h[i] + snc[i] * h[i + 1];
+        h[i + 1] = -snc[i] * h[i] + csc[i] * h[i + 1];
+        h[i] = t;
+      }
+      const double a = h[j], bb = h[j + 1];
+      const double r = hypot(a, bb);
+      const double cc = r > 0 ? a / r : 1.0, ss = r > 0 ? bb / r : 0.0;
+      csc[j] = cc;
+      snc[j] = ss;
+      h[j] = r;
+      h[j + 1] = 0.0;
+      gc[j + 1] = -ss * gc[j];
+      gc[j] = cc * gc[j];
+      res[c] = fabs(gc[j + 1]);
+      kk[c] = j + 1;
+      const bool stop = !(res[c] > thr[c]) || !(hn[c] > 0.0) || r == 0.0;
+      if (stop) {
+        active[c] = 0;
+      } else {
+        ++cnt;
+        ratio = res[c] / thr[c];
+      }
+    }
+    status[1 + c] = ratio;
   }
-  const double a = h[j], bb = h[j + 1];
-  const double r = hypot(a, bb);
-  const double cc = r > 0 ? a / r : 1.0, ss = r > 0 ? bb / r : 0.0;
-  csc[j] = cc;
-  snc[j] = ss;
-  h[j] = r;
-  h[j + 1] = 0.0;
-  gc[j + 1] = -ss * gc[j];
-  gc[j] = cc * gc[j];
-  res[c] = fabs(gc[j + 1]);
-  kk[c] = j + 1;
-  const bool stop = !(res[c] > thr[c]) || !(hn[c] > 0.0) || r == 0.0;
-  if (stop) active[c] = 0;
-  else atomicAdd(nact, 1);
+  __shared__ int sh[GIV_THREADS / 64];
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < GIV_THREADS / 64; ++w) t += sh[w];
+    nact[0] = t;
+    status[0] = (double)t;
+  }
 }
 
 // back substitution R y = g per component over its kk[c] columns
@@ -314,11 +519,6 @@ __global__ void k_expand_active(int64_t nele, int ncomp, const int64_t* __restri
   act64[b] = on;
 }
 
-__global__ void k_izero(int n, int32_t* __restrict__ p) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
 // ---------------------------------------------------------------------------------------
 // partial sums of the context-free dot (one buffer per host thread, so calls from
 // different threads on different streams never share it)
@@ -352,6 +552,14 @@ template int launch_dot<float>(int64_t, const float*, const float*, double*, int
 template int launch_axpy<double>(int64_t, double, const double*, double*, hipStream_t);
 template int launch_axpy<float>(int64_t, double, const float*, float*, hipStream_t);
 
+// 16-byte row groups are aligned in every wide block of V's columns and w
+template <typename T>
+static int gs_vec(const SegDev& sg, const T* V, int64_t ldv, const T* w) {
+  constexpr int N = Row16<T>::N;
+  return sg.wvec4 && ldv % N == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 &&
+         reinterpret_cast<uintptr_t>(w) % 16 == 0;
+}
+
 // exported to the solver translation unit
 namespace kry {
 // ---- segmented launchers
@@ -372,14 +580,68 @@ int sdot(const SegDev& sg, const T* x, const T* y, double* out, double* sq, hipS
 template <typename T>
 int sorth_pass(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h, int64_t hs,
                int accumulate, const int32_t* active, hipStream_t s) {
+  const bool vec = gs_vec(sg, V, ldv, w);
   for (int c0 = 0; c0 < k; c0 += MDOT_K) {
     const int kc = std::min(MDOT_K, k - c0);
-    hipLaunchKernelGGL((k_smdot_partial<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, kc,
-                       V + (int64_t)c0 * ldv, ldv, (const T*)w, sg.partial, active);
-    hipLaunchKernelGGL(k_smdot_final, dim3(sg.ncomp), dim3(RED_THREADS), 0, s, sg.comp_blk,
-                       (const double*)sg.partial, kc, h + c0, hs, sg.coef, accumulate, active);
-    hipLaunchKernelGGL((k_smaxpy<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, kc,
-                       V + (int64_t)c0 * ldv, ldv, (const double*)sg.coef, w, active);
+    const T* Vc = V + (int64_t)c0 * ldv;
+    if (vec && kc <= 4)
+      hipLaunchKernelGGL((k_gs_dots<T, 4, true>), dim3(sg.nwblk), dim3(GS_THREADS), 0, s, sg.wblk,
+                         kc, Vc, ldv, (const T*)w, sg.partial, active);
+    else if (vec)
+      hipLaunchKernelGGL((k_gs_dots<T, MDOT_K, true>), dim3(sg.nwblk), dim3(GS_THREADS), 0, s,
+                         sg.wblk, kc, Vc, ldv, (const T*)w, sg.partial, active);
+    else
+      hipLaunchKernelGGL((k_gs_dots<T, MDOT_K, false>), dim3(sg.nwblk), dim3(GS_THREADS), 0, s,
+                         sg.wblk, kc, Vc, ldv, (const T*)w, sg.partial, active);
+    hipLaunchKernelGGL(k_gs_final, dim3(sg.ncomp), dim3(GS_FINAL), 0, s, sg.comp_wblk,
+                       (const double*)sg.partial, kc, accumulate ? 1 : 0, h + c0, hs, sg.coef,
+                       nullptr, active);
+    hipLaunchKernelGGL((k_smaxpy<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, kc, Vc,
+                       ldv, (const double*)sg.coef, w, active);
+  }
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T, int KC, bool VEC>
+static void launch_cgs2(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h,
+                        int64_t hs, double* hn, const int32_t* active, hipStream_t s) {
+  const dim3 g(sg.nwblk), t(GS_THREADS), f(sg.ncomp), tf(GS_FINAL);
+  hipLaunchKernelGGL((k_gs_dots<T, KC, VEC>), g, t, 0, s, sg.wblk, k, V, ldv, (const T*)w,
+                     sg.partial, active);
+  hipLaunchKernelGGL(k_gs_final, f, tf, 0, s, sg.comp_wblk, (const double*)sg.partial, k, 0, h,
+                     hs, sg.coef, nullptr, active);
+  hipLaunchKernelGGL((k_gs_update<T, KC, 1, VEC>), g, t, 0, s, sg.wblk, k, V, ldv,
+                     (const double*)sg.coef, w, sg.partial, active);
+  hipLaunchKernelGGL(k_gs_final, f, tf, 0, s, sg.comp_wblk, (const double*)sg.partial, k, 1, h,
+                     hs, sg.coef, nullptr, active);
+  hipLaunchKernelGGL((k_gs_update<T, KC, 2, VEC>), g, t, 0, s, sg.wblk, k, V, ldv,
+                     (const double*)sg.coef, w, sg.partial, active);
+  hipLaunchKernelGGL(k_gs_final, f, tf, 0, s, sg.comp_wblk, (const double*)sg.partial, 1, 2,
+                     nullptr, 0, sg.coef, hn, active);
+}
+
+template <typename T>
+int sorth_cgs2(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h, int64_t hs,
+               double* hn, const int32_t* active, hipStream_t s) {
+  if (k < 1 || k > MDOT_K) {
+    set_error("fused CGS2 takes 1.." + std::to_string(MDOT_K) + " vectors");
+    return -2;
+  }
+  if (!gs_vec(sg, V, ldv, w)) {
+    launch_cgs2<T, MDOT_K, false>(sg, k, V, ldv, w, h, hs, hn, active, s);
+  } else {
+    // columns rounded up to a multiple of 4
+    switch ((k + 3) / 4) {
+      case 1: launch_cgs2<T, 4, true>(sg, k, V, ldv, w, h, hs, hn, active, s); break;
+      case 2: launch_cgs2<T, 8, true>(sg, k, V, ldv, w, h, hs, hn, active, s); break;
+      case 3: launch_cgs2<T, 12, true>(sg, k, V, ldv, w, h, hs, hn, active, s); break;
+      case 4: launch_cgs2<T, 16, true>(sg, k, V, ldv, w, h, hs, hn, active, s); break;
+      case 5: launch_cgs2<T, 20, true>(sg, k, V, ldv, w, h, hs, hn, active, s); break;
+      case 6: launch_cgs2<T, 24, true>(sg, k, V, ldv, w, h, hs, hn, active, s); break;
+      case 7: launch_cgs2<T, 28, true>(sg, k, V, ldv, w, h, hs, hn, active, s); break;
+      default: launch_cgs2<T, 32, true>(sg, k, V, ldv, w, h, hs, hn, active, s); break;
+    }
   }
   DGMI_HIP(hipGetLastError());
   return 0;
@@ -421,10 +683,9 @@ int scycle_start(int ncomp, const double* beta, const double* eps, const int32_t
 
 int sgivens(int ncomp, int j, int ldh, int64_t hs, double* H, const double* hn, double* cs,
             double* sn, int64_t rs, double* g, int64_t gs, double* res, const double* thr,
-            int32_t* active, int32_t* kk, int32_t* nact, hipStream_t s) {
-  hipLaunchKernelGGL(k_izero, dim3(1), dim3(64), 0, s, 1, nact);
-  hipLaunchKernelGGL(k_sgivens, dim3((ncomp + 63) / 64), dim3(64), 0, s, ncomp, j, ldh, hs, H,
-                     hn, cs, sn, rs, g, gs, res, thr, active, kk, nact);
+            int32_t* active, int32_t* kk, int32_t* nact, double* status, hipStream_t s) {
+  hipLaunchKernelGGL(k_sgivens, dim3(1), dim3(GIV_THREADS), 0, s, ncomp, j, ldh, hs, H, hn, cs,
+                     sn, rs, g, gs, res, thr, active, kk, nact, status);
   DGMI_HIP(hipGetLastError());
   return 0;
 }
@@ -454,6 +715,10 @@ template int sorth_pass<double>(const SegDev&, int, const double*, int64_t, doub
                                 int64_t, int, const int32_t*, hipStream_t);
 template int sorth_pass<float>(const SegDev&, int, const float*, int64_t, float*, double*,
                                int64_t, int, const int32_t*, hipStream_t);
+template int sorth_cgs2<double>(const SegDev&, int, const double*, int64_t, double*, double*,
+                                int64_t, double*, const int32_t*, hipStream_t);
+template int sorth_cgs2<float>(const SegDev&, int, const float*, int64_t, float*, double*,
+                               int64_t, double*, const int32_t*, hipStream_t);
 template int sscale<double, double>(const SegDev&, const double*, int, const int32_t*,
                                     const double*, double*, hipStream_t);
 template int sscale<float, float>(const SegDev&, const double*, int, const int32_t*,

@@ -4,6 +4,7 @@
 // per-V-cycle argsort/permutation (z02/volume_mf_diff.py:1992-2007) is folded into the
 // transfer kernels' index lists.  The 1-D SFC levels are CSR matrices; level l+1
 // aggregates positions (2I, 2I+1) of level l.
+#include <cstring>
 #include <mutex>
 
 #include "dgmi_internal.h"
@@ -30,26 +31,32 @@ template <int D> __device__ __forceinline__ double Ptab(int n, int k) {
   else return c_P2[n * 3 + k];
 }
 
-// r_sfc[p] = (1/mult) sum over the DG copies (e,k) of vertex p of sum_n P[n][k] r[e][n]
+// r_sfc[p] = (1/mult) sum over the DG copies (e,k) of vertex p of sum_n P[n][k] r[e][n].
+// A group of RESTRICT_LANES lanes per vertex takes its copies side by side (about 24 per
+// vertex of a tetrahedral mesh, 20 loads each) and sums them with shuffles.
+constexpr int RESTRICT_LANES = 8;
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_restrict(int ncg, const int32_t* __restrict__ rptr,
                                                   const int32_t* __restrict__ rent,
                                                   const T* __restrict__ rw,
                                                   const T* __restrict__ r, T* __restrict__ out) {
   constexpr int NL = D == 3 ? 20 : 10, NB = D + 1;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= ncg) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = t / RESTRICT_LANES, sub = t % RESTRICT_LANES;
+  if (p >= ncg) return;  // whole groups leave together
   T s = 0;
-  for (int q = rptr[p]; q < rptr[p + 1]; ++q) {
+  for (int q = rptr[p] + sub; q < rptr[p + 1]; q += RESTRICT_LANES) {
     const int idx = rent[q];
     const int e = idx / NB, k = idx - (idx / NB) * NB;
     const T* re = r + (size_t)e * NL;
-    T t = 0;
+    T u = 0;
 #pragma unroll
-    for (int n = 0; n < NL; ++n) t += (T)Ptab<D>(n, k) * re[n];
-    s += t;
+    for (int n = 0; n < NL; ++n) u += (T)Ptab<D>(n, k) * re[n];
+    s += u;
   }
-  out[p] = s * rw[p];
+#pragma unroll
+  for (int off = RESTRICT_LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (sub == 0) out[p] = s * rw[p];
 }
 
 // x[e][n] += sum_k P[n][k] e_sfc[pos(e,k)] / mult(pos(e,k))
@@ -206,10 +213,11 @@ template <typename T> struct LevelDesc {
 
 // The coarse end of the cycle (levels l0..L-1 of one component per work-group, where a
 // grid launch per sweep would be pure launch latency), with the component's coarse
-// hierarchy staged in LDS: CSR (int32 rowptr/col, T values), diagonal and the r/A/B
-// vectors of every level.  The level table itself is built in LDS by thread 0.  Reads r
-// and A of level l0 from global memory and writes B of level l0 back.  Same operations as
-// the grid kernels above.
+// hierarchy in LDS: CSR (int32 rowptr/col, T values), diagonal and the r/A/B vectors of
+// every level.  The matrices come as a prebuilt byte image of the LDS layout (a table of
+// LdsImgLevel offsets, then the arrays), copied with 16-byte loads that are all issued
+// before the first LDS store; r and A of level l0 are read from global memory and B of
+// level l0 is written back.  Same operations as the grid kernels above.
 template <typename T> struct LdsLevel {
   int n;
   int* rp;
@@ -222,63 +230,65 @@ template <typename T> struct LdsLevel {
 };
 
 constexpr int MAX_LEVELS = 40;
+constexpr int SMALL_THREADS = 1024;
+
+// byte offsets (from the start of the LDS image) of one level's arrays
+struct LdsImgLevel {
+  int n, rp, col, val, diag, r, A, B;
+};
 
 __host__ __device__ inline size_t lds_align(size_t x) { return (x + 15) & ~size_t(15); }
 
 template <typename T>
-__host__ __device__ inline size_t lds_level_bytes(int n, int nnz) {
-  return lds_align(sizeof(int) * (n + 1)) + lds_align(sizeof(int) * nnz) +
-         lds_align(sizeof(T) * nnz) + 4 * lds_align(sizeof(T) * n);
-}
-
-template <typename T>
-__global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* __restrict__ descs,
-                                                           const int32_t* __restrict__ rng,
-                                                           T omega, int pre, int post) {
+__global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
+    const LevelDesc<T>* __restrict__ descs, const int32_t* __restrict__ rng,
+    const uint4* __restrict__ img, const int64_t* __restrict__ imginfo, T omega, int pre,
+    int post) {
   extern __shared__ __align__(16) unsigned char smem[];
-  LdsLevel<T>* tab = reinterpret_cast<LdsLevel<T>*>(smem);
-  const int tid = threadIdx.x, nth = blockDim.x;
-  const LevelDesc<T>* lvs = descs + (size_t)blockIdx.x * MAX_LEVELS;
+  __shared__ LdsLevel<T> tab[MAX_LEVELS];
+  const int tid = threadIdx.x;
+  constexpr int nth = SMALL_THREADS;
   const int l0 = rng[2 * blockIdx.x], L = rng[2 * blockIdx.x + 1];
-  if (tid == 0) {
-    size_t off = lds_align(sizeof(LdsLevel<T>) * MAX_LEVELS);
-    for (int l = l0; l < L; ++l) {
-      const int n = lvs[l].n, nnz = lvs[l].rp[n] - lvs[l].rp[0];
-      LdsLevel<T>& t = tab[l - l0];
-      t.n = n;
-      t.rp = reinterpret_cast<int*>(smem + off);
-      off += lds_align(sizeof(int) * (n + 1));
-      t.col = reinterpret_cast<int*>(smem + off);
-      off += lds_align(sizeof(int) * nnz);
-      t.val = reinterpret_cast<T*>(smem + off);
-      off += lds_align(sizeof(T) * nnz);
-      t.diag = reinterpret_cast<T*>(smem + off);
-      off += lds_align(sizeof(T) * n);
-      t.r = reinterpret_cast<T*>(smem + off);
-      off += lds_align(sizeof(T) * n);
-      t.A = reinterpret_cast<T*>(smem + off);
-      off += lds_align(sizeof(T) * n);
-      t.B = reinterpret_cast<T*>(smem + off);
-      off += lds_align(sizeof(T) * n);
+  const LevelDesc<T>& g0 = descs[(size_t)blockIdx.x * MAX_LEVELS + l0];
+  {
+    const uint4* src = img + imginfo[2 * blockIdx.x];
+    const int nw = (int)imginfo[2 * blockIdx.x + 1];
+    uint4* dst = reinterpret_cast<uint4*>(smem);
+    constexpr int U = 8;
+    for (int base = 0; base < nw; base += U * nth) {
+      uint4 t[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * nth + tid;
+        if (i < nw) t[u] = src[i];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * nth + tid;
+        if (i < nw) dst[i] = t[u];
+      }
     }
   }
   __syncthreads();
-  for (int l = l0; l < L; ++l) {
-    const LevelDesc<T>& g = lvs[l];
-    const LdsLevel<T>& t = tab[l - l0];
-    const int base = g.rp[0];
-    const int n = t.n, nnz = g.rp[n] - base;
-    for (int i = tid; i <= n; i += nth) t.rp[i] = g.rp[i] - base;
-    for (int q = tid; q < nnz; q += nth) {
-      t.col[q] = g.col[base + q] - g.cbase;
-      t.val[q] = g.val[base + q];
+  if (tid < L - l0) {
+    const LdsImgLevel o = reinterpret_cast<const LdsImgLevel*>(smem)[tid];
+    LdsLevel<T>& t = tab[tid];
+    t.n = o.n;
+    t.rp = reinterpret_cast<int*>(smem + o.rp);
+    t.col = reinterpret_cast<int*>(smem + o.col);
+    t.val = reinterpret_cast<T*>(smem + o.val);
+    t.diag = reinterpret_cast<T*>(smem + o.diag);
+    t.r = reinterpret_cast<T*>(smem + o.r);
+    t.A = reinterpret_cast<T*>(smem + o.A);
+    t.B = reinterpret_cast<T*>(smem + o.B);
+  }
+  __syncthreads();
+  {
+    const LdsLevel<T>& t = tab[0];
+    for (int i = tid; i < t.n; i += nth) {
+      t.r[i] = g0.r[i];
+      t.A[i] = g0.A[i];
     }
-    for (int i = tid; i < n; i += nth) t.diag[i] = g.diag[i];
-    if (l == l0)
-      for (int i = tid; i < n; i += nth) {
-        t.r[i] = g.r[i];
-        t.A[i] = g.A[i];
-      }
   }
   __syncthreads();
   // a level of n rows gets G lanes per row (a power of two, at most 16, as many as the
@@ -381,30 +391,84 @@ __global__ __launch_bounds__(1024) void k_small_levels_lds(const LevelDesc<T>* _
   }
   {
     const LdsLevel<T>& t = tab[0];
-    T* out = lvs[l0].B;
+    T* out = g0.B;
     for (int i = tid; i < t.n; i += nth) out[i] = t.B[i];
   }
 }
 
-// LDS bytes needed to stage levels l0.. of one component (level table included)
+// LDS image of levels l0.. of one component: the LdsImgLevel table, then per level rp,
+// col, val, diag (copied from `bytes` below) and the r, A, B work vectors (not copied).
+// Returns the LDS bytes of the whole layout; with out != nullptr also writes the copied
+// part (16-byte multiple) and sets *copy_bytes.
 template <typename T>
-size_t lds_bytes_from(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, int l0) {
-  size_t b = lds_align(sizeof(LdsLevel<T>) * MAX_LEVELS);
-  for (size_t l = l0; l < n.size(); ++l) b += lds_level_bytes<T>((int)n[l], (int)nnz[l]);
-  return b;
+size_t lds_image(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, int l0,
+                 const SfcLevelsHost* h, std::vector<unsigned char>* out, size_t* copy_bytes) {
+  const int L = (int)n.size();
+  std::vector<LdsImgLevel> tab(MAX_LEVELS, LdsImgLevel{0, 0, 0, 0, 0, 0, 0, 0});
+  size_t off = lds_align(sizeof(LdsImgLevel) * MAX_LEVELS);
+  for (int l = l0; l < L; ++l) {
+    LdsImgLevel& t = tab[l - l0];
+    t.n = (int)n[l];
+    t.rp = (int)off;
+    off += lds_align(sizeof(int) * (n[l] + 1));
+    t.col = (int)off;
+    off += lds_align(sizeof(int) * nnz[l]);
+    t.val = (int)off;
+    off += lds_align(sizeof(T) * nnz[l]);
+    t.diag = (int)off;
+    off += lds_align(sizeof(T) * n[l]);
+  }
+  const size_t copied = off;
+  for (int l = l0; l < L; ++l) {
+    LdsImgLevel& t = tab[l - l0];
+    t.r = (int)off;
+    off += lds_align(sizeof(T) * n[l]);
+    t.A = (int)off;
+    off += lds_align(sizeof(T) * n[l]);
+    t.B = (int)off;
+    off += lds_align(sizeof(T) * n[l]);
+  }
+  if (out) {
+    out->assign(copied, 0);
+    std::memcpy(out->data(), tab.data(), sizeof(LdsImgLevel) * MAX_LEVELS);
+    for (int l = l0; l < L; ++l) {
+      const LdsImgLevel& t = tab[l - l0];
+      int32_t* rp = reinterpret_cast<int32_t*>(out->data() + t.rp);
+      int32_t* col = reinterpret_cast<int32_t*>(out->data() + t.col);
+      T* val = reinterpret_cast<T*>(out->data() + t.val);
+      T* diag = reinterpret_cast<T*>(out->data() + t.diag);
+      for (int64_t i = 0; i <= n[l]; ++i) rp[i] = h->rowptr[l][i];
+      for (int64_t k = 0; k < nnz[l]; ++k) {
+        col[k] = h->col[l][k];
+        val[k] = (T)h->val[l][k];
+      }
+      for (int64_t i = 0; i < n[l]; ++i) diag[i] = (T)h->diag[l][i];
+    }
+    *copy_bytes = copied;
+  }
+  return off;
 }
 
-// first level from which the remaining hierarchy fits in `budget` bytes of LDS (FP64)
+template <typename T>
+size_t lds_bytes_from(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, int l0) {
+  return lds_image<T>(n, nnz, l0, nullptr, nullptr, nullptr);
+}
+
+// first level from which the remaining hierarchy fits in `budget` bytes of LDS in
+// precision T
+template <typename T>
 int lds_start_level(const std::vector<int64_t>& n, const std::vector<int64_t>& nnz, size_t budget) {
   const int L = (int)n.size();
   if (L > MAX_LEVELS) return -1;
   int l0 = L - 1;
   for (int l = L - 1; l >= 0; --l) {
-    if (lds_bytes_from<double>(n, nnz, l) <= budget) l0 = l;
+    if (lds_bytes_from<T>(n, nnz, l) <= budget) l0 = l;
     else break;
   }
   return l0;
 }
+template int lds_start_level<double>(const std::vector<int64_t>&, const std::vector<int64_t>&, size_t);
+template int lds_start_level<float>(const std::vector<int64_t>&, const std::vector<int64_t>&, size_t);
 
 // The dynamic-LDS cap is a property of the kernel, shared by every context: it only
 // ever grows, so a context set up later never lowers what another one launches with.
@@ -446,7 +510,7 @@ static T* upload_vec(dgmi_ctx* c, const std::vector<U>& h) {
 template <typename T>
 int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s) {
   const int ncg = (int)c->mg.cg_nonods;
-  const dim3 blk(256), grd((ncg + 255) / 256);
+  const dim3 blk(256), grd((unsigned)(((int64_t)ncg * RESTRICT_LANES + 255) / 256));
   const T* rw = std::is_same<T, double>::value ? (const T*)c->mg.rw64 : (const T*)c->mg.rw32;
   if (c->D == 3)
     hipLaunchKernelGGL((k_restrict<T, 3>), grd, blk, 0, s, ncg, c->mg.rptr, c->mg.rent, rw, r_dg, r_sfc);
@@ -507,7 +571,9 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   auto& lv = mg.lv;
   const bool f64 = std::is_same<T, double>::value;
   const T omega = (T)c->omega;
-  const int l0 = mg.small_level;
+  const int l0 = f64 ? mg.small_level : mg.small_level32;
+  const std::vector<int64_t>& grid_rows = f64 ? lv.grid_rows : lv.grid_rows32;
+  const bool any_start0 = f64 ? lv.any_start0 : lv.any_start0_32;
   auto R = [&](int l) -> T* { return l == 0 ? const_cast<T*>(r_sfc) : vec<T>(lv.r[l]); };
   auto A = [&](int l) -> T* { return vec<T>(lv.e[l]); };
   auto B = [&](int l) -> T* { return l == 0 ? e_sfc : vec<T>(lv.e2[l]); };
@@ -518,7 +584,7 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)nullptr, A(0));
   // down-cycle on grid-sized levels
   for (int l = 0; l < l0; ++l) {
-    const int n = (int)lv.grid_rows[l];
+    const int n = (int)grid_rows[l];
     const dim3 grd((unsigned)(((int64_t)n * ROW_LANES + 255) / 256));
     T* cur = A(l);
     T* nxt = B(l);
@@ -535,16 +601,19 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   }
   // the level tables point level 0 at internal buffers, so components that start at
   // level 0 see the caller's vectors through them
-  if (lv.any_start0)
+  if (any_start0)
     hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], r_sfc, vec<T>(lv.r[0]));
-  hipLaunchKernelGGL((k_small_levels_lds<T>), dim3(c->ncomp), dim3(1024), f64 ? lv.lds64 : lv.lds32,
-                     s, (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32), (const int32_t*)lv.rng,
-                     omega, 3, 3);
-  if (lv.any_start0)
+  hipLaunchKernelGGL((k_small_levels_lds<T>), dim3(c->ncomp), dim3(SMALL_THREADS),
+                     f64 ? lv.lds64 : lv.lds32, s,
+                     (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32),
+                     (const int32_t*)(f64 ? lv.rng64 : lv.rng32),
+                     (const uint4*)(f64 ? lv.img64 : lv.img32),
+                     (const int64_t*)(f64 ? lv.imginfo64 : lv.imginfo32), omega, 3, 3);
+  if (any_start0)
     hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)vec<T>(lv.e2[0]), e_sfc);
   // up-cycle on grid-sized levels
   for (int l = l0 - 1; l >= 0; --l) {
-    const int n = (int)lv.grid_rows[l];
+    const int n = (int)grid_rows[l];
     const dim3 grd((unsigned)(((int64_t)n * ROW_LANES + 255) / 256));
     // (A + P B_{l+1}) -> B, B -> A, A -> B
     hipLaunchKernelGGL((k_sweep<T>), grd, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l), dia(l),
@@ -609,6 +678,7 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
   lv.n.assign(S + 1, 0);
   lv.nnz.assign(S + 1, 0);
   lv.grid_rows.assign(S, 0);
+  lv.grid_rows32.assign(S, 0);
   lv.rowptr.assign(S + 1, nullptr);
   lv.col.assign(S + 1, nullptr);
   lv.val64.assign(S + 1, nullptr);
@@ -631,6 +701,7 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
       segoff[l][q] = rows;
       rpoff[l][q] = (int64_t)rp.size() - 1;
       if (l < S && lay.start[q] > l) lv.grid_rows[l] = rows + seg;
+      if (l < S && lay.start32[q] > l) lv.grid_rows32[l] = rows + seg;
       for (int64_t i = 0; i < seg; ++i) {
         if (i < h.n[l]) {
           for (int32_t k = h.rowptr[l][i]; k < h.rowptr[l][i + 1]; ++k) {
@@ -689,70 +760,88 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
   if (!lv.pool_rp || !lv.pool_col || !lv.pool_val64 || !lv.pool_diag64 || !lv.pool_val32 ||
       !lv.pool_diag32 || !lv.pool_vec)
     return fail();
-  // ---- level tables, components in layout order (one work-group each)
+  // ---- level tables, components in layout order (one work-group each).  A level from
+  // the start up to the component's FP64 start is read from the grid layout, deeper ones
+  // from the pool.
   std::vector<LevelDesc<double>> d64((size_t)nc * MAX_LEVELS);
   std::vector<LevelDesc<float>> d32((size_t)nc * MAX_LEVELS);
-  std::vector<int32_t> rng(2 * nc);
+  std::vector<int32_t> rng64(2 * nc), rng32(2 * nc);
   size_t lds64 = 0, lds32 = 0;
   lv.any_start0 = false;
+  lv.any_start0_32 = false;
+  auto fill = [&](auto& d, int q, int l, auto* val_grid, auto* diag_grid, auto* val_pool,
+                  auto* diag_pool) {
+    using TT = std::remove_pointer_t<decltype(val_grid)>;
+    const SfcLevelsHost& h = lay.lh[q];
+    d.n = (int)h.n[l];
+    if (l <= lay.start[q]) {
+      const int64_t so = segoff[l][q];
+      d.cbase = (int)so;
+      d.rp = lv.rowptr[l] + rpoff[l][q];
+      d.col = lv.col[l];
+      d.val = val_grid;
+      d.diag = diag_grid + so;
+      d.r = (TT*)lv.r[l] + so;
+      d.A = (TT*)lv.e[l] + so;
+      d.B = (TT*)lv.e2[l] + so;
+    } else {
+      const PoolRef& p = pref[q][l];
+      d.cbase = 0;
+      d.rp = lv.pool_rp + p.rp;
+      d.col = lv.pool_col;
+      d.val = val_pool;
+      d.diag = diag_pool + p.diag;
+      TT* v = (TT*)lv.pool_vec + p.vec;  // FP32 views use the first half
+      d.r = v;
+      d.A = v + h.n[l];
+      d.B = v + 2 * h.n[l];
+    }
+  };
+  std::vector<unsigned char> img64, img32, piece;
+  std::vector<int64_t> info64(2 * nc), info32(2 * nc);
   for (int b = 0; b < nc; ++b) {
     const int q = lay.order[b];
     const SfcLevelsHost& h = lay.lh[q];
-    const int s0 = lay.start[q];
+    const int s0 = lay.start[q], s32 = lay.start32[q];
     if (h.nlevel > MAX_LEVELS) {
       set_error("too many SFC levels");
       return -2;
     }
-    rng[2 * b] = s0;
-    rng[2 * b + 1] = h.nlevel;
+    rng64[2 * b] = s0;
+    rng64[2 * b + 1] = h.nlevel;
+    rng32[2 * b] = s32;
+    rng32[2 * b + 1] = h.nlevel;
     lv.any_start0 |= s0 == 0;
+    lv.any_start0_32 |= s32 == 0;
     std::vector<int64_t> nnzq(h.nlevel);
     for (int l = 0; l < h.nlevel; ++l) nnzq[l] = (int64_t)h.col[l].size();
-    lds64 = std::max(lds64, lds_bytes_from<double>(h.n, nnzq, s0));
-    lds32 = std::max(lds32, lds_bytes_from<float>(h.n, nnzq, s0));
-    for (int l = s0; l < h.nlevel; ++l) {
-      LevelDesc<double>& a = d64[(size_t)b * MAX_LEVELS + l];
-      LevelDesc<float>& f = d32[(size_t)b * MAX_LEVELS + l];
-      a.n = f.n = (int)h.n[l];
-      if (l == s0) {
-        const int64_t so = segoff[l][q];
-        a.cbase = f.cbase = (int)so;
-        a.rp = f.rp = lv.rowptr[l] + rpoff[l][q];
-        a.col = f.col = lv.col[l];
-        a.val = lv.val64[l];
-        f.val = lv.val32[l];
-        a.diag = lv.diag64[l] + so;
-        f.diag = lv.diag32[l] + so;
-        a.r = (double*)lv.r[l] + so;
-        a.A = (double*)lv.e[l] + so;
-        a.B = (double*)lv.e2[l] + so;
-        f.r = (float*)lv.r[l] + so;
-        f.A = (float*)lv.e[l] + so;
-        f.B = (float*)lv.e2[l] + so;
-      } else {
-        const PoolRef& p = pref[q][l];
-        a.cbase = f.cbase = 0;
-        a.rp = f.rp = lv.pool_rp + p.rp;
-        a.col = f.col = lv.pool_col;
-        a.val = lv.pool_val64;
-        f.val = lv.pool_val32;
-        a.diag = lv.pool_diag64 + p.diag;
-        f.diag = lv.pool_diag32 + p.diag;
-        double* v64 = (double*)lv.pool_vec + p.vec;
-        float* v32 = (float*)lv.pool_vec + p.vec;  // FP32 views use the first half
-        a.r = v64;
-        a.A = v64 + h.n[l];
-        a.B = v64 + 2 * h.n[l];
-        f.r = v32;
-        f.A = v32 + h.n[l];
-        f.B = v32 + 2 * h.n[l];
-      }
-    }
+    size_t cb = 0;
+    lds64 = std::max(lds64, lds_image<double>(h.n, nnzq, s0, &h, &piece, &cb));
+    info64[2 * b] = (int64_t)(img64.size() / 16);
+    info64[2 * b + 1] = (int64_t)(cb / 16);
+    img64.insert(img64.end(), piece.begin(), piece.end());
+    lds32 = std::max(lds32, lds_image<float>(h.n, nnzq, s32, &h, &piece, &cb));
+    info32[2 * b] = (int64_t)(img32.size() / 16);
+    info32[2 * b + 1] = (int64_t)(cb / 16);
+    img32.insert(img32.end(), piece.begin(), piece.end());
+    for (int l = s0; l < h.nlevel; ++l)
+      fill(d64[(size_t)b * MAX_LEVELS + l], q, l, lv.val64[std::min(l, S)],
+           lv.diag64[std::min(l, S)], lv.pool_val64, lv.pool_diag64);
+    for (int l = s32; l < h.nlevel; ++l)
+      fill(d32[(size_t)b * MAX_LEVELS + l], q, l, lv.val32[std::min(l, S)],
+           lv.diag32[std::min(l, S)], lv.pool_val32, lv.pool_diag32);
   }
   lv.desc64 = upload_vec<LevelDesc<double>>(c, d64);
   lv.desc32 = upload_vec<LevelDesc<float>>(c, d32);
-  lv.rng = upload_vec<int32_t>(c, rng);
-  if (!lv.desc64 || !lv.desc32 || !lv.rng) return fail();
+  lv.rng64 = upload_vec<int32_t>(c, rng64);
+  lv.rng32 = upload_vec<int32_t>(c, rng32);
+  lv.img64 = upload_vec<unsigned char>(c, img64);
+  lv.img32 = upload_vec<unsigned char>(c, img32);
+  lv.imginfo64 = upload_vec<int64_t>(c, info64);
+  lv.imginfo32 = upload_vec<int64_t>(c, info32);
+  if (!lv.desc64 || !lv.desc32 || !lv.rng64 || !lv.rng32 || !lv.img64 || !lv.img32 ||
+      !lv.imginfo64 || !lv.imginfo32)
+    return fail();
   lv.lds64 = lds64;
   lv.lds32 = lds32;
   if (int rc = set_small_lds_attr<double>(lds64)) return rc;

@@ -71,4 +71,34 @@ __device__ __forceinline__ void load_S_row(const T* __restrict__ S, int n, int e
   }
 }
 
+// FP32 smoother matrices are stored packed: the block inverse is symmetric, so entry
+// (a, b), a <= b, of element e at sym_index(a, b)*nele + e carries S[a][b] and S[b][a]
+// (NL(NL+1)/2 vectors of nele values instead of NL*NL).
+template <int NL>
+__host__ __device__ constexpr int sym_index(int a, int b) {
+  return a * NL - a * (a - 1) / 2 + (b - a);
+}
+template <int NL>
+constexpr int sym_size() {
+  return NL * (NL + 1) / 2;
+}
+
+template <int NL>
+__device__ __forceinline__ void load_S_row_sym(const float* __restrict__ S, int n, int e,
+                                               int nele, float* v) {
+#pragma unroll
+  for (int m = 0; m < NL; ++m) {
+    const int a = m < n ? m : n, b = m < n ? n : m;
+    v[m] = S[(size_t)sym_index<NL>(a, b) * nele + e];
+  }
+}
+
+// row n of the smoother of element e in the layout of its precision (FP64 full, FP32 packed)
+template <typename T, int NL>
+__device__ __forceinline__ void load_smoother_row(const T* __restrict__ S, int n, int e, int nele,
+                                                  T* v) {
+  if constexpr (sizeof(T) == 4) load_S_row_sym<NL>(S, n, e, nele, v);
+  else load_S_row<T, NL>(S, n, e, nele, v);
+}
+
 }  // namespace dgmi

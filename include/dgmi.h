@@ -149,6 +149,11 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     2 = MFMA with bf16 inputs.
  *   "graph_vcycle"    1 = replay the V-cycle from a captured HIP graph (default), 0 = launch
  *                     its kernels one by one.
+ *   "smooth_offdiag"  FP32 sweeps of the exact block smoother (DGMI_SMOOTH_DIRECT) inside
+ *                     the preconditioner: 1 = x = (1 - omega) x + S (b - A_off x), which
+ *                     needs no volume term and only the neighbour half of the face terms
+ *                     (default); 0 = x += S (b - A x).  Equal in exact arithmetic; FP64
+ *                     sweeps and dgmi_smooth always use the second form.
  *   "mfma_waves", "bf16_waves"  waves (16 elements each) per work-group of the fp32 / bf16
  *                     matrix-core applies: 2, 4 (default) or 8. */
 int dgmi_set_param(dgmi_ctx* ctx, const char* name, double value);

@@ -65,6 +65,29 @@ def test_preconditioner_matches_oracle(prob):
     assert rel(out32, ref) < 1e-4
 
 
+@pytest.mark.parametrize("D,n", [(2, 8), (3, 4)], ids=["2d", "3d"])
+def test_direct_smoother_fp32_neighbour_form(D, n):
+    """With the exact block smoother the FP32 sweeps inside the preconditioner use
+    x = (1 - omega) x + S (b - A_off x).  It matches the FP64 oracle (which sweeps
+    x += S (b - A x)) to single precision and agrees with the FP32 residual form."""
+    print(f"[case] D={D} n={n} direct smoother", flush=True)
+    m, pb, g, sp = build(D, n, sigma=1.0, blk_solver="direct", face_kappa="harmonic")
+    g._get_RAR_and_sfc_data_Um()
+    mgd = mg.build_mg(sp, m.verts)
+    b = np.random.default_rng(6).standard_normal(sp.ndof)
+    ref = mg.um_left_precond(sp, mgd, b, blk_solver="direct")
+    assert rel(to_np(g._um_left_precond(g.new_vector(), to_dev(b))), ref) < 1e-10
+    out = {}
+    for offd in (1, 0):
+        g.lib.dgmi_set_param(g.ctx, b"smooth_offdiag", float(offd))
+        out[offd] = to_np(g._um_left_precond(g.new_vector(torch.float32),
+                                             to_dev(b, torch.float32)))
+        print("offdiag", offd, "rel err", rel(out[offd], ref), flush=True)
+        assert rel(out[offd], ref) < 1e-4
+    assert rel(out[1], out[0]) < 1e-4
+    g.lib.dgmi_set_param(g.ctx, b"smooth_offdiag", 1.0)
+
+
 def test_grid_level_kernels_match_oracle(monkeypatch):
     """Force the grid-sized SFC-level kernels (k_sweep / k_sweep_restrict) on a small mesh
     by lowering the single-work-group threshold, and compare V-cycle and preconditioner."""
