@@ -45,6 +45,7 @@ SIGNATURES = {
     "dgmi_element_tables": (C.c_int, [_P, _DP, _DP, _DP, _DP, _DP, _DP, _I64P]),
     "dgmi_apply": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "dgmi_rhs": (C.c_int, [_P, _P, _P, _P, _P]),
+    "dgmi_l2_error": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "dgmi_bdiag": (C.c_int, [_P, _P, _P, _P]),
     "dgmi_smoother_setup": (C.c_int, [_P, C.c_int, C.c_double, _P]),
     "dgmi_smooth": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),

@@ -97,6 +97,13 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   T xv[NL], acc[NL];
 #pragma unroll
   for (int n = 0; n < NL; ++n) acc[n] = 0;
+  // this wave's right-hand-side rows, requested before the face and volume work
+  T bv[NPW];
+#pragma unroll
+  for (int k = 0; k < NPW; ++k) {
+    const int n = w * NPW + k;
+    bv[k] = (MODE != 0 && MODE != 4 && valid && n < NL) ? b[(size_t)e * NL + n] : T(0);
+  }
   if (valid) {
     load_row<T, NL>(x + (size_t)e * NL, xv);
   }
@@ -142,10 +149,10 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     const int code = fcode[z];
     const int type = code & 15;
     if (OFFD ? type == 0 : type < 2) {
-      const int a = code >> 4;
       T c[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
+      const int a = code >> 4;
       const T wfe = wf[f * nele + e];
       const T pe = pen[f * nele + e];
       int nbe = 0, nbf = 0;
@@ -204,7 +211,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
       for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
       if constexpr (MODE == 0) rv[k] = -s;
       else if constexpr (MODE == 4) rv[k] = s;
-      else rv[k] = valid ? b[(size_t)e * NL + n] - s : T(0);
+      else rv[k] = valid ? bv[k] - s : T(0);
     }
   }
   if constexpr (MODE <= 1 || MODE == 4) {
@@ -604,6 +611,59 @@ int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t
   DGMI_HIP(hipGetLastError());
   return 0;
 }
+
+// Per block of 256 elements: partial[b] = sum_e |detJ_e| sum_n cn[n] d[e,n]^2 and
+// partial[nb + b] = max |d|, d = x_num - x_ana, cn[n] = sum_g w_g n[n,g]^2 (the
+// reference's l2 squares every basis term separately).  FP64 sums in a fixed tree.
+template <typename T, int NL>
+__global__ __launch_bounds__(256) void k_l2_error(int nele, const T* __restrict__ xn,
+                                                  const T* __restrict__ xa,
+                                                  const double* __restrict__ cn,
+                                                  const double* __restrict__ detJ,
+                                                  double* __restrict__ partial) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0, m = 0.0;
+  if (e < nele) {
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      const double d = (double)xn[(size_t)e * NL + n] - (double)xa[(size_t)e * NL + n];
+      s += cn[n] * d * d;
+      m = fmax(m, fabs(d));
+    }
+    s *= fabs(detJ[e]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off, 64);
+    m = fmax(m, __shfl_xor(m, off, 64));
+  }
+  __shared__ double ss[4], sm[4];
+  if ((threadIdx.x & 63) == 0) {
+    ss[threadIdx.x >> 6] = s;
+    sm[threadIdx.x >> 6] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = (ss[0] + ss[1]) + (ss[2] + ss[3]);
+    partial[gridDim.x + blockIdx.x] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+  }
+}
+
+template <typename T>
+int launch_l2_error(dgmi_ctx* c, const T* xn, const T* xa, const double* cn, double* partial,
+                    int nb, hipStream_t s) {
+  if (c->D == 3)
+    hipLaunchKernelGGL((k_l2_error<T, 20>), dim3(nb), dim3(256), 0, s, (int)c->nele, xn, xa, cn,
+                       c->detJ, partial);
+  else
+    hipLaunchKernelGGL((k_l2_error<T, 10>), dim3(nb), dim3(256), 0, s, (int)c->nele, xn, xa, cn,
+                       c->detJ, partial);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+template int launch_l2_error<double>(dgmi_ctx*, const double*, const double*, const double*,
+                                     double*, int, hipStream_t);
+template int launch_l2_error<float>(dgmi_ctx*, const float*, const float*, const double*,
+                                    double*, int, hipStream_t);
 
 int launch_bdiag(dgmi_ctx* c, const double* refm, double* bdiag, hipStream_t s) {
   const int nele = (int)c->nele;

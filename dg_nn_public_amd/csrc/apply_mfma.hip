@@ -426,22 +426,23 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
         }
       }
     __syncthreads();
+    // the four lanes of an element take NL/4 consecutive rows each (the whole residual
+    // row is in rx), so every lane issues the same number of smoother loads
     if (valid) {
+      constexpr int RPL = (NL + 3) / 4;
 #pragma unroll
-      for (int mt = 0; mt < S::MT; ++mt)
+      for (int i = 0; i < RPL; ++i) {
+        const int n = RPL * q + i;
+        if (n < NL) {
+          float srow[NL];
+          load_S_row_sym<NL>(Sm, n, el, nele, srow);
+          const size_t k = (size_t)el * NL + n;
+          float s = OFFD ? (1.0f - omega) * x[k] : x[k];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int n = 16 * mt + 4 * q + i;
-          if (n < NL) {
-            float srow[NL];
-            load_S_row_sym<NL>(Sm, n, el, nele, srow);
-            const size_t k = (size_t)el * NL + n;
-            float s = OFFD ? (1.0f - omega) * x[k] : x[k];
-#pragma unroll
-            for (int m = 0; m < NL; ++m) s += srow[m] * rx[wl][le][m];
-            x[k] = s;
-          }
+          for (int m = 0; m < NL; ++m) s += srow[m] * rx[wl][le][m];
+          x[k] = s;
         }
+      }
     }
   }
 }

@@ -139,6 +139,8 @@ struct Scratch {
   float* vec32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double* cg64[2] = {nullptr, nullptr};
   float* cg32[2] = {nullptr, nullptr};
+  double* l2cn = nullptr;    // (nloc) sum_g w_g n[n,g]^2
+  double* l2part = nullptr;  // (2 * element blocks) partial sums and maxima
 };
 static Scratch& scratch(dgmi_ctx* c);
 
@@ -779,6 +781,44 @@ int dgmi_rhs(dgmi_ctx* c, const double* f, const double* u_bc, double* b, void* 
   return launch_rhs(c, sc.refm, c->detJ, f, u_bc, b, s);
 }
 
+int dgmi_l2_error(dgmi_ctx* c, int dtype, const void* x_num, const void* x_ana, double* out,
+                  void* stream) {
+  DGMI_CHECK(c && x_num && x_ana && out, "null argument");
+  DGMI_CHECK(dtype == DGMI_F64 || dtype == DGMI_F32, "dtype must be DGMI_F64 or DGMI_F32");
+  hipStream_t s = (hipStream_t)stream;
+  Scratch& sc = scratch(c);
+  const int nb = (int)((c->nele + 255) / 256);
+  if (!sc.l2cn) {
+    // cn[n] = sum_g w_g n[n,g]^2
+    std::vector<double> cn(c->nloc, 0.0);
+    for (int n = 0; n < c->nloc; ++n)
+      for (int g = 0; g < c->ngi; ++g) {
+        const double v = c->el.n[(size_t)n * c->ngi + g];
+        cn[n] += c->el.weight[g] * v * v;
+      }
+    sc.l2cn = upload<double>(c, cn);
+    sc.l2part = static_cast<double*>(c->alloc(sizeof(double) * 2 * nb));
+    DGMI_CHECK(sc.l2cn && sc.l2part, alloc_failure());
+  }
+  if (dtype == DGMI_F64)
+    DGMI_TRY(launch_l2_error<double>(c, (const double*)x_num, (const double*)x_ana, sc.l2cn,
+                                     sc.l2part, nb, s));
+  else
+    DGMI_TRY(launch_l2_error<float>(c, (const float*)x_num, (const float*)x_ana, sc.l2cn,
+                                    sc.l2part, nb, s));
+  std::vector<double> h(2 * nb);
+  DGMI_HIP(hipMemcpyAsync(h.data(), sc.l2part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s));
+  DGMI_HIP(hipStreamSynchronize(s));
+  double l2 = 0.0, linf = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    l2 += h[b];
+    linf = std::max(linf, h[nb + b]);
+  }
+  out[0] = std::sqrt(l2);
+  out[1] = linf;
+  return 0;
+}
+
 int dgmi_bdiag(dgmi_ctx* c, double* bdiag, double* diag, void* stream) {
   DGMI_CHECK(c, "null context");
   hipStream_t s = (hipStream_t)stream;
@@ -1030,6 +1070,7 @@ int dgmi_mg_info(const dgmi_ctx* c, int64_t* info, int cap) {
   info[2] = c->mg.ncolor;
   info[3] = L;
   for (int l = 0; l < L; ++l) info[4 + l] = n[l];
+  if (cap >= 5 + L) info[4 + L] = c->mg.exact32.empty() ? -1 : c->mg.exact32[0];
   return 0;
 }
 
@@ -1175,6 +1216,17 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "mfma_waves" || k == "bf16_waves") {
     DGMI_CHECK(value == 2.0 || value == 4.0 || value == 8.0, k + " must be 2, 4 or 8");
     (k == "mfma_waves" ? c->mf_wpb : c->bf_wpb) = (int)value;
+    return 0;
+  }
+  if (k == "coarse_exact") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "coarse_exact must be 0 or 1");
+    c->mg.coarse_exact = (int)value;
+    // a captured V-cycle holds the old choice
+    for (VGraph* vg : {&c->mg.graph64, &c->mg.graph32})
+      if (vg->exec) {
+        DGMI_HIP(hipGraphExecDestroy(vg->exec));
+        vg->exec = nullptr;
+      }
     return 0;
   }
   if (k == "smooth_offdiag") {

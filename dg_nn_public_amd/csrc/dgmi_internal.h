@@ -84,6 +84,10 @@ struct SfcLevelsDev {
   void* img32 = nullptr;
   int64_t* imginfo64 = nullptr;
   int64_t* imginfo32 = nullptr;
+  // FP32 exact coarse solves: inverses (column-major) and per component (offset, level or
+  // -1), components in layout order
+  float* cinv32 = nullptr;
+  int64_t* cinfo32 = nullptr;
   bool any_start0 = false;    // some component's FP64 kernel starts at level 0
   bool any_start0_32 = false;
   size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the single-work-group kernel
@@ -146,6 +150,10 @@ struct MGDev {
   SfcLevelsDev lv;
   int small_level = 0;    // FP64 grid kernels sweep levels below this one
   int small_level32 = 0;  // FP32 grid kernels sweep levels below this one
+  // FP32 cycles solve the first level of at most 128 rows exactly (default); 0 runs every
+  // level's sweeps as the FP64 cycle does
+  int coarse_exact = 1;
+  std::vector<int> exact32;  // per component (context order): exact level or -1
   // level sizes of every component (host), components in context order
   std::vector<std::vector<int64_t>> comp_levels;
   // V-cycle replayed from a captured graph (set_param "graph_vcycle")
@@ -257,6 +265,10 @@ int launch_rhs(dgmi_ctx* c, const double* refm, const double* detJ, const double
 template <typename T>
 int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t s);
 int launch_bdiag(dgmi_ctx* c, const double* refmats, double* bdiag, hipStream_t s);
+// partial (2 nb doubles): per block of 256 elements the l2^2 part and the max |x_num - x_ana|
+template <typename T>
+int launch_l2_error(dgmi_ctx* c, const T* xn, const T* xa, const double* cn, double* partial,
+                    int nb, hipStream_t s);
 template <typename T>
 int launch_build_S_jacobi(dgmi_ctx* c, const double* bdiag, double omega, T* S, hipStream_t s);
 // omega * inverse of every diagonal block into both smoother copies; *fail = 1 + element

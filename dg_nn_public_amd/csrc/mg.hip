@@ -4,6 +4,7 @@
 // per-V-cycle argsort/permutation (z02/volume_mf_diff.py:1992-2007) is folded into the
 // transfer kernels' index lists.  The 1-D SFC levels are CSR matrices; level l+1
 // aggregates positions (2I, 2I+1) of level l.
+#include <cmath>
 #include <cstring>
 #include <mutex>
 
@@ -239,16 +240,32 @@ struct LdsImgLevel {
 
 __host__ __device__ inline size_t lds_align(size_t x) { return (x + 15) & ~size_t(15); }
 
+// Optional exact coarse solve (cinv != nullptr): cinfo[2b] = offset of the component's
+// inverse in cinv (column-major, n x n, n <= EXACT_MAX_N), cinfo[2b+1] = its level c or
+// -1.  With c >= 0 the cycle stops at level c and sets B_c = A_c^-1 r_c there; the levels
+// below c are not visited.
+constexpr int EXACT_MAX_N = 128;
+
 template <typename T>
 __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     const LevelDesc<T>* __restrict__ descs, const int32_t* __restrict__ rng,
     const uint4* __restrict__ img, const int64_t* __restrict__ imginfo, T omega, int pre,
-    int post) {
+    int post, const T* __restrict__ cinv, const int64_t* __restrict__ cinfo) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ LdsLevel<T> tab[MAX_LEVELS];
+  __shared__ T csum[SMALL_THREADS / EXACT_MAX_N][EXACT_MAX_N];
   const int tid = threadIdx.x;
   constexpr int nth = SMALL_THREADS;
-  const int l0 = rng[2 * blockIdx.x], L = rng[2 * blockIdx.x + 1];
+  const int l0 = rng[2 * blockIdx.x];
+  int L = rng[2 * blockIdx.x + 1];
+  const T* ainv = nullptr;
+  if (cinv) {
+    const int c = (int)cinfo[2 * blockIdx.x + 1];
+    if (c >= l0 && c < L) {
+      L = c + 1;
+      ainv = cinv + cinfo[2 * blockIdx.x];
+    }
+  }
   const LevelDesc<T>& g0 = descs[(size_t)blockIdx.x * MAX_LEVELS + l0];
   {
     const uint4* src = img + imginfo[2 * blockIdx.x];
@@ -284,7 +301,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
   }
   __syncthreads();
   {
-    const LdsLevel<T>& t = tab[0];
+    const LdsLevel<T> t = tab[0];
     for (int i = tid; i < t.n; i += nth) {
       t.r[i] = g0.r[i];
       t.A[i] = g0.A[i];
@@ -299,9 +316,30 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     while (g < 16 && 2 * g * n <= nth) g *= 2;
     return g;
   };
-  auto row = [&](const LdsLevel<T>& t, int i, int sub, int G, const T* ein, const T* ec) {
+  // a lane's entries of a row are taken four at a time, so their LDS loads are in flight
+  // together (the single work-group is latency-bound)
+  // level descriptors are copied out of LDS into registers (by value), so the pointer
+  // loads are not repeated after every LDS store
+  auto row = [&](const LdsLevel<T> t, int i, int sub, int G, const T* ein, const T* ec) {
     T s = 0;
-    for (int q = t.rp[i] + sub; q < t.rp[i + 1]; q += G) {
+    int q = t.rp[i] + sub;
+    const int q1 = t.rp[i + 1];
+    for (; q + 3 * G < q1; q += 4 * G) {
+      const int j0 = t.col[q], j1 = t.col[q + G], j2 = t.col[q + 2 * G], j3 = t.col[q + 3 * G];
+      const T v0 = t.val[q], v1 = t.val[q + G], v2 = t.val[q + 2 * G], v3 = t.val[q + 3 * G];
+      T e0 = ein[j0], e1 = ein[j1], e2 = ein[j2], e3 = ein[j3];
+      if (ec) {
+        e0 += ec[j0 >> 1];
+        e1 += ec[j1 >> 1];
+        e2 += ec[j2 >> 1];
+        e3 += ec[j3 >> 1];
+      }
+      s += v0 * e0;
+      s += v1 * e1;
+      s += v2 * e2;
+      s += v3 * e3;
+    }
+    for (; q < q1; q += G) {
       const int j = t.col[q];
       T ej = ein[j];
       if (ec) ej += ec[j >> 1];
@@ -310,7 +348,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
   };
-  auto sweep = [&](const LdsLevel<T>& t, const T* ein, const T* ec, T* eout) {
+  auto sweep = [&](const LdsLevel<T> t, const T* ein, const T* ec, T* eout) {
     const int G = lanes_for(t.n), R = nth / G;
     for (int base = 0; base < t.n; base += R) {
       const int i = base + tid / G, sub = tid % G;
@@ -326,8 +364,8 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     __syncthreads();
   };
   for (int l = l0; l < L - 1; ++l) {
-    const LdsLevel<T>& t = tab[l - l0];
-    const LdsLevel<T>& tc = tab[l + 1 - l0];
+    const LdsLevel<T> t = tab[l - l0];
+    const LdsLevel<T> tc = tab[l + 1 - l0];
     T* cur = t.A;
     T* nxt = t.B;
     for (int it = 0; it < pre; ++it) {
@@ -367,13 +405,32 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     }
     __syncthreads();
   }
-  {
-    const LdsLevel<T>& t = tab[L - 1 - l0];
+  if (ainv) {
+    // B = A^-1 r: row i of 8 column groups, partial sums combined in LDS in a fixed order
+    const LdsLevel<T> t = tab[L - 1 - l0];
+    const int n = t.n;
+    constexpr int NG = SMALL_THREADS / EXACT_MAX_N;
+    const int i = tid % EXACT_MAX_N, grp = tid / EXACT_MAX_N;
+    const int per = (n + NG - 1) / NG;
+    T sacc = 0;
+    if (i < n)
+      for (int j = grp * per; j < min(n, (grp + 1) * per); ++j) sacc += ainv[(size_t)j * n + i] * t.r[j];
+    csum[grp][i] = sacc;
+    __syncthreads();
+    if (tid < n) {
+      T v = 0;
+#pragma unroll
+      for (int q = 0; q < NG; ++q) v += csum[q][tid];
+      t.B[tid] = v;
+    }
+    __syncthreads();
+  } else {
+    const LdsLevel<T> t = tab[L - 1 - l0];
     for (int i = tid; i < t.n; i += nth) t.B[i] = t.r[i] / t.diag[i];
     __syncthreads();
   }
   for (int l = L - 2; l >= l0; --l) {
-    const LdsLevel<T>& t = tab[l - l0];
+    const LdsLevel<T> t = tab[l - l0];
     const T* ec = tab[l + 1 - l0].B;
     sweep(t, t.A, ec, t.B);
     const T* cur = t.B;
@@ -390,7 +447,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     }
   }
   {
-    const LdsLevel<T>& t = tab[0];
+    const LdsLevel<T> t = tab[0];
     T* out = g0.B;
     for (int i = tid; i < t.n; i += nth) out[i] = t.B[i];
   }
@@ -608,7 +665,9 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
                      (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32),
                      (const int32_t*)(f64 ? lv.rng64 : lv.rng32),
                      (const uint4*)(f64 ? lv.img64 : lv.img32),
-                     (const int64_t*)(f64 ? lv.imginfo64 : lv.imginfo32), omega, 3, 3);
+                     (const int64_t*)(f64 ? lv.imginfo64 : lv.imginfo32), omega, 3, 3,
+                     (const T*)(f64 || !mg.coarse_exact ? nullptr : lv.cinv32),
+                     (const int64_t*)lv.cinfo32);
   if (any_start0)
     hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)vec<T>(lv.e2[0]), e_sfc);
   // up-cycle on grid-sized levels
@@ -660,6 +719,39 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   }
   DGMI_HIP(hipGraphLaunch(vg.exec, s));
   return 0;
+}
+
+// Inverse of the dense level-l operator of one component (Gauss-Jordan with partial
+// pivoting in FP64), column-major in FP32.  False if the level is singular.
+static bool dense_inverse_colmajor(const SfcLevelsHost& h, int l, std::vector<float>& out) {
+  const int n = (int)h.n[l];
+  std::vector<double> a((size_t)n * 2 * n, 0.0);  // [A | I], row-major
+  for (int i = 0; i < n; ++i) {
+    for (int32_t k = h.rowptr[l][i]; k < h.rowptr[l][i + 1]; ++k)
+      a[(size_t)i * 2 * n + h.col[l][k]] += h.val[l][k];
+    a[(size_t)i * 2 * n + n + i] = 1.0;
+  }
+  for (int col = 0; col < n; ++col) {
+    int p = col;
+    for (int r = col + 1; r < n; ++r)
+      if (std::fabs(a[(size_t)r * 2 * n + col]) > std::fabs(a[(size_t)p * 2 * n + col])) p = r;
+    const double pv = a[(size_t)p * 2 * n + col];
+    if (!(std::fabs(pv) > 0.0) || !std::isfinite(pv)) return false;
+    if (p != col)
+      for (int k = 0; k < 2 * n; ++k) std::swap(a[(size_t)p * 2 * n + k], a[(size_t)col * 2 * n + k]);
+    const double d = 1.0 / pv;
+    for (int k = 0; k < 2 * n; ++k) a[(size_t)col * 2 * n + k] *= d;
+    for (int r = 0; r < n; ++r) {
+      if (r == col) continue;
+      const double f = a[(size_t)r * 2 * n + col];
+      if (f == 0.0) continue;
+      for (int k = 0; k < 2 * n; ++k) a[(size_t)r * 2 * n + k] -= f * a[(size_t)col * 2 * n + k];
+    }
+  }
+  out.assign((size_t)n * n, 0.0f);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) out[(size_t)j * n + i] = (float)a[(size_t)i * 2 * n + n + j];
+  return true;
 }
 
 // Device layout of every component's SFC hierarchy (see SfcLevelsDev) and the level
@@ -831,6 +923,32 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
       fill(d32[(size_t)b * MAX_LEVELS + l], q, l, lv.val32[std::min(l, S)],
            lv.diag32[std::min(l, S)], lv.pool_val32, lv.pool_diag32);
   }
+  // FP32 exact coarse solve: the first level at or below a component's FP32 start with at
+  // most EXACT_MAX_N rows (and a coarser level below it) is inverted on the host
+  std::vector<float> cinv;
+  std::vector<int64_t> cinfo(2 * nc, -1);
+  for (int b = 0; b < nc; ++b) {
+    const int q = lay.order[b];
+    const SfcLevelsHost& h = lay.lh[q];
+    int cl = -1;
+    for (int l = lay.start32[q]; l < h.nlevel - 1; ++l)
+      if (h.n[l] <= EXACT_MAX_N) {
+        cl = l;
+        break;
+      }
+    cinfo[2 * b] = (int64_t)cinv.size();
+    if (cl < 0) continue;
+    std::vector<float> inv;
+    if (dense_inverse_colmajor(h, cl, inv)) {
+      cinfo[2 * b + 1] = cl;
+      cinv.insert(cinv.end(), inv.begin(), inv.end());
+    }
+  }
+  lv.cinv32 = upload_vec<float>(c, cinv);
+  lv.cinfo32 = upload_vec<int64_t>(c, cinfo);
+  if (!lv.cinv32 || !lv.cinfo32) return fail();
+  mg.exact32.assign(nc, -1);
+  for (int b = 0; b < nc; ++b) mg.exact32[lay.order[b]] = (int)cinfo[2 * b + 1];
   lv.desc64 = upload_vec<LevelDesc<double>>(c, d64);
   lv.desc32 = upload_vec<LevelDesc<float>>(c, d32);
   lv.rng64 = upload_vec<int32_t>(c, rng64);

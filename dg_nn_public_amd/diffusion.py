@@ -261,6 +261,8 @@ class DGDiffusion:
         check(self.lib.dgmi_mg_info(self.ctx, info, 128), "dgmi_mg_info")
         self.cg_nonods, self.ncola, self.ncolor, self.nlevel = info[0], info[1], info[2], info[3]
         self.nodes_per_level = [info[4 + i] for i in range(self.nlevel)]
+        # FP32 cycles: level solved exactly (-1: none)
+        self.coarse_exact_level = int(info[4 + self.nlevel])
         self._mg_ready = True
         return 0
 
@@ -385,15 +387,14 @@ class DGDiffusion:
 
     def get_l2_error(self, x_num, x_ana):
         """(l2, linf) with the reference's definition (z02/volume_mf_diff.py:2032-2056):
-        l2^2 = sum_e sum_n sum_g (n[n,g] (x_num - x_ana)[e,n])^2 detwei[e,g]."""
-        tabs = self.element_tables()
-        d = (x_num.view(-1) - x_ana.view(-1)).double().cpu().numpy().reshape(self.nele, self.nloc)
-        D = self.D
-        v = self.x_ref_in[:, :, :D + 1]                     # vertices (nele, D, D+1)
-        J = v[:, :, :D] - v[:, :, D:D + 1]                  # columns x_k - x_D
-        det = np.linalg.det(np.transpose(J, (0, 2, 1)))
-        l2 = np.sqrt(np.einsum("ng,g,en,e->", tabs["n"] ** 2, tabs["weight"], d ** 2, det))
-        return l2, float(np.abs(d).max())
+        l2^2 = sum_e sum_n sum_g (n[n,g] (x_num - x_ana)[e,n])^2 detwei[e,g], computed on
+        the device (FP64 sums)."""
+        x_num = self._vec(x_num.view(-1), "x_num")
+        x_ana = self._vec(x_ana.view(-1), "x_ana", (x_num.dtype,))
+        out = (C.c_double * 2)()
+        check(self.lib.dgmi_l2_error(self.ctx, _DTYPES[x_num.dtype], _ptr(x_num), _ptr(x_ana),
+                                     out, _stream()), "dgmi_l2_error")
+        return float(out[0]), float(out[1])
 
 
 # ---------------------------------------------------------------- module-level API

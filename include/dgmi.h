@@ -83,6 +83,14 @@ int dgmi_apply(dgmi_ctx* ctx, int dtype, const void* x, const void* b, void* r, 
  * f, u_bc, b: FP64 device vectors in DOF layout. */
 int dgmi_rhs(dgmi_ctx* ctx, const double* f, const double* u_bc, double* b, void* stream);
 
+/* out[0] = l2, out[1] = linf (host doubles) of x_num - x_ana with the reference's
+ * definition -- get_l2_error (z02/volume_mf_diff.py:2032-2056):
+ *   l2^2 = sum_e sum_n sum_g (n[n,g] (x_num - x_ana)[e,n])^2 detwei[e,g],
+ *   linf = max |x_num - x_ana|.
+ * dtype as dgmi_apply; synchronizes the stream. */
+int dgmi_l2_error(dgmi_ctx* ctx, int dtype, const void* x_num, const void* x_ana, double* out,
+                  void* stream);
+
 /* bdiagK (nele, nloc, nloc) and diagK (nele, nloc) in FP64 -- get_bdiag_diag
  * (z02/get_bdiag_diag.py:30-62).  Either output may be NULL. */
 int dgmi_bdiag(dgmi_ctx* ctx, double* bdiag, double* diag, void* stream);

@@ -243,9 +243,11 @@ def mg_smooth_one_level(lv: SfcLevels, level, e, b, omega=JAC_WEI):
     return e, rr
 
 
-def mg_on_P1CG(r0, lv: SfcLevels, pre=3, post=3, omega=JAC_WEI):
-    """One SFC V-cycle (z02/multigrid_linearelastic.py:112-172)."""
-    L = lv.nlevel
+def mg_on_P1CG(r0, lv: SfcLevels, pre=3, post=3, omega=JAC_WEI, exact_level=None):
+    """One SFC V-cycle (z02/multigrid_linearelastic.py:112-172).  With exact_level = c the
+    cycle stops at level c and solves it exactly (the FP32 cycles of the HIP path do that
+    at their first level of at most 128 rows); None is the reference's cycle."""
+    L = lv.nlevel if exact_level is None else exact_level + 1
     r_s = [np.asarray(r0, dtype=np.float64).copy()]
     e_s = [np.zeros(lv.nodes_per_level[0])]
     for level in range(L - 1):
@@ -301,7 +303,7 @@ def build_mg(sp, verts, rar_method="colored", sfc=None):
 
 
 def um_left_precond(sp, mgd: MGData, x_rhs, pre=3, post=3, blk_solver="jacobian",
-                    omega=JAC_WEI):
+                    omega=JAC_WEI, exact_level=None):
     """One V-cycle from x = 0 (z02/volume_mf_diff.py:1934-2029).  The post-smoothing
     loop runs ``pre`` times, as the reference's loop bound does (:2021)."""
     P = sp.el.p1_prolong
@@ -313,7 +315,7 @@ def um_left_precond(sp, mgd: MGData, x_rhs, pre=3, post=3, blk_solver="jacobian"
     r1 = mgd.I_cf @ restrict_p3_to_p1(P, r)
     sfc = mgd.levels.sfc
     inv = np.argsort(sfc)
-    e_sfc = mg_on_P1CG(r1[inv], mgd.levels, omega=omega)
+    e_sfc = mg_on_P1CG(r1[inv], mgd.levels, omega=omega, exact_level=exact_level)
     e = e_sfc[sfc - 1]
     x += prolong_p1_to_p3(P, mgd.I_fc @ e)
     for _ in range(pre):

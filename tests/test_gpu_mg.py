@@ -43,6 +43,11 @@ def test_restrict_and_prolong_match_oracle(prob):
     assert rel(to_np(x), ref) < 1e-13
 
 
+def _exact(g):
+    """level the FP32 cycles solve exactly (None: every level is swept)"""
+    return None if g.coarse_exact_level < 0 else g.coarse_exact_level
+
+
 def test_sfc_vcycle_matches_oracle(prob):
     m, pb, g, sp, mgd = prob
     r = np.random.default_rng(4).standard_normal(m.nverts)
@@ -50,7 +55,14 @@ def test_sfc_vcycle_matches_oracle(prob):
     out = to_np(g.mg_on_P1CG(to_dev(r)))
     print("vcycle rel err", rel(out, ref))
     assert rel(out, ref) < 1e-11
+    # FP32: exact coarse solve at the first level of at most 128 rows, and without it
+    ref_x = mg.mg_on_P1CG(r, mgd.levels, exact_level=_exact(g))
     out32 = to_np(g.mg_on_P1CG(to_dev(r, torch.float32)))
+    print("fp32 vcycle exact level", g.coarse_exact_level, "rel err", rel(out32, ref_x))
+    assert rel(out32, ref_x) < 1e-4
+    g.lib.dgmi_set_param(g.ctx, b"coarse_exact", 0.0)
+    out32 = to_np(g.mg_on_P1CG(to_dev(r, torch.float32)))
+    g.lib.dgmi_set_param(g.ctx, b"coarse_exact", 1.0)
     assert rel(out32, ref) < 1e-4
 
 
@@ -61,8 +73,9 @@ def test_preconditioner_matches_oracle(prob):
     out = to_np(g._um_left_precond(g.new_vector(), to_dev(b)))
     print("precond rel err", rel(out, ref))
     assert rel(out, ref) < 1e-10
+    ref_x = mg.um_left_precond(sp, mgd, b, exact_level=_exact(g))
     out32 = to_np(g._um_left_precond(g.new_vector(torch.float32), to_dev(b, torch.float32)))
-    assert rel(out32, ref) < 1e-4
+    assert rel(out32, ref_x) < 1e-4
 
 
 @pytest.mark.parametrize("D,n", [(2, 8), (3, 4)], ids=["2d", "3d"])
@@ -77,6 +90,7 @@ def test_direct_smoother_fp32_neighbour_form(D, n):
     b = np.random.default_rng(6).standard_normal(sp.ndof)
     ref = mg.um_left_precond(sp, mgd, b, blk_solver="direct")
     assert rel(to_np(g._um_left_precond(g.new_vector(), to_dev(b))), ref) < 1e-10
+    ref = mg.um_left_precond(sp, mgd, b, blk_solver="direct", exact_level=_exact(g))
     out = {}
     for offd in (1, 0):
         g.lib.dgmi_set_param(g.ctx, b"smooth_offdiag", float(offd))

@@ -61,6 +61,21 @@ def test_solve_for_diff_matches_manufactured_solution():
     assert abs(l2 - l2o) < 1e-12 and l2 < 1e-4
 
 
+@pytest.mark.parametrize("D,n", [(2, 6), (3, 3)], ids=["2d", "3d"])
+def test_l2_error_on_device_matches_oracle(D, n):
+    """get_l2_error runs on the device (FP64 sums) and follows the reference definition."""
+    m, pb, g, sp = build(D, n, sigma=0.5)
+    rng = np.random.default_rng(31)
+    xn, xa = rng.standard_normal((2, sp.ndof))
+    l2o, linfo = dg.get_l2_error(sp, xn, xa)
+    l2, linf = g.get_l2_error(to_dev(xn), to_dev(xa))
+    assert abs(l2 - l2o) <= 1e-12 * l2o and linf == linfo
+    xn32, xa32 = xn.astype(np.float32), xa.astype(np.float32)
+    l2o32, linfo32 = dg.get_l2_error(sp, xn32.astype(np.float64), xa32.astype(np.float64))
+    l2f, linff = g.get_l2_error(to_dev(xn32, torch.float32), to_dev(xa32, torch.float32))
+    assert abs(l2f - l2o32) <= 1e-12 * l2o32 and linff == linfo32
+
+
 def test_mg_solver_converges():
     m, pb, g, sp = build(2, 6, sigma=0.0)
     b = to_dev(dg.get_rhs(sp, pb.f, pb.u_bc))
