@@ -244,7 +244,7 @@ __host__ __device__ inline size_t lds_align(size_t x) { return (x + 15) & ~size_
 // inverse in cinv (column-major, n x n, n <= EXACT_MAX_N), cinfo[2b+1] = its level c or
 // -1.  With c >= 0 the cycle stops at level c and sets B_c = A_c^-1 r_c there; the levels
 // below c are not visited.
-constexpr int EXACT_MAX_N = 128;
+constexpr int EXACT_MAX_N = 512;
 
 template <typename T>
 __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
@@ -406,15 +406,19 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     __syncthreads();
   }
   if (ainv) {
-    // B = A^-1 r: row i of 8 column groups, partial sums combined in LDS in a fixed order
+    // B = A^-1 r: row i over NG column groups, partial sums combined in LDS in a fixed
+    // order; a thread's column loads are issued eight at a time
     const LdsLevel<T> t = tab[L - 1 - l0];
     const int n = t.n;
     constexpr int NG = SMALL_THREADS / EXACT_MAX_N;
     const int i = tid % EXACT_MAX_N, grp = tid / EXACT_MAX_N;
     const int per = (n + NG - 1) / NG;
     T sacc = 0;
-    if (i < n)
-      for (int j = grp * per; j < min(n, (grp + 1) * per); ++j) sacc += ainv[(size_t)j * n + i] * t.r[j];
+    if (i < n) {
+      const int j1 = min(n, (grp + 1) * per);
+#pragma unroll 8
+      for (int j = grp * per; j < j1; ++j) sacc += ainv[(size_t)j * n + i] * t.r[j];
+    }
     csum[grp][i] = sacc;
     __syncthreads();
     if (tid < n) {
@@ -927,22 +931,24 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
   // most EXACT_MAX_N rows (and a coarser level below it) is inverted on the host
   std::vector<float> cinv;
   std::vector<int64_t> cinfo(2 * nc, -1);
-  for (int b = 0; b < nc; ++b) {
-    const int q = lay.order[b];
-    const SfcLevelsHost& h = lay.lh[q];
-    int cl = -1;
-    for (int l = lay.start32[q]; l < h.nlevel - 1; ++l)
-      if (h.n[l] <= EXACT_MAX_N) {
-        cl = l;
-        break;
-      }
-    cinfo[2 * b] = (int64_t)cinv.size();
-    if (cl < 0) continue;
-    std::vector<float> inv;
-    if (dense_inverse_colmajor(h, cl, inv)) {
-      cinfo[2 * b + 1] = cl;
-      cinv.insert(cinv.end(), inv.begin(), inv.end());
+  std::vector<std::vector<float>> invs(nc);
+  std::vector<int> lvl(nc, -1);
+  parallel_chunks(nc, [&](int, int64_t ba, int64_t bb) {
+    for (int64_t b = ba; b < bb; ++b) {
+      const int q = lay.order[b];
+      const SfcLevelsHost& h = lay.lh[q];
+      for (int l = lay.start32[q]; l < h.nlevel - 1; ++l)
+        if (h.n[l] <= EXACT_MAX_N) {
+          if (dense_inverse_colmajor(h, l, invs[b])) lvl[b] = l;
+          break;
+        }
     }
+  }, 1);
+  for (int b = 0; b < nc; ++b) {
+    cinfo[2 * b] = (int64_t)cinv.size();
+    if (lvl[b] < 0) continue;
+    cinfo[2 * b + 1] = lvl[b];
+    cinv.insert(cinv.end(), invs[b].begin(), invs[b].end());
   }
   lv.cinv32 = upload_vec<float>(c, cinv);
   lv.cinfo32 = upload_vec<int64_t>(c, cinfo);

@@ -162,6 +162,11 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     needs no volume term and only the neighbour half of the face terms
  *                     (default); 0 = x += S (b - A x).  Equal in exact arithmetic; FP64
  *                     sweeps and dgmi_smooth always use the second form.
+ *   "coarse_exact"    FP32 V-cycles: 1 = stop at the first SFC level of at most 512 rows
+ *                     and solve it with its dense inverse (formed at dgmi_mg_setup;
+ *                     default); 0 = sweep every level as the FP64 cycle (the reference's)
+ *                     does.  dgmi_mg_info writes the level to info[4 + nlevel] when the
+ *                     buffer holds it (-1: none).
  *   "mfma_waves", "bf16_waves"  waves (16 elements each) per work-group of the fp32 / bf16
  *                     matrix-core applies: 2, 4 (default) or 8. */
 int dgmi_set_param(dgmi_ctx* ctx, const char* name, double value);
