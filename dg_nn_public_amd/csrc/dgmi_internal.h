@@ -84,10 +84,14 @@ struct SfcLevelsDev {
   void* img32 = nullptr;
   int64_t* imginfo64 = nullptr;
   int64_t* imginfo32 = nullptr;
-  // FP32 exact coarse solves: inverses (column-major) and per component (offset, level or
+  // FP32 exact coarse solves: inverses (row-major) and per component (offset, level or
   // -1), components in layout order
   float* cinv32 = nullptr;
   int64_t* cinfo32 = nullptr;
+  // a context of one solve whose exact level is its FP32 start level: the inverse
+  // row-major, applied by a grid kernel in place of the single-work-group kernel
+  float* dense_rm32 = nullptr;
+  int dense_n32 = 0;
   bool any_start0 = false;    // some component's FP64 kernel starts at level 0
   bool any_start0_32 = false;
   size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the single-work-group kernel

@@ -241,7 +241,7 @@ struct LdsImgLevel {
 __host__ __device__ inline size_t lds_align(size_t x) { return (x + 15) & ~size_t(15); }
 
 // Optional exact coarse solve (cinv != nullptr): cinfo[2b] = offset of the component's
-// inverse in cinv (column-major, n x n, n <= EXACT_MAX_N), cinfo[2b+1] = its level c or
+// inverse in cinv (row-major, n x n, n <= EXACT_MAX_N), cinfo[2b+1] = its level c or
 // -1.  With c >= 0 the cycle stops at level c and sets B_c = A_c^-1 r_c there; the levels
 // below c are not visited.
 constexpr int EXACT_MAX_N = 512;
@@ -253,7 +253,6 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     int post, const T* __restrict__ cinv, const int64_t* __restrict__ cinfo) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ LdsLevel<T> tab[MAX_LEVELS];
-  __shared__ T csum[SMALL_THREADS / EXACT_MAX_N][EXACT_MAX_N];
   const int tid = threadIdx.x;
   constexpr int nth = SMALL_THREADS;
   const int l0 = rng[2 * blockIdx.x];
@@ -406,26 +405,16 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     __syncthreads();
   }
   if (ainv) {
-    // B = A^-1 r: row i over NG column groups, partial sums combined in LDS in a fixed
-    // order; a thread's column loads are issued eight at a time
+    // B = A^-1 r, one wave per row, in the same order as k_dense_rows (so a solve gives
+    // the same bits packed with others or alone)
     const LdsLevel<T> t = tab[L - 1 - l0];
-    const int n = t.n;
-    constexpr int NG = SMALL_THREADS / EXACT_MAX_N;
-    const int i = tid % EXACT_MAX_N, grp = tid / EXACT_MAX_N;
-    const int per = (n + NG - 1) / NG;
-    T sacc = 0;
-    if (i < n) {
-      const int j1 = min(n, (grp + 1) * per);
-#pragma unroll 8
-      for (int j = grp * per; j < j1; ++j) sacc += ainv[(size_t)j * n + i] * t.r[j];
-    }
-    csum[grp][i] = sacc;
-    __syncthreads();
-    if (tid < n) {
-      T v = 0;
-#pragma unroll
-      for (int q = 0; q < NG; ++q) v += csum[q][tid];
-      t.B[tid] = v;
+    const int n = t.n, lane = tid & 63;
+    for (int row = tid >> 6; row < n; row += nth / 64) {
+      const T* a = ainv + (size_t)row * n;
+      T acc = 0;
+      for (int j = lane; j < n; j += 64) acc += a[j] * t.r[j];
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if (lane == 0) t.B[row] = acc;
     }
     __syncthreads();
   } else {
@@ -620,6 +609,20 @@ int launch_probe_collect(dgmi_ctx*, const double* r_sfc, const int32_t* spidx,
 // start deeper; one work-group per component then runs the rest.
 // y[i] = x[i] (or 0 with x null): replaces memset/memcpy nodes so the cycle can be
 // captured in a graph
+// out = A^-1 r for one dense level (row-major inverse): one wave per row, lanes over the
+// columns, a fixed shuffle tree
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_rows(int n, const T* __restrict__ ainv,
+                                                    const T* __restrict__ r, T* __restrict__ out) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;  // whole wave
+  const T* a = ainv + (size_t)row * n;
+  T acc = 0;
+  for (int j = lane; j < n; j += 64) acc += a[j] * r[j];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) out[row] = acc;
+}
+
 template <typename T>
 __global__ void k_copy(int64_t n, const T* __restrict__ x, T* __restrict__ y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -660,20 +663,27 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
     hipLaunchKernelGGL((k_sweep_restrict<T>), grc, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l),
                        dia(l), R(l), cur, nxt, R(l + 1), A(l + 1), omega);
   }
-  // the level tables point level 0 at internal buffers, so components that start at
-  // level 0 see the caller's vectors through them
-  if (any_start0)
-    hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], r_sfc, vec<T>(lv.r[0]));
-  hipLaunchKernelGGL((k_small_levels_lds<T>), dim3(c->ncomp), dim3(SMALL_THREADS),
-                     f64 ? lv.lds64 : lv.lds32, s,
-                     (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32),
-                     (const int32_t*)(f64 ? lv.rng64 : lv.rng32),
-                     (const uint4*)(f64 ? lv.img64 : lv.img32),
-                     (const int64_t*)(f64 ? lv.imginfo64 : lv.imginfo32), omega, 3, 3,
-                     (const T*)(f64 || !mg.coarse_exact ? nullptr : lv.cinv32),
-                     (const int64_t*)lv.cinfo32);
-  if (any_start0)
-    hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)vec<T>(lv.e2[0]), e_sfc);
+  if (!f64 && mg.coarse_exact && lv.dense_rm32) {
+    // one solve whose exact level is its first non-grid level: B = A^-1 r across the GPU
+    const int n = lv.dense_n32;
+    hipLaunchKernelGGL((k_dense_rows<T>), dim3((unsigned)((n + 3) / 4)), blk, 0, s, n,
+                       (const T*)lv.dense_rm32, (const T*)R(l0), B(l0));
+  } else {
+    // the level tables point level 0 at internal buffers, so components that start at
+    // level 0 see the caller's vectors through them
+    if (any_start0)
+      hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], r_sfc, vec<T>(lv.r[0]));
+    hipLaunchKernelGGL((k_small_levels_lds<T>), dim3(c->ncomp), dim3(SMALL_THREADS),
+                       f64 ? lv.lds64 : lv.lds32, s,
+                       (const LevelDesc<T>*)(f64 ? lv.desc64 : lv.desc32),
+                       (const int32_t*)(f64 ? lv.rng64 : lv.rng32),
+                       (const uint4*)(f64 ? lv.img64 : lv.img32),
+                       (const int64_t*)(f64 ? lv.imginfo64 : lv.imginfo32), omega, 3, 3,
+                       (const T*)(f64 || !mg.coarse_exact ? nullptr : lv.cinv32),
+                       (const int64_t*)lv.cinfo32);
+    if (any_start0)
+      hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)vec<T>(lv.e2[0]), e_sfc);
+  }
   // up-cycle on grid-sized levels
   for (int l = l0 - 1; l >= 0; --l) {
     const int n = (int)grid_rows[l];
@@ -726,8 +736,8 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
 }
 
 // Inverse of the dense level-l operator of one component (Gauss-Jordan with partial
-// pivoting in FP64), column-major in FP32.  False if the level is singular.
-static bool dense_inverse_colmajor(const SfcLevelsHost& h, int l, std::vector<float>& out) {
+// pivoting in FP64), row-major in FP32.  False if the level is singular.
+static bool dense_inverse(const SfcLevelsHost& h, int l, std::vector<float>& out) {
   const int n = (int)h.n[l];
   std::vector<double> a((size_t)n * 2 * n, 0.0);  // [A | I], row-major
   for (int i = 0; i < n; ++i) {
@@ -754,7 +764,7 @@ static bool dense_inverse_colmajor(const SfcLevelsHost& h, int l, std::vector<fl
   }
   out.assign((size_t)n * n, 0.0f);
   for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) out[(size_t)j * n + i] = (float)a[(size_t)i * 2 * n + n + j];
+    for (int j = 0; j < n; ++j) out[(size_t)i * n + j] = (float)a[(size_t)i * 2 * n + n + j];
   return true;
 }
 
@@ -939,7 +949,7 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
       const SfcLevelsHost& h = lay.lh[q];
       for (int l = lay.start32[q]; l < h.nlevel - 1; ++l)
         if (h.n[l] <= EXACT_MAX_N) {
-          if (dense_inverse_colmajor(h, l, invs[b])) lvl[b] = l;
+          if (dense_inverse(h, l, invs[b])) lvl[b] = l;
           break;
         }
     }
@@ -953,6 +963,13 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
   lv.cinv32 = upload_vec<float>(c, cinv);
   lv.cinfo32 = upload_vec<int64_t>(c, cinfo);
   if (!lv.cinv32 || !lv.cinfo32) return fail();
+  // one solve whose exact level is its FP32 start: the grid kernel applies the inverse
+  lv.dense_rm32 = nullptr;
+  lv.dense_n32 = 0;
+  if (nc == 1 && lvl[0] >= 0 && lvl[0] == lay.start32[lay.order[0]]) {
+    lv.dense_rm32 = lv.cinv32;
+    lv.dense_n32 = (int)lay.lh[lay.order[0]].n[lvl[0]];
+  }
   mg.exact32.assign(nc, -1);
   for (int b = 0; b < nc; ++b) mg.exact32[lay.order[b]] = (int)cinfo[2 * b + 1];
   lv.desc64 = upload_vec<LevelDesc<double>>(c, d64);
