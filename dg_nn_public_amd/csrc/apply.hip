@@ -74,15 +74,20 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
   }
 }
 
-template <typename T, int D, int MODE>
+// EMIT (mode 5 only): also write the face traces of the new x into tout (another buffer
+// than `trace`, which other work-groups are still reading), so the next sweep needs no
+// trace pass.
+template <typename T, int D, int MODE, bool EMIT = false>
 __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     int nele, T* __restrict__ x, const T* b, T* r, const T* __restrict__ vtab,
     const T* __restrict__ wgt, const T* __restrict__ ftab, const T* __restrict__ sw,
     const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
     const T* __restrict__ wf, const T* __restrict__ pen, const int32_t* __restrict__ nbface,
     const int32_t* __restrict__ fcode, const typename Vec2<T>::type* __restrict__ trace,
-    const T* __restrict__ S, const int32_t* __restrict__ act, T omega) {
+    const T* __restrict__ S, const int32_t* __restrict__ act, T omega,
+    typename Vec2<T>::type* __restrict__ tout) {
   using M = Dm<D>;
+  static_assert(!EMIT || MODE == 5, "traces are emitted by the neighbour-form sweep");
   if (act && !act[blockIdx.x]) return;  // whole block: no barrier is reached
   constexpr bool OFFD = MODE == 5;
   constexpr int NL = M::NL, NF = M::NF;
@@ -243,6 +248,39 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
           for (int m = 0; m < NL; ++m) s += srow[m] * red[0][m][lane];
           x[(size_t)e * NL + n] = s;
           if constexpr (MODE == 3) r[(size_t)e * NL + n] = rv[k];
+          if constexpr (EMIT) red[1][n][lane] = s;  // red[1] is not read after the rows
+        }
+      }
+    }
+    if constexpr (EMIT) {
+      // wave w: the traces of face w of the new x (same operations as k_trace_w)
+      __syncthreads();
+      if (valid) {
+        const int f = w;
+        T cc[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) cc[d] = cf[(f * D + d) * nele + e];
+        const T* tf = ftab + f * M::SG * (1 + D) * NL;
+#pragma unroll 2
+        for (int g = 0; g < M::SG; ++g) {
+          const T* t = tf + g * (1 + D) * NL;
+          T u = 0, gd[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) gd[d] = 0;
+#pragma unroll
+          for (int n = 0; n < NL; ++n) {
+            const T xn = red[1][n][lane];
+            u += t[n] * xn;
+#pragma unroll
+            for (int d = 0; d < D; ++d) gd[d] += t[(1 + d) * NL + n] * xn;
+          }
+          T dn = 0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) dn += cc[d] * gd[d];
+          typename Vec2<T>::type tv;
+          tv.x = u;
+          tv.y = dn;
+          tout[(f * M::SG + g) * nele + e] = tv;
         }
       }
     }
@@ -539,29 +577,47 @@ __global__ void k_convert(const double* __restrict__ in, T* __restrict__ out, in
 }
 
 // ---------------------------------------------------------------------------------------
+// Trace buffers of the vector family: 0 = g.trace, 1 = g.trace2.  tin >= 0: x's traces are
+// already in buffer tin (no trace pass); tin < 0: they are computed into buffer 0.  Mode 5
+// with tout >= 0 writes the new x's traces into buffer tout (tout != the buffer read).
 template <typename T, int D>
 static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mode,
-                      hipStream_t s) {
+                      hipStream_t s, int tin = -1, int tout = -1) {
   const int nele = (int)c->nele;
   const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
   const int32_t* act = c->mask_on ? c->act64 : nullptr;
-  hipLaunchKernelGGL((k_trace_w<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf, g.trace,
-                     act);
+  using V2 = typename Vec2<T>::type;
+  V2* bufs[2] = {g.trace, g.trace2};
+  if (tin < 0) {
+    hipLaunchKernelGGL((k_trace_w<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf,
+                       g.trace, act);
+    tin = 0;
+  }
+  if (!(tin <= 1 && tout <= 1 && tout != tin && (tout < 0 || mode == 5) && bufs[tin] &&
+        (tout < 0 || bufs[tout]))) {
+    set_error("bad trace buffers");
+    return -2;
+  }
+  const V2* trace = bufs[tin];
+  V2* to = tout >= 0 ? bufs[tout] : nullptr;
   const T* vt = g.tab + g.off_vtab;
   const T* wg = g.tab + g.off_wgt;
   const T* ft = g.tab + g.off_ftab;
   const T* sw = g.tab + g.off_sw;
-#define DGMI_RES(MODE)                                                                       \
-  hipLaunchKernelGGL((k_residual_w<T, D, MODE>), grd, blk, 0, s, nele, x, b, r, vt, wg, ft, sw, \
-                     c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, g.trace, g.S, act,   \
-                     (T)c->omega)
+#define DGMI_RES(MODE, EMIT)                                                                  \
+  hipLaunchKernelGGL((k_residual_w<T, D, MODE, EMIT>), grd, blk, 0, s, nele, x, b, r, vt, wg,  \
+                     ft, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, trace, g.S, \
+                     act, (T)c->omega, to)
   switch (mode) {
-    case 0: DGMI_RES(0); break;
-    case 1: DGMI_RES(1); break;
-    case 2: DGMI_RES(2); break;
-    case 3: DGMI_RES(3); break;
-    case 5: DGMI_RES(5); break;
-    default: DGMI_RES(4); break;
+    case 0: DGMI_RES(0, false); break;
+    case 1: DGMI_RES(1, false); break;
+    case 2: DGMI_RES(2, false); break;
+    case 3: DGMI_RES(3, false); break;
+    case 5:
+      if (to) DGMI_RES(5, true);
+      else DGMI_RES(5, false);
+      break;
+    default: DGMI_RES(4, false); break;
   }
 #undef DGMI_RES
   DGMI_HIP(hipGetLastError());
@@ -592,10 +648,31 @@ int launch_apply_pos(dgmi_ctx* c, DevGeom<T>& g, const T* x, T* r, hipStream_t s
 // FP32 sweeps of the exact block smoother (S = omega A_ee^-1) use the neighbour-only form
 // (mode 5) unless the context turns it off; FP64 keeps the reference's x += S (b - A x).
 template <typename T>
+static bool smooth_offd(const dgmi_ctx* c) {
+  return std::is_same<T, float>::value && c->smooth_offdiag &&
+         c->smoother_variant == DGMI_SMOOTH_DIRECT;
+}
+
+template <typename T>
 int launch_smooth(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r_out, hipStream_t s) {
-  const bool offd = std::is_same<T, float>::value && !r_out && c->smooth_offdiag &&
-                    c->smoother_variant == DGMI_SMOOTH_DIRECT;
+  const bool offd = !r_out && smooth_offd<T>(c);
   return apply_dispatch<T>(c, g, x, b, r_out, offd ? 5 : (r_out ? 3 : 2), s);
+}
+
+// The preconditioner's sweeps with traces carried between them (FP32 vector kernels in
+// the neighbour form only; trace_io() tells whether the context runs that way).
+bool trace_io(const dgmi_ctx* c) {
+  return c->f32_kernel == 0 && smooth_offd<float>(c) && c->g32.trace2;
+}
+int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s) {
+  if (c->D == 3) return apply_impl<float, 3>(c, c->g32, x, b, nullptr, 5, s, tin, tout);
+  return apply_impl<float, 2>(c, c->g32, x, b, nullptr, 5, s, tin, tout);
+}
+int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
+                    hipStream_t s) {
+  float* xm = const_cast<float*>(x);
+  if (c->D == 3) return apply_impl<float, 3>(c, c->g32, xm, b, r, mode, s, tin, -1);
+  return apply_impl<float, 2>(c, c->g32, xm, b, r, mode, s, tin, -1);
 }
 
 template <typename T>

@@ -87,6 +87,11 @@ static int upload_geom(dgmi_ctx* c, DevGeom<T>& g, const std::vector<double>& ta
   g.pen = upload<T>(c, geo.pen);
   g.trace = static_cast<typename Vec2<T>::type*>(
       c->alloc(sizeof(typename Vec2<T>::type) * c->nface * c->sngi * c->nele));
+  if (std::is_same<T, float>::value) {
+    g.trace2 = static_cast<typename Vec2<T>::type*>(
+        c->alloc(sizeof(typename Vec2<T>::type) * c->nface * c->sngi * c->nele));
+    DGMI_CHECK(g.trace2, alloc_failure());
+  }
   DGMI_CHECK(g.tab && g.G && g.cf && g.wf && g.pen && g.trace, alloc_failure());
   return 0;
 }
@@ -209,8 +214,10 @@ static int compute_bdiag(dgmi_ctx* c, hipStream_t s) {
 }
 
 // _um_left_precond: x = M^-1 b, one V-cycle from x = 0 (z02/volume_mf_diff.py:1934-2029)
+// With tr_out, *tr_out is set to the trace buffer that holds x's traces afterwards (-1:
+// none), for an apply of x that follows at once.
 template <typename T>
-static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s) {
+static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = nullptr) {
   DGMI_CHECK(c->smoother_variant >= 0, "smoother not set up (dgmi_smoother_setup)");
   DGMI_CHECK(c->mg.ready, "multigrid not set up (dgmi_mg_setup)");
   DevGeom<T>& g = geom<T>(c);
@@ -224,6 +231,25 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s) {
     ~KernelScope() { c->f32_kernel = saved; }
   } scope{c, c->f32_kernel};
   if (std::is_same<T, float>::value && c->pc_bf16) c->f32_kernel = 2;
+  if (tr_out) *tr_out = -1;
+  if constexpr (std::is_same<T, float>::value) {
+    if (trace_io(c)) {
+      // the same steps, each sweep writing the traces of its result for the next one
+      // (buffers 0 and 1 alternate); traces are recomputed after the prolongation
+      DGMI_TRY(launch_smooth_zero<T>(c, g, x, b, s));
+      DGMI_TRY(launch_smooth_io(c, x, b, -1, 1, s));
+      DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s));
+      DGMI_TRY(launch_apply_io(c, x, b, r, 1, 0, s));
+      DGMI_TRY(launch_restrict<T>(c, r, rc, s));
+      DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s));
+      DGMI_TRY(launch_prolong_add<T>(c, ec, x, s));
+      DGMI_TRY(launch_smooth_io(c, x, b, -1, 1, s));
+      DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s));
+      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s));
+      if (tr_out) *tr_out = 1;
+      return 0;
+    }
+  }
   // pre-smoothing: the first step starts from x = 0, where the residual is b
   DGMI_TRY(launch_smooth_zero<T>(c, g, x, b, s));
   for (int it = 1; it < 3; ++it) DGMI_TRY(launch_smooth<T>(c, g, x, b, nullptr, s));
@@ -364,8 +390,14 @@ static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool fl
   };
   for (int j = 0; j < m; ++j) {
     if (flexible) {
-      DGMI_TRY(precond<T>(c, Vj(j), Zj(j), s));                  // z_j = M^-1 v_j
-      DGMI_TRY(launch_apply_pos<T>(c, geom<T>(c), Zj(j), w, s));  // w = A z_j
+      int tz = -1;
+      DGMI_TRY(precond<T>(c, Vj(j), Zj(j), s, &tz));  // z_j = M^-1 v_j
+      if constexpr (std::is_same<T, float>::value) {
+        if (tz >= 0) DGMI_TRY(launch_apply_io(c, Zj(j), nullptr, w, 4, tz, s));  // w = A z_j
+        else DGMI_TRY(launch_apply_pos<T>(c, geom<T>(c), Zj(j), w, s));
+      } else {
+        DGMI_TRY(launch_apply_pos<T>(c, geom<T>(c), Zj(j), w, s));
+      }
     } else {
       DGMI_TRY(launch_apply_pos<T>(c, geom<T>(c), Vj(j), t, s));  // t = A v_j
       DGMI_TRY(precond<T>(c, t, w, s));                           // w = M^-1 A v_j

@@ -46,6 +46,7 @@ template <typename T> struct DevGeom {
   // face traces: (nface*sngi, nele) for the vector kernels, (nele, nface, sngi) for the
   // matrix-core kernels
   typename Vec2<T>::type* trace = nullptr;
+  typename Vec2<T>::type* trace2 = nullptr;  // second buffer (FP32 vector sweeps)
 };
 
 // SFC levels on the device.  "Grid" levels 0..small_level hold every component whose
@@ -279,6 +280,11 @@ int launch_build_S_jacobi(dgmi_ctx* c, const double* bdiag, double omega, T* S, 
 // index of a singular block
 int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* fail,
                           hipStream_t s);
+// FP32 vector-family sweeps carrying traces between them (see apply.hip)
+bool trace_io(const dgmi_ctx* c);
+int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s);
+int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
+                    hipStream_t s);
 // g32.S = packed symmetric FP32 copy of g64.S
 int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s);
 template <typename T>

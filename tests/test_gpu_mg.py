@@ -91,15 +91,19 @@ def test_direct_smoother_fp32_neighbour_form(D, n):
     ref = mg.um_left_precond(sp, mgd, b, blk_solver="direct")
     assert rel(to_np(g._um_left_precond(g.new_vector(), to_dev(b))), ref) < 1e-10
     ref = mg.um_left_precond(sp, mgd, b, blk_solver="direct", exact_level=_exact(g))
-    out = {}
-    for offd in (1, 0):
-        g.lib.dgmi_set_param(g.ctx, b"smooth_offdiag", float(offd))
-        out[offd] = to_np(g._um_left_precond(g.new_vector(torch.float32),
-                                             to_dev(b, torch.float32)))
-        print("offdiag", offd, "rel err", rel(out[offd], ref), flush=True)
-        assert rel(out[offd], ref) < 1e-4
-    assert rel(out[1], out[0]) < 1e-4
+    # "vector" with the neighbour form carries the face traces from sweep to sweep
+    for kern in ("mfma", "vector"):
+        g.set_f32_kernel(kern)
+        out = {}
+        for offd in (1, 0):
+            g.lib.dgmi_set_param(g.ctx, b"smooth_offdiag", float(offd))
+            out[offd] = to_np(g._um_left_precond(g.new_vector(torch.float32),
+                                                 to_dev(b, torch.float32)))
+            print(kern, "offdiag", offd, "rel err", rel(out[offd], ref), flush=True)
+            assert rel(out[offd], ref) < 1e-4
+        assert rel(out[1], out[0]) < 1e-4
     g.lib.dgmi_set_param(g.ctx, b"smooth_offdiag", 1.0)
+    g.set_f32_kernel("mfma")
 
 
 def test_grid_level_kernels_match_oracle(monkeypatch):

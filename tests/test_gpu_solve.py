@@ -61,6 +61,27 @@ def test_solve_for_diff_matches_manufactured_solution():
     assert abs(l2 - l2o) < 1e-12 and l2 < 1e-4
 
 
+@pytest.mark.parametrize("kern", ["vector", "mfma", "mfma_bf16"])
+def test_direct_smoother_mixed_solve_matches_oracle(kern):
+    """FP32 refinement with the exact block smoother (neighbour-form sweeps; with the
+    vector kernels the traces are carried from sweep to sweep) reaches 1e-10 and agrees
+    with the oracle's FP64 solve."""
+    m, pb, g, sp = build(3, 4, sigma=1.0, blk_solver="direct", face_kappa="harmonic")
+    g._get_RAR_and_sfc_data_Um()
+    mgd = mg.build_mg(sp, m.verts)
+    b = dg.get_rhs(sp, pb.f, pb.u_bc)
+    g.set_f32_kernel(kern)
+    x, its = g._gmres_mg_solver(g.new_vector(), to_dev(b), rtol=1e-10, inner=torch.float32)
+    xs = to_np(x)
+    relres = np.linalg.norm(b - dg.apply_A(sp, xs)) / np.linalg.norm(b)
+    x_ref, _ = krylov.gmres_mg_solver(
+        sp, mgd, np.zeros(sp.ndof), b, rtol=1e-12, m=30, max_its=600,
+        precond=lambda v: mg.um_left_precond(sp, mgd, v, blk_solver="direct"))
+    print(kern, "its", its, g.last_stats, "oracle relres", relres, "vs oracle", rel(xs, x_ref))
+    assert g.last_stats["relres"] <= 1e-10 and relres <= 2e-10
+    assert rel(xs, x_ref) < 1e-7
+
+
 @pytest.mark.parametrize("D,n", [(2, 6), (3, 3)], ids=["2d", "3d"])
 def test_l2_error_on_device_matches_oracle(D, n):
     """get_l2_error runs on the device (FP64 sums) and follows the reference definition."""
