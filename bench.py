@@ -36,6 +36,10 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12   # MI355X FP64 vector/matrix, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12     # B/s
+# calibrated HBM bytes of one FP32 vector smoothing sweep (k_residual_w<float,3,5,true>) on
+# the 24^3x6 workload: FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3 --pmc passes
+# (profiles/pmc_bytes_apply_tet24.md)
+PMC_SWEEP = 259.3e6
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):
@@ -48,6 +52,24 @@ def flops_per_elem(D, nloc, ngi, sngi, nface):
 
 def bytes_per_elem(D, nloc, nface, s, read_b=True):
     return 2 * nloc * s + s + (D * D + 1) * s + 4 * nface + (nloc * s if read_b else 0)
+
+
+def sweep_bytes_per_elem(D, nloc, nface, sngi):
+    """Algorithmic HBM bytes of one FP32 smoothing sweep x = (1-w)x + S(b - A_off x) with
+    the face traces carried to the next sweep, per element: packed symmetric S, x read and
+    written, b, the neighbours' traces read and this element's traces written (value and
+    normal derivative at every face point), per-face geometry (c, wf, pen, code, nbface)."""
+    s_packed = 4 * nloc * (nloc + 1) // 2
+    vectors = 4 * 3 * nloc
+    traces = 2 * nface * sngi * 8
+    geometry = nface * (4 * D + 16)
+    return s_packed + vectors + traces + geometry
+
+
+def sweep_flops_per_elem(D, nloc, nface, sngi):
+    """Flops of one such sweep per element: neighbour-term lift and trace output (value
+    and D derivatives at every face point), the smoother row products."""
+    return 2 * 2 * nface * sngi * (1 + D) * nloc + 2 * nloc * nloc
 
 
 def main():
@@ -144,10 +166,25 @@ def main():
     print(f"[bench] apply f64 {t_apply64*1e6:.1f} us, f32 vector {t_apply32v*1e6:.1f} us, "
           f"f32 mfma {t_apply32*1e6:.1f} us, bf16 mfma {t_apply16*1e6:.1f} us",
           file=sys.stderr, flush=True)
+    # ---- the dominant kernel of the mixed solve: one FP32 smoothing sweep (autotuned
+    # family), timed with HIP events on the stream the kernels run on
+    g.set_f32_kernel(tuned["f32_kernel"])
+    b32 = b.float()
+    xs32 = torch.zeros(g.ndof, dtype=torch.float32, device=dev)
+    sweeps = 40
+    g.smooth_n(xs32, b32, 4)
+    es0, es1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    es0.record(stream)
+    g.smooth_n(xs32, b32, sweeps)
+    es1.record(stream)
+    torch.cuda.synchronize()
+    t_sweep = es0.elapsed_time(es1) / 1e3 / sweeps
+    print(f"[bench] FP32 smoothing sweep ({tuned['f32_kernel']}): {t_sweep*1e6:.1f} us",
+          file=sys.stderr, flush=True)
+
     if args.apply_only:
         # FP32 preconditioner V-cycles with each kernel family (smoothing kernels for the
         # counter passes)
-        b32 = b.float()
         for kern in ("vector", "mfma", "mfma_bf16"):
             g.set_f32_kernel(kern)
             for _ in range(3):
@@ -251,10 +288,13 @@ def main():
     vec32 = tuned["f32_kernel"] == "vector"
     t32 = t_apply32v if vec32 else t_apply32
     ach32 = F * nele / t32
-    # calibrated HBM bytes per apply of this workload (FETCH_SIZE x 2 + WRITE_SIZE, two
+    # calibrated HBM bytes per launch of this workload (FETCH_SIZE x 2 + WRITE_SIZE, two
     # rocprofv3 --pmc passes, profiles/pmc_bytes_apply_tet24.md)
-    pmc = {"f64": 323e6, "f32_vector": 166e6, "f32_mfma": 125e6}
+    pmc = {"f64": 323e6, "f32_vector": 166e6, "f32_mfma": 125e6, "f32_sweep": PMC_SWEEP}
     on_profiled = args.dim == 3 and args.n == 24
+    SB = sweep_bytes_per_elem(args.dim, nloc, g.nface, g.sngi)
+    SF = sweep_flops_per_elem(args.dim, nloc, g.nface, g.sngi)
+    sweep_gbps = SB * nele / t_sweep / 1e9
     out = {
         "metric": "solves/s + p50/p99 latency at rtol 1e-10; DG apply GDoF/s, mixed vs FP64",
         "value": sps32,
@@ -292,12 +332,23 @@ def main():
         "apply_gdofs": {"f64": gdofs64, "f32_vector": nele * nloc / t_apply32v / 1e9,
                         "f32_mfma": gdofs32, "bf16_mfma": nele * nloc / t_apply16 / 1e9},
         "setup_s": setup_s,
-        "roofline": {"bound": "mfma", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32,
-                     "traffic": (pmc["f32_vector" if vec32 else "f32_mfma"] if on_profiled else None),
-                     "kernel": "FP32 operator apply (trace + residual, "
-                               + ("vector" if vec32 else "MFMA") + " kernels, autotuned)",
-                     "flop_per_elem": F, "elements": nele, "avg_launch_us": t32 * 1e6},
+        "roofline": {"bound": "hbm", "achieved": sweep_gbps, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": sweep_gbps * 1e9 / PEAK_HBM,
+                     "traffic": (pmc["f32_sweep"] if on_profiled and vec32 else None),
+                     "kernel": "FP32 smoothing sweep x = (1-w)x + S(b - A_off x) with the face "
+                               "traces written for the next sweep (" +
+                               ("k_residual_w<float,D,5,true>, one launch" if vec32 else
+                                "trace + residual MFMA launches") + ", autotuned family); "
+                               "the largest share of the mixed solve's kernel time",
+                     "bytes_per_elem": SB, "flop_per_elem": SF, "elements": nele,
+                     "avg_launch_us": t_sweep * 1e6,
+                     "achieved_tflops": SF * nele / t_sweep / 1e12},
+        "roofline_apply": {"bound": "mfma", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12,
+                           "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32,
+                           "traffic": (pmc["f32_vector" if vec32 else "f32_mfma"] if on_profiled else None),
+                           "kernel": "FP32 operator apply (trace + residual, "
+                                     + ("vector" if vec32 else "MFMA") + " kernels, autotuned)",
+                           "flop_per_elem": F, "elements": nele, "avg_launch_us": t32 * 1e6},
         "roofline_fp64": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
                           "traffic": pmc["f64"] if on_profiled else None,

@@ -49,6 +49,7 @@ SIGNATURES = {
     "dgmi_bdiag": (C.c_int, [_P, _P, _P, _P]),
     "dgmi_smoother_setup": (C.c_int, [_P, C.c_int, C.c_double, _P]),
     "dgmi_smooth": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
+    "dgmi_smooth_n": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _P]),
     "dgmi_mg_setup": (C.c_int, [_P, _P]),
     "dgmi_mg_info": (C.c_int, [_P, _I64P, C.c_int]),
     "dgmi_mg_export": (C.c_int, [_P, _I64P, _I64P, _DP, _I64P]),

@@ -17,7 +17,7 @@
 //                 functions; writes r = b - A x, or applies the smoother x += S r.
 // MODE 0: r = -A x; 1: r = b - A x; 2: x += S (b - A x); 3: as 2 and also store r;
 // 4: r = +A x; 5: x = (1 - omega) x + S (b - A_off x), where A_off is A without its
-// diagonal blocks.  Mode 5 equals mode 2 when S = omega A_ee^-1 (the exact block
+// diagonal blocks; 6: x = S b (a sweep from x = 0).  Mode 5 equals mode 2 when S = omega A_ee^-1 (the exact block
 // smoother): it needs no volume term and only the neighbour half of the face terms.
 #include "dgmi.h"
 #include "dgmi_internal.h"
@@ -87,9 +87,10 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     const T* __restrict__ S, const int32_t* __restrict__ act, T omega,
     typename Vec2<T>::type* __restrict__ tout) {
   using M = Dm<D>;
-  static_assert(!EMIT || MODE == 5, "traces are emitted by the neighbour-form sweep");
+  static_assert(!EMIT || MODE == 5 || MODE == 6, "traces are emitted by the sweeps");
   if (act && !act[blockIdx.x]) return;  // whole block: no barrier is reached
   constexpr bool OFFD = MODE == 5;
+  constexpr bool ZERO = MODE == 6;  // sweep from x = 0: x = S b (x is not read)
   constexpr int NL = M::NL, NF = M::NF;
   constexpr int GPW = M::NGI / NF;          // volume points per wave
   constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   const bool valid = e < nele;
   T xv[NL], acc[NL];
 #pragma unroll
-  for (int n = 0; n < NL; ++n) acc[n] = 0;
+  for (int n = 0; n < NL; ++n) acc[n] = xv[n] = 0;
   // this wave's right-hand-side rows, requested before the face and volume work
   T bv[NPW];
 #pragma unroll
@@ -109,10 +110,10 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     const int n = w * NPW + k;
     bv[k] = (MODE != 0 && MODE != 4 && valid && n < NL) ? b[(size_t)e * NL + n] : T(0);
   }
-  if (valid) {
+  if (valid && !ZERO) {
     load_row<T, NL>(x + (size_t)e * NL, xv);
   }
-  if (valid && !OFFD) {
+  if (valid && !OFFD && !ZERO) {
     // ---- volume points of this wave
     T gm[M::NG];
 #pragma unroll
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
       }
     }
   }
-  if (valid) {
+  if (valid && !ZERO) {
     // ---- face w (mode 5: interior faces, neighbour terms only)
     const int f = w;
     const int z = e * NF + f;
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
         if (n < NL) {
           T srow[NL];
           load_smoother_row<T, NL>(S, n, e, nele, srow);
-          T s = OFFD ? (T(1) - omega) * xv[n] : xv[n];
+          T s = ZERO ? T(0) : (OFFD ? (T(1) - omega) * xv[n] : xv[n]);
 #pragma unroll
           for (int m = 0; m < NL; ++m) s += srow[m] * red[0][m][lane];
           x[(size_t)e * NL + n] = s;
@@ -588,12 +589,14 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   const int32_t* act = c->mask_on ? c->act64 : nullptr;
   using V2 = typename Vec2<T>::type;
   V2* bufs[2] = {g.trace, g.trace2};
-  if (tin < 0) {
+  if (mode == 6) {
+    tin = tout == 0 ? 1 : 0;  // a sweep from zero reads no traces
+  } else if (tin < 0) {
     hipLaunchKernelGGL((k_trace_w<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf,
                        g.trace, act);
     tin = 0;
   }
-  if (!(tin <= 1 && tout <= 1 && tout != tin && (tout < 0 || mode == 5) && bufs[tin] &&
+  if (!(tin <= 1 && tout <= 1 && tout != tin && (tout < 0 || mode >= 5) && bufs[tin] &&
         (tout < 0 || bufs[tout]))) {
     set_error("bad trace buffers");
     return -2;
@@ -616,6 +619,10 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
     case 5:
       if (to) DGMI_RES(5, true);
       else DGMI_RES(5, false);
+      break;
+    case 6:
+      if (to) DGMI_RES(6, true);
+      else DGMI_RES(6, false);
       break;
     default: DGMI_RES(4, false); break;
   }
@@ -667,6 +674,10 @@ bool trace_io(const dgmi_ctx* c) {
 int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s) {
   if (c->D == 3) return apply_impl<float, 3>(c, c->g32, x, b, nullptr, 5, s, tin, tout);
   return apply_impl<float, 2>(c, c->g32, x, b, nullptr, 5, s, tin, tout);
+}
+int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s) {
+  if (c->D == 3) return apply_impl<float, 3>(c, c->g32, x, b, nullptr, 6, s, -1, tout);
+  return apply_impl<float, 2>(c, c->g32, x, b, nullptr, 6, s, -1, tout);
 }
 int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
                     hipStream_t s) {

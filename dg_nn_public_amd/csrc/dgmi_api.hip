@@ -234,16 +234,16 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
   if (tr_out) *tr_out = -1;
   if constexpr (std::is_same<T, float>::value) {
     if (trace_io(c)) {
-      // the same steps, each sweep writing the traces of its result for the next one
-      // (buffers 0 and 1 alternate); traces are recomputed after the prolongation
-      DGMI_TRY(launch_smooth_zero<T>(c, g, x, b, s));
-      DGMI_TRY(launch_smooth_io(c, x, b, -1, 1, s));
+      // the same steps, each sweep (and the prolongation) writing the traces of its
+      // result for the next one; buffers 0 and 1 alternate
+      DGMI_TRY(launch_smooth_zero_io(c, x, b, 0, s));
+      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s));
       DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s));
       DGMI_TRY(launch_apply_io(c, x, b, r, 1, 0, s));
       DGMI_TRY(launch_restrict<T>(c, r, rc, s));
       DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s));
-      DGMI_TRY(launch_prolong_add<T>(c, ec, x, s));
-      DGMI_TRY(launch_smooth_io(c, x, b, -1, 1, s));
+      DGMI_TRY(launch_prolong_trace(c, ec, x, 0, s));
+      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s));
       DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s));
       DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s));
       if (tr_out) *tr_out = 1;
@@ -267,6 +267,9 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
 // pinned host mirrors of the per-component scalars the host reads back.
 struct Krylov {
   int m = 0, ncomp = 0;
+  // leading dimension of V and Z: ndof rounded up to 256 values plus 64, so that the
+  // columns of one row group do not all fall on the same HBM channels
+  int64_t ldv = 0;
   size_t vbytes = 0, zbytes = 0;
   void* V = nullptr;  // (m+1) * ndof of T
   void* Z = nullptr;  // m * ndof of T (flexible GMRES)
@@ -309,7 +312,9 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     if (hipHostMalloc((void**)&k->hi, sizeof(int32_t) * (2 * nc + 1)) != hipSuccess) k->hi = nullptr;
     if (!k->beta || !k->nact || !k->status || !k->hd || !k->hi) return nullptr;
   }
-  const size_t need = (size_t)(m + 1) * c->ndof * elem;
+  const int64_t ldv = (c->ndof + 255) / 256 * 256 + 64;
+  k->ldv = ldv;
+  const size_t need = (size_t)(m + 1) * ldv * elem;
   if (k->m < m || k->vbytes < need) {
     k->V = c->alloc(need);
     k->vbytes = need;
@@ -338,7 +343,7 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     }
     k->nring = m + 1;
   }
-  const size_t zneed = (size_t)m * c->ndof * elem;
+  const size_t zneed = (size_t)m * ldv * elem;
   if (flexible && k->zbytes < zneed) {
     k->Z = c->alloc(zneed);
     k->zbytes = zneed;
@@ -357,7 +362,7 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
 template <typename T>
 static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool flexible,
                        hipStream_t s) {
-  const int64_t n = c->ndof;
+  const int64_t n = k->ldv;  // leading dimension of the basis
   const int nc = c->ncomp;
   const SegDev& sg = c->seg;
   T* V = static_cast<T*>(k->V);
@@ -908,6 +913,26 @@ int dgmi_smooth(dgmi_ctx* c, int dtype, void* x, const void* b, void* r_out, voi
     return launch_apply_mf(c, true, (float*)x, (const float*)b, (float*)r_out, r_out ? 3 : 2, s);
   set_error("dtype not supported by dgmi_smooth");
   return -2;
+}
+
+int dgmi_smooth_n(dgmi_ctx* c, int dtype, void* x, const void* b, int n, void* stream) {
+  DGMI_CHECK(c && x && b, "null argument");
+  DGMI_CHECK(n >= 1, "n must be at least 1");
+  DGMI_CHECK(c->smoother_variant >= 0, "smoother not set up (dgmi_smoother_setup)");
+  DGMI_TRY(check_align(x));
+  DGMI_TRY(check_align(b));
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == DGMI_F32 && trace_io(c)) {
+    int tin = -1, tout = 1;
+    for (int k = 0; k < n; ++k) {
+      DGMI_TRY(launch_smooth_io(c, (float*)x, (const float*)b, tin, tout, s));
+      tin = tout;
+      tout = 1 - tout;
+    }
+    return 0;
+  }
+  for (int k = 0; k < n; ++k) DGMI_TRY(dgmi_smooth(c, dtype, x, b, nullptr, stream));
+  return 0;
 }
 
 int dgmi_mg_setup(dgmi_ctx* c, void* stream) {

@@ -89,10 +89,11 @@ struct SfcLevelsDev {
   // -1), components in layout order
   float* cinv32 = nullptr;
   int64_t* cinfo32 = nullptr;
-  // a context of one solve whose exact level is its FP32 start level: the inverse
-  // row-major, applied by a grid kernel in place of the single-work-group kernel
+  // a context of one solve: the inverse of its exact level (row-major), applied by a grid
+  // kernel in place of the single-work-group kernel and of the grid levels below
   float* dense_rm32 = nullptr;
   int dense_n32 = 0;
+  int dense_level32 = -1;
   bool any_start0 = false;    // some component's FP64 kernel starts at level 0
   bool any_start0_32 = false;
   size_t lds64 = 0, lds32 = 0;  // dynamic LDS of the single-work-group kernel
@@ -283,6 +284,8 @@ int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* f
 // FP32 vector-family sweeps carrying traces between them (see apply.hip)
 bool trace_io(const dgmi_ctx* c);
 int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s);
+// x = S b (the first sweep from x = 0), traces of x into buffer tout
+int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s);
 int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
                     hipStream_t s);
 // g32.S = packed symmetric FP32 copy of g64.S
@@ -294,6 +297,8 @@ template <typename T>
 int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s);
 template <typename T>
 int launch_prolong_add(dgmi_ctx* c, const T* e_sfc, T* x_dg, hipStream_t s);
+// FP32 vector family: x += P e with the traces of the new x written into buffer tout
+int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s);
 template <typename T>
 int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s);
 // single-work-group part of the V-cycle: LDS bytes of levels l0.. of one component, the

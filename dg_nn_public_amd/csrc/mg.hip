@@ -84,6 +84,73 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nele, const int32_t* __
   }
 }
 
+// k_prolong_add fused with the face-trace pass of the vector kernels: a work-group of
+// nface waves owns 64 elements; every wave forms the element's whole new row
+// x + P e (cheap: nloc x (D+1) products), writes its share of the rows and the traces of
+// its face w into tout[(w*sngi + g)*nele + e] (same operations as the trace pass).
+// ftab: per face, per point, (phi, dphi/dxi_d) x nloc; cf: (nface*D, nele).
+template <typename T, int D>
+__global__ __launch_bounds__(64 * (D + 1)) void k_prolong_trace(
+    int nele, const int32_t* __restrict__ cellpos, const T* __restrict__ rw,
+    const T* __restrict__ ec, T* __restrict__ x, const T* __restrict__ ftab,
+    const T* __restrict__ cf, typename Vec2<T>::type* __restrict__ tout) {
+  constexpr int NL = D == 3 ? 20 : 10, NB = D + 1, NF = D + 1, SG = D == 3 ? 12 : 4;
+  constexpr int NPW = (NL + NF - 1) / NF;
+  const int lane = threadIdx.x & 63;
+  const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int e = blockIdx.x * 64 + lane;
+  const bool valid = e < nele;
+  T xv[NL];
+  T* xe = x + (size_t)(valid ? e : 0) * NL;
+  if (valid) {
+    T ev[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const int p = cellpos[e * NB + k];
+      ev[k] = ec[p] * rw[p];
+    }
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      T s = xe[n];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) s += (T)Ptab<D>(n, k) * ev[k];
+      xv[n] = s;
+    }
+  }
+  // every wave has read the old row before any wave writes its share of the new one
+  __syncthreads();
+  if (!valid) return;
+#pragma unroll
+  for (int k = 0; k < NPW; ++k) {
+    const int n = f * NPW + k;
+    if (n < NL) xe[n] = xv[n];
+  }
+  T c[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
+  const T* tf = ftab + f * SG * (1 + D) * NL;
+#pragma unroll 2
+  for (int g = 0; g < SG; ++g) {
+    const T* t = tf + g * (1 + D) * NL;
+    T u = 0, gd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gd[d] = 0;
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      u += t[n] * xv[n];
+#pragma unroll
+      for (int d = 0; d < D; ++d) gd[d] += t[(1 + d) * NL + n] * xv[n];
+    }
+    T dn = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) dn += c[d] * gd[d];
+    typename Vec2<T>::type tv;
+    tv.x = u;
+    tv.y = dn;
+    tout[(f * SG + g) * nele + e] = tv;
+  }
+}
+
 // probe vector for colour c: x = P I_fc mask_c (z02/volume_mf_diff.py:1717-1722)
 template <int D>
 __global__ __launch_bounds__(256) void k_probe_mask(int nele, const int32_t* __restrict__ cellpos,
@@ -244,7 +311,8 @@ __host__ __device__ inline size_t lds_align(size_t x) { return (x + 15) & ~size_
 // inverse in cinv (row-major, n x n, n <= EXACT_MAX_N), cinfo[2b+1] = its level c or
 // -1.  With c >= 0 the cycle stops at level c and sets B_c = A_c^-1 r_c there; the levels
 // below c are not visited.
-constexpr int EXACT_MAX_N = 512;
+constexpr int EXACT_MAX_N = 512;    // single-work-group path (packed solves)
+constexpr int DENSE_MAX_N = 2048;   // grid path (a context of one solve)
 
 template <typename T>
 __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
@@ -570,6 +638,26 @@ int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s) {
   return 0;
 }
 
+// FP32 vector family: x += P e and the traces of the new x into buffer tout (0 or 1)
+int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s) {
+  const int nele = (int)c->nele;
+  const DevGeom<float>& g = c->g32;
+  typename Vec2<float>::type* to = tout == 0 ? g.trace : g.trace2;
+  if (!to || (tout != 0 && tout != 1)) {
+    set_error("bad trace buffer");
+    return -2;
+  }
+  const dim3 blk(64 * (c->D + 1)), grd((nele + 63) / 64);
+  if (c->D == 3)
+    hipLaunchKernelGGL((k_prolong_trace<float, 3>), grd, blk, 0, s, nele, c->mg.cellpos,
+                       (const float*)c->mg.rw32, e_sfc, x_dg, g.tab + g.off_ftab, g.cf, to);
+  else
+    hipLaunchKernelGGL((k_prolong_trace<float, 2>), grd, blk, 0, s, nele, c->mg.cellpos,
+                       (const float*)c->mg.rw32, e_sfc, x_dg, g.tab + g.off_ftab, g.cf, to);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
 template <typename T>
 int launch_prolong_add(dgmi_ctx* c, const T* e_sfc, T* x_dg, hipStream_t s) {
   const int nele = (int)c->nele;
@@ -618,6 +706,8 @@ __global__ __launch_bounds__(256) void k_dense_rows(int n, const T* __restrict__
   if (row >= n) return;  // whole wave
   const T* a = ainv + (size_t)row * n;
   T acc = 0;
+  // unrolled for loads in flight; the additions keep their order
+#pragma unroll 8
   for (int j = lane; j < n; j += 64) acc += a[j] * r[j];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if (lane == 0) out[row] = acc;
@@ -635,7 +725,8 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   auto& lv = mg.lv;
   const bool f64 = std::is_same<T, double>::value;
   const T omega = (T)c->omega;
-  const int l0 = f64 ? mg.small_level : mg.small_level32;
+  const bool dense = !f64 && mg.coarse_exact && lv.dense_rm32;
+  const int l0 = f64 ? mg.small_level : (dense ? lv.dense_level32 : mg.small_level32);
   const std::vector<int64_t>& grid_rows = f64 ? lv.grid_rows : lv.grid_rows32;
   const bool any_start0 = f64 ? lv.any_start0 : lv.any_start0_32;
   auto R = [&](int l) -> T* { return l == 0 ? const_cast<T*>(r_sfc) : vec<T>(lv.r[l]); };
@@ -663,7 +754,7 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
     hipLaunchKernelGGL((k_sweep_restrict<T>), grc, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l),
                        dia(l), R(l), cur, nxt, R(l + 1), A(l + 1), omega);
   }
-  if (!f64 && mg.coarse_exact && lv.dense_rm32) {
+  if (dense) {
     // one solve whose exact level is its first non-grid level: B = A^-1 r across the GPU
     const int n = lv.dense_n32;
     hipLaunchKernelGGL((k_dense_rows<T>), dim3((unsigned)((n + 3) / 4)), blk, 0, s, n,
@@ -733,6 +824,79 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   }
   DGMI_HIP(hipGraphLaunch(vg.exec, s));
   return 0;
+}
+
+// In-place Gauss-Jordan inversion of a dense n x n matrix on the device, FP64, without
+// pivoting: the SFC level operators are Galerkin projections of a symmetric positive
+// definite operator, so every pivot is positive (a non-positive or non-finite one sets
+// *fail).  Step k: f = column k; row k /= a_kk; a_ij -= f_i a_kj (i, j != k);
+// column k = -f_i / a_kk; a_kk = 1 / a_kk.
+__global__ void k_gj_col(int n, const double* __restrict__ a, int k, double* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = a[(size_t)i * n + k];
+}
+__global__ void k_gj_row(int n, double* __restrict__ a, int k, const double* __restrict__ f) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n && j != k) a[(size_t)k * n + j] /= f[k];
+}
+__global__ void k_gj_rest(int n, double* __restrict__ a, int k, const double* __restrict__ f) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j < n && i != k && j != k) a[(size_t)i * n + j] -= f[i] * a[(size_t)k * n + j];
+}
+__global__ void k_gj_fix(int n, double* __restrict__ a, int k, const double* __restrict__ f,
+                         int* __restrict__ fail) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double p = f[k];
+  if (i == k) {
+    if (!(p > 0.0) || !isfinite(p)) *fail = 1;
+    a[(size_t)k * n + k] = 1.0 / p;
+  } else {
+    a[(size_t)i * n + k] = -f[i] / p;
+  }
+}
+__global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// FP32 row-major inverse of the dense level-l operator of one component, formed on the
+// device (synchronous); nullptr on failure.
+static float* dense_inverse_device(dgmi_ctx* c, const SfcLevelsHost& h, int l) {
+  const int n = (int)h.n[l];
+  std::vector<double> a((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int32_t q = h.rowptr[l][i]; q < h.rowptr[l][i + 1]; ++q)
+      a[(size_t)i * n + h.col[l][q]] += h.val[l][q];
+  double *da = nullptr, *df = nullptr;
+  int* dfail = nullptr;
+  float* out = static_cast<float*>(c->alloc(sizeof(float) * (size_t)n * n));
+  bool ok = out && hipMalloc(&da, sizeof(double) * (size_t)n * n) == hipSuccess &&
+            hipMalloc(&df, sizeof(double) * n) == hipSuccess &&
+            hipMalloc(&dfail, sizeof(int)) == hipSuccess;
+  int hfail = 0;
+  if (ok) {
+    ok = hipMemcpy(da, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dfail, &hfail, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (ok) {
+    const dim3 blk(256), g1((n + 255) / 256), g2((n + 255) / 256, n);
+    for (int k = 0; k < n; ++k) {
+      hipLaunchKernelGGL(k_gj_col, g1, blk, 0, 0, n, (const double*)da, k, df);
+      hipLaunchKernelGGL(k_gj_row, g1, blk, 0, 0, n, da, k, (const double*)df);
+      hipLaunchKernelGGL(k_gj_rest, g2, blk, 0, 0, n, da, k, (const double*)df);
+      hipLaunchKernelGGL(k_gj_fix, g1, blk, 0, 0, n, da, k, (const double*)df, dfail);
+    }
+    const int64_t nn = (int64_t)n * n;
+    hipLaunchKernelGGL(k_d2f, dim3((unsigned)((nn + 255) / 256)), blk, 0, 0, nn,
+                       (const double*)da, out);
+    ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+         hipMemcpy(&hfail, dfail, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess && !hfail;
+  }
+  if (da) (void)hipFree(da);
+  if (df) (void)hipFree(df);
+  if (dfail) (void)hipFree(dfail);
+  return ok ? out : nullptr;
 }
 
 // Inverse of the dense level-l operator of one component (Gauss-Jordan with partial
@@ -963,15 +1127,29 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
   lv.cinv32 = upload_vec<float>(c, cinv);
   lv.cinfo32 = upload_vec<int64_t>(c, cinfo);
   if (!lv.cinv32 || !lv.cinfo32) return fail();
-  // one solve whose exact level is its FP32 start: the grid kernel applies the inverse
-  lv.dense_rm32 = nullptr;
-  lv.dense_n32 = 0;
-  if (nc == 1 && lvl[0] >= 0 && lvl[0] == lay.start32[lay.order[0]]) {
-    lv.dense_rm32 = lv.cinv32;
-    lv.dense_n32 = (int)lay.lh[lay.order[0]].n[lvl[0]];
-  }
   mg.exact32.assign(nc, -1);
   for (int b = 0; b < nc; ++b) mg.exact32[lay.order[b]] = (int)cinfo[2 * b + 1];
+  // one solve: the exact level is the first of at most DENSE_MAX_N rows (inverted on the
+  // device) and a grid kernel applies its inverse, in place of the single-work-group
+  // kernel and of the grid levels from there down
+  lv.dense_rm32 = nullptr;
+  lv.dense_n32 = 0;
+  lv.dense_level32 = -1;
+  if (nc == 1) {
+    const SfcLevelsHost& h = lay.lh[lay.order[0]];
+    for (int l = 0; l <= std::min(lay.start32[lay.order[0]], h.nlevel - 2); ++l)
+      if (h.n[l] <= DENSE_MAX_N) {
+        // the level the packed path inverts on the host: reuse that inverse, so that a
+        // solve alone and packed with others gives the same bits
+        if (float* inv = l == lvl[0] ? lv.cinv32 : dense_inverse_device(c, h, l)) {
+          lv.dense_rm32 = inv;
+          lv.dense_n32 = (int)h.n[l];
+          lv.dense_level32 = l;
+          mg.exact32[lay.order[0]] = l;
+        }
+        break;
+      }
+  }
   lv.desc64 = upload_vec<LevelDesc<double>>(c, d64);
   lv.desc32 = upload_vec<LevelDesc<float>>(c, d32);
   lv.rng64 = upload_vec<int32_t>(c, rng64);

@@ -237,6 +237,16 @@ class DGDiffusion:
                   "dgmi_apply")
         return r0, x_i
 
+    def smooth_n(self, x, b, n):
+        """n smoothing steps x += omega S (b - A x) in place (the reference's smoothing loop,
+        z02/volume_mf_diff.py:1947-1951, without the residual output)."""
+        x = self._vec(x.view(-1), "x")
+        b = self._vec(b.view(-1), "b", (x.dtype,))
+        self._ensure_smoother()
+        check(self.lib.dgmi_smooth_n(self.ctx, _DTYPES[x.dtype], _ptr(x), _ptr(b), int(n),
+                                     _stream()), "dgmi_smooth_n")
+        return x
+
     def get_bdiag_diag(self):
         """Diagonal blocks and diagonal of A in FP64 (z02/get_bdiag_diag.py:30-62)."""
         bd = torch.empty((self.nele, self.nloc, self.nloc), dtype=torch.float64, device=self.device)

@@ -104,6 +104,10 @@ int dgmi_smoother_setup(dgmi_ctx* ctx, int variant, double omega, void* stream);
 /* One smoothing step: r = b - A x; x += omega * S r (get_residual_or_smooth with
  * do_smooth=True).  r_out may be NULL (the residual is then not stored). */
 int dgmi_smooth(dgmi_ctx* ctx, int dtype, void* x, const void* b, void* r_out, void* stream);
+/* n smoothing steps in a row (n >= 1), as n calls of dgmi_smooth without r_out (the
+ * reference's smoothing loops, z02/volume_mf_diff.py:1947-1951).  The FP32 neighbour-form
+ * sweeps of the vector kernels carry the face traces from one step to the next. */
+int dgmi_smooth_n(dgmi_ctx* ctx, int dtype, void* x, const void* b, int n, void* stream);
 
 /* Multigrid set-up (z02/volume_mf_diff.py:1642-1754, z02/multigrid_linearelastic.py:175-283):
  * P1CG sparsity, distance-2 colouring, RAR by coloured probing with the FP64 apply,
@@ -161,10 +165,12 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     the preconditioner: 1 = x = (1 - omega) x + S (b - A_off x), which
  *                     needs no volume term and only the neighbour half of the face terms
  *                     (default); 0 = x += S (b - A x).  Equal in exact arithmetic; FP64
- *                     sweeps and dgmi_smooth always use the second form.
- *   "coarse_exact"    FP32 V-cycles: 1 = stop at the first SFC level of at most 512 rows
- *                     and solve it with its dense inverse (formed at dgmi_mg_setup;
- *                     default); 0 = sweep every level as the FP64 cycle (the reference's)
+ *                     sweeps and dgmi_smooth with r_out always use the second form.
+ *   "coarse_exact"    FP32 V-cycles: 1 = stop at one SFC level and solve it with its
+ *                     dense inverse formed at dgmi_mg_setup (default): a context of one
+ *                     solve uses the first level of at most 2048 rows, a packed context
+ *                     the first level of at most 512 rows at or below each solve's LDS
+ *                     start; 0 = sweep every level as the FP64 cycle (the reference's)
  *                     does.  dgmi_mg_info writes the level to info[4 + nlevel] when the
  *                     buffer holds it (-1: none).
  *   "mfma_waves", "bf16_waves"  waves (16 elements each) per work-group of the fp32 / bf16

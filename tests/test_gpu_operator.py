@@ -229,3 +229,32 @@ def test_autotune_keeps_a_correct_kernel(prob):
     x = np.random.default_rng(25).standard_normal(sp.ndof)
     assert rel(-to_np(g.apply(to_dev(x, torch.float32))), dg.apply_A(sp, x)) < 1e-6
     g.set_f32_kernel("mfma")
+
+
+@pytest.mark.parametrize("kernel", ["vector", "mfma"])
+def test_smooth_n_matches_repeated_oracle_steps(kernel):
+    """n smoothing steps in one call (FP32 vector sweeps carry the traces from step to
+    step) equal n steps of the oracle's get_residual_or_smooth with the exact block
+    smoother: FP64 to 1e-11, FP32 to single precision."""
+    from dg_nn_public_amd import meshgen
+    from dg_nn_public_amd.diffusion import DGDiffusion
+    from gpu_common import DEV
+    m = meshgen.cube_mesh(4, seed=2)
+    pb = meshgen.sine_problem(m, sigma=1.0)
+    g = DGDiffusion.from_mesh(m, pb.kappa, blk_solver="direct", device=DEV,
+                              face_kappa="harmonic")
+    sp = dg.make_space(m, pb.kappa, face_kappa="harmonic")
+    _, B = dg.get_bdiag_diag(sp)
+    rng = np.random.default_rng(41)
+    x0, b = rng.standard_normal((2, sp.ndof))
+    x_ref = x0.copy()
+    for _ in range(5):
+        r_ref = np.zeros(sp.ndof)
+        dg.get_residual_or_smooth(sp, r_ref, x_ref, b, True, B, "direct")
+    x64 = g.smooth_n(to_dev(x0), to_dev(b), 5)
+    assert rel(to_np(x64), x_ref) < 1e-11
+    g.set_f32_kernel(kernel)
+    x32 = g.smooth_n(to_dev(x0, torch.float32), to_dev(b, torch.float32), 5)
+    print(kernel, "smooth_n fp32 rel err", rel(to_np(x32), x_ref), flush=True)
+    assert rel(to_np(x32), x_ref) < 1e-5
+    g.set_f32_kernel("mfma")
