@@ -21,6 +21,7 @@
 // smoother): it needs no volume term and only the neighbour half of the face terms.
 #include "dgmi.h"
 #include "dgmi_internal.h"
+#include "faces.h"
 #include "rowvec.h"
 
 namespace dgmi {
@@ -36,42 +37,33 @@ template <> struct Dm<2> { static constexpr int NL = 10, NGI = 12, SG = 4, NF = 
 // lanes of work.  Partial residuals are summed through LDS.
 template <typename T, int D>
 __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __restrict__ x,
-                                                          const T* __restrict__ ftab,
+                                                          const T* __restrict__ fc,
+                                                          const int32_t* __restrict__ fnodes,
                                                           const T* __restrict__ cf,
                                                           typename Vec2<T>::type* __restrict__ trace,
                                                           const int32_t* __restrict__ act) {
   using M = Dm<D>;
-  if (act && !act[blockIdx.x]) return;  // 64 elements of finished solves
+  using F = FaceDims<D>;
+  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  if (act && !act[grp]) return;  // 64 elements of finished solves
   const int lane = threadIdx.x & 63;
   const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int e = blockIdx.x * 64 + lane;
+  const int e = grp * 64 + lane;
   if (e >= nele) return;
-  T xv[M::NL];
-  load_row<T, M::NL>(x + (size_t)e * M::NL, xv);
+  const T* xe = x + (size_t)e * M::NL;
+  T xv[M::NL], xf[F::NFN];
+  load_row<T, M::NL>(xe, xv);
+#pragma unroll
+  for (int i = 0; i < F::NFN; ++i) xf[i] = xe[fnodes[f * F::NFN + i]];  // same lines: cached
   T c[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
-  const T* tf = ftab + f * M::SG * (1 + D) * M::NL;
-#pragma unroll 2
-  for (int g = 0; g < M::SG; ++g) {
-    const T* t = tf + g * (1 + D) * M::NL;
-    T u = 0, gd[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) gd[d] = 0;
-#pragma unroll
-    for (int n = 0; n < M::NL; ++n) {
-      u += t[n] * xv[n];
-#pragma unroll
-      for (int d = 0; d < D; ++d) gd[d] += t[(1 + d) * M::NL + n] * xv[n];
-    }
-    T dn = 0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) dn += c[d] * gd[d];
+  face_traces<T, D>(f, fc, xv, xf, c, [&](int g, T u, T dn) {
     typename Vec2<T>::type tv;
     tv.x = u;
     tv.y = dn;
     trace[(f * M::SG + g) * nele + e] = tv;
-  }
+  });
 }
 
 // EMIT (mode 5 only): also write the face traces of the new x into tout (another buffer
@@ -80,15 +72,18 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
 template <typename T, int D, int MODE, bool EMIT = false>
 __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     int nele, T* __restrict__ x, const T* b, T* r, const T* __restrict__ vtab,
-    const T* __restrict__ wgt, const T* __restrict__ ftab, const T* __restrict__ sw,
+    const T* __restrict__ wgt, const T* __restrict__ sw,
     const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
     const T* __restrict__ wf, const T* __restrict__ pen, const int32_t* __restrict__ nbface,
     const int32_t* __restrict__ fcode, const typename Vec2<T>::type* __restrict__ trace,
     const T* __restrict__ S, const int32_t* __restrict__ act, T omega,
-    typename Vec2<T>::type* __restrict__ tout) {
+    typename Vec2<T>::type* __restrict__ tout, const T* __restrict__ fc,
+    const int32_t* __restrict__ fnodes) {
   using M = Dm<D>;
+  using F = FaceDims<D>;
   static_assert(!EMIT || MODE == 5 || MODE == 6, "traces are emitted by the sweeps");
-  if (act && !act[blockIdx.x]) return;  // whole block: no barrier is reached
+  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  if (act && !act[grp]) return;  // whole block: no barrier is reached
   constexpr bool OFFD = MODE == 5;
   constexpr bool ZERO = MODE == 6;  // sweep from x = 0: x = S b (x is not read)
   constexpr int NL = M::NL, NF = M::NF;
@@ -98,7 +93,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   __shared__ T red[NF][NL][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int e = blockIdx.x * 64 + lane;
+  const int e = grp * 64 + lane;
   const bool valid = e < nele;
   T xv[NL], acc[NL];
 #pragma unroll
@@ -148,6 +143,9 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
       }
     }
   }
+  // face w's lift onto its own nodes, added to red[w] after the row sums are stored
+  FaceLift<T, D> fl;
+  fl.zero();
   if (valid && !ZERO) {
     // ---- face w (mode 5: interior faces, neighbour terms only)
     const int f = w;
@@ -167,7 +165,6 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
         nbe = nbz / NF;
         nbf = nbz - nbe * NF;
       }
-      const T* tf = ftab + f * M::SG * (1 + D) * NL;
 #pragma unroll 2
       for (int g = 0; g < M::SG; ++g) {
         const T ws = wfe * sw[g];
@@ -189,22 +186,16 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
           qsn = ws * (pe * tr.x - tr.y);
           qdn = -ws * tr.x;
         }
-        T qd[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) qd[d] = c[d] * qdn;
-        const T* t = tf + g * (1 + D) * NL;
-#pragma unroll
-        for (int n = 0; n < NL; ++n) {
-          T s = acc[n] + t[n] * qsn;
-#pragma unroll
-          for (int d = 0; d < D; ++d) s += t[(1 + d) * NL + n] * qd[d];
-          acc[n] = s;
-        }
+        fl.add(f, g, fc, qsn, qdn);
       }
+      fl.finish(f, fc, c, acc);
     }
   }
 #pragma unroll
   for (int n = 0; n < NL; ++n) red[w][n][lane] = acc[n];
+  // the face's own nodes (a wave-uniform list): read-modify-write of this lane's column
+#pragma unroll
+  for (int i = 0; i < F::NFN; ++i) red[w][fnodes[w * F::NFN + i]][lane] += fl.accf[i];
   __syncthreads();
   // rows n of this wave: sum the partials, form the residual
   T rv[NPW];
@@ -258,31 +249,19 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
       __syncthreads();
       if (valid) {
         const int f = w;
-        T cc[D];
+        T cc[D], xn[NL], xf[F::NFN];
 #pragma unroll
         for (int d = 0; d < D; ++d) cc[d] = cf[(f * D + d) * nele + e];
-        const T* tf = ftab + f * M::SG * (1 + D) * NL;
-#pragma unroll 2
-        for (int g = 0; g < M::SG; ++g) {
-          const T* t = tf + g * (1 + D) * NL;
-          T u = 0, gd[D];
 #pragma unroll
-          for (int d = 0; d < D; ++d) gd[d] = 0;
+        for (int n = 0; n < NL; ++n) xn[n] = red[1][n][lane];
 #pragma unroll
-          for (int n = 0; n < NL; ++n) {
-            const T xn = red[1][n][lane];
-            u += t[n] * xn;
-#pragma unroll
-            for (int d = 0; d < D; ++d) gd[d] += t[(1 + d) * NL + n] * xn;
-          }
-          T dn = 0;
-#pragma unroll
-          for (int d = 0; d < D; ++d) dn += cc[d] * gd[d];
+        for (int i = 0; i < F::NFN; ++i) xf[i] = red[1][fnodes[f * F::NFN + i]][lane];
+        face_traces<T, D>(f, fc, xn, xf, cc, [&](int g, T u, T dn) {
           typename Vec2<T>::type tv;
           tv.x = u;
           tv.y = dn;
           tout[(f * M::SG + g) * nele + e] = tv;
-        }
+        });
       }
     }
   }
@@ -592,8 +571,8 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   if (mode == 6) {
     tin = tout == 0 ? 1 : 0;  // a sweep from zero reads no traces
   } else if (tin < 0) {
-    hipLaunchKernelGGL((k_trace_w<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_ftab, g.cf,
-                       g.trace, act);
+    hipLaunchKernelGGL((k_trace_w<T, D>), grd, blk, 0, s, nele, x, g.tab + g.off_fc, c->fnodes,
+                       g.cf, g.trace, act);
     tin = 0;
   }
   if (!(tin <= 1 && tout <= 1 && tout != tin && (tout < 0 || mode >= 5) && bufs[tin] &&
@@ -605,12 +584,11 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   V2* to = tout >= 0 ? bufs[tout] : nullptr;
   const T* vt = g.tab + g.off_vtab;
   const T* wg = g.tab + g.off_wgt;
-  const T* ft = g.tab + g.off_ftab;
   const T* sw = g.tab + g.off_sw;
 #define DGMI_RES(MODE, EMIT)                                                                  \
   hipLaunchKernelGGL((k_residual_w<T, D, MODE, EMIT>), grd, blk, 0, s, nele, x, b, r, vt, wg,  \
-                     ft, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, trace, g.S, \
-                     act, (T)c->omega, to)
+                     sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, trace, g.S,     \
+                     act, (T)c->omega, to, g.tab + g.off_fc, c->fnodes)
   switch (mode) {
     case 0: DGMI_RES(0, false); break;
     case 1: DGMI_RES(1, false); break;

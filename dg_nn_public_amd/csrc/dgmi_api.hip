@@ -9,6 +9,7 @@
 
 #include "dgmi.h"
 #include "dgmi_internal.h"
+#include "faces.h"
 
 namespace dgmi {
 
@@ -51,7 +52,7 @@ static T* upload(dgmi_ctx* c, const std::vector<U>& h) {
 
 // kernel-friendly table layout (see DevGeom)
 static std::vector<double> table_block(const ElementTables& el, size_t& ov, size_t& ow,
-                                       size_t& of, size_t& os) {
+                                       size_t& of, size_t& os, size_t& oc) {
   const int D = el.D, NL = el.nloc, NG = el.ngi, SG = el.sngi, NF = el.nface;
   std::vector<double> t;
   ov = t.size();
@@ -70,17 +71,23 @@ static std::vector<double> table_block(const ElementTables& el, size_t& ov, size
     }
   os = t.size();
   for (int g = 0; g < SG; ++g) t.push_back(el.sweight[g]);
+  while (t.size() % 4) t.push_back(0.0);
+  oc = t.size();  // fphi | flk | fgk, the layout FaceDims<D> assumes
+  t.insert(t.end(), el.fphi.begin(), el.fphi.end());
+  t.insert(t.end(), el.flk.begin(), el.flk.end());
+  t.insert(t.end(), el.fgk.begin(), el.fgk.end());
   return t;
 }
 
 template <typename T>
 static int upload_geom(dgmi_ctx* c, DevGeom<T>& g, const std::vector<double>& tab, size_t ov,
-                       size_t ow, size_t of, size_t os, const Geometry& geo) {
+                       size_t ow, size_t of, size_t os, size_t oc, const Geometry& geo) {
   g.tab = upload<T>(c, tab);
   g.off_vtab = ov;
   g.off_wgt = ow;
   g.off_ftab = of;
   g.off_sw = os;
+  g.off_fc = oc;
   g.G = upload<T>(c, geo.G);
   g.cf = upload<T>(c, geo.cf);
   g.wf = upload<T>(c, geo.wf);
@@ -662,17 +669,28 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
               std::to_string(geo.max_affine_dev));
     return -2;
   }
-  size_t ov, ow, of, os;
-  const std::vector<double> tab = table_block(c->el, ov, ow, of, os);
-  int rc = upload_geom<double>(c, c->g64, tab, ov, ow, of, os, geo);
-  if (!rc) rc = upload_geom<float>(c, c->g32, tab, ov, ow, of, os, geo);
+  // the face kernels interpolate du/dn quadratically over each face: exact for P3, so the
+  // interpolant reproduces the tabulated gradients to rounding
+  if (!(c->el.face_interp_err < 1e-12) ||
+      c->el.snloc != (c->D == 3 ? FaceDims<3>::NFN : FaceDims<2>::NFN) ||
+      c->el.nfk != (c->D == 3 ? FaceDims<3>::NK : FaceDims<2>::NK)) {
+    dgmi_ctx_destroy(c);
+    set_error("face interpolation tables do not reproduce the face gradients");
+    return -2;
+  }
+  size_t ov, ow, of, os, oc;
+  const std::vector<double> tab = table_block(c->el, ov, ow, of, os, oc);
+  int rc = upload_geom<double>(c, c->g64, tab, ov, ow, of, os, oc, geo);
+  if (!rc) rc = upload_geom<float>(c, c->g32, tab, ov, ow, of, os, oc, geo);
   if (!rc) {
     c->nbface = upload<int32_t>(c, geo.nbface);
     c->fcode = upload<int32_t>(c, geo.fcode);
     std::vector<int32_t> al(c->el.gi_align.begin(), c->el.gi_align.end());
     c->aln = upload<int32_t>(c, al);
+    std::vector<int32_t> fn(c->el.fnodes.begin(), c->el.fnodes.end());
+    c->fnodes = upload<int32_t>(c, fn);
     c->detJ = upload<double>(c, geo.detJ);
-    if (!c->nbface || !c->fcode || !c->aln || !c->detJ) {
+    if (!c->nbface || !c->fcode || !c->aln || !c->fnodes || !c->detJ) {
       set_error(alloc_failure());
       rc = -3;
     }

@@ -38,6 +38,7 @@ template <> struct Vec2<float> { using type = float2; };
 template <typename T> struct DevGeom {
   T* tab = nullptr;
   size_t off_vtab = 0, off_wgt = 0, off_ftab = 0, off_sw = 0;
+  size_t off_fc = 0;  // face-interpolation block (FaceDims in faces.h)
   T* G = nullptr;      // (ng, nele)
   T* cf = nullptr;     // (nface*D, nele)
   T* wf = nullptr;     // (nface, nele)
@@ -217,6 +218,7 @@ struct dgmi_ctx {
   int32_t* nbface = nullptr;
   int32_t* fcode = nullptr;
   int32_t* aln = nullptr;
+  int32_t* fnodes = nullptr;  // (nface, snloc) the nodes on each face
   double* detJ = nullptr;  // (nele) FP64
   dgmi::DevGeom<double> g64;
   dgmi::DevGeom<float> g32;

@@ -1,0 +1,115 @@
+// Face traces and face lifts of the vector kernels through a quadratic interpolant.
+//
+// On face f of a P3 element only the face's own nodes (10 of 20 in 3D, 4 of 10 in 2D)
+// have nonzero basis functions, so the trace u and the lift of the flux term onto u's
+// test functions involve those nodes only.  The normal derivative c . grad(u)
+// (c = J^-T n_f, constant on an affine element) restricted to the face is a quadratic
+// polynomial, fixed by its values at the face's quadratic nodes (3 vertices and 3 edge
+// midpoints in 3D, 2 end points and the midpoint in 2D):
+//   trace:  dn(g) = sum_k L_k(g) c . grad(u)(p_k)                 (instead of per point)
+//   lift:   sum_g q(g) c . grad(phi_n)(g) = sum_k (sum_g L_k(g) q(g)) c . grad(phi_n)(p_k)
+// Per face in 3D this takes 6 x 63 + 12 x 16 = 570 FMAs for the traces and the same for
+// the lift, against 12 x 83 and 12 x 80 point by point.  The tables are built and checked
+// against the tabulated face gradients on the host (exact to rounding for P3).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace dgmi {
+
+// Layout of the face-interpolation block (DevGeom::off_fc), in the table's precision:
+//   PHI [f][g][NFN]     phi of the face's nodes at its points
+//   LK  [g][NK]         quadratic face Lagrange basis at the face points
+//   GK  [f][k][D][NL]   d phi / d xi of every basis function at the face's quadratic nodes
+// and the int32 node lists fnodes[f][NFN].
+// Work-groups are dispatched to the 8 XCDs round-robin by id (group b runs on XCD b % 8),
+// and each XCD has its own L2.  Face kernels read their neighbours' traces, and neighbours
+// are close in the element (SFC) order, so each XCD is given one contiguous range of
+// element groups: group b takes logical group xcd_start(b % 8) + b / 8.  A bijection on
+// [0, n) for any n.
+__device__ __forceinline__ int xcd_group(int b, int n) {
+  constexpr int NX = 8;
+  const int x = b % NX, q = n / NX, r = n % NX;
+  return x * q + (x < r ? x : r) + b / NX;
+}
+
+template <int D> struct FaceDims {
+  static constexpr int NL = D == 3 ? 20 : 10, SG = D == 3 ? 12 : 4, NF = D + 1;
+  static constexpr int NFN = D == 3 ? 10 : 4, NK = D == 3 ? 6 : 3;
+  static constexpr int PHI = 0, LK = NF * SG * NFN, GK = LK + SG * NK;
+};
+
+// Traces (u, du/dn) of face f at its SG points: xv = the element's row, xf = the values of
+// the face's nodes (fnodes order), c = J^-T n_f; store(g, u, dn) writes point g.
+template <typename T, int D, typename Store>
+__device__ __forceinline__ void face_traces(int f, const T* __restrict__ fc, const T* xv,
+                                            const T* xf, const T* c, Store store) {
+  using F = FaceDims<D>;
+  const T* gk = fc + F::GK + f * F::NK * D * F::NL;
+  T dk[F::NK];
+#pragma unroll 2
+  for (int k = 0; k < F::NK; ++k) {
+    T gd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gd[d] = 0;
+#pragma unroll
+    for (int n = 0; n < F::NL; ++n)
+#pragma unroll
+      for (int d = 0; d < D; ++d) gd[d] += gk[(k * D + d) * F::NL + n] * xv[n];
+    T s = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) s += c[d] * gd[d];
+    dk[k] = s;
+  }
+  const T* ph = fc + F::PHI + f * F::SG * F::NFN;
+  const T* lk = fc + F::LK;
+#pragma unroll
+  for (int g = 0; g < F::SG; ++g) {
+    T u = 0, dn = 0;
+#pragma unroll
+    for (int i = 0; i < F::NFN; ++i) u += ph[g * F::NFN + i] * xf[i];
+#pragma unroll
+    for (int k = 0; k < F::NK; ++k) dn += lk[g * F::NK + k] * dk[k];
+    store(g, u, dn);
+  }
+}
+
+// Accumulators of one face's lift: accf[i] += phi_i(g) qs(g) for the face's nodes, and
+// Q[k] += L_k(g) qd(g) for the normal-derivative part.
+template <typename T, int D> struct FaceLift {
+  using F = FaceDims<D>;
+  T accf[F::NFN];
+  T Q[F::NK];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < F::NFN; ++i) accf[i] = 0;
+#pragma unroll
+    for (int k = 0; k < F::NK; ++k) Q[k] = 0;
+  }
+  __device__ __forceinline__ void add(int f, int g, const T* __restrict__ fc, T qs, T qd) {
+    const T* ph = fc + F::PHI + (f * F::SG + g) * F::NFN;
+    const T* lk = fc + F::LK + g * F::NK;
+#pragma unroll
+    for (int i = 0; i < F::NFN; ++i) accf[i] += ph[i] * qs;
+#pragma unroll
+    for (int k = 0; k < F::NK; ++k) Q[k] += lk[k] * qd;
+  }
+  // acc[n] += sum_{k,d} c_d Q_k d phi_n / d xi_d (p_k), over all nodes
+  __device__ __forceinline__ void finish(int f, const T* __restrict__ fc, const T* c, T* acc) const {
+    const T* gk = fc + F::GK + f * F::NK * D * F::NL;
+    T W[F::NK][D];
+#pragma unroll
+    for (int k = 0; k < F::NK; ++k)
+#pragma unroll
+      for (int d = 0; d < D; ++d) W[k][d] = c[d] * Q[k];
+#pragma unroll
+    for (int k = 0; k < F::NK; ++k)
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const T* t = gk + (k * D + d) * F::NL;
+#pragma unroll
+        for (int n = 0; n < F::NL; ++n) acc[n] += t[n] * W[k][d];
+      }
+  }
+};
+
+}  // namespace dgmi

@@ -9,6 +9,7 @@
 #include <mutex>
 
 #include "dgmi_internal.h"
+#include "faces.h"
 
 namespace dgmi {
 
@@ -86,19 +87,23 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nele, const int32_t* __
 
 // k_prolong_add fused with the face-trace pass of the vector kernels: a work-group of
 // nface waves owns 64 elements; every wave forms the element's whole new row
-// x + P e (cheap: nloc x (D+1) products), writes its share of the rows and the traces of
-// its face w into tout[(w*sngi + g)*nele + e] (same operations as the trace pass).
-// ftab: per face, per point, (phi, dphi/dxi_d) x nloc; cf: (nface*D, nele).
+// x + P e (cheap: nloc x (D+1) products), writes its share of the rows (to x and to LDS,
+// where the other waves read their face's nodes) and the traces of its face w into
+// tout[(w*sngi + g)*nele + e] (same operations as the trace pass).
+// fc / fnodes: the face-interpolation tables (faces.h); cf: (nface*D, nele).
 template <typename T, int D>
 __global__ __launch_bounds__(64 * (D + 1)) void k_prolong_trace(
     int nele, const int32_t* __restrict__ cellpos, const T* __restrict__ rw,
-    const T* __restrict__ ec, T* __restrict__ x, const T* __restrict__ ftab,
-    const T* __restrict__ cf, typename Vec2<T>::type* __restrict__ tout) {
-  constexpr int NL = D == 3 ? 20 : 10, NB = D + 1, NF = D + 1, SG = D == 3 ? 12 : 4;
+    const T* __restrict__ ec, T* __restrict__ x, const T* __restrict__ fc,
+    const int32_t* __restrict__ fnodes, const T* __restrict__ cf,
+    typename Vec2<T>::type* __restrict__ tout) {
+  using F = FaceDims<D>;
+  constexpr int NL = F::NL, NB = D + 1, NF = D + 1, SG = F::SG;
   constexpr int NPW = (NL + NF - 1) / NF;
+  __shared__ T xs[NL][64];
   const int lane = threadIdx.x & 63;
   const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int e = blockIdx.x * 64 + lane;
+  const int e = xcd_group(blockIdx.x, gridDim.x) * 64 + lane;
   const bool valid = e < nele;
   T xv[NL];
   T* xe = x + (size_t)(valid ? e : 0) * NL;
@@ -119,36 +124,29 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_prolong_trace(
   }
   // every wave has read the old row before any wave writes its share of the new one
   __syncthreads();
-  if (!valid) return;
+  if (valid) {
 #pragma unroll
-  for (int k = 0; k < NPW; ++k) {
-    const int n = f * NPW + k;
-    if (n < NL) xe[n] = xv[n];
+    for (int k = 0; k < NPW; ++k) {
+      const int n = f * NPW + k;
+      if (n < NL) {
+        xe[n] = xv[n];
+        xs[n][lane] = xv[n];
+      }
+    }
   }
-  T c[D];
+  __syncthreads();
+  if (!valid) return;
+  T c[D], xf[F::NFN];
 #pragma unroll
   for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
-  const T* tf = ftab + f * SG * (1 + D) * NL;
-#pragma unroll 2
-  for (int g = 0; g < SG; ++g) {
-    const T* t = tf + g * (1 + D) * NL;
-    T u = 0, gd[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) gd[d] = 0;
-#pragma unroll
-    for (int n = 0; n < NL; ++n) {
-      u += t[n] * xv[n];
-#pragma unroll
-      for (int d = 0; d < D; ++d) gd[d] += t[(1 + d) * NL + n] * xv[n];
-    }
-    T dn = 0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) dn += c[d] * gd[d];
+  for (int i = 0; i < F::NFN; ++i) xf[i] = xs[fnodes[f * F::NFN + i]][lane];
+  face_traces<T, D>(f, fc, xv, xf, c, [&](int g, T u, T dn) {
     typename Vec2<T>::type tv;
     tv.x = u;
     tv.y = dn;
     tout[(f * SG + g) * nele + e] = tv;
-  }
+  });
 }
 
 // probe vector for colour c: x = P I_fc mask_c (z02/volume_mf_diff.py:1717-1722)
@@ -312,7 +310,7 @@ __host__ __device__ inline size_t lds_align(size_t x) { return (x + 15) & ~size_
 // -1.  With c >= 0 the cycle stops at level c and sets B_c = A_c^-1 r_c there; the levels
 // below c are not visited.
 constexpr int EXACT_MAX_N = 512;    // single-work-group path (packed solves)
-constexpr int DENSE_MAX_N = 2048;   // grid path (a context of one solve)
+constexpr int DENSE_MAX_N = 4096;   // grid path (a context of one solve)
 
 template <typename T>
 __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
@@ -650,10 +648,12 @@ int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout,
   const dim3 blk(64 * (c->D + 1)), grd((nele + 63) / 64);
   if (c->D == 3)
     hipLaunchKernelGGL((k_prolong_trace<float, 3>), grd, blk, 0, s, nele, c->mg.cellpos,
-                       (const float*)c->mg.rw32, e_sfc, x_dg, g.tab + g.off_ftab, g.cf, to);
+                       (const float*)c->mg.rw32, e_sfc, x_dg, g.tab + g.off_fc, c->fnodes, g.cf,
+                       to);
   else
     hipLaunchKernelGGL((k_prolong_trace<float, 2>), grd, blk, 0, s, nele, c->mg.cellpos,
-                       (const float*)c->mg.rw32, e_sfc, x_dg, g.tab + g.off_ftab, g.cf, to);
+                       (const float*)c->mg.rw32, e_sfc, x_dg, g.tab + g.off_fc, c->fnodes, g.cf,
+                       to);
   DGMI_HIP(hipGetLastError());
   return 0;
 }

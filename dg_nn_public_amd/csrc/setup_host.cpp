@@ -136,6 +136,21 @@ void lagrange(const std::vector<int>& index, int nb, const double* lam, double* 
   }
 }
 
+// element barycentrics of the point with face barycentrics p on face f
+// (z02/shape_function.py:763-783 tets, :1105-1116 triangles)
+void face_lam(int D, int f, const double* p, double* lam) {
+  for (int i = 0; i < 4; ++i) lam[i] = 0.0;
+  if (D == 3) {
+    const double p0 = p[0], p1 = p[1], p2 = p[2];
+    const double tab[4][4] = {{0, p1, p0, p2}, {p0, 0, p1, p2}, {p1, p0, 0, p2}, {p0, p1, p2, 0}};
+    for (int i = 0; i < 4; ++i) lam[i] = tab[f][i];
+  } else {
+    const double s0 = p[0], s1 = p[1];
+    const double tab[3][3] = {{s0, s1, 0}, {0, s0, s1}, {s1, 0, s0}};
+    for (int i = 0; i < 3; ++i) lam[i] = tab[f][i];
+  }
+}
+
 }  // namespace
 
 ElementTables make_element(int D) {
@@ -180,18 +195,8 @@ ElementTables make_element(int D) {
   el.snlx.assign(nf * D * nloc * sg, 0.0);
   for (int f = 0; f < nf; ++f) {
     for (int g = 0; g < sg; ++g) {
-      double lam[4] = {0, 0, 0, 0};
-      if (D == 3) {
-        double p0 = fp[g * 3 + 0], p1 = fp[g * 3 + 1], p2 = fp[g * 3 + 2];
-        // z02/shape_function.py:763-783
-        const double tab[4][4] = {{0, p1, p0, p2}, {p0, 0, p1, p2}, {p1, p0, 0, p2}, {p0, p1, p2, 0}};
-        for (int i = 0; i < 4; ++i) lam[i] = tab[f][i];
-      } else {
-        double s0 = fp[g * 2 + 0], s1 = fp[g * 2 + 1];
-        // z02/shape_function.py:1105-1116
-        const double tab[3][3] = {{s0, s1, 0}, {0, s0, s1}, {s1, 0, s0}};
-        for (int i = 0; i < 3; ++i) lam[i] = tab[f][i];
-      }
+      double lam[4];
+      face_lam(D, f, &fp[g * D], lam);
       lagrange(el.index, nb, lam, phi.data(), dxi.data(), nloc);
       for (int n = 0; n < nloc; ++n) {
         el.sn[(f * nloc + n) * sg + g] = phi[n];
@@ -231,6 +236,68 @@ ElementTables make_element(int D) {
       }
       el.gi_align[k * sg + g] = best;
     }
+  }
+  // quadratic interpolation of face gradients: nodes in face barycentrics, their Lagrange
+  // basis at the face points, gradients there; checked against the tabulated snlx
+  {
+    const int fb = D;  // face barycentric coordinates per point
+    std::vector<std::array<double, 3>> qn;
+    if (D == 3)
+      qn = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0.5, 0.5, 0}, {0, 0.5, 0.5}, {0.5, 0, 0.5}};
+    else
+      qn = {{1, 0, 0}, {0, 1, 0}, {0.5, 0.5, 0}};
+    const int nk = (int)qn.size();
+    el.nfk = nk;
+    auto quad_basis = [&](const double* mu, double* L) {
+      if (D == 3) {
+        for (int i = 0; i < 3; ++i) L[i] = mu[i] * (2.0 * mu[i] - 1.0);
+        L[3] = 4.0 * mu[0] * mu[1];
+        L[4] = 4.0 * mu[1] * mu[2];
+        L[5] = 4.0 * mu[0] * mu[2];
+      } else {
+        L[0] = mu[0] * (2.0 * mu[0] - 1.0);
+        L[1] = mu[1] * (2.0 * mu[1] - 1.0);
+        L[2] = 4.0 * mu[0] * mu[1];
+      }
+    };
+    el.flk.assign(sg * nk, 0.0);
+    for (int g = 0; g < sg; ++g) quad_basis(&fp[g * fb], &el.flk[g * nk]);
+    el.fgk.assign(nf * nk * D * nloc, 0.0);
+    el.fnodes.clear();
+    el.fphi.assign(nf * sg * el.snloc, 0.0);
+    double err = 0.0, big = 0.0;
+    for (int f = 0; f < nf; ++f) {
+      for (int k = 0; k < nk; ++k) {
+        double lam[4];
+        face_lam(D, f, qn[k].data(), lam);
+        lagrange(el.index, nb, lam, phi.data(), dxi.data(), nloc);
+        for (int d = 0; d < D; ++d)
+          for (int n = 0; n < nloc; ++n) el.fgk[((f * nk + k) * D + d) * nloc + n] = dxi[d * nloc + n];
+      }
+      std::vector<int> on;
+      for (int n = 0; n < nloc; ++n) {
+        double m = 0.0;
+        for (int g = 0; g < sg; ++g) m = std::max(m, std::fabs(el.sn[(f * nloc + n) * sg + g]));
+        if (m != 0.0) on.push_back(n);
+      }
+      if ((int)on.size() != el.snloc) err = INFINITY;
+      for (int i = 0; i < (int)on.size() && i < el.snloc; ++i) {
+        el.fnodes.push_back(on[i]);
+        for (int g = 0; g < sg; ++g)
+          el.fphi[(f * sg + g) * el.snloc + i] = el.sn[(f * nloc + on[i]) * sg + g];
+      }
+      for (int d = 0; d < D; ++d)
+        for (int n = 0; n < nloc; ++n)
+          for (int g = 0; g < sg; ++g) {
+            double v = 0.0;
+            for (int k = 0; k < nk; ++k)
+              v += el.flk[g * nk + k] * el.fgk[((f * nk + k) * D + d) * nloc + n];
+            const double ref = el.snlx[((f * D + d) * nloc + n) * sg + g];
+            err = std::max(err, std::fabs(v - ref));
+            big = std::max(big, std::fabs(ref));
+          }
+    }
+    el.face_interp_err = big > 0.0 ? err / big : err;
   }
   el.p1_prolong.assign(nloc * nb, 0.0);
   for (int n = 0; n < nloc; ++n)

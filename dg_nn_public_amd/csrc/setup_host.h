@@ -53,6 +53,16 @@ struct ElementTables {
   std::vector<int> gi_align;    // (D, sngi)
   std::vector<double> p1_prolong;  // (nloc, D+1)
   std::vector<int> index;       // (nloc, D+1) barycentric multi-indices
+  // Face traces through a quadratic interpolant.  On face f only the snloc nodes fnodes[f]
+  // have nonzero basis functions, and the reference gradient of every basis function
+  // restricted to the face is a quadratic polynomial, fixed by its values at the nfk
+  // quadratic nodes of the face (vertices and edge midpoints: 6 in 3D, 3 in 2D).
+  int nfk = 0;
+  std::vector<int> fnodes;      // (nface, snloc) nodes on each face, ascending
+  std::vector<double> fphi;     // (nface, sngi, snloc) phi of those nodes at the face points
+  std::vector<double> flk;      // (sngi, nfk) quadratic face Lagrange basis at the face points
+  std::vector<double> fgk;      // (nface, nfk, D, nloc) dphi/dxi at the face's quadratic nodes
+  double face_interp_err = 0.0; // max |interpolated - tabulated| over snlx (set-up check)
 };
 ElementTables make_element(int D);
 

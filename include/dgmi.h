@@ -168,7 +168,7 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     sweeps and dgmi_smooth with r_out always use the second form.
  *   "coarse_exact"    FP32 V-cycles: 1 = stop at one SFC level and solve it with its
  *                     dense inverse formed at dgmi_mg_setup (default): a context of one
- *                     solve uses the first level of at most 2048 rows, a packed context
+ *                     solve uses the first level of at most 4096 rows, a packed context
  *                     the first level of at most 512 rows at or below each solve's LDS
  *                     start; 0 = sweep every level as the FP64 cycle (the reference's)
  *                     does.  dgmi_mg_info writes the level to info[4 + nlevel] when the
