@@ -98,14 +98,22 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   T xv[NL], acc[NL];
 #pragma unroll
   for (int n = 0; n < NL; ++n) acc[n] = xv[n] = 0;
-  // this wave's right-hand-side rows, requested before the face and volume work
-  T bv[NPW];
+  // this wave's rows of b and of x (kept or damped by the sweeps), requested before the
+  // face and volume work; loaded by row index, so no register array is indexed by w
+  constexpr bool KEEPX = MODE == 2 || MODE == 3 || OFFD;
+  T bv[NPW], xo[NPW];
 #pragma unroll
   for (int k = 0; k < NPW; ++k) {
     const int n = w * NPW + k;
-    bv[k] = (MODE != 0 && MODE != 4 && valid && n < NL) ? b[(size_t)e * NL + n] : T(0);
+    bv[k] = (MODE != 0 && MODE != 4 && !ZERO && valid && n < NL) ? b[(size_t)e * NL + n] : T(0);
+    xo[k] = (KEEPX && valid && n < NL) ? x[(size_t)e * NL + n] : T(0);
   }
-  if (valid && !ZERO) {
+  // x = S b reads the whole row of b instead of exchanging it through LDS
+  T bfull[NL];
+  if constexpr (ZERO) {
+    if (valid) load_row<T, NL>(b + (size_t)e * NL, bfull);
+  }
+  if (valid && !ZERO && !OFFD) {
     load_row<T, NL>(x + (size_t)e * NL, xv);
   }
   if (valid && !OFFD && !ZERO) {
@@ -191,24 +199,26 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
       fl.finish(f, fc, c, acc);
     }
   }
-#pragma unroll
-  for (int n = 0; n < NL; ++n) red[w][n][lane] = acc[n];
-  // the face's own nodes (a wave-uniform list): read-modify-write of this lane's column
-#pragma unroll
-  for (int i = 0; i < F::NFN; ++i) red[w][fnodes[w * F::NFN + i]][lane] += fl.accf[i];
-  __syncthreads();
-  // rows n of this wave: sum the partials, form the residual
   T rv[NPW];
+  if constexpr (!ZERO) {
 #pragma unroll
-  for (int k = 0; k < NPW; ++k) {
-    const int n = w * NPW + k;
-    if (n < NL) {
-      T s = 0;
+    for (int n = 0; n < NL; ++n) red[w][n][lane] = acc[n];
+    // the face's own nodes (a wave-uniform list): read-modify-write of this lane's column
 #pragma unroll
-      for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
-      if constexpr (MODE == 0) rv[k] = -s;
-      else if constexpr (MODE == 4) rv[k] = s;
-      else rv[k] = valid ? bv[k] - s : T(0);
+    for (int i = 0; i < F::NFN; ++i) red[w][fnodes[w * F::NFN + i]][lane] += fl.accf[i];
+    __syncthreads();
+    // rows n of this wave: sum the partials, form the residual
+#pragma unroll
+    for (int k = 0; k < NPW; ++k) {
+      const int n = w * NPW + k;
+      if (n < NL) {
+        T s = 0;
+#pragma unroll
+        for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
+        if constexpr (MODE == 0) rv[k] = -s;
+        else if constexpr (MODE == 4) rv[k] = s;
+        else rv[k] = valid ? bv[k] - s : T(0);
+      }
     }
   }
   if constexpr (MODE <= 1 || MODE == 4) {
@@ -222,12 +232,14 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   } else {
     // smoothing: every wave needs the whole residual row; exchange it through red[0]
     // (each wave only read red[*][its rows], so overwriting red[0][its rows] is safe)
+    if constexpr (!ZERO) {
 #pragma unroll
-    for (int k = 0; k < NPW; ++k) {
-      const int n = w * NPW + k;
-      if (n < NL) red[0][n][lane] = rv[k];
+      for (int k = 0; k < NPW; ++k) {
+        const int n = w * NPW + k;
+        if (n < NL) red[0][n][lane] = rv[k];
+      }
+      __syncthreads();
     }
-    __syncthreads();
     if (valid) {
 #pragma unroll
       for (int k = 0; k < NPW; ++k) {
@@ -235,9 +247,9 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
         if (n < NL) {
           T srow[NL];
           load_smoother_row<T, NL>(S, n, e, nele, srow);
-          T s = ZERO ? T(0) : (OFFD ? (T(1) - omega) * xv[n] : xv[n]);
+          T s = ZERO ? T(0) : (OFFD ? (T(1) - omega) * xo[k] : xo[k]);
 #pragma unroll
-          for (int m = 0; m < NL; ++m) s += srow[m] * red[0][m][lane];
+          for (int m = 0; m < NL; ++m) s += srow[m] * (ZERO ? bfull[m] : red[0][m][lane]);
           x[(size_t)e * NL + n] = s;
           if constexpr (MODE == 3) r[(size_t)e * NL + n] = rv[k];
           if constexpr (EMIT) red[1][n][lane] = s;  // red[1] is not read after the rows

@@ -222,8 +222,8 @@ __device__ __forceinline__ void mf_lift(Lift<TIN, D>& lf, const MfLds<TIN, D>& L
 // Does the work-group touch an element of an unfinished solve?  act holds one flag per 64
 // elements; a work-group covers 16*WPB elements.
 template <int WPB>
-__device__ __forceinline__ bool mf_block_active(const int32_t* __restrict__ act, int nele) {
-  const int e0 = blockIdx.x * 16 * WPB;
+__device__ __forceinline__ bool mf_block_active(const int32_t* __restrict__ act, int nele, int grp) {
+  const int e0 = grp * 16 * WPB;
   const int e1 = min(e0 + 16 * WPB, nele) - 1;
   bool on = false;
   for (int b = e0 >> 6; b <= (e1 >> 6); ++b) on |= act[b] != 0;
@@ -242,9 +242,10 @@ __global__ __launch_bounds__(64 * WPB) void k_trace_mf(int nele, const float* __
   using M = Mq<D>;
   using S = MfShape<D>;
   __shared__ MfLds<TIN, D> L;
-  if (act && !mf_block_active<WPB>(act, nele)) return;  // before any barrier
+  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  if (act && !mf_block_active<WPB>(act, nele, grp)) return;  // before any barrier
   const int lane = threadIdx.x & 63, q = lane >> 4;
-  const int wave = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const int wave = grp * WPB + (threadIdx.x >> 6);
   const int el = wave * 16 + (lane & 15);
   const bool valid = el < nele;
   const int ec = valid ? el : nele - 1;
@@ -291,10 +292,11 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
   // zero volume tiles for the pairing
   constexpr bool SKIP_VOL = OFFD && (std::is_same<TIN, float>::value || S::NTV % 2 == 0);
   __shared__ MfLds<TIN, D> L;
-  if (act && !mf_block_active<WPB>(act, nele)) return;  // before any barrier
+  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  if (act && !mf_block_active<WPB>(act, nele, grp)) return;  // before any barrier
   __shared__ float rx[(MODE == 2 || MODE == 3 || OFFD) ? WPB : 1][16][NL + 1];
   const int lane = threadIdx.x & 63, q = lane >> 4, wl = threadIdx.x >> 6;
-  const int wave = blockIdx.x * WPB + wl;
+  const int wave = grp * WPB + wl;
   const int el = wave * 16 + (lane & 15);
   const bool valid = el < nele;
   const int ec = valid ? el : nele - 1;

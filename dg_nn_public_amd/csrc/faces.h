@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "rowvec.h"
+
 namespace dgmi {
 
 // Layout of the face-interpolation block (DevGeom::off_fc), in the table's precision:
@@ -21,17 +23,6 @@ namespace dgmi {
 //   LK  [g][NK]         quadratic face Lagrange basis at the face points
 //   GK  [f][k][D][NL]   d phi / d xi of every basis function at the face's quadratic nodes
 // and the int32 node lists fnodes[f][NFN].
-// Work-groups are dispatched to the 8 XCDs round-robin by id (group b runs on XCD b % 8),
-// and each XCD has its own L2.  Face kernels read their neighbours' traces, and neighbours
-// are close in the element (SFC) order, so each XCD is given one contiguous range of
-// element groups: group b takes logical group xcd_start(b % 8) + b / 8.  A bijection on
-// [0, n) for any n.
-__device__ __forceinline__ int xcd_group(int b, int n) {
-  constexpr int NX = 8;
-  const int x = b % NX, q = n / NX, r = n % NX;
-  return x * q + (x < r ? x : r) + b / NX;
-}
-
 template <int D> struct FaceDims {
   static constexpr int NL = D == 3 ? 20 : 10, SG = D == 3 ? 12 : 4, NF = D + 1;
   static constexpr int NFN = D == 3 ? 10 : 4, NK = D == 3 ? 6 : 3;

@@ -826,53 +826,163 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   return 0;
 }
 
-// In-place Gauss-Jordan inversion of a dense n x n matrix on the device, FP64, without
-// pivoting: the SFC level operators are Galerkin projections of a symmetric positive
-// definite operator, so every pivot is positive (a non-positive or non-finite one sets
-// *fail).  Step k: f = column k; row k /= a_kk; a_ij -= f_i a_kj (i, j != k);
-// column k = -f_i / a_kk; a_kk = 1 / a_kk.
-__global__ void k_gj_col(int n, const double* __restrict__ a, int k, double* __restrict__ f) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) f[i] = a[(size_t)i * n + k];
-}
-__global__ void k_gj_row(int n, double* __restrict__ a, int k, const double* __restrict__ f) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n && j != k) a[(size_t)k * n + j] /= f[k];
-}
-__global__ void k_gj_rest(int n, double* __restrict__ a, int k, const double* __restrict__ f) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-  if (j < n && i != k && j != k) a[(size_t)i * n + j] -= f[i] * a[(size_t)k * n + j];
-}
-__global__ void k_gj_fix(int n, double* __restrict__ a, int k, const double* __restrict__ f,
-                         int* __restrict__ fail) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double p = f[k];
-  if (i == k) {
-    if (!(p > 0.0) || !isfinite(p)) *fail = 1;
-    a[(size_t)k * n + k] = 1.0 / p;
-  } else {
-    a[(size_t)i * n + k] = -f[i] / p;
+// In-place blocked Gauss-Jordan inversion of a dense N x N matrix (row-major, FP64, N a
+// multiple of GJB) on the device, without pivoting: the SFC level operators are Galerkin
+// projections of a symmetric positive definite operator, so every pivot block is positive
+// definite (a non-positive or non-finite pivot sets *fail).  Block column K = [k0, k0+GJB),
+// R = the rest:  P = A_KK^-1;  A_KR <- P A_KR;  A_RR <- A_RR - A_RK A_KR;  A_RK <- -A_RK P;
+// A_KK <- P  (the block form of the scalar steps a_kk <- 1/a_kk, row k /= a_kk,
+// a_ij -= a_ik a_kj, column k *= -1/a_kk).
+constexpr int GJB = 64;
+
+// P = A_KK^-1 by unblocked Gauss-Jordan in LDS; written to A_KK and to p (GJB x GJB)
+__global__ __launch_bounds__(256) void k_bgj_pivot(int N, double* __restrict__ A, int k0,
+                                                   double* __restrict__ p, int* __restrict__ fail) {
+  __shared__ double a[GJB][GJB + 1];
+  __shared__ double fcol[GJB], frow[GJB];
+  const int t = threadIdx.x;
+  for (int q = t; q < GJB * GJB; q += 256) a[q / GJB][q % GJB] = A[(size_t)(k0 + q / GJB) * N + k0 + q % GJB];
+  __syncthreads();
+  for (int k = 0; k < GJB; ++k) {
+    // column and row k before the step, so the step reads nothing it writes
+    if (t < GJB) fcol[t] = a[t][k];
+    else if (t < 2 * GJB) frow[t - GJB] = a[k][t - GJB];
+    __syncthreads();
+    const double piv = fcol[k];
+    if (t == 0 && (!(piv > 0.0) || !isfinite(piv))) *fail = 1;
+    const double inv = 1.0 / piv;
+    for (int q = t; q < GJB * GJB; q += 256) {
+      const int i = q / GJB, j = q % GJB;
+      if (i == k && j == k) a[i][j] = inv;
+      else if (i == k) a[i][j] = frow[j] * inv;
+      else if (j == k) a[i][j] = -fcol[i] * inv;
+      else a[i][j] -= fcol[i] * (frow[j] * inv);
+    }
+    __syncthreads();
+  }
+  for (int q = t; q < GJB * GJB; q += 256) {
+    const double v = a[q / GJB][q % GJB];
+    A[(size_t)(k0 + q / GJB) * N + k0 + q % GJB] = v;
+    p[q] = v;
   }
 }
-__global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (float)a[i];
+
+// A_KJ <- P A_KJ for the column tile J (one work-group per tile, the pivot tile skipped)
+__global__ __launch_bounds__(256) void k_bgj_row(int N, double* __restrict__ A, int k0,
+                                                 const double* __restrict__ p) {
+  const int j0 = blockIdx.x * GJB;
+  if (j0 == k0) return;
+  __shared__ double ps[GJB][GJB + 1], ys[GJB][GJB + 1];
+  const int t = threadIdx.x;
+  for (int q = t; q < GJB * GJB; q += 256) {
+    ps[q / GJB][q % GJB] = p[q];
+    ys[q / GJB][q % GJB] = A[(size_t)(k0 + q / GJB) * N + j0 + q % GJB];
+  }
+  __syncthreads();
+  const int tx = t % 16, ty = t / 16;
+  double c[4][4] = {};
+  for (int k = 0; k < GJB; ++k) {
+    double u[4], v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u[i] = ps[ty * 4 + i][k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = ys[k][tx * 4 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[i][j] += u[i] * v[j];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) A[(size_t)(k0 + ty * 4 + i) * N + j0 + tx * 4 + j] = c[i][j];
+}
+
+// A_IJ -= A_IK A_KJ for tiles I, J outside the pivot block (A_KJ already updated)
+__global__ __launch_bounds__(256) void k_bgj_rest(int N, double* __restrict__ A, int k0) {
+  const int i0 = blockIdx.y * GJB, j0 = blockIdx.x * GJB;
+  if (i0 == k0 || j0 == k0) return;
+  __shared__ double ls[GJB][GJB + 1], rs[GJB][GJB + 1];
+  const int t = threadIdx.x;
+  for (int q = t; q < GJB * GJB; q += 256) {
+    ls[q / GJB][q % GJB] = A[(size_t)(i0 + q / GJB) * N + k0 + q % GJB];
+    rs[q / GJB][q % GJB] = A[(size_t)(k0 + q / GJB) * N + j0 + q % GJB];
+  }
+  __syncthreads();
+  const int tx = t % 16, ty = t / 16;
+  double c[4][4] = {};
+  for (int k = 0; k < GJB; ++k) {
+    double u[4], v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u[i] = ls[ty * 4 + i][k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = rs[k][tx * 4 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[i][j] += u[i] * v[j];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) A[(size_t)(i0 + ty * 4 + i) * N + j0 + tx * 4 + j] -= c[i][j];
+}
+
+// A_IK <- -A_IK P for the row tile I (the pivot tile skipped)
+__global__ __launch_bounds__(256) void k_bgj_col(int N, double* __restrict__ A, int k0,
+                                                 const double* __restrict__ p) {
+  const int i0 = blockIdx.x * GJB;
+  if (i0 == k0) return;
+  __shared__ double ls[GJB][GJB + 1], ps[GJB][GJB + 1];
+  const int t = threadIdx.x;
+  for (int q = t; q < GJB * GJB; q += 256) {
+    ps[q / GJB][q % GJB] = p[q];
+    ls[q / GJB][q % GJB] = A[(size_t)(i0 + q / GJB) * N + k0 + q % GJB];
+  }
+  __syncthreads();
+  const int tx = t % 16, ty = t / 16;
+  double c[4][4] = {};
+  for (int k = 0; k < GJB; ++k) {
+    double u[4], v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u[i] = ls[ty * 4 + i][k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = ps[k][tx * 4 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[i][j] += u[i] * v[j];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) A[(size_t)(i0 + ty * 4 + i) * N + k0 + tx * 4 + j] = -c[i][j];
+}
+
+// out (n x n, FP32 row-major) = the leading n x n block of a (N x N, FP64)
+__global__ void k_d2f_block(int n, int N, const double* __restrict__ a, float* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)n * n) return;
+  const int i = (int)(q / n), j = (int)(q - (int64_t)i * n);
+  out[q] = (float)a[(size_t)i * N + j];
 }
 
 // FP32 row-major inverse of the dense level-l operator of one component, formed on the
-// device (synchronous); nullptr on failure.
+// device (synchronous); nullptr on failure.  The matrix is padded to a multiple of GJB
+// with the identity, which leaves the leading block's inverse unchanged.
 static float* dense_inverse_device(dgmi_ctx* c, const SfcLevelsHost& h, int l) {
   const int n = (int)h.n[l];
-  std::vector<double> a((size_t)n * n, 0.0);
+  const int N = (n + GJB - 1) / GJB * GJB, nb = N / GJB;
+  std::vector<double> a((size_t)N * N, 0.0);
   for (int i = 0; i < n; ++i)
     for (int32_t q = h.rowptr[l][i]; q < h.rowptr[l][i + 1]; ++q)
-      a[(size_t)i * n + h.col[l][q]] += h.val[l][q];
-  double *da = nullptr, *df = nullptr;
+      a[(size_t)i * N + h.col[l][q]] += h.val[l][q];
+  for (int i = n; i < N; ++i) a[(size_t)i * N + i] = 1.0;
+  double *da = nullptr, *dp = nullptr;
   int* dfail = nullptr;
   float* out = static_cast<float*>(c->alloc(sizeof(float) * (size_t)n * n));
-  bool ok = out && hipMalloc(&da, sizeof(double) * (size_t)n * n) == hipSuccess &&
-            hipMalloc(&df, sizeof(double) * n) == hipSuccess &&
+  bool ok = out && hipMalloc(&da, sizeof(double) * (size_t)N * N) == hipSuccess &&
+            hipMalloc(&dp, sizeof(double) * GJB * GJB) == hipSuccess &&
             hipMalloc(&dfail, sizeof(int)) == hipSuccess;
   int hfail = 0;
   if (ok) {
@@ -880,21 +990,22 @@ static float* dense_inverse_device(dgmi_ctx* c, const SfcLevelsHost& h, int l) {
          hipMemcpy(dfail, &hfail, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   }
   if (ok) {
-    const dim3 blk(256), g1((n + 255) / 256), g2((n + 255) / 256, n);
-    for (int k = 0; k < n; ++k) {
-      hipLaunchKernelGGL(k_gj_col, g1, blk, 0, 0, n, (const double*)da, k, df);
-      hipLaunchKernelGGL(k_gj_row, g1, blk, 0, 0, n, da, k, (const double*)df);
-      hipLaunchKernelGGL(k_gj_rest, g2, blk, 0, 0, n, da, k, (const double*)df);
-      hipLaunchKernelGGL(k_gj_fix, g1, blk, 0, 0, n, da, k, (const double*)df, dfail);
+    const dim3 blk(256);
+    for (int kb = 0; kb < nb; ++kb) {
+      const int k0 = kb * GJB;
+      hipLaunchKernelGGL(k_bgj_pivot, dim3(1), blk, 0, 0, N, da, k0, dp, dfail);
+      hipLaunchKernelGGL(k_bgj_row, dim3(nb), blk, 0, 0, N, da, k0, (const double*)dp);
+      hipLaunchKernelGGL(k_bgj_rest, dim3(nb, nb), blk, 0, 0, N, da, k0);
+      hipLaunchKernelGGL(k_bgj_col, dim3(nb), blk, 0, 0, N, da, k0, (const double*)dp);
     }
     const int64_t nn = (int64_t)n * n;
-    hipLaunchKernelGGL(k_d2f, dim3((unsigned)((nn + 255) / 256)), blk, 0, 0, nn,
+    hipLaunchKernelGGL(k_d2f_block, dim3((unsigned)((nn + 255) / 256)), blk, 0, 0, n, N,
                        (const double*)da, out);
     ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
          hipMemcpy(&hfail, dfail, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess && !hfail;
   }
   if (da) (void)hipFree(da);
-  if (df) (void)hipFree(df);
+  if (dp) (void)hipFree(dp);
   if (dfail) (void)hipFree(dfail);
   return ok ? out : nullptr;
 }

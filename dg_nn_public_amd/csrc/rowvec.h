@@ -8,6 +8,17 @@
 
 namespace dgmi {
 
+// Work-groups are dispatched to the 8 XCDs round-robin by id (group b runs on XCD b % 8),
+// and each XCD has its own L2.  The element kernels read their neighbours' face traces,
+// and neighbours are close in the element (SFC) order, so each XCD is given one
+// contiguous range of element groups: group b takes logical group
+// (first group of XCD b % 8) + b / 8.  A bijection on [0, n) for any n.
+__device__ __forceinline__ int xcd_group(int b, int n) {
+  constexpr int NX = 8;
+  const int x = b % NX, q = n / NX, r = n % NX;
+  return x * q + (x < r ? x : r) + b / NX;
+}
+
 // Vector loads/stores of one element's row: 16-byte accesses when the row length
 // allows (3D P3 in both precisions, 2D in FP64), 8-byte otherwise (2D P3 in FP32).
 // DOF vectors are required to be 16-byte aligned.
