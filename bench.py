@@ -39,12 +39,14 @@ PEAK_HBM = 8.0e12     # B/s
 # calibrated HBM bytes of one FP32 vector smoothing sweep (k_residual_w<float,3,5,true>) on
 # the 24^3x6 workload: FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3 --pmc passes
 # (profiles/pmc_bytes_apply_tet24.md)
-PMC_SWEEP = 259.3e6
+PMC_SWEEP = 214.5e6
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):
-    """Algorithmic flops of one operator apply per element (volume GEMMs + metric,
-    face traces, lifts and pointwise flux)."""
+    """Flops of one operator apply per element in the reference's point-by-point form
+    (volume GEMMs + metric, face traces, lifts and pointwise flux).  The vector kernels
+    evaluate the face terms through a quadratic interpolant and perform fewer, so TFLOP/s
+    derived from this count are rates of the reference's formulation."""
     f_vol = 4 * D * ngi * nloc + ngi * (2 * D * D + 1)
     f_face = nface * sngi * (4 * (1 + D) * nloc + 4 * D * D + 10)
     return f_vol + f_face
@@ -67,9 +69,13 @@ def sweep_bytes_per_elem(D, nloc, nface, sngi):
 
 
 def sweep_flops_per_elem(D, nloc, nface, sngi):
-    """Flops of one such sweep per element: neighbour-term lift and trace output (value
-    and D derivatives at every face point), the smoother row products."""
-    return 2 * 2 * nface * sngi * (1 + D) * nloc + 2 * nloc * nloc
+    """Flops of one such sweep per element as the kernel performs them: per face the
+    neighbour-term lift and the trace output through the quadratic face interpolant
+    (nfn face nodes, nk quadratic nodes), then the smoother row products."""
+    nfn, nk = (10, 6) if D == 3 else (4, 3)
+    trace = nk * (D * nloc + D) + sngi * (nfn + nk)
+    lift = sngi * (nfn + nk) + nk * D + nloc * nk * D
+    return 2 * nface * (trace + lift) + 2 * nloc * nloc
 
 
 def main():
@@ -290,7 +296,7 @@ def main():
     ach32 = F * nele / t32
     # calibrated HBM bytes per launch of this workload (FETCH_SIZE x 2 + WRITE_SIZE, two
     # rocprofv3 --pmc passes, profiles/pmc_bytes_apply_tet24.md)
-    pmc = {"f64": 323e6, "f32_vector": 166e6, "f32_mfma": 125e6, "f32_sweep": PMC_SWEEP}
+    pmc = {"f64": 240e6, "f32_vector": 113e6, "f32_mfma": 125e6, "f32_sweep": PMC_SWEEP}
     on_profiled = args.dim == 3 and args.n == 24
     SB = sweep_bytes_per_elem(args.dim, nloc, g.nface, g.sngi)
     SF = sweep_flops_per_elem(args.dim, nloc, g.nface, g.sngi)
