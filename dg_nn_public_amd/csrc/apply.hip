@@ -69,15 +69,19 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
 // EMIT (mode 5 only): also write the face traces of the new x into tout (another buffer
 // than `trace`, which other work-groups are still reading), so the next sweep needs no
 // trace pass.
-template <typename T, int D, int MODE, bool EMIT = false>
+// TR: storage of the face traces read (Vec2<T>, or TraceBf16 in the preconditioner's 16-bit
+// variant), TO: of the traces written; SS: storage of the smoother (T, or uint16_t =
+// bfloat16 packed symmetric).  Arithmetic is in T throughout.
+template <typename T, int D, int MODE, bool EMIT = false, typename TR = typename Vec2<T>::type,
+          typename SS = T, typename TO = TR>
 __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     int nele, T* __restrict__ x, const T* b, T* r, const T* __restrict__ vtab,
     const T* __restrict__ wgt, const T* __restrict__ sw,
     const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
     const T* __restrict__ wf, const T* __restrict__ pen, const int32_t* __restrict__ nbface,
-    const int32_t* __restrict__ fcode, const typename Vec2<T>::type* __restrict__ trace,
-    const T* __restrict__ S, const int32_t* __restrict__ act, T omega,
-    typename Vec2<T>::type* __restrict__ tout, const T* __restrict__ fc,
+    const int32_t* __restrict__ fcode, const TR* __restrict__ trace,
+    const SS* __restrict__ S, const int32_t* __restrict__ act, T omega,
+    TO* __restrict__ tout, const T* __restrict__ fc,
     const int32_t* __restrict__ fnodes) {
   using M = Dm<D>;
   using F = FaceDims<D>;
@@ -179,18 +183,18 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
         T qsn, qdn;
         if constexpr (OFFD) {
           const int ga = aln[a * M::SG + g];
-          const typename Vec2<T>::type tn = trace[(nbf * M::SG + ga) * nele + nbe];
+          const auto tn = trace_value(trace[(nbf * M::SG + ga) * nele + nbe]);
           qsn = ws * (T(0.5) * tn.y - pe * tn.x);
           qdn = ws * (T(0.5) * tn.x);
         } else if (type == 0) {
-          const typename Vec2<T>::type tr = trace[(f * M::SG + g) * nele + e];
+          const auto tr = trace_value(trace[(f * M::SG + g) * nele + e]);
           const int ga = aln[a * M::SG + g];
-          const typename Vec2<T>::type tn = trace[(nbf * M::SG + ga) * nele + nbe];
+          const auto tn = trace_value(trace[(nbf * M::SG + ga) * nele + nbe]);
           const T jump = tr.x - tn.x;
           qsn = ws * (T(-0.5) * (tr.y - tn.y) + pe * jump);
           qdn = ws * (T(-0.5) * jump);
         } else {
-          const typename Vec2<T>::type tr = trace[(f * M::SG + g) * nele + e];
+          const auto tr = trace_value(trace[(f * M::SG + g) * nele + e]);
           qsn = ws * (pe * tr.x - tr.y);
           qdn = -ws * tr.x;
         }
@@ -246,7 +250,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
         const int n = w * NPW + k;
         if (n < NL) {
           T srow[NL];
-          load_smoother_row<T, NL>(S, n, e, nele, srow);
+          load_smoother_row<T, NL, SS>(S, n, e, nele, srow);
           T s = ZERO ? T(0) : (OFFD ? (T(1) - omega) * xo[k] : xo[k]);
 #pragma unroll
           for (int m = 0; m < NL; ++m) s += srow[m] * (ZERO ? bfull[m] : red[0][m][lane]);
@@ -269,10 +273,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
 #pragma unroll
         for (int i = 0; i < F::NFN; ++i) xf[i] = red[1][fnodes[f * F::NFN + i]][lane];
         face_traces<T, D>(f, fc, xn, xf, cc, [&](int g, T u, T dn) {
-          typename Vec2<T>::type tv;
-          tv.x = u;
-          tv.y = dn;
-          tout[(f * M::SG + g) * nele + e] = tv;
+          tout[(f * M::SG + g) * nele + e] = make_trace<TO>(u, dn);
         });
       }
     }
@@ -517,7 +518,8 @@ __global__ __launch_bounds__(256) void k_build_S_direct(int nele, const double* 
 // FP32 packed copy of the FP64 smoother: S32[(a,b)] = (S[a][b] + S[b][a]) / 2, a <= b
 template <int NL>
 __global__ __launch_bounds__(256) void k_pack_S_sym(int nele, const double* __restrict__ S64,
-                                                    float* __restrict__ S32) {
+                                                    float* __restrict__ S32,
+                                                    uint16_t* __restrict__ S16) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nele) return;
   for (int a = 0; a < NL; ++a)
@@ -525,6 +527,7 @@ __global__ __launch_bounds__(256) void k_pack_S_sym(int nele, const double* __re
       const double v = 0.5 * (S64[s_index<double, NL>(a, b, e, nele)] +
                               S64[s_index<double, NL>(b, a, e, nele)]);
       S32[(size_t)sym_index<NL>(a, b) * nele + e] = (float)v;
+      if (S16) S16[(size_t)sym_index<NL>(a, b) * nele + e] = f32_to_bf16((float)v);
     }
 }
 
@@ -546,9 +549,9 @@ int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s) {
   const int nele = (int)c->nele;
   const dim3 blk(256), grd((nele + 255) / 256);
   if (c->D == 3)
-    hipLaunchKernelGGL(k_pack_S_sym<20>, grd, blk, 0, s, nele, c->g64.S, c->g32.S);
+    hipLaunchKernelGGL(k_pack_S_sym<20>, grd, blk, 0, s, nele, c->g64.S, c->g32.S, c->g32.S16);
   else
-    hipLaunchKernelGGL(k_pack_S_sym<10>, grd, blk, 0, s, nele, c->g64.S, c->g32.S);
+    hipLaunchKernelGGL(k_pack_S_sym<10>, grd, blk, 0, s, nele, c->g64.S, c->g32.S, c->g32.S16);
   DGMI_HIP(hipGetLastError());
   return 0;
 }
@@ -674,6 +677,77 @@ int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int m
   float* xm = const_cast<float*>(x);
   if (c->D == 3) return apply_impl<float, 3>(c, c->g32, xm, b, r, mode, s, tin, -1);
   return apply_impl<float, 2>(c, c->g32, xm, b, r, mode, s, tin, -1);
+}
+
+// The preconditioner's 16-bit variant: the neighbour-form sweeps, the sweep from zero and
+// the residual read the smoother and the face traces in bfloat16 (arithmetic in FP32).
+// Mode 1: r = b - A x from the traces in buffer tin; 5 / 6: sweeps writing buffer tout;
+// 7: a sweep as 5 that writes FP32 traces into the FP32 buffer tout (for the FP32 apply
+// that follows the preconditioner).
+template <int D>
+static int apply_impl16(dgmi_ctx* c, float* x, const float* b, float* r, int mode, int tin,
+                        int tout, hipStream_t s) {
+  DevGeom<float>& g = c->g32;
+  const int nele = (int)c->nele;
+  const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
+  const int32_t* act = c->mask_on ? c->act64 : nullptr;
+  TraceBf16* bufs[2] = {reinterpret_cast<TraceBf16*>(g.tr16[0]),
+                        reinterpret_cast<TraceBf16*>(g.tr16[1])};
+  if (mode == 6) tin = tout == 0 ? 1 : 0;
+  const bool out32 = mode == 7;
+  typename Vec2<float>::type* bufs32[2] = {g.trace, g.trace2};
+  if (!(g.S16 && tin >= 0 && tin <= 1 && bufs[tin] &&
+        (mode == 1 ? tout < 0
+                   : (tout >= 0 && tout <= 1 && (out32 ? bufs32[tout] != nullptr
+                                                       : tout != tin))))) {
+    set_error("bad 16-bit trace buffers");
+    return -2;
+  }
+  const TraceBf16* trace = bufs[tin];
+  TraceBf16* to = tout >= 0 && !out32 ? bufs[tout] : nullptr;
+  const float* vt = g.tab + g.off_vtab;
+  const float* wg = g.tab + g.off_wgt;
+  const float* sw = g.tab + g.off_sw;
+#define DGMI_RES16(MODE, EMIT)                                                                \
+  hipLaunchKernelGGL((k_residual_w<float, D, MODE, EMIT, TraceBf16, uint16_t>), grd, blk, 0,   \
+                     s, nele, x, b, r, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface,  \
+                     c->fcode, trace, g.S16, act, (float)c->omega, to, g.tab + g.off_fc,      \
+                     c->fnodes)
+  if (mode == 1) DGMI_RES16(1, false);
+  else if (mode == 5) DGMI_RES16(5, true);
+  else if (mode == 6) DGMI_RES16(6, true);
+  else
+    hipLaunchKernelGGL((k_residual_w<float, D, 5, true, TraceBf16, uint16_t,
+                                     typename Vec2<float>::type>),
+                       grd, blk, 0, s, nele, x, b, r, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen,
+                       c->nbface, c->fcode, trace, g.S16, act, (float)c->omega, bufs32[tout],
+                       g.tab + g.off_fc, c->fnodes);
+#undef DGMI_RES16
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+bool trace_io16(const dgmi_ctx* c) {
+  return trace_io(c) && c->g32.S16 && c->g32.tr16[0] && c->g32.tr16[1];
+}
+int launch_smooth_io16(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s) {
+  if (c->D == 3) return apply_impl16<3>(c, x, b, nullptr, 5, tin, tout, s);
+  return apply_impl16<2>(c, x, b, nullptr, 5, tin, tout, s);
+}
+int launch_smooth_io16_out32(dgmi_ctx* c, float* x, const float* b, int tin, int tout,
+                             hipStream_t s) {
+  if (c->D == 3) return apply_impl16<3>(c, x, b, nullptr, 7, tin, tout, s);
+  return apply_impl16<2>(c, x, b, nullptr, 7, tin, tout, s);
+}
+int launch_smooth_zero_io16(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s) {
+  if (c->D == 3) return apply_impl16<3>(c, x, b, nullptr, 6, -1, tout, s);
+  return apply_impl16<2>(c, x, b, nullptr, 6, -1, tout, s);
+}
+int launch_apply_io16(dgmi_ctx* c, const float* x, const float* b, float* r, int tin,
+                      hipStream_t s) {
+  float* xm = const_cast<float*>(x);
+  if (c->D == 3) return apply_impl16<3>(c, xm, b, r, 1, tin, -1, s);
+  return apply_impl16<2>(c, xm, b, r, 1, tin, -1, s);
 }
 
 template <typename T>

@@ -231,6 +231,27 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
   T* r = vbuf<T>(c, 3);
   T* rc = cgbuf<T>(c, 0);
   T* ec = cgbuf<T>(c, 1);
+  if (tr_out) *tr_out = -1;
+  if constexpr (std::is_same<T, float>::value) {
+    if (c->pc_bf16 && trace_io16(c)) {
+      // bf16 inner cycles with the exact block smoother: the steps below with the smoother
+      // and the face traces read and written in bfloat16 (16-bit buffers 0 and 1
+      // alternate); x stays FP32, and the last sweep writes FP32 traces (FP32 buffer 1)
+      // for the caller's Krylov apply, which keeps the FP32 operator
+      DGMI_TRY(launch_smooth_zero_io16(c, x, b, 0, s));
+      DGMI_TRY(launch_smooth_io16(c, x, b, 0, 1, s));
+      DGMI_TRY(launch_smooth_io16(c, x, b, 1, 0, s));
+      DGMI_TRY(launch_apply_io16(c, x, b, r, 0, s));
+      DGMI_TRY(launch_restrict<T>(c, r, rc, s));
+      DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s));
+      DGMI_TRY(launch_prolong_trace(c, ec, x, 0, s, true));
+      DGMI_TRY(launch_smooth_io16(c, x, b, 0, 1, s));
+      DGMI_TRY(launch_smooth_io16(c, x, b, 1, 0, s));
+      DGMI_TRY(launch_smooth_io16_out32(c, x, b, 0, 1, s));
+      if (tr_out) *tr_out = 1;
+      return 0;
+    }
+  }
   // bf16-input applies inside the preconditioner only (the Krylov operator stays fp32)
   struct KernelScope {
     dgmi_ctx* c;
@@ -238,7 +259,6 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
     ~KernelScope() { c->f32_kernel = saved; }
   } scope{c, c->f32_kernel};
   if (std::is_same<T, float>::value && c->pc_bf16) c->f32_kernel = 2;
-  if (tr_out) *tr_out = -1;
   if constexpr (std::is_same<T, float>::value) {
     if (trace_io(c)) {
       // the same steps, each sweep (and the prolongation) writing the traces of its
@@ -288,10 +308,10 @@ struct Krylov {
   int32_t *want = nullptr, *active = nullptr, *kk = nullptr, *nact = nullptr;
   double* hd = nullptr;   // pinned: 2 * ncomp doubles
   int32_t* hi = nullptr;  // pinned: 2 * ncomp + 1 ints
-  // per-iteration progress: status = (active count, res/thr of each active component) on
-  // the device, copied into ring slot j of pinned memory at the end of iteration j
-  double* status = nullptr;  // (1 + ncomp)
+  // per-iteration progress (active count, res/thr of each active component), written by
+  // the Givens kernel of iteration j straight into ring slot j of pinned memory
   double* ring = nullptr;    // pinned: (m + 1) * (1 + ncomp)
+  double* ring_dev = nullptr;  // the ring's device address
   int nring = 0;
   std::vector<hipEvent_t> ev;  // (m + 1) completion events of the ring slots
 };
@@ -314,10 +334,9 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     k->active = iv(nc);
     k->kk = iv(nc);
     k->nact = iv(1);
-    k->status = dv(nc + 1);
     if (hipHostMalloc((void**)&k->hd, sizeof(double) * 2 * nc) != hipSuccess) k->hd = nullptr;
     if (hipHostMalloc((void**)&k->hi, sizeof(int32_t) * (2 * nc + 1)) != hipSuccess) k->hi = nullptr;
-    if (!k->beta || !k->nact || !k->status || !k->hd || !k->hi) return nullptr;
+    if (!k->beta || !k->nact || !k->hd || !k->hi) return nullptr;
   }
   const int64_t ldv = (c->ndof + 255) / 256 * 256 + 64;
   k->ldv = ldv;
@@ -343,6 +362,7 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     k->ring = nullptr;
     if (hipHostMalloc((void**)&k->ring, sizeof(double) * (m + 1) * (nc + 1)) != hipSuccess)
       return nullptr;
+    if (hipHostGetDevicePointer((void**)&k->ring_dev, k->ring, 0) != hipSuccess) return nullptr;
     while ((int)k->ev.size() < m + 1) {
       hipEvent_t e;
       if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
@@ -431,15 +451,14 @@ static int gmres_cycle(dgmi_ctx* c, Krylov* k, const T* r0, T* d, int m, bool fl
     }
     DGMI_TRY((kry::sscale<T, T>(sg, k->hn, 0, k->active, w, Vj(j + 1), s)));
     DGMI_TRY(kry::sgivens(nc, j, k->ldh, k->hs, k->H, k->hn, k->cs, k->sn, k->rs, k->g, k->gs,
-                          k->res, k->thr, k->active, k->kk, k->nact, k->status, s));
+                          k->res, k->thr, k->active, k->kk, k->nact,
+                          k->ring_dev + (size_t)j * (nc + 1), s));
     if (c->mask_on) DGMI_TRY(kry::expand_active(sg, c->nele, k->active, c->act64, s));
     // Progress is read back late: the host queues iteration j+1 while the device runs
     // iteration j and waits for iteration j itself only when some component could have
     // reached its threshold in it.  An iteration queued after every component stopped
     // changes nothing that is used (inactive components keep H, g, kk; kernels skip them
     // or write slots beyond kk).
-    DGMI_HIP(hipMemcpyAsync(k->ring + (size_t)j * (nc + 1), k->status, sizeof(double) * (nc + 1),
-                            hipMemcpyDeviceToHost, s));
     DGMI_HIP(hipEventRecord(k->ev[j], s));
     while (known < j) {
       const hipError_t q = hipEventQuery(k->ev[known + 1]);
@@ -594,7 +613,8 @@ static int build_seg(dgmi_ctx* c) {
   sg.nblk = (int)blk.size();
   sg.blk = static_cast<SegBlk*>(c->alloc(sizeof(SegBlk) * std::max<size_t>(blk.size(), 1)));
   sg.comp_blk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * cb.size()));
-  sg.partial = static_cast<double*>(c->alloc(sizeof(double) * MDOT_K * std::max(sg.nblk, 1)));
+  sg.partial = static_cast<double*>(
+      c->alloc(sizeof(double) * MDOT_K * std::max<size_t>({(size_t)sg.nblk, wb.size(), (size_t)1})));
   sg.coef = static_cast<double*>(c->alloc(sizeof(double) * MDOT_K * c->ncomp));
   sg.eoff = static_cast<int64_t*>(c->alloc(sizeof(int64_t) * (c->ncomp + 1)));
   c->act64 = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * ((c->nele + 63) / 64)));
@@ -898,6 +918,15 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
   if (!c->g64.S) c->g64.S = static_cast<double*>(c->alloc(sizeof(double) * nS));
   if (!c->g32.S) c->g32.S = static_cast<float*>(c->alloc(sizeof(float) * nS32));
   DGMI_CHECK(c->g64.S && c->g32.S, alloc_failure());
+  if (variant == DGMI_SMOOTH_DIRECT) {
+    // the preconditioner's 16-bit variant (bf16 inner cycles): smoother and face traces in
+    // bfloat16, used by the neighbour-form sweeps
+    const int64_t ntr = c->nele * c->nface * c->sngi;
+    if (!c->g32.S16) c->g32.S16 = static_cast<uint16_t*>(c->alloc(sizeof(uint16_t) * nS32));
+    for (int k = 0; k < 2; ++k)
+      if (!c->g32.tr16[k]) c->g32.tr16[k] = static_cast<uint32_t*>(c->alloc(sizeof(uint32_t) * ntr));
+    DGMI_CHECK(c->g32.S16 && c->g32.tr16[0] && c->g32.tr16[1], alloc_failure());
+  }
   Scratch& sc = scratch(c);
   if (variant == DGMI_SMOOTH_JACOBI3) {
     DGMI_TRY(launch_build_S_jacobi<double>(c, sc.bdiag, omega, c->g64.S, s));

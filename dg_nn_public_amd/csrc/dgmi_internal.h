@@ -48,6 +48,10 @@ template <typename T> struct DevGeom {
   // matrix-core kernels
   typename Vec2<T>::type* trace = nullptr;
   typename Vec2<T>::type* trace2 = nullptr;  // second buffer (FP32 vector sweeps)
+  // the preconditioner's 16-bit variant (FP32 only, exact block smoother): the packed
+  // symmetric smoother in bfloat16 and two trace buffers of (u, du/dn) bfloat16 pairs
+  uint16_t* S16 = nullptr;
+  uint32_t* tr16[2] = {nullptr, nullptr};
 };
 
 // SFC levels on the device.  "Grid" levels 0..small_level hold every component whose
@@ -290,7 +294,19 @@ int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, h
 int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s);
 int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
                     hipStream_t s);
-// g32.S = packed symmetric FP32 copy of g64.S
+// The same three with the smoother and the traces stored in bfloat16 (g32.S16, g32.tr16);
+// trace_io16() tells whether the context has them
+bool trace_io16(const dgmi_ctx* c);
+int launch_smooth_io16(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s);
+int launch_smooth_zero_io16(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s);
+// a 16-bit sweep that writes FP32 traces into the FP32 buffer tout (the last sweep before
+// an FP32 apply)
+int launch_smooth_io16_out32(dgmi_ctx* c, float* x, const float* b, int tin, int tout,
+                             hipStream_t s);
+int launch_apply_io16(dgmi_ctx* c, const float* x, const float* b, float* r, int tin,
+                      hipStream_t s);
+// g32.S = packed symmetric FP32 copy of g64.S (and g32.S16 its bfloat16 rounding, when
+// allocated)
 int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s);
 template <typename T>
 int launch_convert(const double* in, T* out, int64_t n, hipStream_t s);
@@ -300,7 +316,9 @@ int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s);
 template <typename T>
 int launch_prolong_add(dgmi_ctx* c, const T* e_sfc, T* x_dg, hipStream_t s);
 // FP32 vector family: x += P e with the traces of the new x written into buffer tout
-int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s);
+// (bf16: into the 16-bit variant's buffer tout)
+int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s,
+                         bool bf16 = false);
 template <typename T>
 int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s);
 // single-work-group part of the V-cycle: LDS bytes of levels l0.. of one component, the

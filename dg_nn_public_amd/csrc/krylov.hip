@@ -178,9 +178,11 @@ __device__ __forceinline__ void wave_reduce_cols(double (&a)[KC]) {
     const bool up = (lane & o) != 0;  // this lane keeps the upper half of the columns
 #pragma unroll
     for (int i = 0; i < S / 2; ++i) {
-      const double send = up ? a[i] : a[i + S / 2];
-      const double keep = up ? a[i + S / 2] : a[i];
-      a[i] = keep + __shfl_xor(send, o, 64);
+      // both halves cross, and the lane picks a sum: a select between two array elements
+      // would be folded into a lane-dependent index, which puts the array in scratch
+      const double lo = a[i], hi = a[i + S / 2];
+      const double rlo = __shfl_xor(lo, o, 64), rhi = __shfl_xor(hi, o, 64);
+      a[i] = up ? hi + rhi : lo + rlo;
     }
   }
 #pragma unroll

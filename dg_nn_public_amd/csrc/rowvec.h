@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 #include <type_traits>
 
 namespace dgmi {
@@ -104,11 +105,61 @@ __device__ __forceinline__ void load_S_row_sym(const float* __restrict__ S, int 
   }
 }
 
+// bfloat16 storage (the preconditioner's 16-bit variant): round to nearest even on the way
+// in, exact widening on the way out.  Inputs are finite.
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint32_t h) {
+  const uint32_t u = h << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// face trace (u, du/dn) in one 32-bit word: u in the low half, du/dn in the high half
+struct TraceBf16 {
+  uint32_t v;
+};
+__device__ __forceinline__ float2 trace_value(const float2& t) { return t; }
+__device__ __forceinline__ double2 trace_value(const double2& t) { return t; }
+__device__ __forceinline__ float2 trace_value(const TraceBf16& t) {
+  return make_float2(bf16_to_f32(t.v & 0xffffu), bf16_to_f32(t.v >> 16));
+}
+template <typename TR, typename T>
+__device__ __forceinline__ TR make_trace(T u, T dn) {
+  if constexpr (std::is_same<TR, TraceBf16>::value) {
+    TraceBf16 t;
+    t.v = (uint32_t)f32_to_bf16((float)u) | ((uint32_t)f32_to_bf16((float)dn) << 16);
+    return t;
+  } else {
+    TR t;
+    t.x = u;
+    t.y = dn;
+    return t;
+  }
+}
+
+// row n of a packed symmetric smoother stored in bfloat16
+template <int NL>
+__device__ __forceinline__ void load_S_row_sym16(const uint16_t* __restrict__ S, int n, int e,
+                                                 int nele, float* v) {
+#pragma unroll
+  for (int m = 0; m < NL; ++m) {
+    const int a = m < n ? m : n, b = m < n ? n : m;
+    v[m] = bf16_to_f32(S[(size_t)sym_index<NL>(a, b) * nele + e]);
+  }
+}
+
 // row n of the smoother of element e in the layout of its precision (FP64 full, FP32 packed)
-template <typename T, int NL>
-__device__ __forceinline__ void load_smoother_row(const T* __restrict__ S, int n, int e, int nele,
+template <typename T, int NL, typename SS = T>
+__device__ __forceinline__ void load_smoother_row(const SS* __restrict__ S, int n, int e, int nele,
                                                   T* v) {
-  if constexpr (sizeof(T) == 4) load_S_row_sym<NL>(S, n, e, nele, v);
+  if constexpr (std::is_same<SS, uint16_t>::value) load_S_row_sym16<NL>(S, n, e, nele, v);
+  else if constexpr (sizeof(T) == 4) load_S_row_sym<NL>(S, n, e, nele, v);
   else load_S_row<T, NL>(S, n, e, nele, v);
 }
 
