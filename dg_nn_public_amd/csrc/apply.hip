@@ -44,7 +44,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
                                                           const int32_t* __restrict__ act) {
   using M = Dm<D>;
   using F = FaceDims<D>;
-  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  const int grp = act ? (int)blockIdx.x : xcd_group(blockIdx.x, gridDim.x);
   if (act && !act[grp]) return;  // 64 elements of finished solves
   const int lane = threadIdx.x & 63;
   const int f = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   using M = Dm<D>;
   using F = FaceDims<D>;
   static_assert(!EMIT || MODE == 5 || MODE == 6, "traces are emitted by the sweeps");
-  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  const int grp = act ? (int)blockIdx.x : xcd_group(blockIdx.x, gridDim.x);
   if (act && !act[grp]) return;  // whole block: no barrier is reached
   constexpr bool OFFD = MODE == 5;
   constexpr bool ZERO = MODE == 6;  // sweep from x = 0: x = S b (x is not read)
@@ -679,11 +679,10 @@ int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int m
   return apply_impl<float, 2>(c, c->g32, xm, b, r, mode, s, tin, -1);
 }
 
-// The preconditioner's 16-bit variant: the neighbour-form sweeps, the sweep from zero and
-// the residual read the smoother and the face traces in bfloat16 (arithmetic in FP32).
-// Mode 1: r = b - A x from the traces in buffer tin; 5 / 6: sweeps writing buffer tout;
-// 7: a sweep as 5 that writes FP32 traces into the FP32 buffer tout (for the FP32 apply
-// that follows the preconditioner).
+// The preconditioner's 16-bit variant: the neighbour-form sweeps and the sweep from zero
+// read the smoother and the neighbours' face traces in bfloat16 (arithmetic in FP32).
+// Modes 5 / 6: sweeps writing the 16-bit buffer tout; 7: a sweep as 5 that writes FP32
+// traces into the FP32 buffer tout (for an FP32 apply that follows).
 template <int D>
 static int apply_impl16(dgmi_ctx* c, float* x, const float* b, float* r, int mode, int tin,
                         int tout, hipStream_t s) {
@@ -696,15 +695,13 @@ static int apply_impl16(dgmi_ctx* c, float* x, const float* b, float* r, int mod
   if (mode == 6) tin = tout == 0 ? 1 : 0;
   const bool out32 = mode == 7;
   typename Vec2<float>::type* bufs32[2] = {g.trace, g.trace2};
-  if (!(g.S16 && tin >= 0 && tin <= 1 && bufs[tin] &&
-        (mode == 1 ? tout < 0
-                   : (tout >= 0 && tout <= 1 && (out32 ? bufs32[tout] != nullptr
-                                                       : tout != tin))))) {
+  if (!(g.S16 && (mode == 5 || mode == 6 || mode == 7) && tin >= 0 && tin <= 1 && bufs[tin] &&
+        tout >= 0 && tout <= 1 && (out32 ? bufs32[tout] != nullptr : tout != tin))) {
     set_error("bad 16-bit trace buffers");
     return -2;
   }
   const TraceBf16* trace = bufs[tin];
-  TraceBf16* to = tout >= 0 && !out32 ? bufs[tout] : nullptr;
+  TraceBf16* to = out32 ? nullptr : bufs[tout];
   const float* vt = g.tab + g.off_vtab;
   const float* wg = g.tab + g.off_wgt;
   const float* sw = g.tab + g.off_sw;
@@ -713,8 +710,7 @@ static int apply_impl16(dgmi_ctx* c, float* x, const float* b, float* r, int mod
                      s, nele, x, b, r, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface,  \
                      c->fcode, trace, g.S16, act, (float)c->omega, to, g.tab + g.off_fc,      \
                      c->fnodes)
-  if (mode == 1) DGMI_RES16(1, false);
-  else if (mode == 5) DGMI_RES16(5, true);
+  if (mode == 5) DGMI_RES16(5, true);
   else if (mode == 6) DGMI_RES16(6, true);
   else
     hipLaunchKernelGGL((k_residual_w<float, D, 5, true, TraceBf16, uint16_t,
@@ -727,8 +723,11 @@ static int apply_impl16(dgmi_ctx* c, float* x, const float* b, float* r, int mod
   return 0;
 }
 
+// independent of the autotuned FP32 family: the 16-bit variant always runs on the vector
+// kernels (and so does the FP32 apply that reads its last traces)
 bool trace_io16(const dgmi_ctx* c) {
-  return trace_io(c) && c->g32.S16 && c->g32.tr16[0] && c->g32.tr16[1];
+  return smooth_offd<float>(c) && c->g32.trace && c->g32.trace2 && c->g32.S16 && c->g32.tr16[0] &&
+         c->g32.tr16[1];
 }
 int launch_smooth_io16(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s) {
   if (c->D == 3) return apply_impl16<3>(c, x, b, nullptr, 5, tin, tout, s);
@@ -743,12 +742,7 @@ int launch_smooth_zero_io16(dgmi_ctx* c, float* x, const float* b, int tout, hip
   if (c->D == 3) return apply_impl16<3>(c, x, b, nullptr, 6, -1, tout, s);
   return apply_impl16<2>(c, x, b, nullptr, 6, -1, tout, s);
 }
-int launch_apply_io16(dgmi_ctx* c, const float* x, const float* b, float* r, int tin,
-                      hipStream_t s) {
-  float* xm = const_cast<float*>(x);
-  if (c->D == 3) return apply_impl16<3>(c, xm, b, r, 1, tin, -1, s);
-  return apply_impl16<2>(c, xm, b, r, 1, tin, -1, s);
-}
+
 
 template <typename T>
 int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t s) {

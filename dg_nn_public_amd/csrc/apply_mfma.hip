@@ -242,7 +242,7 @@ __global__ __launch_bounds__(64 * WPB) void k_trace_mf(int nele, const float* __
   using M = Mq<D>;
   using S = MfShape<D>;
   __shared__ MfLds<TIN, D> L;
-  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  const int grp = act ? (int)blockIdx.x : xcd_group(blockIdx.x, gridDim.x);
   if (act && !mf_block_active<WPB>(act, nele, grp)) return;  // before any barrier
   const int lane = threadIdx.x & 63, q = lane >> 4;
   const int wave = grp * WPB + (threadIdx.x >> 6);
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
   // zero volume tiles for the pairing
   constexpr bool SKIP_VOL = OFFD && (std::is_same<TIN, float>::value || S::NTV % 2 == 0);
   __shared__ MfLds<TIN, D> L;
-  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  const int grp = act ? (int)blockIdx.x : xcd_group(blockIdx.x, gridDim.x);
   if (act && !mf_block_active<WPB>(act, nele, grp)) return;  // before any barrier
   __shared__ float rx[(MODE == 2 || MODE == 3 || OFFD) ? WPB : 1][16][NL + 1];
   const int lane = threadIdx.x & 63, q = lane >> 4, wl = threadIdx.x >> 6;

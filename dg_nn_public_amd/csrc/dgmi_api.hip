@@ -234,14 +234,16 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
   if (tr_out) *tr_out = -1;
   if constexpr (std::is_same<T, float>::value) {
     if (c->pc_bf16 && trace_io16(c)) {
-      // bf16 inner cycles with the exact block smoother: the steps below with the smoother
-      // and the face traces read and written in bfloat16 (16-bit buffers 0 and 1
-      // alternate); x stays FP32, and the last sweep writes FP32 traces (FP32 buffer 1)
-      // for the caller's Krylov apply, which keeps the FP32 operator
+      // bf16 inner cycles with the exact block smoother: the sweeps below read the smoother
+      // and their neighbours' face traces in bfloat16 (16-bit buffers 0 and 1 alternate);
+      // x stays FP32.  The sweep before an apply writes FP32 traces instead: the residual
+      // b - A x takes differences of the own and the neighbours' traces (the face jumps),
+      // which bfloat16 values would not resolve, and the caller's Krylov apply keeps the
+      // FP32 operator (FP32 buffer 1 on return)
       DGMI_TRY(launch_smooth_zero_io16(c, x, b, 0, s));
       DGMI_TRY(launch_smooth_io16(c, x, b, 0, 1, s));
-      DGMI_TRY(launch_smooth_io16(c, x, b, 1, 0, s));
-      DGMI_TRY(launch_apply_io16(c, x, b, r, 0, s));
+      DGMI_TRY(launch_smooth_io16_out32(c, x, b, 1, 0, s));
+      DGMI_TRY(launch_apply_io(c, x, b, r, 1, 0, s));
       DGMI_TRY(launch_restrict<T>(c, r, rc, s));
       DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s));
       DGMI_TRY(launch_prolong_trace(c, ec, x, 0, s, true));

@@ -294,7 +294,7 @@ int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, h
 int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s);
 int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
                     hipStream_t s);
-// The same three with the smoother and the traces stored in bfloat16 (g32.S16, g32.tr16);
+// The sweeps with the smoother and the traces stored in bfloat16 (g32.S16, g32.tr16);
 // trace_io16() tells whether the context has them
 bool trace_io16(const dgmi_ctx* c);
 int launch_smooth_io16(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s);
@@ -303,8 +303,6 @@ int launch_smooth_zero_io16(dgmi_ctx* c, float* x, const float* b, int tout, hip
 // an FP32 apply)
 int launch_smooth_io16_out32(dgmi_ctx* c, float* x, const float* b, int tin, int tout,
                              hipStream_t s);
-int launch_apply_io16(dgmi_ctx* c, const float* x, const float* b, float* r, int tin,
-                      hipStream_t s);
 // g32.S = packed symmetric FP32 copy of g64.S (and g32.S16 its bfloat16 rounding, when
 // allocated)
 int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s);

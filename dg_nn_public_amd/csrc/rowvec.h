@@ -13,7 +13,9 @@ namespace dgmi {
 // and each XCD has its own L2.  The element kernels read their neighbours' face traces,
 // and neighbours are close in the element (SFC) order, so each XCD is given one
 // contiguous range of element groups: group b takes logical group
-// (first group of XCD b % 8) + b / 8.  A bijection on [0, n) for any n.
+// (first group of XCD b % 8) + b / 8.  A bijection on [0, n) for any n.  Kernels that skip
+// the groups of finished solves (packed contexts) keep the identity: there the remaining
+// work would gather on the XCDs that hold the slow solves.
 __device__ __forceinline__ int xcd_group(int b, int n) {
   constexpr int NX = 8;
   const int x = b % NX, q = n / NX, r = n % NX;

@@ -136,9 +136,10 @@ int dgmi_precond(dgmi_ctx* ctx, int dtype, const void* b, void* x, void* stream)
  * arithmetic of the operator applies and V-cycles inside the Krylov loop; with
  * DGMI_F32 the solve runs as FP64 iterative refinement around FP32 cycles (flexible
  * GMRES unless "inner_flexible" is 0), with DGMI_BF16 likewise with bfloat16 inside the
- * preconditioner: with the FP32 vector kernels and DGMI_SMOOTH_DIRECT the smoother and
- * the face traces of its sweeps are stored in bf16 (arithmetic FP32), otherwise its
- * applies take bf16 inputs on the matrix cores.  The Krylov operator stays FP32.  Stops when ||b - A x|| <= rtol ||b|| (FP64 true residual)
+ * preconditioner: with DGMI_SMOOTH_DIRECT (neighbour-form sweeps) the smoother and the
+ * neighbour face traces its sweeps read are stored in bf16 (FP32 arithmetic, vector
+ * kernels), otherwise its applies take bf16 inputs on the matrix cores.  The residual
+ * inside the V-cycle and the Krylov operator stay FP32.  Stops when ||b - A x|| <= rtol ||b|| (FP64 true residual)
  * or after max_its inner iterations, per component.  Synchronises the stream.  stats
  * (may be NULL), 4 values per component: iterations, final ||r||/||b||, outer
  * refinement steps, restarts. */
