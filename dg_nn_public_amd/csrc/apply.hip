@@ -69,19 +69,17 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
 // EMIT (mode 5 only): also write the face traces of the new x into tout (another buffer
 // than `trace`, which other work-groups are still reading), so the next sweep needs no
 // trace pass.
-// TR: storage of the face traces read (Vec2<T>, or TraceBf16 in the preconditioner's 16-bit
-// variant), TO: of the traces written; SS: storage of the smoother (T, or uint16_t =
-// bfloat16 packed symmetric).  Arithmetic is in T throughout.
-template <typename T, int D, int MODE, bool EMIT = false, typename TR = typename Vec2<T>::type,
-          typename SS = T, typename TO = TR>
+// SS: storage of the smoother (T, or uint16_t = the bfloat16 packed symmetric copy of the
+// bf16 inner cycles).  Arithmetic is in T throughout.
+template <typename T, int D, int MODE, bool EMIT = false, typename SS = T>
 __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     int nele, T* __restrict__ x, const T* b, T* r, const T* __restrict__ vtab,
     const T* __restrict__ wgt, const T* __restrict__ sw,
     const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
     const T* __restrict__ wf, const T* __restrict__ pen, const int32_t* __restrict__ nbface,
-    const int32_t* __restrict__ fcode, const TR* __restrict__ trace,
+    const int32_t* __restrict__ fcode, const typename Vec2<T>::type* __restrict__ trace,
     const SS* __restrict__ S, const int32_t* __restrict__ act, T omega,
-    TO* __restrict__ tout, const T* __restrict__ fc,
+    typename Vec2<T>::type* __restrict__ tout, const T* __restrict__ fc,
     const int32_t* __restrict__ fnodes) {
   using M = Dm<D>;
   using F = FaceDims<D>;
@@ -95,6 +93,8 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
   static_assert(M::NGI % NF == 0, "volume points must split evenly over the waves");
   __shared__ T red[NF][NL][64];
+  // the new rows for the trace output (other waves may still read red when a wave writes)
+  __shared__ T xs[EMIT ? NL : 1][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int e = grp * 64 + lane;
@@ -112,9 +112,16 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
     bv[k] = (MODE != 0 && MODE != 4 && !ZERO && valid && n < NL) ? b[(size_t)e * NL + n] : T(0);
     xo[k] = (KEEPX && valid && n < NL) ? x[(size_t)e * NL + n] : T(0);
   }
-  // x = S b reads the whole row of b instead of exchanging it through LDS
+  // the smoothing modes read the whole row of b: every wave forms the whole residual row
+  // from the partial sums, so the rows need no second exchange through LDS
+  constexpr bool SMOOTH = MODE == 2 || MODE == 3 || OFFD || ZERO;
+  // requested before the face and volume work in FP32 (FP64 loads it after the barrier:
+  // the row would cost 40 more registers over the volume work)
+  constexpr bool BEARLY = ZERO || (SMOOTH && sizeof(T) == 4);
   T bfull[NL];
-  if constexpr (ZERO) {
+#pragma unroll
+  for (int n = 0; n < NL; ++n) bfull[n] = 0;
+  if constexpr (BEARLY) {
     if (valid) load_row<T, NL>(b + (size_t)e * NL, bfull);
   }
   if (valid && !ZERO && !OFFD) {
@@ -211,17 +218,44 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
 #pragma unroll
     for (int i = 0; i < F::NFN; ++i) red[w][fnodes[w * F::NFN + i]][lane] += fl.accf[i];
     __syncthreads();
-    // rows n of this wave: sum the partials, form the residual
+    if constexpr (SMOOTH) {
+      if constexpr (!BEARLY) {
+        if (valid) load_row<T, NL>(b + (size_t)e * NL, bfull);
+      }
+      // the whole residual row, in bfull (b - sum of the partials)
 #pragma unroll
-    for (int k = 0; k < NPW; ++k) {
-      const int n = w * NPW + k;
-      if (n < NL) {
+      for (int n = 0; n < NL; ++n) {
         T s = 0;
 #pragma unroll
         for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
-        if constexpr (MODE == 0) rv[k] = -s;
-        else if constexpr (MODE == 4) rv[k] = s;
-        else rv[k] = valid ? bv[k] - s : T(0);
+        bfull[n] = valid ? bfull[n] - s : T(0);
+      }
+      if constexpr (MODE == 3) {
+        // this wave's rows of r, indexed statically (bfull[w * NPW + k] would not be)
+#pragma unroll
+        for (int k = 0; k < NPW; ++k) {
+          const int n = w * NPW + k;
+          if (n < NL) {
+            T s = 0;
+#pragma unroll
+            for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
+            rv[k] = valid ? bv[k] - s : T(0);
+          }
+        }
+      }
+    } else {
+      // rows n of this wave: sum the partials, form the residual
+#pragma unroll
+      for (int k = 0; k < NPW; ++k) {
+        const int n = w * NPW + k;
+        if (n < NL) {
+          T s = 0;
+#pragma unroll
+          for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
+          if constexpr (MODE == 0) rv[k] = -s;
+          else if constexpr (MODE == 4) rv[k] = s;
+          else rv[k] = valid ? bv[k] - s : T(0);
+        }
       }
     }
   }
@@ -234,16 +268,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
       }
     }
   } else {
-    // smoothing: every wave needs the whole residual row; exchange it through red[0]
-    // (each wave only read red[*][its rows], so overwriting red[0][its rows] is safe)
-    if constexpr (!ZERO) {
-#pragma unroll
-      for (int k = 0; k < NPW; ++k) {
-        const int n = w * NPW + k;
-        if (n < NL) red[0][n][lane] = rv[k];
-      }
-      __syncthreads();
-    }
+    // smoothing: bfull holds the whole residual row (b itself from x = 0)
     if (valid) {
 #pragma unroll
       for (int k = 0; k < NPW; ++k) {
@@ -253,10 +278,10 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
           load_smoother_row<T, NL, SS>(S, n, e, nele, srow);
           T s = ZERO ? T(0) : (OFFD ? (T(1) - omega) * xo[k] : xo[k]);
 #pragma unroll
-          for (int m = 0; m < NL; ++m) s += srow[m] * (ZERO ? bfull[m] : red[0][m][lane]);
+          for (int m = 0; m < NL; ++m) s += srow[m] * bfull[m];
           x[(size_t)e * NL + n] = s;
           if constexpr (MODE == 3) r[(size_t)e * NL + n] = rv[k];
-          if constexpr (EMIT) red[1][n][lane] = s;  // red[1] is not read after the rows
+          if constexpr (EMIT) xs[n][lane] = s;
         }
       }
     }
@@ -269,11 +294,11 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
 #pragma unroll
         for (int d = 0; d < D; ++d) cc[d] = cf[(f * D + d) * nele + e];
 #pragma unroll
-        for (int n = 0; n < NL; ++n) xn[n] = red[1][n][lane];
+        for (int n = 0; n < NL; ++n) xn[n] = xs[n][lane];
 #pragma unroll
-        for (int i = 0; i < F::NFN; ++i) xf[i] = red[1][fnodes[f * F::NFN + i]][lane];
+        for (int i = 0; i < F::NFN; ++i) xf[i] = xs[fnodes[f * F::NFN + i]][lane];
         face_traces<T, D>(f, fc, xn, xf, cc, [&](int g, T u, T dn) {
-          tout[(f * M::SG + g) * nele + e] = make_trace<TO>(u, dn);
+          tout[(f * M::SG + g) * nele + e] = make_trace<typename Vec2<T>::type>(u, dn);
         });
       }
     }
@@ -624,6 +649,38 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   return 0;
 }
 
+// The sweeps of the bf16 inner cycles: as modes 5 and 6 with traces out, reading the
+// bfloat16 copy of the smoother
+template <int D>
+static int sweep_s16(dgmi_ctx* c, float* x, const float* b, int mode, int tin, int tout,
+                     hipStream_t s) {
+  DevGeom<float>& g = c->g32;
+  const int nele = (int)c->nele;
+  const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
+  const int32_t* act = c->mask_on ? c->act64 : nullptr;
+  using V2 = typename Vec2<float>::type;
+  V2* bufs[2] = {g.trace, g.trace2};
+  if (mode == 6) tin = tout == 0 ? 1 : 0;
+  if (!(g.S16 && (mode == 5 || mode == 6) && tin >= 0 && tin <= 1 && tout >= 0 && tout <= 1 &&
+        tin != tout && bufs[tin] && bufs[tout])) {
+    set_error("bad trace buffers or no bf16 smoother");
+    return -2;
+  }
+  const float* vt = g.tab + g.off_vtab;
+  const float* wg = g.tab + g.off_wgt;
+  const float* sw = g.tab + g.off_sw;
+#define DGMI_RES16(MODE)                                                                      \
+  hipLaunchKernelGGL((k_residual_w<float, D, MODE, true, uint16_t>), grd, blk, 0, s, nele, x,  \
+                     b, nullptr, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface,        \
+                     c->fcode, (const V2*)bufs[tin], (const uint16_t*)g.S16, act,             \
+                     (float)c->omega, bufs[tout], g.tab + g.off_fc, c->fnodes)
+  if (mode == 5) DGMI_RES16(5);
+  else DGMI_RES16(6);
+#undef DGMI_RES16
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
 // FP32 vectors go to the matrix-core kernels unless the context selects the vector ones
 template <typename T>
 static int apply_dispatch(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mode,
@@ -664,13 +721,20 @@ int launch_smooth(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r_out, hipStr
 bool trace_io(const dgmi_ctx* c) {
   return c->f32_kernel == 0 && smooth_offd<float>(c) && c->g32.trace2;
 }
-int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s) {
+int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s,
+                     bool s16) {
+  if (s16) return c->D == 3 ? sweep_s16<3>(c, x, b, 5, tin, tout, s) : sweep_s16<2>(c, x, b, 5, tin, tout, s);
   if (c->D == 3) return apply_impl<float, 3>(c, c->g32, x, b, nullptr, 5, s, tin, tout);
   return apply_impl<float, 2>(c, c->g32, x, b, nullptr, 5, s, tin, tout);
 }
-int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s) {
+int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s,
+                          bool s16) {
+  if (s16) return c->D == 3 ? sweep_s16<3>(c, x, b, 6, -1, tout, s) : sweep_s16<2>(c, x, b, 6, -1, tout, s);
   if (c->D == 3) return apply_impl<float, 3>(c, c->g32, x, b, nullptr, 6, s, -1, tout);
   return apply_impl<float, 2>(c, c->g32, x, b, nullptr, 6, s, -1, tout);
+}
+bool bf16_smoother(const dgmi_ctx* c) {
+  return smooth_offd<float>(c) && c->g32.S16 && c->g32.trace && c->g32.trace2;
 }
 int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
                     hipStream_t s) {
@@ -679,69 +743,6 @@ int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int m
   return apply_impl<float, 2>(c, c->g32, xm, b, r, mode, s, tin, -1);
 }
 
-// The preconditioner's 16-bit variant: the neighbour-form sweeps and the sweep from zero
-// read the smoother and the neighbours' face traces in bfloat16 (arithmetic in FP32).
-// Modes 5 / 6: sweeps writing the 16-bit buffer tout; 7: a sweep as 5 that writes FP32
-// traces into the FP32 buffer tout (for an FP32 apply that follows).
-template <int D>
-static int apply_impl16(dgmi_ctx* c, float* x, const float* b, float* r, int mode, int tin,
-                        int tout, hipStream_t s) {
-  DevGeom<float>& g = c->g32;
-  const int nele = (int)c->nele;
-  const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
-  const int32_t* act = c->mask_on ? c->act64 : nullptr;
-  TraceBf16* bufs[2] = {reinterpret_cast<TraceBf16*>(g.tr16[0]),
-                        reinterpret_cast<TraceBf16*>(g.tr16[1])};
-  if (mode == 6) tin = tout == 0 ? 1 : 0;
-  const bool out32 = mode == 7;
-  typename Vec2<float>::type* bufs32[2] = {g.trace, g.trace2};
-  if (!(g.S16 && (mode == 5 || mode == 6 || mode == 7) && tin >= 0 && tin <= 1 && bufs[tin] &&
-        tout >= 0 && tout <= 1 && (out32 ? bufs32[tout] != nullptr : tout != tin))) {
-    set_error("bad 16-bit trace buffers");
-    return -2;
-  }
-  const TraceBf16* trace = bufs[tin];
-  TraceBf16* to = out32 ? nullptr : bufs[tout];
-  const float* vt = g.tab + g.off_vtab;
-  const float* wg = g.tab + g.off_wgt;
-  const float* sw = g.tab + g.off_sw;
-#define DGMI_RES16(MODE, EMIT)                                                                \
-  hipLaunchKernelGGL((k_residual_w<float, D, MODE, EMIT, TraceBf16, uint16_t>), grd, blk, 0,   \
-                     s, nele, x, b, r, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface,  \
-                     c->fcode, trace, g.S16, act, (float)c->omega, to, g.tab + g.off_fc,      \
-                     c->fnodes)
-  if (mode == 5) DGMI_RES16(5, true);
-  else if (mode == 6) DGMI_RES16(6, true);
-  else
-    hipLaunchKernelGGL((k_residual_w<float, D, 5, true, TraceBf16, uint16_t,
-                                     typename Vec2<float>::type>),
-                       grd, blk, 0, s, nele, x, b, r, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen,
-                       c->nbface, c->fcode, trace, g.S16, act, (float)c->omega, bufs32[tout],
-                       g.tab + g.off_fc, c->fnodes);
-#undef DGMI_RES16
-  DGMI_HIP(hipGetLastError());
-  return 0;
-}
-
-// independent of the autotuned FP32 family: the 16-bit variant always runs on the vector
-// kernels (and so does the FP32 apply that reads its last traces)
-bool trace_io16(const dgmi_ctx* c) {
-  return smooth_offd<float>(c) && c->g32.trace && c->g32.trace2 && c->g32.S16 && c->g32.tr16[0] &&
-         c->g32.tr16[1];
-}
-int launch_smooth_io16(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s) {
-  if (c->D == 3) return apply_impl16<3>(c, x, b, nullptr, 5, tin, tout, s);
-  return apply_impl16<2>(c, x, b, nullptr, 5, tin, tout, s);
-}
-int launch_smooth_io16_out32(dgmi_ctx* c, float* x, const float* b, int tin, int tout,
-                             hipStream_t s) {
-  if (c->D == 3) return apply_impl16<3>(c, x, b, nullptr, 7, tin, tout, s);
-  return apply_impl16<2>(c, x, b, nullptr, 7, tin, tout, s);
-}
-int launch_smooth_zero_io16(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s) {
-  if (c->D == 3) return apply_impl16<3>(c, x, b, nullptr, 6, -1, tout, s);
-  return apply_impl16<2>(c, x, b, nullptr, 6, -1, tout, s);
-}
 
 
 template <typename T>

@@ -233,23 +233,23 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
   T* ec = cgbuf<T>(c, 1);
   if (tr_out) *tr_out = -1;
   if constexpr (std::is_same<T, float>::value) {
-    if (c->pc_bf16 && trace_io16(c)) {
-      // bf16 inner cycles with the exact block smoother: the sweeps below read the smoother
-      // and their neighbours' face traces in bfloat16 (16-bit buffers 0 and 1 alternate);
-      // x stays FP32.  The sweep before an apply writes FP32 traces instead: the residual
-      // b - A x takes differences of the own and the neighbours' traces (the face jumps),
-      // which bfloat16 values would not resolve, and the caller's Krylov apply keeps the
-      // FP32 operator (FP32 buffer 1 on return)
-      DGMI_TRY(launch_smooth_zero_io16(c, x, b, 0, s));
-      DGMI_TRY(launch_smooth_io16(c, x, b, 0, 1, s));
-      DGMI_TRY(launch_smooth_io16_out32(c, x, b, 1, 0, s));
+    // bf16 inner cycles with "bf16_smoother": the sweeps read the bfloat16 copy of the
+    // exact block smoother (vector kernels, whatever FP32 family is tuned); the face
+    // traces, the residual and the Krylov operator stay FP32
+    const bool s16 = c->pc_bf16 && c->bf16_smoother && bf16_smoother(c);
+    if (s16 || (!c->pc_bf16 && trace_io(c))) {
+      // each sweep (and the prolongation) writes the traces of its result for the next
+      // one; buffers 0 and 1 alternate
+      DGMI_TRY(launch_smooth_zero_io(c, x, b, 0, s, s16));
+      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s, s16));
+      DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s, s16));
       DGMI_TRY(launch_apply_io(c, x, b, r, 1, 0, s));
       DGMI_TRY(launch_restrict<T>(c, r, rc, s));
       DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s));
-      DGMI_TRY(launch_prolong_trace(c, ec, x, 0, s, true));
-      DGMI_TRY(launch_smooth_io16(c, x, b, 0, 1, s));
-      DGMI_TRY(launch_smooth_io16(c, x, b, 1, 0, s));
-      DGMI_TRY(launch_smooth_io16_out32(c, x, b, 0, 1, s));
+      DGMI_TRY(launch_prolong_trace(c, ec, x, 0, s));
+      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s, s16));
+      DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s, s16));
+      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s, s16));
       if (tr_out) *tr_out = 1;
       return 0;
     }
@@ -261,24 +261,6 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
     ~KernelScope() { c->f32_kernel = saved; }
   } scope{c, c->f32_kernel};
   if (std::is_same<T, float>::value && c->pc_bf16) c->f32_kernel = 2;
-  if constexpr (std::is_same<T, float>::value) {
-    if (trace_io(c)) {
-      // the same steps, each sweep (and the prolongation) writing the traces of its
-      // result for the next one; buffers 0 and 1 alternate
-      DGMI_TRY(launch_smooth_zero_io(c, x, b, 0, s));
-      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s));
-      DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s));
-      DGMI_TRY(launch_apply_io(c, x, b, r, 1, 0, s));
-      DGMI_TRY(launch_restrict<T>(c, r, rc, s));
-      DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s));
-      DGMI_TRY(launch_prolong_trace(c, ec, x, 0, s));
-      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s));
-      DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s));
-      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s));
-      if (tr_out) *tr_out = 1;
-      return 0;
-    }
-  }
   // pre-smoothing: the first step starts from x = 0, where the residual is b
   DGMI_TRY(launch_smooth_zero<T>(c, g, x, b, s));
   for (int it = 1; it < 3; ++it) DGMI_TRY(launch_smooth<T>(c, g, x, b, nullptr, s));
@@ -921,13 +903,9 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
   if (!c->g32.S) c->g32.S = static_cast<float*>(c->alloc(sizeof(float) * nS32));
   DGMI_CHECK(c->g64.S && c->g32.S, alloc_failure());
   if (variant == DGMI_SMOOTH_DIRECT) {
-    // the preconditioner's 16-bit variant (bf16 inner cycles): smoother and face traces in
-    // bfloat16, used by the neighbour-form sweeps
-    const int64_t ntr = c->nele * c->nface * c->sngi;
+    // the bf16 inner cycles' neighbour-form sweeps read a bfloat16 copy of the smoother
     if (!c->g32.S16) c->g32.S16 = static_cast<uint16_t*>(c->alloc(sizeof(uint16_t) * nS32));
-    for (int k = 0; k < 2; ++k)
-      if (!c->g32.tr16[k]) c->g32.tr16[k] = static_cast<uint32_t*>(c->alloc(sizeof(uint32_t) * ntr));
-    DGMI_CHECK(c->g32.S16 && c->g32.tr16[0] && c->g32.tr16[1], alloc_failure());
+    DGMI_CHECK(c->g32.S16, alloc_failure());
   }
   Scratch& sc = scratch(c);
   if (variant == DGMI_SMOOTH_JACOBI3) {
@@ -1338,6 +1316,11 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "smooth_offdiag") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "smooth_offdiag must be 0 or 1");
     c->smooth_offdiag = (int)value;
+    return 0;
+  }
+  if (k == "bf16_smoother") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "bf16_smoother must be 0 or 1");
+    c->bf16_smoother = (int)value;
     return 0;
   }
   if (k == "graph_vcycle") {

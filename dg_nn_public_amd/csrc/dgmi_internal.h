@@ -48,10 +48,9 @@ template <typename T> struct DevGeom {
   // matrix-core kernels
   typename Vec2<T>::type* trace = nullptr;
   typename Vec2<T>::type* trace2 = nullptr;  // second buffer (FP32 vector sweeps)
-  // the preconditioner's 16-bit variant (FP32 only, exact block smoother): the packed
-  // symmetric smoother in bfloat16 and two trace buffers of (u, du/dn) bfloat16 pairs
+  // bf16 inner cycles (FP32 only, exact block smoother): the packed symmetric smoother in
+  // bfloat16
   uint16_t* S16 = nullptr;
-  uint32_t* tr16[2] = {nullptr, nullptr};
 };
 
 // SFC levels on the device.  "Grid" levels 0..small_level hold every component whose
@@ -234,6 +233,9 @@ struct dgmi_ctx {
   // FP32 smoothing sweeps with the exact block smoother in the neighbour-only form
   // x = (1 - omega) x + S (b - A_off x) (no volume term, half the face terms)
   int smooth_offdiag = 1;
+  // bf16 inner cycles with the exact block smoother: 1 = the sweeps read a bfloat16 copy of
+  // the smoother (vector kernels); 0 = bf16-input matrix-core applies
+  int bf16_smoother = 0;
   // waves (16 elements each) per work-group of the fp32 / bf16 matrix-core applies
   int mf_wpb = 4, bf_wpb = 4;
   bool pc_bf16 = false;  // FP32 preconditioner applies with bf16 inputs
@@ -289,20 +291,16 @@ int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* f
                           hipStream_t s);
 // FP32 vector-family sweeps carrying traces between them (see apply.hip)
 bool trace_io(const dgmi_ctx* c);
-int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s);
+// (s16: read the bfloat16 copy of the smoother)
+int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s,
+                     bool s16 = false);
 // x = S b (the first sweep from x = 0), traces of x into buffer tout
-int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s);
+int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s,
+                          bool s16 = false);
 int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
                     hipStream_t s);
-// The sweeps with the smoother and the traces stored in bfloat16 (g32.S16, g32.tr16);
-// trace_io16() tells whether the context has them
-bool trace_io16(const dgmi_ctx* c);
-int launch_smooth_io16(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s);
-int launch_smooth_zero_io16(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s);
-// a 16-bit sweep that writes FP32 traces into the FP32 buffer tout (the last sweep before
-// an FP32 apply)
-int launch_smooth_io16_out32(dgmi_ctx* c, float* x, const float* b, int tin, int tout,
-                             hipStream_t s);
+// the exact block smoother has a bfloat16 copy for the bf16 inner cycles
+bool bf16_smoother(const dgmi_ctx* c);
 // g32.S = packed symmetric FP32 copy of g64.S (and g32.S16 its bfloat16 rounding, when
 // allocated)
 int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s);
@@ -314,9 +312,7 @@ int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s);
 template <typename T>
 int launch_prolong_add(dgmi_ctx* c, const T* e_sfc, T* x_dg, hipStream_t s);
 // FP32 vector family: x += P e with the traces of the new x written into buffer tout
-// (bf16: into the 16-bit variant's buffer tout)
-int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s,
-                         bool bf16 = false);
+int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s);
 template <typename T>
 int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s);
 // single-work-group part of the V-cycle: LDS bytes of levels l0.. of one component, the

@@ -91,11 +91,12 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nele, const int32_t* __
 // where the other waves read their face's nodes) and the traces of its face w into
 // tout[(w*sngi + g)*nele + e] (same operations as the trace pass).
 // fc / fnodes: the face-interpolation tables (faces.h); cf: (nface*D, nele).
-template <typename T, int D, typename TR = typename Vec2<T>::type>
+template <typename T, int D>
 __global__ __launch_bounds__(64 * (D + 1)) void k_prolong_trace(
     int nele, const int32_t* __restrict__ cellpos, const T* __restrict__ rw,
     const T* __restrict__ ec, T* __restrict__ x, const T* __restrict__ fc,
-    const int32_t* __restrict__ fnodes, const T* __restrict__ cf, TR* __restrict__ tout) {
+    const int32_t* __restrict__ fnodes, const T* __restrict__ cf,
+    typename Vec2<T>::type* __restrict__ tout) {
   using F = FaceDims<D>;
   constexpr int NL = F::NL, NB = D + 1, NF = D + 1, SG = F::SG;
   constexpr int NPW = (NL + NF - 1) / NF;
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_prolong_trace(
 #pragma unroll
   for (int i = 0; i < F::NFN; ++i) xf[i] = xs[fnodes[f * F::NFN + i]][lane];
   face_traces<T, D>(f, fc, xv, xf, c, [&](int g, T u, T dn) {
-    tout[(f * SG + g) * nele + e] = make_trace<TR>(u, dn);
+    tout[(f * SG + g) * nele + e] = make_trace<typename Vec2<T>::type>(u, dn);
   });
 }
 
@@ -633,42 +634,23 @@ int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s) {
 }
 
 // FP32 vector family: x += P e and the traces of the new x into buffer tout (0 or 1)
-int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s,
-                         bool bf16) {
+int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s) {
   const int nele = (int)c->nele;
   const DevGeom<float>& g = c->g32;
-  if (tout != 0 && tout != 1) {
+  typename Vec2<float>::type* to = tout == 0 ? g.trace : g.trace2;
+  if (!to || (tout != 0 && tout != 1)) {
     set_error("bad trace buffer");
     return -2;
   }
   const dim3 blk(64 * (c->D + 1)), grd((nele + 63) / 64);
   const float* rw = (const float*)c->mg.rw32;
   const float* fc = g.tab + g.off_fc;
-  if (bf16) {
-    TraceBf16* to = reinterpret_cast<TraceBf16*>(g.tr16[tout]);
-    if (!to) {
-      set_error("bad trace buffer");
-      return -2;
-    }
-    if (c->D == 3)
-      hipLaunchKernelGGL((k_prolong_trace<float, 3, TraceBf16>), grd, blk, 0, s, nele,
-                         c->mg.cellpos, rw, e_sfc, x_dg, fc, c->fnodes, g.cf, to);
-    else
-      hipLaunchKernelGGL((k_prolong_trace<float, 2, TraceBf16>), grd, blk, 0, s, nele,
-                         c->mg.cellpos, rw, e_sfc, x_dg, fc, c->fnodes, g.cf, to);
-  } else {
-    typename Vec2<float>::type* to = tout == 0 ? g.trace : g.trace2;
-    if (!to) {
-      set_error("bad trace buffer");
-      return -2;
-    }
-    if (c->D == 3)
-      hipLaunchKernelGGL((k_prolong_trace<float, 3>), grd, blk, 0, s, nele, c->mg.cellpos, rw,
-                         e_sfc, x_dg, fc, c->fnodes, g.cf, to);
-    else
-      hipLaunchKernelGGL((k_prolong_trace<float, 2>), grd, blk, 0, s, nele, c->mg.cellpos, rw,
-                         e_sfc, x_dg, fc, c->fnodes, g.cf, to);
-  }
+  if (c->D == 3)
+    hipLaunchKernelGGL((k_prolong_trace<float, 3>), grd, blk, 0, s, nele, c->mg.cellpos, rw, e_sfc,
+                       x_dg, fc, c->fnodes, g.cf, to);
+  else
+    hipLaunchKernelGGL((k_prolong_trace<float, 2>), grd, blk, 0, s, nele, c->mg.cellpos, rw, e_sfc,
+                       x_dg, fc, c->fnodes, g.cf, to);
   DGMI_HIP(hipGetLastError());
   return 0;
 }

@@ -107,8 +107,8 @@ __device__ __forceinline__ void load_S_row_sym(const float* __restrict__ S, int 
   }
 }
 
-// bfloat16 storage (the preconditioner's 16-bit variant): round to nearest even on the way
-// in, exact widening on the way out.  Inputs are finite.
+// bfloat16 storage (the smoother copy of the bf16 inner cycles): round to nearest even on
+// the way in, exact widening on the way out.  Inputs are finite.
 __host__ __device__ inline uint16_t f32_to_bf16(float f) {
   uint32_t u;
   __builtin_memcpy(&u, &f, 4);
@@ -122,27 +122,15 @@ __host__ __device__ inline float bf16_to_f32(uint32_t h) {
   return f;
 }
 
-// face trace (u, du/dn) in one 32-bit word: u in the low half, du/dn in the high half
-struct TraceBf16 {
-  uint32_t v;
-};
+// face trace (u, du/dn) of the vector kernels
 __device__ __forceinline__ float2 trace_value(const float2& t) { return t; }
 __device__ __forceinline__ double2 trace_value(const double2& t) { return t; }
-__device__ __forceinline__ float2 trace_value(const TraceBf16& t) {
-  return make_float2(bf16_to_f32(t.v & 0xffffu), bf16_to_f32(t.v >> 16));
-}
 template <typename TR, typename T>
 __device__ __forceinline__ TR make_trace(T u, T dn) {
-  if constexpr (std::is_same<TR, TraceBf16>::value) {
-    TraceBf16 t;
-    t.v = (uint32_t)f32_to_bf16((float)u) | ((uint32_t)f32_to_bf16((float)dn) << 16);
-    return t;
-  } else {
-    TR t;
-    t.x = u;
-    t.y = dn;
-    return t;
-  }
+  TR t;
+  t.x = u;
+  t.y = dn;
+  return t;
 }
 
 // row n of a packed symmetric smoother stored in bfloat16

@@ -39,8 +39,8 @@ GMRES_ITS = 400     # z02/config.py:150
 
 _DTYPES = {torch.float64: _lib.DGMI_F64, torch.float32: _lib.DGMI_F32}
 # inner precision of the iterative-refinement solve; bfloat16 means FP32 Krylov vectors and
-# operator with bfloat16 inside the preconditioner: the smoother and face traces stored in
-# bf16 (vector kernels, exact block smoother) or bf16-input matrix-core applies otherwise
+# operator with bfloat16 inside the preconditioner: a bf16 copy of the smoother (vector
+# kernels, exact block smoother) or bf16-input matrix-core applies otherwise
 _INNER = {torch.float64: _lib.DGMI_F64, torch.float32: _lib.DGMI_F32,
           torch.bfloat16: _lib.DGMI_BF16}
 F32_KERNELS = {"vector": 0, "mfma": 1, "mfma_bf16": 2}
@@ -335,9 +335,8 @@ class DGDiffusion:
         """Left-preconditioned GMRES(m).  The reference stops on the absolute residual
         ``tol``; pass ``rtol`` for ||b - A x|| <= rtol ||b||.  ``inner`` float32 runs the
         Krylov loop in FP32 inside FP64 iterative refinement; bfloat16 does the same with
-        bfloat16 inside the preconditioner (smoother and face traces stored in bf16 with the
-        vector kernels and the exact block smoother, bf16-input matrix-core applies
-        otherwise).  ``orth`` "mgs" is the
+        bfloat16 inside the preconditioner (a bf16 copy of the smoother with the vector
+        kernels and the exact block smoother, bf16-input matrix-core applies otherwise).  ``orth`` "mgs" is the
         reference's modified Gram-Schmidt, "cgs2" fused classical Gram-Schmidt with
         re-orthogonalisation.  Returns (x_i, its) and keeps the statistics in
         ``self.last_stats`` (for a packed batch a list of per-component dicts, and ``its``

@@ -136,10 +136,10 @@ int dgmi_precond(dgmi_ctx* ctx, int dtype, const void* b, void* x, void* stream)
  * arithmetic of the operator applies and V-cycles inside the Krylov loop; with
  * DGMI_F32 the solve runs as FP64 iterative refinement around FP32 cycles (flexible
  * GMRES unless "inner_flexible" is 0), with DGMI_BF16 likewise with bfloat16 inside the
- * preconditioner: with DGMI_SMOOTH_DIRECT (neighbour-form sweeps) the smoother and the
- * neighbour face traces its sweeps read are stored in bf16 (FP32 arithmetic, vector
- * kernels), otherwise its applies take bf16 inputs on the matrix cores.  The residual
- * inside the V-cycle and the Krylov operator stay FP32.  Stops when ||b - A x|| <= rtol ||b|| (FP64 true residual)
+ * preconditioner: its applies take bf16 inputs on the matrix cores, or, with
+ * DGMI_SMOOTH_DIRECT and the parameter "bf16_smoother", its sweeps read a bfloat16 copy of
+ * the smoother (FP32 arithmetic and face traces, vector kernels).  The Krylov operator
+ * stays FP32.  Stops when ||b - A x|| <= rtol ||b|| (FP64 true residual)
  * or after max_its inner iterations, per component.  Synchronises the stream.  stats
  * (may be NULL), 4 values per component: iterations, final ||r||/||b||, outer
  * refinement steps, restarts. */
@@ -169,6 +169,10 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     needs no volume term and only the neighbour half of the face terms
  *                     (default); 0 = x += S (b - A x).  Equal in exact arithmetic; FP64
  *                     sweeps and dgmi_smooth with r_out always use the second form.
+ *   "bf16_smoother"   DGMI_BF16 inner cycles with DGMI_SMOOTH_DIRECT: 1 = the sweeps read a
+ *                     bfloat16 copy of the smoother (half its bytes; iteration counts kept
+ *                     at moderate diffusivity contrast, raised at high contrast); 0 = the
+ *                     bf16-input matrix-core applies (default).
  *   "coarse_exact"    FP32 V-cycles: 1 = stop at one SFC level and solve it with its
  *                     dense inverse formed at dgmi_mg_setup (default): a context of one
  *                     solve uses the first level of at most 4096 rows, a packed context

@@ -84,17 +84,20 @@ def test_direct_smoother_mixed_solve_matches_oracle(kern):
 
 @pytest.mark.parametrize("D,n", [(2, 8), (3, 4)], ids=["2d", "3d"])
 def test_bf16_storage_preconditioner_solve_matches_oracle(D, n):
-    """bf16 inner cycles with the vector kernels and the exact block smoother keep the
-    smoother and the face traces in bfloat16 inside the preconditioner (the Krylov operator
-    stays FP32): the solve still reaches 1e-10 in FP64 and agrees with the oracle, with at
-    most a few more iterations than the FP32 preconditioner."""
+    """bf16 inner cycles with the exact block smoother and "bf16_smoother" read a bfloat16
+    copy of the smoother in the preconditioner's sweeps (traces, residual and Krylov
+    operator stay FP32): the
+    solve still reaches 1e-10 in FP64 and agrees with the oracle, with at most a few more
+    iterations than the FP32 preconditioner."""
     m, pb, g, sp = build(D, n, sigma=1.0, blk_solver="direct", face_kappa="harmonic")
     g._get_RAR_and_sfc_data_Um()
     mgd = mg.build_mg(sp, m.verts)
     b = dg.get_rhs(sp, pb.f, pb.u_bc)
     g.set_f32_kernel("vector")
     x32, its32 = g._gmres_mg_solver(g.new_vector(), to_dev(b), rtol=1e-10, inner=torch.float32)
+    g.lib.dgmi_set_param(g.ctx, b"bf16_smoother", 1.0)
     x, its = g._gmres_mg_solver(g.new_vector(), to_dev(b), rtol=1e-10, inner=torch.bfloat16)
+    g.lib.dgmi_set_param(g.ctx, b"bf16_smoother", 0.0)
     xs = to_np(x)
     relres = np.linalg.norm(b - dg.apply_A(sp, xs)) / np.linalg.norm(b)
     x_ref, _ = krylov.gmres_mg_solver(
