@@ -71,8 +71,12 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
 // trace pass.
 // SS: storage of the smoother (T, or uint16_t = the bfloat16 packed symmetric copy of the
 // bf16 inner cycles).  Arithmetic is in T throughout.
+// FP32 neighbour-form sweeps: at least 6 waves per SIMD, so that a 24^3x6 mesh (1,296
+// work-groups) is resident in one round on 256 CUs; within that register budget the
+// compiler issues the smoother rows' loads ahead of the products.
 template <typename T, int D, int MODE, bool EMIT = false, typename SS = T>
-__global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
+__global__ __launch_bounds__(64 * (D + 1),
+                             sizeof(T) == 4 && (MODE == 5 || MODE == 6) ? 6 : 1) void k_residual_w(
     int nele, T* __restrict__ x, const T* b, T* r, const T* __restrict__ vtab,
     const T* __restrict__ wgt, const T* __restrict__ sw,
     const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
@@ -91,6 +95,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
   constexpr int NL = M::NL, NF = M::NF;
   constexpr int GPW = M::NGI / NF;          // volume points per wave
   constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
+  constexpr bool ROWS_FIT = NL % NF == 0;   // every wave's rows exist (3D): no row guards
   static_assert(M::NGI % NF == 0, "volume points must split evenly over the waves");
   __shared__ T red[NF][NL][64];
   // the new rows for the trace output (other waves may still read red when a wave writes)
@@ -235,7 +240,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
 #pragma unroll
         for (int k = 0; k < NPW; ++k) {
           const int n = w * NPW + k;
-          if (n < NL) {
+          if (ROWS_FIT || n < NL) {
             T s = 0;
 #pragma unroll
             for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
@@ -248,7 +253,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
 #pragma unroll
       for (int k = 0; k < NPW; ++k) {
         const int n = w * NPW + k;
-        if (n < NL) {
+        if (ROWS_FIT || n < NL) {
           T s = 0;
 #pragma unroll
           for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
@@ -264,7 +269,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
 #pragma unroll
       for (int k = 0; k < NPW; ++k) {
         const int n = w * NPW + k;
-        if (n < NL) r[(size_t)e * NL + n] = rv[k];
+        if (ROWS_FIT || n < NL) r[(size_t)e * NL + n] = rv[k];
       }
     }
   } else {
@@ -273,7 +278,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_residual_w(
 #pragma unroll
       for (int k = 0; k < NPW; ++k) {
         const int n = w * NPW + k;
-        if (n < NL) {
+        if (ROWS_FIT || n < NL) {
           T srow[NL];
           load_smoother_row<T, NL, SS>(S, n, e, nele, srow);
           T s = ZERO ? T(0) : (OFFD ? (T(1) - omega) * xo[k] : xo[k]);

@@ -228,15 +228,17 @@ template <typename T, int KC, int N>
 __device__ __forceinline__ void gs_dots_rows(int k, const T* __restrict__ V, int64_t ldv,
                                              const T* __restrict__ w, int64_t i,
                                              double (&acc)[KC]) {
+  // every column's loads are issued before the first product (the grid is a few waves
+  // per SIMD: memory parallelism comes from within the thread)
   T wv[N];
   load_n<T, N>(w + i, wv);
+  T vv[KC][N];
 #pragma unroll
-  for (int j = 0; j < KC; ++j) {
-    T vv[N];
-    load_n<T, N>(V + (int64_t)min(j, k - 1) * ldv + i, vv);
+  for (int j = 0; j < KC; ++j) load_n<T, N>(V + (int64_t)min(j, k - 1) * ldv + i, vv[j]);
 #pragma unroll
-    for (int u = 0; u < N; ++u) acc[j] += (double)vv[u] * (double)wv[u];
-  }
+  for (int j = 0; j < KC; ++j)
+#pragma unroll
+    for (int u = 0; u < N; ++u) acc[j] += (double)vv[j][u] * (double)wv[u];
 }
 
 // w -= sum_j coef_j V_j over rows [i, i + N), then MODE 1: acc[j] += <V_j, w>; MODE 2:
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(GS_THREADS) void k_gs_dots(const SegBlk* __restrict
 #pragma unroll
   for (int j = 0; j < KC; ++j) acc[j] = 0.0;
   int64_t i = b.beg + N * threadIdx.x;
-#pragma unroll 1
+#pragma unroll 2
   for (; i + N <= b.end; i += N * GS_THREADS) gs_dots_rows<T, KC, N>(k, V, ldv, w, i, acc);
   if constexpr (N > 1)
     for (; i < b.end; ++i) gs_dots_rows<T, KC, 1>(k, V, ldv, w, i, acc);  // ragged end
