@@ -218,12 +218,13 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int32_t* __restrict_
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = t / ROW_LANES, sub = t % ROW_LANES;
   if (i >= n) return;  // whole groups leave together
+  // the row's own operands are requested before the row product (one fewer dependent
+  // round trip to memory)
+  T ei = ein[i];
+  if (ec) ei += ec[i >> 1];
+  const T bi = b[i], di = diag[i];
   const T ax = level_row_group<T>(i, sub, rp, col, val, ein, ec);
-  if (sub == 0) {
-    T ei = ein[i];
-    if (ec) ei += ec[i >> 1];
-    eout[i] = ei + omega * (b[i] - ax) / diag[i];
-  }
+  if (sub == 0) eout[i] = ei + omega * (bi - ax) / di;
 }
 
 // Last pre-sweep of level l, fused with the restriction to level l+1: the reference
@@ -247,8 +248,9 @@ __global__ __launch_bounds__(256) void k_sweep_restrict(int n, const int32_t* __
   const int i = 2 * I + h;
   T rr = 0, en = 0;
   if (i < n) {  // uniform over the group
-    rr = b[i] - level_row_group<T>(i, sub, rp, col, val, ein, (const T*)nullptr);
-    en = ein[i] + omega * rr / diag[i];
+    const T bi = b[i], ei = ein[i], di = diag[i];  // requested before the row product
+    rr = bi - level_row_group<T>(i, sub, rp, col, val, ein, (const T*)nullptr);
+    en = ei + omega * rr / di;
     if (sub == 0) eout[i] = en;
   }
   // pair sum: lane l of group 0 with lane l of group 1
@@ -704,7 +706,7 @@ __global__ __launch_bounds__(256) void k_dense_rows(int n, const T* __restrict__
   const T* a = ainv + (size_t)row * n;
   T acc = 0;
   // unrolled for loads in flight; the additions keep their order
-#pragma unroll 8
+#pragma unroll 16
   for (int j = lane; j < n; j += 64) acc += a[j] * r[j];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if (lane == 0) out[row] = acc;
