@@ -39,7 +39,7 @@ PEAK_HBM = 8.0e12     # B/s
 # calibrated HBM bytes of one FP32 vector smoothing sweep (k_residual_w<float,3,5,true>) on
 # the 24^3x6 workload: FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3 --pmc passes
 # (profiles/pmc_bytes_apply_tet24.md)
-PMC_SWEEP = 214.5e6
+PMC_SWEEP = 200.3e6
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):
