@@ -1134,7 +1134,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
       mg.sfc[c->comp_voff[q] + i] = pos[c->comp_voff[q] + i] + 1;
     }
   if (nc > 1) DGMI_TRY(build_transfers(pos, total));
-  DGMI_TRY(upload_sfc_levels(c, lay));
+  DGMI_TRY(upload_sfc_levels(c, lay, s));
   mg.small_level = lay.smax;
   mg.small_level32 = lay.start32[lay.order[0]];
   mg.ready = true;

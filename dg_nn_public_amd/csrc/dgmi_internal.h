@@ -331,7 +331,7 @@ struct SfcLayout {
   std::vector<int64_t> len0;      // level-0 segment length (padded except the last)
   int smax = 0;
 };
-int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay);
+int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s);
 template <typename T>
 int set_small_lds_attr(size_t bytes);
 int launch_probe_mask(dgmi_ctx* c, const int32_t* whichc_dev, int color, double* x_dg,

@@ -967,9 +967,10 @@ __global__ void k_d2f_block(int n, int N, const double* __restrict__ a, float* _
 }
 
 // FP32 row-major inverse of the dense level-l operator of one component, formed on the
-// device (synchronous); nullptr on failure.  The matrix is padded to a multiple of GJB
-// with the identity, which leaves the leading block's inverse unchanged.
-static float* dense_inverse_device(dgmi_ctx* c, const SfcLevelsHost& h, int l) {
+// device on stream s (synchronous with s only); nullptr on failure.  The matrix is padded
+// to a multiple of GJB with the identity, which leaves the leading block's inverse
+// unchanged.
+static float* dense_inverse_device(dgmi_ctx* c, const SfcLevelsHost& h, int l, hipStream_t s) {
   const int n = (int)h.n[l];
   const int N = (n + GJB - 1) / GJB * GJB, nb = N / GJB;
   std::vector<double> a((size_t)N * N, 0.0);
@@ -985,23 +986,25 @@ static float* dense_inverse_device(dgmi_ctx* c, const SfcLevelsHost& h, int l) {
             hipMalloc(&dfail, sizeof(int)) == hipSuccess;
   int hfail = 0;
   if (ok) {
-    ok = hipMemcpy(da, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(dfail, &hfail, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    ok = hipMemcpyAsync(da, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, s) ==
+             hipSuccess &&
+         hipMemcpyAsync(dfail, &hfail, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess;
   }
   if (ok) {
     const dim3 blk(256);
     for (int kb = 0; kb < nb; ++kb) {
       const int k0 = kb * GJB;
-      hipLaunchKernelGGL(k_bgj_pivot, dim3(1), blk, 0, 0, N, da, k0, dp, dfail);
-      hipLaunchKernelGGL(k_bgj_row, dim3(nb), blk, 0, 0, N, da, k0, (const double*)dp);
-      hipLaunchKernelGGL(k_bgj_rest, dim3(nb, nb), blk, 0, 0, N, da, k0);
-      hipLaunchKernelGGL(k_bgj_col, dim3(nb), blk, 0, 0, N, da, k0, (const double*)dp);
+      hipLaunchKernelGGL(k_bgj_pivot, dim3(1), blk, 0, s, N, da, k0, dp, dfail);
+      hipLaunchKernelGGL(k_bgj_row, dim3(nb), blk, 0, s, N, da, k0, (const double*)dp);
+      hipLaunchKernelGGL(k_bgj_rest, dim3(nb, nb), blk, 0, s, N, da, k0);
+      hipLaunchKernelGGL(k_bgj_col, dim3(nb), blk, 0, s, N, da, k0, (const double*)dp);
     }
     const int64_t nn = (int64_t)n * n;
-    hipLaunchKernelGGL(k_d2f_block, dim3((unsigned)((nn + 255) / 256)), blk, 0, 0, n, N,
+    hipLaunchKernelGGL(k_d2f_block, dim3((unsigned)((nn + 255) / 256)), blk, 0, s, n, N,
                        (const double*)da, out);
-    ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
-         hipMemcpy(&hfail, dfail, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess && !hfail;
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(&hfail, dfail, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipStreamSynchronize(s) == hipSuccess && !hfail;
   }
   if (da) (void)hipFree(da);
   if (dp) (void)hipFree(dp);
@@ -1044,7 +1047,7 @@ static bool dense_inverse(const SfcLevelsHost& h, int l, std::vector<float>& out
 
 // Device layout of every component's SFC hierarchy (see SfcLevelsDev) and the level
 // tables of the single-work-group kernel.
-int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
+int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
   auto& mg = c->mg;
   auto& lv = mg.lv;
   const int nc = (int)lay.lh.size();
@@ -1251,7 +1254,7 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay) {
       if (h.n[l] <= DENSE_MAX_N) {
         // the level the packed path inverts on the host: reuse that inverse, so that a
         // solve alone and packed with others gives the same bits
-        if (float* inv = l == lvl[0] ? lv.cinv32 : dense_inverse_device(c, h, l)) {
+        if (float* inv = l == lvl[0] ? lv.cinv32 : dense_inverse_device(c, h, l, s)) {
           lv.dense_rm32 = inv;
           lv.dense_n32 = (int)h.n[l];
           lv.dense_level32 = l;
