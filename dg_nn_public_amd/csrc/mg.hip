@@ -337,16 +337,21 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     uint4* dst = reinterpret_cast<uint4*>(smem);
     constexpr int U = 8;
     for (int base = 0; base < nw; base += U * nth) {
-      uint4 t[U];
+      // plain words (a conditionally filled array of vector structs stays in scratch)
+      unsigned int t[U][4];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int i = base + u * nth + tid;
-        if (i < nw) t[u] = src[i];
+        const uint4 v = i < nw ? src[i] : make_uint4(0u, 0u, 0u, 0u);
+        t[u][0] = v.x;
+        t[u][1] = v.y;
+        t[u][2] = v.z;
+        t[u][3] = v.w;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int i = base + u * nth + tid;
-        if (i < nw) dst[i] = t[u];
+        if (i < nw) dst[i] = make_uint4(t[u][0], t[u][1], t[u][2], t[u][3]);
       }
     }
   }
@@ -375,7 +380,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
   // a level of n rows gets G lanes per row (a power of two, at most 16, as many as the
   // work-group can give every row in one pass): the entries of a row are loaded side by
   // side and summed with shuffles, as in the grid kernels
-  auto lanes_for = [&](int n) {
+  auto lanes_for = [=](int n) {
     int g = 1;
     while (g < 16 && 2 * g * n <= nth) g *= 2;
     return g;
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
   // together (the single work-group is latency-bound)
   // level descriptors are copied out of LDS into registers (by value), so the pointer
   // loads are not repeated after every LDS store
-  auto row = [&](const LdsLevel<T> t, int i, int sub, int G, const T* ein, const T* ec) {
+  auto row = [=](const LdsLevel<T> t, int i, int sub, int G, const T* ein, const T* ec) {
     T s = 0;
     int q = t.rp[i] + sub;
     const int q1 = t.rp[i + 1];
@@ -412,7 +417,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
   };
-  auto sweep = [&](const LdsLevel<T> t, const T* ein, const T* ec, T* eout) {
+  auto sweep = [=](const LdsLevel<T> t, const T* ein, const T* ec, T* eout) {
     const int G = lanes_for(t.n), R = nth / G;
     for (int base = 0; base < t.n; base += R) {
       const int i = base + tid / G, sub = tid % G;
