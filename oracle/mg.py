@@ -246,7 +246,7 @@ def mg_smooth_one_level(lv: SfcLevels, level, e, b, omega=JAC_WEI):
 def mg_on_P1CG(r0, lv: SfcLevels, pre=3, post=3, omega=JAC_WEI, exact_level=None):
     """One SFC V-cycle (z02/multigrid_linearelastic.py:112-172).  With exact_level = c the
     cycle stops at level c and solves it exactly (the FP32 cycles of the HIP path do that
-    at their first level of at most 128 rows); None is the reference's cycle."""
+    at one level, which the context reports); None is the reference's cycle."""
     L = lv.nlevel if exact_level is None else exact_level + 1
     r_s = [np.asarray(r0, dtype=np.float64).copy()]
     e_s = [np.zeros(lv.nodes_per_level[0])]

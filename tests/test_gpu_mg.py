@@ -163,3 +163,23 @@ def test_vector_sweeps_with_carried_traces_equal_matrix_core_sweeps():
     err = rel(out["vector"], out["mfma"])
     print("vector vs mfma preconditioner rel diff", err, flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("D,n", [(2, 24), (3, 8)], ids=["2d", "3d"])
+def test_fp32_dense_coarse_level_matches_oracle(D, n):
+    """A single solve whose first SFC level has more than 512 rows solves it with the
+    device Gauss-Jordan inverse (several 64-wide pivot blocks and identity padding),
+    applied by the grid kernel: the FP32 cycle matches the oracle's cycle with the same
+    exact level.  Tolerance 1e-3: an FP32 inverse of a level of ~700 rows whose
+    condition number is ~1e3."""
+    print(f"[case] dense coarse level D={D} n={n}", flush=True)
+    m, pb, g, sp = build(D, n, sigma=0.5)
+    g._get_RAR_and_sfc_data_Um()
+    mgd = mg.build_mg(sp, m.verts)
+    lvl = g.coarse_exact_level
+    assert lvl >= 0 and mgd.levels.nodes_per_level[lvl] > 512
+    r = np.random.default_rng(21).standard_normal(m.nverts)
+    ref = mg.mg_on_P1CG(r, mgd.levels, exact_level=lvl)
+    out = to_np(g.mg_on_P1CG(to_dev(r, torch.float32)))
+    print("dense level", lvl, "rows", mgd.levels.nodes_per_level[lvl], "rel err", rel(out, ref))
+    assert rel(out, ref) < 1e-3
