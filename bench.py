@@ -97,9 +97,9 @@ def main():
     ap.add_argument("--smoother", default="direct", choices=["direct", "jacobian"],
                     help="block smoother: exact block inverse (the reference's variable-"
                          "diffusivity solver) or 3 Jacobi sweeps (its constant-diffusivity one)")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="launch the SFC V-cycle kernels one by one instead of replaying a "
-                         "captured graph")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the SFC V-cycle from a captured HIP graph instead of "
+                         "launching its kernels one by one (the library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
     ap.add_argument("--workload", default="solve",
@@ -135,8 +135,8 @@ def main():
     g = DGDiffusion.from_mesh(mesh, pb.kappa, device=dev, face_kappa=args.face_kappa,
                               blk_solver=args.smoother)
     g._get_RAR_and_sfc_data_Um()
-    if args.no_graph:
-        g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", 0.0)
+    if args.graph:
+        g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", 1.0)
     tuned = g.autotune(reps=5)  # FP32 kernel family and tile shape for this mesh
     torch.cuda.synchronize()
     setup_s = time.time() - t0

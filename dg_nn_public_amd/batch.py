@@ -65,8 +65,9 @@ class DGBatch:
         self.ncomp = len(meshes)
         self.nloc = self.space.nloc
         self.dof_off = p.elem_off * self.nloc
-        if not graph_vcycle:  # launch the V-cycle kernels one by one (no stream capture)
-            self.space.lib.dgmi_set_param(self.space.ctx, b"graph_vcycle", 0.0)
+        # replay the packed V-cycle from a captured graph, or launch its kernels one by one
+        self.space.lib.dgmi_set_param(self.space.ctx, b"graph_vcycle",
+                                      1.0 if graph_vcycle else 0.0)
         self.space._get_RAR_and_sfc_data_Um()
 
     def rhs(self, fs, u_bcs):

@@ -166,8 +166,9 @@ struct MGDev {
   std::vector<int> exact32;  // per component (context order): exact level or -1
   // level sizes of every component (host), components in context order
   std::vector<std::vector<int64_t>> comp_levels;
-  // V-cycle replayed from a captured graph (set_param "graph_vcycle")
-  bool use_graph = true;
+  // V-cycle replayed from a captured graph (set_param "graph_vcycle"; off by default: the
+  // switch between the graph and the stream's own work costs more than the launches)
+  bool use_graph = false;
   VGraph graph64, graph32;
   hipStream_t capture_stream = nullptr;
 };

@@ -162,8 +162,10 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     GMRES (default), 0 = left-preconditioned like the FP64 solve.
  *   "f32_kernel"      kernels of FP32 applies: 0 = vector FMA, 1 = MFMA fp32 (default),
  *                     2 = MFMA with bf16 inputs.
- *   "graph_vcycle"    1 = replay the V-cycle from a captured HIP graph (default), 0 = launch
- *                     its kernels one by one.
+ *   "graph_vcycle"    1 = replay the V-cycle from a captured HIP graph, 0 = launch its
+ *                     kernels one by one on the stream (default: measured 1% faster in a
+ *                     solve, where the switch between a graph and stream work idles the
+ *                     device for ~15 us per cycle).
  *   "smooth_offdiag"  FP32 sweeps of the exact block smoother (DGMI_SMOOTH_DIRECT) inside
  *                     the preconditioner: 1 = x = (1 - omega) x + S (b - A_off x), which
  *                     needs no volume term and only the neighbour half of the face terms

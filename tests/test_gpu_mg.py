@@ -143,7 +143,7 @@ def test_graph_vcycle_replays_equal_eager(prob):
             outs[mode] = res
         for (b0, x0, _), (b1, x1, _) in zip(outs[0.0], outs[1.0]):
             assert torch.equal(x0, x1), dt
-    g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", 1.0)
+    g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", 0.0)
 
 
 def test_vector_sweeps_with_carried_traces_equal_matrix_core_sweeps():
