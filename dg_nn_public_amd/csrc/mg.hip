@@ -227,6 +227,15 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int32_t* __restrict_
   if (sub == 0) eout[i] = ei + omega * (bi - ax) / di;
 }
 
+// First sweep of a cycle, whose input is zero: eout = omega b / diag (no row product).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_zero(int n, const T* __restrict__ diag,
+                                                    const T* __restrict__ b,
+                                                    T* __restrict__ eout, T omega) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) eout[i] = omega * b[i] / diag[i];
+}
+
 // Last pre-sweep of level l, fused with the restriction to level l+1: the reference
 // evaluates rr = b - A e and applies e += omega rr/diag in the same call (:144-146), then
 // restricts rr and e by pairwise sums (:147-153).  Two lane groups per coarse node, one
@@ -740,7 +749,10 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
   auto dia = [&](int l) -> const T* { return f64 ? (const T*)lv.diag64[l] : (const T*)lv.diag32[l]; };
   const dim3 blk(256);
   const dim3 gcp((unsigned)((lv.n[0] + 255) / 256));
-  hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)nullptr, A(0));
+  // the cycle starts from e = 0 on level 0: a grid level 0 takes it in its first sweep,
+  // components whose level 0 is small read it from A(0)
+  if (l0 == 0 || any_start0)
+    hipLaunchKernelGGL((k_copy<T>), gcp, blk, 0, s, lv.n[0], (const T*)nullptr, A(0));
   // down-cycle on grid-sized levels
   for (int l = 0; l < l0; ++l) {
     const int n = (int)grid_rows[l];
@@ -748,8 +760,12 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
     T* cur = A(l);
     T* nxt = B(l);
     for (int it = 0; it < 3; ++it) {
-      hipLaunchKernelGGL((k_sweep<T>), grd, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l), dia(l),
-                         R(l), cur, (const T*)nullptr, nxt, omega);
+      if (l == 0 && it == 0)
+        hipLaunchKernelGGL((k_sweep_zero<T>), dim3((unsigned)((n + 255) / 256)), blk, 0, s, n,
+                           dia(l), (const T*)R(l), nxt, omega);
+      else
+        hipLaunchKernelGGL((k_sweep<T>), grd, blk, 0, s, n, lv.rowptr[l], lv.col[l], val(l),
+                           dia(l), R(l), cur, (const T*)nullptr, nxt, omega);
       std::swap(cur, nxt);
     }
     // after 3 sweeps cur == B(l); 4th sweep writes A(l) and the restricted r, e
