@@ -244,8 +244,9 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
       DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s, s16));
       DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s, s16));
       DGMI_TRY(launch_apply_io(c, x, b, r, 1, 0, s));
-      DGMI_TRY(launch_restrict<T>(c, r, rc, s));
-      DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s));
+      const bool fz = vcycle_fuses_first<T>(c);
+      DGMI_TRY(launch_restrict<T>(c, r, rc, s, fz ? ec : nullptr));
+      DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s, fz));
       DGMI_TRY(launch_prolong_trace(c, ec, x, 0, s));
       DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s, s16));
       DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s, s16));
@@ -265,8 +266,9 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
   DGMI_TRY(launch_smooth_zero<T>(c, g, x, b, s));
   for (int it = 1; it < 3; ++it) DGMI_TRY(launch_smooth<T>(c, g, x, b, nullptr, s));
   DGMI_TRY(launch_apply<T>(c, g, x, b, r, s));
-  DGMI_TRY(launch_restrict<T>(c, r, rc, s));
-  DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s));
+  const bool fz = vcycle_fuses_first<T>(c);
+  DGMI_TRY(launch_restrict<T>(c, r, rc, s, fz ? ec : nullptr));
+  DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s, fz));
   DGMI_TRY(launch_prolong_add<T>(c, ec, x, s));
   // post-smoothing (the reference's loop runs pre_smooth_its times, :2021)
   for (int it = 0; it < 3; ++it) DGMI_TRY(launch_smooth<T>(c, g, x, b, nullptr, s));

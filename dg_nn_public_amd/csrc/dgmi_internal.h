@@ -308,14 +308,18 @@ int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s);
 template <typename T>
 int launch_convert(const double* in, T* out, int64_t n, hipStream_t s);
 // transfers + SFC levels (mg.hip)
+// with e_first set (only when vcycle_fuses_first), the restriction also writes the
+// V-cycle's first level-0 sweep from zero into e_first, the cycle's e_sfc
 template <typename T>
-int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s);
+int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s, T* e_first = nullptr);
+template <typename T> bool vcycle_fuses_first(const dgmi_ctx* c);
 template <typename T>
 int launch_prolong_add(dgmi_ctx* c, const T* e_sfc, T* x_dg, hipStream_t s);
 // FP32 vector family: x += P e with the traces of the new x written into buffer tout
 int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s);
 template <typename T>
-int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s);
+int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s,
+                      bool first_done = false);
 // single-work-group part of the V-cycle: LDS bytes of levels l0.. of one component, the
 // first level from which a component fits in `budget`, and the device level tables
 template <typename T>

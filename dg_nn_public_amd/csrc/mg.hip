@@ -35,13 +35,16 @@ template <int D> __device__ __forceinline__ double Ptab(int n, int k) {
 
 // r_sfc[p] = (1/mult) sum over the DG copies (e,k) of vertex p of sum_n P[n][k] r[e][n].
 // A group of RESTRICT_LANES lanes per vertex takes its copies side by side (about 24 per
-// vertex of a tetrahedral mesh, 20 loads each) and sums them with shuffles.
+// vertex of a tetrahedral mesh, 20 loads each) and sums them with shuffles.  With `e0`
+// set it also takes the V-cycle's first sweep from zero on level 0, e0 = omega r / diag.
 constexpr int RESTRICT_LANES = 8;
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_restrict(int ncg, const int32_t* __restrict__ rptr,
                                                   const int32_t* __restrict__ rent,
                                                   const T* __restrict__ rw,
-                                                  const T* __restrict__ r, T* __restrict__ out) {
+                                                  const T* __restrict__ r, T* __restrict__ out,
+                                                  const T* __restrict__ dg0,
+                                                  T* __restrict__ e0, T omega) {
   constexpr int NL = D == 3 ? 20 : 10, NB = D + 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int p = t / RESTRICT_LANES, sub = t % RESTRICT_LANES;
@@ -58,7 +61,11 @@ __global__ __launch_bounds__(256) void k_restrict(int ncg, const int32_t* __rest
   }
 #pragma unroll
   for (int off = RESTRICT_LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (sub == 0) out[p] = s * rw[p];
+  if (sub == 0) {
+    const T v = s * rw[p];
+    out[p] = v;
+    if (e0) e0[p] = omega * v / dg0[p];
+  }
 }
 
 // x[e][n] += sum_k P[n][k] e_sfc[pos(e,k)] / mult(pos(e,k))
@@ -637,14 +644,26 @@ static T* upload_vec(dgmi_ctx* c, const std::vector<U>& h) {
 }
 
 template <typename T>
-int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s) {
+int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s, T* e_first) {
   const int ncg = (int)c->mg.cg_nonods;
   const dim3 blk(256), grd((unsigned)(((int64_t)ncg * RESTRICT_LANES + 255) / 256));
-  const T* rw = std::is_same<T, double>::value ? (const T*)c->mg.rw64 : (const T*)c->mg.rw32;
+  const bool f64 = std::is_same<T, double>::value;
+  const T* rw = f64 ? (const T*)c->mg.rw64 : (const T*)c->mg.rw32;
+  const T* dg0 = nullptr;
+  if (e_first) {
+    if (!vcycle_fuses_first<T>(c)) {
+      set_error("first sweep not fusable into the restriction");
+      return -2;
+    }
+    dg0 = f64 ? (const T*)c->mg.lv.diag64[0] : (const T*)c->mg.lv.diag32[0];
+  }
+  const T om = (T)c->omega;
   if (c->D == 3)
-    hipLaunchKernelGGL((k_restrict<T, 3>), grd, blk, 0, s, ncg, c->mg.rptr, c->mg.rent, rw, r_dg, r_sfc);
+    hipLaunchKernelGGL((k_restrict<T, 3>), grd, blk, 0, s, ncg, c->mg.rptr, c->mg.rent, rw, r_dg,
+                       r_sfc, dg0, e_first, om);
   else
-    hipLaunchKernelGGL((k_restrict<T, 2>), grd, blk, 0, s, ncg, c->mg.rptr, c->mg.rent, rw, r_dg, r_sfc);
+    hipLaunchKernelGGL((k_restrict<T, 2>), grd, blk, 0, s, ncg, c->mg.rptr, c->mg.rent, rw, r_dg,
+                       r_sfc, dg0, e_first, om);
   DGMI_HIP(hipGetLastError());
   return 0;
 }
@@ -732,14 +751,33 @@ __global__ void k_copy(int64_t n, const T* __restrict__ x, T* __restrict__ y) {
   if (i < n) y[i] = x ? x[i] : T(0);
 }
 
+// Grid-sized levels of a V-cycle (those before the single-work-group or dense end).
+template <typename T> int vcycle_grid_levels(const dgmi_ctx* c) {
+  const auto& mg = c->mg;
+  const bool f64 = std::is_same<T, double>::value;
+  const bool dense = !f64 && mg.coarse_exact && mg.lv.dense_rm32;
+  return f64 ? mg.small_level : (dense ? mg.lv.dense_level32 : mg.small_level32);
+}
+
+// The first sweep from zero can ride on the restriction when level 0 is a grid level of
+// every component and the cycle is launched on the stream.
+template <typename T> bool vcycle_fuses_first(const dgmi_ctx* c) {
+  const auto& lv = c->mg.lv;
+  const bool f64 = std::is_same<T, double>::value;
+  const bool any_start0 = f64 ? lv.any_start0 : lv.any_start0_32;
+  const auto& grid_rows = f64 ? lv.grid_rows : lv.grid_rows32;
+  return !c->mg.use_graph && vcycle_grid_levels<T>(c) > 0 && !any_start0 &&
+         !grid_rows.empty() && grid_rows[0] == c->mg.cg_nonods && lv.n[0] == c->mg.cg_nonods;
+}
+
 template <typename T>
-static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
+static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s, bool first_done) {
   auto& mg = c->mg;
   auto& lv = mg.lv;
   const bool f64 = std::is_same<T, double>::value;
   const T omega = (T)c->omega;
   const bool dense = !f64 && mg.coarse_exact && lv.dense_rm32;
-  const int l0 = f64 ? mg.small_level : (dense ? lv.dense_level32 : mg.small_level32);
+  const int l0 = vcycle_grid_levels<T>(c);
   const std::vector<int64_t>& grid_rows = f64 ? lv.grid_rows : lv.grid_rows32;
   const bool any_start0 = f64 ? lv.any_start0 : lv.any_start0_32;
   auto R = [&](int l) -> T* { return l == 0 ? const_cast<T*>(r_sfc) : vec<T>(lv.r[l]); };
@@ -760,7 +798,9 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
     T* cur = A(l);
     T* nxt = B(l);
     for (int it = 0; it < 3; ++it) {
-      if (l == 0 && it == 0)
+      if (l == 0 && it == 0 && first_done) {
+        // the restriction has written e = omega r / diag into nxt (== e_sfc)
+      } else if (l == 0 && it == 0)
         hipLaunchKernelGGL((k_sweep_zero<T>), dim3((unsigned)((n + 255) / 256)), blk, 0, s, n,
                            dia(l), (const T*)R(l), nxt, omega);
       else
@@ -815,9 +855,13 @@ static int vcycle_body(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
 // once per (precision, r, e, omega) into a graph on a private stream and replayed on the
 // caller's stream.
 template <typename T>
-int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
+int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s, bool first_done) {
   auto& mg = c->mg;
-  if (!mg.use_graph) return vcycle_body<T>(c, r_sfc, e_sfc, s);
+  if (!mg.use_graph) return vcycle_body<T>(c, r_sfc, e_sfc, s, first_done);
+  if (first_done) {
+    set_error("a captured V-cycle takes its own first sweep");
+    return -2;
+  }
   VGraph& vg = std::is_same<T, double>::value ? mg.graph64 : mg.graph32;
   if (!vg.exec || vg.r != (const void*)r_sfc || vg.e != (void*)e_sfc || vg.omega != c->omega) {
     if (vg.exec) {
@@ -827,7 +871,7 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s) {
     if (!mg.capture_stream)
       DGMI_HIP(hipStreamCreateWithFlags(&mg.capture_stream, hipStreamNonBlocking));
     DGMI_HIP(hipStreamBeginCapture(mg.capture_stream, hipStreamCaptureModeRelaxed));
-    const int rc = vcycle_body<T>(c, r_sfc, e_sfc, mg.capture_stream);
+    const int rc = vcycle_body<T>(c, r_sfc, e_sfc, mg.capture_stream, false);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(mg.capture_stream, &graph);
     if (rc) {
@@ -1301,11 +1345,13 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
   return set_small_lds_attr<float>(lds32);
 }
 
-template int launch_restrict<double>(dgmi_ctx*, const double*, double*, hipStream_t);
-template int launch_restrict<float>(dgmi_ctx*, const float*, float*, hipStream_t);
+template int launch_restrict<double>(dgmi_ctx*, const double*, double*, hipStream_t, double*);
+template int launch_restrict<float>(dgmi_ctx*, const float*, float*, hipStream_t, float*);
+template bool vcycle_fuses_first<double>(const dgmi_ctx*);
+template bool vcycle_fuses_first<float>(const dgmi_ctx*);
 template int launch_prolong_add<double>(dgmi_ctx*, const double*, double*, hipStream_t);
 template int launch_prolong_add<float>(dgmi_ctx*, const float*, float*, hipStream_t);
-template int launch_sfc_vcycle<double>(dgmi_ctx*, const double*, double*, hipStream_t);
-template int launch_sfc_vcycle<float>(dgmi_ctx*, const float*, float*, hipStream_t);
+template int launch_sfc_vcycle<double>(dgmi_ctx*, const double*, double*, hipStream_t, bool);
+template int launch_sfc_vcycle<float>(dgmi_ctx*, const float*, float*, hipStream_t, bool);
 
 }  // namespace dgmi
