@@ -121,6 +121,15 @@ def test_grid_level_kernels_match_oracle(monkeypatch):
     b = np.random.default_rng(7).standard_normal(sp.ndof)
     ref = mg.um_left_precond(sp, mgd, b)
     assert rel(to_np(g._um_left_precond(g.new_vector(), to_dev(b))), ref) < 1e-10
+    # FP32 on the same grid levels, no exact level: inside the preconditioner the
+    # restriction also takes the cycle's first level-0 sweep (FP32 tolerance 1e-4)
+    g.lib.dgmi_set_param(g.ctx, b"coarse_exact", 0.0)
+    try:
+        out32 = to_np(g._um_left_precond(g.new_vector(torch.float32), to_dev(b, torch.float32)))
+    finally:
+        g.lib.dgmi_set_param(g.ctx, b"coarse_exact", 1.0)
+    print("grid-level fp32 precond rel err", rel(out32, ref))
+    assert rel(out32, ref) < 1e-4
 
 
 def test_graph_vcycle_replays_equal_eager(prob):
