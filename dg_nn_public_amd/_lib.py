@@ -44,7 +44,7 @@ SIGNATURES = {
     "dgmi_topology": (C.c_int, [_P, _I64P, _I64P, _I64P]),
     "dgmi_element_tables": (C.c_int, [_P, _DP, _DP, _DP, _DP, _DP, _DP, _I64P]),
     "dgmi_apply": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
-    "dgmi_rhs": (C.c_int, [_P, _P, _P, _P, _P]),
+    "dgmi_rhs": (C.c_int, [_P, _P, _P, _P, _P, C.c_double, _P, _P]),
     "dgmi_l2_error": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "dgmi_bdiag": (C.c_int, [_P, _P, _P, _P]),
     "dgmi_smoother_setup": (C.c_int, [_P, C.c_int, C.c_double, _P]),
@@ -58,6 +58,12 @@ SIGNATURES = {
     "dgmi_sfc_vcycle": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "dgmi_precond": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "dgmi_gmres_solve": (C.c_int, [_P, C.c_int, _P, _P, C.c_double, C.c_int, C.c_int, _DP, _P]),
+    "dgmi_cg_solve": (C.c_int, [_P, _P, _P, C.c_double, C.c_double, C.c_int, _DP, _P]),
+    "dgmi_level_smooth": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, _P, C.c_double, _P, _P, _P,
+                                    _P]),
+    "dgmi_level_restrict": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P]),
+    "dgmi_level_prolong_add": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P]),
+    "dgmi_pn_p1dg": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, _P, _P, _P]),
     "dgmi_set_param": (C.c_int, [_P, C.c_char_p, C.c_double]),
     "dgmi_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32), _DP, _P]),
     "dgmi_dot": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, _P]),

@@ -392,10 +392,13 @@ __global__ __launch_bounds__(256) void k_bdiag(int nele, const double* __restric
   bdiag[(size_t)e * NN + nm] = s;
 }
 
-// Right-hand side (z02/volume_mf_diff.py:195-239 volume, :298-357 Dirichlet faces, with the
-// owner's kappa as z05/volume_mf_um_diff_varprop.py:320-342):
-//   b_n = detJ sum_m Mref[n][m] f_m
+// Right-hand side (z02/volume_mf_diff.py:195-239 volume, :298-357 Dirichlet faces with the
+// owner's kappa as z05/volume_mf_um_diff_varprop.py:320-342, :360-400 Neumann faces):
+//   b_n = detJ sum_m Mref[n][m] (f_m + alpha_m / dt)
 //       + sum_{Dirichlet f} wf_f (pen_f sum_m M_f[n][m] u_m - sum_d c_fd sum_m C_fd[m][n] u_m)
+//       + sum_{Neumann f} sdet_f sum_m M_f[n][m] g_m
+// alpha (may be null) is the transient term's nodal data (z02/volume_mf_diff.py:230-238),
+// g (may be null: no Neumann data) the nodal Neumann flux; on Neumann faces wf = sdet.
 template <int D>
 __global__ __launch_bounds__(256) void k_rhs(int nele, const double* __restrict__ refm,
                                              const double* __restrict__ detJ,
@@ -405,6 +408,8 @@ __global__ __launch_bounds__(256) void k_rhs(int nele, const double* __restrict_
                                              const int32_t* __restrict__ fcode,
                                              const double* __restrict__ fsrc,
                                              const double* __restrict__ ubc,
+                                             const double* __restrict__ gneu,
+                                             const double* __restrict__ alpha, double inv_dt,
                                              double* __restrict__ b) {
   using M = Dm<D>;
   constexpr int NL = M::NL, NN = NL * NL;
@@ -419,33 +424,48 @@ __global__ __launch_bounds__(256) void k_rhs(int nele, const double* __restrict_
   const double* ue = ubc + (size_t)e * NL;
   double s = 0.0;
   for (int m = 0; m < NL; ++m) s += Mref[n * NL + m] * fe[m];
+  if (alpha) {
+    const double* ae = alpha + (size_t)e * NL;
+    double t = 0.0;
+    for (int m = 0; m < NL; ++m) t += Mref[n * NL + m] * ae[m];
+    s += t * inv_dt;
+  }
   s *= detJ[e];
   for (int f = 0; f < M::NF; ++f) {
-    if ((fcode[e * M::NF + f] & 15) != 1) continue;
-    double pm = 0.0, cm = 0.0;
-    for (int m = 0; m < NL; ++m) {
-      pm += Mf[f * NN + n * NL + m] * ue[m];
-      double t = 0.0;
+    const int type = fcode[e * M::NF + f] & 15;
+    if (type == 1) {
+      double pm = 0.0, cm = 0.0;
+      for (int m = 0; m < NL; ++m) {
+        pm += Mf[f * NN + n * NL + m] * ue[m];
+        double t = 0.0;
 #pragma unroll
-      for (int d = 0; d < D; ++d) t += cf[(f * D + d) * nele + e] * Cf[(f * D + d) * NN + m * NL + n];
-      cm += t * ue[m];
+        for (int d = 0; d < D; ++d)
+          t += cf[(f * D + d) * nele + e] * Cf[(f * D + d) * NN + m * NL + n];
+        cm += t * ue[m];
+      }
+      s += wf[f * nele + e] * (pen[f * nele + e] * pm - cm);
+    } else if (type == 3 && gneu) {
+      const double* ge = gneu + (size_t)e * NL;
+      double pm = 0.0;
+      for (int m = 0; m < NL; ++m) pm += Mf[f * NN + n * NL + m] * ge[m];
+      s += wf[f * nele + e] * pm;
     }
-    s += wf[f * nele + e] * (pen[f * nele + e] * pm - cm);
   }
   b[tid] = s;
 }
 
 int launch_rhs(dgmi_ctx* c, const double* refm, const double* detJ, const double* f,
-               const double* ubc, double* b, hipStream_t s) {
+               const double* ubc, const double* gneu, const double* alpha, double inv_dt,
+               double* b, hipStream_t s) {
   const int nele = (int)c->nele;
   const int64_t nthreads = (int64_t)nele * c->nloc;
   const dim3 blk(256), grd((unsigned)((nthreads + 255) / 256));
   if (c->D == 3)
     hipLaunchKernelGGL(k_rhs<3>, grd, blk, 0, s, nele, refm, detJ, c->g64.cf, c->g64.wf,
-                       c->g64.pen, c->fcode, f, ubc, b);
+                       c->g64.pen, c->fcode, f, ubc, gneu, alpha, inv_dt, b);
   else
     hipLaunchKernelGGL(k_rhs<2>, grd, blk, 0, s, nele, refm, detJ, c->g64.cf, c->g64.wf,
-                       c->g64.pen, c->fcode, f, ubc, b);
+                       c->g64.pen, c->fcode, f, ubc, gneu, alpha, inv_dt, b);
   DGMI_HIP(hipGetLastError());
   return 0;
 }

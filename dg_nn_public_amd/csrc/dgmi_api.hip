@@ -1,6 +1,7 @@
 // extern "C" entry points of libdgmi.so (declared in include/dgmi.h), context set-up,
 // multigrid set-up, the V-cycle preconditioner and the GMRES driver.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -565,6 +566,168 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
   return 0;
 }
 
+// Workspace of the CG driver: per-component scalars on the device and a pinned ring of
+// per-iteration "components still active" counts that the host reads a little late.
+struct CgWork {
+  int ncomp = 0;
+  double *rz = nullptr, *rznew = nullptr, *pap = nullptr, *rr = nullptr, *thr = nullptr,
+         *tol = nullptr, *bn = nullptr;
+  int32_t *its = nullptr, *active = nullptr, *want = nullptr;
+  double* hd = nullptr;   // pinned: ncomp doubles
+  int32_t* hi = nullptr;  // pinned: 2 ncomp ints
+  static constexpr int NRING = 4;
+  double* ring = nullptr;  // pinned: NRING status slots
+  double* ring_dev = nullptr;
+  hipEvent_t ev[NRING] = {};
+};
+static void free_cg(CgWork* w) {
+  if (!w) return;
+  if (w->hd) (void)hipHostFree(w->hd);
+  if (w->hi) (void)hipHostFree(w->hi);
+  if (w->ring) (void)hipHostFree(w->ring);
+  for (hipEvent_t e : w->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete w;
+}
+static CgWork* cg_work(dgmi_ctx* c) {
+  if (c->cgwork) return static_cast<CgWork*>(c->cgwork);
+  auto* w = new CgWork{};
+  c->cgwork = w;
+  const int nc = c->ncomp;
+  w->ncomp = nc;
+  auto dv = [&](size_t n) { return static_cast<double*>(c->alloc(sizeof(double) * n)); };
+  auto iv = [&](size_t n) { return static_cast<int32_t*>(c->alloc(sizeof(int32_t) * n)); };
+  w->rz = dv(nc);
+  w->rznew = dv(nc);
+  w->pap = dv(nc);
+  w->rr = dv(nc);
+  w->thr = dv(nc);
+  w->tol = dv(nc);
+  w->bn = dv(nc);
+  w->its = iv(nc);
+  w->active = iv(nc);
+  w->want = iv(nc);
+  bool ok = w->rz && w->rznew && w->pap && w->rr && w->thr && w->tol && w->bn && w->its &&
+            w->active && w->want;
+  ok = ok && hipHostMalloc((void**)&w->hd, sizeof(double) * nc) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&w->hi, sizeof(int32_t) * 2 * nc) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&w->ring, sizeof(double) * CgWork::NRING) == hipSuccess;
+  ok = ok && hipHostGetDevicePointer((void**)&w->ring_dev, w->ring, 0) == hipSuccess;
+  for (int i = 0; ok && i < CgWork::NRING; ++i)
+    ok = hipEventCreateWithFlags(&w->ev[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    free_cg(w);
+    c->cgwork = nullptr;
+    return nullptr;
+  }
+  return w;
+}
+
+// MG-preconditioned CG (z02/volume_mf_diff.py:1875-1931) in FP64 for every component:
+// r = b - A x, z = M^-1 r, p = z; each iteration Ap = A p, alpha = (r.z)/(p.Ap),
+// x += alpha p, r -= alpha Ap, z = M^-1 r, beta = (r.z)_new/(r.z), p = z + beta p, until the
+// recurrence residual ||r|| <= thr (abs_tol >= 0: that absolute threshold, as the
+// reference; else rtol ||b||) or max_its.  stats as gmres_solve: its, final true
+// ||b - A x|| / ||b||, 1, 1 per component.
+static int cg_solve(dgmi_ctx* c, const double* b, double* x, double rtol, double abs_tol,
+                    int max_its, double* stats, hipStream_t s) {
+  const int nc = c->ncomp;
+  const SegDev& sg = c->seg;
+  CgWork* w = cg_work(c);
+  DGMI_CHECK(w, alloc_failure("CG workspace"));
+  double* r = vbuf<double>(c, 0);
+  double* z = vbuf<double>(c, 1);
+  double* p = vbuf<double>(c, 2);
+  double* ap = corr_buf<double>(c);
+  DGMI_CHECK(r && z && p && ap, alloc_failure());
+  struct MaskScope {
+    dgmi_ctx* c;
+    ~MaskScope() { c->mask_on = false; }
+  } mask_scope{c};
+  c->mask_on = false;
+  DGMI_TRY(kry::sdot<double>(sg, b, b, nullptr, w->bn, s));
+  DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r, s));  // r = b - A x
+  DGMI_TRY(kry::sdot<double>(sg, r, r, w->rr, nullptr, s));
+  DGMI_HIP(hipMemcpyAsync(w->hd, w->bn, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  DGMI_HIP(hipStreamSynchronize(s));
+  std::vector<double> bnorm(w->hd, w->hd + nc);
+  for (int q = 0; q < nc; ++q) {
+    DGMI_CHECK(std::isfinite(bnorm[q]), "non-finite right-hand side");
+    w->hd[q] = abs_tol >= 0.0 ? abs_tol : rtol * bnorm[q];
+    w->hi[q] = max_its > 0 ? 1 : 0;
+  }
+  DGMI_HIP(hipMemcpyAsync(w->tol, w->hd, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+  DGMI_HIP(hipMemcpyAsync(w->want, w->hi, sizeof(int32_t) * nc, hipMemcpyHostToDevice, s));
+  DGMI_TRY(kry::cg_start(nc, w->rr, w->tol, w->want, w->thr, w->its, w->active, s));
+  if (nc > 1) {
+    c->mask_on = true;
+    DGMI_TRY(kry::expand_active(sg, c->nele, w->active, c->act64, s));
+  }
+  DGMI_TRY(precond<double>(c, r, z, s));  // z = M^-1 r
+  DGMI_HIP(hipMemcpyAsync(p, z, sizeof(double) * c->ndof, hipMemcpyDeviceToDevice, s));
+  DGMI_TRY(kry::sdot<double>(sg, r, z, w->rz, nullptr, s, w->active));
+  // nothing to do when every component starts converged
+  DGMI_HIP(hipMemcpyAsync(w->hi, w->active, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, s));
+  DGMI_HIP(hipStreamSynchronize(s));
+  int nstart = 0;
+  for (int q = 0; q < nc; ++q) nstart += w->hi[q];
+  int known = -1;
+  for (int j = 0; nstart > 0 && j < max_its; ++j) {
+    DGMI_TRY(launch_apply_pos<double>(c, c->g64, p, ap, s));  // Ap
+    DGMI_TRY(kry::sdot<double>(sg, ap, p, w->pap, nullptr, s, w->active));
+    DGMI_TRY(kry::cg_xr<double>(sg, w->rz, w->pap, w->active, p, ap, x, r, s));
+    DGMI_TRY(precond<double>(c, r, z, s));
+    DGMI_TRY(kry::sdot<double>(sg, r, r, w->rr, nullptr, s, w->active));
+    DGMI_TRY(kry::sdot<double>(sg, r, z, w->rznew, nullptr, s, w->active));
+    DGMI_TRY(kry::cg_p<double>(sg, w->rznew, w->rz, w->active, z, p, s));
+    const int slot = j % CgWork::NRING;
+    DGMI_TRY(kry::cg_step(nc, w->rr, w->rz, w->rznew, w->pap, w->thr, max_its, w->its,
+                          w->active, w->ring_dev + slot, s));
+    if (nc > 1) DGMI_TRY(kry::expand_active(sg, c->nele, w->active, c->act64, s));
+    DGMI_HIP(hipEventRecord(w->ev[slot], s));
+    // progress is read back late: the device runs ahead by up to NRING - 1 iterations;
+    // iterations queued after every component stopped change nothing
+    bool stop = false;
+    while (known < j) {
+      const int q = (known + 1) % CgWork::NRING;
+      const bool must = j - known >= CgWork::NRING - 1;
+      if (must) {
+        DGMI_HIP(hipEventSynchronize(w->ev[q]));
+      } else {
+        const hipError_t e = hipEventQuery(w->ev[q]);
+        if (e == hipErrorNotReady) {
+          (void)hipGetLastError();
+          break;
+        }
+        DGMI_HIP(e);
+      }
+      ++known;
+      if (w->ring[q] == 0.0) {
+        stop = true;
+        break;
+      }
+    }
+    if (stop) break;
+  }
+  // true residual of every component
+  c->mask_on = false;
+  DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r, s));
+  DGMI_TRY(kry::sdot<double>(sg, r, r, w->rr, nullptr, s));
+  DGMI_HIP(hipMemcpyAsync(w->hd, w->rr, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+  DGMI_HIP(hipMemcpyAsync(w->hi, w->its, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, s));
+  DGMI_HIP(hipStreamSynchronize(s));
+  for (int q = 0; q < nc; ++q) {
+    DGMI_CHECK(std::isfinite(w->hd[q]), "non-finite residual in component " + std::to_string(q));
+    if (stats) {
+      stats[4 * q + 0] = w->hi[q];
+      stats[4 * q + 1] = bnorm[q] > 0 ? std::sqrt(w->hd[q]) / bnorm[q] : 0.0;
+      stats[4 * q + 2] = 1;
+      stats[4 * q + 3] = 1;
+    }
+  }
+  return 0;
+}
+
 // Segmented blocking of DOF vectors by component (chunks of SEG_CHUNK DOFs).
 static int build_seg(dgmi_ctx* c) {
   SegDev& sg = c->seg;
@@ -752,6 +915,7 @@ int dgmi_ctx_destroy(dgmi_ctx* c) {
   if (c->mg.capture_stream) (void)hipStreamDestroy(c->mg.capture_stream);
   for (void* p : c->allocs) (void)hipFree(p);
   delete static_cast<Scratch*>(c->scratch);
+  free_cg(static_cast<CgWork*>(c->cgwork));
   if (Krylov* k = static_cast<Krylov*>(c->krylov)) {
     if (k->hd) (void)hipHostFree(k->hd);
     if (k->hi) (void)hipHostFree(k->hi);
@@ -831,15 +995,19 @@ int dgmi_apply(dgmi_ctx* c, int dtype, const void* x, const void* b, void* r, vo
   return -2;
 }
 
-int dgmi_rhs(dgmi_ctx* c, const double* f, const double* u_bc, double* b, void* stream) {
+int dgmi_rhs(dgmi_ctx* c, const double* f, const double* u_bc, const double* u_neumann,
+             const double* alpha_x_i, double dt, double* b, void* stream) {
   DGMI_CHECK(c && f && u_bc && b, "null argument");
+  DGMI_CHECK(!alpha_x_i || (dt > 0.0 && std::isfinite(dt)),
+             "the transient term needs a positive, finite dt");
   hipStream_t s = (hipStream_t)stream;
   Scratch& sc = scratch(c);
   if (!sc.refm) {
     sc.refm = upload<double>(c, bdiag_refmats(c->el));
     DGMI_CHECK(sc.refm, alloc_failure());
   }
-  return launch_rhs(c, sc.refm, c->detJ, f, u_bc, b, s);
+  return launch_rhs(c, sc.refm, c->detJ, f, u_bc, u_neumann, alpha_x_i,
+                    alpha_x_i ? 1.0 / dt : 0.0, b, s);
 }
 
 int dgmi_l2_error(dgmi_ctx* c, int dtype, const void* x_num, const void* x_ana, double* out,
@@ -1226,6 +1394,77 @@ int dgmi_gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, double 
   DGMI_TRY(check_align(x));
   Bf16Precond scope(c, inner == DGMI_BF16);
   return gmres_solve(c, inner, b, x, rtol, m, max_its, stats, (hipStream_t)stream);
+}
+
+int dgmi_cg_solve(dgmi_ctx* c, const double* b, double* x, double rtol, double abs_tol,
+                  int max_its, double* stats, void* stream) {
+  DGMI_CHECK(c && b && x, "null argument");
+  DGMI_CHECK(max_its >= 0, "max_its must be non-negative");
+  DGMI_CHECK(c->smoother_variant >= 0, "smoother not set up (dgmi_smoother_setup)");
+  DGMI_CHECK(c->mg.ready, "multigrid not set up (dgmi_mg_setup)");
+  DGMI_TRY(check_align(b));
+  DGMI_TRY(check_align(x));
+  return cg_solve(c, b, x, rtol, abs_tol, max_its, stats, (hipStream_t)stream);
+}
+
+int dgmi_level_smooth(int dtype, int64_t n, const int32_t* rowptr, const int32_t* col,
+                      const void* val, const void* diag, double omega, void* e, const void* b,
+                      void* rr, void* stream) {
+  DGMI_CHECK(rowptr && col && val && diag && e && b && rr, "null argument");
+  DGMI_CHECK(n >= 1 && n <= INT32_MAX / 16, "level size out of range");
+  DGMI_CHECK(rr != e && rr != b, "rr must not alias e or b");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == DGMI_F64)
+    return launch_level_smooth<double>((int)n, rowptr, col, (const double*)val,
+                                       (const double*)diag, omega, (double*)e,
+                                       (const double*)b, (double*)rr, s);
+  if (dtype == DGMI_F32)
+    return launch_level_smooth<float>((int)n, rowptr, col, (const float*)val, (const float*)diag,
+                                      (float)omega, (float*)e, (const float*)b, (float*)rr, s);
+  set_error("dtype not supported");
+  return -2;
+}
+
+int dgmi_level_restrict(int dtype, int64_t n_fine, const void* fine, void* coarse,
+                        void* stream) {
+  DGMI_CHECK(fine && coarse, "null argument");
+  DGMI_CHECK(n_fine >= 1 && n_fine <= INT32_MAX - 1, "level size out of range");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == DGMI_F64)
+    return launch_level_pairsum<double>((int)n_fine, (const double*)fine, (double*)coarse, s);
+  if (dtype == DGMI_F32)
+    return launch_level_pairsum<float>((int)n_fine, (const float*)fine, (float*)coarse, s);
+  set_error("dtype not supported");
+  return -2;
+}
+
+int dgmi_level_prolong_add(int dtype, int64_t n_fine, const void* coarse, void* fine,
+                           void* stream) {
+  DGMI_CHECK(fine && coarse, "null argument");
+  DGMI_CHECK(n_fine >= 1 && n_fine <= INT32_MAX - 1, "level size out of range");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == DGMI_F64)
+    return launch_level_upsample_add<double>((int)n_fine, (const double*)coarse, (double*)fine, s);
+  if (dtype == DGMI_F32)
+    return launch_level_upsample_add<float>((int)n_fine, (const float*)coarse, (float*)fine, s);
+  set_error("dtype not supported");
+  return -2;
+}
+
+int dgmi_pn_p1dg(int D, int dtype, int direction, int64_t nele, const void* x, void* y,
+                 void* stream) {
+  DGMI_CHECK(x && y, "null argument");
+  DGMI_CHECK(D == 2 || D == 3, "D must be 2 or 3");
+  DGMI_CHECK(direction == 0 || direction == 1, "direction must be 0 (to P1DG) or 1 (to PnDG)");
+  DGMI_CHECK(nele >= 1 && nele * 20 <= INT32_MAX, "nele out of range");
+  DGMI_CHECK(x != y, "x and y must not alias");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == DGMI_F64)
+    return launch_pn_p1dg<double>(D, direction, (int)nele, (const double*)x, (double*)y, s);
+  if (dtype == DGMI_F32)
+    return launch_pn_p1dg<float>(D, direction, (int)nele, (const float*)x, (float*)y, s);
+  set_error("dtype not supported");
+  return -2;
 }
 
 int dgmi_autotune(dgmi_ctx* c, int reps, int32_t* choice, double* times_us, void* stream) {

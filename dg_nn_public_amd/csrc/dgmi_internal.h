@@ -204,6 +204,20 @@ int sgivens(int ncomp, int j, int ldh, int64_t hs, double* H, const double* hn, 
             int32_t* active, int32_t* kk, int32_t* nact, double* status, hipStream_t s);
 int sbacksolve(int ncomp, int ldh, int64_t hs, const double* H, const double* g, int64_t gs,
                const int32_t* kk, double* yv, int64_t ys, hipStream_t s);
+// MG-preconditioned CG per component (z02/volume_mf_diff.py:1875-1931): start flags,
+// x += alpha p and r -= alpha Ap, p = z + beta p, and the end-of-iteration step (its,
+// rz = rznew, convergence; status[0] = components still active)
+int cg_start(int ncomp, const double* rr, const double* tol, const int32_t* want, double* thr,
+             int32_t* its, int32_t* active, hipStream_t s);
+template <typename T>
+int cg_xr(const SegDev& sg, const double* rz, const double* pap, const int32_t* active,
+          const T* p, const T* ap, T* x, T* r, hipStream_t s);
+template <typename T>
+int cg_p(const SegDev& sg, const double* rznew, const double* rz, const int32_t* active,
+         const T* z, T* p, hipStream_t s);
+int cg_step(int ncomp, const double* rr, double* rz, const double* rznew, const double* pap,
+            const double* thr, int max_its, int32_t* its, int32_t* active, double* status,
+            hipStream_t s);
 // act64[b] = any component overlapping elements [64b, 64b+64) has flag[c] != 0
 int expand_active(const SegDev& sg, int64_t nele, const int32_t* flag, int32_t* act64,
                   hipStream_t s);
@@ -253,6 +267,7 @@ struct dgmi_ctx {
   bool mask_on = false;
   void* scratch = nullptr;  // solver scratch vectors (dgmi_api.hip)
   void* krylov = nullptr;   // Krylov workspace (dgmi_api.hip)
+  void* cgwork = nullptr;   // CG workspace (dgmi_api.hip)
   dgmi::MfTabs mf;
   double omega = 2.0 / 3.0;
   dgmi::MGDev mg;
@@ -275,8 +290,10 @@ int launch_extract_diag(dgmi_ctx* c, const double* B, double* d, hipStream_t s);
 int build_mfma_tables(dgmi_ctx* c);
 int launch_apply_mf(dgmi_ctx* c, bool bf16, float* x, const float* b, float* r, int mode,
                     hipStream_t s);
+// gneu (Neumann flux data) and alpha (transient term, scaled by inv_dt) may be null
 int launch_rhs(dgmi_ctx* c, const double* refm, const double* detJ, const double* f,
-               const double* ubc, double* b, hipStream_t s);
+               const double* ubc, const double* gneu, const double* alpha, double inv_dt,
+               double* b, hipStream_t s);
 template <typename T>
 int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t s);
 int launch_bdiag(dgmi_ctx* c, const double* refmats, double* bdiag, hipStream_t s);
@@ -339,6 +356,18 @@ struct SfcLayout {
 int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s);
 template <typename T>
 int set_small_lds_attr(size_t bytes);
+// stand-alone level operations on caller-owned CSR levels (mg.hip): one Jacobi step with
+// its residual rr = b - A e, e += omega rr / diag; pairwise restriction (zero pad); nearest
+// upsampling added into the fine vector; P3DG <-> P1DG element transfers (dir 0: to P1DG)
+template <typename T>
+int launch_level_smooth(int n, const int32_t* rp, const int32_t* col, const T* val,
+                        const T* diag, T omega, T* e, const T* b, T* rr, hipStream_t s);
+template <typename T>
+int launch_level_pairsum(int n, const T* fine, T* coarse, hipStream_t s);
+template <typename T>
+int launch_level_upsample_add(int n, const T* coarse, T* fine, hipStream_t s);
+template <typename T>
+int launch_pn_p1dg(int D, int dir, int nele, const T* x, T* y, hipStream_t s);
 int launch_probe_mask(dgmi_ctx* c, const int32_t* whichc_dev, int color, double* x_dg,
                       hipStream_t s);
 int launch_probe_collect(dgmi_ctx* c, const double* r_sfc, const int32_t* spidx,

@@ -524,6 +524,87 @@ __global__ void k_expand_active(int64_t nele, int ncomp, const int64_t* __restri
 }
 
 // ---------------------------------------------------------------------------------------
+// MG-preconditioned conjugate gradients (z02/volume_mf_diff.py:1875-1931), every component
+// at once; a component's scalars (rz = r.z, pap = p.Ap, rr = r.r) live in device arrays,
+// and finished components (active[c] = 0) are left untouched by the vector updates.
+// x += alpha p, r -= alpha Ap with alpha = rz / pap
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_cg_xr(const SegBlk* __restrict__ sb,
+                                                       const double* __restrict__ rz,
+                                                       const double* __restrict__ pap,
+                                                       const int32_t* __restrict__ active,
+                                                       const T* __restrict__ p,
+                                                       const T* __restrict__ ap,
+                                                       T* __restrict__ x, T* __restrict__ r) {
+  const SegBlk b = sb[blockIdx.x];
+  if (!active[b.comp]) return;
+  const double a = rz[b.comp] / pap[b.comp];
+  if (!isfinite(a)) return;  // breakdown: k_cg_step stops the component
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) {
+    x[i] = (T)((double)x[i] + a * (double)p[i]);
+    r[i] = (T)((double)r[i] - a * (double)ap[i]);
+  }
+}
+
+// p = z + beta p with beta = rznew / rz
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_cg_p(const SegBlk* __restrict__ sb,
+                                                      const double* __restrict__ rznew,
+                                                      const double* __restrict__ rz,
+                                                      const int32_t* __restrict__ active,
+                                                      const T* __restrict__ z,
+                                                      T* __restrict__ p) {
+  const SegBlk b = sb[blockIdx.x];
+  if (!active[b.comp]) return;
+  const double beta = rznew[b.comp] / rz[b.comp];
+  if (!isfinite(beta)) return;
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x)
+    p[i] = (T)((double)z[i] + beta * (double)p[i]);
+}
+
+// End of an iteration, one work-group: its[c] += 1, rz[c] = rznew[c], and the component
+// stays active while sqrt(rr[c]) > thr[c], its[c] < max_its and alpha was finite.
+// status[0] = the number of components still active.
+__global__ __launch_bounds__(GIV_THREADS) void k_cg_step(
+    int ncomp, const double* __restrict__ rr, double* __restrict__ rz,
+    const double* __restrict__ rznew, const double* __restrict__ pap,
+    const double* __restrict__ thr, int max_its, int32_t* __restrict__ its,
+    int32_t* __restrict__ active, double* __restrict__ status) {
+  int cnt = 0;
+  for (int c = threadIdx.x; c < ncomp; c += blockDim.x) {
+    if (!active[c]) continue;
+    const double a = rz[c] / pap[c];
+    const int k = its[c] + 1;
+    its[c] = k;
+    rz[c] = rznew[c];
+    const bool on = isfinite(a) && sqrt(rr[c]) > thr[c] && k < max_its;
+    active[c] = on ? 1 : 0;
+    cnt += on;
+  }
+  __shared__ int sh[GIV_THREADS / 64];
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < GIV_THREADS / 64; ++w) t += sh[w];
+    status[0] = (double)t;
+  }
+}
+
+// Start: active[c] = want[c] and sqrt(rr[c]) > thr[c] = tol[c]; its[c] = 0.
+__global__ void k_cg_start(int ncomp, const double* __restrict__ rr,
+                           const double* __restrict__ tol, const int32_t* __restrict__ want,
+                           double* __restrict__ thr, int32_t* __restrict__ its,
+                           int32_t* __restrict__ active) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncomp) return;
+  thr[c] = tol[c];
+  its[c] = 0;
+  active[c] = (want[c] && sqrt(rr[c]) > tol[c]) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------------
 // partial sums of the context-free dot (one buffer per host thread, so calls from
 // different threads on different streams never share it)
 static thread_local double* g_partial = nullptr;
@@ -711,6 +792,45 @@ int expand_active(const SegDev& sg, int64_t nele, const int32_t* flag, int32_t* 
   return 0;
 }
 
+int cg_start(int ncomp, const double* rr, const double* tol, const int32_t* want, double* thr,
+             int32_t* its, int32_t* active, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_start, dim3((ncomp + 63) / 64), dim3(64), 0, s, ncomp, rr, tol, want,
+                     thr, its, active);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int cg_xr(const SegDev& sg, const double* rz, const double* pap, const int32_t* active,
+          const T* p, const T* ap, T* x, T* r, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_xr<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, rz, pap,
+                     active, p, ap, x, r);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int cg_p(const SegDev& sg, const double* rznew, const double* rz, const int32_t* active,
+         const T* z, T* p, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_p<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, rznew, rz,
+                     active, z, p);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+int cg_step(int ncomp, const double* rr, double* rz, const double* rznew, const double* pap,
+            const double* thr, int max_its, int32_t* its, int32_t* active, double* status,
+            hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_step, dim3(1), dim3(GIV_THREADS), 0, s, ncomp, rr, rz, rznew, pap, thr,
+                     max_its, its, active, status);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template int cg_xr<double>(const SegDev&, const double*, const double*, const int32_t*,
+                           const double*, const double*, double*, double*, hipStream_t);
+template int cg_p<double>(const SegDev&, const double*, const double*, const int32_t*,
+                          const double*, double*, hipStream_t);
 template int sdot<double>(const SegDev&, const double*, const double*, double*, double*,
                           hipStream_t, const int32_t*);
 template int sdot<float>(const SegDev&, const float*, const float*, double*, double*, hipStream_t,

@@ -1345,6 +1345,145 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
   return set_small_lds_attr<float>(lds32);
 }
 
+// ---------------------------------------------------------------------------------------
+// Stand-alone level operations on caller-owned CSR levels and vectors: the pieces of the
+// reference's Python V-cycle (z02/multigrid_linearelastic.py:25-39, 112-172) for a caller
+// that drives the cycle itself.  The context's own V-cycle fuses them (above).
+// rr = b - A e (row groups as k_sweep)
+template <typename T>
+__global__ __launch_bounds__(256) void k_level_rr(int n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col,
+                                                  const T* __restrict__ val,
+                                                  const T* __restrict__ b,
+                                                  const T* __restrict__ e,
+                                                  T* __restrict__ rr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / ROW_LANES, sub = t % ROW_LANES;
+  if (i >= n) return;  // whole groups leave together
+  const T bi = b[i];
+  const T ax = level_row_group<T>(i, sub, rp, col, val, e, (const T*)nullptr);
+  if (sub == 0) rr[i] = bi - ax;
+}
+
+// e += omega rr / diag
+template <typename T>
+__global__ __launch_bounds__(256) void k_level_update(int n, const T* __restrict__ diag,
+                                                      const T* __restrict__ rr,
+                                                      T* __restrict__ e, T omega) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) e[i] += rr[i] / diag[i] * omega;
+}
+
+// coarse[I] = fine[2I] + fine[2I+1], the fine vector padded with one zero
+// (F.pad + conv1d [1, 1] stride 2, :147-153)
+template <typename T>
+__global__ __launch_bounds__(256) void k_level_pairsum(int n, const T* __restrict__ fine,
+                                                       T* __restrict__ coarse) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = (n - 1) / 2 + 1;
+  if (I >= nc) return;
+  T s = fine[2 * I];
+  if (2 * I + 1 < n) s += fine[2 * I + 1];
+  coarse[I] = s;
+}
+
+// fine[i] += coarse[i >> 1] (nearest upsampling, :163-164)
+template <typename T>
+__global__ __launch_bounds__(256) void k_level_upsample_add(int n, const T* __restrict__ coarse,
+                                                            T* __restrict__ fine) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) fine[i] += coarse[i >> 1];
+}
+
+// P3DG -> P1DG per element, y[e][k] = sum_n P[n][k] x[e][n]
+// (vel_pndg_to_p1dg_restrictor, z02/multigrid_linearelastic.py:286-290)
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_pn_to_p1dg(int nele, const T* __restrict__ x,
+                                                    T* __restrict__ y) {
+  constexpr int NL = D == 3 ? 20 : 10, NB = D + 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nele * NB) return;
+  const int e = t / NB, k = t - (t / NB) * NB;
+  const T* xe = x + (size_t)e * NL;
+  T s = 0;
+#pragma unroll
+  for (int n = 0; n < NL; ++n) s += (T)Ptab<D>(n, k) * xe[n];
+  y[t] = s;
+}
+
+// P1DG -> P3DG per element, y[e][n] = sum_k P[n][k] x[e][k]
+// (vel_p1dg_to_pndg_prolongator, :293-297)
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_p1dg_to_pn(int nele, const T* __restrict__ x,
+                                                    T* __restrict__ y) {
+  constexpr int NL = D == 3 ? 20 : 10, NB = D + 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nele * NL) return;
+  const int e = t / NL, n = t - (t / NL) * NL;
+  const T* xe = x + (size_t)e * NB;
+  T s = 0;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) s += (T)Ptab<D>(n, k) * xe[k];
+  y[t] = s;
+}
+
+template <typename T>
+int launch_level_smooth(int n, const int32_t* rp, const int32_t* col, const T* val,
+                        const T* diag, T omega, T* e, const T* b, T* rr, hipStream_t s) {
+  const int64_t th = (int64_t)n * ROW_LANES;
+  hipLaunchKernelGGL((k_level_rr<T>), dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, n,
+                     rp, col, val, b, (const T*)e, rr);
+  hipLaunchKernelGGL((k_level_update<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                     diag, (const T*)rr, e, omega);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_level_pairsum(int n, const T* fine, T* coarse, hipStream_t s) {
+  const int nc = (n - 1) / 2 + 1;
+  hipLaunchKernelGGL((k_level_pairsum<T>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s,
+                     n, fine, coarse);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_level_upsample_add(int n, const T* coarse, T* fine, hipStream_t s) {
+  hipLaunchKernelGGL((k_level_upsample_add<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     s, n, coarse, fine);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_pn_p1dg(int D, int dir, int nele, const T* x, T* y, hipStream_t s) {
+  const int64_t nt = (int64_t)nele * (dir == 0 ? D + 1 : (D == 3 ? 20 : 10));
+  const dim3 g((unsigned)((nt + 255) / 256)), b(256);
+  if (D == 3) {
+    if (dir == 0) hipLaunchKernelGGL((k_pn_to_p1dg<T, 3>), g, b, 0, s, nele, x, y);
+    else hipLaunchKernelGGL((k_p1dg_to_pn<T, 3>), g, b, 0, s, nele, x, y);
+  } else {
+    if (dir == 0) hipLaunchKernelGGL((k_pn_to_p1dg<T, 2>), g, b, 0, s, nele, x, y);
+    else hipLaunchKernelGGL((k_p1dg_to_pn<T, 2>), g, b, 0, s, nele, x, y);
+  }
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template int launch_level_smooth<double>(int, const int32_t*, const int32_t*, const double*,
+                                         const double*, double, double*, const double*, double*,
+                                         hipStream_t);
+template int launch_level_smooth<float>(int, const int32_t*, const int32_t*, const float*,
+                                        const float*, float, float*, const float*, float*,
+                                        hipStream_t);
+template int launch_level_pairsum<double>(int, const double*, double*, hipStream_t);
+template int launch_level_pairsum<float>(int, const float*, float*, hipStream_t);
+template int launch_level_upsample_add<double>(int, const double*, double*, hipStream_t);
+template int launch_level_upsample_add<float>(int, const float*, float*, hipStream_t);
+template int launch_pn_p1dg<double>(int, int, int, const double*, double*, hipStream_t);
+template int launch_pn_p1dg<float>(int, int, int, const float*, float*, hipStream_t);
+
 template int launch_restrict<double>(dgmi_ctx*, const double*, double*, hipStream_t, double*);
 template int launch_restrict<float>(dgmi_ctx*, const float*, float*, hipStream_t, float*);
 template bool vcycle_fuses_first<double>(const dgmi_ctx*);

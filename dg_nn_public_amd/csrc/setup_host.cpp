@@ -495,6 +495,11 @@ bool build_geometry(const ElementTables& el, const double* x, const double* kapp
       } else if (type == 0) {
         kf = ke;
         g.fcode[z] = 1;
+      } else if (type == 1) {
+        // Neumann: no operator term; the right-hand side adds int v g_N without kappa
+        // (z02/volume_mf_diff.py:360-400)
+        kf = 1.0;
+        g.fcode[z] = 3;
       } else {
         kf = 0.0;
         g.fcode[z] = 2;

@@ -81,11 +81,14 @@ struct Geometry {
   std::vector<double> G;
   // c_f = invJ^T n_f : (nface*D) comps
   std::vector<double> cf;
-  // wf = kappa_f * sdet_f (0 on faces that carry no operator term), pen = eta/h : nface comps
+  // wf = kappa_f * sdet_f on interior and Dirichlet faces, sdet_f on Neumann faces (their
+  // right-hand-side term carries no kappa), 0 on faces without a term; pen = eta/h
   std::vector<double> wf, pen;
   // face record: neighbour global face (N*nface + f') or -1; alignment; type
   std::vector<int32_t> nbface;  // (nele*nface) element-major
-  std::vector<int32_t> fcode;   // (nele*nface): (type+1) | (alnmt << 4); type -1 int, 0 Dirichlet
+  // (nele*nface): code | (alnmt << 4) with code 0 interior (bctype -1), 1 Dirichlet (0),
+  // 3 Neumann (1), 2 any other type (no term, as the reference's FSI interface faces)
+  std::vector<int32_t> fcode;
   std::vector<double> detJ;     // (nele)
   double max_affine_dev = 0.0;  // max |x_node - affine position| / h
 };

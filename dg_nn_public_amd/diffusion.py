@@ -10,11 +10,16 @@ names, argument meaning, in-place mutation and return values:
   get_bdiag_diag() -> (diagK, bdiagK)      z02/get_bdiag_diag.py:30-62
   _get_RAR_and_sfc_data_Um()               z02/volume_mf_diff.py:1642-1693
   _um_left_precond(x_i, x_rhs) -> x_i      z02/volume_mf_diff.py:1934-2029
-  mg_on_P1CG(r0_sfc) -> e_sfc              z02/multigrid_linearelastic.py:112-172
+  mg_on_P1CG(r0, variables_sfc, nlevel, nodes_per_level, cg_nonods) -> e
+                                          z02/multigrid_linearelastic.py:112-172
+  mg_smooth_one_level(level, e_i, b, variables_sfc) -> (e_i, rr_i)   :25-39
+  mg_on_P1CG_prep(fina, cola, RARvalues, sparse_in)
+      -> (sfc, variables_sfc, nlevel, nodes_per_level)               :175-283
+  vel_pndg_to_p1dg_restrictor(x), vel_p1dg_to_pndg_prolongator(x)    :286-297
   _gmres_mg_solver(x_i, x_rhs, tol) -> (x_i, its)   z02/volume_mf_diff.py:1757-1841
   _mg_solver / _cg_mg_solver               z02/volume_mf_diff.py:1844-1931
-  _get_rhs(x_rhs, f, u_bc) -> x_rhs        z02/volume_mf_diff.py:153-192
-  solve_for_diff(x_i, f, u_bc, alpha_x_i, t=None) -> x_i   z02/volume_mf_diff.py:33-55
+  _get_rhs(x_rhs, f, u_bc, alpha_x_i) -> x_rhs      z02/volume_mf_diff.py:153-400
+  solve_for_diff(x_i, f, u_bc, alpha_x_i, t=None) -> x_i   z02/volume_mf_diff.py:33-148
   get_l2_error(x_num, x_ana) -> (l2, linf) z02/volume_mf_diff.py:2032-2056
 
 Module-level functions of the same names act on the space installed with
@@ -36,6 +41,9 @@ ETA_E = 36.0        # z02/config.py:245
 JAC_WEI = 2.0 / 3.0  # z02/config.py:108
 GMRES_M = 30        # restart length of the workloads (the reference config uses 10, :149)
 GMRES_ITS = 400     # z02/config.py:150
+PRE_SMOOTH_ITS = 3   # z02/config.py:112
+POST_SMOOTH_ITS = 3  # z02/config.py:113
+LINEAR_SOLVERS = ("gmres-mg", "cg-mg", "mg")  # z02/config.py:95
 
 _DTYPES = {torch.float64: _lib.DGMI_F64, torch.float32: _lib.DGMI_F32}
 # inner precision of the iterative-refinement solve; bfloat16 means FP32 Krylov vectors and
@@ -54,18 +62,95 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+class LevelCSR:
+    """One SFC level operator on the device: CSR with int32 ``crow``/``col`` and values in
+    the level's dtype (the reference keeps a torch sparse CSR tensor here,
+    z02/multigrid_linearelastic.py:93-99)."""
+
+    def __init__(self, crow, col, values, n):
+        self.crow, self.col, self.values, self.n = crow, col, values, int(n)
+
+    def to(self, dtype):
+        return LevelCSR(self.crow, self.col, self.values.to(dtype), self.n)
+
+
+class SfcData:
+    """The reference's ``sf_nd_nb.sfc_data_Um``: SFC numbering (1-based, shape (n, 1) as
+    the Fortran curve returns it), level operators, level count and sizes."""
+
+    def __init__(self, space_filling_curve_numbering, variables_sfc, nlevel, nodes_per_level):
+        self.space_filling_curve_numbering = space_filling_curve_numbering
+        self.variables_sfc = variables_sfc
+        self.nlevel = nlevel
+        self.nodes_per_level = nodes_per_level
+
+
+def _levels_to_device(levels, device):
+    """[(rowptr, col, val)] numpy per level -> variables_sfc list of
+    (LevelCSR, diag (1, n), n), the reference's (a_sfc_sparse, diagA, nonods)."""
+    out = []
+    for rp, col, val in levels:
+        n = rp.size - 1
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        diag = np.zeros(n)
+        on = col == rows
+        diag[rows[on]] = val[on]
+        csr = LevelCSR(torch.as_tensor(rp.astype(np.int32), device=device),
+                       torch.as_tensor(col.astype(np.int32), device=device),
+                       torch.as_tensor(val, dtype=torch.float64, device=device), n)
+        out.append((csr, torch.as_tensor(diag, device=device).view(1, n), n))
+    return out
+
+
+class _ContextLevels:
+    """variables_sfc of a context's own hierarchy: built on first access from the exported
+    RAR and SFC numbering (the same host builder the context used); ``mg_on_P1CG`` given
+    this object runs the context's fused V-cycle."""
+
+    def __init__(self, space):
+        self._space = space
+        self._levels = None
+
+    def _get(self):
+        if self._levels is None:
+            sp = self._space
+            if sp.ncomp != 1:
+                raise ValueError("the level operators of a packed context are not exported")
+            from . import host
+            fina, cola, rar, sfc = sp.mg_export()
+            self._levels = _levels_to_device(host.sfc_levels(fina, cola, rar, sfc), sp.device)
+        return self._levels
+
+    def __len__(self):
+        return self._space.nlevel
+
+    def __getitem__(self, i):
+        return self._get()[i]
+
+    def __iter__(self):
+        return iter(self._get())
+
+
 class DGDiffusion:
     """One P3 DG mesh, its SIPG operator and SFC multigrid hierarchy on one GPU."""
 
     def __init__(self, x_ref_in, cells, verts, kappa=None, bctype=None, eta_e: float = ETA_E,
                  blk_solver: str = "jacobian", jac_wei: float = JAC_WEI, device="cuda:0",
-                 face_kappa: str = "mean", components=None):
+                 face_kappa: str = "mean", components=None, is_transient: bool = False,
+                 dt: float | None = None, linear_solver: str = "gmres-mg"):
         """face_kappa: "mean" is the reference's interior-face diffusivity (kE + kN)/2;
         "harmonic" uses 2 kE kN/(kE + kN) (symmetric weighted interior penalty), which
         keeps the operator positive definite at high contrast.  ``components`` =
         (elem_off, vert_off) packs independent solves into this one mesh (see
         ``dg_nn_public_amd.batch``): each keeps its own multigrid hierarchy, Krylov space and
-        convergence test, while every kernel launch covers all of them."""
+        convergence test, while every kernel launch covers all of them.
+        ``is_transient`` / ``dt``: the reference's ``sf_nd_nb.isTransient`` and ``dt``
+        (z02/config.py:33-36, 62-63): when transient, ``_get_rhs`` adds M alpha_x_i / dt.
+        ``linear_solver``: the solver ``solve_for_diff`` runs (z02/config.py:95, :137-148)."""
+        if linear_solver not in LINEAR_SOLVERS:
+            raise ValueError(f"linear_solver must be one of {LINEAR_SOLVERS}")
+        if is_transient and not (dt is not None and dt > 0 and np.isfinite(dt)):
+            raise ValueError("a transient space needs a positive, finite dt")
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -126,6 +211,11 @@ class DGDiffusion:
         self.blk_solver = blk_solver
         self.face_kappa = face_kappa
         self.jac_wei = jac_wei
+        self.isTransient = bool(is_transient)
+        self.dt = dt
+        self.linear_solver = linear_solver
+        self.bctype = bc if bc is not None else self.topology()[2]
+        self.has_neumann = bool(np.any(self.bctype == 1))
         self._smoother_ready = False
         self.diagK = None
         self.bdiagK = None
@@ -265,7 +355,8 @@ class DGDiffusion:
 
     # ---------------------------------------------------------------- multigrid
     def _get_RAR_and_sfc_data_Um(self):
-        """P1CG operator RAR by coloured probing + SFC level operators."""
+        """P1CG operator RAR by coloured probing + SFC level operators
+        (z02/volume_mf_diff.py:1642-1693); sets ``sfc_data_Um`` as the reference does."""
         self._ensure_smoother()
         check(self.lib.dgmi_mg_setup(self.ctx, _stream()), "dgmi_mg_setup")
         info = (C.c_int64 * 128)()
@@ -275,6 +366,11 @@ class DGDiffusion:
         # FP32 cycles: level solved exactly (-1: none)
         self.coarse_exact_level = int(info[4 + self.nlevel])
         self._mg_ready = True
+        self._sfc = None
+        self.sfc_data_Um = SfcData(None, _ContextLevels(self), self.nlevel,
+                                   list(self.nodes_per_level))
+        if self.ncomp == 1:
+            self.sfc_data_Um.space_filling_curve_numbering = self.mg_export()[3].reshape(-1, 1)
         return 0
 
     def mg_export(self):
@@ -292,6 +388,20 @@ class DGDiffusion:
         if not getattr(self, "_mg_ready", False):
             self._get_RAR_and_sfc_data_Um()
 
+    def _cg_vec(self, t, name, dtypes=(torch.float64, torch.float32), n=None):
+        """A P1CG / SFC-level vector: device tensor on this space's device, contiguous, n
+        values (cg_nonods by default), one of ``dtypes``."""
+        n = self.cg_nonods if n is None else n
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{name} must be a torch tensor")
+        if t.device.type != "cuda" or (self.device.index is not None and t.device != self.device):
+            raise ValueError(f"{name} must live on {self.device}")
+        if t.dtype not in dtypes:
+            raise ValueError(f"{name} has dtype {t.dtype}; expected one of {dtypes}")
+        if t.numel() != n or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous tensor of {n} values")
+        return t
+
     def restrict_to_p1cg(self, r):
         """P3DG -> P1CG in SFC order (I_cf P^T, permuted)."""
         self._need_mg()
@@ -305,19 +415,133 @@ class DGDiffusion:
         """x += P I_fc e (e in SFC order)."""
         self._need_mg()
         self._vec(x, "x")
+        self._cg_vec(e_sfc, "e_sfc", (x.dtype,))
         check(self.lib.dgmi_prolong_add(self.ctx, _DTYPES[x.dtype], _ptr(e_sfc), _ptr(x),
                                         _stream()), "dgmi_prolong_add")
         return x
 
-    def mg_on_P1CG(self, r0_sfc):
-        """One SFC V-cycle, r and e in SFC order (z02/multigrid_linearelastic.py:112-172)."""
-        self._need_mg()
-        if r0_sfc.numel() != self.cg_nonods or r0_sfc.dtype not in _DTYPES:
-            raise ValueError("r0 must hold cg_nonods values of float64/float32")
-        e = torch.empty_like(r0_sfc)
-        check(self.lib.dgmi_sfc_vcycle(self.ctx, _DTYPES[r0_sfc.dtype], _ptr(r0_sfc), _ptr(e),
-                                       _stream()), "dgmi_sfc_vcycle")
-        return e
+    def vel_pndg_to_p1dg_restrictor(self, x):
+        """y[e, k] = sum_n P[n, k] x[e, n]: P3DG -> P1DG per element
+        (z02/multigrid_linearelastic.py:286-290)."""
+        x = self._vec(x.reshape(-1), "x")
+        y = torch.empty(self.nele * (self.D + 1), dtype=x.dtype, device=self.device)
+        check(self.lib.dgmi_pn_p1dg(self.D, _DTYPES[x.dtype], 0, self.nele, _ptr(x), _ptr(y),
+                                    _stream()), "dgmi_pn_p1dg")
+        return y
+
+    def vel_p1dg_to_pndg_prolongator(self, x):
+        """y[e, n] = sum_k P[n, k] x[e, k]: P1DG -> P3DG per element (:293-297)."""
+        x = self._cg_vec(x.reshape(-1), "x", n=self.nele * (self.D + 1))
+        y = torch.empty(self.ndof, dtype=x.dtype, device=self.device)
+        check(self.lib.dgmi_pn_p1dg(self.D, _DTYPES[x.dtype], 1, self.nele, _ptr(x), _ptr(y),
+                                    _stream()), "dgmi_pn_p1dg")
+        return y
+
+    def mg_smooth_one_level(self, level, e_i, b, variables_sfc=None):
+        """rr = b - A_l e; e += jac_wei rr / diag(A_l) on SFC level ``level``
+        (z02/multigrid_linearelastic.py:25-39).  ``variables_sfc`` as returned by
+        ``mg_on_P1CG_prep`` (default: this space's own levels).  Returns (e_i, rr_i), e_i
+        updated in place."""
+        if variables_sfc is None:
+            self._need_mg()
+            variables_sfc = self.sfc_data_Um.variables_sfc
+        a, diag, n = variables_sfc[level]
+        e = e_i.reshape(-1)
+        bb = b.reshape(-1)
+        self._cg_vec(e, "e_i", n=n)
+        self._cg_vec(bb, "b", (e.dtype,), n=n)
+        if a.values.dtype != e.dtype:
+            a = a.to(e.dtype)
+            diag = diag.to(e.dtype)
+        rr = torch.empty_like(e)
+        check(self.lib.dgmi_level_smooth(_DTYPES[e.dtype], n, _ptr(a.crow), _ptr(a.col),
+                                         _ptr(a.values), _ptr(diag), self.jac_wei, _ptr(e),
+                                         _ptr(bb), _ptr(rr), _stream()), "dgmi_level_smooth")
+        return e_i, rr
+
+    def mg_on_P1CG_prep(self, fina, cola, RARvalues, sparse_in=None):
+        """SFC numbering and level operators of a P1CG matrix given by (fina, cola,
+        RARvalues) (z02/multigrid_linearelastic.py:175-283).  The numbering is this build's
+        Hilbert order of the P1 vertices (``sparse_in`` may supply ``sfc``, 1-based, or
+        ``verts``; default: this space's vertices).  Returns (sfc (n, 1), variables_sfc,
+        nlevel, nodes_per_level) with variables_sfc[l] = (LevelCSR, diag (1, n_l), n_l)."""
+        from . import host
+        to_np = lambda a: a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)  # noqa: E731
+        fina = np.ascontiguousarray(to_np(fina), dtype=np.int64)
+        cola = np.ascontiguousarray(to_np(cola), dtype=np.int64)
+        vals = np.ascontiguousarray(to_np(RARvalues), dtype=np.float64).reshape(-1)
+        if vals.size != cola.size or fina[-1] != cola.size:
+            raise ValueError("fina, cola and RARvalues do not describe one CSR matrix")
+        n = fina.size - 1
+        src = self if sparse_in is None else sparse_in
+        sfc = getattr(src, "sfc", None)
+        if sfc is None:
+            verts = getattr(src, "verts", None)
+            if verts is None:
+                raise ValueError("sparse_in must provide sfc or verts")
+            sfc = host.sfc_order(np.asarray(verts, dtype=np.float64))
+        sfc = np.ascontiguousarray(np.asarray(sfc, dtype=np.int64).reshape(-1))
+        if sfc.size != n:
+            raise ValueError("the SFC numbering must have one entry per P1CG node")
+        levels = host.sfc_levels(fina, cola, vals, sfc)
+        variables_sfc = _levels_to_device(levels, self.device)
+        nodes = [v[2] for v in variables_sfc]
+        return sfc.reshape(-1, 1), variables_sfc, len(nodes), nodes
+
+    def mg_on_P1CG(self, r0, variables_sfc=None, nlevel=None, nodes_per_level=None,
+                   cg_nonods=None):
+        """One SFC V-cycle, r and e in SFC order (z02/multigrid_linearelastic.py:112-172).
+        With this space's own levels (the default, or ``sfc_data_Um.variables_sfc``) the
+        cycle runs fused in the library; with levels from ``mg_on_P1CG_prep`` it runs the
+        reference's steps one call each: pre-smoothing, the residual sweep (which also
+        updates e, as the reference's call does), pairwise restriction of r and e, the
+        one-node direct solve, nearest upsampling and post-smoothing."""
+        own = variables_sfc is None or isinstance(variables_sfc, _ContextLevels)
+        if own:
+            self._need_mg()
+            r0 = self._cg_vec(r0.reshape(-1), "r0")
+            e = torch.empty_like(r0)
+            check(self.lib.dgmi_sfc_vcycle(self.ctx, _DTYPES[r0.dtype], _ptr(r0), _ptr(e),
+                                           _stream()), "dgmi_sfc_vcycle")
+            return e
+        nlevel = len(variables_sfc) if nlevel is None else int(nlevel)
+        nodes = [v[2] for v in variables_sfc] if nodes_per_level is None else list(nodes_per_level)
+        n0 = nodes[0] if cg_nonods is None else int(cg_nonods)
+        r0 = self._cg_vec(r0.reshape(-1), "r0", n=n0)
+        dt = _DTYPES[r0.dtype]
+        start = nlevel - 1
+        r_s = [r0]
+        e_s = [torch.zeros_like(r0)]
+        for level in range(start):
+            for _ in range(PRE_SMOOTH_ITS):
+                self.mg_smooth_one_level(level, e_s[level], r_s[level], variables_sfc)
+            _, rr = self.mg_smooth_one_level(level, e_s[level], r_s[level], variables_sfc)
+            rc = torch.empty(nodes[level + 1], dtype=r0.dtype, device=self.device)
+            ec = torch.empty_like(rc)
+            check(self.lib.dgmi_level_restrict(dt, nodes[level], _ptr(rr), _ptr(rc), _stream()),
+                  "dgmi_level_restrict")
+            check(self.lib.dgmi_level_restrict(dt, nodes[level], _ptr(e_s[level]), _ptr(ec),
+                                               _stream()), "dgmi_level_restrict")
+            r_s.append(rc)
+            e_s.append(ec)
+        # direct solve on the last level: one node (calculate_nlevel_sfc goes down to 1)
+        if nodes[start] != 1:
+            raise ValueError("the last SFC level must hold one node")
+        a, diag, _ = variables_sfc[start]
+        if a.values.dtype != r0.dtype:
+            a, diag = a.to(r0.dtype), diag.to(r0.dtype)
+        e_s[start] = torch.zeros(1, dtype=r0.dtype, device=self.device)
+        scratch = torch.empty_like(e_s[start])
+        check(self.lib.dgmi_level_smooth(dt, 1, _ptr(a.crow), _ptr(a.col), _ptr(a.values),
+                                         _ptr(diag), 1.0, _ptr(e_s[start]), _ptr(r_s[start]),
+                                         _ptr(scratch), _stream()), "dgmi_level_smooth")
+        for level in range(start - 1, -1, -1):
+            check(self.lib.dgmi_level_prolong_add(dt, nodes[level], _ptr(e_s[level + 1]),
+                                                  _ptr(e_s[level]), _stream()),
+                  "dgmi_level_prolong_add")
+            for _ in range(POST_SMOOTH_ITS):
+                self.mg_smooth_one_level(level, e_s[level], r_s[level], variables_sfc)
+        return e_s[0]
 
     def _um_left_precond(self, x_i, x_rhs):
         """x_i = M^-1 x_rhs, one V-cycle from x_i = 0 (every caller in the reference passes a
@@ -362,10 +586,13 @@ class DGDiffusion:
         return x_i, max(p["its"] for p in per)
 
     def _mg_solver(self, x_i, x_rhs, tol=None, rtol=None, max_its=GMRES_ITS):
-        """Stand-alone V-cycle iteration (z02/volume_mf_diff.py:1844-1873)."""
+        """Stand-alone V-cycle iteration (z02/volume_mf_diff.py:1844-1873): x += M^-1 r
+        until ||r|| <= tol (absolute, the reference's test) or rtol ||b||."""
         self._need_mg()
         x_i = self._vec(x_i.view(-1), "x_i", (torch.float64,))
         b = self._vec(x_rhs.view(-1), "x_rhs", (torch.float64,))
+        if tol is None and rtol is None:
+            raise ValueError("give tol (absolute) or rtol (relative)")
         bn = float(torch.linalg.vector_norm(b))
         thr = tol if rtol is None else rtol * bn
         r = self.apply(x_i, b)
@@ -377,24 +604,81 @@ class DGDiffusion:
                   "dgmi_axpy")
             self.apply(x_i, b, out=r)
             its += 1
+        rn = float(torch.linalg.vector_norm(r))
+        self.last_stats = dict(its=its, relres=rn / bn if bn > 0 else 0.0, outer=its, cycles=its)
         return x_i, its
 
+    def _cg_mg_solver(self, x_i, x_rhs, tol=None, rtol=None, max_its=GMRES_ITS):
+        """MG-preconditioned conjugate gradients (z02/volume_mf_diff.py:1875-1931), on the
+        device in FP64.  The reference stops on the absolute norm of the updated residual
+        ``tol``; pass ``rtol`` for ||r|| <= rtol ||b||.  Returns (x_i, its); the final true
+        relative residual is in ``last_stats`` (a list per component for a packed space)."""
+        self._need_mg()
+        x_i = self._vec(x_i.view(-1), "x_i", (torch.float64,))
+        b = self._vec(x_rhs.view(-1), "x_rhs", (torch.float64,))
+        if tol is None and rtol is None:
+            raise ValueError("give tol (absolute) or rtol (relative)")
+        abs_tol = -1.0 if rtol is not None else float(tol)
+        stats = (C.c_double * (4 * self.ncomp))()
+        check(self.lib.dgmi_cg_solve(self.ctx, _ptr(b), _ptr(x_i),
+                                     0.0 if rtol is None else float(rtol), abs_tol, int(max_its),
+                                     stats, _stream()), "dgmi_cg_solve")
+        per = [dict(its=int(stats[4 * q]), relres=stats[4 * q + 1], outer=1, cycles=1)
+               for q in range(self.ncomp)]
+        self.last_stats = per[0] if self.ncomp == 1 else per
+        return x_i, max(p["its"] for p in per)
+
+    def _dof_data(self, a, name):
+        return self._vec(torch.as_tensor(a, dtype=torch.float64, device=self.device)
+                         .reshape(-1).contiguous(), name, (torch.float64,))
+
     def _get_rhs(self, x_rhs, f, u_bc, alpha_x_i=None):
-        """x_rhs = M f + Dirichlet terms (z02/volume_mf_diff.py:153-192)."""
+        """x_rhs += M f + Dirichlet terms + Neumann terms (+ M alpha_x_i / dt when the
+        space is transient), as z02/volume_mf_diff.py:153-400 accumulates into x_rhs.
+
+        ``u_bc`` is the reference's list of per-boundary-label nodal data (z02/bc_f.py:892):
+        u_bc[0] Dirichlet values, u_bc[1] the Neumann flux on faces of glb_bcface_type 1;
+        a single array is taken as the Dirichlet data of a mesh without Neumann faces.
+        ``alpha_x_i`` is used only when the space is transient (``is_transient``), as the
+        reference tests ``sf_nd_nb.isTransient`` (:230)."""
         x_rhs = self._vec(x_rhs.view(-1), "x_rhs", (torch.float64,))
-        f = self._vec(torch.as_tensor(f, dtype=torch.float64, device=self.device).reshape(-1).contiguous(), "f", (torch.float64,))
-        u = self._vec(torch.as_tensor(u_bc, dtype=torch.float64, device=self.device).reshape(-1).contiguous(), "u_bc", (torch.float64,))
-        check(self.lib.dgmi_rhs(self.ctx, _ptr(f), _ptr(u), _ptr(x_rhs), _stream()), "dgmi_rhs")
+        f = self._dof_data(f, "f")
+        if isinstance(u_bc, (list, tuple)):
+            if len(u_bc) == 0:
+                raise ValueError("u_bc holds no boundary data")
+            ud = self._dof_data(u_bc[0], "u_bc[0]")
+            un = self._dof_data(u_bc[1], "u_bc[1]") if len(u_bc) > 1 else None
+        else:
+            ud, un = self._dof_data(u_bc, "u_bc"), None
+        if self.has_neumann and un is None:
+            raise ValueError("the mesh has Neumann faces (glb_bcface_type 1): pass u_bc as "
+                             "[dirichlet, neumann]")
+        al = None
+        if self.isTransient and alpha_x_i is not None:
+            al = self._dof_data(alpha_x_i, "alpha_x_i")
+        tmp = torch.empty_like(x_rhs)
+        check(self.lib.dgmi_rhs(self.ctx, _ptr(f), _ptr(ud), _ptr(un), _ptr(al),
+                                float(self.dt) if al is not None else 0.0, _ptr(tmp), _stream()),
+              "dgmi_rhs")
+        check(self.lib.dgmi_axpy(_lib.DGMI_F64, self.ndof, 1.0, _ptr(tmp), _ptr(x_rhs),
+                                 _stream()), "dgmi_axpy")
         return x_rhs
 
-    def solve_for_diff(self, x_i, f, u_bc, alpha_x_i=None, t=None, rtol=1e-10,
+    def solve_for_diff(self, x_i, f, u_bc, alpha_x_i=None, t=None, rtol=1e-10, tol=None,
                        inner=torch.float64, m=GMRES_M, max_its=GMRES_ITS):
-        """RHS, multigrid set-up and GMRES solve (z02/volume_mf_diff.py:33-148)."""
+        """RHS, multigrid set-up and the solve with ``linear_solver`` (z02/volume_mf_diff.py:
+        33-148).  ``tol`` (absolute) replaces ``rtol`` when given; ``inner`` applies to
+        "gmres-mg"."""
         self._need_mg()
         shape = x_i.shape
-        b = self._get_rhs(self.new_vector(), f, u_bc)
-        x_i, _ = self._gmres_mg_solver(x_i.view(-1), b, rtol=rtol, inner=inner, m=m,
-                                       max_its=max_its)
+        b = self._get_rhs(self.new_vector(), f, u_bc, alpha_x_i)
+        kw = dict(tol=tol) if tol is not None else dict(rtol=rtol)
+        if self.linear_solver == "gmres-mg":
+            x_i, _ = self._gmres_mg_solver(x_i.view(-1), b, inner=inner, m=m, max_its=max_its, **kw)
+        elif self.linear_solver == "cg-mg":
+            x_i, _ = self._cg_mg_solver(x_i.view(-1), b, max_its=max_its, **kw)
+        else:
+            x_i, _ = self._mg_solver(x_i.view(-1), b, max_its=max_its, **kw)
         return x_i.view(shape)
 
     def get_l2_error(self, x_num, x_ana):
@@ -447,6 +731,34 @@ def _gmres_mg_solver(x_i, x_rhs, tol):
 
 def _mg_solver(x_i, x_rhs, tol):
     return _active()._mg_solver(x_i, x_rhs, tol)
+
+
+def _cg_mg_solver(x_i, x_rhs, tol):
+    return _active()._cg_mg_solver(x_i, x_rhs, tol)
+
+
+def _get_rhs(x_rhs, f, u_bc, alpha_x_i):
+    return _active()._get_rhs(x_rhs, f, u_bc, alpha_x_i)
+
+
+def mg_on_P1CG(r0, variables_sfc, nlevel, nodes_per_level, cg_nonods):
+    return _active().mg_on_P1CG(r0, variables_sfc, nlevel, nodes_per_level, cg_nonods)
+
+
+def mg_smooth_one_level(level, e_i, b, variables_sfc):
+    return _active().mg_smooth_one_level(level, e_i, b, variables_sfc)
+
+
+def mg_on_P1CG_prep(fina, cola, RARvalues, sparse_in):
+    return _active().mg_on_P1CG_prep(fina, cola, RARvalues, sparse_in)
+
+
+def vel_pndg_to_p1dg_restrictor(x):
+    return _active().vel_pndg_to_p1dg_restrictor(x)
+
+
+def vel_p1dg_to_pndg_prolongator(x):
+    return _active().vel_p1dg_to_pndg_prolongator(x)
 
 
 def solve_for_diff(x_i, f, u_bc, alpha_x_i=None, t=None):

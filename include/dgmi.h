@@ -41,8 +41,10 @@ typedef struct {
   const double* verts;     /* host (nverts, D) vertex coordinates                        */
   const double* kappa;     /* host (nele) diffusivity per element                        */
   const int64_t* bctype;   /* host (nele*(D+1)) face types, or NULL: boundary faces are
-                              Dirichlet (0), matched faces interior (-1); other types
-                              carry no operator term (z02/mesh_init.py:697-741)          */
+                              Dirichlet (0), matched faces interior (-1); Neumann faces
+                              (1) carry no operator term and add their flux to the
+                              right-hand side; other types carry no term at all
+                              (z02/mesh_init.py:697-741, z02/volume_mf_diff.py:1575-1589) */
   double eta_e;            /* penalty coefficient (z02/config.py:245: 36)                 */
   int32_t face_kappa;      /* interior-face diffusivity: 0 = arithmetic mean, the
                               reference's (z05/volume_mf_um_diff_varprop.py:567);
@@ -78,10 +80,15 @@ int dgmi_element_tables(const dgmi_ctx* ctx, double* n, double* nlx, double* wei
  * z05/volume_mf_um_diff_varprop.py:347-436, 509-724). */
 int dgmi_apply(dgmi_ctx* ctx, int dtype, const void* x, const void* b, void* r, void* stream);
 
-/* b = sum_e M_e f_e + Dirichlet-face terms with data u_bc, scaled by the owner's kappa --
- * _get_rhs (z02/volume_mf_diff.py:153-357; kappa as z05/volume_mf_um_diff_varprop.py:320-342).
- * f, u_bc, b: FP64 device vectors in DOF layout. */
-int dgmi_rhs(dgmi_ctx* ctx, const double* f, const double* u_bc, double* b, void* stream);
+/* b = sum_e M_e (f_e + alpha_e / dt) + Dirichlet-face terms with data u_bc, scaled by the
+ * owner's kappa, + Neumann-face terms int v g_N -- _get_rhs (z02/volume_mf_diff.py:153-400:
+ * volume :195-239 with the transient term :230-238, Dirichlet faces :298-357, Neumann faces
+ * (glb_bcface_type 1) :360-400, not scaled by kappa there; kappa on Dirichlet faces as
+ * z05/volume_mf_um_diff_varprop.py:320-342).  f, u_bc, b: FP64 device vectors in DOF layout;
+ * u_neumann (nodal Neumann flux, the reference's u_bc[1]) and alpha_x_i (nodal data of the
+ * transient term, used only when not NULL, with dt > 0) may be NULL. */
+int dgmi_rhs(dgmi_ctx* ctx, const double* f, const double* u_bc, const double* u_neumann,
+             const double* alpha_x_i, double dt, double* b, void* stream);
 
 /* out[0] = l2, out[1] = linf (host doubles) of x_num - x_ana with the reference's
  * definition -- get_l2_error (z02/volume_mf_diff.py:2032-2056):
@@ -145,6 +152,38 @@ int dgmi_precond(dgmi_ctx* ctx, int dtype, const void* b, void* x, void* stream)
  * refinement steps, restarts. */
 int dgmi_gmres_solve(dgmi_ctx* ctx, int inner_dtype, const double* b, double* x, double rtol,
                      int m, int max_its, double* stats, void* stream);
+
+/* MG-preconditioned conjugate gradients for A x = b on device vectors (FP64 x, b) --
+ * _cg_mg_solver (z02/volume_mf_diff.py:1875-1931): r = b - A x, z = M^-1 r (one V-cycle,
+ * as dgmi_precond), p = z; per iteration alpha = (r.z)/(p.Ap), x += alpha p,
+ * r -= alpha Ap, z = M^-1 r, beta = (r.z)_new/(r.z)_old, p = z + beta p.  Stops, per
+ * component, when the updated residual satisfies ||r|| <= abs_tol (abs_tol >= 0: the
+ * reference's absolute test) or ||r|| <= rtol ||b|| (abs_tol < 0), or after max_its
+ * iterations.  Synchronises the stream.  stats (may be NULL), 4 values per component:
+ * iterations, final true ||b - A x|| / ||b||, 1, 1. */
+int dgmi_cg_solve(dgmi_ctx* ctx, const double* b, double* x, double rtol, double abs_tol,
+                  int max_its, double* stats, void* stream);
+
+/* Level operations on caller-owned SFC levels (CSR with int32 rowptr/col, values and
+ * diagonal of the level's dtype), for a caller that drives the V-cycle itself:
+ * dgmi_level_smooth    rr = b - A e; e += omega rr / diag, with rr returned
+ *                      (mg_smooth_one_level, z02/multigrid_linearelastic.py:25-39);
+ * dgmi_level_restrict  coarse[I] = fine[2I] + fine[2I+1], fine padded with one zero
+ *                      ((n_fine - 1)/2 + 1 coarse values; the restriction of r and e in
+ *                      mg_on_P1CG, :147-153);
+ * dgmi_level_prolong_add  fine[i] += coarse[i >> 1] (nearest upsampling, :163-164). */
+int dgmi_level_smooth(int dtype, int64_t n, const int32_t* rowptr, const int32_t* col,
+                      const void* val, const void* diag, double omega, void* e, const void* b,
+                      void* rr, void* stream);
+int dgmi_level_restrict(int dtype, int64_t n_fine, const void* fine, void* coarse, void* stream);
+int dgmi_level_prolong_add(int dtype, int64_t n_fine, const void* coarse, void* fine,
+                           void* stream);
+/* Element-local P3DG <-> P1DG transfers with the P1 interpolation table at the P3 nodes:
+ * direction 0 y[e][k] = sum_n P[n][k] x[e][n] (vel_pndg_to_p1dg_restrictor,
+ * z02/multigrid_linearelastic.py:286-290), direction 1 y[e][n] = sum_k P[n][k] x[e][k]
+ * (vel_p1dg_to_pndg_prolongator, :293-297); P from z02/mesh_init.py:224-235, 610-631. */
+int dgmi_pn_p1dg(int D, int dtype, int direction, int64_t nele, const void* x, void* y,
+                 void* stream);
 
 /* Autotuner of the FP32 apply: times (reps launches each, on this context's mesh) the
  * vector kernels and the matrix-core kernels with 2, 4 and 8 waves (32, 64, 128 elements)

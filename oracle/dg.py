@@ -131,10 +131,20 @@ def face_kappa_value(kE, kN, mode="mean"):
     raise ValueError("face_kappa must be 'mean' or 'harmonic'")
 
 
-def make_space(mesh, kappa=None, eta_e: float = ETA_E, face_kappa: str = "mean") -> Space:
+def make_space(mesh, kappa=None, eta_e: float = ETA_E, face_kappa: str = "mean",
+               bctype=None) -> Space:
+    """``bctype`` (nele*nface) overrides the boundary faces' ``glb_bcface_type`` (0
+    Dirichlet, 1 Neumann, other values: no term), as the physical tags of a gmsh mesh do
+    (z02/mesh_init.py:697-741); interior faces stay -1."""
     D = mesh.D
     el = tables.make_element(D)
-    nbf, alnmt, bctype = topology(mesh.cells, D)
+    nbf, alnmt, bct = topology(mesh.cells, D)
+    if bctype is not None:
+        bctype = np.asarray(bctype, dtype=np.int64).reshape(-1)
+        if bctype.shape != bct.shape or np.any((bctype < 0) != (nbf >= 0)):
+            raise ValueError("bctype must be -1 exactly on the interior faces")
+        bct = bctype.copy()
+    bctype = bct
     nele = mesh.nele
     x = mesh.x_ref_in
     if kappa is None:
@@ -317,14 +327,26 @@ def get_bdiag_diag(sp: Space):
     return diag, B
 
 
-def get_rhs(sp: Space, f, u_bc):
+def get_rhs(sp: Space, f, u_bc, u_neumann=None, alpha_x_i=None, dt=None):
     """b = sum_e M_e f_e - int (grad v . n) u_D kappa + gamma int v u_D kappa on Dirichlet
-    faces (z02/volume_mf_diff.py:195-239, 298-357; kappa as z05 :320-342)."""
+    faces (z02/volume_mf_diff.py:195-239, 298-357; kappa as z05 :320-342)
+    + int v g_N on Neumann faces (glb_bcface_type 1, :360-400: sn sn sdetwei, no kappa)
+    + sum_e M_e alpha_e / dt when ``alpha_x_i`` is given (the transient term, :230-238)."""
     el = sp.el
     nele, nloc, nface = sp.nele, el.nloc, el.nface
     F = np.asarray(f).reshape(nele, nloc)
     U = np.asarray(u_bc).reshape(nele, nloc)
     b = np.einsum("mg,ng,eg,en->em", el.n, el.n, sp.detwei, F)
+    if alpha_x_i is not None:
+        Al = np.asarray(alpha_x_i).reshape(nele, nloc)
+        b += np.einsum("mg,ng,eg,en->em", el.n, el.n, sp.detwei, Al) / dt
+    if u_neumann is not None:
+        G = np.asarray(u_neumann).reshape(nele, nloc)
+        for fc in range(nface):
+            e = np.nonzero(sp.bctype[np.arange(nele) * nface + fc] == 1)[0]
+            if e.size:
+                b[e] += np.einsum("mg,ng,eg,en->em", el.sn[fc], el.sn[fc],
+                                  sp.sdetwei[e, fc], G[e])
     for fc in range(nface):
         gfi = np.arange(nele) * nface + fc
         dirich = sp.bctype[gfi] == 0
