@@ -102,7 +102,7 @@ def test_level_operations_match_oracle(prob):
         A = lv.A[l]
         assert np.array_equal(to_np(a.crow).astype(np.int64), A.indptr)
         assert np.array_equal(to_np(a.col).astype(np.int64), A.indices)
-        assert rel(to_np(a.values), A.data) < 1e-15 and rel(to_np(diag)[0], lv.diag[l]) < 1e-15
+        assert rel(to_np(a.values), A.data) < 1e-13 and rel(to_np(diag)[0], lv.diag[l]) < 1e-13
     rng = np.random.default_rng(3)
     for l in (0, 1, nlevel // 2):
         e0 = rng.standard_normal(nodes[l])
@@ -111,11 +111,11 @@ def test_level_operations_match_oracle(prob):
         e = to_dev(e0)
         e_out, rr = g.mg_smooth_one_level(l, e, to_dev(bl), vs)
         assert e_out is e
-        assert rel(to_np(e), e_ref) < 1e-14 and rel(to_np(rr), rr_ref) < 1e-14
+        assert rel(to_np(e), e_ref) < 1e-13 and rel(to_np(rr), rr_ref) < 1e-13
         # the context's own levels through sfc_data_Um give the same sweep
         e2 = to_dev(e0)
         g.mg_smooth_one_level(l, e2, to_dev(bl), g.sfc_data_Um.variables_sfc)
-        assert rel(to_np(e2), e_ref) < 1e-14
+        assert rel(to_np(e2), e_ref) < 1e-13
     r = rng.standard_normal(nodes[0])
     ref = mg.mg_on_P1CG(r, lv)
     out = to_np(g.mg_on_P1CG(to_dev(r), vs, nlevel, nodes, nodes[0]))
