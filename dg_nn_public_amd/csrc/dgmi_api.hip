@@ -1593,6 +1593,20 @@ int dgmi_dot(int dtype, int64_t n, const void* x, const void* y, double* out, vo
   return -2;
 }
 
+int dgmi_dot_noalloc(int dtype, int64_t n, const void* x, const void* y, double* out,
+                     void* stream) {
+  DGMI_CHECK(x && y && out, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == DGMI_F64)
+    return launch_dot_noalloc<double>(n, (const double*)x, (const double*)y, out, s);
+  if (dtype == DGMI_F32)
+    return launch_dot_noalloc<float>(n, (const float*)x, (const float*)y, out, s);
+  set_error("dtype not supported");
+  return -2;
+}
+
+int64_t dgmi_dot_noalloc_max_n(void) { return dot_noalloc_max_n(); }
+
 int dgmi_axpy(int dtype, int64_t n, double a, const void* x, void* y, void* stream) {
   DGMI_CHECK(x && y, "null argument");
   hipStream_t s = (hipStream_t)stream;

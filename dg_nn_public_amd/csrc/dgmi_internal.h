@@ -378,4 +378,9 @@ template <typename T>
 int launch_dot(int64_t n, const T* x, const T* y, double* out, int mode, hipStream_t s);
 template <typename T>
 int launch_axpy(int64_t n, double a, const T* x, T* y, hipStream_t s);
+// the dot product with its partial sums in a fixed device array (no allocation); n up to
+// dot_noalloc_max_n()
+template <typename T>
+int launch_dot_noalloc(int64_t n, const T* x, const T* y, double* out, hipStream_t s);
+int64_t dot_noalloc_max_n();
 }  // namespace dgmi

@@ -1,6 +1,8 @@
 // Krylov vector kernels (gfx950): deterministic dot products, AXPYs with device-resident
 // scalars, Givens updates of the Hessenberg column, and the basis combination x += V y.
 // All reductions use a fixed grid and a fixed tree, so results do not depend on timing.
+#include <mutex>
+
 #include "dgmi_internal.h"
 
 namespace dgmi {
@@ -623,6 +625,77 @@ int launch_dot(int64_t n, const T* x, const T* y, double* out, int mode, hipStre
   DGMI_HIP(hipGetLastError());
   return 0;
 }
+
+// ---------------------------------------------------------------------------------------
+// Allocation-free dot product: the partial sums live in `scratch`, a fixed device array of
+// this library (one copy per GPU), instead of a buffer from hipMalloc.  Block b sums
+// values [b*NOALLOC_CHUNK, (b+1)*NOALLOC_CHUNK) in a fixed order; one block then sums the
+// partials in a fixed order.  Every call on a GPU uses that GPU's array, so calls that
+// overlap on one GPU (several streams or threads) race.
+constexpr int NOALLOC_BLOCKS = 32768;
+constexpr int64_t NOALLOC_CHUNK = 8192;
+static __device__ __align__(256) double scratch[NOALLOC_BLOCKS];
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_dot_chunk(int64_t n, const T* __restrict__ x,
+                                                           const T* __restrict__ y,
+                                                           double* __restrict__ partial) {
+  const int64_t lo = (int64_t)blockIdx.x * NOALLOC_CHUNK;
+  const int64_t hi = lo + NOALLOC_CHUNK < n ? lo + NOALLOC_CHUNK : n;
+  double s = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += RED_THREADS) s += (double)x[i] * (double)y[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// device address of `scratch` on each GPU, looked up once per device
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {};
+
+static int noalloc_scratch(double** out) {
+  int dev = -1;
+  DGMI_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("device index out of the supported range (0..63)");
+    return -2;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    DGMI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("hipGetSymbolAddress returned no address for the dot scratch");
+      return -3;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<double*>(g_scratch_ptr[dev]);
+  return 0;
+}
+
+int64_t dot_noalloc_max_n() { return (int64_t)NOALLOC_BLOCKS * NOALLOC_CHUNK; }
+
+template <typename T>
+int launch_dot_noalloc(int64_t n, const T* x, const T* y, double* out, hipStream_t s) {
+  if (n < 0 || n > dot_noalloc_max_n()) {
+    set_error("dgmi_dot_noalloc: n = " + std::to_string(n) + " is outside 0.." +
+              std::to_string(dot_noalloc_max_n()) + " (" + std::to_string(NOALLOC_BLOCKS) +
+              " partial sums of " + std::to_string(NOALLOC_CHUNK) + " values)");
+    return -2;
+  }
+  double* part = nullptr;
+  if (int rc = noalloc_scratch(&part)) return rc;
+  const int nb = n == 0 ? 1 : (int)((n + NOALLOC_CHUNK - 1) / NOALLOC_CHUNK);
+  hipLaunchKernelGGL((k_dot_chunk<T>), dim3(nb), dim3(RED_THREADS), 0, s, n, x, y, part);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, (const double*)part, nb, out,
+                     0);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template int launch_dot_noalloc<double>(int64_t, const double*, const double*, double*,
+                                        hipStream_t);
+template int launch_dot_noalloc<float>(int64_t, const float*, const float*, double*, hipStream_t);
 
 template <typename T>
 int launch_axpy(int64_t n, double a, const T* x, T* y, hipStream_t s) {

@@ -228,6 +228,18 @@ int dgmi_set_param(dgmi_ctx* ctx, const char* name, double value);
 /* Krylov vector primitives on device vectors of n FP64 or FP32 values. */
 int dgmi_dot(int dtype, int64_t n, const void* x, const void* y, double* out_dev, void* stream);
 int dgmi_axpy(int dtype, int64_t n, double a, const void* x, void* y, void* stream);
+/* Allocation-free dgmi_dot: out_dev[0] = x . y (FP64 accumulation) with the partial sums
+ * in a fixed array of this library in device memory (256 KiB per GPU, held until the
+ * process ends) instead of a buffer from hipMalloc.  No counterpart in the reference; it is
+ * for callers that cannot allocate: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation.  n must lie in 0..dgmi_dot_noalloc_max_n() (2^28
+ * values: 32768 partial sums of 8192 values each); larger n returns an error.  The first
+ * call on a device looks up that device's copy of the array (no allocation).  Every call
+ * on one GPU shares its array: calls that overlap on one GPU (other streams or host
+ * threads) race.  The result is written before the stream's next work. */
+int dgmi_dot_noalloc(int dtype, int64_t n, const void* x, const void* y, double* out_dev,
+                     void* stream);
+int64_t dgmi_dot_noalloc_max_n(void);
 
 /* ---- host-only set-up (no device needed) ------------------------------------------
  * Replacements of the reference's f2py Fortran modules and Python set-up loops; also
