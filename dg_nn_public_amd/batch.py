@@ -55,20 +55,24 @@ class DGBatch:
     variable-diffusivity solver uses (z05/volume_mf_um_diff_varprop.py:384-392)."""
 
     def __init__(self, meshes, kappas, device="cuda:0", face_kappa="harmonic",
-                 blk_solver="direct", graph_vcycle=True):
+                 blk_solver="direct", graph_vcycle=True, alloc_limit=0):
         p = pack(meshes)
         kappa = np.concatenate([np.asarray(k, dtype=np.float64).reshape(-1) for k in kappas])
         self.space = DGDiffusion(p.x_ref_in, p.cells, p.verts, kappa, device=device,
                                  face_kappa=face_kappa, blk_solver=blk_solver,
-                                 components=(p.elem_off, p.vert_off))
+                                 components=(p.elem_off, p.vert_off), alloc_limit=alloc_limit)
         self.packed = p
         self.ncomp = len(meshes)
         self.nloc = self.space.nloc
         self.dof_off = p.elem_off * self.nloc
-        # replay the packed V-cycle from a captured graph, or launch its kernels one by one
-        self.space.lib.dgmi_set_param(self.space.ctx, b"graph_vcycle",
-                                      1.0 if graph_vcycle else 0.0)
-        self.space._get_RAR_and_sfc_data_Um()
+        try:
+            # replay the packed V-cycle from a captured graph, or launch its kernels one by one
+            self.space.lib.dgmi_set_param(self.space.ctx, b"graph_vcycle",
+                                          1.0 if graph_vcycle else 0.0)
+            self.space._get_RAR_and_sfc_data_Um()
+        except BaseException:
+            self.space.close()  # the device memory goes back at once, not at collection
+            raise
 
     def rhs(self, fs, u_bcs):
         """b = M f + Dirichlet terms of every component (nodal f and u_bc per mesh)."""

@@ -162,6 +162,11 @@ static Scratch& scratch(dgmi_ctx* c);
 using namespace dgmi;
 
 void* dgmi_ctx::alloc(size_t bytes) {
+  if (alloc_limit > 0 && dev_bytes + bytes > alloc_limit) {
+    g_alloc_err = "the context's device-byte limit (" + std::to_string(alloc_limit) +
+                  " bytes) would be exceeded by " + std::to_string(bytes) + " bytes";
+    return nullptr;
+  }
   void* p = nullptr;
   const hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess || !p) {
@@ -232,6 +237,7 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
   T* r = vbuf<T>(c, 3);
   T* rc = cgbuf<T>(c, 0);
   T* ec = cgbuf<T>(c, 1);
+  DGMI_CHECK(r && rc && ec, alloc_failure("preconditioner work vectors"));
   if (tr_out) *tr_out = -1;
   if constexpr (std::is_same<T, float>::value) {
     // bf16 inner cycles with "bf16_smoother": the sweeps read the bfloat16 copy of the
@@ -281,6 +287,7 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
 // pinned host mirrors of the per-component scalars the host reads back.
 struct Krylov {
   int m = 0, ncomp = 0;
+  bool init_ok = false;  // the per-component scalars and pinned mirrors exist
   // leading dimension of V and Z: ndof rounded up to 256 values plus 64, so that the
   // columns of one row group do not all fall on the same HBM channels
   int64_t ldv = 0;
@@ -306,7 +313,7 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
   if (!c->krylov) c->krylov = new Krylov{};
   Krylov* k = static_cast<Krylov*>(c->krylov);
   const int nc = c->ncomp;
-  if (!k->beta) {
+  if (!k->init_ok) {  // (again after a failed allocation: every pointer is re-made)
     k->ncomp = nc;
     auto dv = [&](size_t n) { return static_cast<double*>(c->alloc(sizeof(double) * n)); };
     auto iv = [&](size_t n) { return static_cast<int32_t*>(c->alloc(sizeof(int32_t) * n)); };
@@ -321,9 +328,14 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     k->active = iv(nc);
     k->kk = iv(nc);
     k->nact = iv(1);
+    if (k->hd) (void)hipHostFree(k->hd);
+    if (k->hi) (void)hipHostFree(k->hi);
     if (hipHostMalloc((void**)&k->hd, sizeof(double) * 2 * nc) != hipSuccess) k->hd = nullptr;
     if (hipHostMalloc((void**)&k->hi, sizeof(int32_t) * (2 * nc + 1)) != hipSuccess) k->hi = nullptr;
-    if (!k->beta || !k->nact || !k->hd || !k->hi) return nullptr;
+    if (!k->beta || !k->hn || !k->res || !k->thr || !k->eps || !k->rnorm || !k->bnorm ||
+        !k->want || !k->active || !k->kk || !k->nact || !k->hd || !k->hi)
+      return nullptr;
+    k->init_ok = true;
   }
   const int64_t ldv = (c->ndof + 255) / 256 * 256 + 64;
   k->ldv = ldv;
@@ -344,23 +356,32 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     k->g = static_cast<double*>(c->alloc(sizeof(double) * k->gs * nc));
     k->y = static_cast<double*>(c->alloc(sizeof(double) * k->ys * nc));
     k->m = m;
-    if (!k->V || !k->H || !k->cs || !k->sn || !k->g || !k->y) return nullptr;
+    if (!k->V || !k->H || !k->cs || !k->sn || !k->g || !k->y) {
+      k->m = 0;  // a later call allocates the basis again
+      k->vbytes = 0;
+      return nullptr;
+    }
     if (k->ring) (void)hipHostFree(k->ring);
     k->ring = nullptr;
-    if (hipHostMalloc((void**)&k->ring, sizeof(double) * (m + 1) * (nc + 1)) != hipSuccess)
-      return nullptr;
-    if (hipHostGetDevicePointer((void**)&k->ring_dev, k->ring, 0) != hipSuccess) return nullptr;
-    while ((int)k->ev.size() < m + 1) {
+    bool ok = hipHostMalloc((void**)&k->ring, sizeof(double) * (m + 1) * (nc + 1)) == hipSuccess;
+    if (!ok) k->ring = nullptr;
+    ok = ok && hipHostGetDevicePointer((void**)&k->ring_dev, k->ring, 0) == hipSuccess;
+    while (ok && (int)k->ev.size() < m + 1) {
       hipEvent_t e;
-      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
-      k->ev.push_back(e);
+      ok = hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+      if (ok) k->ev.push_back(e);
+    }
+    if (!ok) {
+      k->m = 0;
+      k->vbytes = 0;
+      return nullptr;
     }
     k->nring = m + 1;
   }
   const size_t zneed = (size_t)m * ldv * elem;
   if (flexible && k->zbytes < zneed) {
     k->Z = c->alloc(zneed);
-    k->zbytes = zneed;
+    k->zbytes = k->Z ? zneed : 0;
     if (!k->Z) return nullptr;
   }
   return k;
@@ -490,7 +511,15 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
   std::vector<char> done(nc, 0);
   DGMI_HIP(hipMemcpyAsync(bnorm.data(), k->bnorm, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
   DGMI_HIP(hipStreamSynchronize(s));
-  for (double v : bnorm) DGMI_CHECK(std::isfinite(v), "non-finite right-hand side");
+  // a packed context isolates a component with non-finite data: it stops at once with
+  // relres = NaN and the others go on; a single solve fails with an error
+  std::vector<char> failed(nc, 0);
+  for (int q = 0; q < nc; ++q)
+    if (!std::isfinite(bnorm[q])) {
+      DGMI_CHECK(nc > 1, "non-finite right-hand side");
+      failed[q] = done[q] = 1;
+      rnorm[q] = NAN;
+    }
   int cycles = 0;
   // element kernels skip the solves that have finished (several components only)
   struct MaskScope {
@@ -514,7 +543,13 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
         continue;
       }
       rnorm[q] = k->hd[q];
-      DGMI_CHECK(std::isfinite(rnorm[q]), "non-finite residual in component " + std::to_string(q));
+      if (!std::isfinite(rnorm[q])) {
+        DGMI_CHECK(nc > 1, "non-finite residual");
+        failed[q] = done[q] = 1;
+        k->hi[q] = 0;
+        eps[q] = 0.5;
+        continue;
+      }
       const double tol = rtol * bnorm[q];
       const bool want = !(rnorm[q] <= tol || its[q] >= max_its || bnorm[q] == 0.0);
       done[q] = !want;
@@ -559,7 +594,7 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
   if (stats)
     for (int q = 0; q < nc; ++q) {
       stats[4 * q + 0] = its[q];
-      stats[4 * q + 1] = bnorm[q] > 0 ? rnorm[q] / bnorm[q] : 0.0;
+      stats[4 * q + 1] = failed[q] ? NAN : (bnorm[q] > 0 ? rnorm[q] / bnorm[q] : 0.0);
       stats[4 * q + 2] = outer[q];
       stats[4 * q + 3] = outer[q];
     }
@@ -651,10 +686,15 @@ static int cg_solve(dgmi_ctx* c, const double* b, double* x, double rtol, double
   DGMI_HIP(hipMemcpyAsync(w->hd, w->bn, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
   DGMI_HIP(hipStreamSynchronize(s));
   std::vector<double> bnorm(w->hd, w->hd + nc);
+  std::vector<char> failed(nc, 0);
   for (int q = 0; q < nc; ++q) {
-    DGMI_CHECK(std::isfinite(bnorm[q]), "non-finite right-hand side");
+    // as gmres_solve: a packed context isolates a component with non-finite data
+    if (!std::isfinite(bnorm[q])) {
+      DGMI_CHECK(nc > 1, "non-finite right-hand side");
+      failed[q] = 1;
+    }
     w->hd[q] = abs_tol >= 0.0 ? abs_tol : rtol * bnorm[q];
-    w->hi[q] = max_its > 0 ? 1 : 0;
+    w->hi[q] = (max_its > 0 && !failed[q]) ? 1 : 0;
   }
   DGMI_HIP(hipMemcpyAsync(w->tol, w->hd, sizeof(double) * nc, hipMemcpyHostToDevice, s));
   DGMI_HIP(hipMemcpyAsync(w->want, w->hi, sizeof(int32_t) * nc, hipMemcpyHostToDevice, s));
@@ -717,10 +757,13 @@ static int cg_solve(dgmi_ctx* c, const double* b, double* x, double rtol, double
   DGMI_HIP(hipMemcpyAsync(w->hi, w->its, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, s));
   DGMI_HIP(hipStreamSynchronize(s));
   for (int q = 0; q < nc; ++q) {
-    DGMI_CHECK(std::isfinite(w->hd[q]), "non-finite residual in component " + std::to_string(q));
+    if (!std::isfinite(w->hd[q])) {
+      DGMI_CHECK(nc > 1, "non-finite residual");
+      failed[q] = 1;
+    }
     if (stats) {
       stats[4 * q + 0] = w->hi[q];
-      stats[4 * q + 1] = bnorm[q] > 0 ? std::sqrt(w->hd[q]) / bnorm[q] : 0.0;
+      stats[4 * q + 1] = failed[q] ? NAN : (bnorm[q] > 0 ? std::sqrt(w->hd[q]) / bnorm[q] : 0.0);
       stats[4 * q + 2] = 1;
       stats[4 * q + 3] = 1;
     }
@@ -1572,6 +1615,11 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "inner_flexible") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "inner_flexible must be 0 or 1");
     c->inner_flexible = (int)value;
+    return 0;
+  }
+  if (k == "alloc_limit_bytes") {
+    DGMI_CHECK(value >= 0.0 && value < 1e18, "alloc_limit_bytes must be in [0, 1e18)");
+    c->alloc_limit = (size_t)value;  // 0: no limit
     return 0;
   }
   if (k == "f32_kernel") {

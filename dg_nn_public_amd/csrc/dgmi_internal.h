@@ -274,6 +274,8 @@ struct dgmi_ctx {
   // solver scratch (FP64 and FP32 vectors of ndof), created on demand
   std::vector<void*> allocs;
   size_t dev_bytes = 0;
+  // allocations that would take dev_bytes above this many bytes fail (0: no limit)
+  size_t alloc_limit = 0;
   void* alloc(size_t bytes);
 };
 

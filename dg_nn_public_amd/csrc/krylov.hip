@@ -385,7 +385,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_sscale(const SegBlk* __restrict
   const double av = a[b.comp];
   const bool on = (!active || active[b.comp]) && (mode == 1 || av != 0.0);
   const double f = on ? (mode == 0 ? 1.0 / av : av) : 0.0;
-  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) y[i] = (Tout)((double)x[i] * f);
+  // an inactive component gets exact zeros (also where x is not finite)
+  for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x)
+    y[i] = on ? (Tout)((double)x[i] * f) : (Tout)0;
 }
 
 // y64 += a[c] * x
@@ -396,6 +398,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_sadd(const SegBlk* __restrict__
                                                       double* __restrict__ y) {
   const SegBlk b = sb[blockIdx.x];
   const double av = a ? a[b.comp] : 1.0;
+  if (!isfinite(av)) return;  // a component isolated for non-finite data keeps its y
   for (int64_t i = b.beg + threadIdx.x; i < b.end; i += blockDim.x) y[i] += av * (double)x[i];
 }
 

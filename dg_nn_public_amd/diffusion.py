@@ -137,7 +137,8 @@ class DGDiffusion:
     def __init__(self, x_ref_in, cells, verts, kappa=None, bctype=None, eta_e: float = ETA_E,
                  blk_solver: str = "jacobian", jac_wei: float = JAC_WEI, device="cuda:0",
                  face_kappa: str = "mean", components=None, is_transient: bool = False,
-                 dt: float | None = None, linear_solver: str = "gmres-mg"):
+                 dt: float | None = None, linear_solver: str = "gmres-mg",
+                 alloc_limit: int = 0):
         """face_kappa: "mean" is the reference's interior-face diffusivity (kE + kN)/2;
         "harmonic" uses 2 kE kN/(kE + kN) (symmetric weighted interior penalty), which
         keeps the operator positive definite at high contrast.  ``components`` =
@@ -146,7 +147,9 @@ class DGDiffusion:
         convergence test, while every kernel launch covers all of them.
         ``is_transient`` / ``dt``: the reference's ``sf_nd_nb.isTransient`` and ``dt``
         (z02/config.py:33-36, 62-63): when transient, ``_get_rhs`` adds M alpha_x_i / dt.
-        ``linear_solver``: the solver ``solve_for_diff`` runs (z02/config.py:95, :137-148)."""
+        ``linear_solver``: the solver ``solve_for_diff`` runs (z02/config.py:95, :137-148).
+        ``alloc_limit``: device bytes the context may hold (0: no cap), set right after
+        creation (``set_alloc_limit``)."""
         if linear_solver not in LINEAR_SOLVERS:
             raise ValueError(f"linear_solver must be one of {LINEAR_SOLVERS}")
         if is_transient and not (dt is not None and dt > 0 and np.isfinite(dt)):
@@ -199,6 +202,8 @@ class DGDiffusion:
                                                  vo.ctypes.data_as(_lib._I64P), C.byref(ctx)),
                   "dgmi_ctx_create_batch")
         self.ctx = ctx
+        if alloc_limit:
+            self.set_alloc_limit(alloc_limit)
         info = (C.c_int64 * 8)()
         check(self.lib.dgmi_ctx_info(ctx, info), "dgmi_ctx_info")
         self.D = D
@@ -278,6 +283,19 @@ class DGDiffusion:
             raise ValueError(f"f32 kernel must be one of {sorted(F32_KERNELS)}")
         check(self.lib.dgmi_set_param(self.ctx, b"f32_kernel", float(F32_KERNELS[name])),
               "dgmi_set_param")
+
+    def set_alloc_limit(self, nbytes):
+        """Cap the device bytes this context may hold (0: no cap).  An allocation that would
+        pass the cap fails and the call that needed it raises ``DgmiError`` without
+        launching work."""
+        check(self.lib.dgmi_set_param(self.ctx, b"alloc_limit_bytes", float(nbytes)),
+              "dgmi_set_param")
+
+    def device_bytes(self):
+        """Device bytes this context holds."""
+        info = (C.c_int64 * 8)()
+        check(self.lib.dgmi_ctx_info(self.ctx, info), "dgmi_ctx_info")
+        return int(info[7])
 
     def autotune(self, reps=5):
         """Time the FP32 apply kernels (vector; matrix cores with 32/64/128 elements per

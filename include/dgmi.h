@@ -221,6 +221,11 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     start; 0 = sweep every level as the FP64 cycle (the reference's)
  *                     does.  dgmi_mg_info writes the level to info[4 + nlevel] when the
  *                     buffer holds it (-1: none).
+ *   "alloc_limit_bytes"  device bytes this context may hold (0 = no limit, the default):
+ *                     a later allocation that would pass it fails, and the entry point
+ *                     that needed it returns -2 ("device allocation failed") without
+ *                     launching anything; the serving layer caps each job at its
+ *                     admitted budget with it.
  *   "mfma_waves", "bf16_waves"  waves (16 elements each) per work-group of the fp32 / bf16
  *                     matrix-core applies: 2, 4 (default) or 8. */
 int dgmi_set_param(dgmi_ctx* ctx, const char* name, double value);

@@ -192,3 +192,20 @@ def test_fp32_dense_coarse_level_matches_oracle(D, n):
     out = to_np(g.mg_on_P1CG(to_dev(r, torch.float32)))
     print("dense level", lvl, "rows", mgd.levels.nodes_per_level[lvl], "rel err", rel(out, ref))
     assert rel(out, ref) < 1e-3
+
+
+def test_preconditioner_allocation_failure_is_an_error():
+    """A work vector that cannot be allocated (here: the context's byte cap) makes the call
+    fail with an error before any launch; with the cap lifted it runs and matches."""
+    from dg_nn_public_amd._lib import DgmiError
+    m, pb, g, sp = build(3, 2, sigma=0.5)
+    g._get_RAR_and_sfc_data_Um()
+    g.set_alloc_limit(g.device_bytes())
+    b = np.random.default_rng(5).standard_normal(sp.ndof)
+    for dt in (torch.float64, torch.float32):
+        with pytest.raises(DgmiError, match="allocation failed"):
+            g._um_left_precond(g.new_vector(dt), to_dev(b, dt))
+    g.set_alloc_limit(0)
+    mgd = mg.build_mg(sp, m.verts)
+    out = to_np(g._um_left_precond(g.new_vector(), to_dev(b)))
+    assert rel(out, mg.um_left_precond(sp, mgd, b)) < 1e-10
