@@ -4,10 +4,21 @@ VTU output.
 The reference reads gmsh ``.msh`` files through meshio (z02/config.py:53-56), caches its
 pre-processing as ``.npy`` files next to the mesh, saves the solution with ``torch.save``
 (z02/main.py:216) and writes VTU for visualisation (z02/output.py).  meshio is not a
-dependency here; the reader below parses the ASCII gmsh formats 2.2 and 4.1 itself and
-builds the P3 DG mesh (straight-sided P3 nodes in the reference's local order, oriented
-as z02/mesh_init.py:276-289).  Boundary faces take ``glb_bcface_type`` from the physical
-tag of the matching boundary facet (default 0, Dirichlet), interior faces -1.
+dependency here; the reader below parses the ASCII gmsh formats 2.2 and 4.1 itself.
+
+Cells: the reference reads P3 meshes (``tetra20`` / ``triangle10``, gmsh types 29 / 21,
+z02/config.py:69-80); linear ones (types 4 / 2) are read too and get straight-sided P3
+nodes.  A P3 cell's nodes are put in the reference's local order: a gmsh tetrahedron with
+det(x1-x0, x2-x0, x3-x0) > 0 is renumbered with the reference's table
+(z02/mesh_init.py:276-289); one with det < 0, which the reference leaves in gmsh order,
+is first reflected (vertices 1 and 2 exchanged, with their edge and face nodes) and then
+renumbered the same way; a clockwise triangle is reversed with the reference's table
+(z02/mesh_init.py:19-31, whose orientation test reads the wrong points and never fires).
+Boundary faces take ``glb_bcface_type`` from the physical group of the matching boundary
+facet: ``bc_of_tag`` maps physical tags to types; without it, the boundary groups named in
+``$PhysicalNames`` are numbered 0, 1, ... in file order, as the reference numbers its
+cell sets (z02/mesh_init.py:697-741: e.g. "diri" 0, "neu" 1); untagged facets are 0
+(Dirichlet); interior faces -1.
 """
 from __future__ import annotations
 
@@ -19,6 +30,16 @@ import torch
 from . import meshgen
 
 _GMSH_TRI, _GMSH_TET, _GMSH_LINE = 2, 4, 1
+_GMSH_TRI10, _GMSH_TET20, _GMSH_LINE4 = 21, 29, 26
+# the reference's renumbering of a gmsh tetra20 with det(x1-x0, x2-x0, x3-x0) > 0 into its
+# local order (z02/mesh_init.py:276-281): local node i is gmsh node TET20_ORDER[i]
+TET20_ORDER = [3, 1, 2, 0, 12, 13, 14, 15, 6, 7, 10, 11, 5, 4, 8, 9, 16, 19, 18, 17]
+# gmsh tetra20 with vertices 1 and 2 exchanged (its edge and face nodes follow): a left-
+# handed cell becomes right-handed
+TET20_REFLECT = [0, 2, 1, 3, 9, 8, 7, 6, 5, 4, 10, 11, 14, 15, 12, 13, 16, 18, 17, 19]
+# triangle10: gmsh order is the reference's for counter-clockwise cells; clockwise ones
+# are reversed with the reference's table (z02/mesh_init.py:20)
+TRI10_REVERSE = [0, 2, 1, 8, 7, 6, 5, 4, 3, 9]
 
 
 # ---------------------------------------------------------------------------- gmsh
@@ -95,12 +116,42 @@ def _parse_v4(nodes_sec, elems_sec, ent_sec):
     return tags, xyz, elems
 
 
-def read_msh(path, bc_of_tag=None):
-    """Read an ASCII gmsh 2.2 / 4.1 mesh of triangles (2D) or tetrahedra (3D).
+def _physical_names(lines):
+    """[(dim, tag, name)] of $PhysicalNames in file order."""
+    sec = _section(lines, "PhysicalNames")
+    out = []
+    if sec:
+        for ln in sec[1:1 + int(sec[0])]:
+            p = ln.split(maxsplit=2)
+            out.append((int(p[0]), int(p[1]), p[2].strip().strip('"')))
+    return out
 
-    Returns (mesh, bctype): ``meshgen.Mesh`` with P3 nodes at the affine positions, and
-    ``glb_bcface_type`` per (element, face): -1 interior, else the value of
-    ``bc_of_tag[physical tag]`` of the matching boundary facet (default 0, Dirichlet).
+
+def _p3_cells(vol, index, xyz, D):
+    """Node tags of P3 cells in the reference's local order (see the module docstring)."""
+    tags = np.array([e[2][:10 if D == 2 else 20] for e in vol], dtype=np.int64)
+    pts = xyz[np.vectorize(index.__getitem__, otypes=[np.int64])(tags)][..., :D]
+    if D == 3:
+        v = pts[:, :4]
+        det = np.linalg.det(v[:, 1:4] - v[:, 0:1])
+        refl = det < 0
+        tags[refl] = tags[refl][:, TET20_REFLECT]
+        return tags[:, TET20_ORDER]
+    v = pts[:, :3]
+    det = (v[:, 1, 0] - v[:, 0, 0]) * (v[:, 2, 1] - v[:, 0, 1]) - \
+        (v[:, 1, 1] - v[:, 0, 1]) * (v[:, 2, 0] - v[:, 0, 0])
+    cw = det < 0
+    tags[cw] = tags[cw][:, TRI10_REVERSE]
+    return tags
+
+
+def read_msh(path, bc_of_tag=None):
+    """Read an ASCII gmsh 2.2 / 4.1 mesh of P3 (tetra20 / triangle10) or linear
+    tetrahedra / triangles.
+
+    Returns (mesh, bctype): ``meshgen.Mesh`` with the P3 nodes in the reference's local
+    order (the file's node positions for P3 cells, straight-sided ones for linear cells)
+    and ``glb_bcface_type`` per (element, face) as described in the module docstring.
     """
     lines = _tokens(path)
     fmt = _section(lines, "MeshFormat")
@@ -118,28 +169,46 @@ def read_msh(path, bc_of_tag=None):
         tags, xyz, elems = _parse_v4(nodes_sec, elems_sec, _section(lines, "Entities"))
     else:
         raise ValueError(f"{path}: gmsh format {version} is not supported")
-    has_tet = any(t == _GMSH_TET for t, _, _ in elems)
-    D = 3 if has_tet else 2
-    vol_type, facet_type = (_GMSH_TET, _GMSH_TRI) if D == 3 else (_GMSH_TRI, _GMSH_LINE)
-    vol = [e for e in elems if e[0] == vol_type]
-    if not vol:
-        raise ValueError(f"{path}: no triangles or tetrahedra")
+    types = {t for t, _, _ in elems}
+    if _GMSH_TET20 in types or _GMSH_TET in types:
+        D = 3
+    elif _GMSH_TRI10 in types or _GMSH_TRI in types:
+        D = 2
+    else:
+        raise ValueError(f"{path}: no triangles or tetrahedra (gmsh types 2, 4, 21, 29)")
+    p3_type, lin_type = (_GMSH_TET20, _GMSH_TET) if D == 3 else (_GMSH_TRI10, _GMSH_TRI)
+    facet_types = (_GMSH_TRI10, _GMSH_TRI) if D == 3 else (_GMSH_LINE4, _GMSH_LINE)
     index = {int(t): i for i, t in enumerate(tags)}
-    cells_tag = np.array([e[2][:D + 1] for e in vol], dtype=np.int64)
+    p3 = [e for e in elems if e[0] == p3_type]
+    lin = [e for e in elems if e[0] == lin_type]
+    if p3 and lin:
+        raise ValueError(f"{path}: mixes P3 and linear cells")
+    if p3:
+        local = _p3_cells(p3, index, xyz, D)
+        cells_tag = local[:, :D + 1]
+    else:
+        cells_tag = np.array([e[2][:D + 1] for e in lin], dtype=np.int64)
     used = np.unique(cells_tag)
     new_id = {int(t): i for i, t in enumerate(used)}
     verts = np.ascontiguousarray(xyz[[index[int(t)] for t in used], :D])
     cells = np.vectorize(new_id.__getitem__, otypes=[np.int64])(cells_tag)
-    cells = meshgen._orient(verts, cells)
-    table = meshgen.TET_P3 if D == 3 else meshgen.TRI_P3
-    mesh = meshgen.Mesh(D=D, x_ref_in=meshgen._p3_nodes(verts, cells, table), cells=cells,
-                        verts=verts)
-    # boundary condition type of every face from the tagged facets
+    if p3:
+        rows = np.vectorize(index.__getitem__, otypes=[np.int64])(local)
+        x_ref_in = np.ascontiguousarray(np.moveaxis(xyz[rows][..., :D], 2, 1))
+    else:
+        cells = meshgen._orient(verts, cells)
+        table = meshgen.TET_P3 if D == 3 else meshgen.TRI_P3
+        x_ref_in = meshgen._p3_nodes(verts, cells, table)
+    mesh = meshgen.Mesh(D=D, x_ref_in=x_ref_in, cells=cells, verts=verts)
+    # physical group of every boundary facet -> glb_bcface_type
+    if bc_of_tag is None:
+        bnd = [t for dim, t, _ in _physical_names(lines) if dim == D - 1]
+        bc_of_tag = {t: i for i, t in enumerate(bnd)}
     facet_bc = {}
     for etype, ph, nodes in elems:
-        if etype == facet_type and all(int(t) in new_id for t in nodes[:D]):
+        if etype in facet_types and all(int(t) in new_id for t in nodes[:D]):
             key = tuple(sorted(new_id[int(t)] for t in nodes[:D]))
-            facet_bc[key] = (bc_of_tag or {}).get(ph, 0)
+            facet_bc[key] = bc_of_tag.get(ph, 0)
     from .host import topology
     nbf, _, _ = topology(cells, D)
     face_verts = _FACE_VERTS[D]
@@ -151,32 +220,117 @@ def read_msh(path, bc_of_tag=None):
     return mesh, bctype
 
 
+def affine_deviation(mesh):
+    """Largest distance of a P3 node from its straight-sided position, relative to the
+    element size (0 for an affine mesh; the solver accepts only affine elements)."""
+    table = meshgen.TET_P3 if mesh.D == 3 else meshgen.TRI_P3
+    ref = meshgen._p3_nodes(mesh.verts, mesh.cells, table)
+    v = mesh.verts[mesh.cells]
+    h = np.abs(v[:, :mesh.D] - v[:, mesh.D:]).max(axis=(1, 2))
+    return float((np.abs(mesh.x_ref_in - ref).max(axis=(1, 2)) / h).max())
+
+
 # local vertex lists of the faces (z02/mesh_init.py:305-320 and z02/shape_function.py:1105-1116)
 _FACE_VERTS = {3: [(2, 1, 3), (0, 2, 3), (1, 0, 3), (0, 1, 2)], 2: [(0, 1), (1, 2), (2, 0)]}
 
 
-def write_msh(path, mesh, bctype=None, tag_of_bc=None):
-    """Write ``mesh`` as ASCII gmsh 2.2: vertices, boundary facets (physical tag
-    ``tag_of_bc[bctype]``, default 1) and the triangles / tetrahedra (physical tag 0)."""
+# gmsh's own node numbering of the cubic cells (gmsh reference manual, "node ordering"):
+# edges and their direction, then face-interior nodes by face vertex triples
+_GMSH_TET_EDGES = [(0, 1), (1, 2), (2, 0), (3, 0), (3, 2), (3, 1)]
+_GMSH_TET_FACES = [(0, 2, 1), (0, 1, 3), (0, 3, 2), (3, 1, 2)]
+_GMSH_TRI_EDGES = [(0, 1), (1, 2), (2, 0)]
+
+
+def write_msh(path, mesh, bctype=None, tag_of_bc=None, order=1, physical_names=None,
+              reverse=None):
+    """Write ``mesh`` as ASCII gmsh 2.2: nodes, boundary facets (physical tag
+    ``tag_of_bc[bctype]``, default 1) and the cells (physical tag 0, or the tag named by
+    ``physical_names`` for dimension D).
+
+    order 1 writes linear cells (types 4 / 2, facets 2 / 1); order 3 writes cubic cells
+    (tetra20 / triangle10, facets triangle10 / line4) with nodes shared between cells and
+    numbered as gmsh numbers them, at straight-sided positions.  ``physical_names``
+    ({tag: (dim, name)}) adds a $PhysicalNames section; ``reverse`` (bool per cell)
+    writes those cells with their vertices 1 and 2 exchanged (opposite orientation)."""
     D = mesh.D
-    vol_type, facet_type = (_GMSH_TET, _GMSH_TRI) if D == 3 else (_GMSH_TRI, _GMSH_LINE)
+    if order not in (1, 3):
+        raise ValueError("order must be 1 or 3")
     from .host import topology
     nbf, _, bct = topology(mesh.cells, D)
     if bctype is None:
         bctype = bct
-    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(mesh.nverts)]
-    for i, v in enumerate(mesh.verts):
-        z = float(v[2]) if D == 3 else 0.0
-        lines.append(f"{i + 1} {float(v[0])!r} {float(v[1])!r} {z!r}")
-    lines.append("$EndNodes")
+    pts = [np.asarray(v, dtype=np.float64) for v in mesh.verts]
+    edge_nodes, face_nodes = {}, {}
+
+    def node_at(x):
+        pts.append(np.asarray(x, dtype=np.float64))
+        return len(pts) - 1
+
+    def edge(a, b):
+        """the two nodes on edge a -> b, in that direction"""
+        key = (min(a, b), max(a, b))
+        if key not in edge_nodes:
+            xa, xb = mesh.verts[key[0]], mesh.verts[key[1]]
+            edge_nodes[key] = (node_at(xa + (xb - xa) / 3.0), node_at(xa + 2.0 * (xb - xa) / 3.0))
+        n0, n1 = edge_nodes[key]
+        return [n0, n1] if key == (a, b) else [n1, n0]
+
+    def face(a, b, c):
+        key = tuple(sorted((a, b, c)))
+        if key not in face_nodes:
+            face_nodes[key] = node_at(mesh.verts[list(key)].mean(axis=0))
+        return face_nodes[key]
+
+    def cubic(vs):
+        if D == 3:
+            out = list(vs)
+            for a, b in _GMSH_TET_EDGES:
+                out += edge(vs[a], vs[b])
+            out += [face(vs[a], vs[b], vs[c]) for a, b, c in _GMSH_TET_FACES]
+            return out
+        out = list(vs)
+        for a, b in _GMSH_TRI_EDGES:
+            out += edge(vs[a], vs[b])
+        return out + [face(*vs)]
+
+    vol_type = (_GMSH_TET20 if D == 3 else _GMSH_TRI10) if order == 3 else \
+        (_GMSH_TET if D == 3 else _GMSH_TRI)
+    facet_type = (_GMSH_TRI10 if D == 3 else _GMSH_LINE4) if order == 3 else \
+        (_GMSH_TRI if D == 3 else _GMSH_LINE)
+    vol_tag = 0
+    for t, (dim, _) in (physical_names or {}).items():
+        if dim == D:
+            vol_tag = t
     elems = []
     for z in np.nonzero(nbf < 0)[0]:
         e, f = divmod(int(z), D + 1)
         tag = (tag_of_bc or {}).get(int(bctype[z]), 1)
-        nodes = " ".join(str(int(mesh.cells[e, k]) + 1) for k in _FACE_VERTS[D][f])
-        elems.append(f"{facet_type} 2 {tag} {tag} {nodes}")
-    for c in mesh.cells:
-        elems.append(f"{vol_type} 2 0 0 " + " ".join(str(int(k) + 1) for k in c))
+        fv = [int(mesh.cells[e, k]) for k in _FACE_VERTS[D][f]]
+        if order == 3:
+            if D == 3:  # triangle10 facet: vertices, edges (0,1), (1,2), (2,0), centre
+                nodes = fv + edge(fv[0], fv[1]) + edge(fv[1], fv[2]) + edge(fv[2], fv[0]) + \
+                    [face(*fv)]
+            else:  # line4: end points, then the two interior nodes
+                nodes = fv + edge(fv[0], fv[1])
+        else:
+            nodes = fv
+        elems.append(f"{facet_type} 2 {tag} {tag} " + " ".join(str(k + 1) for k in nodes))
+    for i, c in enumerate(mesh.cells):
+        vs = [int(k) for k in c]
+        if reverse is not None and reverse[i]:
+            vs[1], vs[2] = vs[2], vs[1]
+        nodes = cubic(vs) if order == 3 else vs
+        elems.append(f"{vol_type} 2 {vol_tag} {vol_tag} " + " ".join(str(k + 1) for k in nodes))
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat"]
+    if physical_names:
+        lines += ["$PhysicalNames", str(len(physical_names))]
+        lines += [f'{dim} {t} "{name}"' for t, (dim, name) in physical_names.items()]
+        lines.append("$EndPhysicalNames")
+    lines += ["$Nodes", str(len(pts))]
+    for i, v in enumerate(pts):
+        z = float(v[2]) if D == 3 else 0.0
+        lines.append(f"{i + 1} {float(v[0])!r} {float(v[1])!r} {z!r}")
+    lines.append("$EndNodes")
     lines += ["$Elements", str(len(elems))]
     lines += [f"{i + 1} {s}" for i, s in enumerate(elems)]
     lines.append("$EndElements")

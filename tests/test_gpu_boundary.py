@@ -174,3 +174,28 @@ def test_reference_3d_problem_solves_and_converges():
         g.close()
     print("L2 errors", errs)
     assert errs[1] / errs[2] > 13.0 and errs[0] / errs[1] > 10.0
+
+
+def test_solve_from_a_p3_gmsh_file(tmp_path):
+    """The reference's workflow: a P3 gmsh file with physical groups "diri" and "neu"
+    (z21-cube-mesh/cube_ho_poi.geo) is read (tetra20 cells renumbered into the reference's
+    local order, groups numbered 0 and 1), the exp problem is posed on it and solved on
+    the GPU; the result equals the oracle's solve on the same mesh and is O(h^4)-close to
+    the exact solution."""
+    from dg_nn_public_amd import io
+    m0 = meshgen.cube_mesh(3, jitter=0.1, seed=3)
+    bct0 = _top_neumann(m0)
+    p = tmp_path / "cube_ho_poi.msh"
+    io.write_msh(p, m0, bctype=bct0, tag_of_bc={0: 1, 1: 2}, order=3,
+                 physical_names={1: (2, "diri"), 2: (2, "neu"), 3: (3, "fluid")},
+                 reverse=np.arange(m0.nele) % 3 == 0)
+    m, bct = io.read_msh(p)
+    assert (bct == 1).sum() == (bct0 == 1).sum() > 0
+    u, f, gn = _exp_problem(m)
+    g = DGDiffusion.from_mesh(m, None, bctype=bct, device=DEV, linear_solver="cg-mg")
+    x = to_np(g.solve_for_diff(g.new_vector(), f, [u, gn], rtol=1e-12))
+    sp = dg.make_space(m, bctype=bct)
+    x_ref = np.linalg.solve(dg.assemble_dense(sp), dg.get_rhs(sp, f, u, u_neumann=gn))
+    print("file solve vs oracle", rel(x, x_ref), "l2 err", dg.get_l2_error(sp, x, u)[0])
+    assert rel(x, x_ref) < 1e-9
+    assert dg.get_l2_error(sp, x, u)[0] < 1e-4

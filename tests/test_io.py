@@ -116,3 +116,89 @@ def test_vtu_output(tmp_path):
     assert np.allclose(u, pb.u_exact.reshape(m.nele, m.nloc)[:, :3].ravel())
     types = np.array(piece.find("Cells/DataArray[@Name='types']").text.split(), dtype=int)
     assert (types == 5).all()
+
+
+def _match_nodes(a, b):
+    """per element, the permutation p with a[:, :, p] == b (node positions)"""
+    out = np.empty(a.shape[::2], dtype=np.int64)
+    for e in range(a.shape[0]):
+        d = np.abs(a[e][:, :, None] - b[e][:, None, :]).max(axis=0)  # (nloc_a, nloc_b)
+        out[e] = d.argmin(axis=0)
+        assert d.min(axis=0).max() < 1e-12
+    return out
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_cubic_msh_is_read_in_the_reference_node_order(tmp_path, D):
+    """A P3 gmsh file (tetra20 / triangle10 in gmsh's own node numbering, some cells
+    reversed) is read into the reference's local order: every node sits where the
+    reference's P3 table puts it for the element's vertices (z02/mesh_init.py:276-289),
+    every element has det J > 0, and the operator equals the one of the original mesh up
+    to the element-local renumbering."""
+    from oracle import dg
+    m = meshgen.square_mesh(4, seed=1) if D == 2 else meshgen.cube_mesh(2, seed=2)
+    rev = np.random.default_rng(0).random(m.nele) < 0.4
+    p = tmp_path / "p3.msh"
+    io.write_msh(p, m, order=3, reverse=rev)
+    r, bct = io.read_msh(p)
+    assert r.D == D and r.nele == m.nele and r.x_ref_in.shape == m.x_ref_in.shape
+    assert io.affine_deviation(r) < 1e-12
+    v = r.verts[r.cells]
+    J = v[:, :D] - v[:, D:]
+    assert np.all(np.linalg.det(J) > 0)
+    perm = _match_nodes(r.x_ref_in, m.x_ref_in)  # read node perm[e, n] = original node n
+    kap = np.exp(np.random.default_rng(1).standard_normal(m.nele))
+    u = np.random.default_rng(2).standard_normal((m.nele, m.nloc))
+    ur = np.take_along_axis(np.zeros_like(u), perm, 1)
+    ur[np.arange(m.nele)[:, None], perm] = u
+    y = dg.apply_A(dg.make_space(m, kap), u.reshape(-1)).reshape(m.nele, -1)
+    yr = dg.apply_A(dg.make_space(r, kap), ur.reshape(-1)).reshape(m.nele, -1)
+    assert np.abs(yr[np.arange(m.nele)[:, None], perm] - y).max() < 1e-11 * np.abs(y).max()
+    _, _, ref_bct = host.topology(r.cells, D)
+    assert np.array_equal(bct, ref_bct)
+
+
+def test_tetra20_right_handed_cells_use_the_reference_renumbering(tmp_path):
+    """For a gmsh cell with det(x1-x0, x2-x0, x3-x0) > 0 the reader's local node i is
+    gmsh node TET20_ORDER[i] (the reference's table): its vertices are gmsh 3, 1, 2, 0."""
+    m = meshgen.cube_mesh(1, jitter=0.0)
+    one = meshgen.Mesh(D=3, x_ref_in=m.x_ref_in[:1], cells=m.cells[:1], verts=m.verts)
+    gm = one.verts[one.cells[0]]
+    if np.linalg.det(gm[1:] - gm[0]) < 0:  # make the written cell right-handed in gmsh terms
+        c = one.cells.copy()
+        c[0, [1, 2]] = c[0, [2, 1]]
+        one = meshgen.Mesh(D=3, x_ref_in=one.x_ref_in, cells=c, verts=one.verts)
+    p = tmp_path / "one.msh"
+    io.write_msh(p, one, order=3)
+    r, _ = io.read_msh(p)
+    g = one.cells[0]
+    assert np.allclose(r.verts[r.cells[0]], one.verts[[g[3], g[1], g[2], g[0]]], atol=1e-15)
+    assert io.affine_deviation(r) < 1e-12
+
+
+def test_physical_names_number_boundary_groups_in_file_order(tmp_path):
+    """The reference's cube (z21-cube-mesh/cube_ho_poi.geo): groups "diri" (five faces)
+    and "neu" (z = 1) become glb_bcface_type 0 and 1, as its cell-set order numbers them
+    (z02/mesh_init.py:697-741)."""
+    m = meshgen.cube_mesh(2, seed=5)
+    _, _, bct = host.topology(m.cells, 3)
+    fv = [(2, 1, 3), (0, 2, 3), (1, 0, 3), (0, 1, 2)]
+    want = bct.copy()
+    for z in np.nonzero(bct == 0)[0]:
+        e, f = divmod(int(z), 4)
+        if np.all(np.abs(m.verts[m.cells[e, list(fv[f])], 2] - 1.0) < 1e-12):
+            want[z] = 1
+    p = tmp_path / "cube.msh"
+    io.write_msh(p, m, bctype=want, tag_of_bc={0: 1, 1: 2}, order=3,
+                 physical_names={1: (2, "diri"), 2: (2, "neu"), 3: (3, "fluid")})
+    r, got = io.read_msh(p)
+
+    def by_face(mesh, t):  # face centroid -> type (the reader renumbers local faces)
+        out = {}
+        for z in np.nonzero(t >= 0)[0]:
+            e, f = divmod(int(z), 4)
+            c = mesh.verts[mesh.cells[e, list(fv[f])]].mean(axis=0)
+            out[tuple(np.round(c, 12))] = int(t[z])
+        return out
+
+    assert by_face(r, got) == by_face(m, want) and (got == 1).sum() == 8
