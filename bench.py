@@ -1,26 +1,31 @@
 """Benchmark of the IP-DG diffusion solve on one MI355X.
 
-Workload (3D, the configuration the metric is quoted on): P3 tetrahedra in the unit
-cube from a 24^3 hex grid, 6 Kuhn tets per hex (82,944 elements, 1,658,880 DG DOFs),
-interior vertices jittered +-0.15h (seed 0), lognormal diffusivity per element
-(mu 0, sigma 1, seed 1), Dirichlet boundary, manufactured sine source.  A step is one
-full solve ||b - A x|| <= 1e-10 ||b|| from x = 0 with GMRES(30) + SFC multigrid, inputs
-resident in HBM.
+Default line (``python bench.py``): the serving workload the metric is quoted on.  Poisson
+arrivals of diffusion-solve requests (half 2D P3 triangle meshes, half 3D P3 tet meshes,
+2e3..2e5 elements log-uniform, plus 5 % large 3D jobs of 55^3 x 6 = 998,250 tets; jittered
+structured grids, lognormal diffusivity sigma U[0.5, 2], sine source of amplitude
+U[0.5, 2]; seeded) are served by the continuous-batching loop (``dg_nn_public_amd.serve``)
+under an HBM budget, every request solved to ||b - A x|| <= 1e-10 ||b||.  A step is one
+stream of ``--nreq`` requests arriving at ``--rate`` req/s; ``value`` is the served solves/s
+of the mixed-precision solver (FP32 flexible GMRES cycles inside FP64 iterative
+refinement) over the timed steps, with p50/p99 latency; the plain FP64 solver serves the
+same streams beside it.  Inputs (meshes, diffusivity, sources) are in host memory when a
+request arrives, as a served request's are; each job's set-up (context, multigrid
+hierarchy, smoother) is part of its latency.
 
-Prints one JSON line: solves/s of the mixed-precision solver (FP32 flexible GMRES cycles
-inside FP64 iterative refinement to ||r||/||b|| <= 1e-10) as `value`, its latency p50/p99,
-the plain FP64 solver and the bf16-preconditioned variant beside it, operator-apply GDoF/s
-per kernel family, the roofline of the FP32 apply (and of the FP64 one), and the CPU oracle
-baseline.
+Beside it: the 48^3 x 6 tet mesh (663,552 elements, 13.3M DOFs, sigma 2) with the operator
+apply in GDoF/s per kernel family and the roofline of the FP32 smoothing sweep (the
+dominant kernel of the mixed solve, HBM-bound) and of the FP32 / FP64 applies; the
+24^3 x 6 single-solve line (FP64, FP32-IR, bf16-IR); and the CPU baseline: the FP64 NumPy
+oracle timed on this host's cores (a 32 x 32 x 2 triangle solve end to end, and one
+24^3 x 6 apply), 5 warm-ups and the median of 5 timed runs each.
 
 Other workloads (``--workload``), each printing its own JSON line:
-  precision  48^3 x 6 tets (663,552 elements), sigma 2: operator error of the FP32 and
-             bf16-input applies against FP64, iteration counts of FP64 / FP32 / bf16 solves
-  batch      256 independent solves (half 2D, half 3D, 2e3..2e5 elements log-uniform,
-             sigma 0.5..2) packed per dimension into one context each
-  serve      Poisson arrivals of solve requests (the batch size mix plus 5 % 3D jobs of
-             55^3 x 6 = 998,250 tets) at several rates, continuous batching under an
-             HBM budget; p50/p99 latency and solves/s, FP32-IR and FP64
+  solve      the 24^3x6 single-solve line alone (the previous default)
+  precision  48^3 x 6 tets, sigma 2: operator error of the FP32 and bf16-input applies
+             against FP64, iteration counts of FP64 / FP32 / bf16 solves
+  batch      256 independent solves packed per dimension into one context each
+  sweep      the serving workload at several arrival rates
 """
 from __future__ import annotations
 
@@ -36,10 +41,10 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12   # MI355X FP64 vector/matrix, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12     # B/s
-# calibrated HBM bytes of one FP32 vector smoothing sweep (k_residual_w<float,3,5,true>) on
-# the 24^3x6 workload: FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3 --pmc passes
-# (profiles/pmc_bytes_apply_tet24.md)
-PMC_SWEEP = 200.3e6
+METRIC = "solves/s + p50/p99 latency at rtol 1e-10; DG apply GDoF/s, mixed vs FP64"
+# calibrated HBM bytes per launch of the FP32 smoothing sweep (k_residual_w<float,3,5,true>):
+# FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3 --pmc passes (profiles/), by element count
+PMC_SWEEP = {82944: 200.3e6}
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):
@@ -78,49 +83,361 @@ def sweep_flops_per_elem(D, nloc, nface, sngi):
     return 2 * nface * (trace + lift) + 2 * nloc * nloc
 
 
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=24, help="cells per direction")
-    ap.add_argument("--dim", type=int, default=3, choices=[2, 3],
-                    help="2: P3 triangles on the unit square (n x n x 2, jitter 0.2h)")
+    ap.add_argument("--workload", default="serve",
+                    choices=["serve", "solve", "precision", "batch", "sweep"])
+    # serving
+    ap.add_argument("--nreq", type=int, default=48, help="requests per stream")
+    ap.add_argument("--rate", type=float, default=500.0, help="arrival rate of a stream, req/s")
+    ap.add_argument("--rates", default="10,50,200,500", help="sweep: arrival rates, req/s")
+    ap.add_argument("--large-n", type=int, default=55, help="hex cells of the large jobs")
+    ap.add_argument("--emax", type=float, default=2e5, help="largest element count of the mix")
+    ap.add_argument("--jobs", type=int, default=4, help="concurrent job slots")
+    ap.add_argument("--budget", type=float, default=200e9, help="HBM budget, bytes")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 serving streams")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--cpu-warmup", type=int, default=5)
+    # single-solve / apply settings
+    ap.add_argument("--n", type=int, default=24, help="cells per direction (solve workload)")
+    ap.add_argument("--dim", type=int, default=3, choices=[2, 3])
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--applies", type=int, default=50)
+    ap.add_argument("--sweeps", type=int, default=40)
     ap.add_argument("--face-kappa", default="harmonic", choices=["harmonic", "mean"],
                     help="interior-face diffusivity; 'mean' is the reference's arithmetic "
                          "mean, which is not coercive at sigma=1 and stagnates")
     ap.add_argument("--apply-only", action="store_true",
-                    help="only operator applies (FP64, FP32) and a 256 MiB elementwise copy "
-                         "that calibrates HBM byte counters; for rocprofv3 --pmc passes")
-    ap.add_argument("--smoother", default="direct", choices=["direct", "jacobian"],
-                    help="block smoother: exact block inverse (the reference's variable-"
-                         "diffusivity solver) or 3 Jacobi sweeps (its constant-diffusivity one)")
-    ap.add_argument("--graph", action="store_true",
-                    help="replay the SFC V-cycle from a captured HIP graph instead of "
-                         "launching its kernels one by one (the library default)")
-    ap.add_argument("--no-cpu", action="store_true")
+                    help="solve workload: only applies, sweeps and a calibration copy "
+                         "(for rocprofv3 --pmc passes)")
+    ap.add_argument("--smoother", default="direct", choices=["direct", "jacobian"])
+    ap.add_argument("--graph", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
-    ap.add_argument("--workload", default="solve",
-                    choices=["solve", "precision", "batch", "serve"])
-    ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates, req/s")
-    ap.add_argument("--nreq", type=int, default=48, help="serve: requests per rate")
-    ap.add_argument("--large-n", type=int, default=55, help="serve: hex cells of large jobs")
     ap.add_argument("--nsolves", type=int, default=256, help="batch workload size")
-    ap.add_argument("--emax", type=float, default=2e5, help="batch: largest element count")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not "
                                    "supported (multi-GPU sharding is design-only)"}))
         sys.exit(2)
-    if args.workload == "precision":
-        return run_precision(args)
-    if args.workload == "batch":
-        return run_batch(args)
-    if args.workload == "serve":
-        return run_serve(args)
+    {"serve": run_headline, "solve": run_solve, "precision": run_precision,
+     "batch": run_batch, "sweep": run_serve}[args.workload](args)
 
+
+# ----------------------------------------------------------------------------------------
+def build_stream(nreq, large_n, emax, seed=11):
+    """The serving mix: ``nreq`` requests, half 2D / half 3D with element counts
+    log-uniform in [2e3, emax], 5 % (at least one) large 3D jobs of large_n^3 x 6 tets;
+    sigma and source amplitude U[0.5, 2]; shuffled, seeded."""
+    import numpy as np
+
+    from dg_nn_public_amd import meshgen
+    rng = np.random.default_rng(7)
+    specs = _batch_requests(nreq, emax, seed=seed)
+    rng.shuffle(specs)
+    nlarge = max(1, int(round(0.05 * nreq)))
+    for i in range(nlarge):
+        specs[i * (len(specs) // nlarge)] = dict(D=3, n=large_n, mesh_seed=5000 + i,
+                                                 kappa_seed=6000 + i, sigma=1.0)
+    payload = []
+    for i, r in enumerate(specs):
+        m = meshgen.square_mesh(r["n"], seed=r["mesh_seed"]) if r["D"] == 2 else \
+            meshgen.cube_mesh(r["n"], seed=r["mesh_seed"])
+        sig = float(rng.uniform(0.5, 2.0))
+        amp = float(rng.uniform(0.5, 2.0))
+        payload.append((m, meshgen.sine_problem(m, sigma=sig, kappa_seed=r["kappa_seed"],
+                                                amplitude=amp)))
+        if i % 12 == 11:
+            log(f"generated {i + 1}/{len(specs)} request meshes")
+    return payload, nlarge
+
+
+def serve_stream(payload, rate, inner, args, seed):
+    import numpy as np
+
+    from dg_nn_public_amd.serve import Request, Server
+    ar = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, size=len(payload)))
+    reqs = [Request(i, m, p.kappa, p.f, p.u_bc, float(a))
+            for i, ((m, p), a) in enumerate(zip(payload, ar))]
+    srv = Server(hbm_budget=args.budget, max_jobs=args.jobs, max_batch=64, inner=inner)
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    return res, time.perf_counter() - t0
+
+
+def run_headline(args):
+    import numpy as np
+    import torch
+
+    from dg_nn_public_amd.serve import summarize
+    torch.zeros(1, device="cuda:0")
+    t0 = time.time()
+    payload, nlarge = build_stream(args.nreq, args.large_n, args.emax)
+    nele = sum(m.nele for m, _ in payload)
+    log(f"{len(payload)} requests, {nele} elements ({nlarge} of {args.large_n}^3x6), meshes in "
+        f"{time.time() - t0:.1f}s")
+    # warm-up: a short stream (kernel loads, allocator, host thread pools)
+    for w in range(args.warmup):
+        serve_stream(payload[: max(4, len(payload) // 6)], args.rate, torch.float32, args, 99 + w)
+    log("warm-up done")
+    out = {}
+    variants = [("f32_ir", torch.float32)] + ([] if args.no_fp64 else [("f64", torch.float64)])
+    for name, inner in variants:
+        results, walls = [], []
+        for k in range(args.steps):
+            torch.cuda.synchronize()
+            res, wall = serve_stream(payload, args.rate, inner, args, seed=1000 + k)
+            torch.cuda.synchronize()
+            results += res
+            walls.append(wall)
+            s = summarize(res)
+            log(f"{name} stream {k}: {s}")
+        lat = np.array([r.latency for r in results if r.error is None])
+        good = [r for r in results if r.error is None]
+        jobs = {}
+        for i, r in enumerate(results):  # one job_info per (stream, job)
+            if r.job_info is not None:
+                jobs[(i // len(payload), r.job)] = r.job_info
+        phases = {}
+        for j in jobs.values():
+            for k, v in j["phases"].items():
+                phases[k] = phases.get(k, 0.0) + v / args.steps
+        out[name] = {"solves_per_s": len(good) / sum(walls), "stream_s": walls,
+                     "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                     "max_relres": float(max(r.relres for r in good)),
+                     "its_mean": float(np.mean([r.its for r in good])),
+                     "not_converged": sum(1 for r in good if not r.converged),
+                     "errors": len(results) - len(good),
+                     "jobs_per_stream": len(jobs) / args.steps,
+                     "job_setup_s_per_stream": sum(j["setup_s"] for j in jobs.values()) / args.steps,
+                     "job_solve_s_per_stream": sum(j["solve_s"] for j in jobs.values()) / args.steps,
+                     "setup_phases_s_per_stream": phases}
+    head = out["f32_ir"]
+    line = {
+        "metric": METRIC,
+        "value": head["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * float(np.mean(head["stream_s"])),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {len(payload)} diffusion-solve requests per step, "
+                               f"Poisson arrivals at {args.rate:g} req/s; half 2D P3 tris / half "
+                               f"3D P3 tets, 2e3..{args.emax:.0e} elements log-uniform, "
+                               f"{nlarge} job(s) of {args.large_n}^3x6 tets; lognormal kappa "
+                               "sigma U[0.5,2], harmonic face kappa, exact block smoother, "
+                               "GMRES(30)+SFC-MG, FP32 flexible cycles in FP64 iterative "
+                               f"refinement to rtol 1e-10; {args.jobs} job slots, "
+                               f"{args.budget / 1e9:g} GB HBM budget",
+                   "requests_per_step": len(payload), "elements_per_step": nele,
+                   "parallelism": "single GPU"},
+        "latency_s": {"p50": head["p50_s"], "p99": head["p99_s"]},
+        "serve": out,
+        "speedup_vs_fp64": (head["solves_per_s"] / out["f64"]["solves_per_s"]
+                            if "f64" in out else None),
+    }
+    if not args.no_c3:
+        c3 = c3_block(args)
+        line["c3_48cube"] = c3
+        line["roofline"] = c3.pop("roofline")
+    if not args.no_c2:
+        line["c2_24cube"] = c2_block(args)
+    line["cpu_baseline"] = None if args.no_cpu else cpu_block(args)
+    print(json.dumps(line))
+
+
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def c3_block(args):
+    """48^3 x 6 tets, sigma 2: apply GDoF/s per kernel family and the rooflines, all timed
+    with HIP events on the stream the library launches on."""
+    import torch
+
+    from dg_nn_public_amd import meshgen
+    from dg_nn_public_amd.diffusion import DGDiffusion
+    t0 = time.time()
+    mesh = meshgen.cube_mesh(48, jitter=0.15, seed=0)
+    pb = meshgen.sine_problem(mesh, sigma=2.0, kappa_seed=1)
+    g = DGDiffusion.from_mesh(mesh, pb.kappa, face_kappa="harmonic", blk_solver="direct")
+    g._get_RAR_and_sfc_data_Um()
+    tuned = g.autotune(reps=5)
+    b = g._get_rhs(g.new_vector(), pb.f, pb.u_bc)
+    torch.cuda.synchronize()
+    log(f"C3 set up in {time.time() - t0:.1f}s, autotune {tuned}")
+    stream = torch.cuda.current_stream()
+    nele, nloc = g.nele, g.nloc
+    ta = {"f64": _time_apply(g, torch.float64, args.applies),
+          "f32_vector": _time_apply(g, torch.float32, args.applies, "vector"),
+          "f32_mfma": _time_apply(g, torch.float32, args.applies, "mfma"),
+          "bf16_mfma": _time_apply(g, torch.float32, args.applies, bf16=True)}
+    g.set_f32_kernel(tuned["f32_kernel"])
+    b32 = b.float()
+    xs = torch.zeros(g.ndof, dtype=torch.float32, device="cuda:0")
+    g.smooth_n(xs, b32, 4)
+    e0, e1 = _events()
+    e0.record(stream)
+    g.smooth_n(xs, b32, args.sweeps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_sweep = e0.elapsed_time(e1) / 1e3 / args.sweeps
+    log(f"C3 apply us {({k: round(v * 1e6, 1) for k, v in ta.items()})}, sweep {t_sweep * 1e6:.1f} us")
+    F = flops_per_elem(3, nloc, g.ngi, g.sngi, g.nface)
+    SB = sweep_bytes_per_elem(3, nloc, g.nface, g.sngi)
+    SF = sweep_flops_per_elem(3, nloc, g.nface, g.sngi)
+    vec = tuned["f32_kernel"] == "vector"
+    t32 = ta["f32_vector"] if vec else ta["f32_mfma"]
+    gbps = SB * nele / t_sweep / 1e9
+    out = {
+        "workload": f"3D P3 tets 48^3x6 = {nele} elements, {nele * nloc} DOFs, lognormal kappa "
+                    "sigma 2, harmonic face kappa",
+        "apply_us": {k: v * 1e6 for k, v in ta.items()},
+        "apply_gdofs": {k: nele * nloc / v / 1e9 for k, v in ta.items()},
+        "autotune": tuned,
+        "roofline": {"bound": "hbm", "achieved": gbps, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": gbps * 1e9 / PEAK_HBM, "traffic": PMC_SWEEP.get(nele),
+                     "kernel": "FP32 smoothing sweep x = (1-w)x + S(b - A_off x) with the face "
+                               "traces written for the next sweep (" +
+                               ("k_residual_w<float,3,5,true>" if vec else "MFMA trace + residual")
+                               + "), 48^3x6 tets; the largest share of the mixed solve's kernel "
+                               "time",
+                     "bytes_per_elem": SB, "flop_per_elem": SF, "elements": nele,
+                     "avg_launch_us": t_sweep * 1e6, "launches_timed": args.sweeps,
+                     "achieved_tflops": SF * nele / t_sweep / 1e12},
+        "roofline_apply_f32": {"bound": "mfma", "achieved": F * nele / t32 / 1e12,
+                               "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                               "frac": F * nele / t32 / PEAK_FP32, "flop_per_elem": F,
+                               "kernel": "FP32 operator apply (" + ("vector" if vec else "MFMA")
+                                         + " kernels, autotuned)",
+                               "achieved_GBps_min_traffic":
+                                   bytes_per_elem(3, nloc, g.nface, 4) * nele / t32 / 1e9},
+        "roofline_apply_f64": {"bound": "mfma", "achieved": F * nele / ta["f64"] / 1e12,
+                               "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                               "frac": F * nele / ta["f64"] / PEAK_FP64,
+                               "achieved_GBps_min_traffic":
+                                   bytes_per_elem(3, nloc, g.nface, 8) * nele / ta["f64"] / 1e9},
+    }
+    g.close()
+    return out
+
+
+def c2_block(args):
+    """24^3 x 6 tets, sigma 1: one solve to 1e-10 with each inner precision."""
+    import numpy as np
+    import torch
+
+    from dg_nn_public_amd import meshgen
+    from dg_nn_public_amd.diffusion import DGDiffusion
+    mesh = meshgen.cube_mesh(24, jitter=0.15, seed=0)
+    pb = meshgen.sine_problem(mesh, sigma=1.0, kappa_seed=1)
+    g = DGDiffusion.from_mesh(mesh, pb.kappa, face_kappa="harmonic", blk_solver="direct")
+    g._get_RAR_and_sfc_data_Um()
+    g.autotune(reps=5)
+    b = g._get_rhs(g.new_vector(), pb.f, pb.u_bc)
+    out = {"workload": f"3D P3 tets 24^3x6 = {g.nele} elements, {g.ndof} DOFs, lognormal "
+                       "kappa sigma 1, harmonic face kappa, one solve to rtol 1e-10"}
+    for name, inner in (("f64", torch.float64), ("f32_ir", torch.float32),
+                        ("bf16_ir", torch.bfloat16)):
+        lat = []
+        for i in range(1 + args.steps):
+            x = g.new_vector()
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            g._gmres_mg_solver(x, b, rtol=1e-10, inner=inner)
+            torch.cuda.synchronize()
+            if i:
+                lat.append(time.perf_counter() - ts)
+        out[name] = {"solves_per_s": len(lat) / sum(lat), "ms": 1e3 * float(np.median(lat)),
+                     **g.last_stats}
+    out["speedup_f32_vs_f64"] = out["f32_ir"]["solves_per_s"] / out["f64"]["solves_per_s"]
+    log(f"C2 {out}")
+    g.close()
+    return out
+
+
+def _cpu_info():
+    import platform
+    model = platform.processor()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def cpu_block(args):
+    """The FP64 NumPy oracle on this host: a 32 x 32 x 2 P3 triangle solve end to end
+    (space, multigrid set-up, right-hand side, GMRES(30) to 1e-10) and one 24^3 x 6 apply;
+    BLAS threads = the cores this process may use; warm-ups, then the median of the timed
+    runs."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from dg_nn_public_amd import meshgen
+    from oracle import dg, krylov, mg
+    cores = len(os.sched_getaffinity(0))
+    with threadpool_limits(cores):
+        m = meshgen.square_mesh(32, jitter=0.2, seed=0)
+        pb = meshgen.sine_problem(m)
+
+        def solve():
+            sp = dg.make_space(m, pb.kappa)
+            mgd = mg.build_mg(sp, m.verts)
+            b = dg.get_rhs(sp, pb.f, pb.u_bc)
+            return krylov.gmres_mg_solver(sp, mgd, np.zeros(sp.ndof), b, rtol=1e-10, m=30)
+
+        ts = []
+        for i in range(args.cpu_warmup + args.cpu_reps):
+            t = time.perf_counter()
+            _, its = solve()
+            if i >= args.cpu_warmup:
+                ts.append(time.perf_counter() - t)
+        t_c1 = float(np.median(ts))
+        log(f"CPU C1 solve {t_c1:.2f}s ({its} its)")
+        m2 = meshgen.cube_mesh(24, jitter=0.15, seed=0)
+        pb2 = meshgen.sine_problem(m2, sigma=1.0, kappa_seed=1)
+        sp2 = dg.make_space(m2, pb2.kappa, face_kappa="harmonic")
+        x = np.random.default_rng(0).standard_normal(sp2.ndof)
+        ta = []
+        for i in range(args.cpu_warmup + args.cpu_reps):
+            t = time.perf_counter()
+            dg.apply_A(sp2, x)
+            if i >= args.cpu_warmup:
+                ta.append(time.perf_counter() - t)
+        t_c2 = float(np.median(ta))
+        log(f"CPU C2 apply {t_c2:.2f}s")
+    return {"value": 1.0 / t_c1, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 NumPy oracle: 2D P3 32x32x2 triangles (20,480 DOFs) solved end to end "
+                      f"to rtol 1e-10 ({its} GMRES(30) its), median of {args.cpu_reps} after "
+                      f"{args.cpu_warmup} warm-ups; and one 24^3x6 tet apply (1.66M DOFs)",
+            "c1_solve_s": t_c1, "c2_apply_s": t_c2,
+            "c2_apply_gdofs": m2.nele * m2.nloc / t_c2 / 1e9,
+            "os_cpu_count": os.cpu_count(), "threads": cores, "cpu_model": _cpu_info()}
+
+
+def run_solve(args):
+    """The 24^3x6 (or --n/--dim) single-solve line: FP64, FP32-IR and bf16-IR solves, apply
+    GDoF/s per kernel family, the FP32 sweep roofline, the oracle apply baseline."""
     import numpy as np
     import torch
 
@@ -296,7 +613,7 @@ def main():
     ach32 = F * nele / t32
     # calibrated HBM bytes per launch of this workload (FETCH_SIZE x 2 + WRITE_SIZE, two
     # rocprofv3 --pmc passes, profiles/pmc_bytes_apply_tet24.md)
-    pmc = {"f64": 240e6, "f32_vector": 113e6, "f32_mfma": 125e6, "f32_sweep": PMC_SWEEP}
+    pmc = {"f64": 240e6, "f32_vector": 113e6, "f32_mfma": 125e6, "f32_sweep": PMC_SWEEP.get(82944)}
     on_profiled = args.dim == 3 and args.n == 24
     SB = sweep_bytes_per_elem(args.dim, nloc, g.nface, g.sngi)
     SF = sweep_flops_per_elem(args.dim, nloc, g.nface, g.sngi)
@@ -365,6 +682,8 @@ def main():
     print(json.dumps(out))
 
 
+
+
 def _time_apply(g, dtype, k, kernel="mfma", bf16=False):
     import torch
     if dtype == torch.float32:
@@ -381,7 +700,7 @@ def _time_apply(g, dtype, k, kernel="mfma", bf16=False):
         g.apply(x, bb, out=r, bf16=bf16)
     e1.record(stream)
     torch.cuda.synchronize()
-    g.set_f32_kernel("mfma")
+    g.set_f32_kernel("vector")
     return e0.elapsed_time(e1) / 1e3 / k
 
 
@@ -427,7 +746,7 @@ def run_precision(args):
             g.set_f32_kernel(kern)
             y = g.apply(x64.float(), bf16=bf).double()
             out[kname] = float(torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref))
-        g.set_f32_kernel("mfma")
+        g.set_f32_kernel("vector")
         return out
 
     gen = torch.Generator(device="cpu").manual_seed(2)

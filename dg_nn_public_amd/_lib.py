@@ -40,7 +40,10 @@ SIGNATURES = {
     "dgmi_ctx_destroy": (C.c_int, [_P]),
     "dgmi_ctx_create_batch": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, _I64P, _I64P,
                                         C.POINTER(_P)]),
+    "dgmi_ctx_create_batch_parts": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, _I64P, _I64P,
+                                              C.POINTER(_DP), C.POINTER(_P)]),
     "dgmi_ctx_info": (C.c_int, [_P, _I64P]),
+    "dgmi_setup_timing": (C.c_int, [_P, _DP]),
     "dgmi_topology": (C.c_int, [_P, _I64P, _I64P, _I64P]),
     "dgmi_element_tables": (C.c_int, [_P, _DP, _DP, _DP, _DP, _DP, _DP, _I64P]),
     "dgmi_apply": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),

@@ -21,13 +21,19 @@ from .diffusion import GMRES_ITS, GMRES_M, DGDiffusion
 @dataclass
 class Packed:
     """Union of meshes of one dimension; component c owns elements
-    [elem_off[c], elem_off[c+1]) and vertices [vert_off[c], vert_off[c+1])."""
+    [elem_off[c], elem_off[c+1]) and vertices [vert_off[c], vert_off[c+1]).  The P3 node
+    coordinates stay in the components' own arrays (``x_parts``); ``x_ref_in`` concatenates
+    them on first use."""
     D: int
-    x_ref_in: np.ndarray
+    x_parts: list
     cells: np.ndarray
     verts: np.ndarray
     elem_off: np.ndarray
     vert_off: np.ndarray
+
+    @property
+    def x_ref_in(self):
+        return np.concatenate(self.x_parts)
 
 
 def pack(meshes) -> Packed:
@@ -45,8 +51,8 @@ def pack(meshes) -> Packed:
         vo[i + 1] = vo[i] + m.verts.shape[0]
     cells = np.concatenate([np.asarray(m.cells, dtype=np.int64) + vo[i]
                             for i, m in enumerate(meshes)])
-    return Packed(D, np.concatenate([m.x_ref_in for m in meshes]), cells,
-                  np.concatenate([m.verts for m in meshes]), eo, vo)
+    parts = [np.ascontiguousarray(m.x_ref_in, dtype=np.float64) for m in meshes]
+    return Packed(D, parts, cells, np.concatenate([m.verts for m in meshes]), eo, vo)
 
 
 class DGBatch:
@@ -58,7 +64,7 @@ class DGBatch:
                  blk_solver="direct", graph_vcycle=True, alloc_limit=0):
         p = pack(meshes)
         kappa = np.concatenate([np.asarray(k, dtype=np.float64).reshape(-1) for k in kappas])
-        self.space = DGDiffusion(p.x_ref_in, p.cells, p.verts, kappa, device=device,
+        self.space = DGDiffusion(p.x_parts, p.cells, p.verts, kappa, device=device,
                                  face_kappa=face_kappa, blk_solver=blk_solver,
                                  components=(p.elem_off, p.vert_off), alloc_limit=alloc_limit)
         self.packed = p

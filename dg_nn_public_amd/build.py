@@ -19,7 +19,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("DGMI_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["apply.hip", "apply_mfma.hip", "mg.hip", "krylov.hip", "dgmi_api.hip"]
+HIP_SOURCES = ["apply.hip", "apply_mfma.hip", "mg.hip", "krylov.hip", "geometry.hip",
+               "dgmi_api.hip"]
 HOST_SOURCES = ["setup_host.cpp"]
 
 HIP_FLAGS = [

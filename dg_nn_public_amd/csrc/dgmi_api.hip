@@ -1,6 +1,7 @@
 // extern "C" entry points of libdgmi.so (declared in include/dgmi.h), context set-up,
 // multigrid set-up, the V-cycle preconditioner and the GMRES driver.
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
@@ -13,6 +14,11 @@
 #include "faces.h"
 
 namespace dgmi {
+
+double wall_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
 
 static thread_local std::string g_err;
 static thread_local std::string g_alloc_err;
@@ -38,16 +44,29 @@ std::string alloc_failure(const char* what) {
 
 template <typename T, typename U>
 static T* upload(dgmi_ctx* c, const std::vector<U>& h) {
-  std::vector<T> t(h.begin(), h.end());
-  T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(t.size(), 1)));
+  T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(h.size(), 1)));
   if (!d) return nullptr;
-  if (!t.empty()) {
-    const hipError_t e = hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+  if (!h.empty()) {
+    hipError_t e;
+    if constexpr (std::is_same<T, U>::value) {  // no host copy
+      e = hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+    } else {
+      std::vector<T> t(h.begin(), h.end());
+      e = hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
       g_alloc_err = std::string("hipMemcpy to device: ") + hipGetErrorString(e);
       return nullptr;
     }
   }
+  return d;
+}
+
+// device FP32 copy of a device FP64 array (converted on the device, no host copy)
+static float* device_f32(dgmi_ctx* c, const double* src, size_t n) {
+  float* d = static_cast<float*>(c->alloc(sizeof(float) * std::max<size_t>(n, 1)));
+  if (!d || !src) return nullptr;
+  if (n && launch_convert<float>(src, d, (int64_t)n, nullptr) != 0) return nullptr;
   return d;
 }
 
@@ -80,19 +99,28 @@ static std::vector<double> table_block(const ElementTables& el, size_t& ov, size
   return t;
 }
 
+// FP64 geometry from the host; FP32 (f64 given) converted on the device from the FP64 copy
+// element tables of precision T; the FP64 geometry is already on the device (formed by
+// build_geometry_device), the FP32 copy is converted from it on the device
 template <typename T>
 static int upload_geom(dgmi_ctx* c, DevGeom<T>& g, const std::vector<double>& tab, size_t ov,
-                       size_t ow, size_t of, size_t os, size_t oc, const Geometry& geo) {
+                       size_t ow, size_t of, size_t os, size_t oc) {
   g.tab = upload<T>(c, tab);
   g.off_vtab = ov;
   g.off_wgt = ow;
   g.off_ftab = of;
   g.off_sw = os;
   g.off_fc = oc;
-  g.G = upload<T>(c, geo.G);
-  g.cf = upload<T>(c, geo.cf);
-  g.wf = upload<T>(c, geo.wf);
-  g.pen = upload<T>(c, geo.pen);
+  if constexpr (std::is_same<T, float>::value) {
+    const DevGeom<double>& f = c->g64;
+    const size_t nz = (size_t)c->nele * c->nface;
+    g.G = device_f32(c, f.G, (size_t)c->ng * c->nele);
+    g.cf = device_f32(c, f.cf, nz * c->D);
+    g.wf = device_f32(c, f.wf, nz);
+    g.pen = device_f32(c, f.pen, nz);
+    // the conversions ran on the null stream: done before any caller stream reads them
+    DGMI_HIP(hipStreamSynchronize(nullptr));
+  }
   g.trace = static_cast<typename Vec2<T>::type*>(
       c->alloc(sizeof(typename Vec2<T>::type) * c->nface * c->sngi * c->nele));
   if (std::is_same<T, float>::value) {
@@ -826,11 +854,15 @@ extern "C" {
 
 const char* dgmi_last_error(void) { return g_err.c_str(); }
 
-int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
+// context creation; the P3 node coordinates come in nparts pieces (piece q holds elements
+// [part_off[q], part_off[q+1]))
+static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts,
+                           const int64_t* part_off, int nparts, dgmi_ctx** out) {
   DGMI_CHECK(d && out, "null argument");
   DGMI_CHECK(d->D == 2 || d->D == 3, "D must be 2 or 3");
   DGMI_CHECK(d->nele > 0 && d->nverts > 0, "empty mesh");
-  DGMI_CHECK(d->x_ref_in && d->cells && d->verts && d->kappa, "null mesh array");
+  DGMI_CHECK(x_parts && d->cells && d->verts && d->kappa, "null mesh array");
+  for (int q = 0; q < nparts; ++q) DGMI_CHECK(x_parts[q], "null mesh array");
   // the largest 32-bit kernel index is (nface*sngi)*nele (face traces): 48 nele in 3D
   DGMI_CHECK(d->nele <= 40000000, "nele above the 32-bit index range of the kernels (4e7)");
   auto* c = new dgmi_ctx();
@@ -854,7 +886,10 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
       set_error("cell vertex id out of range");
       return -2;
     }
+  double tp = wall_seconds();
   c->topo = build_topology(c->cells.data(), c->nele, c->D);
+  c->t_setup[PH_TOPOLOGY] += wall_seconds() - tp;
+  tp = wall_seconds();
   if (d->bctype) {
     for (int64_t z = 0; z < c->nele * c->nface; ++z) {
       const int64_t t = d->bctype[z];
@@ -866,20 +901,26 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
       c->topo.bctype[z] = t;
     }
   }
-  Geometry geo;
-  std::string err;
-  DGMI_CHECK(d->face_kappa == 0 || d->face_kappa == 1, "face_kappa must be 0 or 1");
-  if (!build_geometry(c->el, d->x_ref_in, c->kappa.data(), c->topo, c->nele, c->eta_e,
-                      d->face_kappa, geo, err)) {
+  if (d->face_kappa != 0 && d->face_kappa != 1) {
     delete c;
-    set_error(err);
+    set_error("face_kappa must be 0 or 1");
     return -2;
   }
-  if (geo.max_affine_dev > 1e-8) {
-    delete c;
-    set_error("curved (non-affine) P3 elements are not supported: max node deviation " +
-              std::to_string(geo.max_affine_dev));
-    return -2;
+  {
+    double max_dev = 0.0;
+    int64_t bad = -1;
+    const int rg = build_geometry_device(c, x_parts, part_off, nparts, d->face_kappa, &max_dev,
+                                         &bad);
+    std::string msg;
+    if (rg == 0 && bad >= 0) msg = "degenerate element " + std::to_string(bad);
+    else if (rg == 0 && !(max_dev <= 1e-8))
+      msg = "curved (non-affine) P3 elements are not supported: max node deviation " +
+            std::to_string(max_dev);
+    if (rg || !msg.empty()) {
+      dgmi_ctx_destroy(c);
+      if (!msg.empty()) set_error(msg);
+      return rg ? rg : -2;
+    }
   }
   // the face kernels interpolate du/dn quadratically over each face: exact for P3, so the
   // interpolant reproduces the tabulated gradients to rounding
@@ -890,18 +931,17 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
     set_error("face interpolation tables do not reproduce the face gradients");
     return -2;
   }
+  c->t_setup[PH_GEOMETRY] += wall_seconds() - tp;
+  tp = wall_seconds();
   size_t ov, ow, of, os, oc;
   const std::vector<double> tab = table_block(c->el, ov, ow, of, os, oc);
-  int rc = upload_geom<double>(c, c->g64, tab, ov, ow, of, os, oc, geo);
-  if (!rc) rc = upload_geom<float>(c, c->g32, tab, ov, ow, of, os, oc, geo);
+  int rc = upload_geom<double>(c, c->g64, tab, ov, ow, of, os, oc);
+  if (!rc) rc = upload_geom<float>(c, c->g32, tab, ov, ow, of, os, oc);
   if (!rc) {
-    c->nbface = upload<int32_t>(c, geo.nbface);
-    c->fcode = upload<int32_t>(c, geo.fcode);
     std::vector<int32_t> al(c->el.gi_align.begin(), c->el.gi_align.end());
     c->aln = upload<int32_t>(c, al);
     std::vector<int32_t> fn(c->el.fnodes.begin(), c->el.fnodes.end());
     c->fnodes = upload<int32_t>(c, fn);
-    c->detJ = upload<double>(c, geo.detJ);
     if (!c->nbface || !c->fcode || !c->aln || !c->fnodes || !c->detJ) {
       set_error(alloc_failure());
       rc = -3;
@@ -918,12 +958,21 @@ int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
     dgmi_ctx_destroy(c);
     return rc;
   }
+  c->t_setup[PH_UPLOAD] += wall_seconds() - tp;
   *out = c;
   return 0;
 }
 
-int dgmi_ctx_create_batch(const dgmi_mesh_desc* d, int32_t ncomp, const int64_t* eoff,
-                          const int64_t* voff, dgmi_ctx** out) {
+int dgmi_ctx_create(const dgmi_mesh_desc* d, dgmi_ctx** out) {
+  DGMI_CHECK(d, "null argument");
+  const int64_t off[2] = {0, d->nele};
+  const double* parts[1] = {d->x_ref_in};
+  return ctx_create_impl(d, parts, off, 1, out);
+}
+
+static int ctx_create_batch_impl(const dgmi_mesh_desc* d, int32_t ncomp, const int64_t* eoff,
+                                 const int64_t* voff, const double* const* x_parts,
+                                 dgmi_ctx** out) {
   DGMI_CHECK(d && eoff && voff && out, "null argument");
   DGMI_CHECK(ncomp >= 1, "ncomp must be at least 1");
   DGMI_CHECK(eoff[0] == 0 && eoff[ncomp] == d->nele && voff[0] == 0 && voff[ncomp] == d->nverts,
@@ -937,7 +986,11 @@ int dgmi_ctx_create_batch(const dgmi_mesh_desc* d, int32_t ncomp, const int64_t*
         DGMI_CHECK(v >= voff[q] && v < voff[q + 1], "a cell uses a vertex of another component");
       }
   dgmi_ctx* c = nullptr;
-  DGMI_TRY(dgmi_ctx_create(d, &c));
+  if (x_parts) {
+    DGMI_TRY(ctx_create_impl(d, x_parts, eoff, ncomp, &c));
+  } else {
+    DGMI_TRY(dgmi_ctx_create(d, &c));
+  }
   c->ncomp = ncomp;
   c->comp_eoff.assign(eoff, eoff + ncomp + 1);
   c->comp_voff.assign(voff, voff + ncomp + 1);
@@ -948,6 +1001,18 @@ int dgmi_ctx_create_batch(const dgmi_mesh_desc* d, int32_t ncomp, const int64_t*
   }
   *out = c;
   return 0;
+}
+
+int dgmi_ctx_create_batch(const dgmi_mesh_desc* d, int32_t ncomp, const int64_t* eoff,
+                          const int64_t* voff, dgmi_ctx** out) {
+  return ctx_create_batch_impl(d, ncomp, eoff, voff, nullptr, out);
+}
+
+int dgmi_ctx_create_batch_parts(const dgmi_mesh_desc* d, int32_t ncomp, const int64_t* eoff,
+                                const int64_t* voff, const double* const* x_parts,
+                                dgmi_ctx** out) {
+  DGMI_CHECK(x_parts, "null argument");
+  return ctx_create_batch_impl(d, ncomp, eoff, voff, x_parts, out);
 }
 
 int dgmi_ctx_destroy(dgmi_ctx* c) {
@@ -967,6 +1032,12 @@ int dgmi_ctx_destroy(dgmi_ctx* c) {
     delete k;
   }
   delete c;
+  return 0;
+}
+
+int dgmi_setup_timing(const dgmi_ctx* c, double* seconds) {
+  DGMI_CHECK(c && seconds, "null argument");
+  std::copy(c->t_setup, c->t_setup + 8, seconds);
   return 0;
 }
 
@@ -1103,8 +1174,17 @@ int dgmi_bdiag(dgmi_ctx* c, double* bdiag, double* diag, void* stream) {
   return 0;
 }
 
+// adds the wall time of a scope to one set-up phase
+struct PhaseScope {
+  double& acc;
+  double t0;
+  explicit PhaseScope(double& a) : acc(a), t0(wall_seconds()) {}
+  ~PhaseScope() { acc += wall_seconds() - t0; }
+};
+
 int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
   DGMI_CHECK(c, "null context");
+  PhaseScope phase(c->t_setup[PH_SMOOTHER]);
   DGMI_CHECK(variant == DGMI_SMOOTH_JACOBI3 || variant == DGMI_SMOOTH_DIRECT,
              "unknown smoother variant");
   hipStream_t s = (hipStream_t)stream;
@@ -1181,16 +1261,34 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   MGDev& mg = c->mg;
   const int D = c->D;
   const int nc = c->ncomp;
+  double tp = wall_seconds();
+  auto mark = [&](int ph) {
+    const double t = wall_seconds();
+    c->t_setup[ph] += t - tp;
+    tp = t;
+  };
+  struct LayoutMark {  // whatever follows the level build, until the call returns
+    dgmi_ctx* c;
+    double* tp;
+    ~LayoutMark() { c->t_setup[PH_LAYOUT] += wall_seconds() - *tp; }
+  };
   p1cg_sparsity(c->cells.data(), c->nele, D, c->nverts, mg.fina, mg.cola);
   mg.ncola = (int64_t)mg.cola.size();
-  std::vector<int32_t> whichc;
-  mg.ncolor = color2(mg.fina, mg.cola, whichc);
   // components own contiguous vertex ranges and share no P1CG coupling
   for (int q = 0; q < nc; ++q)
     for (int64_t i = c->comp_voff[q]; i < c->comp_voff[q + 1]; ++i)
       for (int64_t k = mg.fina[i]; k < mg.fina[i + 1]; ++k)
         DGMI_CHECK(mg.cola[k] >= c->comp_voff[q] && mg.cola[k] < c->comp_voff[q + 1],
                    "components share P1 vertices");
+  // distance-2 colouring, each component on its own (the colouring of the union in
+  // vertex order is the components' colourings side by side)
+  std::vector<int32_t> whichc(c->nverts, 0);
+  std::vector<int> ncq(nc, 0);
+  parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
+    for (int64_t q = q0; q < q1; ++q)
+      ncq[q] = color2_range(mg.fina, mg.cola, c->comp_voff[q], c->comp_voff[q + 1], whichc);
+  }, 1);
+  mg.ncolor = *std::max_element(ncq.begin(), ncq.end());
   // Hilbert order of each component's own vertices (local 1-based positions)
   std::vector<std::vector<int64_t>> lsfc(nc);
   parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
@@ -1199,6 +1297,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
       hilbert_order(c->verts.data() + v0 * D, nv, D, lsfc[q]);
     }
   }, 1);
+  mark(PH_GRAPH);
   // restriction lists for a layout: SFC position of every vertex (0-based)
   auto build_transfers = [&](const std::vector<int64_t>& pos, int64_t ncg) -> int {
     const int64_t nv = c->nverts;
@@ -1272,6 +1371,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   DGMI_HIP(hipMemcpyAsync(mg.rar.data(), vals_d, sizeof(double) * mg.ncola,
                           hipMemcpyDeviceToHost, s));
   DGMI_HIP(hipStreamSynchronize(s));
+  mark(PH_PROBE);
   // SFC levels of every component (z00-fortran-src/compact_fcode.f90:7-225)
   SfcLayout lay;
   lay.lh.resize(nc);
@@ -1317,6 +1417,8 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     mg.comp_levels[q] = h.n;
   }
   }, 1);
+  mark(PH_LEVELS);
+  LayoutMark layout_mark{c, &tp};
   for (int q = 0; q < nc; ++q) DGMI_CHECK(!too_deep[q], "too many SFC levels");
   // final layout: components by start level, descending; segments padded to a multiple
   // of 2^start so that every component's pairs stay inside it (the last one needs none)

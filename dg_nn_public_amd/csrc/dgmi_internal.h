@@ -242,9 +242,10 @@ struct dgmi_ctx {
   dgmi::DevGeom<float> g32;
   int smoother_variant = -1;
   int gmres_orth = 1;  // 0: modified Gram-Schmidt, 1: CGS2 (fused passes)
-  // kernel family of every FP32-vector apply/smooth: 0 vector FMA, 1 MFMA fp32,
-  // 2 MFMA with bf16 inputs
-  int f32_kernel = 1;
+  // kernel family of every FP32-vector apply/smooth: 0 vector FMA (default: faster on
+  // every mesh measured, and its sweeps carry the face traces), 1 MFMA fp32, 2 MFMA with
+  // bf16 inputs
+  int f32_kernel = 0;
   // FP32 smoothing sweeps with the exact block smoother in the neighbour-only form
   // x = (1 - omega) x + S (b - A_off x) (no volume term, half the face terms)
   int smooth_offdiag = 1;
@@ -276,10 +277,24 @@ struct dgmi_ctx {
   size_t dev_bytes = 0;
   // allocations that would take dev_bytes above this many bytes fail (0: no limit)
   size_t alloc_limit = 0;
+  // host wall-clock seconds of the set-up phases (dgmi_setup_timing)
+  double t_setup[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   void* alloc(size_t bytes);
 };
 
 namespace dgmi {
+// set-up phases timed into dgmi_ctx::t_setup
+enum SetupPhase {
+  PH_TOPOLOGY = 0, PH_GEOMETRY, PH_UPLOAD, PH_SMOOTHER, PH_GRAPH, PH_PROBE, PH_LEVELS, PH_LAYOUT
+};
+double wall_seconds();
+// FP64 element and face geometry (c->g64.G, cf, wf, pen; c->nbface, fcode, detJ) from the
+// host P3 node coordinates, given in nparts pieces (piece q holds elements
+// [part_off[q], part_off[q+1])), formed on the device; *max_dev = the largest node
+// deviation from the affine position relative to the element size, *bad_elem = the first
+// degenerate element or -1 (geometry.hip)
+int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64_t* part_off,
+                          int nparts, int face_kappa, double* max_dev, int64_t* bad_elem);
 // kernel launchers (apply.hip)
 template <typename T>
 int launch_apply(dgmi_ctx* c, DevGeom<T>& g, const T* x, const T* b, T* r, hipStream_t s);

@@ -630,11 +630,16 @@ template <typename T> static T* vec(void* p) { return reinterpret_cast<T*>(p); }
 // device copy of a host vector (converted to T); nullptr on failure
 template <typename T, typename U>
 static T* upload_vec(dgmi_ctx* c, const std::vector<U>& h) {
-  std::vector<T> t(h.begin(), h.end());
-  T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(t.size(), 1)));
+  T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(h.size(), 1)));
   if (!d) return nullptr;
-  if (!t.empty()) {
-    const hipError_t e = hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+  if (!h.empty()) {
+    hipError_t e;
+    if constexpr (std::is_same<T, U>::value) {
+      e = hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+    } else {
+      const std::vector<T> t(h.begin(), h.end());
+      e = hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
       note_alloc_error(std::string("hipMemcpy to device: ") + hipGetErrorString(e));
       return nullptr;
@@ -890,18 +895,48 @@ int launch_sfc_vcycle(dgmi_ctx* c, const T* r_sfc, T* e_sfc, hipStream_t s, bool
   return 0;
 }
 
-// In-place blocked Gauss-Jordan inversion of a dense N x N matrix (row-major, FP64, N a
+// In-place blocked Gauss-Jordan inversion of dense N x N matrices (row-major, FP64, N a
 // multiple of GJB) on the device, without pivoting: the SFC level operators are Galerkin
 // projections of a symmetric positive definite operator, so every pivot block is positive
-// definite (a non-positive or non-finite pivot sets *fail).  Block column K = [k0, k0+GJB),
-// R = the rest:  P = A_KK^-1;  A_KR <- P A_KR;  A_RR <- A_RR - A_RK A_KR;  A_RK <- -A_RK P;
-// A_KK <- P  (the block form of the scalar steps a_kk <- 1/a_kk, row k /= a_kk,
-// a_ij -= a_ik a_kj, column k *= -1/a_kk).
+// definite (a non-positive or non-finite pivot sets the matrix's fail flag).  Block column
+// K = [k0, k0+GJB), R = the rest:  P = A_KK^-1;  A_KR <- P A_KR;  A_RR <- A_RR - A_RK A_KR;
+// A_RK <- -A_RK P;  A_KK <- P  (the block form of the scalar steps a_kk <- 1/a_kk, row k
+// /= a_kk, a_ij -= a_ik a_kj, column k *= -1/a_kk).  Several matrices of different sizes
+// are inverted together: blockIdx.z selects the matrix, and a launch for block column k0
+// skips the matrices already done.
 constexpr int GJB = 64;
 
+struct GjMat {
+  double* a;        // N x N work matrix
+  double* p;        // GJB x GJB pivot block
+  float* out;       // n x n FP32 row-major inverse
+  const int32_t* rp;  // CSR of the level (n rows)
+  const int32_t* col;
+  const double* val;
+  int n, N, fail;   // fail: index into the fail flags
+};
+
+// Entries of one CSR level stored into the zeroed dense matrix, and the identity on the
+// padding rows i >= n: one thread per row of the padded matrix.
+__global__ __launch_bounds__(256) void k_csr_to_dense(const GjMat* __restrict__ mats) {
+  const GjMat m = mats[blockIdx.z];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.N) return;
+  double* row = m.a + (size_t)i * m.N;
+  if (i >= m.n) {
+    row[i] = 1.0;
+    return;
+  }
+  for (int32_t q = m.rp[i]; q < m.rp[i + 1]; ++q) row[m.col[q]] += m.val[q];
+}
+
 // P = A_KK^-1 by unblocked Gauss-Jordan in LDS; written to A_KK and to p (GJB x GJB)
-__global__ __launch_bounds__(256) void k_bgj_pivot(int N, double* __restrict__ A, int k0,
-                                                   double* __restrict__ p, int* __restrict__ fail) {
+__global__ __launch_bounds__(256) void k_bgj_pivot(const GjMat* __restrict__ mats, int k0,
+                                                   int* __restrict__ fail) {
+  const GjMat m = mats[blockIdx.z];
+  if (k0 >= m.N) return;
+  const int N = m.N;
+  double* A = m.a;
   __shared__ double a[GJB][GJB + 1];
   __shared__ double fcol[GJB], frow[GJB];
   const int t = threadIdx.x;
@@ -913,7 +948,7 @@ __global__ __launch_bounds__(256) void k_bgj_pivot(int N, double* __restrict__ A
     else if (t < 2 * GJB) frow[t - GJB] = a[k][t - GJB];
     __syncthreads();
     const double piv = fcol[k];
-    if (t == 0 && (!(piv > 0.0) || !isfinite(piv))) *fail = 1;
+    if (t == 0 && (!(piv > 0.0) || !isfinite(piv))) fail[m.fail] = 1;
     const double inv = 1.0 / piv;
     for (int q = t; q < GJB * GJB; q += 256) {
       const int i = q / GJB, j = q % GJB;
@@ -927,19 +962,21 @@ __global__ __launch_bounds__(256) void k_bgj_pivot(int N, double* __restrict__ A
   for (int q = t; q < GJB * GJB; q += 256) {
     const double v = a[q / GJB][q % GJB];
     A[(size_t)(k0 + q / GJB) * N + k0 + q % GJB] = v;
-    p[q] = v;
+    m.p[q] = v;
   }
 }
 
 // A_KJ <- P A_KJ for the column tile J (one work-group per tile, the pivot tile skipped)
-__global__ __launch_bounds__(256) void k_bgj_row(int N, double* __restrict__ A, int k0,
-                                                 const double* __restrict__ p) {
+__global__ __launch_bounds__(256) void k_bgj_row(const GjMat* __restrict__ mats, int k0) {
+  const GjMat m = mats[blockIdx.z];
   const int j0 = blockIdx.x * GJB;
-  if (j0 == k0) return;
+  if (k0 >= m.N || j0 >= m.N || j0 == k0) return;
+  const int N = m.N;
+  double* A = m.a;
   __shared__ double ps[GJB][GJB + 1], ys[GJB][GJB + 1];
   const int t = threadIdx.x;
   for (int q = t; q < GJB * GJB; q += 256) {
-    ps[q / GJB][q % GJB] = p[q];
+    ps[q / GJB][q % GJB] = m.p[q];
     ys[q / GJB][q % GJB] = A[(size_t)(k0 + q / GJB) * N + j0 + q % GJB];
   }
   __syncthreads();
@@ -963,9 +1000,12 @@ __global__ __launch_bounds__(256) void k_bgj_row(int N, double* __restrict__ A, 
 }
 
 // A_IJ -= A_IK A_KJ for tiles I, J outside the pivot block (A_KJ already updated)
-__global__ __launch_bounds__(256) void k_bgj_rest(int N, double* __restrict__ A, int k0) {
+__global__ __launch_bounds__(256) void k_bgj_rest(const GjMat* __restrict__ mats, int k0) {
+  const GjMat m = mats[blockIdx.z];
   const int i0 = blockIdx.y * GJB, j0 = blockIdx.x * GJB;
-  if (i0 == k0 || j0 == k0) return;
+  if (k0 >= m.N || i0 >= m.N || j0 >= m.N || i0 == k0 || j0 == k0) return;
+  const int N = m.N;
+  double* A = m.a;
   __shared__ double ls[GJB][GJB + 1], rs[GJB][GJB + 1];
   const int t = threadIdx.x;
   for (int q = t; q < GJB * GJB; q += 256) {
@@ -993,14 +1033,16 @@ __global__ __launch_bounds__(256) void k_bgj_rest(int N, double* __restrict__ A,
 }
 
 // A_IK <- -A_IK P for the row tile I (the pivot tile skipped)
-__global__ __launch_bounds__(256) void k_bgj_col(int N, double* __restrict__ A, int k0,
-                                                 const double* __restrict__ p) {
+__global__ __launch_bounds__(256) void k_bgj_col(const GjMat* __restrict__ mats, int k0) {
+  const GjMat m = mats[blockIdx.z];
   const int i0 = blockIdx.x * GJB;
-  if (i0 == k0) return;
+  if (k0 >= m.N || i0 >= m.N || i0 == k0) return;
+  const int N = m.N;
+  double* A = m.a;
   __shared__ double ls[GJB][GJB + 1], ps[GJB][GJB + 1];
   const int t = threadIdx.x;
   for (int q = t; q < GJB * GJB; q += 256) {
-    ps[q / GJB][q % GJB] = p[q];
+    ps[q / GJB][q % GJB] = m.p[q];
     ls[q / GJB][q % GJB] = A[(size_t)(i0 + q / GJB) * N + k0 + q % GJB];
   }
   __syncthreads();
@@ -1024,90 +1066,121 @@ __global__ __launch_bounds__(256) void k_bgj_col(int N, double* __restrict__ A, 
 }
 
 // out (n x n, FP32 row-major) = the leading n x n block of a (N x N, FP64)
-__global__ void k_d2f_block(int n, int N, const double* __restrict__ a, float* __restrict__ out) {
+__global__ void k_d2f_block(const GjMat* __restrict__ mats) {
+  const GjMat m = mats[blockIdx.z];
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (int64_t)n * n) return;
-  const int i = (int)(q / n), j = (int)(q - (int64_t)i * n);
-  out[q] = (float)a[(size_t)i * N + j];
+  if (q >= (int64_t)m.n * m.n) return;
+  const int i = (int)(q / m.n), j = (int)(q - (int64_t)i * m.n);
+  m.out[q] = (float)m.a[(size_t)i * m.N + j];
+}
+
+// One level inverse to form: level l of hierarchy h, FP32 result written to out.
+struct LevelInverse {
+  const SfcLevelsHost* h;
+  int l;
+  float* out;
+};
+
+// FP64 blocked Gauss-Jordan inverses of the given levels, formed together on stream s and
+// written as FP32 row-major; returns 0 and fails[i] = 1 for a level that is not positive
+// definite, or an error code (the stream is synchronised on return).
+static int invert_levels(const std::vector<LevelInverse>& jobs, std::vector<int>& fails,
+                         hipStream_t s) {
+  const int nm = (int)jobs.size();
+  fails.assign(nm, 0);
+  if (nm == 0) return 0;
+  std::vector<GjMat> mats(nm);
+  size_t asz = 0, nnz = 0, nrp = 0;
+  int nbmax = 0, nmax = 0;
+  for (int i = 0; i < nm; ++i) {
+    const int n = (int)jobs[i].h->n[jobs[i].l];
+    const int N = (n + GJB - 1) / GJB * GJB;
+    mats[i].n = n;
+    mats[i].N = N;
+    mats[i].fail = i;
+    asz += (size_t)N * N;
+    nnz += jobs[i].h->col[jobs[i].l].size();
+    nrp += (size_t)n + 1;
+    nbmax = std::max(nbmax, N / GJB);
+    nmax = std::max(nmax, N);
+  }
+  double *da = nullptr, *dp = nullptr, *dval = nullptr;
+  int32_t *drp = nullptr, *dcol = nullptr;
+  int* dfail = nullptr;
+  GjMat* dm = nullptr;
+  int rc = 0;
+  auto chk = [&](hipError_t e) {
+    if (e != hipSuccess && !rc) {
+      set_error(std::string("coarse inverse: ") + hipGetErrorString(e));
+      rc = -3;
+    }
+    return rc == 0;
+  };
+  if (chk(hipMalloc(&da, sizeof(double) * asz)) &&
+      chk(hipMalloc(&dp, sizeof(double) * GJB * GJB * nm)) &&
+      chk(hipMalloc(&dval, sizeof(double) * std::max<size_t>(nnz, 1))) &&
+      chk(hipMalloc(&dcol, sizeof(int32_t) * std::max<size_t>(nnz, 1))) &&
+      chk(hipMalloc(&drp, sizeof(int32_t) * nrp)) && chk(hipMalloc(&dfail, sizeof(int) * nm)) &&
+      chk(hipMalloc(&dm, sizeof(GjMat) * nm))) {
+    size_t ao = 0, vo = 0, ro = 0;
+    for (int i = 0; i < nm && rc == 0; ++i) {
+      const SfcLevelsHost& h = *jobs[i].h;
+      const int l = jobs[i].l;
+      const size_t k = h.col[l].size();
+      mats[i].a = da + ao;
+      mats[i].p = dp + (size_t)GJB * GJB * i;
+      mats[i].out = jobs[i].out;
+      mats[i].rp = drp + ro;
+      mats[i].col = dcol + vo;
+      mats[i].val = dval + vo;
+      chk(hipMemcpyAsync(drp + ro, h.rowptr[l].data(), sizeof(int32_t) * (mats[i].n + 1),
+                         hipMemcpyHostToDevice, s));
+      if (k) {
+        chk(hipMemcpyAsync(dcol + vo, h.col[l].data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, s));
+        chk(hipMemcpyAsync(dval + vo, h.val[l].data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+      }
+      ao += (size_t)mats[i].N * mats[i].N;
+      vo += k;
+      ro += (size_t)mats[i].n + 1;
+    }
+    if (rc == 0 && chk(hipMemcpyAsync(dm, mats.data(), sizeof(GjMat) * nm, hipMemcpyHostToDevice, s)) &&
+        chk(hipMemsetAsync(dfail, 0, sizeof(int) * nm, s)) &&
+        chk(hipMemsetAsync(da, 0, sizeof(double) * asz, s))) {
+      const dim3 blk(256);
+      hipLaunchKernelGGL(k_csr_to_dense, dim3((nmax + 255) / 256, 1, nm), blk, 0, s,
+                         (const GjMat*)dm);
+      for (int kb = 0; kb < nbmax; ++kb) {
+        const int k0 = kb * GJB;
+        hipLaunchKernelGGL(k_bgj_pivot, dim3(1, 1, nm), blk, 0, s, (const GjMat*)dm, k0, dfail);
+        hipLaunchKernelGGL(k_bgj_row, dim3(nbmax, 1, nm), blk, 0, s, (const GjMat*)dm, k0);
+        hipLaunchKernelGGL(k_bgj_rest, dim3(nbmax, nbmax, nm), blk, 0, s, (const GjMat*)dm, k0);
+        hipLaunchKernelGGL(k_bgj_col, dim3(nbmax, 1, nm), blk, 0, s, (const GjMat*)dm, k0);
+      }
+      const int64_t nn = (int64_t)nmax * nmax;
+      hipLaunchKernelGGL(k_d2f_block, dim3((unsigned)((nn + 255) / 256), 1, nm), blk, 0, s,
+                         (const GjMat*)dm);
+      chk(hipGetLastError());
+      if (rc == 0 && chk(hipMemcpyAsync(fails.data(), dfail, sizeof(int) * nm,
+                                        hipMemcpyDeviceToHost, s)))
+        chk(hipStreamSynchronize(s));
+    }
+  }
+  (void)hipStreamSynchronize(s);  // nothing in flight reads the work space any more
+  for (void* q : {(void*)da, (void*)dp, (void*)dval, (void*)dcol, (void*)drp, (void*)dfail,
+                  (void*)dm})
+    if (q) (void)hipFree(q);
+  return rc;
 }
 
 // FP32 row-major inverse of the dense level-l operator of one component, formed on the
-// device on stream s (synchronous with s only); nullptr on failure.  The matrix is padded
-// to a multiple of GJB with the identity, which leaves the leading block's inverse
-// unchanged.
+// device on stream s (synchronous with s only); nullptr on failure.
 static float* dense_inverse_device(dgmi_ctx* c, const SfcLevelsHost& h, int l, hipStream_t s) {
   const int n = (int)h.n[l];
-  const int N = (n + GJB - 1) / GJB * GJB, nb = N / GJB;
-  std::vector<double> a((size_t)N * N, 0.0);
-  for (int i = 0; i < n; ++i)
-    for (int32_t q = h.rowptr[l][i]; q < h.rowptr[l][i + 1]; ++q)
-      a[(size_t)i * N + h.col[l][q]] += h.val[l][q];
-  for (int i = n; i < N; ++i) a[(size_t)i * N + i] = 1.0;
-  double *da = nullptr, *dp = nullptr;
-  int* dfail = nullptr;
   float* out = static_cast<float*>(c->alloc(sizeof(float) * (size_t)n * n));
-  bool ok = out && hipMalloc(&da, sizeof(double) * (size_t)N * N) == hipSuccess &&
-            hipMalloc(&dp, sizeof(double) * GJB * GJB) == hipSuccess &&
-            hipMalloc(&dfail, sizeof(int)) == hipSuccess;
-  int hfail = 0;
-  if (ok) {
-    ok = hipMemcpyAsync(da, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, s) ==
-             hipSuccess &&
-         hipMemcpyAsync(dfail, &hfail, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess;
-  }
-  if (ok) {
-    const dim3 blk(256);
-    for (int kb = 0; kb < nb; ++kb) {
-      const int k0 = kb * GJB;
-      hipLaunchKernelGGL(k_bgj_pivot, dim3(1), blk, 0, s, N, da, k0, dp, dfail);
-      hipLaunchKernelGGL(k_bgj_row, dim3(nb), blk, 0, s, N, da, k0, (const double*)dp);
-      hipLaunchKernelGGL(k_bgj_rest, dim3(nb, nb), blk, 0, s, N, da, k0);
-      hipLaunchKernelGGL(k_bgj_col, dim3(nb), blk, 0, s, N, da, k0, (const double*)dp);
-    }
-    const int64_t nn = (int64_t)n * n;
-    hipLaunchKernelGGL(k_d2f_block, dim3((unsigned)((nn + 255) / 256)), blk, 0, s, n, N,
-                       (const double*)da, out);
-    ok = hipGetLastError() == hipSuccess &&
-         hipMemcpyAsync(&hfail, dfail, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
-         hipStreamSynchronize(s) == hipSuccess && !hfail;
-  }
-  if (da) (void)hipFree(da);
-  if (dp) (void)hipFree(dp);
-  if (dfail) (void)hipFree(dfail);
-  return ok ? out : nullptr;
-}
-
-// Inverse of the dense level-l operator of one component (Gauss-Jordan with partial
-// pivoting in FP64), row-major in FP32.  False if the level is singular.
-static bool dense_inverse(const SfcLevelsHost& h, int l, std::vector<float>& out) {
-  const int n = (int)h.n[l];
-  std::vector<double> a((size_t)n * 2 * n, 0.0);  // [A | I], row-major
-  for (int i = 0; i < n; ++i) {
-    for (int32_t k = h.rowptr[l][i]; k < h.rowptr[l][i + 1]; ++k)
-      a[(size_t)i * 2 * n + h.col[l][k]] += h.val[l][k];
-    a[(size_t)i * 2 * n + n + i] = 1.0;
-  }
-  for (int col = 0; col < n; ++col) {
-    int p = col;
-    for (int r = col + 1; r < n; ++r)
-      if (std::fabs(a[(size_t)r * 2 * n + col]) > std::fabs(a[(size_t)p * 2 * n + col])) p = r;
-    const double pv = a[(size_t)p * 2 * n + col];
-    if (!(std::fabs(pv) > 0.0) || !std::isfinite(pv)) return false;
-    if (p != col)
-      for (int k = 0; k < 2 * n; ++k) std::swap(a[(size_t)p * 2 * n + k], a[(size_t)col * 2 * n + k]);
-    const double d = 1.0 / pv;
-    for (int k = 0; k < 2 * n; ++k) a[(size_t)col * 2 * n + k] *= d;
-    for (int r = 0; r < n; ++r) {
-      if (r == col) continue;
-      const double f = a[(size_t)r * 2 * n + col];
-      if (f == 0.0) continue;
-      for (int k = 0; k < 2 * n; ++k) a[(size_t)r * 2 * n + k] -= f * a[(size_t)col * 2 * n + k];
-    }
-  }
-  out.assign((size_t)n * n, 0.0f);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) out[(size_t)i * n + j] = (float)a[(size_t)i * 2 * n + n + j];
-  return true;
+  if (!out) return nullptr;
+  std::vector<int> fails;
+  if (invert_levels({LevelInverse{&h, l, out}}, fails, s) != 0 || fails[0]) return nullptr;
+  return out;
 }
 
 // Device layout of every component's SFC hierarchy (see SfcLevelsDev) and the level
@@ -1139,45 +1212,70 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
   std::vector<std::vector<int64_t>> segoff(S + 1, std::vector<int64_t>(nc, -1));
   std::vector<std::vector<int64_t>> rpoff(S + 1, std::vector<int64_t>(nc, 0));
   for (int l = 0; l <= S; ++l) {
-    std::vector<int32_t> rp(1, 0), col;
-    std::vector<double> val, diag;
-    int64_t rows = 0;
+    // components swept at level l, in layout order: their rows and entries side by side,
+    // each segment padded to its layout length with empty rows (diagonal 1)
+    std::vector<int> comps;
+    std::vector<int64_t> rowbase(1, 0), nzbase(1, 0);
     for (int q : lay.order) {
       if (lay.start[q] < l) continue;
       const SfcLevelsHost& h = lay.lh[q];
       const int64_t seg = (q == lay.order.back()) ? h.n[l] : (lay.len0[q] >> l);
+      const int64_t rows = rowbase.back();
       segoff[l][q] = rows;
-      rpoff[l][q] = (int64_t)rp.size() - 1;
+      rpoff[l][q] = rows;
       if (l < S && lay.start[q] > l) lv.grid_rows[l] = rows + seg;
       if (l < S && lay.start32[q] > l) lv.grid_rows32[l] = rows + seg;
-      for (int64_t i = 0; i < seg; ++i) {
-        if (i < h.n[l]) {
-          for (int32_t k = h.rowptr[l][i]; k < h.rowptr[l][i + 1]; ++k) {
-            col.push_back((int32_t)(h.col[l][k] + rows));
-            val.push_back(h.val[l][k]);
-          }
-          diag.push_back(h.diag[l][i]);
-        } else {
-          diag.push_back(1.0);  // padding row: no entries, stays zero
-        }
-        rp.push_back((int32_t)col.size());
-      }
-      rows += seg;
+      comps.push_back(q);
+      rowbase.push_back(rows + seg);
+      nzbase.push_back(nzbase.back() + (int64_t)h.col[l].size());
     }
+    const int64_t R = rowbase.back(), NZ = nzbase.back();
+    std::vector<int32_t> rp(R + 1), col(NZ);
+    std::vector<double> val(NZ), diag(R);
+    rp[R] = (int32_t)NZ;
+    parallel_chunks(R, [&](int, int64_t a, int64_t b) {
+      int k = (int)(std::upper_bound(rowbase.begin(), rowbase.end(), a) - rowbase.begin()) - 1;
+      for (int64_t r = a; r < b; ++r) {
+        while (r >= rowbase[k + 1]) ++k;
+        const SfcLevelsHost& h = lay.lh[comps[k]];
+        const int64_t i = r - rowbase[k];
+        const int64_t n = h.n[l];
+        if (i < n) {
+          const int32_t q0 = h.rowptr[l][i], q1 = h.rowptr[l][i + 1];
+          rp[r] = (int32_t)(nzbase[k] + q0);
+          for (int32_t q = q0; q < q1; ++q) {
+            col[nzbase[k] + q] = (int32_t)(h.col[l][q] + rowbase[k]);
+            val[nzbase[k] + q] = h.val[l][q];
+          }
+          diag[r] = h.diag[l][i];
+        } else {
+          rp[r] = (int32_t)(nzbase[k] + h.rowptr[l][n]);
+          diag[r] = 1.0;  // padding row: no entries, stays zero
+        }
+      }
+    });
+    const int64_t rows = R;
     lv.n[l] = rows;
-    lv.nnz[l] = (int64_t)col.size();
+    lv.nnz[l] = NZ;
     lv.rowptr[l] = upload_vec<int32_t>(c, rp);
     lv.col[l] = upload_vec<int32_t>(c, col);
     lv.val64[l] = upload_vec<double>(c, val);
     lv.diag64[l] = upload_vec<double>(c, diag);
-    lv.val32[l] = upload_vec<float>(c, val);
-    lv.diag32[l] = upload_vec<float>(c, diag);
+    lv.val32[l] = static_cast<float*>(c->alloc(sizeof(float) * std::max<int64_t>(NZ, 1)));
+    lv.diag32[l] = static_cast<float*>(c->alloc(sizeof(float) * std::max<int64_t>(rows, 1)));
     lv.r[l] = c->alloc(sizeof(double) * std::max<int64_t>(rows, 1));
     lv.e[l] = c->alloc(sizeof(double) * std::max<int64_t>(rows, 1));
     lv.e2[l] = c->alloc(sizeof(double) * std::max<int64_t>(rows, 1));
     if (!lv.rowptr[l] || !lv.col[l] || !lv.val64[l] || !lv.diag64[l] || !lv.val32[l] ||
         !lv.diag32[l] || !lv.r[l] || !lv.e[l] || !lv.e2[l])
       return fail();
+    // FP32 copies converted on the device
+    if (NZ) {
+      if (int rc = launch_convert<float>(lv.val64[l], lv.val32[l], NZ, s)) return rc;
+    }
+    if (rows) {
+      if (int rc = launch_convert<float>(lv.diag64[l], lv.diag32[l], rows, s)) return rc;
+    }
   }
   // ---- pooled deeper levels
   std::vector<int32_t> prp, pcol;
@@ -1280,31 +1378,40 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
            lv.diag32[std::min(l, S)], lv.pool_val32, lv.pool_diag32);
   }
   // FP32 exact coarse solve: the first level at or below a component's FP32 start with at
-  // most EXACT_MAX_N rows (and a coarser level below it) is inverted on the host
-  std::vector<float> cinv;
+  // most EXACT_MAX_N rows (and a coarser level below it) is inverted on the device, every
+  // component's in turn on the stream
   std::vector<int64_t> cinfo(2 * nc, -1);
-  std::vector<std::vector<float>> invs(nc);
   std::vector<int> lvl(nc, -1);
-  parallel_chunks(nc, [&](int, int64_t ba, int64_t bb) {
-    for (int64_t b = ba; b < bb; ++b) {
-      const int q = lay.order[b];
-      const SfcLevelsHost& h = lay.lh[q];
-      for (int l = lay.start32[q]; l < h.nlevel - 1; ++l)
-        if (h.n[l] <= EXACT_MAX_N) {
-          if (dense_inverse(h, l, invs[b])) lvl[b] = l;
-          break;
-        }
-    }
-  }, 1);
+  int64_t ctotal = 0;
   for (int b = 0; b < nc; ++b) {
-    cinfo[2 * b] = (int64_t)cinv.size();
-    if (lvl[b] < 0) continue;
-    cinfo[2 * b + 1] = lvl[b];
-    cinv.insert(cinv.end(), invs[b].begin(), invs[b].end());
+    const int q = lay.order[b];
+    const SfcLevelsHost& h = lay.lh[q];
+    cinfo[2 * b] = ctotal;
+    for (int l = lay.start32[q]; l < h.nlevel - 1; ++l)
+      if (h.n[l] <= EXACT_MAX_N) {
+        lvl[b] = l;
+        ctotal += h.n[l] * h.n[l];
+        break;
+      }
   }
-  lv.cinv32 = upload_vec<float>(c, cinv);
+  lv.cinv32 = static_cast<float*>(c->alloc(sizeof(float) * std::max<int64_t>(ctotal, 1)));
+  if (!lv.cinv32) return fail();
+  {
+    std::vector<LevelInverse> jobs;
+    std::vector<int> which;
+    for (int b = 0; b < nc; ++b)
+      if (lvl[b] >= 0) {
+        jobs.push_back({&lay.lh[lay.order[b]], lvl[b], lv.cinv32 + cinfo[2 * b]});
+        which.push_back(b);
+      }
+    std::vector<int> fails;
+    if (int rc = invert_levels(jobs, fails, s)) return rc;
+    for (size_t i = 0; i < which.size(); ++i)
+      if (fails[i]) lvl[which[i]] = -1;  // not positive definite: this component sweeps every level
+  }
+  for (int b = 0; b < nc; ++b) cinfo[2 * b + 1] = lvl[b];
   lv.cinfo32 = upload_vec<int64_t>(c, cinfo);
-  if (!lv.cinv32 || !lv.cinfo32) return fail();
+  if (!lv.cinfo32) return fail();
   mg.exact32.assign(nc, -1);
   for (int b = 0; b < nc; ++b) mg.exact32[lay.order[b]] = (int)cinfo[2 * b + 1];
   // one solve: the exact level is the first of at most DENSE_MAX_N rows (inverted on the
@@ -1317,8 +1424,8 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
     const SfcLevelsHost& h = lay.lh[lay.order[0]];
     for (int l = 0; l <= std::min(lay.start32[lay.order[0]], h.nlevel - 2); ++l)
       if (h.n[l] <= DENSE_MAX_N) {
-        // the level the packed path inverts on the host: reuse that inverse, so that a
-        // solve alone and packed with others gives the same bits
+        // the level the packed path inverts: reuse that inverse, so that a solve alone
+        // and packed with others gives the same bits
         if (float* inv = l == lvl[0] ? lv.cinv32 : dense_inverse_device(c, h, l, s)) {
           lv.dense_rm32 = inv;
           lv.dense_n32 = (int)h.n[l];

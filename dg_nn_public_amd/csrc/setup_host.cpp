@@ -302,6 +302,8 @@ ElementTables make_element(int D) {
   el.p1_prolong.assign(nloc * nb, 0.0);
   for (int n = 0; n < nloc; ++n)
     for (int k = 0; k < nb; ++k) el.p1_prolong[n * nb + k] = el.index[n * nb + k] / double(P);
+  if (D == 3) el.drst.assign(&TET_DRST[0][0][0], &TET_DRST[0][0][0] + 24);
+  else el.drst.assign(&TRI_DRST[0][0], &TRI_DRST[0][0] + 6);
   return el;
 }
 
@@ -318,9 +320,10 @@ static void vertex_elements(const int64_t* cells, int64_t nele, int nb, int64_t 
 }
 
 Topology build_topology(const int64_t* cells, int64_t nele, int D) {
-  // The neighbour across face (e, f) is the other element holding the face's vertices:
-  // scan the elements of the face's smallest vertex.  The alignment of a matched pair is
-  // read from the pair (lower global face id first) and stored on both sides.
+  // Two faces match when their vertex sets agree.  Faces are bucketed by their smallest
+  // vertex (a counting sort), each bucket is sorted by the remaining vertices, and equal
+  // neighbours in a bucket are paired.  The alignment of a matched pair is read from the
+  // pair (lower global face id first) and stored on both sides.
   const int nf = D + 1, nb = D + 1;
   const int64_t nfaces = nele * nf;
   Topology t;
@@ -329,193 +332,63 @@ Topology build_topology(const int64_t* cells, int64_t nele, int D) {
   t.bctype.assign(nfaces, 0);
   int64_t nverts = 0;
   for (int64_t q = 0; q < nele * nb; ++q) nverts = std::max(nverts, cells[q] + 1);
-  std::vector<int64_t> vptr, velem;
-  vertex_elements(cells, nele, nb, nverts, vptr, velem);
-  auto face_verts = [&](int64_t e, int f, int64_t* v) {
+  auto face_verts = [&](int64_t z, int64_t* v) {
+    const int64_t e = z / nf;
+    const int f = (int)(z - e * nf);
     for (int i = 0; i < D; ++i)
       v[i] = cells[e * nb + (D == 3 ? TET_FACE_VERTS[f][i] : TRI_FACE_VERTS[f][i])];
   };
-  parallel_chunks(nele, [&](int, int64_t e0, int64_t e1) {
-    for (int64_t e = e0; e < e1; ++e)
-      for (int f = 0; f < nf; ++f) {
-        int64_t v[3] = {0, 0, 0}, s3[3] = {0, 0, 0};
-        face_verts(e, f, v);
-        std::copy(v, v + D, s3);
-        std::sort(s3, s3 + D);
-        for (int64_t q = vptr[s3[0]]; q < vptr[s3[0] + 1]; ++q) {
-          const int64_t e2 = velem[q];
-          if (e2 == e) continue;
-          int hit = 0;
-          for (int k = 0; k < nb; ++k)
-            for (int i = 1; i < D; ++i) hit += cells[e2 * nb + k] == s3[i];
-          if (hit != D - 1) continue;
-          for (int f2 = 0; f2 < nf; ++f2) {
-            int64_t w[3] = {0, 0, 0}, ws[3] = {0, 0, 0};
-            face_verts(e2, f2, w);
-            std::copy(w, w + D, ws);
-            std::sort(ws, ws + D);
-            if (!std::equal(ws, ws + D, s3)) continue;
-            const int64_t z = e * nf + f, z2 = e2 * nf + f2;
-            const int64_t* va = z < z2 ? v : w;
-            const int64_t* vb = z < z2 ? w : v;
-            t.nbf[z] = z2;
-            t.alnmt[z] = va[0] == vb[0] ? 0 : (va[1] == vb[0] ? 1 : 2);
-            break;
-          }
-          break;
-        }
+  // faces bucketed by their smallest vertex, each record holding the other two sorted
+  // vertices (D == 2: the second is 0) and the face id
+  struct Rec {
+    int64_t k1, k2, z;
+  };
+  std::vector<int64_t> bptr(nverts + 1, 0);
+  std::vector<int32_t> kmin(nfaces);
+  parallel_chunks(nfaces, [&](int, int64_t a, int64_t b) {
+    for (int64_t z = a; z < b; ++z) {
+      int64_t v[3] = {0, 0, 0};
+      face_verts(z, v);
+      kmin[z] = (int32_t)*std::min_element(v, v + D);
+    }
+  });
+  for (int64_t z = 0; z < nfaces; ++z) bptr[kmin[z] + 1]++;
+  for (int64_t v = 0; v < nverts; ++v) bptr[v + 1] += bptr[v];
+  std::vector<Rec> rec(nfaces);
+  {
+    std::vector<int64_t> fill(bptr.begin(), bptr.end() - 1);
+    for (int64_t z = 0; z < nfaces; ++z) rec[fill[kmin[z]]++].z = z;
+  }
+  parallel_chunks(nverts, [&](int, int64_t va, int64_t vb) {
+    for (int64_t v = va; v < vb; ++v) {
+      Rec* lo = rec.data() + bptr[v];
+      Rec* hi = rec.data() + bptr[v + 1];
+      for (Rec* r = lo; r < hi; ++r) {
+        int64_t w[3] = {0, 0, 0};
+        face_verts(r->z, w);
+        std::sort(w, w + D);
+        r->k1 = w[1];
+        r->k2 = D == 3 ? w[2] : 0;
       }
+      std::sort(lo, hi, [](const Rec& x, const Rec& y) {
+        return x.k1 != y.k1 ? x.k1 < y.k1 : (x.k2 != y.k2 ? x.k2 < y.k2 : x.z < y.z);
+      });
+      for (Rec* p = lo; p + 1 < hi; ++p) {
+        if (p[0].k1 != p[1].k1 || p[0].k2 != p[1].k2) continue;
+        const int64_t z = p[0].z, z2 = p[1].z;
+        int64_t va3[3] = {0, 0, 0}, vb3[3] = {0, 0, 0};
+        face_verts(z, va3);  // z < z2: the lower global face id first
+        face_verts(z2, vb3);
+        const int64_t al = va3[0] == vb3[0] ? 0 : (va3[1] == vb3[0] ? 1 : 2);
+        t.nbf[z] = z2;
+        t.nbf[z2] = z;
+        t.alnmt[z] = t.alnmt[z2] = al;
+        ++p;  // a face pairs once
+      }
+    }
   });
   for (int64_t z = 0; z < nfaces; ++z) t.bctype[z] = t.nbf[z] >= 0 ? -1 : 0;
   return t;
-}
-
-bool build_geometry(const ElementTables& el, const double* x, const double* kappa,
-                    const Topology& topo, int64_t nele, double eta_e, int face_kappa, Geometry& g,
-                    std::string& err) {
-  const int D = el.D, nloc = el.nloc, nf = el.nface, nb = D + 1;
-  const int ng = D == 3 ? 6 : 3;
-  g.G.assign(ng * nele, 0.0);
-  g.cf.assign(nf * D * nele, 0.0);
-  g.wf.assign(nf * nele, 0.0);
-  g.pen.assign(nf * nele, 0.0);
-  g.nbface.assign(nf * nele, -1);
-  g.fcode.assign(nf * nele, 0);
-  g.detJ.assign(nele, 0.0);
-  g.max_affine_dev = 0.0;
-  double swsum = 0.0;
-  for (double w : el.sweight) swsum += w;
-  // elements are independent: chunks per host thread, per-thread deviation and error
-  const int nth = host_threads();
-  std::vector<double> tdev(nth, 0.0);
-  std::vector<int64_t> tbad(nth, -1);
-  parallel_chunks(nele, [&](int tid, int64_t e0, int64_t e1) {
-  for (int64_t e = e0; e < e1; ++e) {
-    const double* xe = x + e * D * nloc;  // (D, nloc)
-    double v[4][3] = {{0}};
-    for (int k = 0; k < nb; ++k)
-      for (int j = 0; j < D; ++j) v[k][j] = xe[j * nloc + k];
-    double J[3][3] = {{0}};
-    for (int i = 0; i < D; ++i)
-      for (int j = 0; j < D; ++j) J[i][j] = v[i][j] - v[D][j];
-    // affinity check of the P3 nodes
-    double hscale = 0.0;
-    for (int i = 0; i < D; ++i)
-      for (int j = 0; j < D; ++j) hscale = std::max(hscale, std::fabs(J[i][j]));
-    for (int n = 0; n < nloc; ++n)
-      for (int j = 0; j < D; ++j) {
-        double exp_x = 0.0;
-        for (int k = 0; k < nb; ++k) exp_x += el.p1_prolong[n * nb + k] * v[k][j];
-        double dev = std::fabs(xe[j * nloc + n] - exp_x) / (hscale > 0 ? hscale : 1.0);
-        tdev[tid] = std::max(tdev[tid], dev);
-      }
-    double det, inv[3][3];
-    if (D == 3) {
-      det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-      inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
-      inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
-      inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
-      inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
-      inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
-      inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
-      inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
-      inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
-      inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
-    } else {
-      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-      inv[0][0] = J[1][1] / det;
-      inv[0][1] = -J[0][1] / det;
-      inv[1][0] = -J[1][0] / det;
-      inv[1][1] = J[0][0] / det;
-    }
-    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
-      if (tbad[tid] < 0) tbad[tid] = e;
-      continue;
-    }
-    g.detJ[e] = det;
-    const double ke = kappa[e];
-    // metric kappa detJ invJ^T invJ  (G[j][k] = sum_i inv[i][j] inv[i][k])
-    auto Gjk = [&](int j, int k) {
-      double s = 0.0;
-      for (int i = 0; i < D; ++i) s += inv[i][j] * inv[i][k];
-      return ke * det * s;
-    };
-    if (D == 3) {
-      const double gv[6] = {Gjk(0, 0), Gjk(1, 1), Gjk(2, 2), Gjk(0, 1), Gjk(0, 2), Gjk(1, 2)};
-      for (int c = 0; c < 6; ++c) g.G[c * nele + e] = gv[c];
-    } else {
-      const double gv[3] = {Gjk(0, 0), Gjk(1, 1), Gjk(0, 1)};
-      for (int c = 0; c < 3; ++c) g.G[c * nele + e] = gv[c];
-    }
-    for (int f = 0; f < nf; ++f) {
-      double nrm[3] = {0, 0, 0}, sdet;
-      if (D == 3) {
-        double t[2][3];
-        for (int k = 0; k < 2; ++k)
-          for (int j = 0; j < 3; ++j) {
-            double s = 0.0;
-            for (int i = 0; i < 3; ++i) s += J[i][j] * TET_DRST[f][i][k];
-            t[k][j] = s;
-          }
-        double cr[3] = {t[0][1] * t[1][2] - t[0][2] * t[1][1],
-                        t[0][2] * t[1][0] - t[0][0] * t[1][2],
-                        t[0][0] * t[1][1] - t[0][1] * t[1][0]};
-        sdet = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
-        for (int j = 0; j < 3; ++j) nrm[j] = cr[j] / sdet;
-      } else {
-        double t[2];
-        for (int j = 0; j < 2; ++j) {
-          double s = 0.0;
-          for (int i = 0; i < 2; ++i) s += J[i][j] * TRI_DRST[f][i];
-          t[j] = s;
-        }
-        sdet = std::sqrt(t[0] * t[0] + t[1] * t[1]);
-        nrm[0] = t[1] / sdet;
-        nrm[1] = -t[0] / sdet;
-      }
-      for (int j = 0; j < D; ++j) {
-        double s = 0.0;
-        for (int i = 0; i < D; ++i) s += inv[i][j] * nrm[i];
-        g.cf[(f * D + j) * nele + e] = s;
-      }
-      double h = sdet * swsum;
-      if (D == 3) h = std::sqrt(h);
-      g.pen[f * nele + e] = eta_e / h;
-      const int64_t z = e * nf + f;
-      const int64_t type = topo.bctype[z];
-      double kf = 0.0;
-      if (type < 0 && topo.nbf[z] >= 0) {
-        const int64_t nbe = topo.nbf[z] / nf;
-        const double kn = kappa[nbe];
-        kf = face_kappa == 1 ? 2.0 * ke * kn / (ke + kn) : 0.5 * (ke + kn);
-        g.nbface[z] = (int32_t)topo.nbf[z];
-        g.fcode[z] = 0 | ((int32_t)topo.alnmt[z] << 4);
-      } else if (type == 0) {
-        kf = ke;
-        g.fcode[z] = 1;
-      } else if (type == 1) {
-        // Neumann: no operator term; the right-hand side adds int v g_N without kappa
-        // (z02/volume_mf_diff.py:360-400)
-        kf = 1.0;
-        g.fcode[z] = 3;
-      } else {
-        kf = 0.0;
-        g.fcode[z] = 2;
-      }
-      g.wf[f * nele + e] = kf * sdet;
-    }
-  }
-  });
-  for (int t = 0; t < nth; ++t) {
-    g.max_affine_dev = std::max(g.max_affine_dev, tdev[t]);
-    if (tbad[t] >= 0) {
-      err = "degenerate element " + std::to_string(tbad[t]);
-      return false;
-    }
-  }
-  return true;
 }
 
 void p1cg_sparsity(const int64_t* cells, int64_t nele, int D, int64_t nverts,
@@ -550,31 +423,39 @@ void p1cg_sparsity(const int64_t* cells, int64_t nele, int D, int64_t nverts,
   });
 }
 
+int color2_range(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola, int64_t v0,
+                 int64_t v1, std::vector<int32_t>& whichc) {
+  // First-fit in vertex order: vertex i takes the smallest colour that no distance-1 or
+  // distance-2 neighbour before it holds.  This is the colouring of the reference's
+  // multi-pass loop (z02/sparsity.py:235-269), which gives colour c in pass c to every
+  // uncoloured vertex that no earlier vertex of colour c is near: both decide vertex i from
+  // the vertices before it only.  Rows of [v0, v1) must reference columns in [v0, v1).
+  std::vector<int64_t> seen(64, -1);  // seen[c] == i: colour c is taken near vertex i
+  int ncolor = 0;
+  for (int64_t i = v0; i < v1; ++i) {
+    for (int64_t c = fina[i]; c < fina[i + 1]; ++c) {
+      const int64_t j = cola[c];
+      const int32_t cj = whichc[j];
+      if (cj > 0) seen[cj] = i;
+      for (int64_t c2 = fina[j]; c2 < fina[j + 1]; ++c2) {
+        const int32_t ck = whichc[cola[c2]];
+        if (ck > 0) seen[ck] = i;
+      }
+    }
+    int32_t k = 1;
+    while (k < (int32_t)seen.size() && seen[k] == i) ++k;
+    if (k >= (int32_t)seen.size()) seen.resize(2 * seen.size(), -1);
+    whichc[i] = k;
+    ncolor = std::max(ncolor, (int)k);
+  }
+  return ncolor;
+}
+
 int color2(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
            std::vector<int32_t>& whichc) {
   const int64_t n = (int64_t)fina.size() - 1;
   whichc.assign(n, 0);
-  int ncolor = 0;
-  std::vector<int64_t> todo(n);
-  std::iota(todo.begin(), todo.end(), 0);
-  while (!todo.empty()) {
-    ++ncolor;
-    for (int64_t i : todo) {
-      bool clash = false;
-      for (int64_t c = fina[i]; c < fina[i + 1] && !clash; ++c) {
-        int64_t j = cola[c];
-        if (whichc[j] == ncolor) { clash = true; break; }
-        for (int64_t c2 = fina[j]; c2 < fina[j + 1]; ++c2)
-          if (whichc[cola[c2]] == ncolor) { clash = true; break; }
-      }
-      if (!clash) whichc[i] = ncolor;
-    }
-    std::vector<int64_t> next;
-    for (int64_t i : todo)
-      if (whichc[i] == 0) next.push_back(i);
-    todo.swap(next);
-  }
-  return ncolor;
+  return n > 0 ? color2_range(fina, cola, 0, n, whichc) : 0;
 }
 
 void hilbert_order(const double* verts, int64_t nv, int D, std::vector<int64_t>& sfc) {
@@ -645,53 +526,100 @@ int nlevel_sfc(int64_t n) {
 void build_sfc_levels(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
                       const std::vector<double>& rar, const std::vector<int64_t>& sfc,
                       SfcLevelsHost& lv) {
+  // Level 0 is RAR renumbered into SFC order, each row sorted by column.  Row I of level
+  // l+1 gathers rows 2I and 2I+1 of level l with columns c -> c >> 1 (positions p and q of
+  // one level are summed into p >> 1, q >> 1: the aggregation of compact_fcode.f90:7-225);
+  // entries with one column are summed in the order: row 2I by column, then row 2I+1 by
+  // column.  Rows are independent: each pass counts, then fills, in parallel.
   const int64_t n0 = (int64_t)fina.size() - 1;
   lv.nlevel = nlevel_sfc(n0);
-  lv.n.clear();
-  lv.rowptr.clear();
-  lv.col.clear();
-  lv.val.clear();
-  lv.diag.clear();
-  // level 0 in SFC order: entries (pos(i), pos(j), a_ij)
-  struct Ent { int64_t r, c; double v; };
-  std::vector<Ent> ents;
-  ents.reserve(cola.size());
-  for (int64_t i = 0; i < n0; ++i)
-    for (int64_t c = fina[i]; c < fina[i + 1]; ++c)
-      ents.push_back({sfc[i] - 1, sfc[cola[c]] - 1, rar[c]});
-  int64_t nl = n0;
-  for (int l = 0; l < lv.nlevel; ++l) {
-    if (l > 0) {
-      for (auto& e : ents) {
-        e.r >>= 1;
-        e.c >>= 1;
-      }
-      nl = (nl - 1) / 2 + 1;
-    }
-    std::sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) {
-      return a.r != b.r ? a.r < b.r : a.c < b.c;
+  lv.n.assign(lv.nlevel, 0);
+  lv.rowptr.assign(lv.nlevel, {});
+  lv.col.assign(lv.nlevel, {});
+  lv.val.assign(lv.nlevel, {});
+  lv.diag.assign(lv.nlevel, {});
+  auto finish = [&](int l) {
+    const int64_t n = lv.n[l];
+    const std::vector<int32_t>& rp = lv.rowptr[l];
+    lv.diag[l].assign(n, 0.0);
+    parallel_chunks(n, [&](int, int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+          if (lv.col[l][k] == i) lv.diag[l][i] = lv.val[l][k];
     });
-    std::vector<Ent> red;
-    red.reserve(ents.size());
-    for (auto& e : ents) {
-      if (!red.empty() && red.back().r == e.r && red.back().c == e.c) red.back().v += e.v;
-      else red.push_back(e);
-    }
-    ents.swap(red);
-    std::vector<int32_t> rp(nl + 1, 0), cl(ents.size());
-    std::vector<double> vv(ents.size()), dg(nl, 0.0);
-    for (size_t k = 0; k < ents.size(); ++k) {
-      rp[ents[k].r + 1]++;
-      cl[k] = (int32_t)ents[k].c;
-      vv[k] = ents[k].v;
-      if (ents[k].r == ents[k].c) dg[ents[k].r] = ents[k].v;
-    }
-    for (int64_t i = 0; i < nl; ++i) rp[i + 1] += rp[i];
-    lv.n.push_back(nl);
-    lv.rowptr.push_back(std::move(rp));
-    lv.col.push_back(std::move(cl));
-    lv.val.push_back(std::move(vv));
-    lv.diag.push_back(std::move(dg));
+  };
+  // ---- level 0
+  {
+    std::vector<int64_t> node(n0);  // node at each SFC position
+    for (int64_t i = 0; i < n0; ++i) node[sfc[i] - 1] = i;
+    std::vector<int32_t>& rp = lv.rowptr[0];
+    rp.assign(n0 + 1, 0);
+    for (int64_t p = 0; p < n0; ++p) rp[p + 1] = (int32_t)(fina[node[p] + 1] - fina[node[p]]);
+    for (int64_t p = 0; p < n0; ++p) rp[p + 1] += rp[p];
+    lv.col[0].resize(rp[n0]);
+    lv.val[0].resize(rp[n0]);
+    parallel_chunks(n0, [&](int, int64_t a, int64_t b) {
+      std::vector<std::pair<int32_t, double>> row;
+      for (int64_t p = a; p < b; ++p) {
+        const int64_t i = node[p];
+        row.clear();
+        for (int64_t c = fina[i]; c < fina[i + 1]; ++c)
+          row.push_back({(int32_t)(sfc[cola[c]] - 1), rar[c]});
+        std::stable_sort(row.begin(), row.end(),
+                         [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (size_t k = 0; k < row.size(); ++k) {
+          lv.col[0][rp[p] + k] = row[k].first;
+          lv.val[0][rp[p] + k] = row[k].second;
+        }
+      }
+    });
+    lv.n[0] = n0;
+    finish(0);
+  }
+  // ---- coarser levels
+  for (int l = 1; l < lv.nlevel; ++l) {
+    const int64_t nf = lv.n[l - 1], nc = (nf - 1) / 2 + 1;
+    const std::vector<int32_t>& frp = lv.rowptr[l - 1];
+    const std::vector<int32_t>& fcol = lv.col[l - 1];
+    const std::vector<double>& fval = lv.val[l - 1];
+    std::vector<int32_t>& rp = lv.rowptr[l];
+    rp.assign(nc + 1, 0);
+    // the coarse row of the pair as (column, value), merged; count then fill
+    auto merge = [&](int64_t I, std::vector<std::pair<int32_t, double>>& row) {
+      row.clear();
+      for (int64_t i = 2 * I; i < std::min(2 * I + 2, nf); ++i)
+        for (int32_t k = frp[i]; k < frp[i + 1]; ++k) row.push_back({fcol[k] >> 1, fval[k]});
+      std::stable_sort(row.begin(), row.end(),
+                       [](const auto& x, const auto& y) { return x.first < y.first; });
+      size_t w = 0;
+      for (size_t k = 0; k < row.size(); ++k) {
+        if (w > 0 && row[w - 1].first == row[k].first) row[w - 1].second += row[k].second;
+        else row[w++] = row[k];
+      }
+      row.resize(w);
+    };
+    parallel_chunks(nc, [&](int, int64_t a, int64_t b) {
+      std::vector<std::pair<int32_t, double>> row;
+      for (int64_t I = a; I < b; ++I) {
+        merge(I, row);
+        rp[I + 1] = (int32_t)row.size();
+      }
+    });
+    for (int64_t I = 0; I < nc; ++I) rp[I + 1] += rp[I];
+    lv.col[l].resize(rp[nc]);
+    lv.val[l].resize(rp[nc]);
+    parallel_chunks(nc, [&](int, int64_t a, int64_t b) {
+      std::vector<std::pair<int32_t, double>> row;
+      for (int64_t I = a; I < b; ++I) {
+        merge(I, row);
+        for (size_t k = 0; k < row.size(); ++k) {
+          lv.col[l][rp[I] + k] = row[k].first;
+          lv.val[l][rp[I] + k] = row[k].second;
+        }
+      }
+    });
+    lv.n[l] = nc;
+    finish(l);
   }
 }
 

@@ -52,6 +52,9 @@ struct ElementTables {
   std::vector<double> sweight;  // (sngi)
   std::vector<int> gi_align;    // (D, sngi)
   std::vector<double> p1_prolong;  // (nloc, D+1)
+  // d(xi)/d(face params) per face (z02/function_space.py:200-213): 3D (nface, 3, 2),
+  // 2D (nface, 2)
+  std::vector<double> drst;
   std::vector<int> index;       // (nloc, D+1) barycentric multi-indices
   // Face traces through a quadratic interpolant.  On face f only the snloc nodes fnodes[f]
   // have nonzero basis functions, and the reference gradient of every basis function
@@ -72,36 +75,15 @@ struct Topology {
 };
 Topology build_topology(const int64_t* cells, int64_t nele, int D);
 
-// Affine geometry per element.  J[i][j] = x_i[j] - x_D[j] (rows are edge vectors
-// from vertex D), which equals the reference's iso-parametric Jacobian for
-// straight-sided P3 elements.  All arrays are structure-of-arrays, length nele each
-// component.
-struct Geometry {
-  // metric  kappa*detJ*(invJ^T invJ): D==3 -> 6 comps (00,11,22,01,02,12); D==2 -> 3 (00,11,01)
-  std::vector<double> G;
-  // c_f = invJ^T n_f : (nface*D) comps
-  std::vector<double> cf;
-  // wf = kappa_f * sdet_f on interior and Dirichlet faces, sdet_f on Neumann faces (their
-  // right-hand-side term carries no kappa), 0 on faces without a term; pen = eta/h
-  std::vector<double> wf, pen;
-  // face record: neighbour global face (N*nface + f') or -1; alignment; type
-  std::vector<int32_t> nbface;  // (nele*nface) element-major
-  // (nele*nface): code | (alnmt << 4) with code 0 interior (bctype -1), 1 Dirichlet (0),
-  // 3 Neumann (1), 2 any other type (no term, as the reference's FSI interface faces)
-  std::vector<int32_t> fcode;
-  std::vector<double> detJ;     // (nele)
-  double max_affine_dev = 0.0;  // max |x_node - affine position| / h
-};
-bool build_geometry(const ElementTables& el, const double* x_ref_in, const double* kappa,
-                    const Topology& topo, int64_t nele, double eta_e, int face_kappa, Geometry& g,
-                    std::string& err);
-
 // P1CG CSR graph (z02/sparsity.py:102-122): all vertex pairs per element, sorted.
 void p1cg_sparsity(const int64_t* cells, int64_t nele, int D, int64_t nverts,
                    std::vector<int64_t>& fina, std::vector<int64_t>& cola);
 // greedy distance-2 colouring, colours 1..ncolor (z02/sparsity.py:235-269)
 int color2(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
            std::vector<int32_t>& whichc);
+// the same on the vertex range [v0, v1) of an independent block (whichc zero there before)
+int color2_range(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola, int64_t v0,
+                 int64_t v1, std::vector<int32_t>& whichc);
 // Hilbert SFC numbering: sfc[node] = 1-based position (ties broken by node id).
 void hilbert_order(const double* verts, int64_t nverts, int D, std::vector<int64_t>& sfc);
 int nlevel_sfc(int64_t n);

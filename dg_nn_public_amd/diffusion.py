@@ -159,19 +159,31 @@ class DGDiffusion:
         if self.device.type != "cuda":
             raise ValueError("DGDiffusion runs on a GPU device")
         torch.cuda.set_device(self.device)
-        x = np.ascontiguousarray(np.asarray(x_ref_in, dtype=np.float64))
+        # x_ref_in: one (nele, D, nloc) array, or with components one array per component
+        parts = None
+        if isinstance(x_ref_in, (list, tuple)):
+            if components is None:
+                raise ValueError("x_ref_in given in pieces needs components")
+            parts = [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in x_ref_in]
+            if not parts or any(a.ndim != 3 or a.shape[1:] != parts[0].shape[1:] for a in parts):
+                raise ValueError("every piece of x_ref_in must be (nele_c, D, nloc)")
+            x = parts[0][:0]
+            D, nele = parts[0].shape[1], sum(a.shape[0] for a in parts)
+        else:
+            x = np.ascontiguousarray(np.asarray(x_ref_in, dtype=np.float64))
+            if x.ndim != 3:
+                raise ValueError("x_ref_in must be (nele, D, nloc) with D in (2, 3)")
+            D, nele = x.shape[1], x.shape[0]
         cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int64))
         verts = np.ascontiguousarray(np.asarray(verts, dtype=np.float64))
-        if x.ndim != 3 or x.shape[1] not in (2, 3):
+        if D not in (2, 3) or x.shape[2] != (10 if D == 2 else 20):
             raise ValueError("x_ref_in must be (nele, D, nloc) with D in (2, 3)")
-        D = x.shape[1]
-        nele = x.shape[0]
         if cells.shape != (nele, D + 1) or verts.ndim != 2 or verts.shape[1] != D:
             raise ValueError("cells must be (nele, D+1) and verts (nverts, D)")
         kappa = np.ones(nele) if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
         if kappa.shape != (nele,):
             raise ValueError("kappa must have one value per element")
-        self._keep = [x, cells, verts, kappa]
+        self._keep = [x, cells, verts, kappa] + (parts or [])
         bc = None
         if bctype is not None:
             bc = np.ascontiguousarray(np.asarray(bctype, dtype=np.int64).reshape(-1))
@@ -197,10 +209,21 @@ class DGDiffusion:
                 raise ValueError("components must be two offset arrays of equal length >= 2")
             self.ncomp = eo.size - 1
             self.elem_off, self.vert_off = eo, vo
-            check(self.lib.dgmi_ctx_create_batch(C.byref(desc), self.ncomp,
-                                                 eo.ctypes.data_as(_lib._I64P),
-                                                 vo.ctypes.data_as(_lib._I64P), C.byref(ctx)),
-                  "dgmi_ctx_create_batch")
+            if parts is None:
+                check(self.lib.dgmi_ctx_create_batch(C.byref(desc), self.ncomp,
+                                                     eo.ctypes.data_as(_lib._I64P),
+                                                     vo.ctypes.data_as(_lib._I64P), C.byref(ctx)),
+                      "dgmi_ctx_create_batch")
+            else:
+                if len(parts) != self.ncomp or any(a.shape[0] != eo[q + 1] - eo[q]
+                                                   for q, a in enumerate(parts)):
+                    raise ValueError("one piece of x_ref_in per component, of its element count")
+                ptrs = (_lib._DP * len(parts))(*[a.ctypes.data_as(_lib._DP) for a in parts])
+                check(self.lib.dgmi_ctx_create_batch_parts(C.byref(desc), self.ncomp,
+                                                           eo.ctypes.data_as(_lib._I64P),
+                                                           vo.ctypes.data_as(_lib._I64P), ptrs,
+                                                           C.byref(ctx)),
+                      "dgmi_ctx_create_batch_parts")
         self.ctx = ctx
         if alloc_limit:
             self.set_alloc_limit(alloc_limit)
@@ -209,7 +232,7 @@ class DGDiffusion:
         self.D = D
         self.nele, self.nloc, self.ndof, self.nface = info[0], info[1], info[2], info[3]
         self.ngi, self.sngi, self.nverts = info[4], info[5], info[6]
-        self.x_ref_in = x
+        self.x_ref_in = x if parts is None else parts
         self.cells = cells
         self.verts = verts
         self.kappa = kappa
@@ -277,8 +300,8 @@ class DGDiffusion:
 
     # ---------------------------------------------------------------- operator
     def set_f32_kernel(self, name):
-        """Kernel family of FP32 applies and smooths: "vector" (FMA), "mfma" (fp32 matrix
-        cores, the default) or "mfma_bf16" (bf16 inputs, fp32 accumulation)."""
+        """Kernel family of FP32 applies and smooths: "vector" (FMA, the default), "mfma"
+        (fp32 matrix cores) or "mfma_bf16" (bf16 inputs, fp32 accumulation)."""
         if name not in F32_KERNELS:
             raise ValueError(f"f32 kernel must be one of {sorted(F32_KERNELS)}")
         check(self.lib.dgmi_set_param(self.ctx, b"f32_kernel", float(F32_KERNELS[name])),
@@ -290,6 +313,15 @@ class DGDiffusion:
         launching work."""
         check(self.lib.dgmi_set_param(self.ctx, b"alloc_limit_bytes", float(nbytes)),
               "dgmi_set_param")
+
+    SETUP_PHASES = ("topology", "geometry", "upload", "smoother", "graph", "probe", "levels",
+                    "layout")
+
+    def setup_timing(self):
+        """Host wall-clock seconds of this context's set-up phases (dgmi_setup_timing)."""
+        out = (C.c_double * 8)()
+        check(self.lib.dgmi_setup_timing(self.ctx, out), "dgmi_setup_timing")
+        return dict(zip(self.SETUP_PHASES, list(out)))
 
     def device_bytes(self):
         """Device bytes this context holds."""

@@ -58,6 +58,9 @@ class Result:
     x: torch.Tensor | None = None
     converged: bool = True
     error: str | None = None
+    # the job's wall seconds of set-up (context, multigrid, right-hand side) and solve, its
+    # element count and the library's set-up phase timings (shared by the job's results)
+    job_info: dict | None = None
 
     @property
     def latency(self):
@@ -147,13 +150,20 @@ class Server:
                 try:
                     # jobs set up and solve concurrently: no stream capture while other
                     # threads allocate and copy
+                    ta = time.perf_counter()
                     bt = DGBatch([r.mesh for r in job.reqs], [r.kappa for r in job.reqs],
                                  device=self.device, graph_vcycle=False,
                                  alloc_limit=self.job_byte_limit)
                     b = bt.rhs([r.f for r in job.reqs], [r.u_bc for r in job.reqs])
+                    stream.synchronize()
+                    tb = time.perf_counter()
                     x, st = bt.solve(b, rtol=self.rtol, inner=self.inner, max_its=self.max_its)
                     stream.synchronize()
                     done = time.perf_counter() - t0
+                    info = {"setup_s": tb - ta, "solve_s": time.perf_counter() - tb,
+                            "start": ta - t0, "setup_end": tb - t0,
+                            "elements": int(bt.space.nele), "solves": len(job.reqs),
+                            "phases": bt.space.setup_timing()}
                     used = float(bt.space.device_bytes())
                     ne = sum(r.mesh.cells.shape[0] for r in job.reqs)
                     parts = bt.split(x) if self.keep else [None] * len(job.reqs)
@@ -163,7 +173,7 @@ class Server:
                         out.append(Result(r.rid, r.arrival, job.admitted, done, s["its"], rel,
                                           job.jid, p.clone() if p is not None and err is None
                                           else None, converged=err is None and rel <= self.rtol,
-                                          error=err))
+                                          error=err, job_info=info))
                 finally:
                     if bt is not None:
                         bt.close()

@@ -66,8 +66,21 @@ int dgmi_ctx_destroy(dgmi_ctx* ctx);
  * serving workloads. */
 int dgmi_ctx_create_batch(const dgmi_mesh_desc* desc, int32_t ncomp, const int64_t* elem_off,
                           const int64_t* vert_off, dgmi_ctx** out);
+/* dgmi_ctx_create_batch with component c's P3 node coordinates at x_parts[c] (host,
+ * (nele_c, D, nloc)) instead of one concatenated desc->x_ref_in (ignored): the pieces are
+ * copied to the device one by one, with no host copy of the union. */
+int dgmi_ctx_create_batch_parts(const dgmi_mesh_desc* desc, int32_t ncomp,
+                                const int64_t* elem_off, const int64_t* vert_off,
+                                const double* const* x_parts, dgmi_ctx** out);
 /* info[0..7] = nele, nloc, ndof, nface, ngi, sngi, nverts, device bytes held */
 int dgmi_ctx_info(const dgmi_ctx* ctx, int64_t* info);
+/* Host wall-clock seconds spent in the set-up phases of this context so far, the analogue
+ * of the reference's time.time() prints around its set-up (z02/main.py:32-110):
+ * seconds[0] topology, [1] geometry, [2] device tables and uploads (dgmi_ctx_create);
+ * [3] block diagonal and smoother (dgmi_smoother_setup); [4] P1CG graph, colouring and SFC
+ * order, [5] RAR probing (up to the host copy of RAR), [6] SFC level operators, [7] device
+ * layout of the levels and the coarse inverses (dgmi_mg_setup). */
+int dgmi_setup_timing(const dgmi_ctx* ctx, double* seconds);
 /* host copies of nbf, alnmt, glb_bcface_type (each nele*nface) */
 int dgmi_topology(const dgmi_ctx* ctx, int64_t* nbf, int64_t* alnmt, int64_t* bctype);
 /* host copy of the element tables: n (nloc*ngi), nlx (D*nloc*ngi), weight (ngi),
@@ -199,7 +212,7 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     one re-orthogonalisation (default).
  *   "inner_flexible"  FP32/BF16 refinement cycles: 1 = flexible (right-preconditioned)
  *                     GMRES (default), 0 = left-preconditioned like the FP64 solve.
- *   "f32_kernel"      kernels of FP32 applies: 0 = vector FMA, 1 = MFMA fp32 (default),
+ *   "f32_kernel"      kernels of FP32 applies: 0 = vector FMA (default), 1 = MFMA fp32,
  *                     2 = MFMA with bf16 inputs.
  *   "graph_vcycle"    1 = replay the V-cycle from a captured HIP graph, 0 = launch its
  *                     kernels one by one on the stream (default: measured 1% faster in a
