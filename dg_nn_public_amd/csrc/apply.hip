@@ -565,9 +565,11 @@ __global__ __launch_bounds__(256) void k_build_S_direct(int nele, const double* 
   }
 }
 
-// FP32 packed copy of the FP64 smoother: S32[(a,b)] = (S[a][b] + S[b][a]) / 2, a <= b
+// Packed symmetric copies of the full FP64 smoother: P[(a,b)] = (S[a][b] + S[b][a]) / 2,
+// a <= b, in FP64, FP32 and (when S16 is set) bfloat16
 template <int NL>
 __global__ __launch_bounds__(256) void k_pack_S_sym(int nele, const double* __restrict__ S64,
+                                                    double* __restrict__ P64,
                                                     float* __restrict__ S32,
                                                     uint16_t* __restrict__ S16) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -576,32 +578,31 @@ __global__ __launch_bounds__(256) void k_pack_S_sym(int nele, const double* __re
     for (int b = a; b < NL; ++b) {
       const double v = 0.5 * (S64[s_index<double, NL>(a, b, e, nele)] +
                               S64[s_index<double, NL>(b, a, e, nele)]);
+      P64[(size_t)sym_index<NL>(a, b) * nele + e] = v;
       S32[(size_t)sym_index<NL>(a, b) * nele + e] = (float)v;
       if (S16) S16[(size_t)sym_index<NL>(a, b) * nele + e] = f32_to_bf16((float)v);
     }
 }
 
-int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* fail,
+int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, double* S, int* fail,
                           hipStream_t s) {
   const int nele = (int)c->nele;
   const dim3 blk(256), grd((nele + 3) / 4);
   if (c->D == 3)
-    hipLaunchKernelGGL(k_build_S_direct<20>, grd, blk, 0, s, nele, bdiag, omega, c->g64.S,
-                       fail);
+    hipLaunchKernelGGL(k_build_S_direct<20>, grd, blk, 0, s, nele, bdiag, omega, S, fail);
   else
-    hipLaunchKernelGGL(k_build_S_direct<10>, grd, blk, 0, s, nele, bdiag, omega, c->g64.S,
-                       fail);
+    hipLaunchKernelGGL(k_build_S_direct<10>, grd, blk, 0, s, nele, bdiag, omega, S, fail);
   DGMI_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s) {
+int launch_pack_S_sym(dgmi_ctx* c, const double* S, hipStream_t s) {
   const int nele = (int)c->nele;
   const dim3 blk(256), grd((nele + 255) / 256);
   if (c->D == 3)
-    hipLaunchKernelGGL(k_pack_S_sym<20>, grd, blk, 0, s, nele, c->g64.S, c->g32.S, c->g32.S16);
+    hipLaunchKernelGGL(k_pack_S_sym<20>, grd, blk, 0, s, nele, S, c->g64.S, c->g32.S, c->g32.S16);
   else
-    hipLaunchKernelGGL(k_pack_S_sym<10>, grd, blk, 0, s, nele, c->g64.S, c->g32.S, c->g32.S16);
+    hipLaunchKernelGGL(k_pack_S_sym<10>, grd, blk, 0, s, nele, S, c->g64.S, c->g32.S, c->g32.S16);
   DGMI_HIP(hipGetLastError());
   return 0;
 }

@@ -1189,31 +1189,46 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
              "unknown smoother variant");
   hipStream_t s = (hipStream_t)stream;
   DGMI_TRY(compute_bdiag(c, s));
-  // FP64 full blocks; FP32 the packed symmetric form (NL(NL+1)/2 entries per element)
+  // built as full FP64 blocks in a work buffer, kept as the packed symmetric form
+  // (NL(NL+1)/2 entries per element) in FP64 and FP32
   const int64_t nS = c->nele * c->nloc * c->nloc;
-  const int64_t nS32 = c->nele * (c->nloc * (c->nloc + 1) / 2);
-  if (!c->g64.S) c->g64.S = static_cast<double*>(c->alloc(sizeof(double) * nS));
-  if (!c->g32.S) c->g32.S = static_cast<float*>(c->alloc(sizeof(float) * nS32));
+  const int64_t nSp = c->nele * (c->nloc * (c->nloc + 1) / 2);
+  if (!c->g64.S) c->g64.S = static_cast<double*>(c->alloc(sizeof(double) * nSp));
+  if (!c->g32.S) c->g32.S = static_cast<float*>(c->alloc(sizeof(float) * nSp));
   DGMI_CHECK(c->g64.S && c->g32.S, alloc_failure());
   if (variant == DGMI_SMOOTH_DIRECT) {
     // the bf16 inner cycles' neighbour-form sweeps read a bfloat16 copy of the smoother
-    if (!c->g32.S16) c->g32.S16 = static_cast<uint16_t*>(c->alloc(sizeof(uint16_t) * nS32));
+    if (!c->g32.S16) c->g32.S16 = static_cast<uint16_t*>(c->alloc(sizeof(uint16_t) * nSp));
     DGMI_CHECK(c->g32.S16, alloc_failure());
   }
   Scratch& sc = scratch(c);
+  double* full = nullptr;
+  if (c->alloc_limit > 0 && c->dev_bytes + sizeof(double) * nS > c->alloc_limit) {
+    set_error(alloc_failure("smoother work buffer: the context's device-byte limit"));
+    return -2;
+  }
+  DGMI_HIP(hipMalloc(&full, sizeof(double) * nS));
+  struct FreeFull {
+    double* p;
+    hipStream_t s;
+    ~FreeFull() {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(p);
+    }
+  } free_full{full, s};
   if (variant == DGMI_SMOOTH_JACOBI3) {
-    DGMI_TRY(launch_build_S_jacobi<double>(c, sc.bdiag, omega, c->g64.S, s));
+    DGMI_TRY(launch_build_S_jacobi<double>(c, sc.bdiag, omega, full, s));
   } else {
     int* fail = static_cast<int*>(c->alloc(sizeof(int)));
     DGMI_CHECK(fail, alloc_failure());
     DGMI_HIP(hipMemsetAsync(fail, 0, sizeof(int), s));
-    DGMI_TRY(launch_build_S_direct(c, sc.bdiag, omega, fail, s));
+    DGMI_TRY(launch_build_S_direct(c, sc.bdiag, omega, full, fail, s));
     int hf = 0;
     DGMI_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, s));
     DGMI_HIP(hipStreamSynchronize(s));
     DGMI_CHECK(hf == 0, "singular diagonal block " + std::to_string(hf - 1));
   }
-  DGMI_TRY(launch_pack_S_sym(c, s));
+  DGMI_TRY(launch_pack_S_sym(c, full, s));
   c->smoother_variant = variant;
   c->omega = omega;
   return 0;

@@ -43,7 +43,7 @@ template <typename T> struct DevGeom {
   T* cf = nullptr;     // (nface*D, nele)
   T* wf = nullptr;     // (nface, nele)
   T* pen = nullptr;    // (nface, nele)
-  T* S = nullptr;      // smoother (nloc*nloc, nele), row n col m at (n*nloc+m)*nele + e
+  T* S = nullptr;      // smoother, packed symmetric: (a, b), a <= b, at sym_index(a, b)*nele + e
   // face traces: (nface*sngi, nele) for the vector kernels, (nele, nface, sngi) for the
   // matrix-core kernels
   typename Vec2<T>::type* trace = nullptr;
@@ -322,7 +322,7 @@ template <typename T>
 int launch_build_S_jacobi(dgmi_ctx* c, const double* bdiag, double omega, T* S, hipStream_t s);
 // omega * inverse of every diagonal block into both smoother copies; *fail = 1 + element
 // index of a singular block
-int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, int* fail,
+int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, double* S, int* fail,
                           hipStream_t s);
 // FP32 vector-family sweeps carrying traces between them (see apply.hip)
 bool trace_io(const dgmi_ctx* c);
@@ -336,9 +336,9 @@ int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int m
                     hipStream_t s);
 // the exact block smoother has a bfloat16 copy for the bf16 inner cycles
 bool bf16_smoother(const dgmi_ctx* c);
-// g32.S = packed symmetric FP32 copy of g64.S (and g32.S16 its bfloat16 rounding, when
-// allocated)
-int launch_pack_S_sym(dgmi_ctx* c, hipStream_t s);
+// g64.S / g32.S = packed symmetric FP64 / FP32 copies of the full FP64 smoother S (and
+// g32.S16 its bfloat16 rounding, when allocated)
+int launch_pack_S_sym(dgmi_ctx* c, const double* S, hipStream_t s);
 template <typename T>
 int launch_convert(const double* in, T* out, int64_t n, hipStream_t s);
 // transfers + SFC levels (mg.hip)

@@ -85,7 +85,7 @@ __device__ __forceinline__ void load_S_row(const T* __restrict__ S, int n, int e
   }
 }
 
-// FP32 smoother matrices are stored packed: the block inverse is symmetric, so entry
+// Smoother matrices are stored packed: both smoothers are symmetric, so entry
 // (a, b), a <= b, of element e at sym_index(a, b)*nele + e carries S[a][b] and S[b][a]
 // (NL(NL+1)/2 vectors of nele values instead of NL*NL).
 template <int NL>
@@ -97,9 +97,9 @@ constexpr int sym_size() {
   return NL * (NL + 1) / 2;
 }
 
-template <int NL>
-__device__ __forceinline__ void load_S_row_sym(const float* __restrict__ S, int n, int e,
-                                               int nele, float* v) {
+template <int NL, typename T>
+__device__ __forceinline__ void load_S_row_sym(const T* __restrict__ S, int n, int e, int nele,
+                                               T* v) {
 #pragma unroll
   for (int m = 0; m < NL; ++m) {
     const int a = m < n ? m : n, b = m < n ? n : m;
@@ -144,13 +144,12 @@ __device__ __forceinline__ void load_S_row_sym16(const uint16_t* __restrict__ S,
   }
 }
 
-// row n of the smoother of element e in the layout of its precision (FP64 full, FP32 packed)
+// row n of the packed symmetric smoother of element e (FP64, FP32 or bfloat16 storage)
 template <typename T, int NL, typename SS = T>
 __device__ __forceinline__ void load_smoother_row(const SS* __restrict__ S, int n, int e, int nele,
                                                   T* v) {
   if constexpr (std::is_same<SS, uint16_t>::value) load_S_row_sym16<NL>(S, n, e, nele, v);
-  else if constexpr (sizeof(T) == 4) load_S_row_sym<NL>(S, n, e, nele, v);
-  else load_S_row<T, NL>(S, n, e, nele, v);
+  else load_S_row_sym<NL, T>(S, n, e, nele, v);
 }
 
 }  // namespace dgmi
