@@ -44,7 +44,7 @@ PEAK_HBM = 8.0e12     # B/s
 METRIC = "solves/s + p50/p99 latency at rtol 1e-10; DG apply GDoF/s, mixed vs FP64"
 # calibrated HBM bytes per launch of the FP32 smoothing sweep (k_residual_w<float,3,5,true>):
 # FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3 --pmc passes (profiles/), by element count
-PMC_SWEEP = {82944: 200.3e6}
+PMC_SWEEP = {82944: 200.3e6, 663552: 1701.4e6}
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):

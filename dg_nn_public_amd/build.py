@@ -27,7 +27,9 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # keep packed FP32 instructions out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast",
+    # fused multiply-adds everywhere except where a function turns contraction off
+    # (the error-free transformations of the double-double residual)
+    "-ffp-contract=fast-honor-pragmas",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", f"-I{INCLUDE}",

@@ -19,6 +19,7 @@
 // 4: r = +A x; 5: x = (1 - omega) x + S (b - A_off x), where A_off is A without its
 // diagonal blocks; 6: x = S b (a sweep from x = 0).  Mode 5 equals mode 2 when S = omega A_ee^-1 (the exact block
 // smoother): it needs no volume term and only the neighbour half of the face terms.
+#include "dd.h"
 #include "dgmi.h"
 #include "dgmi_internal.h"
 #include "faces.h"
@@ -306,6 +307,145 @@ __global__ __launch_bounds__(64 * (D + 1),
           tout[(f * M::SG + g) * nele + e] = make_trace<typename Vec2<T>::type>(u, dn);
         });
       }
+    }
+  }
+}
+
+// r = b - A x with every operation in double-double arithmetic (dd.h) and the result
+// rounded once to FP64.  The operator is the FP64 one of k_residual_w (same geometry,
+// tables and face terms; its coefficients are taken as exact), but the sum is carried to
+// about 106 bits: in FP64 the volume and face contributions of a smooth x cancel from
+// |A||x| down to |b - A x|, and the rounding of those terms alone can reach 1e-10 ||b||
+// for strongly varying diffusivity on fine meshes.  One thread per element computes its
+// own and its neighbours' face traces directly from x (no trace buffer); the solvers call
+// it only for the few true residuals that decide convergence.
+// The face terms are formed without lane-divergent branches (with this kernel's register
+// pressure a branching version returned non-finite values on faces that part of a wave
+// skipped): a face of another type than
+// interior (0) or Dirichlet (1) gets weight 0, a boundary face reads its own element as
+// the "neighbour" with that neighbour's values multiplied out, and the interior and
+// Dirichlet fluxes share one formula,
+//   q_s = w (-a (du/dn - du'/dn) + pen (u - u')),  q_d = -w a (u - u'),
+// with a = 1/2 and the neighbour's u', du'/dn (outward normal of the neighbour) on an
+// interior face, a = 1 and u' = du'/dn = 0 on a Dirichlet face.
+template <int D>
+__global__ __launch_bounds__(64) void k_residual_dd(
+    int nele, const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
+    const double* __restrict__ vtab, const double* __restrict__ wgt,
+    const double* __restrict__ sw, const int32_t* __restrict__ aln,
+    const double* __restrict__ G, const double* __restrict__ cf, const double* __restrict__ wf,
+    const double* __restrict__ pen, const int32_t* __restrict__ nbface,
+    const int32_t* __restrict__ fcode, const double* __restrict__ fc,
+    const int32_t* __restrict__ fnodes) {
+  using M = Dm<D>;
+  using F = FaceDims<D>;
+  constexpr int NL = M::NL, NF = M::NF, SG = M::SG;
+  const int e = min((int)(blockIdx.x * blockDim.x + threadIdx.x), nele - 1);
+  const bool store = (int)(blockIdx.x * blockDim.x + threadIdx.x) < nele;
+  dd xv[NL], acc[NL];
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    xv[n] = dd(x[(size_t)e * NL + n]);
+    acc[n] = dd(0.0);
+  }
+  // ---- volume: acc_n = sum_g w_g t_g(n) . (G t_g(x))
+  double gm[M::NG];
+#pragma unroll
+  for (int k = 0; k < M::NG; ++k) gm[k] = G[k * nele + e];
+#pragma unroll 1
+  for (int g = 0; g < M::NGI; ++g) {
+    const double* t = vtab + g * D * NL;
+    dd gd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gd[d] = dd(0.0);
+#pragma unroll
+    for (int n = 0; n < NL; ++n)
+#pragma unroll
+      for (int d = 0; d < D; ++d) gd[d] += t[d * NL + n] * xv[n];
+    const double wq = wgt[g];
+    dd y[D];
+    if constexpr (D == 3) {
+      y[0] = wq * (gm[0] * gd[0] + gm[3] * gd[1] + gm[4] * gd[2]);
+      y[1] = wq * (gm[3] * gd[0] + gm[1] * gd[1] + gm[5] * gd[2]);
+      y[2] = wq * (gm[4] * gd[0] + gm[5] * gd[1] + gm[2] * gd[2]);
+    } else {
+      y[0] = wq * (gm[0] * gd[0] + gm[2] * gd[1]);
+      y[1] = wq * (gm[2] * gd[0] + gm[1] * gd[1]);
+    }
+#pragma unroll
+    for (int n = 0; n < NL; ++n)
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[n] += t[d * NL + n] * y[d];
+  }
+  // ---- faces
+#pragma unroll 1
+  for (int f = 0; f < NF; ++f) {
+    const int z = e * NF + f;
+    const int code = fcode[z];
+    const int type = code & 15;
+    const bool interior = type == 0;
+    const double wfe = (type < 2) ? wf[f * nele + e] : 0.0;
+    const double pe = pen[f * nele + e];
+    const double half = interior ? 0.5 : 1.0;
+    const double keep = interior ? 1.0 : 0.0;  // multiplies the neighbour's values
+    const int nbz = interior ? nbface[z] : z;   // a boundary face reads itself
+    const int nbe = nbz / NF, nbf = nbz - (nbz / NF) * NF;
+    const int a = interior ? (code >> 4) : 0;
+    dd c[D], xf[F::NFN], tu[SG], tdn[SG], nu[SG], ndn[SG];
+#pragma unroll
+    for (int d = 0; d < D; ++d) c[d] = dd(cf[(f * D + d) * nele + e]);
+#pragma unroll
+    for (int i = 0; i < F::NFN; ++i) xf[i] = dd(x[(size_t)e * NL + fnodes[f * F::NFN + i]]);
+    face_traces<dd, D>(f, fc, xv, xf, c, [&](int g, dd u, dd dn) {
+      tu[g] = u;
+      tdn[g] = dn;
+    });
+    {
+      dd xn[NL], xnf[F::NFN], cn[D];
+#pragma unroll
+      for (int n = 0; n < NL; ++n) xn[n] = dd(x[(size_t)nbe * NL + n]);
+#pragma unroll
+      for (int i = 0; i < F::NFN; ++i)
+        xnf[i] = dd(x[(size_t)nbe * NL + fnodes[nbf * F::NFN + i]]);
+#pragma unroll
+      for (int d = 0; d < D; ++d) cn[d] = dd(cf[(nbf * D + d) * nele + nbe]);
+      face_traces<dd, D>(nbf, fc, xn, xnf, cn, [&](int g, dd u, dd dn) {
+        nu[g] = keep * u;
+        ndn[g] = keep * dn;
+      });
+    }
+    FaceLift<dd, D> fl;
+    fl.zero();
+#pragma unroll
+    for (int g = 0; g < SG; ++g) {
+      const dd ws = wfe * dd(sw[g]);
+      // the neighbour's point aligned with g, selected without indexing a register array
+      const int ga = aln[a * SG + g];
+      dd tnu(0.0), tndn(0.0);
+#pragma unroll
+      for (int g2 = 0; g2 < SG; ++g2) {
+        tnu = g2 == ga ? nu[g2] : tnu;
+        tndn = g2 == ga ? ndn[g2] : tndn;
+      }
+      const dd jump = tu[g] - tnu;
+      const dd qsn = ws * (-half * (tdn[g] - tndn) + pe * jump);
+      const dd qdn = ws * (-half * jump);
+      fl.add(f, g, fc, qsn, qdn);
+    }
+    fl.finish(f, fc, c, acc);
+#pragma unroll
+    for (int i = 0; i < F::NFN; ++i) {
+      const int node = fnodes[f * F::NFN + i];  // wave-uniform
+#pragma unroll
+      for (int n = 0; n < NL; ++n)
+        if (n == node) acc[n] += fl.accf[i];
+    }
+  }
+  if (store) {
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      const dd s = dd(b[(size_t)e * NL + n]) - acc[n];
+      r[(size_t)e * NL + n] = s.hi;
     }
   }
 }
@@ -770,6 +910,21 @@ int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int m
 }
 
 
+
+int launch_residual_dd(dgmi_ctx* c, const double* x, const double* b, double* r, hipStream_t s) {
+  const int nele = (int)c->nele;
+  const DevGeom<double>& g = c->g64;
+  const dim3 blk(64), grd((nele + 63) / 64);
+#define DGMI_RDD(D)                                                                            \
+  hipLaunchKernelGGL(k_residual_dd<D>, grd, blk, 0, s, nele, x, b, r, g.tab + g.off_vtab,       \
+                     g.tab + g.off_wgt, g.tab + g.off_sw, c->aln, g.G, g.cf, g.wf, g.pen,       \
+                     c->nbface, c->fcode, g.tab + g.off_fc, c->fnodes)
+  if (c->D == 3) DGMI_RDD(3);
+  else DGMI_RDD(2);
+#undef DGMI_RDD
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
 
 template <typename T>
 int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t s) {

@@ -555,14 +555,43 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
     ~MaskScope() { c->mask_on = false; }
   } mask_scope{c};
   c->mask_on = false;
+  // The true residual is evaluated in FP64 until an evaluation would declare some
+  // component converged, or shows a cycle that left it nearly unchanged; that evaluation
+  // and all later ones are repeated in double-double, so that convergence is never
+  // decided on the rounding noise of the FP64 evaluation.
+  constexpr double DD_STALL = 0.5;
+  bool use_dd = c->residual_dd == 2;
+  std::vector<double> rprev(nc, INFINITY);
+  // FP32 refinement: once a cycle leaves a component's (accurately evaluated) residual
+  // nearly unchanged, the FP32 arithmetic no longer resolves the modes that remain (its
+  // rounding of A z is about eps_32 cond(A) |r| on them); the remaining cycles then run in
+  // FP64, left-preconditioned, with the basis that fits in the FP32 workspace
+  const int m64 = inner == DGMI_F64
+                      ? m
+                      : std::min(m, (int)(k->vbytes / (sizeof(double) * (size_t)k->ldv)) - 1);
+  bool tail64 = false;
+  std::vector<int> cyc64(nc, 0);
   for (;;) {
-    // FP64 true residual of every component (a component that converged inside the last
-    // cycle still needs its new residual)
+    // true residual of every component (a component that converged inside the last cycle
+    // still needs its new residual)
     c->mask_on = false;
-    DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r64, s));
+    if (use_dd) DGMI_TRY(launch_residual_dd(c, x, b, r64, s));
+    else DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r64, s));
     DGMI_TRY(kry::sdot<double>(sg, r64, r64, nullptr, k->rnorm, s));
     DGMI_HIP(hipMemcpyAsync(k->hd, k->rnorm, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
     DGMI_HIP(hipStreamSynchronize(s));
+    if (!use_dd && c->residual_dd == 1) {
+      bool near = false;
+      for (int q = 0; q < nc; ++q) {
+        const double rq = k->hd[q];
+        if (done[q] || !std::isfinite(rq)) continue;
+        near = near || rq <= rtol * bnorm[q] || rq > DD_STALL * rprev[q];
+      }
+      if (near) {
+        use_dd = true;
+        continue;  // the same residual again, in double-double
+      }
+    }
     int nwant = 0;
     for (int q = 0; q < nc; ++q) {
       if (done[q]) {  // finished earlier: its residual was not recomputed, keep the final one
@@ -579,15 +608,24 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
         continue;
       }
       const double tol = rtol * bnorm[q];
+      const bool stalled = outer[q] > 0 && rnorm[q] > DD_STALL * rprev[q];
+      rprev[q] = rnorm[q];
       const bool want = !(rnorm[q] <= tol || its[q] >= max_its || bnorm[q] == 0.0);
       done[q] = !want;
       k->hi[q] = want ? 1 : 0;
       nwant += want;
-      double e = want ? std::min(0.5, 0.5 * tol / rnorm[q]) : 0.5;
-      if (inner != DGMI_F64) e = std::max(e, 1e-6);  // FP32 cycles stop near fp32 accuracy
-      eps[q] = e;
+      if (want && stalled && inner != DGMI_F64 && c->fp64_tail && m64 >= 2) tail64 = true;
     }
     if (nwant == 0) break;
+    for (int q = 0; q < nc; ++q) {
+      if (!k->hi[q]) {
+        eps[q] = 0.5;
+        continue;
+      }
+      double e = std::min(0.5, 0.5 * rtol * bnorm[q] / rnorm[q]);
+      if (inner != DGMI_F64 && !tail64) e = std::max(e, 1e-6);  // FP32 stops near its accuracy
+      eps[q] = e;
+    }
     std::memcpy(k->hd + nc, eps.data(), sizeof(double) * nc);
     DGMI_HIP(hipMemcpyAsync(k->want, k->hi, sizeof(int32_t) * nc, hipMemcpyHostToDevice, s));
     DGMI_HIP(hipMemcpyAsync(k->eps, k->hd + nc, sizeof(double) * nc, hipMemcpyHostToDevice, s));
@@ -595,11 +633,11 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
       c->mask_on = true;
       DGMI_TRY(kry::expand_active(sg, c->nele, k->want, c->act64, s));
     }
-    if (inner == DGMI_F64) {
+    if (inner == DGMI_F64 || tail64) {
       // the cycle uses scratch slots 1..3; the correction has its own slot
       double* dv = corr_buf<double>(c);
       DGMI_CHECK(dv, alloc_failure());
-      DGMI_TRY(gmres_cycle<double>(c, k, r64, dv, m, false, s));
+      DGMI_TRY(gmres_cycle<double>(c, k, r64, dv, m64, false, s));
       DGMI_TRY(kry::sadd<double>(sg, nullptr, dv, x, s));
     } else {  // FP32 cycle (bf16-input preconditioner applies when inner == DGMI_BF16)
       float* r32 = vbuf<float>(c, 0);
@@ -616,6 +654,7 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
       if (k->hi[q]) {
         its[q] += k->hi[nc + q];
         ++outer[q];
+        cyc64[q] += inner == DGMI_F64 || tail64;
       }
     ++cycles;
   }
@@ -624,7 +663,7 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
       stats[4 * q + 0] = its[q];
       stats[4 * q + 1] = failed[q] ? NAN : (bnorm[q] > 0 ? rnorm[q] / bnorm[q] : 0.0);
       stats[4 * q + 2] = outer[q];
-      stats[4 * q + 3] = outer[q];
+      stats[4 * q + 3] = cyc64[q];
     }
   return 0;
 }
@@ -777,9 +816,10 @@ static int cg_solve(dgmi_ctx* c, const double* b, double* x, double rtol, double
     }
     if (stop) break;
   }
-  // true residual of every component
+  // true residual of every component (double-double unless the context keeps FP64)
   c->mask_on = false;
-  DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r, s));
+  if (c->residual_dd != 0) DGMI_TRY(launch_residual_dd(c, x, b, r, s));
+  else DGMI_TRY(launch_apply<double>(c, c->g64, x, b, r, s));
   DGMI_TRY(kry::sdot<double>(sg, r, r, w->rr, nullptr, s));
   DGMI_HIP(hipMemcpyAsync(w->hd, w->rr, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
   DGMI_HIP(hipMemcpyAsync(w->hi, w->its, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, s));
@@ -1107,6 +1147,11 @@ int dgmi_apply(dgmi_ctx* c, int dtype, const void* x, const void* b, void* r, vo
                            (float*)r, b ? 1 : 0, s);
   set_error("dtype not supported by dgmi_apply");
   return -2;
+}
+
+int dgmi_residual_dd(dgmi_ctx* c, const double* x, const double* b, double* r, void* stream) {
+  DGMI_CHECK(c && x && b && r, "null argument");
+  return launch_residual_dd(c, x, b, r, (hipStream_t)stream);
 }
 
 int dgmi_rhs(dgmi_ctx* c, const double* f, const double* u_bc, const double* u_neumann,
@@ -1727,6 +1772,17 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "graph_vcycle") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "graph_vcycle must be 0 or 1");
     c->mg.use_graph = value != 0.0;
+    return 0;
+  }
+  if (k == "fp64_tail") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "fp64_tail must be 0 or 1");
+    c->fp64_tail = (int)value;
+    return 0;
+  }
+  if (k == "residual_dd") {
+    DGMI_CHECK(value == 0.0 || value == 1.0 || value == 2.0,
+               "residual_dd must be 0 (FP64), 1 (double-double near convergence) or 2 (always)");
+    c->residual_dd = (int)value;
     return 0;
   }
   if (k == "inner_flexible") {

@@ -257,6 +257,12 @@ struct dgmi_ctx {
   bool pc_bf16 = false;  // FP32 preconditioner applies with bf16 inputs
   // FP32 refinement cycles: 1 flexible (right-preconditioned) GMRES, 0 left-preconditioned
   int inner_flexible = 1;
+  // the solvers' true residual b - A x: 0 FP64 always; 1 double-double once a component
+  // nears its threshold or stalls (FP64 rounding of A x can reach the threshold); 2 always
+  int residual_dd = 1;
+  // FP32/BF16 refinement: 1 = once a cycle leaves a component's residual nearly unchanged,
+  // the remaining cycles run in FP64; 0 = every cycle in the inner precision
+  int fp64_tail = 1;
   // independent solves packed in this context: component c owns elements
   // [comp_eoff[c], comp_eoff[c+1]) and vertices [comp_voff[c], comp_voff[c+1])
   int ncomp = 1;
@@ -300,6 +306,8 @@ template <typename T>
 int launch_apply(dgmi_ctx* c, DevGeom<T>& g, const T* x, const T* b, T* r, hipStream_t s);
 template <typename T>
 int launch_apply_pos(dgmi_ctx* c, DevGeom<T>& g, const T* x, T* r, hipStream_t s);
+// r = b - A x of the FP64 operator evaluated in double-double arithmetic, rounded to FP64
+int launch_residual_dd(dgmi_ctx* c, const double* x, const double* b, double* r, hipStream_t s);
 template <typename T>
 int launch_smooth(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r_out, hipStream_t s);
 int launch_extract_diag(dgmi_ctx* c, const double* B, double* d, hipStream_t s);

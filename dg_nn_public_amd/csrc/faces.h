@@ -31,11 +31,12 @@ template <int D> struct FaceDims {
 
 // Traces (u, du/dn) of face f at its SG points: xv = the element's row, xf = the values of
 // the face's nodes (fnodes order), c = J^-T n_f; store(g, u, dn) writes point g.
-template <typename T, int D, typename Store>
-__device__ __forceinline__ void face_traces(int f, const T* __restrict__ fc, const T* xv,
+// The tables (fc) may be held in another type TT than the arithmetic T.
+template <typename T, int D, typename TT, typename Store>
+__device__ __forceinline__ void face_traces(int f, const TT* __restrict__ fc, const T* xv,
                                             const T* xf, const T* c, Store store) {
   using F = FaceDims<D>;
-  const T* gk = fc + F::GK + f * F::NK * D * F::NL;
+  const TT* gk = fc + F::GK + f * F::NK * D * F::NL;
   T dk[F::NK];
 #pragma unroll 2
   for (int k = 0; k < F::NK; ++k) {
@@ -51,8 +52,8 @@ __device__ __forceinline__ void face_traces(int f, const T* __restrict__ fc, con
     for (int d = 0; d < D; ++d) s += c[d] * gd[d];
     dk[k] = s;
   }
-  const T* ph = fc + F::PHI + f * F::SG * F::NFN;
-  const T* lk = fc + F::LK;
+  const TT* ph = fc + F::PHI + f * F::SG * F::NFN;
+  const TT* lk = fc + F::LK;
 #pragma unroll
   for (int g = 0; g < F::SG; ++g) {
     T u = 0, dn = 0;
@@ -76,17 +77,20 @@ template <typename T, int D> struct FaceLift {
 #pragma unroll
     for (int k = 0; k < F::NK; ++k) Q[k] = 0;
   }
-  __device__ __forceinline__ void add(int f, int g, const T* __restrict__ fc, T qs, T qd) {
-    const T* ph = fc + F::PHI + (f * F::SG + g) * F::NFN;
-    const T* lk = fc + F::LK + g * F::NK;
+  template <typename TT>
+  __device__ __forceinline__ void add(int f, int g, const TT* __restrict__ fc, T qs, T qd) {
+    const TT* ph = fc + F::PHI + (f * F::SG + g) * F::NFN;
+    const TT* lk = fc + F::LK + g * F::NK;
 #pragma unroll
     for (int i = 0; i < F::NFN; ++i) accf[i] += ph[i] * qs;
 #pragma unroll
     for (int k = 0; k < F::NK; ++k) Q[k] += lk[k] * qd;
   }
   // acc[n] += sum_{k,d} c_d Q_k d phi_n / d xi_d (p_k), over all nodes
-  __device__ __forceinline__ void finish(int f, const T* __restrict__ fc, const T* c, T* acc) const {
-    const T* gk = fc + F::GK + f * F::NK * D * F::NL;
+  template <typename TT>
+  __device__ __forceinline__ void finish(int f, const TT* __restrict__ fc, const T* c,
+                                         T* acc) const {
+    const TT* gk = fc + F::GK + f * F::NK * D * F::NL;
     T W[F::NK][D];
 #pragma unroll
     for (int k = 0; k < F::NK; ++k)
@@ -96,7 +100,7 @@ template <typename T, int D> struct FaceLift {
     for (int k = 0; k < F::NK; ++k)
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const T* t = gk + (k * D + d) * F::NL;
+        const TT* t = gk + (k * D + d) * F::NL;
 #pragma unroll
         for (int n = 0; n < F::NL; ++n) acc[n] += t[n] * W[k][d];
       }

@@ -356,6 +356,22 @@ class DGDiffusion:
               "dgmi_apply")
         return out
 
+    def residual_dd(self, x, b, out=None):
+        """out = b - A x (FP64 vectors) evaluated in double-double arithmetic and rounded
+        once to FP64: the residual without the rounding noise of the FP64 evaluation,
+        which can reach 1e-10 ||b|| at strong diffusivity contrast on fine meshes."""
+        self._vec(x, "x", (torch.float64,))
+        self._vec(b, "b", (torch.float64,))
+        out = torch.empty_like(x) if out is None else self._vec(out, "out", (torch.float64,))
+        check(self.lib.dgmi_residual_dd(self.ctx, _ptr(x), _ptr(b), _ptr(out), _stream()),
+              "dgmi_residual_dd")
+        return out
+
+    def set_residual_dd(self, mode):
+        """True residuals of the solvers: 0 FP64, 1 double-double near convergence or on
+        stagnation (default), 2 always double-double."""
+        check(self.lib.dgmi_set_param(self.ctx, b"residual_dd", float(mode)), "dgmi_set_param")
+
     def get_residual_or_smooth(self, r0, x_i, x_rhs, do_smooth=False):
         """r0 += x_rhs - A x_i; with do_smooth also x_i += omega c, c from the block
         smoother applied to r0.  Both tensors are mutated and returned flattened
@@ -631,7 +647,8 @@ class DGDiffusion:
         check(self.lib.dgmi_gmres_solve(self.ctx, _INNER[inner], _ptr(b), _ptr(x_i), rtol, m,
                                         max_its, stats, _stream()), "dgmi_gmres_solve")
         per = [dict(its=int(stats[4 * q]), relres=stats[4 * q + 1], outer=int(stats[4 * q + 2]),
-                    cycles=int(stats[4 * q + 3])) for q in range(self.ncomp)]
+                    cycles=int(stats[4 * q + 2]), f64_cycles=int(stats[4 * q + 3]))
+               for q in range(self.ncomp)]
         self.last_stats = per[0] if self.ncomp == 1 else per
         return x_i, max(p["its"] for p in per)
 

@@ -93,6 +93,14 @@ int dgmi_element_tables(const dgmi_ctx* ctx, double* n, double* nlx, double* wei
  * z05/volume_mf_um_diff_varprop.py:347-436, 509-724). */
 int dgmi_apply(dgmi_ctx* ctx, int dtype, const void* x, const void* b, void* r, void* stream);
 
+/* r = b - A x of the FP64 operator (FP64 x, b, r; b required), every operation carried in
+ * double-double arithmetic (about 106 bits) and the result rounded once to FP64.  The
+ * accurate counterpart of dgmi_apply for the residuals that decide convergence: the FP64
+ * evaluation rounds at about eps |A||x|, which for strongly varying diffusivity on fine
+ * meshes reaches 1e-10 ||b||.  No counterpart in the reference, which evaluates the
+ * residual as get_residual_or_smooth does (z02/volume_mf_diff.py:403-483). */
+int dgmi_residual_dd(dgmi_ctx* ctx, const double* x, const double* b, double* r, void* stream);
+
 /* b = sum_e M_e (f_e + alpha_e / dt) + Dirichlet-face terms with data u_bc, scaled by the
  * owner's kappa, + Neumann-face terms int v g_N -- _get_rhs (z02/volume_mf_diff.py:153-400:
  * volume :195-239 with the transient term :230-238, Dirichlet faces :298-357, Neumann faces
@@ -161,8 +169,10 @@ int dgmi_precond(dgmi_ctx* ctx, int dtype, const void* b, void* x, void* stream)
  * the smoother (FP32 arithmetic and face traces, vector kernels).  The Krylov operator
  * stays FP32.  Stops when ||b - A x|| <= rtol ||b|| (FP64 true residual)
  * or after max_its inner iterations, per component.  Synchronises the stream.  stats
- * (may be NULL), 4 values per component: iterations, final ||r||/||b||, outer
- * refinement steps, restarts. */
+ * (may be NULL), 4 values per component: iterations, final ||r||/||b||, cycles (outer
+ * refinement steps), cycles run in FP64 arithmetic.  The true residual is evaluated as
+ * "residual_dd" selects; FP32/BF16 refinement finishes in FP64 cycles as "fp64_tail"
+ * selects (dgmi_set_param). */
 int dgmi_gmres_solve(dgmi_ctx* ctx, int inner_dtype, const double* b, double* x, double rtol,
                      int m, int max_its, double* stats, void* stream);
 
@@ -210,6 +220,17 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *   "gmres_orth"      0 = modified Gram-Schmidt, as the reference
  *                     (z02/volume_mf_diff.py:1813-1816); 1 = classical Gram-Schmidt with
  *                     one re-orthogonalisation (default).
+ *   "residual_dd"     true residuals of dgmi_gmres_solve and the final one of
+ *                     dgmi_cg_solve: 0 = FP64 (dgmi_apply); 1 = FP64 until an evaluation
+ *                     would declare a component converged or shows that the last cycle
+ *                     reduced its residual by less than half, then that evaluation and the
+ *                     later ones in double-double (dgmi_residual_dd) (default); 2 = always
+ *                     double-double.
+ *   "fp64_tail"       FP32/BF16 refinement: 1 = once a cycle reduces a component's true
+ *                     residual by less than half, the remaining cycles of the solve run in
+ *                     FP64 (left-preconditioned, with the basis that fits in the FP32
+ *                     workspace: 14 vectors for m = 30) (default); 0 = every cycle in the
+ *                     inner precision.
  *   "inner_flexible"  FP32/BF16 refinement cycles: 1 = flexible (right-preconditioned)
  *                     GMRES (default), 0 = left-preconditioned like the FP64 solve.
  *   "f32_kernel"      kernels of FP32 applies: 0 = vector FMA (default), 1 = MFMA fp32,

@@ -248,8 +248,9 @@ def apply_A(sp: Space, x: np.ndarray) -> np.ndarray:
     dn_nb = np.matmul(Xn, c["dnv_nb"])[:, 0]
     w = sdw[zi]
     jump = u_own - u_nb
-    q_sn = w * (-0.5 * (dn_own + dn_nb) + gam[zi][:, None] * jump) * c["kap_i"][:, None]
-    q_dn = w * (-0.5 * jump) * c["kap_i"][:, None]
+    # halves written as divisions: the same FP64 values, and exact for Fraction arrays
+    q_sn = w * (-(dn_own + dn_nb) / 2 + gam[zi][:, None] * jump) * c["kap_i"][:, None]
+    q_dn = w * (-jump / 2) * c["kap_i"][:, None]
     contrib = np.matmul(sn_i, q_sn[:, :, None])[:, :, 0] + np.matmul(dnv[zi], q_dn[:, :, None])[:, :, 0]
     np.add.at(out, E[zi], contrib)
     # Dirichlet faces
@@ -265,6 +266,48 @@ def apply_A(sp: Space, x: np.ndarray) -> np.ndarray:
         contrib = np.matmul(sn_b, q_sn[:, :, None])[:, :, 0] + np.matmul(dnv[zb], q_dn[:, :, None])[:, :, 0]
         np.add.at(out, E[zb], contrib)
     return out.reshape(-1)
+
+
+def with_dtype(sp: Space, dtype=np.longdouble) -> Space:
+    """A copy of ``sp`` whose operator arrays hold the same FP64 coefficients as ``dtype``
+    (``np.longdouble``: the x87 extended format on x86-64, 64-bit significand; ``object``:
+    exact ``fractions.Fraction`` values), so that ``apply_A`` evaluates the same operator
+    with less (or no) rounding.  The FP64 evaluation of b - A x rounds at about
+    eps |A||x|, which reaches 1e-10 ||b|| for strongly varying diffusivity on fine meshes;
+    not a step of the reference, which evaluates in FP64 (z02/volume_mf_diff.py:403-483)."""
+    import copy
+    from fractions import Fraction
+
+    def conv(a):
+        a = np.asarray(a)
+        if a.dtype != np.float64:
+            return a
+        if dtype is object:
+            return np.array([Fraction(float(v)) for v in a.ravel()],
+                            dtype=object).reshape(a.shape)
+        return a.astype(dtype)
+
+    fc = _face_cache(sp)
+    out = copy.copy(sp)
+    for k in ("kappa", "nx", "detwei", "snx", "snormal", "sdetwei", "gamma"):
+        setattr(out, k, conv(getattr(sp, k)))
+    out._fc = {k: conv(v) for k, v in fc.items()}
+    out.el = copy.copy(sp.el)
+    out.el.sn = conv(sp.el.sn)
+    return out
+
+
+def residual_extended(sp: Space, b, x, dtype=np.longdouble):
+    """b - A x with the FP64 operator of ``sp`` evaluated in ``dtype`` (with_dtype); b and x
+    are taken exactly.  Returns the residual in ``dtype``."""
+    se = with_dtype(sp, dtype)
+    if dtype is object:
+        from fractions import Fraction
+        xe = np.array([Fraction(float(v)) for v in x], dtype=object)
+        be = np.array([Fraction(float(v)) for v in b], dtype=object)
+    else:
+        xe, be = np.asarray(x).astype(dtype), np.asarray(b).astype(dtype)
+    return be - apply_A(se, xe)
 
 
 def get_residual_or_smooth(sp: Space, r0, x_i, x_rhs, do_smooth=False, bdiag=None,

@@ -199,3 +199,18 @@ def test_solve_from_a_p3_gmsh_file(tmp_path):
     print("file solve vs oracle", rel(x, x_ref), "l2 err", dg.get_l2_error(sp, x, u)[0])
     assert rel(x, x_ref) < 1e-9
     assert dg.get_l2_error(sp, x, u)[0] < 1e-4
+
+
+@pytest.mark.parametrize("D,n", [(2, 6), (3, 4)])
+def test_double_double_residual_on_every_face_type(D, n):
+    """dgmi_residual_dd evaluates the same operator as the FP64 apply on a mesh with
+    interior, Dirichlet and Neumann faces (Neumann faces of either local index)."""
+    print(f"[case] residual_dd D={D} n={n}", flush=True)
+    m = meshgen.square_mesh(n) if D == 2 else meshgen.cube_mesh(n, jitter=0.0)
+    kap = np.exp(np.random.default_rng(7).standard_normal(m.nele))
+    g = DGDiffusion.from_mesh(m, kap, bctype=_top_neumann(m), device=DEV)
+    rng = np.random.default_rng(8)
+    x, b = to_dev(rng.standard_normal(g.ndof)), to_dev(rng.standard_normal(g.ndof))
+    rdd, r64 = to_np(g.residual_dd(x, b)), to_np(g.apply(x, b))
+    assert np.all(np.isfinite(rdd)) and rel(rdd, r64) < 1e-14
+    g.close()

@@ -155,6 +155,37 @@ def test_graph_vcycle_replays_equal_eager(prob):
     g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", 0.0)
 
 
+@pytest.mark.parametrize("packed", [False, True], ids=["single", "batch"])
+def test_fused_first_sweep_equals_unfused(monkeypatch, packed):
+    """With grid-sized SFC levels the eagerly launched V-cycle lets the restriction take the
+    first level-0 sweep from zero; the graph-replayed cycle launches that sweep on its own.
+    The two orders of the same arithmetic agree to rounding in FP64 and FP32, on one
+    context and on a packed batch of three components (padded level-0 layout)."""
+    monkeypatch.setenv("DGMI_SMALL_LEVEL_N", "8")
+    from dg_nn_public_amd import meshgen
+    from dg_nn_public_amd.batch import DGBatch
+    ms = [meshgen.cube_mesh(n, seed=n) for n in ((4,) if not packed else (3, 4, 2))]
+    ks = [meshgen.sine_problem(m, sigma=1.0, kappa_seed=5 + i).kappa for i, m in enumerate(ms)]
+    print(f"[case] fused vs unfused first sweep, {len(ms)} component(s)", flush=True)
+    bt = DGBatch(ms, ks, graph_vcycle=False)
+    g = bt.space
+    rng = np.random.default_rng(40)
+    try:
+        for dt, coarse_exact, tol in ((torch.float64, 1.0, 1e-14), (torch.float32, 0.0, 2e-6)):
+            g.lib.dgmi_set_param(g.ctx, b"coarse_exact", coarse_exact)
+            b = to_dev(rng.standard_normal(g.ndof), dt)
+            out = {}
+            for mode in (0.0, 1.0):
+                g.lib.dgmi_set_param(g.ctx, b"graph_vcycle", mode)
+                out[mode] = to_np(g._um_left_precond(g.new_vector(dt), b.clone()))
+            err = rel(out[0.0], out[1.0])
+            print(dt, "fused vs unfused rel diff", err, flush=True)
+            assert np.all(np.isfinite(out[0.0])) and err < tol
+    finally:
+        g.lib.dgmi_set_param(g.ctx, b"coarse_exact", 1.0)
+        bt.close()
+
+
 def test_vector_sweeps_with_carried_traces_equal_matrix_core_sweeps():
     """On a mesh of many work-groups, the FP32 preconditioner whose vector sweeps carry the
     face traces (zero sweep, fused prolongation, neighbour-form sweeps) agrees with the

@@ -247,3 +247,31 @@ def test_reference_mean_form_is_indefinite_at_sigma_one():
         lo[mode] = spla.eigsh(op, k=1, which="SA", tol=1e-6, maxiter=20000,
                               v0=np.ones(sp.ndof))[0][0]
     assert lo["mean"] < 0 < lo["harmonic"]
+
+
+# -------------------------------------------------------------- extended-precision residual
+@pytest.mark.parametrize("D", [2, 3], ids=["2d", "3d"])
+def test_extended_residual_equals_exact_rational_evaluation(D):
+    """residual_extended evaluates the same FP64 operator with a 64-bit significand: on a
+    residual in the cancellation regime (b = A x rounded, so b - A x is of the size of one
+    rounding of A x) it agrees with the exact evaluation in rational arithmetic to 1e-18 of
+    max |A x|, while the FP64 evaluation is off by about its own size."""
+    assert np.finfo(np.longdouble).eps < 1e-18  # x87 extended format
+    m = _mesh(D, 2, seed=3)
+    p = meshgen.sine_problem(m, sigma=2.0, kappa_seed=4)
+    sp = dg.make_space(m, p.kappa, face_kappa="harmonic")
+    x = np.random.default_rng(1).standard_normal(sp.ndof)
+    ax = dg.apply_A(dg.with_dtype(sp), x.astype(np.longdouble))
+    b = ax.astype(np.float64)
+    exact = np.array([float(v) for v in dg.residual_extended(sp, b, x, dtype=object)])
+    ext = dg.residual_extended(sp, b, x).astype(np.float64)
+    r64 = b - dg.apply_A(sp, x)
+    scale = np.abs(ax).max()
+    err_ext = float(np.abs(ext - exact).max() / scale)
+    err_64 = float(np.abs(r64 - exact).max() / scale)
+    print("exact |r|", np.abs(exact).max() / scale, "extended err", err_ext, "FP64 err", err_64)
+    assert err_ext <= 1e-18 and err_64 >= 1e-17
+    # the same operator: on an ordinary residual the three agree to FP64 rounding
+    b2 = np.random.default_rng(2).standard_normal(sp.ndof)
+    r_ext = dg.residual_extended(sp, b2, x).astype(np.float64)
+    assert np.linalg.norm(r_ext - (b2 - dg.apply_A(sp, x))) <= 1e-14 * np.linalg.norm(r_ext)
