@@ -48,6 +48,7 @@ SIGNATURES = {
     "dgmi_element_tables": (C.c_int, [_P, _DP, _DP, _DP, _DP, _DP, _DP, _I64P]),
     "dgmi_apply": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "dgmi_residual_dd": (C.c_int, [_P, _P, _P, _P, _P]),
+    "dgmi_release_cached_memory": (C.c_int, []),
     "dgmi_rhs": (C.c_int, [_P, _P, _P, _P, _P, C.c_double, _P, _P]),
     "dgmi_l2_error": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "dgmi_bdiag": (C.c_int, [_P, _P, _P, _P]),

@@ -98,5 +98,7 @@ class DGBatch:
         """Per-component views of a packed DOF vector."""
         return [v[self.dof_off[c]:self.dof_off[c + 1]] for c in range(self.ncomp)]
 
-    def close(self):
-        self.space.close()
+    def close(self, synced=False):
+        """Give the device memory back (``synced``: the caller's queued work has finished,
+        see ``DGDiffusion.close``)."""
+        self.space.close(synced=synced)

@@ -7,7 +7,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <list>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <unordered_map>
 
 #include "dgmi.h"
 #include "dgmi_internal.h"
@@ -41,6 +44,133 @@ std::string alloc_failure(const char* what) {
     int rc_ = (call);             \
     if (rc_ != 0) return rc_;     \
   } while (0)
+
+// ---- device memory cache (dev_alloc / dev_free)
+namespace {
+struct DevCache {
+  std::mutex mu;
+  std::unordered_map<void*, std::pair<int, size_t>> live;  // block -> (device, bytes)
+  std::map<int, std::multimap<size_t, void*>> idle;        // device -> bytes -> block
+};
+DevCache& dev_cache() {
+  static DevCache* c = new DevCache{};  // never destroyed: blocks may outlive static teardown
+  return *c;
+}
+size_t block_bytes(size_t b) {
+  b = std::max<size_t>(b, 1);
+  constexpr size_t SMALL = 512, LARGE = size_t(2) << 20;
+  return b <= (size_t(1) << 20) ? (b + SMALL - 1) / SMALL * SMALL : (b + LARGE - 1) / LARGE * LARGE;
+}
+// frees the idle blocks of one device (dev < 0: of every device)
+void release_idle(int dev) {
+  std::vector<std::pair<int, void*>> blocks;
+  {
+    DevCache& C = dev_cache();
+    std::lock_guard<std::mutex> lk(C.mu);
+    for (auto& kv : C.idle) {
+      if (dev >= 0 && kv.first != dev) continue;
+      for (auto& b : kv.second) blocks.emplace_back(kv.first, b.second);
+      kv.second.clear();
+    }
+  }
+  if (blocks.empty()) return;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto& b : blocks) {
+    (void)hipSetDevice(b.first);
+    (void)hipFree(b.second);
+  }
+  (void)hipSetDevice(cur);
+}
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+  const size_t rb = block_bytes(bytes);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  DevCache& C = dev_cache();
+  {
+    std::lock_guard<std::mutex> lk(C.mu);
+    auto& idle = C.idle[dev];
+    auto it = idle.lower_bound(rb);
+    if (it != idle.end() && it->first <= rb + rb / 4) {
+      void* p = it->second;
+      C.live[p] = {dev, it->first};
+      idle.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, rb);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    release_idle(dev);
+    e = hipMalloc(&p, rb);
+  }
+  if (e != hipSuccess || !p) {
+    g_alloc_err = "hipMalloc(" + std::to_string(rb) + " bytes): " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(C.mu);
+  C.live[p] = {dev, rb};
+  return p;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  DevCache& C = dev_cache();
+  std::lock_guard<std::mutex> lk(C.mu);
+  auto it = C.live.find(p);
+  if (it == C.live.end()) return;  // not a block of this cache
+  C.idle[it->second.first].emplace(it->second.second, p);
+  C.live.erase(it);
+}
+
+// Pinned host buffers of the solver drivers (progress records, per-component scalars):
+// kept for reuse like the device blocks, since hipHostFree also waits for the device.
+namespace {
+DevCache& pinned_cache() {
+  static DevCache* c = new DevCache{};
+  return *c;
+}
+}  // namespace
+
+static hipError_t pinned_alloc(void** out, size_t bytes) {
+  const size_t rb = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  DevCache& C = pinned_cache();
+  {
+    std::lock_guard<std::mutex> lk(C.mu);
+    auto& idle = C.idle[dev];
+    auto it = idle.find(rb);
+    if (it != idle.end()) {
+      *out = it->second;
+      C.live[*out] = {dev, rb};
+      idle.erase(it);
+      return hipSuccess;
+    }
+  }
+  const hipError_t e = hipHostMalloc(out, rb);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return e;
+  }
+  std::lock_guard<std::mutex> lk(C.mu);
+  C.live[*out] = {dev, rb};
+  return hipSuccess;
+}
+
+static void pinned_free(void* p) {
+  if (!p) return;
+  DevCache& C = pinned_cache();
+  std::lock_guard<std::mutex> lk(C.mu);
+  auto it = C.live.find(p);
+  if (it == C.live.end()) return;
+  C.idle[it->second.first].emplace(it->second.second, p);
+  C.live.erase(it);
+}
 
 template <typename T, typename U>
 static T* upload(dgmi_ctx* c, const std::vector<U>& h) {
@@ -195,12 +325,8 @@ void* dgmi_ctx::alloc(size_t bytes) {
                   " bytes) would be exceeded by " + std::to_string(bytes) + " bytes";
     return nullptr;
   }
-  void* p = nullptr;
-  const hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess || !p) {
-    g_alloc_err = "hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e);
-    return nullptr;
-  }
+  void* p = dev_alloc(bytes);
+  if (!p) return nullptr;
   allocs.push_back(p);
   dev_bytes += bytes;
   return p;
@@ -356,10 +482,10 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
     k->active = iv(nc);
     k->kk = iv(nc);
     k->nact = iv(1);
-    if (k->hd) (void)hipHostFree(k->hd);
-    if (k->hi) (void)hipHostFree(k->hi);
-    if (hipHostMalloc((void**)&k->hd, sizeof(double) * 2 * nc) != hipSuccess) k->hd = nullptr;
-    if (hipHostMalloc((void**)&k->hi, sizeof(int32_t) * (2 * nc + 1)) != hipSuccess) k->hi = nullptr;
+    if (k->hd) pinned_free(k->hd);
+    if (k->hi) pinned_free(k->hi);
+    if (pinned_alloc((void**)&k->hd, sizeof(double) * 2 * nc) != hipSuccess) k->hd = nullptr;
+    if (pinned_alloc((void**)&k->hi, sizeof(int32_t) * (2 * nc + 1)) != hipSuccess) k->hi = nullptr;
     if (!k->beta || !k->hn || !k->res || !k->thr || !k->eps || !k->rnorm || !k->bnorm ||
         !k->want || !k->active || !k->kk || !k->nact || !k->hd || !k->hi)
       return nullptr;
@@ -389,9 +515,9 @@ static Krylov* krylov(dgmi_ctx* c, int m, size_t elem, bool flexible) {
       k->vbytes = 0;
       return nullptr;
     }
-    if (k->ring) (void)hipHostFree(k->ring);
+    if (k->ring) pinned_free(k->ring);
     k->ring = nullptr;
-    bool ok = hipHostMalloc((void**)&k->ring, sizeof(double) * (m + 1) * (nc + 1)) == hipSuccess;
+    bool ok = pinned_alloc((void**)&k->ring, sizeof(double) * (m + 1) * (nc + 1)) == hipSuccess;
     if (!ok) k->ring = nullptr;
     ok = ok && hipHostGetDevicePointer((void**)&k->ring_dev, k->ring, 0) == hipSuccess;
     while (ok && (int)k->ev.size() < m + 1) {
@@ -684,9 +810,9 @@ struct CgWork {
 };
 static void free_cg(CgWork* w) {
   if (!w) return;
-  if (w->hd) (void)hipHostFree(w->hd);
-  if (w->hi) (void)hipHostFree(w->hi);
-  if (w->ring) (void)hipHostFree(w->ring);
+  if (w->hd) pinned_free(w->hd);
+  if (w->hi) pinned_free(w->hi);
+  if (w->ring) pinned_free(w->ring);
   for (hipEvent_t e : w->ev)
     if (e) (void)hipEventDestroy(e);
   delete w;
@@ -711,9 +837,9 @@ static CgWork* cg_work(dgmi_ctx* c) {
   w->want = iv(nc);
   bool ok = w->rz && w->rznew && w->pap && w->rr && w->thr && w->tol && w->bn && w->its &&
             w->active && w->want;
-  ok = ok && hipHostMalloc((void**)&w->hd, sizeof(double) * nc) == hipSuccess;
-  ok = ok && hipHostMalloc((void**)&w->hi, sizeof(int32_t) * 2 * nc) == hipSuccess;
-  ok = ok && hipHostMalloc((void**)&w->ring, sizeof(double) * CgWork::NRING) == hipSuccess;
+  ok = ok && pinned_alloc((void**)&w->hd, sizeof(double) * nc) == hipSuccess;
+  ok = ok && pinned_alloc((void**)&w->hi, sizeof(int32_t) * 2 * nc) == hipSuccess;
+  ok = ok && pinned_alloc((void**)&w->ring, sizeof(double) * CgWork::NRING) == hipSuccess;
   ok = ok && hipHostGetDevicePointer((void**)&w->ring_dev, w->ring, 0) == hipSuccess;
   for (int i = 0; ok && i < CgWork::NRING; ++i)
     ok = hipEventCreateWithFlags(&w->ev[i], hipEventDisableTiming) == hipSuccess;
@@ -1057,17 +1183,17 @@ int dgmi_ctx_create_batch_parts(const dgmi_mesh_desc* d, int32_t ncomp, const in
 
 int dgmi_ctx_destroy(dgmi_ctx* c) {
   if (!c) return 0;
-  (void)hipDeviceSynchronize();
+  if (c->destroy_sync) (void)hipDeviceSynchronize();
   if (c->mg.graph64.exec) (void)hipGraphExecDestroy(c->mg.graph64.exec);
   if (c->mg.graph32.exec) (void)hipGraphExecDestroy(c->mg.graph32.exec);
   if (c->mg.capture_stream) (void)hipStreamDestroy(c->mg.capture_stream);
-  for (void* p : c->allocs) (void)hipFree(p);
+  for (void* p : c->allocs) dev_free(p);
   delete static_cast<Scratch*>(c->scratch);
   free_cg(static_cast<CgWork*>(c->cgwork));
   if (Krylov* k = static_cast<Krylov*>(c->krylov)) {
-    if (k->hd) (void)hipHostFree(k->hd);
-    if (k->hi) (void)hipHostFree(k->hi);
-    if (k->ring) (void)hipHostFree(k->ring);
+    if (k->hd) pinned_free(k->hd);
+    if (k->hi) pinned_free(k->hi);
+    if (k->ring) pinned_free(k->ring);
     for (hipEvent_t e : k->ev) (void)hipEventDestroy(e);
     delete k;
   }
@@ -1147,6 +1273,21 @@ int dgmi_apply(dgmi_ctx* c, int dtype, const void* x, const void* b, void* r, vo
                            (float*)r, b ? 1 : 0, s);
   set_error("dtype not supported by dgmi_apply");
   return -2;
+}
+
+int dgmi_release_cached_memory(void) {
+  release_idle(-1);
+  std::vector<void*> hosts;
+  {
+    DevCache& C = pinned_cache();
+    std::lock_guard<std::mutex> lk(C.mu);
+    for (auto& kv : C.idle) {
+      for (auto& b : kv.second) hosts.push_back(b.second);
+      kv.second.clear();
+    }
+  }
+  for (void* p : hosts) (void)hipHostFree(p);
+  return 0;
 }
 
 int dgmi_residual_dd(dgmi_ctx* c, const double* x, const double* b, double* r, void* stream) {
@@ -1252,13 +1393,14 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
     set_error(alloc_failure("smoother work buffer: the context's device-byte limit"));
     return -2;
   }
-  DGMI_HIP(hipMalloc(&full, sizeof(double) * nS));
+  full = static_cast<double*>(dev_alloc(sizeof(double) * nS));
+  DGMI_CHECK(full, alloc_failure("smoother work buffer"));
   struct FreeFull {
     double* p;
     hipStream_t s;
     ~FreeFull() {
       (void)hipStreamSynchronize(s);
-      (void)hipFree(p);
+      dev_free(p);
     }
   } free_full{full, s};
   if (variant == DGMI_SMOOTH_JACOBI3) {
@@ -1772,6 +1914,11 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "graph_vcycle") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "graph_vcycle must be 0 or 1");
     c->mg.use_graph = value != 0.0;
+    return 0;
+  }
+  if (k == "destroy_sync") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "destroy_sync must be 0 or 1");
+    c->destroy_sync = (int)value;
     return 0;
   }
   if (k == "fp64_tail") {

@@ -260,6 +260,9 @@ struct dgmi_ctx {
   // the solvers' true residual b - A x: 0 FP64 always; 1 double-double once a component
   // nears its threshold or stalls (FP64 rounding of A x can reach the threshold); 2 always
   int residual_dd = 1;
+  // dgmi_ctx_destroy: 1 = synchronise the device before giving the memory back; 0 = the
+  // caller has finished all work it queued for this context
+  int destroy_sync = 1;
   // FP32/BF16 refinement: 1 = once a cycle leaves a component's residual nearly unchanged,
   // the remaining cycles run in FP64; 0 = every cycle in the inner precision
   int fp64_tail = 1;
@@ -289,6 +292,14 @@ struct dgmi_ctx {
 };
 
 namespace dgmi {
+// Device memory of contexts and set-up work buffers.  A block given back with dev_free is
+// kept in a per-device cache and handed out again by a later dev_alloc of about its size
+// (within 1/4): hipFree waits for the whole device to go idle, which in a serving process
+// means for every other job's queued work.  A block must not be given back while queued
+// work still uses it.  When hipMalloc fails, the cached blocks of the device are freed and
+// the allocation is tried once more; dgmi_release_cached_memory frees them on request.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
 // set-up phases timed into dgmi_ctx::t_setup
 enum SetupPhase {
   PH_TOPOLOGY = 0, PH_GEOMETRY, PH_UPLOAD, PH_SMOOTHER, PH_GRAPH, PH_PROBE, PH_LEVELS, PH_LAYOUT

@@ -176,7 +176,8 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(
 
 template <typename T>
 static int to_dev(const std::vector<T>& h, T** d, hipStream_t s) {
-  DGMI_HIP(hipMalloc(d, sizeof(T) * std::max<size_t>(h.size(), 1)));
+  *d = static_cast<T*>(dev_alloc(sizeof(T) * std::max<size_t>(h.size(), 1)));
+  if (!*d) return -3;
   if (!h.empty()) DGMI_HIP(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
   return 0;
 }
@@ -219,9 +220,10 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
   int* dbad = nullptr;
   int rc = 0;
   auto free_all = [&]() {
+    (void)hipStreamSynchronize(s);  // nothing in flight reads the work buffers any more
     for (void* p : {(void*)dx, (void*)dk, (void*)dmax, (void*)dnb, (void*)dbt, (void*)dal,
                     (void*)dbad})
-      if (p) (void)hipFree(p);
+      dev_free(p);
   };
   auto step = [&](int r) {
     if (r && !rc) rc = r;
@@ -238,11 +240,14 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
     }
     return 0;
   };
-  if (step(hipMalloc(&dx, sizeof(double) * nx) == hipSuccess ? 0 : -3) && step(copy_parts()) &&
+  auto take = [&](auto** d, size_t bytes) {
+    *d = static_cast<std::remove_pointer_t<decltype(d)>>(dev_alloc(bytes));
+    return *d ? 0 : -3;
+  };
+  if (step(take(&dx, sizeof(double) * nx)) && step(copy_parts()) &&
       step(to_dev(c->kappa, &dk, s)) && step(to_dev(nb32, &dnb, s)) && step(to_dev(bt32, &dbt, s)) &&
-      step(to_dev(al32, &dal, s)) &&
-      step(hipMalloc(&dmax, sizeof(double) * nb) == hipSuccess ? 0 : -3) &&
-      step(hipMalloc(&dbad, sizeof(int)) == hipSuccess ? 0 : -3)) {
+      step(to_dev(al32, &dal, s)) && step(take(&dmax, sizeof(double) * nb)) &&
+      step(take(&dbad, sizeof(int)))) {
     const int big = INT_MAX;
     if (step(hipMemcpyAsync(dbad, &big, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess ? 0 : -3)) {
       if (D == 3)

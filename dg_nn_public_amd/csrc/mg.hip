@@ -1116,12 +1116,15 @@ static int invert_levels(const std::vector<LevelInverse>& jobs, std::vector<int>
     }
     return rc == 0;
   };
-  if (chk(hipMalloc(&da, sizeof(double) * asz)) &&
-      chk(hipMalloc(&dp, sizeof(double) * GJB * GJB * nm)) &&
-      chk(hipMalloc(&dval, sizeof(double) * std::max<size_t>(nnz, 1))) &&
-      chk(hipMalloc(&dcol, sizeof(int32_t) * std::max<size_t>(nnz, 1))) &&
-      chk(hipMalloc(&drp, sizeof(int32_t) * nrp)) && chk(hipMalloc(&dfail, sizeof(int) * nm)) &&
-      chk(hipMalloc(&dm, sizeof(GjMat) * nm))) {
+  auto take = [&](auto** d, size_t bytes) {
+    *d = static_cast<std::remove_pointer_t<decltype(d)>>(dev_alloc(bytes));
+    return chk(*d ? hipSuccess : hipErrorOutOfMemory);
+  };
+  if (take(&da, sizeof(double) * asz) && take(&dp, sizeof(double) * GJB * GJB * nm) &&
+      take(&dval, sizeof(double) * std::max<size_t>(nnz, 1)) &&
+      take(&dcol, sizeof(int32_t) * std::max<size_t>(nnz, 1)) &&
+      take(&drp, sizeof(int32_t) * nrp) && take(&dfail, sizeof(int) * nm) &&
+      take(&dm, sizeof(GjMat) * nm)) {
     size_t ao = 0, vo = 0, ro = 0;
     for (int i = 0; i < nm && rc == 0; ++i) {
       const SfcLevelsHost& h = *jobs[i].h;
@@ -1168,7 +1171,7 @@ static int invert_levels(const std::vector<LevelInverse>& jobs, std::vector<int>
   (void)hipStreamSynchronize(s);  // nothing in flight reads the work space any more
   for (void* q : {(void*)da, (void*)dp, (void*)dval, (void*)dcol, (void*)drp, (void*)dfail,
                   (void*)dm})
-    if (q) (void)hipFree(q);
+    dev_free(q);
   return rc;
 }
 

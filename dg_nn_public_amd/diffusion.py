@@ -253,8 +253,13 @@ class DGDiffusion:
         """Build from a ``meshgen.Mesh``."""
         return cls(mesh.x_ref_in, mesh.cells, mesh.verts, kappa, **kw)
 
-    def close(self):
+    def close(self, synced=False):
+        """Give the context's device memory back to the library's block cache.  With
+        ``synced`` the caller has finished all work it queued for this context (e.g. it
+        synchronised its stream), and the device-wide synchronisation is skipped."""
         if getattr(self, "ctx", None):
+            if synced:
+                self.lib.dgmi_set_param(self.ctx, b"destroy_sync", 0.0)
             self.lib.dgmi_ctx_destroy(self.ctx)
             self.ctx = None
 
@@ -790,6 +795,11 @@ def _get_RAR_and_sfc_data_Um():
 
 def _um_left_precond(x_i, x_rhs):
     return _active()._um_left_precond(x_i, x_rhs)
+
+
+def release_cached_memory():
+    """Free the device blocks the library keeps for reuse after contexts are closed."""
+    check(_lib.load().dgmi_release_cached_memory(), "dgmi_release_cached_memory")
 
 
 def _gmres_mg_solver(x_i, x_rhs, tol):

@@ -109,6 +109,18 @@ def square_mesh(n: int, jitter: float = 0.2, seed: int = 0) -> Mesh:
     return Mesh(D=2, x_ref_in=_p3_nodes(verts, cells, TRI_P3), cells=cells, verts=verts)
 
 
+def _morton_order(idx: np.ndarray) -> np.ndarray:
+    """Order of integer grid cells (rows of idx) along the Z-order curve (bits of the
+    coordinates interleaved)."""
+    idx = np.asarray(idx, dtype=np.int64)
+    code = np.zeros(idx.shape[0], dtype=np.int64)
+    d = idx.shape[1]
+    for bit in range(21):
+        for a in range(d):
+            code |= ((idx[:, a] >> bit) & 1) << (bit * d + a)
+    return np.argsort(code, kind="stable")
+
+
 def cube_mesh(n: int, jitter: float = 0.15, seed: int = 0) -> Mesh:
     """n^3 cells of the unit cube, six Kuhn tetrahedra per cell (6 n^3 elements)."""
     verts = _jittered_grid(n, 3, jitter, seed)
@@ -125,7 +137,9 @@ def cube_mesh(n: int, jitter: float = 0.15, seed: int = 0) -> Mesh:
             step[ax] += 1
             p[:, s + 1] = base + step
         cells.append(p)
-    p = np.stack(cells, axis=1).reshape(-1, 4, 3)  # cell-major, 6 tets per cell
+    # cell-major, 6 tets per cell, cells in Morton (Z-curve) order, so that the face
+    # neighbours of an element lie close to it in memory in all three directions
+    p = np.stack(cells, axis=1)[_morton_order(base)].reshape(-1, 4, 3)
     ids = (p[..., 0] * m + p[..., 1]) * m + p[..., 2]
     cells = _orient(verts, ids.astype(np.int64))
     return Mesh(D=3, x_ref_in=_p3_nodes(verts, cells, TET_P3), cells=cells, verts=verts)

@@ -7,9 +7,14 @@ operator apply, smoothing step and V-cycle level of the job is a single launch o
 of its solves.  Up to ``max_jobs`` jobs run at once, each on its own stream and host
 thread (library calls release the GIL), so requests that arrive while others are on the
 GPU start as soon as a slot and memory free up and their work overlaps on the device.  A
-job's geometry, multigrid hierarchy and Krylov state stay device-resident until the job
-ends; its memory then returns to the budget.  Each request's latency runs from its
-arrival to the end of the job that solved it.
+request of at least ``solo_elements`` elements runs as a job of its own, so that small
+requests are never held back by a large one packed with them, and is admitted ahead of
+the small ones (the longest job first: it then overlaps them); packing stops at
+``job_elements`` elements, so that packed jobs stay short.  A job's geometry,
+multigrid hierarchy and Krylov state stay device-resident until the job ends; its slot
+frees when its results are published, its memory returns to the budget once the library
+has taken it back (into its block cache: no device-wide synchronisation).  Each request's
+latency runs from its arrival to the end of the job that solved it.
 
 Failures stay with the request that caused them:
 * a request with invalid data (non-finite or non-positive diffusivity, non-finite source
@@ -112,17 +117,21 @@ class _Job:
     admitted: float = 0.0
     thread: threading.Thread | None = None
     alloc_failed: bool = False
+    published: bool = False  # results handed out: the job no longer holds a slot
 
 
 class Server:
     """``hbm_budget`` bounds the estimated bytes of the running jobs (a request larger than
-    the budget still runs when nothing else does); ``hbm_capacity`` (default: the device's
-    free memory when ``run`` starts) rejects requests that could not run even alone."""
+    the budget still runs when nothing else does); ``hbm_capacity`` (default: 90% of the
+    device's memory) rejects requests that could not run even alone."""
 
     def __init__(self, hbm_budget=200e9, max_jobs=4, max_batch=64, inner=torch.float32,
                  rtol=1e-10, max_its=400, device="cuda:0", keep_solutions=False,
-                 hbm_capacity=None, job_byte_limit=0):
+                 hbm_capacity=None, job_byte_limit=0, solo_elements=250_000,
+                 job_elements=400_000):
         self.budget = float(hbm_budget)
+        self.solo_elements = int(solo_elements)
+        self.job_elements = int(job_elements)  # packing stops once a job holds this many
         self.capacity = hbm_capacity
         self.max_jobs = max_jobs
         self.max_batch = max_batch
@@ -174,9 +183,21 @@ class Server:
                                           job.jid, p.clone() if p is not None and err is None
                                           else None, converged=err is None and rel <= self.rtol,
                                           error=err, job_info=info))
+                    stream.synchronize()
+                    with self._cv:  # publish now: the slot frees before the memory goes back
+                        results.extend(out)
+                        job.published = True
+                        self._cv.notify_all()
                 finally:
                     if bt is not None:
-                        bt.close()
+                        try:
+                            stream.synchronize()
+                            synced = True
+                        except Exception:  # noqa: BLE001 - then the library synchronises
+                            synced = False
+                        bt.close(synced=synced)
+                        if out:  # when the job's device memory went back
+                            out[0].job_info["released"] = time.perf_counter() - t0
         except DgmiError as e:
             if "allocation failed" in str(e):
                 job.alloc_failed = True
@@ -194,7 +215,9 @@ class Server:
                 if job.alloc_failed and len(job.reqs) == 1:
                     job.alloc_failed = False
                     out = self._failed(job, t0, "device allocation failed for this request")
-                results.extend(out)
+                if not job.published:
+                    results.extend(out)
+                    job.published = not job.alloc_failed
                 self._cv.notify_all()
 
     def _failed(self, job, t0, msg):
@@ -209,8 +232,8 @@ class Server:
         waiting, jobs, results = [], [], []
         in_use = 0
         cap = self.capacity
-        if cap is None:
-            cap = float(torch.cuda.mem_get_info(torch.device(self.device))[0])
+        if cap is None:  # what one job could hold at most (the library caches freed blocks)
+            cap = 0.9 * float(torch.cuda.mem_get_info(torch.device(self.device))[1])
         t0 = time.perf_counter()
         jid = 0
         with self._cv:
@@ -237,12 +260,23 @@ class Server:
                             self._admit(part, jid, t0, jobs, results)
                             jid += 1
                             in_use += jobs[-1].bytes
-                # admit: oldest waiting request's dimension, FIFO, within the budget
-                while waiting and len(jobs) < self.max_jobs:
-                    D = waiting[0].mesh.D
-                    take, need = [], 0
-                    for r in waiting:
+                # admit: oldest waiting request's dimension, FIFO, within the budget; a
+                # large request runs alone.  Jobs that published their results hold memory
+                # until the library has it back, but no slot.
+                while waiting and sum(not j.published for j in jobs) < self.max_jobs:
+                    # the oldest large request first: it is the longest job, started early
+                    # it overlaps the small ones
+                    solo_first = [r for r in waiting
+                                  if r.mesh.cells.shape[0] >= self.solo_elements][:1]
+                    D = (solo_first or waiting)[0].mesh.D
+                    take, need, nel = [], 0, 0
+                    for r in solo_first or waiting:
                         if r.mesh.D != D or len(take) >= self.max_batch:
+                            continue
+                        if take and nel + r.mesh.cells.shape[0] > self.job_elements:
+                            continue
+                        solo = r.mesh.cells.shape[0] >= self.solo_elements
+                        if solo and take:
                             continue
                         e = self._estimate(r)
                         if in_use + need + e > self.budget and take:
@@ -251,6 +285,9 @@ class Server:
                             break  # wait for memory to free up
                         take.append(r)
                         need += e
+                        nel += r.mesh.cells.shape[0]
+                        if solo:
+                            break
                     if not take:
                         break
                     for r in take:

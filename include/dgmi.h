@@ -55,7 +55,15 @@ typedef struct {
 /* Context: topology (z02/mesh_init.py:260-658), reference-element tables
  * (z02/shape_function.py:459-1317), affine geometry, uploaded to the current device. */
 int dgmi_ctx_create(const dgmi_mesh_desc* desc, dgmi_ctx** out);
+/* Gives the context's device memory back to the library's per-device block cache, from
+ * which later contexts of about the same sizes are served without hipMalloc (hipFree waits
+ * for the whole device to go idle).  Synchronises the device first unless the parameter
+ * "destroy_sync" is 0, with which the caller promises that all work it queued for the
+ * context has finished. */
 int dgmi_ctx_destroy(dgmi_ctx* ctx);
+/* Frees the idle blocks of the block cache on every device (they are also freed, and the
+ * allocation retried, when a hipMalloc fails).  No counterpart in the reference. */
+int dgmi_release_cached_memory(void);
 /* A context holding ncomp independent solves ("components") packed into one mesh: the
  * union of their cells and vertices, component c owning elements [elem_off[c],
  * elem_off[c+1]) and vertices [vert_off[c], vert_off[c+1]).  Operator applies, smoothing
@@ -255,6 +263,9 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     start; 0 = sweep every level as the FP64 cycle (the reference's)
  *                     does.  dgmi_mg_info writes the level to info[4 + nlevel] when the
  *                     buffer holds it (-1: none).
+ *   "destroy_sync"    dgmi_ctx_destroy: 1 = synchronise the device first (default); 0 =
+ *                     the caller has finished all work it queued for the context (the
+ *                     serving layer, which synchronises its job's stream).
  *   "alloc_limit_bytes"  device bytes this context may hold (0 = no limit, the default):
  *                     a later allocation that would pass it fails, and the entry point
  *                     that needed it returns -2 ("device allocation failed") without

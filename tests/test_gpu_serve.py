@@ -141,3 +141,37 @@ def test_packed_solve_isolates_a_nonfinite_component(inner):
         assert st[c]["relres"] <= 1e-10 and st[c]["its"] == st_ok[c]["its"]
         assert rel(to_np(bt.split(x)[c]), to_np(bt.split(x_ok)[c])) < 1e-10
     bt.close()
+
+
+def test_closed_context_memory_is_reused_then_released():
+    """A closed context's device memory stays in the library's block cache and serves the
+    next context of the same mesh without new device allocations; release_cached_memory
+    gives it back to the device."""
+    from dg_nn_public_amd.diffusion import release_cached_memory
+    print("[case] block cache reuse", flush=True)
+    m = meshgen.cube_mesh(10, seed=3)
+    p = meshgen.sine_problem(m, sigma=1.0, kappa_seed=4)
+
+    def solve():
+        g = DGDiffusion.from_mesh(m, p.kappa, face_kappa="harmonic", blk_solver="direct")
+        g._get_RAR_and_sfc_data_Um()
+        b = g._get_rhs(g.new_vector(), p.f, p.u_bc)
+        x, _ = g._gmres_mg_solver(g.new_vector(), b, rtol=1e-10, inner=torch.float32)
+        torch.cuda.synchronize()
+        return g, to_np(x), g.last_stats["relres"], g.device_bytes()
+
+    release_cached_memory()
+    g, x1, rel1, nbytes = solve()
+    g.close(synced=True)
+    free1 = torch.cuda.mem_get_info()[0]
+    g, x2, rel2, _ = solve()
+    free2 = torch.cuda.mem_get_info()[0]
+    g.close(synced=True)
+    print("context bytes", nbytes, "free after close", free1, "after the second set-up", free2,
+          flush=True)
+    assert rel1 <= 1e-10 and rel2 <= 1e-10 and np.array_equal(x1, x2)
+    assert free2 >= free1 - 0.1 * nbytes  # served from the cache
+    release_cached_memory()
+    free3 = torch.cuda.mem_get_info()[0]
+    print("after release", free3, flush=True)
+    assert free3 >= free2 + 0.8 * nbytes
