@@ -75,7 +75,12 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
 // FP32 neighbour-form sweeps: at least 6 waves per SIMD, so that a 24^3x6 mesh (1,296
 // work-groups) is resident in one round on 256 CUs; within that register budget the
 // compiler issues the smoother rows' loads ahead of the products.
-template <typename T, int D, int MODE, bool EMIT = false, typename SS = T>
+// NBX (single pass): the face traces are not read from a trace buffer but computed here,
+// the element's own from its row of x and the neighbour's from the neighbour's row (80 B in
+// FP32, read through L2), so no trace pass runs before and no trace buffer is written or
+// read.  Modes 0, 1, 4 (x is read only) and 5 (x_new = the output, x read only: the caller
+// passes the new rows in r).
+template <typename T, int D, int MODE, bool EMIT = false, typename SS = T, bool NBX = false>
 __global__ __launch_bounds__(64 * (D + 1),
                              sizeof(T) == 4 && (MODE == 5 || MODE == 6) ? 6 : 1) void k_residual_w(
     int nele, T* __restrict__ x, const T* b, T* r, const T* __restrict__ vtab,
@@ -89,6 +94,8 @@ __global__ __launch_bounds__(64 * (D + 1),
   using M = Dm<D>;
   using F = FaceDims<D>;
   static_assert(!EMIT || MODE == 5 || MODE == 6, "traces are emitted by the sweeps");
+  static_assert(!NBX || (!EMIT && (MODE <= 1 || MODE == 4 || MODE == 5)),
+                "single-pass form: applies and neighbour-form sweeps");
   const int grp = act ? (int)blockIdx.x : xcd_group(blockIdx.x, gridDim.x);
   if (act && !act[grp]) return;  // whole block: no barrier is reached
   constexpr bool OFFD = MODE == 5;
@@ -98,7 +105,10 @@ __global__ __launch_bounds__(64 * (D + 1),
   constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
   constexpr bool ROWS_FIT = NL % NF == 0;   // every wave's rows exist (3D): no row guards
   static_assert(M::NGI % NF == 0, "volume points must split evenly over the waves");
-  __shared__ T red[NF][NL][64];
+  // partial residual rows; in the single-pass form wave w first stages its face's
+  // neighbour traces (2 x SG values per lane) in its own rows red[w]
+  constexpr int RROWS = NBX && 2 * M::SG > NL ? 2 * M::SG : NL;
+  __shared__ T red[NF][RROWS][64];
   // the new rows for the trace output (other waves may still read red when a wave writes)
   __shared__ T xs[EMIT ? NL : 1][64];
   const int lane = threadIdx.x & 63;
@@ -171,7 +181,79 @@ __global__ __launch_bounds__(64 * (D + 1),
   // face w's lift onto its own nodes, added to red[w] after the row sums are stored
   FaceLift<T, D> fl;
   fl.zero();
-  if (valid && !ZERO) {
+  if constexpr (NBX) {
+    // ---- face w in one pass, without lane-divergent branches: a lane whose face has no
+    // term here (or whose element is past the end) gets weight 0; a boundary face reads its
+    // own element as the "neighbour" with the neighbour's values multiplied by 0, and the
+    // interior and Dirichlet fluxes share one formula (a = 1/2 and the neighbour's trace on
+    // an interior face, a = 1 and no neighbour on a Dirichlet face):
+    //   q_s = w (-a (du/dn - du'/dn) + pen (u - u')),  q_d = -w a (u - u')
+    // and in the neighbour form (mode 5) only the neighbour's part.
+    const int f = w;
+    const int ec = valid ? e : nele - 1;
+    const int z = ec * NF + f;
+    const int code = fcode[z];
+    const int type = code & 15;
+    const bool interior = type == 0;
+    const bool on = valid && (OFFD ? interior : type < 2);
+    T c[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + ec];
+    const T wfe = on ? wf[f * nele + ec] : T(0);
+    const T pe = pen[f * nele + ec];
+    const T half = interior ? T(0.5) : T(1);
+    const T keep = interior ? T(1) : T(0);
+    const int nbz = interior ? nbface[z] : z;
+    const int nbe = nbz / NF, nbf = nbz - (nbz / NF) * NF;
+    const int a = interior ? (code >> 4) : 0;
+    {
+      // the neighbour's traces at its own face points, staged in this wave's LDS rows
+      T xn[NL], xnf[F::NFN], cn[D];
+      load_row<T, NL>(x + (size_t)nbe * NL, xn);
+#pragma unroll
+      for (int i = 0; i < F::NFN; ++i) xnf[i] = x[(size_t)nbe * NL + fnodes[nbf * F::NFN + i]];
+#pragma unroll
+      for (int d = 0; d < D; ++d) cn[d] = cf[(nbf * D + d) * nele + nbe];
+      face_traces<T, D>(nbf, fc, xn, xnf, cn, [&](int g, T u, T dn) {
+        red[w][g][lane] = keep * u;
+        red[w][M::SG + g][lane] = keep * dn;
+      });
+    }
+    T tu[M::SG], tdn[M::SG];
+    if constexpr (!OFFD) {
+      T xf[F::NFN];
+#pragma unroll
+      for (int i = 0; i < F::NFN; ++i) xf[i] = x[(size_t)ec * NL + fnodes[f * F::NFN + i]];
+      face_traces<T, D>(f, fc, xv, xf, c, [&](int g, T u, T dn) {
+        tu[g] = u;
+        tdn[g] = dn;
+      });
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int g = 0; g < M::SG; ++g) {
+      const T ws = wfe * sw[g];
+      const int ga = aln[a * M::SG + g];
+      const T tnu = red[w][ga][lane], tndn = red[w][M::SG + ga][lane];
+      T qsn, qdn;
+      if constexpr (OFFD) {
+        qsn = ws * (T(0.5) * tndn - pe * tnu);
+        qdn = ws * (T(0.5) * tnu);
+      } else {
+        const T jump = tu[g] - tnu;
+        qsn = ws * (-half * (tdn[g] - tndn) + pe * jump);
+        qdn = ws * (-half * jump);
+      }
+      fl.add(f, g, fc, qsn, qdn);
+    }
+    fl.finish(f, fc, c, acc);
+    // this wave reads its staged traces before it writes its partial rows into red[w]
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else if (valid && !ZERO) {
     // ---- face w (mode 5: interior faces, neighbour terms only)
     const int f = w;
     const int z = e * NF + f;
@@ -190,6 +272,7 @@ __global__ __launch_bounds__(64 * (D + 1),
         nbe = nbz / NF;
         nbf = nbz - nbe * NF;
       }
+      {
 #pragma unroll 2
       for (int g = 0; g < M::SG; ++g) {
         const T ws = wfe * sw[g];
@@ -212,6 +295,7 @@ __global__ __launch_bounds__(64 * (D + 1),
           qdn = -ws * tr.x;
         }
         fl.add(f, g, fc, qsn, qdn);
+      }
       }
       fl.finish(f, fc, c, acc);
     }
@@ -285,7 +369,8 @@ __global__ __launch_bounds__(64 * (D + 1),
           T s = ZERO ? T(0) : (OFFD ? (T(1) - omega) * xo[k] : xo[k]);
 #pragma unroll
           for (int m = 0; m < NL; ++m) s += srow[m] * bfull[m];
-          x[(size_t)e * NL + n] = s;
+          if constexpr (NBX) r[(size_t)e * NL + n] = s;  // the new rows go to another vector
+          else x[(size_t)e * NL + n] = s;
           if constexpr (MODE == 3) r[(size_t)e * NL + n] = rv[k];
           if constexpr (EMIT) xs[n][lane] = s;
         }
@@ -774,6 +859,23 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   const int32_t* act = c->mask_on ? c->act64 : nullptr;
   using V2 = typename Vec2<T>::type;
   V2* bufs[2] = {g.trace, g.trace2};
+  if ((mode <= 1 || mode == 4) && tin < 0 && (c->single_pass == 2 || (c->single_pass == 1 && D == 2))) {
+    // one pass: traces formed from the rows of x (own and neighbours')
+    const T* vt = g.tab + g.off_vtab;
+    const T* wg = g.tab + g.off_wgt;
+    const T* sw = g.tab + g.off_sw;
+#define DGMI_RES1(MODE)                                                                       \
+  hipLaunchKernelGGL((k_residual_w<T, D, MODE, false, T, true>), grd, blk, 0, s, nele, x, b, r, \
+                     vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,          \
+                     (const V2*)nullptr, g.S, act, (T)c->omega, (V2*)nullptr,                  \
+                     g.tab + g.off_fc, c->fnodes)
+    if (mode == 0) DGMI_RES1(0);
+    else if (mode == 1) DGMI_RES1(1);
+    else DGMI_RES1(4);
+#undef DGMI_RES1
+    DGMI_HIP(hipGetLastError());
+    return 0;
+  }
   if (mode == 6) {
     tin = tout == 0 ? 1 : 0;  // a sweep from zero reads no traces
   } else if (tin < 0) {
@@ -816,8 +918,8 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
 }
 
 // The sweeps of the bf16 inner cycles: as modes 5 and 6 with traces out, reading the
-// bfloat16 copy of the smoother
-template <int D>
+// bfloat16 copy of the smoother (SS = uint16_t)
+template <int D, typename SS>
 static int sweep_s16(dgmi_ctx* c, float* x, const float* b, int mode, int tin, int tout,
                      hipStream_t s) {
   DevGeom<float>& g = c->g32;
@@ -826,8 +928,9 @@ static int sweep_s16(dgmi_ctx* c, float* x, const float* b, int mode, int tin, i
   const int32_t* act = c->mask_on ? c->act64 : nullptr;
   using V2 = typename Vec2<float>::type;
   V2* bufs[2] = {g.trace, g.trace2};
+  const SS* S16 = (const SS*)g.S16;
   if (mode == 6) tin = tout == 0 ? 1 : 0;
-  if (!(g.S16 && (mode == 5 || mode == 6) && tin >= 0 && tin <= 1 && tout >= 0 && tout <= 1 &&
+  if (!(S16 && (mode == 5 || mode == 6) && tin >= 0 && tin <= 1 && tout >= 0 && tout <= 1 &&
         tin != tout && bufs[tin] && bufs[tout])) {
     set_error("bad trace buffers or no bf16 smoother");
     return -2;
@@ -836,15 +939,21 @@ static int sweep_s16(dgmi_ctx* c, float* x, const float* b, int mode, int tin, i
   const float* wg = g.tab + g.off_wgt;
   const float* sw = g.tab + g.off_sw;
 #define DGMI_RES16(MODE)                                                                      \
-  hipLaunchKernelGGL((k_residual_w<float, D, MODE, true, uint16_t>), grd, blk, 0, s, nele, x,  \
-                     b, nullptr, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface,        \
-                     c->fcode, (const V2*)bufs[tin], (const uint16_t*)g.S16, act,             \
-                     (float)c->omega, bufs[tout], g.tab + g.off_fc, c->fnodes)
+  hipLaunchKernelGGL((k_residual_w<float, D, MODE, true, SS>), grd, blk, 0, s, nele, x, b,     \
+                     nullptr, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface,           \
+                     c->fcode, (const V2*)bufs[tin], S16, act, (float)c->omega, bufs[tout],    \
+                     g.tab + g.off_fc, c->fnodes)
   if (mode == 5) DGMI_RES16(5);
   else DGMI_RES16(6);
 #undef DGMI_RES16
   DGMI_HIP(hipGetLastError());
   return 0;
+}
+template <typename SS>
+static int sweep16(dgmi_ctx* c, float* x, const float* b, int mode, int tin, int tout,
+                   hipStream_t s) {
+  return c->D == 3 ? sweep_s16<3, SS>(c, x, b, mode, tin, tout, s)
+                   : sweep_s16<2, SS>(c, x, b, mode, tin, tout, s);
 }
 
 // FP32 vectors go to the matrix-core kernels unless the context selects the vector ones
@@ -889,13 +998,13 @@ bool trace_io(const dgmi_ctx* c) {
 }
 int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s,
                      bool s16) {
-  if (s16) return c->D == 3 ? sweep_s16<3>(c, x, b, 5, tin, tout, s) : sweep_s16<2>(c, x, b, 5, tin, tout, s);
+  if (s16) return sweep16<uint16_t>(c, x, b, 5, tin, tout, s);
   if (c->D == 3) return apply_impl<float, 3>(c, c->g32, x, b, nullptr, 5, s, tin, tout);
   return apply_impl<float, 2>(c, c->g32, x, b, nullptr, 5, s, tin, tout);
 }
 int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipStream_t s,
                           bool s16) {
-  if (s16) return c->D == 3 ? sweep_s16<3>(c, x, b, 6, -1, tout, s) : sweep_s16<2>(c, x, b, 6, -1, tout, s);
+  if (s16) return sweep16<uint16_t>(c, x, b, 6, -1, tout, s);
   if (c->D == 3) return apply_impl<float, 3>(c, c->g32, x, b, nullptr, 6, s, -1, tout);
   return apply_impl<float, 2>(c, c->g32, x, b, nullptr, 6, s, -1, tout);
 }

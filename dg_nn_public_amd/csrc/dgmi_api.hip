@@ -1916,6 +1916,12 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
     c->mg.use_graph = value != 0.0;
     return 0;
   }
+  if (k == "single_pass") {
+    DGMI_CHECK(value == 0.0 || value == 1.0 || value == 2.0,
+               "single_pass must be 0 (never), 1 (on triangles) or 2 (always)");
+    c->single_pass = (int)value;
+    return 0;
+  }
   if (k == "destroy_sync") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "destroy_sync must be 0 or 1");
     c->destroy_sync = (int)value;

@@ -246,6 +246,12 @@ struct dgmi_ctx {
   // every mesh measured, and its sweeps carry the face traces), 1 MFMA fp32, 2 MFMA with
   // bf16 inputs
   int f32_kernel = 0;
+  // vector-kernel applies in one pass (face traces formed from the rows of x, own and
+  // neighbours') instead of a trace pass and a residual pass: 0 never, 1 on triangles
+  // (faster there: 23 against 28 us for an FP32 apply on 180k triangles; slower on
+  // tetrahedra, where the neighbour's face tables are gathered per lane and the kernel
+  // holds 160 registers: 58 against 44 us at 83k tets), 2 always
+  int single_pass = 1;
   // FP32 smoothing sweeps with the exact block smoother in the neighbour-only form
   // x = (1 - omega) x + S (b - A_off x) (no volume term, half the face terms)
   int smooth_offdiag = 1;

@@ -263,6 +263,10 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     start; 0 = sweep every level as the FP64 cycle (the reference's)
  *                     does.  dgmi_mg_info writes the level to info[4 + nlevel] when the
  *                     buffer holds it (-1: none).
+ *   "single_pass"     vector-kernel applies (dgmi_apply, the solvers' operator) in one
+ *                     kernel that forms the face traces from the rows of x, own and
+ *                     neighbours', instead of a trace pass and a residual pass: 0 never,
+ *                     1 on triangles (default; faster there, slower on tetrahedra), 2 always.
  *   "destroy_sync"    dgmi_ctx_destroy: 1 = synchronise the device first (default); 0 =
  *                     the caller has finished all work it queued for the context (the
  *                     serving layer, which synchronises its job's stream).
