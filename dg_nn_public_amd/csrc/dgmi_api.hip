@@ -401,18 +401,18 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
     if (s16 || (!c->pc_bf16 && trace_io(c))) {
       // each sweep (and the prolongation) writes the traces of its result for the next
       // one; buffers 0 and 1 alternate
-      DGMI_TRY(launch_smooth_zero_io(c, x, b, 0, s, s16));
-      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s, s16));
-      DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s, s16));
-      DGMI_TRY(launch_apply_io(c, x, b, r, 1, 0, s));
+      const int ns = c->f32_smooth_its;
+      int tb = 0;  // the trace buffer holding the current x's traces
+      DGMI_TRY(launch_smooth_zero_io(c, x, b, tb, s, s16));
+      for (int k = 1; k < ns; ++k, tb = 1 - tb) DGMI_TRY(launch_smooth_io(c, x, b, tb, 1 - tb, s, s16));
+      DGMI_TRY(launch_apply_io(c, x, b, r, 1, tb, s));
       const bool fz = vcycle_fuses_first<T>(c);
       DGMI_TRY(launch_restrict<T>(c, r, rc, s, fz ? ec : nullptr));
       DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s, fz));
       DGMI_TRY(launch_prolong_trace(c, ec, x, 0, s));
-      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s, s16));
-      DGMI_TRY(launch_smooth_io(c, x, b, 1, 0, s, s16));
-      DGMI_TRY(launch_smooth_io(c, x, b, 0, 1, s, s16));
-      if (tr_out) *tr_out = 1;
+      tb = 0;
+      for (int k = 0; k < ns; ++k, tb = 1 - tb) DGMI_TRY(launch_smooth_io(c, x, b, tb, 1 - tb, s, s16));
+      if (tr_out) *tr_out = tb;
       return 0;
     }
   }
@@ -1914,6 +1914,18 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "graph_vcycle") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "graph_vcycle must be 0 or 1");
     c->mg.use_graph = value != 0.0;
+    return 0;
+  }
+  if (k == "dense_max_n") {
+    DGMI_CHECK(value >= 64.0 && value <= 16384.0 && value == (int)value,
+               "dense_max_n must be an integer in [64, 16384]");
+    c->dense_max_n = (int)value;
+    return 0;
+  }
+  if (k == "f32_smooth_its") {
+    DGMI_CHECK(value >= 1.0 && value <= 8.0 && value == (int)value,
+               "f32_smooth_its must be an integer in [1, 8]");
+    c->f32_smooth_its = (int)value;
     return 0;
   }
   if (k == "single_pass") {

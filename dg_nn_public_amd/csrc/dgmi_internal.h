@@ -252,6 +252,12 @@ struct dgmi_ctx {
   // tetrahedra, where the neighbour's face tables are gathered per lane and the kernel
   // holds 160 registers: 58 against 44 us at 83k tets), 2 always
   int single_pass = 1;
+  // pre- and post-smoothing sweeps of the FP32 cycles with carried traces (the reference's
+  // pre_smooth_its, z02/config.py:112)
+  int f32_smooth_its = 3;
+  // FP32 cycles of a context of one solve: the exact (dense inverse) level is the first SFC
+  // level of at most this many rows (set before dgmi_mg_setup)
+  int dense_max_n = 4096;
   // FP32 smoothing sweeps with the exact block smoother in the neighbour-only form
   // x = (1 - omega) x + S (b - A_off x) (no volume term, half the face terms)
   int smooth_offdiag = 1;

@@ -325,7 +325,7 @@ __host__ __device__ inline size_t lds_align(size_t x) { return (x + 15) & ~size_
 // -1.  With c >= 0 the cycle stops at level c and sets B_c = A_c^-1 r_c there; the levels
 // below c are not visited.
 constexpr int EXACT_MAX_N = 512;    // single-work-group path (packed solves)
-constexpr int DENSE_MAX_N = 4096;   // grid path (a context of one solve)
+// grid path (a context of one solve): the limit is the context's dense_max_n
 
 template <typename T>
 __global__ __launch_bounds__(SMALL_THREADS) void k_small_levels_lds(
@@ -1417,7 +1417,7 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
   if (!lv.cinfo32) return fail();
   mg.exact32.assign(nc, -1);
   for (int b = 0; b < nc; ++b) mg.exact32[lay.order[b]] = (int)cinfo[2 * b + 1];
-  // one solve: the exact level is the first of at most DENSE_MAX_N rows (inverted on the
+  // one solve: the exact level is the first of at most dense_max_n rows (inverted on the
   // device) and a grid kernel applies its inverse, in place of the single-work-group
   // kernel and of the grid levels from there down
   lv.dense_rm32 = nullptr;
@@ -1426,7 +1426,7 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
   if (nc == 1) {
     const SfcLevelsHost& h = lay.lh[lay.order[0]];
     for (int l = 0; l <= std::min(lay.start32[lay.order[0]], h.nlevel - 2); ++l)
-      if (h.n[l] <= DENSE_MAX_N) {
+      if (h.n[l] <= c->dense_max_n) {
         // the level the packed path inverts: reuse that inverse, so that a solve alone
         // and packed with others gives the same bits
         if (float* inv = l == lvl[0] ? lv.cinv32 : dense_inverse_device(c, h, l, s)) {

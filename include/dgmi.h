@@ -263,6 +263,12 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     start; 0 = sweep every level as the FP64 cycle (the reference's)
  *                     does.  dgmi_mg_info writes the level to info[4 + nlevel] when the
  *                     buffer holds it (-1: none).
+ *   "f32_smooth_its"  pre- and post-smoothing sweeps of the FP32 cycles with carried traces
+ *                     (default 3, the reference's pre_smooth_its, z02/config.py:112; 2 or 4
+ *                     measured no faster overall).
+ *   "dense_max_n"     FP32 cycles of a context of one solve: the exact level is the first SFC
+ *                     level of at most this many rows (default 4096; set before
+ *                     dgmi_mg_setup; 8192 and 16384 measured no fewer iterations).
  *   "single_pass"     vector-kernel applies (dgmi_apply, the solvers' operator) in one
  *                     kernel that forms the face traces from the rows of x, own and
  *                     neighbours', instead of a trace pass and a residual pass: 0 never,
