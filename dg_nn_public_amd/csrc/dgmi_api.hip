@@ -991,6 +991,27 @@ static int build_seg(dgmi_ctx* c) {
   }
   wcb[c->ncomp] = (int32_t)wb.size();
   sg.nwblk = (int)wb.size();
+  // super-blocks of the two-stage partial sums (never across a component)
+  std::vector<int32_t> sbs, scb(c->ncomp + 1, 0);
+  int32_t most = 0;
+  for (int q = 0; q < c->ncomp; ++q) {
+    scb[q] = (int32_t)sbs.size();
+    most = std::max(most, wcb[q + 1] - wcb[q]);
+    for (int32_t a = wcb[q]; a < wcb[q + 1]; a += GS_SUPER) sbs.push_back(a);
+  }
+  scb[c->ncomp] = (int32_t)sbs.size();
+  sg.nsblk = (int)sbs.size();
+  sbs.push_back(wcb[c->ncomp]);
+  sg.two_stage = most > GS_TWO_STAGE;
+  if (sg.two_stage) {
+    sg.sblk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * sbs.size()));
+    sg.comp_sblk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * scb.size()));
+    sg.spartial = static_cast<double*>(c->alloc(sizeof(double) * MDOT_K * std::max(sg.nsblk, 1)));
+    DGMI_CHECK(sg.sblk && sg.comp_sblk && sg.spartial, alloc_failure());
+    DGMI_HIP(hipMemcpy(sg.sblk, sbs.data(), sizeof(int32_t) * sbs.size(), hipMemcpyHostToDevice));
+    DGMI_HIP(hipMemcpy(sg.comp_sblk, scb.data(), sizeof(int32_t) * scb.size(),
+                       hipMemcpyHostToDevice));
+  }
   sg.wblk = static_cast<SegBlk*>(c->alloc(sizeof(SegBlk) * std::max<size_t>(wb.size(), 1)));
   sg.comp_wblk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * wcb.size()));
   DGMI_CHECK(sg.wblk && sg.comp_wblk, alloc_failure());

@@ -125,7 +125,16 @@ struct SegDev {
   SegBlk* wblk = nullptr;        // (nwblk)
   int32_t* comp_wblk = nullptr;  // (ncomp + 1)
   bool wvec4 = false;            // every wide block starts at a multiple of 4 DOFs
+  // contexts with a component of many wide blocks sum the Gram-Schmidt partials in two
+  // stages: super-blocks of GS_SUPER consecutive wide blocks of one component first
+  bool two_stage = false;
+  int nsblk = 0;
+  int32_t* sblk = nullptr;       // (nsblk + 1) first wide block of each super-block
+  int32_t* comp_sblk = nullptr;  // (ncomp + 1) first super-block of each component
+  double* spartial = nullptr;    // (nsblk * MDOT_K)
 };
+constexpr int GS_SUPER = 32;         // wide blocks per super-block
+constexpr int GS_TWO_STAGE = 256;    // a component with more wide blocks uses two stages
 
 // Operand tables of the matrix-core apply (apply_mfma.hip), in MFMA lane order.
 struct MfTabs {

@@ -373,6 +373,43 @@ __global__ __launch_bounds__(GS_FINAL) void k_gs_final(const int32_t* __restrict
   }
 }
 
+// First stage of the two-stage partial sums: super-block sb sums the partials of its wide
+// blocks [sblk[sb], sblk[sb + 1]) column by column in a fixed order.  (A single work-group
+// per component walking thousands of partials is latency-bound: ~100 us at 5,000 wide
+// blocks.)
+__global__ __launch_bounds__(256) void k_gs_stage1(const int32_t* __restrict__ sblk,
+                                                  const double* __restrict__ partial, int k,
+                                                  double* __restrict__ spartial) {
+  constexpr int G = 256 / MDOT_K;
+  const int sb = blockIdx.x;
+  const int j = threadIdx.x % MDOT_K, grp = threadIdx.x / MDOT_K;
+  double a = 0.0;
+  if (j < k)
+    for (int bb = sblk[sb] + grp; bb < sblk[sb + 1]; bb += G) a += partial[(int64_t)bb * MDOT_K + j];
+  __shared__ double sh[G][MDOT_K + 1];
+  sh[grp][j] = a;
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double t = 0.0;
+    for (int q = 0; q < G; ++q) t += sh[q][threadIdx.x];
+    spartial[(int64_t)sb * MDOT_K + threadIdx.x] = t;
+  }
+}
+
+// The final sums of the Gram-Schmidt partials, in one or two stages
+static void gs_final(const SegDev& sg, int k, int mode, double* h, int64_t hs, double* coef,
+                     double* hn, const int32_t* active, hipStream_t s) {
+  if (sg.two_stage) {
+    hipLaunchKernelGGL(k_gs_stage1, dim3(sg.nsblk), dim3(256), 0, s, sg.sblk,
+                       (const double*)sg.partial, k, sg.spartial);
+    hipLaunchKernelGGL(k_gs_final, dim3(sg.ncomp), dim3(GS_FINAL), 0, s, sg.comp_sblk,
+                       (const double*)sg.spartial, k, mode, h, hs, coef, hn, active);
+  } else {
+    hipLaunchKernelGGL(k_gs_final, dim3(sg.ncomp), dim3(GS_FINAL), 0, s, sg.comp_wblk,
+                       (const double*)sg.partial, k, mode, h, hs, coef, hn, active);
+  }
+}
+
 // y = scale * x per component: mode 0 scale = 1/a[c]; 1 scale = a[c].  Inactive
 // components (or a[c] = 0 in mode 0) get y = 0.
 template <typename Tin, typename Tout>
@@ -754,9 +791,7 @@ int sorth_pass(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h
     else
       hipLaunchKernelGGL((k_gs_dots<T, MDOT_K, false>), dim3(sg.nwblk), dim3(GS_THREADS), 0, s,
                          sg.wblk, kc, Vc, ldv, (const T*)w, sg.partial, active);
-    hipLaunchKernelGGL(k_gs_final, dim3(sg.ncomp), dim3(GS_FINAL), 0, s, sg.comp_wblk,
-                       (const double*)sg.partial, kc, accumulate ? 1 : 0, h + c0, hs, sg.coef,
-                       nullptr, active);
+    gs_final(sg, kc, accumulate ? 1 : 0, h + c0, hs, sg.coef, nullptr, active, s);
     hipLaunchKernelGGL((k_smaxpy<T>), dim3(sg.nblk), dim3(RED_THREADS), 0, s, sg.blk, kc, Vc,
                        ldv, (const double*)sg.coef, w, active);
   }
@@ -767,19 +802,16 @@ int sorth_pass(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h
 template <typename T, int KC, bool VEC>
 static void launch_cgs2(const SegDev& sg, int k, const T* V, int64_t ldv, T* w, double* h,
                         int64_t hs, double* hn, const int32_t* active, hipStream_t s) {
-  const dim3 g(sg.nwblk), t(GS_THREADS), f(sg.ncomp), tf(GS_FINAL);
+  const dim3 g(sg.nwblk), t(GS_THREADS);
   hipLaunchKernelGGL((k_gs_dots<T, KC, VEC>), g, t, 0, s, sg.wblk, k, V, ldv, (const T*)w,
                      sg.partial, active);
-  hipLaunchKernelGGL(k_gs_final, f, tf, 0, s, sg.comp_wblk, (const double*)sg.partial, k, 0, h,
-                     hs, sg.coef, nullptr, active);
+  gs_final(sg, k, 0, h, hs, sg.coef, nullptr, active, s);
   hipLaunchKernelGGL((k_gs_update<T, KC, 1, VEC>), g, t, 0, s, sg.wblk, k, V, ldv,
                      (const double*)sg.coef, w, sg.partial, active);
-  hipLaunchKernelGGL(k_gs_final, f, tf, 0, s, sg.comp_wblk, (const double*)sg.partial, k, 1, h,
-                     hs, sg.coef, nullptr, active);
+  gs_final(sg, k, 1, h, hs, sg.coef, nullptr, active, s);
   hipLaunchKernelGGL((k_gs_update<T, KC, 2, VEC>), g, t, 0, s, sg.wblk, k, V, ldv,
                      (const double*)sg.coef, w, sg.partial, active);
-  hipLaunchKernelGGL(k_gs_final, f, tf, 0, s, sg.comp_wblk, (const double*)sg.partial, 1, 2,
-                     nullptr, 0, sg.coef, hn, active);
+  gs_final(sg, 1, 2, nullptr, 0, sg.coef, hn, active, s);
 }
 
 template <typename T>
