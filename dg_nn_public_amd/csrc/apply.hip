@@ -74,7 +74,9 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
 // bf16 inner cycles).  Arithmetic is in T throughout.
 // FP32 neighbour-form sweeps: at least 6 waves per SIMD, so that a 24^3x6 mesh (1,296
 // work-groups) is resident in one round on 256 CUs; within that register budget the
-// compiler issues the smoother rows' loads ahead of the products.
+// compiler issues the smoother rows' loads ahead of the products.  The single-pass sweep
+// needs more registers for the neighbour's row and traces: 4 waves (128 registers, no
+// spills; 6 waves spill and measured 10% slower at 48^3x6).
 // NBX (single pass): the face traces are not read from a trace buffer but computed here,
 // the element's own from its row of x and the neighbour's from the neighbour's row (80 B in
 // FP32, read through L2), so no trace pass runs before and no trace buffer is written or
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(64 * (D + 1)) void k_trace_w(int nele, const T* __r
 // passes the new rows in r).
 template <typename T, int D, int MODE, bool EMIT = false, typename SS = T, bool NBX = false>
 __global__ __launch_bounds__(64 * (D + 1),
-                             sizeof(T) == 4 && (MODE == 5 || MODE == 6) ? 6 : 1) void k_residual_w(
+                             sizeof(T) == 4 && (MODE == 5 || MODE == 6) ? (NBX ? 4 : 6) : 1) void k_residual_w(
     int nele, T* __restrict__ x, const T* b, T* r, const T* __restrict__ vtab,
     const T* __restrict__ wgt, const T* __restrict__ sw,
     const int32_t* __restrict__ aln, const T* __restrict__ G, const T* __restrict__ cf,
@@ -90,7 +92,8 @@ __global__ __launch_bounds__(64 * (D + 1),
     const int32_t* __restrict__ fcode, const typename Vec2<T>::type* __restrict__ trace,
     const SS* __restrict__ S, const int32_t* __restrict__ act, T omega,
     typename Vec2<T>::type* __restrict__ tout, const T* __restrict__ fc,
-    const int32_t* __restrict__ fnodes) {
+    const int32_t* __restrict__ fnodes, const T* __restrict__ cnb,
+    const uint8_t* __restrict__ nperm) {
   using M = Dm<D>;
   using F = FaceDims<D>;
   static_assert(!EMIT || MODE == 5 || MODE == 6, "traces are emitted by the sweeps");
@@ -105,10 +108,16 @@ __global__ __launch_bounds__(64 * (D + 1),
   constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
   constexpr bool ROWS_FIT = NL % NF == 0;   // every wave's rows exist (3D): no row guards
   static_assert(M::NGI % NF == 0, "volume points must split evenly over the waves");
-  // partial residual rows; in the single-pass form wave w first stages its face's
-  // neighbour traces (2 x SG values per lane) in its own rows red[w]
-  constexpr int RROWS = NBX && 2 * M::SG > NL ? 2 * M::SG : NL;
-  __shared__ T red[NF][RROWS][64];
+  // partial residual rows; in the single-pass form wave w first relabels its face's
+  // neighbour row through its own rows red[w]
+  __shared__ T red[NF][NL][64];
+  // node relabellings of the single-pass form (NPERM vertex permutations x NL nodes)
+  constexpr int NPERM = D == 3 ? 24 : 6;
+  __shared__ uint8_t ptab[NBX ? NPERM * NL : 1];
+  if constexpr (NBX) {
+    for (int i = threadIdx.x; i < NPERM * NL; i += 64 * NF) ptab[i] = nperm[i];
+    __syncthreads();
+  }
   // the new rows for the trace output (other waves may still read red when a wave writes)
   __shared__ T xs[EMIT ? NL : 1][64];
   const int lane = threadIdx.x & 63;
@@ -189,6 +198,11 @@ __global__ __launch_bounds__(64 * (D + 1),
     // an interior face, a = 1 and no neighbour on a Dirichlet face):
     //   q_s = w (-a (du/dn - du'/dn) + pen (u - u')),  q_d = -w a (u - u')
     // and in the neighbour form (mode 5) only the neighbour's part.
+    // The neighbour's traces are formed in this face's frame: its vertices relabelled so
+    // that the shared face is its face f with this element's vertex order (node n of the
+    // relabelled neighbour is its node nperm[pid][n]; cnb = its J^-T n in that frame).
+    // Then its face point g is this side's point g, and the trace tables are those of face
+    // f for every lane (wave-uniform reads), whatever face the neighbour meets it with.
     const int f = w;
     const int ec = valid ? e : nele - 1;
     const int z = ec * NF + f;
@@ -203,20 +217,26 @@ __global__ __launch_bounds__(64 * (D + 1),
     const T pe = pen[f * nele + ec];
     const T half = interior ? T(0.5) : T(1);
     const T keep = interior ? T(1) : T(0);
-    const int nbz = interior ? nbface[z] : z;
-    const int nbe = nbz / NF, nbf = nbz - (nbz / NF) * NF;
-    const int a = interior ? (code >> 4) : 0;
+    const int nbe = interior ? nbface[z] / NF : ec;
+    const int pid = interior ? (code >> 8) & 31 : 0;  // 0: the identity
+    T tnu[M::SG], tndn[M::SG];
     {
-      // the neighbour's traces at its own face points, staged in this wave's LDS rows
-      T xn[NL], xnf[F::NFN], cn[D];
+      // the neighbour's row, relabelled through this lane's column of red[w]
+      T xn[NL];
       load_row<T, NL>(x + (size_t)nbe * NL, xn);
 #pragma unroll
-      for (int i = 0; i < F::NFN; ++i) xnf[i] = x[(size_t)nbe * NL + fnodes[nbf * F::NFN + i]];
+      for (int n = 0; n < NL; ++n) red[w][n][lane] = xn[n];
+      const uint8_t* pp = ptab + pid * NL;
+      T xr[NL], xrf[F::NFN], cn[D];
 #pragma unroll
-      for (int d = 0; d < D; ++d) cn[d] = cf[(nbf * D + d) * nele + nbe];
-      face_traces<T, D>(nbf, fc, xn, xnf, cn, [&](int g, T u, T dn) {
-        red[w][g][lane] = keep * u;
-        red[w][M::SG + g][lane] = keep * dn;
+      for (int n = 0; n < NL; ++n) xr[n] = red[w][pp[n]][lane];
+#pragma unroll
+      for (int i = 0; i < F::NFN; ++i) xrf[i] = red[w][pp[fnodes[f * F::NFN + i]]][lane];
+#pragma unroll
+      for (int d = 0; d < D; ++d) cn[d] = cnb[(f * D + d) * nele + ec];
+      face_traces<T, D>(f, fc, xr, xrf, cn, [&](int g, T u, T dn) {
+        tnu[g] = keep * u;
+        tndn[g] = keep * dn;
       });
     }
     T tu[M::SG], tdn[M::SG];
@@ -229,30 +249,22 @@ __global__ __launch_bounds__(64 * (D + 1),
         tdn[g] = dn;
       });
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int g = 0; g < M::SG; ++g) {
       const T ws = wfe * sw[g];
-      const int ga = aln[a * M::SG + g];
-      const T tnu = red[w][ga][lane], tndn = red[w][M::SG + ga][lane];
       T qsn, qdn;
       if constexpr (OFFD) {
-        qsn = ws * (T(0.5) * tndn - pe * tnu);
-        qdn = ws * (T(0.5) * tnu);
+        qsn = ws * (T(0.5) * tndn[g] - pe * tnu[g]);
+        qdn = ws * (T(0.5) * tnu[g]);
       } else {
-        const T jump = tu[g] - tnu;
-        qsn = ws * (-half * (tdn[g] - tndn) + pe * jump);
+        const T jump = tu[g] - tnu[g];
+        qsn = ws * (-half * (tdn[g] - tndn[g]) + pe * jump);
         qdn = ws * (-half * jump);
       }
       fl.add(f, g, fc, qsn, qdn);
     }
     fl.finish(f, fc, c, acc);
-    // this wave reads its staged traces before it writes its partial rows into red[w]
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // (red[w] is next written by this lane only, after its reads above: program order)
   } else if (valid && !ZERO) {
     // ---- face w (mode 5: interior faces, neighbour terms only)
     const int f = w;
@@ -263,7 +275,7 @@ __global__ __launch_bounds__(64 * (D + 1),
       T c[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) c[d] = cf[(f * D + d) * nele + e];
-      const int a = code >> 4;
+      const int a = (code >> 4) & 15;
       const T wfe = wf[f * nele + e];
       const T pe = pen[f * nele + e];
       int nbe = 0, nbf = 0;
@@ -475,7 +487,7 @@ __global__ __launch_bounds__(64) void k_residual_dd(
     const double keep = interior ? 1.0 : 0.0;  // multiplies the neighbour's values
     const int nbz = interior ? nbface[z] : z;   // a boundary face reads itself
     const int nbe = nbz / NF, nbf = nbz - (nbz / NF) * NF;
-    const int a = interior ? (code >> 4) : 0;
+    const int a = interior ? ((code >> 4) & 15) : 0;
     dd c[D], xf[F::NFN], tu[SG], tdn[SG], nu[SG], ndn[SG];
 #pragma unroll
     for (int d = 0; d < D; ++d) c[d] = dd(cf[(f * D + d) * nele + e]);
@@ -851,6 +863,26 @@ __global__ void k_convert(const double* __restrict__ in, T* __restrict__ out, in
 // Trace buffers of the vector family: 0 = g.trace, 1 = g.trace2.  tin >= 0: x's traces are
 // already in buffer tin (no trace pass); tin < 0: they are computed into buffer 0.  Mode 5
 // with tout >= 0 writes the new x's traces into buffer tout (tout != the buffer read).
+// Face-trace buffers of the two-pass kernels, allocated at their first use (the
+// single-pass kernels need none): FP64 one, FP32 two (sweeps alternate them)
+template <typename T>
+bool ensure_traces(dgmi_ctx* c, DevGeom<T>& g) {
+  using V2 = typename Vec2<T>::type;
+  const size_t bytes = sizeof(V2) * c->nface * c->sngi * c->nele;
+  if (!g.trace) g.trace = static_cast<V2*>(c->alloc(bytes));
+  if (std::is_same<T, float>::value && !g.trace2) g.trace2 = static_cast<V2*>(c->alloc(bytes));
+  if (!g.trace || (std::is_same<T, float>::value && !g.trace2)) {
+    set_error(alloc_failure("face traces"));
+    return false;
+  }
+  return true;
+}
+template bool ensure_traces<double>(dgmi_ctx*, DevGeom<double>&);
+template bool ensure_traces<float>(dgmi_ctx*, DevGeom<float>&);
+
+// the vector-kernel applies in one pass (every interior face needs its frame)
+bool single_pass_apply(const dgmi_ctx* c) { return c->frames_ok && c->single_pass >= 1; }
+
 template <typename T, int D>
 static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mode,
                       hipStream_t s, int tin = -1, int tout = -1) {
@@ -859,7 +891,7 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   const int32_t* act = c->mask_on ? c->act64 : nullptr;
   using V2 = typename Vec2<T>::type;
   V2* bufs[2] = {g.trace, g.trace2};
-  if ((mode <= 1 || mode == 4) && tin < 0 && (c->single_pass == 2 || (c->single_pass == 1 && D == 2))) {
+  if ((mode <= 1 || mode == 4) && tin < 0 && single_pass_apply(c)) {
     // one pass: traces formed from the rows of x (own and neighbours')
     const T* vt = g.tab + g.off_vtab;
     const T* wg = g.tab + g.off_wgt;
@@ -868,7 +900,7 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
   hipLaunchKernelGGL((k_residual_w<T, D, MODE, false, T, true>), grd, blk, 0, s, nele, x, b, r, \
                      vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,          \
                      (const V2*)nullptr, g.S, act, (T)c->omega, (V2*)nullptr,                  \
-                     g.tab + g.off_fc, c->fnodes)
+                     g.tab + g.off_fc, c->fnodes, g.cnb, c->nperm)
     if (mode == 0) DGMI_RES1(0);
     else if (mode == 1) DGMI_RES1(1);
     else DGMI_RES1(4);
@@ -876,6 +908,9 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
     DGMI_HIP(hipGetLastError());
     return 0;
   }
+  if (!ensure_traces(c, g)) return -3;
+  bufs[0] = g.trace;
+  bufs[1] = g.trace2;
   if (mode == 6) {
     tin = tout == 0 ? 1 : 0;  // a sweep from zero reads no traces
   } else if (tin < 0) {
@@ -896,7 +931,7 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
 #define DGMI_RES(MODE, EMIT)                                                                  \
   hipLaunchKernelGGL((k_residual_w<T, D, MODE, EMIT>), grd, blk, 0, s, nele, x, b, r, vt, wg,  \
                      sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode, trace, g.S,     \
-                     act, (T)c->omega, to, g.tab + g.off_fc, c->fnodes)
+                     act, (T)c->omega, to, g.tab + g.off_fc, c->fnodes, g.cnb, c->nperm)
   switch (mode) {
     case 0: DGMI_RES(0, false); break;
     case 1: DGMI_RES(1, false); break;
@@ -929,6 +964,9 @@ static int sweep_s16(dgmi_ctx* c, float* x, const float* b, int mode, int tin, i
   using V2 = typename Vec2<float>::type;
   V2* bufs[2] = {g.trace, g.trace2};
   const SS* S16 = (const SS*)g.S16;
+  if (!ensure_traces(c, g)) return -3;
+  bufs[0] = g.trace;
+  bufs[1] = g.trace2;
   if (mode == 6) tin = tout == 0 ? 1 : 0;
   if (!(S16 && (mode == 5 || mode == 6) && tin >= 0 && tin <= 1 && tout >= 0 && tout <= 1 &&
         tin != tout && bufs[tin] && bufs[tout])) {
@@ -942,7 +980,7 @@ static int sweep_s16(dgmi_ctx* c, float* x, const float* b, int mode, int tin, i
   hipLaunchKernelGGL((k_residual_w<float, D, MODE, true, SS>), grd, blk, 0, s, nele, x, b,     \
                      nullptr, vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface,           \
                      c->fcode, (const V2*)bufs[tin], S16, act, (float)c->omega, bufs[tout],    \
-                     g.tab + g.off_fc, c->fnodes)
+                     g.tab + g.off_fc, c->fnodes, g.cnb, c->nperm)
   if (mode == 5) DGMI_RES16(5);
   else DGMI_RES16(6);
 #undef DGMI_RES16
@@ -994,7 +1032,7 @@ int launch_smooth(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r_out, hipStr
 // The preconditioner's sweeps with traces carried between them (FP32 vector kernels in
 // the neighbour form only; trace_io() tells whether the context runs that way).
 bool trace_io(const dgmi_ctx* c) {
-  return c->f32_kernel == 0 && smooth_offd<float>(c) && c->g32.trace2;
+  return c->f32_kernel == 0 && smooth_offd<float>(c);
 }
 int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s,
                      bool s16) {
@@ -1009,13 +1047,41 @@ int launch_smooth_zero_io(dgmi_ctx* c, float* x, const float* b, int tout, hipSt
   return apply_impl<float, 2>(c, c->g32, x, b, nullptr, 6, s, -1, tout);
 }
 bool bf16_smoother(const dgmi_ctx* c) {
-  return smooth_offd<float>(c) && c->g32.S16 && c->g32.trace && c->g32.trace2;
+  return smooth_offd<float>(c) && c->g32.S16;
 }
 int launch_apply_io(dgmi_ctx* c, const float* x, const float* b, float* r, int mode, int tin,
                     hipStream_t s) {
   float* xm = const_cast<float*>(x);
   if (c->D == 3) return apply_impl<float, 3>(c, c->g32, xm, b, r, mode, s, tin, -1);
   return apply_impl<float, 2>(c, c->g32, xm, b, r, mode, s, tin, -1);
+}
+
+// FP32 neighbour-form sweeps in one pass: x_new = (1 - omega) x + S (b - A_off x), the
+// neighbours' face traces formed from their rows of x; x_new must be another vector than x
+bool single_pass_sweeps(const dgmi_ctx* c) {
+  return c->single_pass >= 2 && c->frames_ok && c->f32_kernel == 0 && smooth_offd<float>(c);
+}
+template <int D>
+static int sweep_nbx(dgmi_ctx* c, const float* x, const float* b, float* xnew, hipStream_t s) {
+  DevGeom<float>& g = c->g32;
+  const int nele = (int)c->nele;
+  const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
+  const int32_t* act = c->mask_on ? c->act64 : nullptr;
+  using V2 = typename Vec2<float>::type;
+  hipLaunchKernelGGL((k_residual_w<float, D, 5, false, float, true>), grd, blk, 0, s, nele,
+                     const_cast<float*>(x), b, xnew, g.tab + g.off_vtab, g.tab + g.off_wgt,
+                     g.tab + g.off_sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,
+                     (const V2*)nullptr, g.S, act, (float)c->omega, (V2*)nullptr,
+                     g.tab + g.off_fc, c->fnodes, g.cnb, c->nperm);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+int launch_sweep_nbx(dgmi_ctx* c, const float* x, const float* b, float* xnew, hipStream_t s) {
+  if (x == xnew) {
+    set_error("a single-pass sweep writes another vector than it reads");
+    return -2;
+  }
+  return c->D == 3 ? sweep_nbx<3>(c, x, b, xnew, s) : sweep_nbx<2>(c, x, b, xnew, s);
 }
 
 

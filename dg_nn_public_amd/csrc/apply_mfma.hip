@@ -325,7 +325,7 @@ __global__ __launch_bounds__(64 * WPB) void k_residual_mf(
   float2 tn[NF][FT];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    const int type = code[f] & 15, a = code[f] >> 4;
+    const int type = code[f] & 15, a = (code[f] >> 4) & 15;
 #pragma unroll
     for (int tt = 0; tt < FT; ++tt) {
       tn[f][tt] = make_float2(0.f, 0.f);
@@ -525,6 +525,7 @@ static int apply_mf_impl(dgmi_ctx* c, float* x, const float* b, float* r, int mo
   const MfArgs ta{c->mf.tT, c->mf.b1, c->mf.b2, c->aln};
   const dim3 blk(64 * WPB), grd((nele + 16 * WPB - 1) / (16 * WPB));
   const int32_t* act = c->mask_on ? c->act64 : nullptr;
+  if (!ensure_traces(c, g)) return -3;
   hipLaunchKernelGGL((k_trace_mf<TIN, D, WPB>), grd, blk, 0, s, nele, x, ta, g.cf,
                      (float2*)g.trace, act);
 #define DGMI_RES_MF(MODE)                                                                      \

@@ -246,19 +246,14 @@ static int upload_geom(dgmi_ctx* c, DevGeom<T>& g, const std::vector<double>& ta
     const size_t nz = (size_t)c->nele * c->nface;
     g.G = device_f32(c, f.G, (size_t)c->ng * c->nele);
     g.cf = device_f32(c, f.cf, nz * c->D);
+    g.cnb = device_f32(c, f.cnb, nz * c->D);
     g.wf = device_f32(c, f.wf, nz);
     g.pen = device_f32(c, f.pen, nz);
     // the conversions ran on the null stream: done before any caller stream reads them
     DGMI_HIP(hipStreamSynchronize(nullptr));
   }
-  g.trace = static_cast<typename Vec2<T>::type*>(
-      c->alloc(sizeof(typename Vec2<T>::type) * c->nface * c->sngi * c->nele));
-  if (std::is_same<T, float>::value) {
-    g.trace2 = static_cast<typename Vec2<T>::type*>(
-        c->alloc(sizeof(typename Vec2<T>::type) * c->nface * c->sngi * c->nele));
-    DGMI_CHECK(g.trace2, alloc_failure());
-  }
-  DGMI_CHECK(g.tab && g.G && g.cf && g.wf && g.pen && g.trace, alloc_failure());
+  // (the face-trace buffers of the two-pass kernels come at their first use)
+  DGMI_CHECK(g.tab && g.G && g.cf && g.cnb && g.wf && g.pen, alloc_failure());
   return 0;
 }
 
@@ -306,8 +301,8 @@ static std::vector<double> bdiag_refmats(const ElementTables& el) {
 struct Scratch {
   double* bdiag = nullptr;
   double* refm = nullptr;
-  double* vec64[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  float* vec32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* vec64[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  float* vec32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double* cg64[2] = {nullptr, nullptr};
   float* cg32[2] = {nullptr, nullptr};
   double* l2cn = nullptr;    // (nloc) sum_g w_g n[n,g]^2
@@ -398,6 +393,33 @@ static int precond(dgmi_ctx* c, const T* b, T* x, hipStream_t s, int* tr_out = n
     // exact block smoother (vector kernels, whatever FP32 family is tuned); the face
     // traces, the residual and the Krylov operator stay FP32
     const bool s16 = c->pc_bf16 && c->bf16_smoother && bf16_smoother(c);
+    if (!s16 && !c->pc_bf16 && single_pass_sweeps(c)) {
+      // single-pass sweeps read x and write the new rows to another vector (neighbours
+      // still read the old ones), so x and y alternate; the pre-smoothing ends in `mid`,
+      // chosen so that the post-smoothing ends in x
+      float* y = vbuf<float>(c, 5);
+      DGMI_CHECK(y, alloc_failure("preconditioner work vectors"));
+      const int ns = c->f32_smooth_its;
+      auto other = [&](float* v) { return v == x ? y : x; };
+      float* mid = (ns % 2) ? y : x;
+      float* cur = ((ns - 1) % 2) ? other(mid) : mid;
+      DGMI_TRY(launch_smooth_zero<float>(c, c->g32, cur, b, s));
+      for (int k = 1; k < ns; ++k) {
+        DGMI_TRY(launch_sweep_nbx(c, cur, b, other(cur), s));
+        cur = other(cur);
+      }
+      DGMI_TRY(launch_apply<float>(c, c->g32, cur, b, r, s));
+      const bool fz = vcycle_fuses_first<T>(c);
+      DGMI_TRY(launch_restrict<T>(c, r, rc, s, fz ? ec : nullptr));
+      DGMI_TRY(launch_sfc_vcycle<T>(c, rc, ec, s, fz));
+      DGMI_TRY(launch_prolong_add<T>(c, ec, cur, s));
+      for (int k = 0; k < ns; ++k) {
+        DGMI_TRY(launch_sweep_nbx(c, cur, b, other(cur), s));
+        cur = other(cur);
+      }
+      DGMI_CHECK(cur == x, "sweep buffers out of step");
+      return 0;
+    }
     if (s16 || (!c->pc_bf16 && trace_io(c))) {
       // each sweep (and the prolongation) writes the traces of its result for the next
       // one; buffers 0 and 1 alternate
@@ -1951,7 +1973,7 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   }
   if (k == "single_pass") {
     DGMI_CHECK(value == 0.0 || value == 1.0 || value == 2.0,
-               "single_pass must be 0 (never), 1 (on triangles) or 2 (always)");
+               "single_pass must be 0 (never), 1 (applies) or 2 (applies and FP32 sweeps)");
     c->single_pass = (int)value;
     return 0;
   }

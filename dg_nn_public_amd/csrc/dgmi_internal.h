@@ -41,6 +41,7 @@ template <typename T> struct DevGeom {
   size_t off_fc = 0;  // face-interpolation block (FaceDims in faces.h)
   T* G = nullptr;      // (ng, nele)
   T* cf = nullptr;     // (nface*D, nele)
+  T* cnb = nullptr;    // (nface*D, nele) the neighbour's J^-T n in the face's frame
   T* wf = nullptr;     // (nface, nele)
   T* pen = nullptr;    // (nface, nele)
   T* S = nullptr;      // smoother, packed symmetric: (a, b), a <= b, at sym_index(a, b)*nele + e
@@ -246,6 +247,10 @@ struct dgmi_ctx {
   int32_t* fcode = nullptr;
   int32_t* aln = nullptr;
   int32_t* fnodes = nullptr;  // (nface, snloc) the nodes on each face
+  // node relabellings of the face frames ((D+1)! x nloc; geometry.hip), and whether every
+  // interior face has its frame (the single-pass kernels need them)
+  uint8_t* nperm = nullptr;
+  bool frames_ok = false;
   double* detJ = nullptr;  // (nele) FP64
   dgmi::DevGeom<double> g64;
   dgmi::DevGeom<float> g32;
@@ -255,12 +260,12 @@ struct dgmi_ctx {
   // every mesh measured, and its sweeps carry the face traces), 1 MFMA fp32, 2 MFMA with
   // bf16 inputs
   int f32_kernel = 0;
-  // vector-kernel applies in one pass (face traces formed from the rows of x, own and
-  // neighbours') instead of a trace pass and a residual pass: 0 never, 1 on triangles
-  // (faster there: 23 against 28 us for an FP32 apply on 180k triangles; slower on
-  // tetrahedra, where the neighbour's face tables are gathered per lane and the kernel
-  // holds 160 registers: 58 against 44 us at 83k tets), 2 always
-  int single_pass = 1;
+  // vector kernels in one pass (face traces formed from the rows of x, own and
+  // neighbours', the neighbour's row relabelled into the face's frame) instead of a trace
+  // pass before them: 0 never, 1 the applies, 2 the applies and the FP32 preconditioner's
+  // sweeps (at 48^3x6 tets: FP32 apply 292 -> 234 us, FP64 609 -> 466 us, sweep 321 ->
+  // ~240 us)
+  int single_pass = 2;
   // pre- and post-smoothing sweeps of the FP32 cycles with carried traces (the reference's
   // pre_smooth_its, z02/config.py:112)
   int f32_smooth_its = 3;
@@ -366,6 +371,12 @@ int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, double
                           hipStream_t s);
 // FP32 vector-family sweeps carrying traces between them (see apply.hip)
 bool trace_io(const dgmi_ctx* c);
+// face-trace buffers of the two-pass kernels, allocated at first use (false: no memory)
+template <typename T> bool ensure_traces(dgmi_ctx* c, DevGeom<T>& g);
+// single-pass applies / FP32 neighbour-form sweeps (face traces from the rows of x)
+bool single_pass_apply(const dgmi_ctx* c);
+bool single_pass_sweeps(const dgmi_ctx* c);
+int launch_sweep_nbx(dgmi_ctx* c, const float* x, const float* b, float* xnew, hipStream_t s);
 // (s16: read the bfloat16 copy of the smoother)
 int launch_smooth_io(dgmi_ctx* c, float* x, const float* b, int tin, int tout, hipStream_t s,
                      bool s16 = false);

@@ -5,6 +5,7 @@
 // cross product of J . drst_duv, h = sqrt(sum sdetwei) in 3D) for straight-sided
 // elements, where J is constant: J[i][j] = x_i[j] - x_D[j] (rows are edge vectors from
 // vertex D), and checks that every P3 node sits at its affine position.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -19,6 +20,7 @@ struct GeoTabs {
   double swsum;          // sum of the face quadrature weights
   double eta;            // penalty coefficient
   int face_kappa;        // 0 arithmetic mean, 1 harmonic mean on interior faces
+  int opp[4];            // the vertex opposite face f
 };
 
 constexpr int GEO_THREADS = 256;
@@ -28,8 +30,15 @@ constexpr int GEO_THREADS = 256;
 //   cf[(f*D + j)*nele + e] invJ^T n_f
 //   wf[f*nele + e]         kappa_f |face| (kappa_f = 1 on Neumann faces, 0 without a term)
 //   pen[f*nele + e]        eta / h_f
-//   nbface, fcode          neighbour face id / code | alignment << 4
+//   nbface, fcode          neighbour face id / code | alignment << 4 | frame << 8
+//   cnb[(f*D + j)*nele + e] invJ'^T n' of the neighbour across face f in this face's frame
 //   detJ[e]
+// The frame of an interior face f: the neighbour's vertices relabelled so that the shared
+// vertices keep this element's local numbers and its fourth (third) vertex takes the
+// number of this element's vertex opposite f.  sigma[k] = the neighbour's own number of
+// relabelled vertex k, matched by position; frame = the lexicographic rank of sigma among
+// the permutations of D+1 vertices (0 = identity); J' = its Jacobian in that numbering,
+// n' = -n_f its outward normal.  *badface = the smallest face whose vertices do not match.
 // blockmax[block] = max over the block's elements of the node deviation from the affine
 // position relative to the element size; *bad = the smallest degenerate element index.
 template <int D>
@@ -39,7 +48,8 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(
     const int32_t* __restrict__ alnmt, GeoTabs t, double* __restrict__ G,
     double* __restrict__ cf, double* __restrict__ wf, double* __restrict__ pen,
     int32_t* __restrict__ nbface, int32_t* __restrict__ fcode, double* __restrict__ detJ,
-    double* __restrict__ blockmax, int* __restrict__ bad) {
+    double* __restrict__ cnb, double* __restrict__ blockmax, int* __restrict__ bad,
+    int* __restrict__ badface) {
   constexpr int NL = D == 3 ? 20 : 10, NB = D + 1, NF = D + 1;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double dev = 0.0;
@@ -144,11 +154,75 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(
       const int nb = nbf[z];
       double kf = 0.0;
       int code = 2, nbz = -1;
+      double cn[3] = {0.0, 0.0, 0.0};
       if (type < 0 && nb >= 0) {
         const double kn = kappa[nb / NF];
         kf = t.face_kappa == 1 ? 2.0 * ke * kn / (ke + kn) : 0.5 * (ke + kn);
         nbz = nb;
-        code = alnmt[z] << 4;
+        // the neighbour in this face's frame
+        const double* xn = x + (size_t)(nb / NF) * D * NL;
+        double vn[4][3] = {};
+        for (int k = 0; k < NB; ++k)
+          for (int j = 0; j < D; ++j) vn[k][j] = xn[j * NL + k];
+        int sig[4] = {0, 0, 0, 0}, used = 0;
+        bool ok = true;
+        for (int k = 0; k < NB; ++k) {
+          if (k == t.opp[f]) continue;
+          int best = 0;
+          double bd = INFINITY;
+          for (int q = 0; q < NB; ++q) {
+            double d2 = 0.0;
+            for (int j = 0; j < D; ++j) d2 += (vn[q][j] - v[k][j]) * (vn[q][j] - v[k][j]);
+            if (d2 < bd) {
+              bd = d2;
+              best = q;
+            }
+          }
+          ok = ok && !((used >> best) & 1);
+          used |= 1 << best;
+          sig[k] = best;
+        }
+        for (int q = 0; q < NB; ++q)
+          if (!((used >> q) & 1)) sig[t.opp[f]] = q;
+        int rank = 0;
+        for (int i = 0; i < NB; ++i) {
+          int less = 0;
+          for (int j = i + 1; j < NB; ++j) less += sig[j] < sig[i];
+          rank = rank * (NB - i) + less;
+        }
+        if (!ok) {
+          atomicMin(badface, z);
+          rank = 0;
+        }
+        double Jn[3][3] = {}, in[3][3] = {};
+        for (int i = 0; i < D; ++i)
+          for (int j = 0; j < D; ++j) Jn[i][j] = vn[sig[i]][j] - vn[sig[D]][j];
+        if constexpr (D == 3) {
+          const double dn = Jn[0][0] * (Jn[1][1] * Jn[2][2] - Jn[1][2] * Jn[2][1]) -
+                            Jn[0][1] * (Jn[1][0] * Jn[2][2] - Jn[1][2] * Jn[2][0]) +
+                            Jn[0][2] * (Jn[1][0] * Jn[2][1] - Jn[1][1] * Jn[2][0]);
+          in[0][0] = (Jn[1][1] * Jn[2][2] - Jn[1][2] * Jn[2][1]) / dn;
+          in[0][1] = (Jn[0][2] * Jn[2][1] - Jn[0][1] * Jn[2][2]) / dn;
+          in[0][2] = (Jn[0][1] * Jn[1][2] - Jn[0][2] * Jn[1][1]) / dn;
+          in[1][0] = (Jn[1][2] * Jn[2][0] - Jn[1][0] * Jn[2][2]) / dn;
+          in[1][1] = (Jn[0][0] * Jn[2][2] - Jn[0][2] * Jn[2][0]) / dn;
+          in[1][2] = (Jn[0][2] * Jn[1][0] - Jn[0][0] * Jn[1][2]) / dn;
+          in[2][0] = (Jn[1][0] * Jn[2][1] - Jn[1][1] * Jn[2][0]) / dn;
+          in[2][1] = (Jn[0][1] * Jn[2][0] - Jn[0][0] * Jn[2][1]) / dn;
+          in[2][2] = (Jn[0][0] * Jn[1][1] - Jn[0][1] * Jn[1][0]) / dn;
+        } else {
+          const double dn = Jn[0][0] * Jn[1][1] - Jn[0][1] * Jn[1][0];
+          in[0][0] = Jn[1][1] / dn;
+          in[0][1] = -Jn[0][1] / dn;
+          in[1][0] = -Jn[1][0] / dn;
+          in[1][1] = Jn[0][0] / dn;
+        }
+        for (int j = 0; j < D; ++j) {
+          double s = 0.0;
+          for (int i = 0; i < D; ++i) s -= in[i][j] * nrm[i];
+          cn[j] = isfinite(s) ? s : 0.0;
+        }
+        code = (int)(alnmt[z] << 4) | (rank << 8);
       } else if (type == 0) {
         kf = ke;
         code = 1;
@@ -160,6 +234,7 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(
       }
       nbface[z] = nbz;
       fcode[z] = code;
+      for (int j = 0; j < D; ++j) cnb[(size_t)(f * D + j) * nele + e] = cn[j];
       wf[(size_t)f * nele + e] = kf * sdet;
     }
   }
@@ -196,7 +271,8 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
   c->nbface = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * nz));
   c->fcode = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * nz));
   c->detJ = static_cast<double*>(c->alloc(sizeof(double) * nele));
-  if (!g.G || !g.cf || !g.wf || !g.pen || !c->nbface || !c->fcode || !c->detJ) {
+  g.cnb = static_cast<double*>(c->alloc(sizeof(double) * nz * D));
+  if (!g.G || !g.cf || !g.wf || !g.pen || !c->nbface || !c->fcode || !c->detJ || !g.cnb) {
     set_error(alloc_failure("geometry"));
     return -3;
   }
@@ -208,6 +284,43 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
   for (double w : c->el.sweight) t.swsum += w;
   t.eta = c->eta_e;
   t.face_kappa = face_kappa;
+  for (int f = 0; f < NF; ++f) {
+    int mask = 0;
+    for (int i = 0; i < D; ++i) mask |= 1 << c->el.face_verts[f * D + i];
+    for (int k = 0; k <= D; ++k)
+      if (!((mask >> k) & 1)) t.opp[f] = k;
+  }
+  // node relabellings of the face frames: for the vertex permutation sigma of rank r
+  // (lexicographic), node n of the relabelled element is the node m of the original
+  // whose barycentric weights satisfy w_m[sigma[k]] = w_n[k] for every vertex k
+  {
+    const int NB = D + 1, NL = c->nloc;
+    std::vector<int> sig(NB);
+    for (int k = 0; k < NB; ++k) sig[k] = k;
+    std::vector<uint8_t> perm;
+    const double* w = c->el.p1_prolong.data();  // (nloc, D+1)
+    do {
+      for (int n = 0; n < NL; ++n) {
+        int found = -1;
+        for (int m = 0; m < NL && found < 0; ++m) {
+          bool eq = true;
+          for (int k = 0; k < NB; ++k) eq = eq && std::fabs(w[m * NB + sig[k]] - w[n * NB + k]) < 1e-12;
+          if (eq) found = m;
+        }
+        if (found < 0) {
+          set_error("P3 nodes are not closed under vertex permutations");
+          return -2;
+        }
+        perm.push_back((uint8_t)found);
+      }
+    } while (std::next_permutation(sig.begin(), sig.end()));
+    c->nperm = static_cast<uint8_t*>(c->alloc(perm.size()));
+    if (!c->nperm) {
+      set_error(alloc_failure("geometry"));
+      return -3;
+    }
+    DGMI_HIP(hipMemcpy(c->nperm, perm.data(), perm.size(), hipMemcpyHostToDevice));
+  }
   std::vector<int32_t> nb32(nz), bt32(nz), al32(nz);
   for (int64_t z = 0; z < nz; ++z) {
     nb32[z] = (int32_t)c->topo.nbf[z];
@@ -218,11 +331,12 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
   double *dx = nullptr, *dk = nullptr, *dmax = nullptr;
   int32_t *dnb = nullptr, *dbt = nullptr, *dal = nullptr;
   int* dbad = nullptr;
+  int* dbadf = nullptr;
   int rc = 0;
   auto free_all = [&]() {
     (void)hipStreamSynchronize(s);  // nothing in flight reads the work buffers any more
     for (void* p : {(void*)dx, (void*)dk, (void*)dmax, (void*)dnb, (void*)dbt, (void*)dal,
-                    (void*)dbad})
+                    (void*)dbad, (void*)dbadf})
       dev_free(p);
   };
   auto step = [&](int r) {
@@ -247,29 +361,34 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
   if (step(take(&dx, sizeof(double) * nx)) && step(copy_parts()) &&
       step(to_dev(c->kappa, &dk, s)) && step(to_dev(nb32, &dnb, s)) && step(to_dev(bt32, &dbt, s)) &&
       step(to_dev(al32, &dal, s)) && step(take(&dmax, sizeof(double) * nb)) &&
-      step(take(&dbad, sizeof(int)))) {
+      step(take(&dbad, sizeof(int))) && step(take(&dbadf, sizeof(int)))) {
     const int big = INT_MAX;
-    if (step(hipMemcpyAsync(dbad, &big, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess ? 0 : -3)) {
+    if (step(hipMemcpyAsync(dbad, &big, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess ? 0 : -3) &&
+        step(hipMemcpyAsync(dbadf, &big, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess ? 0 : -3)) {
       if (D == 3)
         hipLaunchKernelGGL(k_geometry<3>, dim3(nb), dim3(GEO_THREADS), 0, s, (int)nele, dx, dk,
                            dnb, dbt, dal, t, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,
-                           c->detJ, dmax, dbad);
+                           c->detJ, g.cnb, dmax, dbad, dbadf);
       else
         hipLaunchKernelGGL(k_geometry<2>, dim3(nb), dim3(GEO_THREADS), 0, s, (int)nele, dx, dk,
                            dnb, dbt, dal, t, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,
-                           c->detJ, dmax, dbad);
+                           c->detJ, g.cnb, dmax, dbad, dbadf);
       std::vector<double> bm(nb);
-      int hbad = INT_MAX;
+      int hbad = INT_MAX, hbadf = INT_MAX;
       if (step(hipGetLastError() == hipSuccess ? 0 : -3) &&
           step(hipMemcpyAsync(bm.data(), dmax, sizeof(double) * nb, hipMemcpyDeviceToHost, s) ==
                        hipSuccess ? 0 : -3) &&
           step(hipMemcpyAsync(&hbad, dbad, sizeof(int), hipMemcpyDeviceToHost, s) ==
+                       hipSuccess ? 0 : -3) &&
+          step(hipMemcpyAsync(&hbadf, dbadf, sizeof(int), hipMemcpyDeviceToHost, s) ==
                        hipSuccess ? 0 : -3) &&
           step(hipStreamSynchronize(s) == hipSuccess ? 0 : -3)) {
         double m = 0.0;
         for (double v : bm) m = std::max(m, v);
         *max_dev = m;
         *bad_elem = hbad == INT_MAX ? -1 : hbad;
+        // single-pass kernels need every interior face's frame
+        c->frames_ok = hbadf == INT_MAX;
       }
     }
   }

@@ -676,7 +676,8 @@ int launch_restrict(dgmi_ctx* c, const T* r_dg, T* r_sfc, hipStream_t s, T* e_fi
 // FP32 vector family: x += P e and the traces of the new x into buffer tout (0 or 1)
 int launch_prolong_trace(dgmi_ctx* c, const float* e_sfc, float* x_dg, int tout, hipStream_t s) {
   const int nele = (int)c->nele;
-  const DevGeom<float>& g = c->g32;
+  DevGeom<float>& g = c->g32;
+  if (!ensure_traces(c, g)) return -3;
   typename Vec2<float>::type* to = tout == 0 ? g.trace : g.trace2;
   if (!to || (tout != 0 && tout != 1)) {
     set_error("bad trace buffer");

@@ -304,6 +304,8 @@ ElementTables make_element(int D) {
     for (int k = 0; k < nb; ++k) el.p1_prolong[n * nb + k] = el.index[n * nb + k] / double(P);
   if (D == 3) el.drst.assign(&TET_DRST[0][0][0], &TET_DRST[0][0][0] + 24);
   else el.drst.assign(&TRI_DRST[0][0], &TRI_DRST[0][0] + 6);
+  if (D == 3) el.face_verts.assign(&TET_FACE_VERTS[0][0], &TET_FACE_VERTS[0][0] + 12);
+  else el.face_verts.assign(&TRI_FACE_VERTS[0][0], &TRI_FACE_VERTS[0][0] + 6);
   return el;
 }
 

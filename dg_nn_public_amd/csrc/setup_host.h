@@ -52,6 +52,7 @@ struct ElementTables {
   std::vector<double> sweight;  // (sngi)
   std::vector<int> gi_align;    // (D, sngi)
   std::vector<double> p1_prolong;  // (nloc, D+1)
+  std::vector<int> face_verts;     // (nface, D) local vertices of each face
   // d(xi)/d(face params) per face (z02/function_space.py:200-213): 3D (nface, 3, 2),
   // 2D (nface, 2)
   std::vector<double> drst;

@@ -269,10 +269,11 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *   "dense_max_n"     FP32 cycles of a context of one solve: the exact level is the first SFC
  *                     level of at most this many rows (default 4096; set before
  *                     dgmi_mg_setup; 8192 and 16384 measured no fewer iterations).
- *   "single_pass"     vector-kernel applies (dgmi_apply, the solvers' operator) in one
- *                     kernel that forms the face traces from the rows of x, own and
- *                     neighbours', instead of a trace pass and a residual pass: 0 never,
- *                     1 on triangles (default; faster there, slower on tetrahedra), 2 always.
+ *   "single_pass"     vector kernels that form the face traces from the rows of x, own
+ *                     and neighbours' (the neighbour's row relabelled into the face's own
+ *                     frame), instead of a trace pass before them: 0 never (traces through
+ *                     trace buffers), 1 the applies (dgmi_apply, the solvers' operator and
+ *                     residuals), 2 also the FP32 preconditioner's sweeps (default).
  *   "destroy_sync"    dgmi_ctx_destroy: 1 = synchronise the device first (default); 0 =
  *                     the caller has finished all work it queued for the context (the
  *                     serving layer, which synchronises its job's stream).

@@ -9,6 +9,7 @@ import torch
 
 from dg_nn_public_amd import meshgen
 from dg_nn_public_amd.diffusion import DGDiffusion
+from dg_nn_public_amd._lib import check
 from gpu_common import DEV, build, rel, to_dev, to_np
 from oracle import dg, krylov, mg, tables
 
@@ -216,23 +217,76 @@ def test_double_double_residual_on_every_face_type(D, n):
     g.close()
 
 
+def _scrambled(m, seed):
+    """The same mesh with every element's vertices listed in a random order (kept
+    positively oriented), so that face pairs meet in many relative orientations."""
+    rng = np.random.default_rng(seed)
+    cells = np.stack([c[rng.permutation(m.D + 1)] for c in m.cells])
+    cells = meshgen._orient(m.verts, cells)
+    table = meshgen.TET_P3 if m.D == 3 else meshgen.TRI_P3
+    return meshgen.Mesh(D=m.D, x_ref_in=meshgen._p3_nodes(m.verts, cells, table), cells=cells,
+                        verts=m.verts)
+
+
+def _frame_ranks(m):
+    """Lexicographic ranks of the vertex relabellings that carry each interior face's
+    neighbour into the face's frame (shared vertices keep this side's numbers)."""
+    D = m.D
+    nbf, _, _ = dg.topology(m.cells, D)
+    fv = tables.TET_FACE_VERTS if D == 3 else tables.TRI_FACE_VERTS
+    ranks = set()
+    for z in np.nonzero(nbf >= 0)[0]:
+        e, f = divmod(int(z), D + 1)
+        nb = int(nbf[z]) // (D + 1)
+        opp = [k for k in range(D + 1) if k not in fv[f]][0]
+        own, other = list(m.cells[e]), list(m.cells[nb])
+        sig = [other.index(own[k]) if k != opp else -1 for k in range(D + 1)]
+        sig[opp] = (set(range(D + 1)) - set(sig)).pop()
+        r = 0
+        for i in range(D + 1):
+            r = r * (D + 1 - i) + sum(sig[j] < sig[i] for j in range(i + 1, D + 1))
+        ranks.add(r)
+    return ranks
+
+
 @pytest.mark.parametrize("D,n", [(2, 7), (3, 3)])
-def test_single_pass_apply_equals_two_pass(D, n):
-    """The one-kernel apply (face traces formed from the rows of x, own and neighbours')
-    gives the same bits as the trace pass + residual pass, in FP64 and FP32, for r = b - A x
-    and for A x, on a mesh with interior, Dirichlet and Neumann faces."""
+def test_single_pass_kernels_match_oracle_and_two_pass(D, n):
+    """The one-kernel apply and FP32 sweeps form the neighbour's face traces from its row of
+    x in the face's frame (its vertices relabelled so that the shared face is this side's
+    face with this side's vertex order, whatever orientation the two elements meet in).  On
+    a mesh whose elements list their vertices in random orders, with interior, Dirichlet
+    and Neumann faces: the FP64 single-pass apply equals the oracle's operator to rounding,
+    the FP32 one to FP32 rounding, both for r = b - A x and for A x; the FP32
+    preconditioner with single-pass sweeps equals the one whose sweeps carry face traces;
+    and an FP32-refinement solve with them reaches 1e-10 by the oracle's operator."""
     print(f"[case] single pass D={D} n={n}", flush=True)
-    m = meshgen.square_mesh(n) if D == 2 else meshgen.cube_mesh(n)
+    m = _scrambled(meshgen.square_mesh(n) if D == 2 else meshgen.cube_mesh(n), 11)
+    # every relabelling two positively oriented elements can meet in (half of the (D+1)!)
+    assert len(_frame_ranks(m)) == (3 if D == 2 else 12)
     kap = np.exp(np.random.default_rng(9).standard_normal(m.nele))
-    g = DGDiffusion.from_mesh(m, kap, bctype=_top_neumann(m), device=DEV,
-                              face_kappa="harmonic")
+    bct = _top_neumann(m)
+    g = DGDiffusion.from_mesh(m, kap, bctype=bct, device=DEV, face_kappa="harmonic",
+                              blk_solver="direct")
+    sp = dg.make_space(m, kap, bctype=bct, face_kappa="harmonic")
     rng = np.random.default_rng(10)
-    for dt in (torch.float64, torch.float32):
-        x, b = to_dev(rng.standard_normal(g.ndof), dt), to_dev(rng.standard_normal(g.ndof), dt)
-        out = {}
-        for sp in (0.0, 2.0):
-            g.lib.dgmi_set_param(g.ctx, b"single_pass", sp)
-            out[sp] = (g.apply(x, b).clone(), g.apply(x).clone())
-        assert torch.equal(out[0.0][0], out[2.0][0]) and torch.equal(out[0.0][1], out[2.0][1])
-    g.lib.dgmi_set_param(g.ctx, b"single_pass", 1.0)
+    x, b = rng.standard_normal(sp.ndof), rng.standard_normal(sp.ndof)
+    ax = dg.apply_A(sp, x)
+    for mode in (0.0, 1.0):
+        check(g.lib.dgmi_set_param(g.ctx, b"single_pass", mode), "dgmi_set_param")
+        for dt, tol in ((torch.float64, 1e-13), (torch.float32, 2e-6)):
+            r = to_np(g.apply(to_dev(x, dt), to_dev(b, dt)))
+            y = -to_np(g.apply(to_dev(x, dt)))
+            print(mode, dt, rel(r, b - ax), rel(y, ax), flush=True)
+            assert rel(r, b - ax) < tol and rel(y, ax) < tol
+    g._get_RAR_and_sfc_data_Um()
+    z = {}
+    for mode in (0.0, 2.0):
+        check(g.lib.dgmi_set_param(g.ctx, b"single_pass", mode), "dgmi_set_param")
+        z[mode] = to_np(g._um_left_precond(g.new_vector(torch.float32), to_dev(b, torch.float32)))
+    print("preconditioner", rel(z[2.0], z[0.0]), flush=True)
+    assert rel(z[2.0], z[0.0]) < 1e-5
+    xs, its = g._gmres_mg_solver(g.new_vector(), to_dev(b), rtol=1e-10, inner=torch.float32)
+    relres = np.linalg.norm(b - dg.apply_A(sp, to_np(xs))) / np.linalg.norm(b)
+    print("solve", its, g.last_stats, relres, flush=True)
+    assert relres <= 2e-10
     g.close()
