@@ -42,9 +42,10 @@ PEAK_FP64 = 78.6e12   # MI355X FP64 vector/matrix, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12     # B/s
 METRIC = "solves/s + p50/p99 latency at rtol 1e-10; DG apply GDoF/s, mixed vs FP64"
-# calibrated HBM bytes per launch of the FP32 smoothing sweep (k_residual_w<float,3,5,true>):
-# FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3 --pmc passes (profiles/), by element count
-PMC_SWEEP = {82944: 200.3e6, 663552: 1701.4e6}
+# calibrated HBM bytes per launch of the FP32 single-pass smoothing sweep
+# (k_residual_w<float,3,5,false,float,true>): FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3
+# --pmc passes (profiles/), by element count
+PMC_SWEEP = {}
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):
@@ -61,13 +62,17 @@ def bytes_per_elem(D, nloc, nface, s, read_b=True):
     return 2 * nloc * s + s + (D * D + 1) * s + 4 * nface + (nloc * s if read_b else 0)
 
 
-def sweep_bytes_per_elem(D, nloc, nface, sngi):
-    """Algorithmic HBM bytes of one FP32 smoothing sweep x = (1-w)x + S(b - A_off x) with
-    the face traces carried to the next sweep, per element: packed symmetric S, x read and
-    written, b, the neighbours' traces read and this element's traces written (value and
-    normal derivative at every face point), per-face geometry (c, wf, pen, code, nbface)."""
+def sweep_bytes_per_elem(D, nloc, nface, sngi, single_pass=True):
+    """Algorithmic HBM bytes of one FP32 smoothing sweep x_new = (1-w)x + S(b - A_off x),
+    per element.  Single pass (the default kernel): packed symmetric S, x read once (the
+    element's own row and, through L2, its neighbours' rows), b read, x_new written, and
+    per face c = J^-T n on both sides, wf, pen, code and nbface.  Two-pass form: the same
+    with the neighbours' traces read and this element's traces written (value and normal
+    derivative at every face point) and c on one side only."""
     s_packed = 4 * nloc * (nloc + 1) // 2
     vectors = 4 * 3 * nloc
+    if single_pass:
+        return s_packed + vectors + nface * (8 * D + 16)
     traces = 2 * nface * sngi * 8
     geometry = nface * (4 * D + 16)
     return s_packed + vectors + traces + geometry
@@ -312,10 +317,11 @@ def c3_block(args):
         "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": gbps, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": gbps * 1e9 / PEAK_HBM, "traffic": PMC_SWEEP.get(nele),
-                     "kernel": "FP32 smoothing sweep x = (1-w)x + S(b - A_off x) with the face "
-                               "traces written for the next sweep (" +
-                               ("k_residual_w<float,3,5,true>" if vec else "MFMA trace + residual")
-                               + "), 48^3x6 tets; the largest share of the mixed solve's kernel "
+                     "kernel": "FP32 smoothing sweep x_new = (1-w)x + S(b - A_off x), the "
+                               "neighbours' face traces formed from their rows of x (" +
+                               ("k_residual_w<float,3,5,false,float,true>, one launch" if vec
+                                else "MFMA trace + residual") +
+                               "), 48^3x6 tets; the largest share of the mixed solve's kernel "
                                "time",
                      "bytes_per_elem": SB, "flop_per_elem": SF, "elements": nele,
                      "avg_launch_us": t_sweep * 1e6, "launches_timed": args.sweeps,
@@ -658,10 +664,10 @@ def run_solve(args):
         "roofline": {"bound": "hbm", "achieved": sweep_gbps, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": sweep_gbps * 1e9 / PEAK_HBM,
                      "traffic": (pmc["f32_sweep"] if on_profiled and vec32 else None),
-                     "kernel": "FP32 smoothing sweep x = (1-w)x + S(b - A_off x) with the face "
-                               "traces written for the next sweep (" +
-                               ("k_residual_w<float,D,5,true>, one launch" if vec32 else
-                                "trace + residual MFMA launches") + ", autotuned family); "
+                     "kernel": "FP32 smoothing sweep x_new = (1-w)x + S(b - A_off x), the "
+                               "neighbours' face traces formed from their rows of x (" +
+                               ("k_residual_w<float,D,5,false,float,true>, one launch" if vec32
+                                else "trace + residual MFMA launches") + ", autotuned family); "
                                "the largest share of the mixed solve's kernel time",
                      "bytes_per_elem": SB, "flop_per_elem": SF, "elements": nele,
                      "avg_launch_us": t_sweep * 1e6,

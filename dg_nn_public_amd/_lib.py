@@ -32,6 +32,7 @@ class MeshDesc(C.Structure):
         ("bctype", _I64P),
         ("eta_e", C.c_double),
         ("face_kappa", C.c_int32),
+        ("stream", C.c_void_p),
     ]
 
 

@@ -499,11 +499,7 @@ int build_mfma_tables(dgmi_ctx* c) {
   auto up = [&](const void* h, size_t bytes) -> void* {
     void* d = c->alloc(bytes);
     if (!d) return nullptr;
-    const hipError_t e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      note_alloc_error(std::string("hipMemcpy to device: ") + hipGetErrorString(e));
-      return nullptr;
-    }
+    if (h2d(c, d, h, bytes, c->ustream)) return nullptr;
     return d;
   };
   c->mf.b1 = up(b1.data(), b1.size() * 2);

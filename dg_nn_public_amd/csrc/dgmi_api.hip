@@ -172,20 +172,126 @@ static void pinned_free(void* p) {
   C.live.erase(it);
 }
 
+// ---- set-up transfers through pinned buffers (h2d, h2d_release, d2h)
+namespace {
+constexpr size_t STAGE_BYTES = size_t(16) << 20;  // one staging buffer
+struct Stager {
+  char* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool busy[2] = {false, false};  // ev[k] marks the end of copies still reading buf[k]
+  int cur = 0;                    // the buffer being filled
+  size_t off = 0;                 // bytes of buf[cur] handed to copies
+  hipStream_t s = nullptr;        // stream of the copies out of buf[cur]
+};
+// host memcpy, split over threads for large pieces
+void host_copy(void* dst, const void* src, size_t n) {
+  if (n < (size_t(4) << 20)) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  parallel_chunks((int64_t)n, [&](int, int64_t a, int64_t b) {
+    std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, (size_t)(b - a));
+  }, int64_t(1) << 20);
+}
+// close buf[cur] (its copies so far end at ev[cur]) and make the other buffer current,
+// waiting for the copies that still read it
+int stager_turn(Stager& st) {
+  if (st.off > 0) {
+    DGMI_HIP(hipEventRecord(st.ev[st.cur], st.s));
+    st.busy[st.cur] = true;
+  }
+  st.cur ^= 1;
+  st.off = 0;
+  if (st.busy[st.cur]) {
+    DGMI_HIP(hipEventSynchronize(st.ev[st.cur]));
+    st.busy[st.cur] = false;
+  }
+  return 0;
+}
+}  // namespace
+
+int h2d(dgmi_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!bytes) return 0;
+  if (!c->stager) c->stager = new Stager{};
+  Stager& st = *static_cast<Stager*>(c->stager);
+  for (int k = 0; k < 2; ++k) {
+    if (!st.buf[k]) {
+      void* p = nullptr;
+      DGMI_HIP(pinned_alloc(&p, STAGE_BYTES));
+      st.buf[k] = static_cast<char*>(p);
+    }
+    if (!st.ev[k]) DGMI_HIP(hipEventCreateWithFlags(&st.ev[k], hipEventDisableTiming));
+  }
+  if (st.off > 0 && st.s != s) DGMI_TRY(stager_turn(st));
+  char* d = static_cast<char*>(dst);
+  const char* h = static_cast<const char*>(src);
+  while (bytes > 0) {
+    if (st.off == STAGE_BYTES) DGMI_TRY(stager_turn(st));
+    // pieces start 256-byte aligned in the buffer
+    const size_t n = std::min(bytes, STAGE_BYTES - st.off);
+    host_copy(st.buf[st.cur] + st.off, h, n);
+    st.s = s;
+    DGMI_HIP(hipMemcpyAsync(d, st.buf[st.cur] + st.off, n, hipMemcpyHostToDevice, s));
+    st.off = std::min(STAGE_BYTES, (st.off + n + 255) / 256 * 256);
+    d += n;
+    h += n;
+    bytes -= n;
+  }
+  return 0;
+}
+
+int h2d_release(dgmi_ctx* c) {
+  Stager* st = static_cast<Stager*>(c->stager);
+  if (!st) return 0;
+  int rc = 0;
+  if (st->off > 0) rc = stager_turn(*st);
+  for (int k = 0; k < 2; ++k) {
+    if (st->busy[k] && hipEventSynchronize(st->ev[k]) != hipSuccess && !rc) {
+      set_error("waiting for staged copies failed");
+      rc = -3;
+    }
+    st->busy[k] = false;
+    if (st->ev[k]) (void)hipEventDestroy(st->ev[k]);
+    pinned_free(st->buf[k]);
+  }
+  delete st;
+  c->stager = nullptr;
+  return rc;
+}
+
+int d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!bytes) return 0;
+  void* p = nullptr;
+  const size_t chunk = std::min(bytes, STAGE_BYTES);
+  DGMI_HIP(pinned_alloc(&p, chunk));
+  struct Give {
+    void* p;
+    ~Give() { pinned_free(p); }
+  } give{p};
+  for (size_t o = 0; o < bytes; o += chunk) {
+    const size_t n = std::min(chunk, bytes - o);
+    DGMI_HIP(hipMemcpyAsync(p, static_cast<const char*>(src) + o, n, hipMemcpyDeviceToHost, s));
+    DGMI_HIP(hipStreamSynchronize(s));
+    host_copy(static_cast<char*>(dst) + o, p, n);
+  }
+  return 0;
+}
+
+// device copy of a host vector (converted to T), queued on s; nullptr on failure
 template <typename T, typename U>
-static T* upload(dgmi_ctx* c, const std::vector<U>& h) {
+static T* upload(dgmi_ctx* c, const std::vector<U>& h, hipStream_t s) {
   T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(h.size(), 1)));
   if (!d) return nullptr;
   if (!h.empty()) {
-    hipError_t e;
+    int rc;
     if constexpr (std::is_same<T, U>::value) {  // no host copy
-      e = hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+      rc = h2d(c, d, h.data(), sizeof(T) * h.size(), s);
     } else {
-      std::vector<T> t(h.begin(), h.end());
-      e = hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+      const std::vector<T> t(h.begin(), h.end());
+      rc = h2d(c, d, t.data(), sizeof(T) * t.size(), s);
     }
-    if (e != hipSuccess) {
-      g_alloc_err = std::string("hipMemcpy to device: ") + hipGetErrorString(e);
+    if (rc != 0) {
+      g_alloc_err = "copy to device: " + g_err;
       return nullptr;
     }
   }
@@ -196,7 +302,7 @@ static T* upload(dgmi_ctx* c, const std::vector<U>& h) {
 static float* device_f32(dgmi_ctx* c, const double* src, size_t n) {
   float* d = static_cast<float*>(c->alloc(sizeof(float) * std::max<size_t>(n, 1)));
   if (!d || !src) return nullptr;
-  if (n && launch_convert<float>(src, d, (int64_t)n, nullptr) != 0) return nullptr;
+  if (n && launch_convert<float>(src, d, (int64_t)n, c->ustream) != 0) return nullptr;
   return d;
 }
 
@@ -235,7 +341,7 @@ static std::vector<double> table_block(const ElementTables& el, size_t& ov, size
 template <typename T>
 static int upload_geom(dgmi_ctx* c, DevGeom<T>& g, const std::vector<double>& tab, size_t ov,
                        size_t ow, size_t of, size_t os, size_t oc) {
-  g.tab = upload<T>(c, tab);
+  g.tab = upload<T>(c, tab, c->ustream);
   g.off_vtab = ov;
   g.off_wgt = ow;
   g.off_ftab = of;
@@ -249,8 +355,8 @@ static int upload_geom(dgmi_ctx* c, DevGeom<T>& g, const std::vector<double>& ta
     g.cnb = device_f32(c, f.cnb, nz * c->D);
     g.wf = device_f32(c, f.wf, nz);
     g.pen = device_f32(c, f.pen, nz);
-    // the conversions ran on the null stream: done before any caller stream reads them
-    DGMI_HIP(hipStreamSynchronize(nullptr));
+    // the conversions ran on the set-up stream: done before any other stream reads them
+    DGMI_HIP(hipStreamSynchronize(c->ustream));
   }
   // (the face-trace buffers of the two-pass kernels come at their first use)
   DGMI_CHECK(g.tab && g.G && g.cf && g.cnb && g.wf && g.pen, alloc_failure());
@@ -365,7 +471,7 @@ template <> DevGeom<float>& geom<float>(dgmi_ctx* c) { return c->g32; }
 static int compute_bdiag(dgmi_ctx* c, hipStream_t s) {
   Scratch& sc = scratch(c);
   if (!sc.refm) {
-    sc.refm = upload<double>(c, bdiag_refmats(c->el));
+    sc.refm = upload<double>(c, bdiag_refmats(c->el), s);
     DGMI_CHECK(sc.refm, alloc_failure());
   }
   if (!sc.bdiag) {
@@ -685,8 +791,7 @@ static int gmres_solve(dgmi_ctx* c, int inner, const double* b, double* x, doubl
   std::vector<double> bnorm(nc), rnorm(nc), eps(nc);
   std::vector<int> its(nc, 0), outer(nc, 0);
   std::vector<char> done(nc, 0);
-  DGMI_HIP(hipMemcpyAsync(bnorm.data(), k->bnorm, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
-  DGMI_HIP(hipStreamSynchronize(s));
+  DGMI_TRY(d2h(bnorm.data(), k->bnorm, sizeof(double) * nc, s));
   // a packed context isolates a component with non-finite data: it stops at once with
   // relres = NaN and the others go on; a single solve fails with an error
   std::vector<char> failed(nc, 0);
@@ -1030,15 +1135,14 @@ static int build_seg(dgmi_ctx* c) {
     sg.comp_sblk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * scb.size()));
     sg.spartial = static_cast<double*>(c->alloc(sizeof(double) * MDOT_K * std::max(sg.nsblk, 1)));
     DGMI_CHECK(sg.sblk && sg.comp_sblk && sg.spartial, alloc_failure());
-    DGMI_HIP(hipMemcpy(sg.sblk, sbs.data(), sizeof(int32_t) * sbs.size(), hipMemcpyHostToDevice));
-    DGMI_HIP(hipMemcpy(sg.comp_sblk, scb.data(), sizeof(int32_t) * scb.size(),
-                       hipMemcpyHostToDevice));
+    DGMI_TRY(h2d(c, sg.sblk, sbs.data(), sizeof(int32_t) * sbs.size(), c->ustream));
+    DGMI_TRY(h2d(c, sg.comp_sblk, scb.data(), sizeof(int32_t) * scb.size(), c->ustream));
   }
   sg.wblk = static_cast<SegBlk*>(c->alloc(sizeof(SegBlk) * std::max<size_t>(wb.size(), 1)));
   sg.comp_wblk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * wcb.size()));
   DGMI_CHECK(sg.wblk && sg.comp_wblk, alloc_failure());
-  DGMI_HIP(hipMemcpy(sg.wblk, wb.data(), sizeof(SegBlk) * wb.size(), hipMemcpyHostToDevice));
-  DGMI_HIP(hipMemcpy(sg.comp_wblk, wcb.data(), sizeof(int32_t) * wcb.size(), hipMemcpyHostToDevice));
+  DGMI_TRY(h2d(c, sg.wblk, wb.data(), sizeof(SegBlk) * wb.size(), c->ustream));
+  DGMI_TRY(h2d(c, sg.comp_wblk, wcb.data(), sizeof(int32_t) * wcb.size(), c->ustream));
   sg.nblk = (int)blk.size();
   sg.blk = static_cast<SegBlk*>(c->alloc(sizeof(SegBlk) * std::max<size_t>(blk.size(), 1)));
   sg.comp_blk = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * cb.size()));
@@ -1049,10 +1153,9 @@ static int build_seg(dgmi_ctx* c) {
   c->act64 = static_cast<int32_t*>(c->alloc(sizeof(int32_t) * ((c->nele + 63) / 64)));
   DGMI_CHECK(sg.blk && sg.comp_blk && sg.partial && sg.coef && sg.eoff && c->act64,
              alloc_failure());
-  DGMI_HIP(hipMemcpy(sg.eoff, c->comp_eoff.data(), sizeof(int64_t) * (c->ncomp + 1),
-                     hipMemcpyHostToDevice));
-  DGMI_HIP(hipMemcpy(sg.blk, blk.data(), sizeof(SegBlk) * blk.size(), hipMemcpyHostToDevice));
-  DGMI_HIP(hipMemcpy(sg.comp_blk, cb.data(), sizeof(int32_t) * cb.size(), hipMemcpyHostToDevice));
+  DGMI_TRY(h2d(c, sg.eoff, c->comp_eoff.data(), sizeof(int64_t) * (c->ncomp + 1), c->ustream));
+  DGMI_TRY(h2d(c, sg.blk, blk.data(), sizeof(SegBlk) * blk.size(), c->ustream));
+  DGMI_TRY(h2d(c, sg.comp_blk, cb.data(), sizeof(int32_t) * cb.size(), c->ustream));
   return 0;
 }
 
@@ -1075,6 +1178,7 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
   // the largest 32-bit kernel index is (nface*sngi)*nele (face traces): 48 nele in 3D
   DGMI_CHECK(d->nele <= 40000000, "nele above the 32-bit index range of the kernels (4e7)");
   auto* c = new dgmi_ctx();
+  c->ustream = static_cast<hipStream_t>(d->stream);
   c->D = d->D;
   c->nele = d->nele;
   c->nverts = d->nverts;
@@ -1148,9 +1252,9 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
   if (!rc) rc = upload_geom<float>(c, c->g32, tab, ov, ow, of, os, oc);
   if (!rc) {
     std::vector<int32_t> al(c->el.gi_align.begin(), c->el.gi_align.end());
-    c->aln = upload<int32_t>(c, al);
+    c->aln = upload<int32_t>(c, al, c->ustream);
     std::vector<int32_t> fn(c->el.fnodes.begin(), c->el.fnodes.end());
-    c->fnodes = upload<int32_t>(c, fn);
+    c->fnodes = upload<int32_t>(c, fn, c->ustream);
     if (!c->nbface || !c->fcode || !c->aln || !c->fnodes || !c->detJ) {
       set_error(alloc_failure());
       rc = -3;
@@ -1163,6 +1267,7 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
     c->comp_voff = {0, c->nverts};
     rc = build_seg(c);
   }
+  if (!rc) rc = h2d_release(c);  // every staged copy done: the context is ready on any stream
   if (rc) {
     dgmi_ctx_destroy(c);
     return rc;
@@ -1227,6 +1332,7 @@ int dgmi_ctx_create_batch_parts(const dgmi_mesh_desc* d, int32_t ncomp, const in
 int dgmi_ctx_destroy(dgmi_ctx* c) {
   if (!c) return 0;
   if (c->destroy_sync) (void)hipDeviceSynchronize();
+  (void)h2d_release(c);
   if (c->mg.graph64.exec) (void)hipGraphExecDestroy(c->mg.graph64.exec);
   if (c->mg.graph32.exec) (void)hipGraphExecDestroy(c->mg.graph32.exec);
   if (c->mg.capture_stream) (void)hipStreamDestroy(c->mg.capture_stream);
@@ -1346,7 +1452,7 @@ int dgmi_rhs(dgmi_ctx* c, const double* f, const double* u_bc, const double* u_n
   hipStream_t s = (hipStream_t)stream;
   Scratch& sc = scratch(c);
   if (!sc.refm) {
-    sc.refm = upload<double>(c, bdiag_refmats(c->el));
+    sc.refm = upload<double>(c, bdiag_refmats(c->el), s);
     DGMI_CHECK(sc.refm, alloc_failure());
   }
   return launch_rhs(c, sc.refm, c->detJ, f, u_bc, u_neumann, alpha_x_i,
@@ -1368,7 +1474,7 @@ int dgmi_l2_error(dgmi_ctx* c, int dtype, const void* x_num, const void* x_ana, 
         const double v = c->el.n[(size_t)n * c->ngi + g];
         cn[n] += c->el.weight[g] * v * v;
       }
-    sc.l2cn = upload<double>(c, cn);
+    sc.l2cn = upload<double>(c, cn, s);
     sc.l2part = static_cast<double*>(c->alloc(sizeof(double) * 2 * nb));
     DGMI_CHECK(sc.l2cn && sc.l2part, alloc_failure());
   }
@@ -1379,8 +1485,7 @@ int dgmi_l2_error(dgmi_ctx* c, int dtype, const void* x_num, const void* x_ana, 
     DGMI_TRY(launch_l2_error<float>(c, (const float*)x_num, (const float*)x_ana, sc.l2cn,
                                     sc.l2part, nb, s));
   std::vector<double> h(2 * nb);
-  DGMI_HIP(hipMemcpyAsync(h.data(), sc.l2part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s));
-  DGMI_HIP(hipStreamSynchronize(s));
+  DGMI_TRY(d2h(h.data(), sc.l2part, sizeof(double) * 2 * nb, s));
   double l2 = 0.0, linf = 0.0;
   for (int b = 0; b < nb; ++b) {
     l2 += h[b];
@@ -1454,8 +1559,7 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
     DGMI_HIP(hipMemsetAsync(fail, 0, sizeof(int), s));
     DGMI_TRY(launch_build_S_direct(c, sc.bdiag, omega, full, fail, s));
     int hf = 0;
-    DGMI_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, s));
-    DGMI_HIP(hipStreamSynchronize(s));
+    DGMI_TRY(d2h(&hf, fail, sizeof(int), s));
     DGMI_CHECK(hf == 0, "singular diagonal block " + std::to_string(hf - 1));
   }
   DGMI_TRY(launch_pack_S_sym(c, full, s));
@@ -1487,6 +1591,24 @@ int dgmi_smooth_n(dgmi_ctx* c, int dtype, void* x, const void* b, int n, void* s
   DGMI_TRY(check_align(x));
   DGMI_TRY(check_align(b));
   hipStream_t s = (hipStream_t)stream;
+  if (dtype == DGMI_F32 && single_pass_sweeps(c)) {
+    // the preconditioner's single-pass sweeps: the new rows go to a work vector and back,
+    // the last one into x (one copy when n is odd)
+    float* y = vbuf<float>(c, 5);
+    DGMI_CHECK(y, alloc_failure("sweep work vector"));
+    float* xf = static_cast<float*>(x);
+    const float* bf = static_cast<const float*>(b);
+    float* cur = xf;
+    for (int k = 0; k < n; ++k) {
+      float* nxt = cur == xf ? y : xf;
+      DGMI_TRY(launch_sweep_nbx(c, cur, bf, nxt, s));
+      cur = nxt;
+    }
+    if (cur != xf)
+      DGMI_HIP(hipMemcpyAsync(xf, cur, sizeof(float) * c->nele * c->nloc,
+                              hipMemcpyDeviceToDevice, s));
+    return 0;
+  }
   if (dtype == DGMI_F32 && trace_io(c)) {
     int tin = -1, tout = 1;
     for (int k = 0; k < n; ++k) {
@@ -1562,11 +1684,11 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
       DGMI_CHECK(mult[i] > 0, "P1 vertex not used by any element");
       rw[pos[i]] = 1.0 / mult[i];
     }
-    mg.rptr = upload<int32_t>(c, rptr);
-    mg.rent = upload<int32_t>(c, rent);
-    mg.cellpos = upload<int32_t>(c, cellpos);
-    mg.rw64 = upload<double>(c, rw);
-    mg.rw32 = upload<float>(c, rw);
+    mg.rptr = upload<int32_t>(c, rptr, s);
+    mg.rent = upload<int32_t>(c, rent, s);
+    mg.cellpos = upload<int32_t>(c, cellpos, s);
+    mg.rw64 = upload<double>(c, rw, s);
+    mg.rw32 = upload<float>(c, rw, s);
     DGMI_CHECK(mg.rptr && mg.rent && mg.cellpos && mg.rw64 && mg.rw32, alloc_failure());
     mg.cg_nonods = ncg;
     return 0;
@@ -1579,7 +1701,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   DGMI_TRY(build_transfers(pos, c->nverts));
   std::vector<int32_t> whichc_sfc(c->nverts);
   for (int64_t i = 0; i < c->nverts; ++i) whichc_sfc[pos[i]] = whichc[i];
-  int32_t* whichc_d = upload<int32_t>(c, whichc_sfc);
+  int32_t* whichc_d = upload<int32_t>(c, whichc_sfc, s);
   DGMI_CHECK(whichc_d, alloc_failure());
   // coloured probing of RAR (z02/volume_mf_diff.py:1696-1754): entries grouped by the
   // colour of their column; each row holds at most one entry per colour
@@ -1598,8 +1720,8 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
         f2[cc]++;
       }
   }
-  int32_t* spidx_d = upload<int32_t>(c, spidx);
-  int32_t* rowpos_d = upload<int32_t>(c, rowpos);
+  int32_t* spidx_d = upload<int32_t>(c, spidx, s);
+  int32_t* rowpos_d = upload<int32_t>(c, rowpos, s);
   double* vals_d = static_cast<double*>(c->alloc(sizeof(double) * mg.ncola));
   double* probe = vbuf<double>(c, 0);
   double* aprobe = vbuf<double>(c, 1);
@@ -1613,9 +1735,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
                                   off[cc + 1] - off[cc], vals_d, s));
   }
   mg.rar.assign(mg.ncola, 0.0);
-  DGMI_HIP(hipMemcpyAsync(mg.rar.data(), vals_d, sizeof(double) * mg.ncola,
-                          hipMemcpyDeviceToHost, s));
-  DGMI_HIP(hipStreamSynchronize(s));
+  DGMI_TRY(d2h(mg.rar.data(), vals_d, sizeof(double) * mg.ncola, s));
   mark(PH_PROBE);
   // SFC levels of every component (z00-fortran-src/compact_fcode.f90:7-225)
   SfcLayout lay;
@@ -1695,6 +1815,7 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     }
   if (nc > 1) DGMI_TRY(build_transfers(pos, total));
   DGMI_TRY(upload_sfc_levels(c, lay, s));
+  DGMI_TRY(h2d_release(c));
   mg.small_level = lay.smax;
   mg.small_level32 = lay.start32[lay.order[0]];
   mg.ready = true;

@@ -314,6 +314,9 @@ struct dgmi_ctx {
   size_t alloc_limit = 0;
   // host wall-clock seconds of the set-up phases (dgmi_setup_timing)
   double t_setup[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // stream of the set-up work of context creation (dgmi_mesh_desc::stream)
+  hipStream_t ustream = nullptr;
+  void* stager = nullptr;  // pinned staging buffers of h2d (dgmi_api.hip)
   void* alloc(size_t bytes);
 };
 
@@ -326,6 +329,19 @@ namespace dgmi {
 // the allocation is tried once more; dgmi_release_cached_memory frees them on request.
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
+// Set-up transfers between host memory and the device.  A copy from pageable host memory
+// is staged by the HIP runtime in pieces, each of which waits in a hardware queue: where
+// streams share one (more streams than GPU_MAX_HW_QUEUES), behind every kernel another
+// stream queued there before it, which in a serving process can be a whole solve.  From
+// pinned memory the copy engine moves the data without waiting there.  h2d copies the
+// host data into the context's two pinned staging buffers (filled in turn, so that the
+// host fills one while the other's copies run) and queues the copies on s; the host data
+// may be reused at once, the device data are ready in the order of s.  h2d_release waits
+// for the staged copies and gives the staging buffers back.  d2h copies device data into
+// host memory through a pinned buffer and returns when they are there.
+int h2d(dgmi_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s);
+int h2d_release(dgmi_ctx* c);
+int d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
 // set-up phases timed into dgmi_ctx::t_setup
 enum SetupPhase {
   PH_TOPOLOGY = 0, PH_GEOMETRY, PH_UPLOAD, PH_SMOOTHER, PH_GRAPH, PH_PROBE, PH_LEVELS, PH_LAYOUT

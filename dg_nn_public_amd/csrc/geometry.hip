@@ -250,18 +250,17 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(
 }
 
 template <typename T>
-static int to_dev(const std::vector<T>& h, T** d, hipStream_t s) {
+static int to_dev(dgmi_ctx* c, const std::vector<T>& h, T** d, hipStream_t s) {
   *d = static_cast<T*>(dev_alloc(sizeof(T) * std::max<size_t>(h.size(), 1)));
   if (!*d) return -3;
-  if (!h.empty()) DGMI_HIP(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
-  return 0;
+  return h2d(c, *d, h.data(), sizeof(T) * h.size(), s);
 }
 
 int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64_t* part_off,
                           int nparts, int face_kappa, double* max_dev, int64_t* bad_elem) {
   const int D = c->D, NF = c->nface;
   const int64_t nele = c->nele, nz = nele * NF;
-  hipStream_t s = nullptr;
+  hipStream_t s = c->ustream;
   DevGeom<double>& g = c->g64;
   const size_t ng = (size_t)c->ng * nele;
   g.G = static_cast<double*>(c->alloc(sizeof(double) * ng));
@@ -319,7 +318,7 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
       set_error(alloc_failure("geometry"));
       return -3;
     }
-    DGMI_HIP(hipMemcpy(c->nperm, perm.data(), perm.size(), hipMemcpyHostToDevice));
+    if (h2d(c, c->nperm, perm.data(), perm.size(), s)) return -3;
   }
   std::vector<int32_t> nb32(nz), bt32(nz), al32(nz);
   for (int64_t z = 0; z < nz; ++z) {
@@ -348,8 +347,7 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
   auto copy_parts = [&]() {
     for (int q = 0; q < nparts; ++q) {
       const size_t n = (size_t)(part_off[q + 1] - part_off[q]) * per_elem;
-      if (n && hipMemcpyAsync(dx + (size_t)part_off[q] * per_elem, x_parts[q],
-                              sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess)
+      if (n && h2d(c, dx + (size_t)part_off[q] * per_elem, x_parts[q], sizeof(double) * n, s))
         return -3;
     }
     return 0;
@@ -359,12 +357,12 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
     return *d ? 0 : -3;
   };
   if (step(take(&dx, sizeof(double) * nx)) && step(copy_parts()) &&
-      step(to_dev(c->kappa, &dk, s)) && step(to_dev(nb32, &dnb, s)) && step(to_dev(bt32, &dbt, s)) &&
-      step(to_dev(al32, &dal, s)) && step(take(&dmax, sizeof(double) * nb)) &&
+      step(to_dev(c, c->kappa, &dk, s)) && step(to_dev(c, nb32, &dnb, s)) &&
+      step(to_dev(c, bt32, &dbt, s)) && step(to_dev(c, al32, &dal, s)) && step(take(&dmax, sizeof(double) * nb)) &&
       step(take(&dbad, sizeof(int))) && step(take(&dbadf, sizeof(int)))) {
     const int big = INT_MAX;
-    if (step(hipMemcpyAsync(dbad, &big, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess ? 0 : -3) &&
-        step(hipMemcpyAsync(dbadf, &big, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess ? 0 : -3)) {
+    if (step(h2d(c, dbad, &big, sizeof(int), s) ? -3 : 0) &&
+        step(h2d(c, dbadf, &big, sizeof(int), s) ? -3 : 0)) {
       if (D == 3)
         hipLaunchKernelGGL(k_geometry<3>, dim3(nb), dim3(GEO_THREADS), 0, s, (int)nele, dx, dk,
                            dnb, dbt, dal, t, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,
@@ -376,13 +374,9 @@ int build_geometry_device(dgmi_ctx* c, const double* const* x_parts, const int64
       std::vector<double> bm(nb);
       int hbad = INT_MAX, hbadf = INT_MAX;
       if (step(hipGetLastError() == hipSuccess ? 0 : -3) &&
-          step(hipMemcpyAsync(bm.data(), dmax, sizeof(double) * nb, hipMemcpyDeviceToHost, s) ==
-                       hipSuccess ? 0 : -3) &&
-          step(hipMemcpyAsync(&hbad, dbad, sizeof(int), hipMemcpyDeviceToHost, s) ==
-                       hipSuccess ? 0 : -3) &&
-          step(hipMemcpyAsync(&hbadf, dbadf, sizeof(int), hipMemcpyDeviceToHost, s) ==
-                       hipSuccess ? 0 : -3) &&
-          step(hipStreamSynchronize(s) == hipSuccess ? 0 : -3)) {
+          step(d2h(bm.data(), dmax, sizeof(double) * nb, s) ? -3 : 0) &&
+          step(d2h(&hbad, dbad, sizeof(int), s) ? -3 : 0) &&
+          step(d2h(&hbadf, dbadf, sizeof(int), s) ? -3 : 0)) {
         double m = 0.0;
         for (double v : bm) m = std::max(m, v);
         *max_dev = m;

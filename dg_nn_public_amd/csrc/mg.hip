@@ -627,21 +627,21 @@ template size_t lds_bytes_from<float>(const std::vector<int64_t>&, const std::ve
 // ---------------------------------------------------------------------------------------
 template <typename T> static T* vec(void* p) { return reinterpret_cast<T*>(p); }
 
-// device copy of a host vector (converted to T); nullptr on failure
+// device copy of a host vector (converted to T), queued on s; nullptr on failure
 template <typename T, typename U>
-static T* upload_vec(dgmi_ctx* c, const std::vector<U>& h) {
+static T* upload_vec(dgmi_ctx* c, const std::vector<U>& h, hipStream_t s) {
   T* d = static_cast<T*>(c->alloc(sizeof(T) * std::max<size_t>(h.size(), 1)));
   if (!d) return nullptr;
   if (!h.empty()) {
-    hipError_t e;
+    int rc;
     if constexpr (std::is_same<T, U>::value) {
-      e = hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+      rc = h2d(c, d, h.data(), sizeof(T) * h.size(), s);
     } else {
       const std::vector<T> t(h.begin(), h.end());
-      e = hipMemcpy(d, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+      rc = h2d(c, d, t.data(), sizeof(T) * t.size(), s);
     }
-    if (e != hipSuccess) {
-      note_alloc_error(std::string("hipMemcpy to device: ") + hipGetErrorString(e));
+    if (rc) {
+      note_alloc_error("copy to device failed");
       return nullptr;
     }
   }
@@ -1085,8 +1085,8 @@ struct LevelInverse {
 // FP64 blocked Gauss-Jordan inverses of the given levels, formed together on stream s and
 // written as FP32 row-major; returns 0 and fails[i] = 1 for a level that is not positive
 // definite, or an error code (the stream is synchronised on return).
-static int invert_levels(const std::vector<LevelInverse>& jobs, std::vector<int>& fails,
-                         hipStream_t s) {
+static int invert_levels(dgmi_ctx* c, const std::vector<LevelInverse>& jobs,
+                         std::vector<int>& fails, hipStream_t s) {
   const int nm = (int)jobs.size();
   fails.assign(nm, 0);
   if (nm == 0) return 0;
@@ -1137,17 +1137,16 @@ static int invert_levels(const std::vector<LevelInverse>& jobs, std::vector<int>
       mats[i].rp = drp + ro;
       mats[i].col = dcol + vo;
       mats[i].val = dval + vo;
-      chk(hipMemcpyAsync(drp + ro, h.rowptr[l].data(), sizeof(int32_t) * (mats[i].n + 1),
-                         hipMemcpyHostToDevice, s));
-      if (k) {
-        chk(hipMemcpyAsync(dcol + vo, h.col[l].data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, s));
-        chk(hipMemcpyAsync(dval + vo, h.val[l].data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      }
+      if (h2d(c, drp + ro, h.rowptr[l].data(), sizeof(int32_t) * (mats[i].n + 1), s) && !rc)
+        rc = -3;
+      if (k && rc == 0 && (h2d(c, dcol + vo, h.col[l].data(), sizeof(int32_t) * k, s) ||
+                           h2d(c, dval + vo, h.val[l].data(), sizeof(double) * k, s)))
+        rc = -3;
       ao += (size_t)mats[i].N * mats[i].N;
       vo += k;
       ro += (size_t)mats[i].n + 1;
     }
-    if (rc == 0 && chk(hipMemcpyAsync(dm, mats.data(), sizeof(GjMat) * nm, hipMemcpyHostToDevice, s)) &&
+    if (rc == 0 && (h2d(c, dm, mats.data(), sizeof(GjMat) * nm, s) == 0 || (rc = -3, false)) &&
         chk(hipMemsetAsync(dfail, 0, sizeof(int) * nm, s)) &&
         chk(hipMemsetAsync(da, 0, sizeof(double) * asz, s))) {
       const dim3 blk(256);
@@ -1164,9 +1163,7 @@ static int invert_levels(const std::vector<LevelInverse>& jobs, std::vector<int>
       hipLaunchKernelGGL(k_d2f_block, dim3((unsigned)((nn + 255) / 256), 1, nm), blk, 0, s,
                          (const GjMat*)dm);
       chk(hipGetLastError());
-      if (rc == 0 && chk(hipMemcpyAsync(fails.data(), dfail, sizeof(int) * nm,
-                                        hipMemcpyDeviceToHost, s)))
-        chk(hipStreamSynchronize(s));
+      if (rc == 0 && d2h(fails.data(), dfail, sizeof(int) * nm, s)) rc = -3;
     }
   }
   (void)hipStreamSynchronize(s);  // nothing in flight reads the work space any more
@@ -1183,7 +1180,7 @@ static float* dense_inverse_device(dgmi_ctx* c, const SfcLevelsHost& h, int l, h
   float* out = static_cast<float*>(c->alloc(sizeof(float) * (size_t)n * n));
   if (!out) return nullptr;
   std::vector<int> fails;
-  if (invert_levels({LevelInverse{&h, l, out}}, fails, s) != 0 || fails[0]) return nullptr;
+  if (invert_levels(c, {LevelInverse{&h, l, out}}, fails, s) != 0 || fails[0]) return nullptr;
   return out;
 }
 
@@ -1261,10 +1258,10 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
     const int64_t rows = R;
     lv.n[l] = rows;
     lv.nnz[l] = NZ;
-    lv.rowptr[l] = upload_vec<int32_t>(c, rp);
-    lv.col[l] = upload_vec<int32_t>(c, col);
-    lv.val64[l] = upload_vec<double>(c, val);
-    lv.diag64[l] = upload_vec<double>(c, diag);
+    lv.rowptr[l] = upload_vec<int32_t>(c, rp, s);
+    lv.col[l] = upload_vec<int32_t>(c, col, s);
+    lv.val64[l] = upload_vec<double>(c, val, s);
+    lv.diag64[l] = upload_vec<double>(c, diag, s);
     lv.val32[l] = static_cast<float*>(c->alloc(sizeof(float) * std::max<int64_t>(NZ, 1)));
     lv.diag32[l] = static_cast<float*>(c->alloc(sizeof(float) * std::max<int64_t>(rows, 1)));
     lv.r[l] = c->alloc(sizeof(double) * std::max<int64_t>(rows, 1));
@@ -1300,12 +1297,12 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
       pvec += 3 * h.n[l];
     }
   }
-  lv.pool_rp = upload_vec<int32_t>(c, prp);
-  lv.pool_col = upload_vec<int32_t>(c, pcol);
-  lv.pool_val64 = upload_vec<double>(c, pval);
-  lv.pool_diag64 = upload_vec<double>(c, pdiag);
-  lv.pool_val32 = upload_vec<float>(c, pval);
-  lv.pool_diag32 = upload_vec<float>(c, pdiag);
+  lv.pool_rp = upload_vec<int32_t>(c, prp, s);
+  lv.pool_col = upload_vec<int32_t>(c, pcol, s);
+  lv.pool_val64 = upload_vec<double>(c, pval, s);
+  lv.pool_diag64 = upload_vec<double>(c, pdiag, s);
+  lv.pool_val32 = upload_vec<float>(c, pval, s);
+  lv.pool_diag32 = upload_vec<float>(c, pdiag, s);
   lv.pool_vec = c->alloc(sizeof(double) * std::max<int64_t>(pvec, 1));
   if (!lv.pool_rp || !lv.pool_col || !lv.pool_val64 || !lv.pool_diag64 || !lv.pool_val32 ||
       !lv.pool_diag32 || !lv.pool_vec)
@@ -1409,12 +1406,12 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
         which.push_back(b);
       }
     std::vector<int> fails;
-    if (int rc = invert_levels(jobs, fails, s)) return rc;
+    if (int rc = invert_levels(c, jobs, fails, s)) return rc;
     for (size_t i = 0; i < which.size(); ++i)
       if (fails[i]) lvl[which[i]] = -1;  // not positive definite: this component sweeps every level
   }
   for (int b = 0; b < nc; ++b) cinfo[2 * b + 1] = lvl[b];
-  lv.cinfo32 = upload_vec<int64_t>(c, cinfo);
+  lv.cinfo32 = upload_vec<int64_t>(c, cinfo, s);
   if (!lv.cinfo32) return fail();
   mg.exact32.assign(nc, -1);
   for (int b = 0; b < nc; ++b) mg.exact32[lay.order[b]] = (int)cinfo[2 * b + 1];
@@ -1439,14 +1436,14 @@ int upload_sfc_levels(dgmi_ctx* c, const SfcLayout& lay, hipStream_t s) {
         break;
       }
   }
-  lv.desc64 = upload_vec<LevelDesc<double>>(c, d64);
-  lv.desc32 = upload_vec<LevelDesc<float>>(c, d32);
-  lv.rng64 = upload_vec<int32_t>(c, rng64);
-  lv.rng32 = upload_vec<int32_t>(c, rng32);
-  lv.img64 = upload_vec<unsigned char>(c, img64);
-  lv.img32 = upload_vec<unsigned char>(c, img32);
-  lv.imginfo64 = upload_vec<int64_t>(c, info64);
-  lv.imginfo32 = upload_vec<int64_t>(c, info32);
+  lv.desc64 = upload_vec<LevelDesc<double>>(c, d64, s);
+  lv.desc32 = upload_vec<LevelDesc<float>>(c, d32, s);
+  lv.rng64 = upload_vec<int32_t>(c, rng64, s);
+  lv.rng32 = upload_vec<int32_t>(c, rng32, s);
+  lv.img64 = upload_vec<unsigned char>(c, img64, s);
+  lv.img32 = upload_vec<unsigned char>(c, img32, s);
+  lv.imginfo64 = upload_vec<int64_t>(c, info64, s);
+  lv.imginfo32 = upload_vec<int64_t>(c, info32, s);
   if (!lv.desc64 || !lv.desc32 || !lv.rng64 || !lv.rng32 || !lv.img64 || !lv.img32 ||
       !lv.imginfo64 || !lv.imginfo32)
     return fail();

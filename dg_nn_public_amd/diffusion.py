@@ -195,7 +195,7 @@ class DGDiffusion:
             x_ref_in=x.ctypes.data_as(_lib._DP), cells=cells.ctypes.data_as(_lib._I64P),
             verts=verts.ctypes.data_as(_lib._DP), kappa=kappa.ctypes.data_as(_lib._DP),
             bctype=bc.ctypes.data_as(_lib._I64P) if bc is not None else None, eta_e=eta_e,
-            face_kappa={"mean": 0, "harmonic": 1}[face_kappa])
+            face_kappa={"mean": 0, "harmonic": 1}[face_kappa], stream=_stream().value)
         ctx = C.c_void_p()
         self.ncomp = 1
         if components is None:

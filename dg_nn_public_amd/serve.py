@@ -118,6 +118,7 @@ class _Job:
     thread: threading.Thread | None = None
     alloc_failed: bool = False
     published: bool = False  # results handed out: the job no longer holds a slot
+    stream: torch.cuda.Stream | None = None
 
 
 class Server:
@@ -144,6 +145,11 @@ class Server:
         self.bpe = dict(_BYTES_PER_ELEM)
         self._lock = threading.Lock()
         self._cv = threading.Condition(self._lock)
+        # one stream per job slot, created together: HIP spreads streams over its hardware
+        # queues in creation order, so streams created one after another get different
+        # queues, and work on streams that share a queue runs in queue order (a job's
+        # set-up would wait behind another job's queued solve)
+        self._streams = []
 
     def _estimate(self, req):
         return int(self.bpe[req.mesh.D] * req.mesh.cells.shape[0]) + (64 << 20)
@@ -153,7 +159,7 @@ class Server:
         used, ne = 0.0, 0
         try:
             torch.cuda.set_device(self.device)
-            stream = torch.cuda.Stream(device=self.device)
+            stream = job.stream
             with torch.cuda.stream(stream):
                 bt = None
                 try:
@@ -187,6 +193,8 @@ class Server:
                     with self._cv:  # publish now: the slot frees before the memory goes back
                         results.extend(out)
                         job.published = True
+                        # the next job may queue its work behind this job's last host steps
+                        self._streams.append(stream)
                         self._cv.notify_all()
                 finally:
                     if bt is not None:
@@ -218,6 +226,7 @@ class Server:
                 if not job.published:
                     results.extend(out)
                     job.published = not job.alloc_failed
+                    self._streams.append(job.stream)
                 self._cv.notify_all()
 
     def _failed(self, job, t0, msg):
@@ -234,6 +243,8 @@ class Server:
         cap = self.capacity
         if cap is None:  # what one job could hold at most (the library caches freed blocks)
             cap = 0.9 * float(torch.cuda.mem_get_info(torch.device(self.device))[1])
+        if not self._streams:
+            self._streams = [torch.cuda.Stream(device=self.device) for _ in range(self.max_jobs)]
         t0 = time.perf_counter()
         jid = 0
         with self._cv:
@@ -305,6 +316,10 @@ class Server:
 
     def _admit(self, reqs, jid, t0, jobs, results):
         job = _Job(jid, list(reqs), sum(self._estimate(r) for r in reqs))
+        # a free slot's stream (a job re-queued after an allocation failure may find none
+        # free: then a stream of its own)
+        job.stream = (self._streams.pop(0) if self._streams
+                      else torch.cuda.Stream(device=self.device))
         job.admitted = time.perf_counter() - t0
         job.thread = threading.Thread(target=self._run_job, args=(job, t0, results), daemon=True)
         jobs.append(job)

@@ -50,6 +50,8 @@ typedef struct {
                               reference's (z05/volume_mf_um_diff_varprop.py:567);
                               1 = harmonic mean (symmetric weighted interior penalty,
                               coercive at any contrast)                                  */
+  void* stream;            /* hipStream_t of the creation's device work and copies, or
+                              NULL (the null stream); creation returns when it is done   */
 } dgmi_mesh_desc;
 
 /* Context: topology (z02/mesh_init.py:260-658), reference-element tables
