@@ -19,6 +19,8 @@
 // 4: r = +A x; 5: x = (1 - omega) x + S (b - A_off x), where A_off is A without its
 // diagonal blocks; 6: x = S b (a sweep from x = 0).  Mode 5 equals mode 2 when S = omega A_ee^-1 (the exact block
 // smoother): it needs no volume term and only the neighbour half of the face terms.
+#include <utility>
+
 #include "dd.h"
 #include "dgmi.h"
 #include "dgmi_internal.h"
@@ -403,6 +405,325 @@ __global__ __launch_bounds__(64 * (D + 1),
         face_traces<T, D>(f, fc, xn, xf, cc, [&](int g, T u, T dn) {
           tout[(f * M::SG + g) * nele + e] = make_trace<typename Vec2<T>::type>(u, dn);
         });
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// FP32 smoothing sweep of the exact block smoother in one pass, the form the FP32
+// preconditioner runs:  x_new = (1 - omega) x + S (b - A_off x),  S = omega A_ee^-1 packed
+// symmetric, A_off = the face coupling to the neighbours.  A block of NF waves owns 64
+// elements (one per lane):
+//   0. the block's rows of b and x (64 x NL contiguous values each) are read with
+//      coalesced vector loads and transposed into LDS, so no wave reads a row of its own
+//      from global memory (a lane's row is NL contiguous values, 80 B in 3D);
+//   1. wave w forms the neighbour term of face w: the neighbour's row (the only gather),
+//      relabelled into the face's frame, its traces, the flux and its lift (as the
+//      single-pass k_residual_w);
+//   2. every wave forms the element's residual row r = b - sum of the face partials, then
+//      its share of S r from a contiguous quarter of the packed entries (each entry read
+//      once per block, for both S[a][b] and S[b][a]);
+//   3. the partial products are summed per row, damped x added, and the block's new rows
+//      written with coalesced vector stores.
+// x_new must be another vector than x (the neighbours read x).
+template <int NL, int IDX>
+__device__ __forceinline__ void sym_term(const float* __restrict__ S, int nele, int e,
+                                         const float* r, float* y) {
+  constexpr int A = [] {
+    int i = IDX, a = 0;
+    while (i >= NL - a) i -= NL - a++;
+    return a;
+  }();
+  constexpr int B = [] {
+    int i = IDX, a = 0;
+    while (i >= NL - a) i -= NL - a++;
+    return a + i;
+  }();
+  static_assert(sym_index<NL>(A, B) == IDX, "packed index");
+  const float s = S[(size_t)IDX * nele + e];
+  y[A] += s * r[B];
+  if constexpr (A != B) y[B] += s * r[A];
+}
+template <int NL, int P0, int... J>
+__device__ __forceinline__ void sym_part(const float* __restrict__ S, int nele, int e,
+                                         const float* r, float* y,
+                                         std::integer_sequence<int, J...>) {
+  (sym_term<NL, P0 + J>(S, nele, e, r, y), ...);
+}
+// wave W of NF: packed entries [W NS / NF, (W+1) NS / NF)
+template <int NL, int NF, int W>
+__device__ __forceinline__ void sym_share(const float* __restrict__ S, int nele, int e,
+                                          const float* r, float* y) {
+  constexpr int NS = sym_size<NL>();
+  constexpr int P0 = W * NS / NF, P1 = (W + 1) * NS / NF;
+  sym_part<NL, P0>(S, nele, e, r, y, std::make_integer_sequence<int, P1 - P0>{});
+}
+
+template <int D, int MODE>
+__global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void k_staged(
+    int nele, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ out,
+    const float* __restrict__ vtab, const float* __restrict__ wgt, const float* __restrict__ G,
+    const float* __restrict__ sw, const float* __restrict__ cf, const float* __restrict__ wf,
+    const float* __restrict__ pen, const int32_t* __restrict__ nbface,
+    const int32_t* __restrict__ fcode, const float* __restrict__ S,
+    const int32_t* __restrict__ act, float omega, const float* __restrict__ fc,
+    const int32_t* __restrict__ fnodes, const float* __restrict__ cnb,
+    const uint8_t* __restrict__ nperm) {
+  using M = Dm<D>;
+  using F = FaceDims<D>;
+  using RV = RowVec<float, M::NL>;
+  using V = typename RV::V;
+  static_assert(MODE == 0 || MODE == 1 || MODE == 4 || MODE == 5 || MODE == 6, "mode");
+  constexpr bool SWEEP = MODE == 5 || MODE == 6;  // x_new = (1 - omega) x + S r
+  constexpr bool ZERO = MODE == 6;                 // from x = 0: x_new = S b
+  constexpr bool APPLY = !SWEEP;                   // the whole operator (volume and faces)
+  constexpr bool READB = MODE == 1 || SWEEP;
+  constexpr int NL = M::NL, NF = M::NF, W = RV::W;
+  constexpr int GPW = M::NGI / NF;          // volume points per wave
+  constexpr int NPW = (NL + NF - 1) / NF;   // output rows per wave
+  constexpr int NPERM = D == 3 ? 24 : 6;    // vertex permutations
+  constexpr int NT = 64 * NF;               // threads
+  constexpr int NVEC = 64 * NL / W;         // vectors of a block's rows
+  static_assert(M::NGI % NF == 0, "volume points must split evenly over the waves");
+  const int grp = act ? (int)blockIdx.x : xcd_group(blockIdx.x, gridDim.x);
+  if (act && !act[grp]) return;  // whole block: no barrier is reached
+  __shared__ float red[NF][NL][64];       // face relabelling, partial rows, partial S r
+  __shared__ float bs[READB ? NL : 1][65];  // the block's rows of b, element per column
+  __shared__ float xs[NL][65];            // rows of x, then of the output
+  __shared__ uint8_t ptab[ZERO ? 1 : NPERM * NL];
+  // (applies) the P2 mass matrix times the P2 coefficients of d u / d xi_d, per direction
+  __shared__ float vtmp[APPLY ? D : 1][APPLY ? F::NP2 : 1][64];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int e0 = grp * 64;
+  const int e = e0 + lane;
+  const bool valid = e < nele;
+  const int ec = valid ? e : nele - 1;
+  const int nvec = (nele - e0 < 64 ? nele - e0 : 64) * NL / W;  // vectors of real elements
+  // ---- 0. rows of b and x into LDS (zeros past the last element)
+  {
+    const V* bv = reinterpret_cast<const V*>(b + (size_t)e0 * NL);
+    const V* xv = reinterpret_cast<const V*>(x + (size_t)e0 * NL);
+#pragma unroll
+    for (int i0 = 0; i0 < NVEC; i0 += NT) {
+      const int i = i0 + tid;
+      if (i < NVEC) {
+        float tb[W], tx[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) tb[k] = tx[k] = 0.0f;
+        if (i < nvec) {
+          if constexpr (READB) {
+            const V v = bv[i];
+            const float* pv = reinterpret_cast<const float*>(&v);
+#pragma unroll
+            for (int k = 0; k < W; ++k) tb[k] = pv[k];
+          }
+          if constexpr (!ZERO) {
+            const V v = xv[i];
+            const float* pv = reinterpret_cast<const float*>(&v);
+#pragma unroll
+            for (int k = 0; k < W; ++k) tx[k] = pv[k];
+          }
+        }
+        const int el = (W * i) / NL, n = (W * i) % NL;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+          if constexpr (READB) bs[n + k][el] = tb[k];
+          xs[n + k][el] = tx[k];
+        }
+      }
+    }
+    if constexpr (!ZERO)
+      for (int i = tid; i < NPERM * NL; i += NT) ptab[i] = nperm[i];
+  }
+  __syncthreads();
+  if constexpr (!ZERO) {
+    float acc[NL];
+#pragma unroll
+    for (int n = 0; n < NL; ++n) acc[n] = 0.0f;
+    // ---- 1a. (applies) volume term, first half: d u / d xi_w is a P2 polynomial; its
+    // P2 coefficients g_w = VD_w x, times the P2 mass matrix, go to the other waves
+    // (wave w < D takes direction w; the volume integral is
+    //   r_n = sum_{d,e} G_de sum_ij VD_d[i][n] VM[i][j] VD_e[j][m] x_m,
+    // exact for the affine element, the rule being exact for P2 x P2)
+    if constexpr (APPLY) {
+      if (w < D) {
+        const float* vd = fc + F::VD + w * F::NP2 * NL;
+        const float* vm = fc + F::VM;
+        float xv[NL], gco[F::NP2];
+#pragma unroll
+        for (int n = 0; n < NL; ++n) xv[n] = xs[n][lane];
+#pragma unroll
+        for (int i = 0; i < F::NP2; ++i) {
+          float s = 0.0f;
+#pragma unroll
+          for (int n = 0; n < NL; ++n) s += vd[i * NL + n] * xv[n];
+          gco[i] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < F::NP2; ++i) {
+          float s = 0.0f;
+#pragma unroll
+          for (int j = 0; j < F::NP2; ++j) s += vm[i * F::NP2 + j] * gco[j];
+          vtmp[w][i][lane] = s;
+        }
+      }
+    }
+    // ---- 1b. face w, without lane-divergent branches: a face without a term here (or a
+    // lane past the last element) gets weight 0, and a boundary face reads its own element
+    // as the "neighbour" with the neighbour's values multiplied by 0.  The neighbour's
+    // traces are formed in this face's frame: its row relabelled so that the shared face
+    // is its face f with this element's vertex order (node n of the relabelled neighbour
+    // is its node nperm[pid][n]; cnb = its J^-T n in that frame), so its face point g is
+    // this side's point g and every trace table read is wave-uniform.  Interior and
+    // Dirichlet fluxes share one formula (a = 1/2 and the neighbour's traces on an
+    // interior face, a = 1 and no neighbour on a Dirichlet face):
+    //   q_s = w (-a (du/dn - du'/dn) + pen (u - u')),  q_d = -w a (u - u'),
+    // and the sweep (A without its diagonal blocks) keeps the neighbour's part only:
+    //   q_s = w (a du'/dn - pen u'),  q_d = w a u'.
+    FaceLift<float, D> fl;
+    fl.zero();
+    const int f = w;
+    {
+      const int z = ec * NF + f;
+      const int code = fcode[z];
+      const int type = code & 15;
+      const bool interior = type == 0;
+      const bool on = valid && (SWEEP ? interior : type < 2);
+      float c[D], cn[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        c[d] = cf[(f * D + d) * nele + ec];
+        cn[d] = cnb[(f * D + d) * nele + ec];
+      }
+      const float wfe = on ? wf[f * nele + ec] : 0.0f;
+      const float pe = pen[f * nele + ec];
+      const float half = interior ? 0.5f : 1.0f;
+      const float keep = interior ? 1.0f : 0.0f;
+      const int nbe = interior ? nbface[z] / NF : ec;
+      const int pid = interior ? (code >> 8) & 31 : 0;  // 0: the identity
+      // the neighbour's row, relabelled through this lane's column of red[w]: its
+      // face-node values and c' . grad(u') at the face's quadratic nodes
+      float xrf[F::NFN], dkn[F::NK];
+      {
+        float xn[NL];
+        load_row<float, NL>(x + (size_t)nbe * NL, xn);
+#pragma unroll
+        for (int n = 0; n < NL; ++n) red[w][n][lane] = xn[n];
+        const uint8_t* pp = ptab + pid * NL;
+        float xr[NL];
+#pragma unroll
+        for (int n = 0; n < NL; ++n) xr[n] = red[w][pp[n]][lane];
+#pragma unroll
+        for (int i = 0; i < F::NFN; ++i) xrf[i] = keep * red[w][pp[fnodes[f * F::NFN + i]]][lane];
+        face_dk<float, D>(f, fc, xr, cn, dkn);
+#pragma unroll
+        for (int k2 = 0; k2 < F::NK; ++k2) dkn[k2] *= keep;
+      }
+      if constexpr (APPLY) {
+        // own traces in node form: the jump u - u' at the face's nodes and
+        // du/dn - du'/dn at its quadratic nodes
+        float xv[NL], jump[F::NFN], ddk[F::NK];
+#pragma unroll
+        for (int n = 0; n < NL; ++n) xv[n] = xs[n][lane];
+        face_dk<float, D>(f, fc, xv, c, ddk);
+#pragma unroll
+        for (int k2 = 0; k2 < F::NK; ++k2) ddk[k2] -= dkn[k2];
+#pragma unroll
+        for (int i = 0; i < F::NFN; ++i) jump[i] = xs[fnodes[f * F::NFN + i]][lane] - xrf[i];
+        fl.set_nodes(f, fc, jump, ddk, wfe * pe, -half * wfe, -half * wfe);
+      } else {
+        fl.set_nodes(f, fc, xrf, dkn, -pe * wfe, 0.5f * wfe, 0.5f * wfe);
+      }
+      fl.finish(f, fc, c, acc);
+    }
+    if constexpr (APPLY) {
+      // ---- 1c. volume term, second half: h_w = sum_e G_we (VM g_e), r += VD_w^T h_w
+      __syncthreads();  // vtmp of every direction
+      if (w < D) {
+        const float* vd = fc + F::VD + w * F::NP2 * NL;
+        float gr[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) gr[d] = G[(w == d ? w : D - 1 + w + d) * nele + ec];
+#pragma unroll
+        for (int i = 0; i < F::NP2; ++i) {
+          float h = 0.0f;
+#pragma unroll
+          for (int d = 0; d < D; ++d) h += gr[d] * vtmp[d][i][lane];
+#pragma unroll
+          for (int n = 0; n < NL; ++n) acc[n] += vd[i * NL + n] * h;
+        }
+      }
+    }
+    // (red[w] is next written by this lane only, after its reads above: program order)
+#pragma unroll
+    for (int n = 0; n < NL; ++n) red[w][n][lane] = acc[n];
+#pragma unroll
+    for (int i = 0; i < F::NFN; ++i) red[w][fnodes[w * F::NFN + i]][lane] += fl.accf[i];
+    __syncthreads();
+  }
+  if constexpr (APPLY) {
+    // ---- 2. rows of this wave: r = -A x, b - A x or A x
+#pragma unroll
+    for (int k = 0; k < NPW; ++k) {
+      const int n = w * NPW + k;
+      if (NL % NF == 0 || n < NL) {
+        float s = 0.0f;
+#pragma unroll
+        for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
+        if constexpr (MODE == 0) s = -s;
+        else if constexpr (MODE == 1) s = bs[n][lane] - s;
+        xs[n][lane] = s;
+      }
+    }
+  } else {
+    // ---- 2. the residual row (b itself from x = 0), then this wave's share of S r
+    float r[NL], y[NL];
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      float s = bs[n][lane];
+      if constexpr (!ZERO) {
+#pragma unroll
+        for (int ww = 0; ww < NF; ++ww) s -= red[ww][n][lane];
+      }
+      r[n] = s;
+      y[n] = 0.0f;
+    }
+    if (w == 0) sym_share<NL, NF, 0>(S, nele, ec, r, y);
+    else if (w == 1) sym_share<NL, NF, 1>(S, nele, ec, r, y);
+    else if (w == 2) sym_share<NL, NF, 2>(S, nele, ec, r, y);
+    else if constexpr (NF > 3) sym_share<NL, NF, 3>(S, nele, ec, r, y);
+    if constexpr (!ZERO) __syncthreads();  // every wave has read the face partials
+#pragma unroll
+    for (int n = 0; n < NL; ++n) red[w][n][lane] = y[n];
+    __syncthreads();
+    // ---- 3. rows of this wave: x_new = (1 - omega) x + S r
+#pragma unroll
+    for (int k = 0; k < NPW; ++k) {
+      const int n = w * NPW + k;
+      if (NL % NF == 0 || n < NL) {
+        float s = ZERO ? 0.0f : (1.0f - omega) * xs[n][lane];
+#pragma unroll
+        for (int ww = 0; ww < NF; ++ww) s += red[ww][n][lane];
+        xs[n][lane] = s;
+      }
+    }
+  }
+  __syncthreads();
+  // ---- 4. the block's output rows, coalesced
+  {
+    V* ov = reinterpret_cast<V*>(out + (size_t)e0 * NL);
+#pragma unroll
+    for (int i0 = 0; i0 < NVEC; i0 += NT) {
+      const int i = i0 + tid;
+      if (i < nvec) {
+        const int el = (W * i) / NL, n = (W * i) % NL;
+        V v;
+        float* pv = reinterpret_cast<float*>(&v);
+#pragma unroll
+        for (int k = 0; k < W; ++k) pv[k] = xs[n + k][el];
+        ov[i] = v;
       }
     }
   }
@@ -896,6 +1217,20 @@ static int apply_impl(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, T* r, int mo
     const T* vt = g.tab + g.off_vtab;
     const T* wg = g.tab + g.off_wgt;
     const T* sw = g.tab + g.off_sw;
+    if constexpr (std::is_same<T, float>::value) {
+      if (c->staged) {
+#define DGMI_ST(MODE)                                                                         \
+  hipLaunchKernelGGL((k_staged<D, MODE>), grd, blk, 0, s, nele, x, b, r, vt, wg, g.G, sw, g.cf, \
+                     g.wf, g.pen, c->nbface, c->fcode, g.S, act, (float)c->omega,              \
+                     g.tab + g.off_fc, c->fnodes, g.cnb, c->nperm)
+        if (mode == 0) DGMI_ST(0);
+        else if (mode == 1) DGMI_ST(1);
+        else DGMI_ST(4);
+#undef DGMI_ST
+        DGMI_HIP(hipGetLastError());
+        return 0;
+      }
+    }
 #define DGMI_RES1(MODE)                                                                       \
   hipLaunchKernelGGL((k_residual_w<T, D, MODE, false, T, true>), grd, blk, 0, s, nele, x, b, r, \
                      vt, wg, sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,          \
@@ -1068,11 +1403,17 @@ static int sweep_nbx(dgmi_ctx* c, const float* x, const float* b, float* xnew, h
   const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
   const int32_t* act = c->mask_on ? c->act64 : nullptr;
   using V2 = typename Vec2<float>::type;
-  hipLaunchKernelGGL((k_residual_w<float, D, 5, false, float, true>), grd, blk, 0, s, nele,
-                     const_cast<float*>(x), b, xnew, g.tab + g.off_vtab, g.tab + g.off_wgt,
-                     g.tab + g.off_sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,
-                     (const V2*)nullptr, g.S, act, (float)c->omega, (V2*)nullptr,
-                     g.tab + g.off_fc, c->fnodes, g.cnb, c->nperm);
+  if (c->staged)
+    hipLaunchKernelGGL((k_staged<D, 5>), grd, blk, 0, s, nele, x, b, xnew, g.tab + g.off_vtab,
+                       g.tab + g.off_wgt, g.G, g.tab + g.off_sw, g.cf, g.wf, g.pen, c->nbface,
+                       c->fcode, g.S, act, (float)c->omega, g.tab + g.off_fc, c->fnodes, g.cnb,
+                       c->nperm);
+  else
+    hipLaunchKernelGGL((k_residual_w<float, D, 5, false, float, true>), grd, blk, 0, s, nele,
+                       const_cast<float*>(x), b, xnew, g.tab + g.off_vtab, g.tab + g.off_wgt,
+                       g.tab + g.off_sw, c->aln, g.G, g.cf, g.wf, g.pen, c->nbface, c->fcode,
+                       (const V2*)nullptr, g.S, act, (float)c->omega, (V2*)nullptr,
+                       g.tab + g.off_fc, c->fnodes, g.cnb, c->nperm);
   DGMI_HIP(hipGetLastError());
   return 0;
 }
@@ -1104,6 +1445,22 @@ int launch_residual_dd(dgmi_ctx* c, const double* x, const double* b, double* r,
 template <typename T>
 int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t s) {
   const int nele = (int)c->nele;
+  if constexpr (std::is_same<T, float>::value) {
+    if (c->staged) {
+      const int32_t* act = c->mask_on ? c->act64 : nullptr;
+      const dim3 grd((nele + 63) / 64);
+#define DGMI_ZERO(D)                                                                          \
+  hipLaunchKernelGGL((k_staged<D, 6>), grd, dim3(64 * (D + 1)), 0, s, nele, x, b, x,           \
+                     g.tab + g.off_vtab, g.tab + g.off_wgt, g.G, g.tab + g.off_sw, g.cf, g.wf, \
+                     g.pen, c->nbface, c->fcode, g.S, act, (float)c->omega, g.tab + g.off_fc, \
+                     c->fnodes, g.cnb, c->nperm)
+      if (c->D == 3) DGMI_ZERO(3);
+      else DGMI_ZERO(2);
+#undef DGMI_ZERO
+      DGMI_HIP(hipGetLastError());
+      return 0;
+    }
+  }
   const dim3 blk(256), grd((nele + 255) / 256);
   if (c->D == 3)
     hipLaunchKernelGGL((k_smooth_zero<T, 20>), grd, blk, 0, s, nele, x, b, g.S,

@@ -328,10 +328,14 @@ static std::vector<double> table_block(const ElementTables& el, size_t& ov, size
   os = t.size();
   for (int g = 0; g < SG; ++g) t.push_back(el.sweight[g]);
   while (t.size() % 4) t.push_back(0.0);
-  oc = t.size();  // fphi | flk | fgk, the layout FaceDims<D> assumes
+  oc = t.size();  // fphi | flk | fgk | fmf | fml | vd | vm2, the layout FaceDims<D> assumes
   t.insert(t.end(), el.fphi.begin(), el.fphi.end());
   t.insert(t.end(), el.flk.begin(), el.flk.end());
   t.insert(t.end(), el.fgk.begin(), el.fgk.end());
+  t.insert(t.end(), el.fmf.begin(), el.fmf.end());
+  t.insert(t.end(), el.fml.begin(), el.fml.end());
+  t.insert(t.end(), el.vd.begin(), el.vd.end());
+  t.insert(t.end(), el.vm2.begin(), el.vm2.end());
   return t;
 }
 
@@ -1237,9 +1241,10 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
   }
   // the face kernels interpolate du/dn quadratically over each face: exact for P3, so the
   // interpolant reproduces the tabulated gradients to rounding
-  if (!(c->el.face_interp_err < 1e-12) ||
+  if (!(c->el.face_interp_err < 1e-12) || !(c->el.vol_factor_err < 1e-12) ||
       c->el.snloc != (c->D == 3 ? FaceDims<3>::NFN : FaceDims<2>::NFN) ||
-      c->el.nfk != (c->D == 3 ? FaceDims<3>::NK : FaceDims<2>::NK)) {
+      c->el.nfk != (c->D == 3 ? FaceDims<3>::NK : FaceDims<2>::NK) ||
+      c->el.np2 != (c->D == 3 ? FaceDims<3>::NP2 : FaceDims<2>::NP2)) {
     dgmi_ctx_destroy(c);
     set_error("face interpolation tables do not reproduce the face gradients");
     return -2;
@@ -2096,6 +2101,11 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
     DGMI_CHECK(value == 0.0 || value == 1.0 || value == 2.0,
                "single_pass must be 0 (never), 1 (applies) or 2 (applies and FP32 sweeps)");
     c->single_pass = (int)value;
+    return 0;
+  }
+  if (k == "staged") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "staged must be 0 or 1");
+    c->staged = (int)value;
     return 0;
   }
   if (k == "destroy_sync") {

@@ -266,6 +266,10 @@ struct dgmi_ctx {
   // sweeps (at 48^3x6 tets: FP32 apply 292 -> 234 us, FP64 609 -> 466 us, sweep 321 ->
   // ~240 us)
   int single_pass = 2;
+  // FP32 single-pass applies and sweeps and the FP32 sweep from zero: 1 = k_staged (the
+  // block's rows read and written through LDS with coalesced vector accesses, each packed
+  // smoother entry read once per block), 0 = k_residual_w / k_smooth_zero
+  int staged = 1;
   // pre- and post-smoothing sweeps of the FP32 cycles with carried traces (the reference's
   // pre_smooth_its, z02/config.py:112)
   int f32_smooth_its = 3;

@@ -22,22 +22,25 @@ namespace dgmi {
 //   PHI [f][g][NFN]     phi of the face's nodes at its points
 //   LK  [g][NK]         quadratic face Lagrange basis at the face points
 //   GK  [f][k][D][NL]   d phi / d xi of every basis function at the face's quadratic nodes
+//   MF  [f][i][j]       sum_g sw_g phi_i(g) phi_j(g) over the face's nodes (face mass)
+//   ML  [f][i][k]       sum_g sw_g phi_i(g) L_k(g)
+//   VD  [d][i][NL]      d phi / d xi_d at the element's P2 nodes (NP2 of them)
+//   VM  [i][j]          sum_g w_g psi_i(g) psi_j(g), psi the P2 Lagrange basis
 // and the int32 node lists fnodes[f][NFN].
 template <int D> struct FaceDims {
   static constexpr int NL = D == 3 ? 20 : 10, SG = D == 3 ? 12 : 4, NF = D + 1;
-  static constexpr int NFN = D == 3 ? 10 : 4, NK = D == 3 ? 6 : 3;
+  static constexpr int NFN = D == 3 ? 10 : 4, NK = D == 3 ? 6 : 3, NP2 = D == 3 ? 10 : 6;
   static constexpr int PHI = 0, LK = NF * SG * NFN, GK = LK + SG * NK;
+  static constexpr int MF = GK + NF * NK * D * NL, ML = MF + NF * NFN * NFN;
+  static constexpr int VD = ML + NF * NFN * NK, VM = VD + D * NP2 * NL;
 };
 
-// Traces (u, du/dn) of face f at its SG points: xv = the element's row, xf = the values of
-// the face's nodes (fnodes order), c = J^-T n_f; store(g, u, dn) writes point g.
-// The tables (fc) may be held in another type TT than the arithmetic T.
-template <typename T, int D, typename TT, typename Store>
-__device__ __forceinline__ void face_traces(int f, const TT* __restrict__ fc, const T* xv,
-                                            const T* xf, const T* c, Store store) {
+// c . grad(u) at the NK quadratic nodes of face f (c = J^-T n_f), from the row xv
+template <typename T, int D, typename TT>
+__device__ __forceinline__ void face_dk(int f, const TT* __restrict__ fc, const T* xv,
+                                        const T* c, T* dk) {
   using F = FaceDims<D>;
   const TT* gk = fc + F::GK + f * F::NK * D * F::NL;
-  T dk[F::NK];
 #pragma unroll 2
   for (int k = 0; k < F::NK; ++k) {
     T gd[D];
@@ -52,6 +55,17 @@ __device__ __forceinline__ void face_traces(int f, const TT* __restrict__ fc, co
     for (int d = 0; d < D; ++d) s += c[d] * gd[d];
     dk[k] = s;
   }
+}
+
+// Traces (u, du/dn) of face f at its SG points: xv = the element's row, xf = the values of
+// the face's nodes (fnodes order), c = J^-T n_f; store(g, u, dn) writes point g.
+// The tables (fc) may be held in another type TT than the arithmetic T.
+template <typename T, int D, typename TT, typename Store>
+__device__ __forceinline__ void face_traces(int f, const TT* __restrict__ fc, const T* xv,
+                                            const T* xf, const T* c, Store store) {
+  using F = FaceDims<D>;
+  T dk[F::NK];
+  face_dk<T, D>(f, fc, xv, c, dk);
   const TT* ph = fc + F::PHI + f * F::SG * F::NFN;
   const TT* lk = fc + F::LK;
 #pragma unroll
@@ -85,6 +99,33 @@ template <typename T, int D> struct FaceLift {
     for (int i = 0; i < F::NFN; ++i) accf[i] += ph[i] * qs;
 #pragma unroll
     for (int k = 0; k < F::NK; ++k) Q[k] += lk[k] * qd;
+  }
+  // The same sums in node form, for fluxes that are linear in the face values:
+  //   qs(g) = sw_g (sv v(g) + sd dv(g)),  qd(g) = sw_g sq v(g),
+  // with v given by its values at the face's nodes (v(g) = sum_j phi_j(g) v_j) and dv by
+  // its values at the quadratic nodes (dv(g) = sum_k L_k(g) dv_k):
+  //   accf = sv MF v + sd ML dv,  Q = sq ML^T v.
+  template <typename TT>
+  __device__ __forceinline__ void set_nodes(int f, const TT* __restrict__ fc, const T* v,
+                                            const T* dv, T sv, T sd, T sq) {
+    const TT* mf = fc + F::MF + f * F::NFN * F::NFN;
+    const TT* ml = fc + F::ML + f * F::NFN * F::NK;
+#pragma unroll
+    for (int k = 0; k < F::NK; ++k) Q[k] = 0;
+#pragma unroll
+    for (int i = 0; i < F::NFN; ++i) {
+      T a = 0, b = 0;
+#pragma unroll
+      for (int j = 0; j < F::NFN; ++j) a += mf[i * F::NFN + j] * v[j];
+#pragma unroll
+      for (int k = 0; k < F::NK; ++k) {
+        b += ml[i * F::NK + k] * dv[k];
+        Q[k] += ml[i * F::NK + k] * v[i];
+      }
+      accf[i] = sv * a + sd * b;
+    }
+#pragma unroll
+    for (int k = 0; k < F::NK; ++k) Q[k] *= sq;
   }
   // acc[n] += sum_{k,d} c_d Q_k d phi_n / d xi_d (p_k), over all nodes
   template <typename TT>

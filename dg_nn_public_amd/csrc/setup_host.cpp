@@ -298,6 +298,68 @@ ElementTables make_element(int D) {
           }
     }
     el.face_interp_err = big > 0.0 ? err / big : err;
+    // face mass matrix and mixed face matrix of the node-form face terms
+    const int sn = el.snloc;
+    el.fmf.assign(nf * sn * sn, 0.0);
+    el.fml.assign(nf * sn * nk, 0.0);
+    for (int f = 0; f < nf; ++f)
+      for (int g = 0; g < sg; ++g) {
+        const double* ph = &el.fphi[(f * sg + g) * sn];
+        for (int i = 0; i < sn; ++i) {
+          for (int j = 0; j < sn; ++j) el.fmf[(f * sn + i) * sn + j] += fw[g] * ph[i] * ph[j];
+          for (int k = 0; k < nk; ++k)
+            el.fml[(f * sn + i) * nk + k] += fw[g] * ph[i] * el.flk[g * nk + k];
+        }
+      }
+  }
+  // volume term through the quadratic gradient (P2 nodes: vertices, then edge midpoints)
+  {
+    std::vector<std::array<int, 2>> p2;
+    for (int i = 0; i < nb; ++i) p2.push_back({i, i});
+    for (int i = 0; i < nb; ++i)
+      for (int j = i + 1; j < nb; ++j) p2.push_back({i, j});
+    const int np2 = (int)p2.size();
+    el.np2 = np2;
+    auto psi = [&](const double* lam, double* out) {
+      for (int i = 0; i < np2; ++i) {
+        const int a = p2[i][0], b = p2[i][1];
+        out[i] = a == b ? lam[a] * (2.0 * lam[a] - 1.0) : 4.0 * lam[a] * lam[b];
+      }
+    };
+    el.vd.assign(D * np2 * nloc, 0.0);
+    for (int i = 0; i < np2; ++i) {
+      double lam[4] = {0, 0, 0, 0};
+      lam[p2[i][0]] += 0.5;
+      lam[p2[i][1]] += 0.5;
+      lagrange(el.index, nb, lam, phi.data(), dxi.data(), nloc);
+      for (int d = 0; d < D; ++d)
+        for (int n = 0; n < nloc; ++n) el.vd[(d * np2 + i) * nloc + n] = dxi[d * nloc + n];
+    }
+    el.vm2.assign(np2 * np2, 0.0);
+    std::vector<double> ps(np2);
+    for (int g = 0; g < el.ngi; ++g) {
+      psi(&vlam[g * nb], ps.data());
+      for (int i = 0; i < np2; ++i)
+        for (int j = 0; j < np2; ++j) el.vm2[i * np2 + j] += vw[g] * ps[i] * ps[j];
+    }
+    double err = 0.0, big = 0.0;
+    std::vector<double> t(np2);
+    for (int d = 0; d < D; ++d)
+      for (int e2 = 0; e2 < D; ++e2)
+        for (int n = 0; n < nloc; ++n)
+          for (int m = 0; m < nloc; ++m) {
+            double ref = 0.0;
+            for (int g = 0; g < el.ngi; ++g)
+              ref += vw[g] * el.nlx[(d * nloc + n) * el.ngi + g] * el.nlx[(e2 * nloc + m) * el.ngi + g];
+            double v = 0.0;
+            for (int i = 0; i < np2; ++i)
+              for (int j = 0; j < np2; ++j)
+                v += el.vd[(d * np2 + i) * nloc + n] * el.vm2[i * np2 + j] *
+                     el.vd[(e2 * np2 + j) * nloc + m];
+            err = std::max(err, std::fabs(v - ref));
+            big = std::max(big, std::fabs(ref));
+          }
+    el.vol_factor_err = big > 0.0 ? err / big : err;
   }
   el.p1_prolong.assign(nloc * nb, 0.0);
   for (int n = 0; n < nloc; ++n)

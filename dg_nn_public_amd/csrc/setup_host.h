@@ -67,6 +67,19 @@ struct ElementTables {
   std::vector<double> flk;      // (sngi, nfk) quadratic face Lagrange basis at the face points
   std::vector<double> fgk;      // (nface, nfk, D, nloc) dphi/dxi at the face's quadratic nodes
   double face_interp_err = 0.0; // max |interpolated - tabulated| over snlx (set-up check)
+  // Face terms in node form: with u on face f given by its face-node values and du/dn by
+  // its values at the quadratic nodes, the lifts sum_g sw_g phi_i(g) (.)(g) become
+  //   fmf (nface, snloc, snloc)  sum_g sw_g phi_i(g) phi_j(g)   (face mass matrix)
+  //   fml (nface, snloc, nfk)    sum_g sw_g phi_i(g) L_k(g)
+  std::vector<double> fmf, fml;
+  // Volume term through the quadratic gradient: d phi_n / d xi_d is a P2 polynomial, fixed
+  // by its values at the np2 P2 Lagrange nodes (vertices and edge midpoints), so
+  //   sum_g w_g dphi_n/dxi_d(g) dphi_m/dxi_e(g) = sum_ij vd[d][i][n] vm2[i][j] vd[e][j][m]
+  //   vd  (D, np2, nloc)  d phi_n / d xi_d at P2 node i
+  //   vm2 (np2, np2)      sum_g w_g psi_i(g) psi_j(g)  (P2 mass matrix, exact for the rule)
+  int np2 = 0;
+  std::vector<double> vd, vm2;
+  double vol_factor_err = 0.0;  // max |factored - tabulated| of those products (set-up check)
 };
 ElementTables make_element(int D);
 
