@@ -43,9 +43,9 @@ PEAK_FP32 = 157.3e12  # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12     # B/s
 METRIC = "solves/s + p50/p99 latency at rtol 1e-10; DG apply GDoF/s, mixed vs FP64"
 # calibrated HBM bytes per launch of the FP32 single-pass smoothing sweep
-# (k_residual_w<float,3,5,false,float,true>): FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3
+# (k_staged<3,5>): FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3
 # --pmc passes (profiles/), by element count
-PMC_SWEEP = {}
+PMC_SWEEP = {663552: 847.1e6}
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):
@@ -80,12 +80,14 @@ def sweep_bytes_per_elem(D, nloc, nface, sngi, single_pass=True):
 
 def sweep_flops_per_elem(D, nloc, nface, sngi):
     """Flops of one such sweep per element as the kernel performs them: per face the
-    neighbour-term lift and the trace output through the quadratic face interpolant
-    (nfn face nodes, nk quadratic nodes), then the smoother row products."""
+    neighbour's normal derivative at the face's nk quadratic nodes, the flux in node form
+    (face mass and mixed face matrices over the nfn face nodes) and its lift, then the
+    packed smoother product."""
     nfn, nk = (10, 6) if D == 3 else (4, 3)
-    trace = nk * (D * nloc + D) + sngi * (nfn + nk)
-    lift = sngi * (nfn + nk) + nk * D + nloc * nk * D
-    return 2 * nface * (trace + lift) + 2 * nloc * nloc
+    dk = nk * (D * nloc + D)
+    nodes = nfn * nfn + 2 * nfn * nk
+    lift = nk * D + nloc * nk * D
+    return 2 * nface * (dk + nodes + lift) + 2 * nloc * nloc
 
 
 def log(*a):
@@ -319,7 +321,7 @@ def c3_block(args):
                      "frac": gbps * 1e9 / PEAK_HBM, "traffic": PMC_SWEEP.get(nele),
                      "kernel": "FP32 smoothing sweep x_new = (1-w)x + S(b - A_off x), the "
                                "neighbours' face traces formed from their rows of x (" +
-                               ("k_residual_w<float,3,5,false,float,true>, one launch" if vec
+                               ("k_staged<3,5>, one launch" if vec
                                 else "MFMA trace + residual") +
                                "), 48^3x6 tets; the largest share of the mixed solve's kernel "
                                "time",
@@ -666,7 +668,7 @@ def run_solve(args):
                      "traffic": (pmc["f32_sweep"] if on_profiled and vec32 else None),
                      "kernel": "FP32 smoothing sweep x_new = (1-w)x + S(b - A_off x), the "
                                "neighbours' face traces formed from their rows of x (" +
-                               ("k_residual_w<float,D,5,false,float,true>, one launch" if vec32
+                               ("k_staged<D,5>, one launch" if vec32
                                 else "trace + residual MFMA launches") + ", autotuned family); "
                                "the largest share of the mixed solve's kernel time",
                      "bytes_per_elem": SB, "flop_per_elem": SF, "elements": nele,
