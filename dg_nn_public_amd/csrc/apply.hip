@@ -868,6 +868,189 @@ __global__ __launch_bounds__(64) void k_residual_dd(
   }
 }
 
+// The same residual with the face terms in node form and the neighbour in the face's
+// frame (as k_staged), which the single-pass kernels need every interior face to have:
+// per face, c . grad(u) at the quadratic nodes on both sides, the jump of u at the face's
+// nodes, then accf = w (pen MF jump - a ML ddk) onto the face's nodes and
+// Q = -a w ML^T jump lifted through the gradient table.  Far fewer double-double values
+// live at once than in the per-point form, so the kernel holds them in registers.
+template <int D>
+__device__ __forceinline__ void dd_face_dk(int f, const double* __restrict__ fc,
+                                           const double* xv, const dd* c, dd* dk) {
+  using F = FaceDims<D>;
+  const double* gk = fc + F::GK + f * F::NK * D * F::NL;
+#pragma unroll
+  for (int k = 0; k < F::NK; ++k) {
+    dd gd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gd[d] = dd(0.0);
+#pragma unroll
+    for (int n = 0; n < F::NL; ++n)
+#pragma unroll
+      for (int d = 0; d < D; ++d) gd[d] += two_prod(gk[(k * D + d) * F::NL + n], xv[n]);
+    dd s(0.0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) s += c[d] * gd[d];
+    dk[k] = s;
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(128, 2) void k_residual_dd_nodes(
+    int nele, const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
+    const double* __restrict__ vtab, const double* __restrict__ wgt,
+    const double* __restrict__ G, const double* __restrict__ cf, const double* __restrict__ cnb,
+    const double* __restrict__ wf, const double* __restrict__ pen,
+    const int32_t* __restrict__ nbface, const int32_t* __restrict__ fcode,
+    const double* __restrict__ fc, const uint8_t* __restrict__ nperm) {
+  using M = Dm<D>;
+  using F = FaceDims<D>;
+  using FN = FaceNodes<D>;
+  constexpr int NL = M::NL, NF = M::NF, NFN = F::NFN, NK = F::NK;
+  constexpr int NPERM = D == 3 ? 24 : 6;
+  __shared__ uint8_t ptab[NPERM * NL];
+  // the rows accumulate in LDS (each thread its own column), which keeps the row of
+  // double-double sums out of the registers the volume and face work need
+  __shared__ double ahi[NL][128], alo[NL][128];
+  for (int i = threadIdx.x; i < NPERM * NL; i += blockDim.x) ptab[i] = nperm[i];
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = min(t, nele - 1);
+  const bool store = t < nele;
+  double xv[NL];
+  load_row<double, NL>(x + (size_t)e * NL, xv);
+  const int tl = threadIdx.x;
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    ahi[n][tl] = 0.0;
+    alo[n][tl] = 0.0;
+  }
+  auto add_row = [&](int n, dd v) {
+    const dd a = dd(ahi[n][tl], alo[n][tl]) + v;
+    ahi[n][tl] = a.hi;
+    alo[n][tl] = a.lo;
+  };
+  // ---- volume: acc_n = sum_g w_g t_g(n) . (G t_g(x))
+  double gm[M::NG];
+#pragma unroll
+  for (int k = 0; k < M::NG; ++k) gm[k] = G[k * nele + e];
+#pragma unroll 1
+  for (int g = 0; g < M::NGI; ++g) {
+    const double* tt = vtab + g * D * NL;
+    dd gd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gd[d] = dd(0.0);
+#pragma unroll
+    for (int n = 0; n < NL; ++n)
+#pragma unroll
+      for (int d = 0; d < D; ++d) gd[d] += two_prod(tt[d * NL + n], xv[n]);
+    const double wq = wgt[g];
+    dd y[D];
+    if constexpr (D == 3) {
+      y[0] = wq * (gm[0] * gd[0] + gm[3] * gd[1] + gm[4] * gd[2]);
+      y[1] = wq * (gm[3] * gd[0] + gm[1] * gd[1] + gm[5] * gd[2]);
+      y[2] = wq * (gm[4] * gd[0] + gm[5] * gd[1] + gm[2] * gd[2]);
+    } else {
+      y[0] = wq * (gm[0] * gd[0] + gm[2] * gd[1]);
+      y[1] = wq * (gm[2] * gd[0] + gm[1] * gd[1]);
+    }
+#pragma unroll 4
+    for (int n = 0; n < NL; ++n) {
+      dd v = tt[n] * y[0];
+#pragma unroll
+      for (int d = 1; d < D; ++d) v += tt[d * NL + n] * y[d];
+      add_row(n, v);
+    }
+  }
+  // ---- faces (unrolled: the face-node positions are compile-time)
+  auto face = [&](auto fi) {
+    constexpr int f = decltype(fi)::value;
+    const int z = e * NF + f;
+    const int code = fcode[z];
+    const int type = code & 15;
+    const bool interior = type == 0;
+    const double wfe = (type < 2) ? wf[f * nele + e] : 0.0;
+    const double pe = pen[f * nele + e];
+    const double half = interior ? 0.5 : 1.0;
+    const double keep = interior ? 1.0 : 0.0;  // multiplies the neighbour's values
+    const int nbe = interior ? nbface[z] / NF : e;
+    const int pid = interior ? (code >> 8) & 31 : 0;
+    dd c[D], cn[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      c[d] = dd(cf[(f * D + d) * nele + e]);
+      cn[d] = dd(cnb[(f * D + d) * nele + e]);
+    }
+    // the neighbour relabelled into this face's frame: c' . grad(u') at the quadratic
+    // nodes and u' at the face's nodes
+    dd ddk[NK];
+    double xrf[NFN];
+    {
+      double xr[NL];
+      const uint8_t* pp = ptab + pid * NL;
+#pragma unroll
+      for (int n = 0; n < NL; ++n) xr[n] = x[(size_t)nbe * NL + pp[n]];
+#pragma unroll
+      for (int i = 0; i < NFN; ++i) xrf[i] = keep * xr[FN::v[f][i]];
+      dd_face_dk<D>(f, fc, xr, cn, ddk);
+    }
+    {
+      dd dk[NK];
+      dd_face_dk<D>(f, fc, xv, c, dk);
+#pragma unroll
+      for (int k = 0; k < NK; ++k) ddk[k] = dk[k] - keep * ddk[k];
+    }
+    dd jump[NFN];
+#pragma unroll
+    for (int i = 0; i < NFN; ++i) jump[i] = two_sum(xv[FN::v[f][i]], -xrf[i]);
+    const double* mf = fc + F::MF + f * NFN * NFN;
+    const double* ml = fc + F::ML + f * NFN * NK;
+    const double sv = wfe * pe, sd = -half * wfe;  // (products rounded once: coefficients)
+    dd Q[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) Q[k] = dd(0.0);
+#pragma unroll
+    for (int i = 0; i < NFN; ++i) {
+      dd a(0.0), bq(0.0);
+#pragma unroll
+      for (int j = 0; j < NFN; ++j) a += mf[i * NFN + j] * jump[j];
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        bq += ml[i * NK + k] * ddk[k];
+        Q[k] += ml[i * NK + k] * jump[i];
+      }
+      add_row(FN::v[f][i], sv * a + sd * bq);
+    }
+    // lift of the normal-derivative part: acc_n += sum_{k,d} gk[k][d][n] c_d sd Q_k
+    const double* gk = fc + F::GK + f * NK * D * NL;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      const dd qk = sd * Q[k];
+      dd wk[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) wk[d] = c[d] * qk;
+#pragma unroll 4
+      for (int n = 0; n < NL; ++n) {
+        dd v = gk[(k * D) * NL + n] * wk[0];
+#pragma unroll
+        for (int d = 1; d < D; ++d) v += gk[(k * D + d) * NL + n] * wk[d];
+        add_row(n, v);
+      }
+    }
+  };
+  face(std::integral_constant<int, 0>{});
+  face(std::integral_constant<int, 1>{});
+  face(std::integral_constant<int, 2>{});
+  if constexpr (NF > 3) face(std::integral_constant<int, 3>{});
+  if (store) {
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      const dd s = dd(b[(size_t)e * NL + n]) - dd(ahi[n][tl], alo[n][tl]);
+      r[(size_t)e * NL + n] = s.hi;
+    }
+  }
+}
+
 // x = S b  (a smoothing step from x = 0, where the residual is b itself)
 template <typename T, int NL>
 __global__ __launch_bounds__(256) void k_smooth_zero(int nele, T* __restrict__ x,
@@ -1430,6 +1613,18 @@ int launch_sweep_nbx(dgmi_ctx* c, const float* x, const float* b, float* xnew, h
 int launch_residual_dd(dgmi_ctx* c, const double* x, const double* b, double* r, hipStream_t s) {
   const int nele = (int)c->nele;
   const DevGeom<double>& g = c->g64;
+  if (c->frames_ok) {
+    const dim3 blk(128), grd((nele + 127) / 128);
+#define DGMI_RDD2(D)                                                                           \
+  hipLaunchKernelGGL(k_residual_dd_nodes<D>, grd, blk, 0, s, nele, x, b, r, g.tab + g.off_vtab, \
+                     g.tab + g.off_wgt, g.G, g.cf, g.cnb, g.wf, g.pen, c->nbface, c->fcode,    \
+                     g.tab + g.off_fc, c->nperm)
+    if (c->D == 3) DGMI_RDD2(3);
+    else DGMI_RDD2(2);
+#undef DGMI_RDD2
+    DGMI_HIP(hipGetLastError());
+    return 0;
+  }
   const dim3 blk(64), grd((nele + 63) / 64);
 #define DGMI_RDD(D)                                                                            \
   hipLaunchKernelGGL(k_residual_dd<D>, grd, blk, 0, s, nele, x, b, r, g.tab + g.off_vtab,       \

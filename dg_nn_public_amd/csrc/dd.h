@@ -65,6 +65,13 @@ __device__ __forceinline__ dd operator*(double a, dd b) {
 }
 
 __device__ __forceinline__ dd operator*(dd a, double b) { return b * a; }
+
+// a * b exactly (TwoProd)
+__device__ __forceinline__ dd two_prod(double a, double b) {
+#pragma clang fp contract(off)
+  const double p = a * b;
+  return dd(p, fma(a, b, -p));
+}
 __device__ __forceinline__ dd& operator+=(dd& a, dd b) { return a = a + b; }
 
 }  // namespace dgmi

@@ -1241,7 +1241,12 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
   }
   // the face kernels interpolate du/dn quadratically over each face: exact for P3, so the
   // interpolant reproduces the tabulated gradients to rounding
-  if (!(c->el.face_interp_err < 1e-12) || !(c->el.vol_factor_err < 1e-12) ||
+  bool fn_ok = (int)c->el.fnodes.size() == c->nface * c->el.snloc;
+  for (int f = 0; fn_ok && f < c->nface; ++f)
+    for (int i = 0; i < c->el.snloc; ++i)
+      fn_ok = fn_ok && c->el.fnodes[f * c->el.snloc + i] ==
+                           (c->D == 3 ? FaceNodes<3>::v[f][i] : FaceNodes<2>::v[f][i]);
+  if (!fn_ok || !(c->el.face_interp_err < 1e-12) || !(c->el.vol_factor_err < 1e-12) ||
       c->el.snloc != (c->D == 3 ? FaceDims<3>::NFN : FaceDims<2>::NFN) ||
       c->el.nfk != (c->D == 3 ? FaceDims<3>::NK : FaceDims<2>::NK) ||
       c->el.np2 != (c->D == 3 ? FaceDims<3>::NP2 : FaceDims<2>::NP2)) {

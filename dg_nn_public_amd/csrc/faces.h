@@ -35,6 +35,19 @@ template <int D> struct FaceDims {
   static constexpr int VD = ML + NF * NFN * NK, VM = VD + D * NP2 * NL;
 };
 
+// The nodes of each face (ascending local node numbers; the host checks them against the
+// element tables' fnodes), for kernels that index register rows by them at compile time.
+template <int D> struct FaceNodes;
+template <> struct FaceNodes<3> {
+  static constexpr int v[4][10] = {{1, 2, 3, 8, 9, 12, 13, 14, 15, 16},
+                                   {0, 2, 3, 4, 5, 10, 11, 14, 15, 18},
+                                   {0, 1, 3, 6, 7, 10, 11, 12, 13, 19},
+                                   {0, 1, 2, 4, 5, 6, 7, 8, 9, 17}};
+};
+template <> struct FaceNodes<2> {
+  static constexpr int v[3][4] = {{0, 1, 3, 4}, {1, 2, 5, 6}, {0, 2, 7, 8}};
+};
+
 // c . grad(u) at the NK quadratic nodes of face f (c = J^-T n_f), from the row xv
 template <typename T, int D, typename TT>
 __device__ __forceinline__ void face_dk(int f, const TT* __restrict__ fc, const T* xv,
