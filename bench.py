@@ -168,14 +168,21 @@ def build_stream(nreq, large_n, emax, seed=11):
     return payload, nlarge
 
 
-def serve_stream(payload, rate, inner, args, seed):
+# the FP64 solver's V-cycle (the reference's): sweeps x += S (b - A x) with the whole
+# operator, every SFC level swept; run in FP32 refinement it separates the effect of
+# precision from that of the FP32 path's cycle (neighbour-form sweeps, exact coarse level)
+REFERENCE_CYCLE = {"smooth_offdiag": 0, "coarse_exact": 0}
+
+
+def serve_stream(payload, rate, inner, args, seed, params=None):
     import numpy as np
 
     from dg_nn_public_amd.serve import Request, Server
     ar = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, size=len(payload)))
     reqs = [Request(i, m, p.kappa, p.f, p.u_bc, float(a))
             for i, ((m, p), a) in enumerate(zip(payload, ar))]
-    srv = Server(hbm_budget=args.budget, max_jobs=args.jobs, max_batch=64, inner=inner)
+    srv = Server(hbm_budget=args.budget, max_jobs=args.jobs, max_batch=64, inner=inner,
+                 params=params)
     t0 = time.perf_counter()
     res = srv.run(reqs)
     return res, time.perf_counter() - t0
@@ -197,12 +204,16 @@ def run_headline(args):
         serve_stream(payload[: max(4, len(payload) // 6)], args.rate, torch.float32, args, 99 + w)
     log("warm-up done")
     out = {}
-    variants = [("f32_ir", torch.float32)] + ([] if args.no_fp64 else [("f64", torch.float64)])
-    for name, inner in variants:
+    variants = [("f32_ir", torch.float32, None)]
+    if not args.no_fp64:
+        variants += [("f64", torch.float64, None),
+                     ("f32_ir_reference_cycle", torch.float32, REFERENCE_CYCLE)]
+    for name, inner, params in variants:
         results, walls = [], []
         for k in range(args.steps):
             torch.cuda.synchronize()
-            res, wall = serve_stream(payload, args.rate, inner, args, seed=1000 + k)
+            res, wall = serve_stream(payload, args.rate, inner, args, seed=1000 + k,
+                                     params=params)
             torch.cuda.synchronize()
             results += res
             walls.append(wall)
@@ -257,6 +268,12 @@ def run_headline(args):
         "speedup_vs_fp64": (head["solves_per_s"] / out["f64"]["solves_per_s"]
                             if "f64" in out else None),
     }
+    if "f64" in out:
+        # the speed-up split: FP32 refinement with the FP64 solver's own cycle against FP64
+        # (precision alone), and the FP32 path's cycle against that (algorithm)
+        ref = out["f32_ir_reference_cycle"]["solves_per_s"]
+        line["speedup_split"] = {"precision": ref / out["f64"]["solves_per_s"],
+                                 "cycle": head["solves_per_s"] / ref}
     if not args.no_c3:
         c3 = c3_block(args)
         line["c3_48cube"] = c3
@@ -360,8 +377,11 @@ def c2_block(args):
     b = g._get_rhs(g.new_vector(), pb.f, pb.u_bc)
     out = {"workload": f"3D P3 tets 24^3x6 = {g.nele} elements, {g.ndof} DOFs, lognormal "
                        "kappa sigma 1, harmonic face kappa, one solve to rtol 1e-10"}
-    for name, inner in (("f64", torch.float64), ("f32_ir", torch.float32),
-                        ("bf16_ir", torch.bfloat16)):
+    for name, inner, params in (("f64", torch.float64, None), ("f32_ir", torch.float32, None),
+                                ("f32_ir_reference_cycle", torch.float32, REFERENCE_CYCLE),
+                                ("bf16_ir", torch.bfloat16, None)):
+        for k, v in (params or {}).items():
+            g.set_param(k, v)
         lat = []
         for i in range(1 + args.steps):
             x = g.new_vector()
@@ -371,9 +391,14 @@ def c2_block(args):
             torch.cuda.synchronize()
             if i:
                 lat.append(time.perf_counter() - ts)
+        for k in (params or {}):
+            g.set_param(k, 1)  # both parameters default to 1
         out[name] = {"solves_per_s": len(lat) / sum(lat), "ms": 1e3 * float(np.median(lat)),
                      **g.last_stats}
     out["speedup_f32_vs_f64"] = out["f32_ir"]["solves_per_s"] / out["f64"]["solves_per_s"]
+    out["speedup_split"] = {
+        "precision": out["f32_ir_reference_cycle"]["solves_per_s"] / out["f64"]["solves_per_s"],
+        "cycle": out["f32_ir"]["solves_per_s"] / out["f32_ir_reference_cycle"]["solves_per_s"]}
     log(f"C2 {out}")
     g.close()
     return out

@@ -61,7 +61,9 @@ class DGBatch:
     variable-diffusivity solver uses (z05/volume_mf_um_diff_varprop.py:384-392)."""
 
     def __init__(self, meshes, kappas, device="cuda:0", face_kappa="harmonic",
-                 blk_solver="direct", graph_vcycle=True, alloc_limit=0):
+                 blk_solver="direct", graph_vcycle=True, alloc_limit=0, params=None):
+        """``params``: solver parameters (``dgmi_set_param`` names and values) set on the
+        context before its multigrid set-up."""
         p = pack(meshes)
         kappa = np.concatenate([np.asarray(k, dtype=np.float64).reshape(-1) for k in kappas])
         self.space = DGDiffusion(p.x_parts, p.cells, p.verts, kappa, device=device,
@@ -75,6 +77,8 @@ class DGBatch:
             # replay the packed V-cycle from a captured graph, or launch its kernels one by one
             self.space.lib.dgmi_set_param(self.space.ctx, b"graph_vcycle",
                                           1.0 if graph_vcycle else 0.0)
+            for name, value in (params or {}).items():
+                self.space.set_param(name, value)
             self.space._get_RAR_and_sfc_data_Um()
         except BaseException:
             self.space.close()  # the device memory goes back at once, not at collection

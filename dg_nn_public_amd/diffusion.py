@@ -312,6 +312,12 @@ class DGDiffusion:
         check(self.lib.dgmi_set_param(self.ctx, b"f32_kernel", float(F32_KERNELS[name])),
               "dgmi_set_param")
 
+    def set_param(self, name, value):
+        """Set a solver parameter of the context (``dgmi_set_param``: e.g. "coarse_exact",
+        "smooth_offdiag", "single_pass", "staged", "f32_smooth_its"; see include/dgmi.h)."""
+        check(self.lib.dgmi_set_param(self.ctx, str(name).encode(), float(value)),
+              "dgmi_set_param")
+
     def set_alloc_limit(self, nbytes):
         """Cap the device bytes this context may hold (0: no cap).  An allocation that would
         pass the cap fails and the call that needed it raises ``DgmiError`` without

@@ -129,7 +129,7 @@ class Server:
     def __init__(self, hbm_budget=200e9, max_jobs=4, max_batch=64, inner=torch.float32,
                  rtol=1e-10, max_its=400, device="cuda:0", keep_solutions=False,
                  hbm_capacity=None, job_byte_limit=0, solo_elements=250_000,
-                 job_elements=400_000):
+                 job_elements=400_000, params=None):
         self.budget = float(hbm_budget)
         self.solo_elements = int(solo_elements)
         self.job_elements = int(job_elements)  # packing stops once a job holds this many
@@ -142,6 +142,7 @@ class Server:
         self.device = device
         self.keep = keep_solutions
         self.job_byte_limit = int(job_byte_limit)  # device bytes one job may hold (0: any)
+        self.params = dict(params or {})  # solver parameters of every job's context
         self.bpe = dict(_BYTES_PER_ELEM)
         self._lock = threading.Lock()
         self._cv = threading.Condition(self._lock)
@@ -168,7 +169,7 @@ class Server:
                     ta = time.perf_counter()
                     bt = DGBatch([r.mesh for r in job.reqs], [r.kappa for r in job.reqs],
                                  device=self.device, graph_vcycle=False,
-                                 alloc_limit=self.job_byte_limit)
+                                 alloc_limit=self.job_byte_limit, params=self.params)
                     b = bt.rhs([r.f for r in job.reqs], [r.u_bc for r in job.reqs])
                     stream.synchronize()
                     tb = time.perf_counter()
