@@ -276,6 +276,10 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     frame), instead of a trace pass before them: 0 never (traces through
  *                     trace buffers), 1 the applies (dgmi_apply, the solvers' operator and
  *                     residuals), 2 also the FP32 preconditioner's sweeps (default).
+ *   "staged"          FP32 single-pass applies, sweeps and the sweep from zero: 1 = the
+ *                     block-staged kernels (rows read and written through LDS, face terms
+ *                     in node form, volume term through the P2 gradient; default); 0 = the
+ *                     per-wave single-pass kernels (same operator, other rounding).
  *   "destroy_sync"    dgmi_ctx_destroy: 1 = synchronise the device first (default); 0 =
  *                     the caller has finished all work it queued for the context (the
  *                     serving layer, which synchronises its job's stream).
