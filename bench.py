@@ -42,10 +42,10 @@ PEAK_FP64 = 78.6e12   # MI355X FP64 vector/matrix, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12     # B/s
 METRIC = "solves/s + p50/p99 latency at rtol 1e-10; DG apply GDoF/s, mixed vs FP64"
-# calibrated HBM bytes per launch of the FP32 single-pass smoothing sweep
-# (k_staged<3,5>): FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3
+# calibrated HBM bytes per launch of the FP32 single-pass smoothing sweep with the FP16
+# smoother copy (k_staged<3,5,true>, the 3D default): FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3
 # --pmc passes (profiles/), by element count
-PMC_SWEEP = {663552: 847.1e6}
+PMC_SWEEP = {663552: 563.1e6}
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):
@@ -62,14 +62,16 @@ def bytes_per_elem(D, nloc, nface, s, read_b=True):
     return 2 * nloc * s + s + (D * D + 1) * s + 4 * nface + (nloc * s if read_b else 0)
 
 
-def sweep_bytes_per_elem(D, nloc, nface, sngi, single_pass=True):
+def sweep_bytes_per_elem(D, nloc, nface, sngi, single_pass=True, half=False):
     """Algorithmic HBM bytes of one FP32 smoothing sweep x_new = (1-w)x + S(b - A_off x),
-    per element.  Single pass (the default kernel): packed symmetric S, x read once (the
-    element's own row and, through L2, its neighbours' rows), b read, x_new written, and
-    per face c = J^-T n on both sides, wf, pen, code and nbface.  Two-pass form: the same
-    with the neighbours' traces read and this element's traces written (value and normal
-    derivative at every face point) and c on one side only."""
-    s_packed = 4 * nloc * (nloc + 1) // 2
+    per element.  Single pass (the default kernel): packed symmetric S (FP32, or with
+    ``half`` its FP16 copy in 32-bit pairs plus one FP32 scale), x read once (the element's
+    own row and, through L2, its neighbours' rows), b read, x_new written, and per face
+    c = J^-T n on both sides, wf, pen, code and nbface.  Two-pass form: the same with the
+    neighbours' traces read and this element's traces written (value and normal derivative
+    at every face point) and c on one side only."""
+    ns = nloc * (nloc + 1) // 2
+    s_packed = 4 * ((ns + 1) // 2) + 4 if half else 4 * ns
     vectors = 4 * 3 * nloc
     if single_pass:
         return s_packed + vectors + nface * (8 * D + 16)
@@ -323,7 +325,7 @@ def c3_block(args):
     t_sweep = e0.elapsed_time(e1) / 1e3 / args.sweeps
     log(f"C3 apply us {({k: round(v * 1e6, 1) for k, v in ta.items()})}, sweep {t_sweep * 1e6:.1f} us")
     F = flops_per_elem(3, nloc, g.ngi, g.sngi, g.nface)
-    SB = sweep_bytes_per_elem(3, nloc, g.nface, g.sngi)
+    SB = sweep_bytes_per_elem(3, nloc, g.nface, g.sngi, half=g.smoother_half)
     SF = sweep_flops_per_elem(3, nloc, g.nface, g.sngi)
     vec = tuned["f32_kernel"] == "vector"
     t32 = ta["f32_vector"] if vec else ta["f32_mfma"]
@@ -337,7 +339,9 @@ def c3_block(args):
         "roofline": {"bound": "hbm", "achieved": gbps, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": gbps * 1e9 / PEAK_HBM, "traffic": PMC_SWEEP.get(nele),
                      "kernel": "FP32 smoothing sweep x_new = (1-w)x + S(b - A_off x), the "
-                               "neighbours' face traces formed from their rows of x (" +
+                               "neighbours' face traces formed from their rows of x, the "
+                               "smoother read " + ("from its FP16 copy" if g.smoother_half
+                                                   else "in FP32") + " (" +
                                ("k_staged<3,5>, one launch" if vec
                                 else "MFMA trace + residual") +
                                "), 48^3x6 tets; the largest share of the mixed solve's kernel "
@@ -648,7 +652,7 @@ def run_solve(args):
     # rocprofv3 --pmc passes, profiles/pmc_bytes_apply_tet24.md)
     pmc = {"f64": 240e6, "f32_vector": 113e6, "f32_mfma": 125e6, "f32_sweep": PMC_SWEEP.get(82944)}
     on_profiled = args.dim == 3 and args.n == 24
-    SB = sweep_bytes_per_elem(args.dim, nloc, g.nface, g.sngi)
+    SB = sweep_bytes_per_elem(args.dim, nloc, g.nface, g.sngi, half=g.smoother_half)
     SF = sweep_flops_per_elem(args.dim, nloc, g.nface, g.sngi)
     sweep_gbps = SB * nele / t_sweep / 1e9
     out = {

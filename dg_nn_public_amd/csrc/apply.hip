@@ -445,6 +445,21 @@ __device__ __forceinline__ void sym_term(const float* __restrict__ S, int nele, 
   y[A] += s * r[B];
   if constexpr (A != B) y[B] += s * r[A];
 }
+template <int NL, int IDX>
+__device__ __forceinline__ void sym_term_s(float s, const float* r, float* y) {
+  constexpr int A = [] {
+    int i = IDX, a = 0;
+    while (i >= NL - a) i -= NL - a++;
+    return a;
+  }();
+  constexpr int B = [] {
+    int i = IDX, a = 0;
+    while (i >= NL - a) i -= NL - a++;
+    return a + i;
+  }();
+  y[A] += s * r[B];
+  if constexpr (A != B) y[B] += s * r[A];
+}
 template <int NL, int P0, int... J>
 __device__ __forceinline__ void sym_part(const float* __restrict__ S, int nele, int e,
                                          const float* r, float* y,
@@ -460,7 +475,37 @@ __device__ __forceinline__ void sym_share(const float* __restrict__ S, int nele,
   sym_part<NL, P0>(S, nele, e, r, y, std::make_integer_sequence<int, P1 - P0>{});
 }
 
-template <int D, int MODE>
+// The same share from the FP16 copy: packed entries in pairs (2p, 2p+1) per 32-bit word,
+// wave W of NF taking pairs [W NP / NF, (W+1) NP / NF); the caller multiplies y by the
+// element's scale
+__device__ __forceinline__ float half_lo(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+}
+template <int NL, int PAIR>
+__device__ __forceinline__ void sym_pair_h(const uint32_t* __restrict__ H, int nele, int e,
+                                           const float* r, float* y) {
+  const uint32_t wv = H[(size_t)PAIR * nele + e];
+  sym_term_s<NL, 2 * PAIR>(half_lo(wv), r, y);
+  if constexpr (2 * PAIR + 1 < sym_size<NL>()) sym_term_s<NL, 2 * PAIR + 1>(half_hi(wv), r, y);
+}
+template <int NL, int P0, int... J>
+__device__ __forceinline__ void sym_part_h(const uint32_t* __restrict__ H, int nele, int e,
+                                           const float* r, float* y,
+                                           std::integer_sequence<int, J...>) {
+  (sym_pair_h<NL, P0 + J>(H, nele, e, r, y), ...);
+}
+template <int NL, int NF, int W>
+__device__ __forceinline__ void sym_share_h(const uint32_t* __restrict__ H, int nele, int e,
+                                            const float* r, float* y) {
+  constexpr int NP = (sym_size<NL>() + 1) / 2;
+  constexpr int P0 = W * NP / NF, P1 = (W + 1) * NP / NF;
+  sym_part_h<NL, P0>(H, nele, e, r, y, std::make_integer_sequence<int, P1 - P0>{});
+}
+
+template <int D, int MODE, bool HALF = false>
 __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void k_staged(
     int nele, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ out,
     const float* __restrict__ vtab, const float* __restrict__ wgt, const float* __restrict__ G,
@@ -469,7 +514,8 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
     const int32_t* __restrict__ fcode, const float* __restrict__ S,
     const int32_t* __restrict__ act, float omega, const float* __restrict__ fc,
     const int32_t* __restrict__ fnodes, const float* __restrict__ cnb,
-    const uint8_t* __restrict__ nperm) {
+    const uint8_t* __restrict__ nperm, const uint32_t* __restrict__ Sh = nullptr,
+    const float* __restrict__ Shs = nullptr) {
   using M = Dm<D>;
   using F = FaceDims<D>;
   using RV = RowVec<float, M::NL>;
@@ -690,10 +736,20 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
       r[n] = s;
       y[n] = 0.0f;
     }
-    if (w == 0) sym_share<NL, NF, 0>(S, nele, ec, r, y);
-    else if (w == 1) sym_share<NL, NF, 1>(S, nele, ec, r, y);
-    else if (w == 2) sym_share<NL, NF, 2>(S, nele, ec, r, y);
-    else if constexpr (NF > 3) sym_share<NL, NF, 3>(S, nele, ec, r, y);
+    if constexpr (HALF) {
+      if (w == 0) sym_share_h<NL, NF, 0>(Sh, nele, ec, r, y);
+      else if (w == 1) sym_share_h<NL, NF, 1>(Sh, nele, ec, r, y);
+      else if (w == 2) sym_share_h<NL, NF, 2>(Sh, nele, ec, r, y);
+      else if constexpr (NF > 3) sym_share_h<NL, NF, 3>(Sh, nele, ec, r, y);
+      const float sc = Shs[ec];
+#pragma unroll
+      for (int n = 0; n < NL; ++n) y[n] *= sc;
+    } else {
+      if (w == 0) sym_share<NL, NF, 0>(S, nele, ec, r, y);
+      else if (w == 1) sym_share<NL, NF, 1>(S, nele, ec, r, y);
+      else if (w == 2) sym_share<NL, NF, 2>(S, nele, ec, r, y);
+      else if constexpr (NF > 3) sym_share<NL, NF, 3>(S, nele, ec, r, y);
+    }
     if constexpr (!ZERO) __syncthreads();  // every wave has read the face partials
 #pragma unroll
     for (int n = 0; n < NL; ++n) red[w][n][lane] = y[n];
@@ -1337,6 +1393,41 @@ int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, double
   return 0;
 }
 
+// FP16 copy of the packed smoother: per element the largest magnitude m (kept in FP32) and
+// the entries divided by m, rounded to FP16 (relative error 2^-11 of m), two per word
+template <int NL>
+__global__ __launch_bounds__(256) void k_pack_S_half(int nele, const double* __restrict__ P64,
+                                                     uint32_t* __restrict__ H,
+                                                     float* __restrict__ scale) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nele) return;
+  constexpr int NS = sym_size<NL>(), NP = (NS + 1) / 2;
+  double m = 0.0;
+  for (int i = 0; i < NS; ++i) m = fmax(m, fabs(P64[(size_t)i * nele + e]));
+  const float mf = (float)m;
+  const double inv = m > 0.0 ? 1.0 / (double)mf : 0.0;
+  scale[e] = mf;
+  for (int p = 0; p < NP; ++p) {
+    const int i0 = 2 * p, i1 = 2 * p + 1;
+    const _Float16 lo = (_Float16)(float)(P64[(size_t)i0 * nele + e] * inv);
+    const _Float16 hi = i1 < NS ? (_Float16)(float)(P64[(size_t)i1 * nele + e] * inv) : (_Float16)0.0f;
+    H[(size_t)p * nele + e] = (uint32_t)__builtin_bit_cast(uint16_t, lo) |
+                              ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+  }
+}
+int launch_pack_S_half(dgmi_ctx* c, hipStream_t s) {
+  const int nele = (int)c->nele;
+  const dim3 blk(256), grd((nele + 255) / 256);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_pack_S_half<20>, grd, blk, 0, s, nele, (const double*)c->g64.S,
+                       c->g32.Sh, c->g32.Shs);
+  else
+    hipLaunchKernelGGL(k_pack_S_half<10>, grd, blk, 0, s, nele, (const double*)c->g64.S,
+                       c->g32.Sh, c->g32.Shs);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_pack_S_sym(dgmi_ctx* c, const double* S, hipStream_t s) {
   const int nele = (int)c->nele;
   const dim3 blk(256), grd((nele + 255) / 256);
@@ -1586,11 +1677,16 @@ static int sweep_nbx(dgmi_ctx* c, const float* x, const float* b, float* xnew, h
   const dim3 blk(64 * (D + 1)), grd((nele + 63) / 64);
   const int32_t* act = c->mask_on ? c->act64 : nullptr;
   using V2 = typename Vec2<float>::type;
-  if (c->staged)
+  if (c->staged && c->smoother_half && g.Sh)
+    hipLaunchKernelGGL((k_staged<D, 5, true>), grd, blk, 0, s, nele, x, b, xnew,
+                       g.tab + g.off_vtab, g.tab + g.off_wgt, g.G, g.tab + g.off_sw, g.cf, g.wf,
+                       g.pen, c->nbface, c->fcode, g.S, act, (float)c->omega, g.tab + g.off_fc,
+                       c->fnodes, g.cnb, c->nperm, (const uint32_t*)g.Sh, (const float*)g.Shs);
+  else if (c->staged)
     hipLaunchKernelGGL((k_staged<D, 5>), grd, blk, 0, s, nele, x, b, xnew, g.tab + g.off_vtab,
                        g.tab + g.off_wgt, g.G, g.tab + g.off_sw, g.cf, g.wf, g.pen, c->nbface,
                        c->fcode, g.S, act, (float)c->omega, g.tab + g.off_fc, c->fnodes, g.cnb,
-                       c->nperm);
+                       c->nperm, (const uint32_t*)nullptr, (const float*)nullptr);
   else
     hipLaunchKernelGGL((k_residual_w<float, D, 5, false, float, true>), grd, blk, 0, s, nele,
                        const_cast<float*>(x), b, xnew, g.tab + g.off_vtab, g.tab + g.off_wgt,
@@ -1644,13 +1740,19 @@ int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t
     if (c->staged) {
       const int32_t* act = c->mask_on ? c->act64 : nullptr;
       const dim3 grd((nele + 63) / 64);
-#define DGMI_ZERO(D)                                                                          \
-  hipLaunchKernelGGL((k_staged<D, 6>), grd, dim3(64 * (D + 1)), 0, s, nele, x, b, x,           \
+#define DGMI_ZERO(D, H)                                                                       \
+  hipLaunchKernelGGL((k_staged<D, 6, H>), grd, dim3(64 * (D + 1)), 0, s, nele, x, b, x,        \
                      g.tab + g.off_vtab, g.tab + g.off_wgt, g.G, g.tab + g.off_sw, g.cf, g.wf, \
                      g.pen, c->nbface, c->fcode, g.S, act, (float)c->omega, g.tab + g.off_fc, \
-                     c->fnodes, g.cnb, c->nperm)
-      if (c->D == 3) DGMI_ZERO(3);
-      else DGMI_ZERO(2);
+                     c->fnodes, g.cnb, c->nperm, (const uint32_t*)g.Sh, (const float*)g.Shs)
+      const bool half = c->smoother_half && g.Sh;
+      if (c->D == 3) {
+        if (half) DGMI_ZERO(3, true);
+        else DGMI_ZERO(3, false);
+      } else {
+        if (half) DGMI_ZERO(2, true);
+        else DGMI_ZERO(2, false);
+      }
 #undef DGMI_ZERO
       DGMI_HIP(hipGetLastError());
       return 0;

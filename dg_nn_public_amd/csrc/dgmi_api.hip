@@ -1573,6 +1573,14 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
     DGMI_CHECK(hf == 0, "singular diagonal block " + std::to_string(hf - 1));
   }
   DGMI_TRY(launch_pack_S_sym(c, full, s));
+  if (c->smoother_half < 0) c->smoother_half = c->D == 3 ? 1 : 0;
+  if (c->smoother_half) {
+    const int64_t npair = c->nele * ((c->nloc * (c->nloc + 1) / 2 + 1) / 2);
+    if (!c->g32.Sh) c->g32.Sh = static_cast<uint32_t*>(c->alloc(sizeof(uint32_t) * npair));
+    if (!c->g32.Shs) c->g32.Shs = static_cast<float*>(c->alloc(sizeof(float) * c->nele));
+    DGMI_CHECK(c->g32.Sh && c->g32.Shs, alloc_failure());
+    DGMI_TRY(launch_pack_S_half(c, s));
+  }
   c->smoother_variant = variant;
   c->omega = omega;
   return 0;
@@ -2106,6 +2114,11 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
     DGMI_CHECK(value == 0.0 || value == 1.0 || value == 2.0,
                "single_pass must be 0 (never), 1 (applies) or 2 (applies and FP32 sweeps)");
     c->single_pass = (int)value;
+    return 0;
+  }
+  if (k == "smoother_half") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "smoother_half must be 0 or 1");
+    c->smoother_half = (int)value;
     return 0;
   }
   if (k == "staged") {

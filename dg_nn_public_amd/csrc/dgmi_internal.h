@@ -52,6 +52,11 @@ template <typename T> struct DevGeom {
   // bf16 inner cycles (FP32 only, exact block smoother): the packed symmetric smoother in
   // bfloat16
   uint16_t* S16 = nullptr;
+  // FP32 sweeps with "smoother_half": the packed smoother of each element scaled by its
+  // largest magnitude and stored in FP16, two entries per 32-bit word (packed entries 2p
+  // and 2p+1 at Sh[p*nele + e]), and the scale Shs[e]
+  uint32_t* Sh = nullptr;
+  float* Shs = nullptr;
 };
 
 // SFC levels on the device.  "Grid" levels 0..small_level hold every component whose
@@ -270,6 +275,11 @@ struct dgmi_ctx {
   // block's rows read and written through LDS with coalesced vector accesses, each packed
   // smoother entry read once per block), 0 = k_residual_w / k_smooth_zero
   int staged = 1;
+  // FP32 block-staged sweeps read the FP16 copy of the packed smoother (g32.Sh, formed by
+  // dgmi_smoother_setup when set): 3D by default (48^3x6: sweep 188 -> 137 us, 102 -> 107
+  // iterations, solve 9% faster); 2D off (its smoother is a smaller share of the sweep's
+  // bytes, and at high contrast 219 -> 254 iterations)
+  int smoother_half = -1;  // -1: by dimension
   // pre- and post-smoothing sweeps of the FP32 cycles with carried traces (the reference's
   // pre_smooth_its, z02/config.py:112)
   int f32_smooth_its = 3;
@@ -410,6 +420,8 @@ bool bf16_smoother(const dgmi_ctx* c);
 // g64.S / g32.S = packed symmetric FP64 / FP32 copies of the full FP64 smoother S (and
 // g32.S16 its bfloat16 rounding, when allocated)
 int launch_pack_S_sym(dgmi_ctx* c, const double* S, hipStream_t s);
+// g32.Sh / g32.Shs from the packed FP64 smoother g64.S
+int launch_pack_S_half(dgmi_ctx* c, hipStream_t s);
 template <typename T>
 int launch_convert(const double* in, T* out, int64_t n, hipStream_t s);
 // transfers + SFC levels (mg.hip)

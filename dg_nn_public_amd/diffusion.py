@@ -225,6 +225,7 @@ class DGDiffusion:
                                                            C.byref(ctx)),
                       "dgmi_ctx_create_batch_parts")
         self.ctx = ctx
+        self._params = {}
         if alloc_limit:
             self.set_alloc_limit(alloc_limit)
         info = (C.c_int64 * 8)()
@@ -314,9 +315,17 @@ class DGDiffusion:
 
     def set_param(self, name, value):
         """Set a solver parameter of the context (``dgmi_set_param``: e.g. "coarse_exact",
-        "smooth_offdiag", "single_pass", "staged", "f32_smooth_its"; see include/dgmi.h)."""
+        "smooth_offdiag", "single_pass", "staged", "smoother_half", "f32_smooth_its"; see
+        include/dgmi.h)."""
         check(self.lib.dgmi_set_param(self.ctx, str(name).encode(), float(value)),
               "dgmi_set_param")
+        self._params[str(name)] = float(value)
+
+    @property
+    def smoother_half(self):
+        """Whether the FP32 block-staged sweeps read the FP16 copy of the smoother (the
+        library's default: in 3D)."""
+        return bool(self._params.get("smoother_half", 1.0 if self.D == 3 else 0.0))
 
     def set_alloc_limit(self, nbytes):
         """Cap the device bytes this context may hold (0: no cap).  An allocation that would

@@ -276,6 +276,12 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     frame), instead of a trace pass before them: 0 never (traces through
  *                     trace buffers), 1 the applies (dgmi_apply, the solvers' operator and
  *                     residuals), 2 also the FP32 preconditioner's sweeps (default).
+ *   "smoother_half"   FP32 block-staged sweeps read an FP16 copy of the packed smoother
+ *                     (each element's entries divided by its largest magnitude, kept in
+ *                     FP32): 1 on, 0 off; by default on in 3D and off in 2D.  The copy is
+ *                     formed by dgmi_smoother_setup when the parameter is on then (set it
+ *                     before to use it in 2D); FP64 refinement keeps the final accuracy,
+ *                     the iteration count can rise at high contrast.
  *   "staged"          FP32 single-pass applies, sweeps and the sweep from zero: 1 = the
  *                     block-staged kernels (rows read and written through LDS, face terms
  *                     in node form, volume term through the P2 gradient; default); 0 = the

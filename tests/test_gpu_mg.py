@@ -85,6 +85,7 @@ def test_direct_smoother_fp32_neighbour_form(D, n):
     x += S (b - A x)) to single precision and agrees with the FP32 residual form."""
     print(f"[case] D={D} n={n} direct smoother", flush=True)
     m, pb, g, sp = build(D, n, sigma=1.0, blk_solver="direct", face_kappa="harmonic")
+    g.set_param("smoother_half", 0)  # the FP32 smoother (its FP16 copy: test_fp16_smoother_copy)
     g._get_RAR_and_sfc_data_Um()
     mgd = mg.build_mg(sp, m.verts)
     b = np.random.default_rng(6).standard_normal(sp.ndof)
@@ -193,6 +194,7 @@ def test_vector_sweeps_with_carried_traces_equal_matrix_core_sweeps():
     doubly-updated row would show as a large difference."""
     print("[case] D=3 n=12 direct smoother, vector vs mfma preconditioner", flush=True)
     m, pb, g, sp = build(3, 12, sigma=1.0, blk_solver="direct", face_kappa="harmonic")
+    g.set_param("smoother_half", 0)  # both families with the FP32 smoother
     g._get_RAR_and_sfc_data_Um()
     b = to_dev(np.random.default_rng(8).standard_normal(sp.ndof), torch.float32)
     out = {}
@@ -240,3 +242,35 @@ def test_preconditioner_allocation_failure_is_an_error():
     mgd = mg.build_mg(sp, m.verts)
     out = to_np(g._um_left_precond(g.new_vector(), to_dev(b)))
     assert rel(out, mg.um_left_precond(sp, mgd, b)) < 1e-10
+
+
+@pytest.mark.parametrize("D,n", [(2, 10), (3, 5)], ids=["2d", "3d"])
+def test_fp16_smoother_copy(D, n):
+    """The FP32 block-staged sweeps can read an FP16 copy of the packed exact block
+    smoother (each element's entries divided by its largest magnitude; the default in 3D,
+    switched on here for 2D before the set-up).  Against the FP32 smoother: smoothing steps
+    and the preconditioner differ by no more than the FP16 rounding allows (a few 1e-4
+    relative; bound 5e-3), but they do differ (the copy is used), and an FP32-refinement
+    solve with it reaches 1e-10 by the oracle's operator."""
+    print(f"[case] fp16 smoother D={D} n={n}", flush=True)
+    m, pb, g, sp = build(D, n, sigma=1.0, blk_solver="direct", face_kappa="harmonic")
+    g.set_param("smoother_half", 1)
+    g._get_RAR_and_sfc_data_Um()
+    rng = np.random.default_rng(12)
+    x0, b = rng.standard_normal((2, sp.ndof))
+    out = {}
+    for half in (1, 0):
+        g.set_param("smoother_half", half)
+        xs = g.smooth_n(to_dev(x0, torch.float32), to_dev(b, torch.float32), 3)
+        z = g._um_left_precond(g.new_vector(torch.float32), to_dev(b, torch.float32))
+        out[half] = (to_np(xs), to_np(z))
+    d_sweep, d_pc = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
+    print("fp16 vs fp32 smoother: sweeps", d_sweep, "preconditioner", d_pc, flush=True)
+    assert 1e-7 < d_sweep < 5e-3 and 1e-7 < d_pc < 5e-3
+    g.set_param("smoother_half", 1)
+    bb = dg.get_rhs(sp, pb.f, pb.u_bc)
+    xs, its = g._gmres_mg_solver(g.new_vector(), to_dev(bb), rtol=1e-10, inner=torch.float32)
+    relres = np.linalg.norm(bb - dg.apply_A(sp, to_np(xs))) / np.linalg.norm(bb)
+    print("solve", its, relres, flush=True)
+    assert relres <= 2e-10
+    g.close()

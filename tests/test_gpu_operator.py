@@ -243,6 +243,7 @@ def test_smooth_n_matches_repeated_oracle_steps(kernel):
     pb = meshgen.sine_problem(m, sigma=1.0)
     g = DGDiffusion.from_mesh(m, pb.kappa, blk_solver="direct", device=DEV,
                               face_kappa="harmonic")
+    g.set_param("smoother_half", 0)  # FP32 arithmetic throughout (FP16 copy: test_gpu_mg)
     sp = dg.make_space(m, pb.kappa, face_kappa="harmonic")
     _, B = dg.get_bdiag_diag(sp)
     rng = np.random.default_rng(41)
