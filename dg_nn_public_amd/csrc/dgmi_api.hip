@@ -1640,6 +1640,103 @@ int dgmi_smooth_n(dgmi_ctx* c, int dtype, void* x, const void* b, int n, void* s
   return 0;
 }
 
+// coloured probing of RAR (z02/volume_mf_diff.py:1696-1754) on the provisional layout pos:
+// entries grouped by the colour of their column; each row holds at most one entry per colour
+static int rar_by_probing(dgmi_ctx* c, const std::vector<int32_t>& whichc,
+                          const std::vector<int64_t>& pos, hipStream_t s) {
+  MGDev& mg = c->mg;
+  std::vector<int32_t> whichc_sfc(c->nverts);
+  for (int64_t i = 0; i < c->nverts; ++i) whichc_sfc[pos[i]] = whichc[i];
+  int32_t* whichc_d = upload<int32_t>(c, whichc_sfc, s);
+  DGMI_CHECK(whichc_d, alloc_failure());
+  std::vector<int64_t> cnt(mg.ncolor + 1, 0);
+  for (int64_t q = 0; q < mg.ncola; ++q) cnt[whichc[mg.cola[q]]]++;
+  std::vector<int64_t> off(mg.ncolor + 2, 0);
+  for (int cc = 1; cc <= mg.ncolor; ++cc) off[cc + 1] = off[cc] + cnt[cc];
+  std::vector<int32_t> spidx(mg.ncola), rowpos(mg.ncola);
+  {
+    std::vector<int64_t> f2(off.begin(), off.end());
+    for (int64_t i = 0; i < c->nverts; ++i)
+      for (int64_t q = mg.fina[i]; q < mg.fina[i + 1]; ++q) {
+        const int cc = whichc[mg.cola[q]];
+        spidx[f2[cc]] = (int32_t)q;
+        rowpos[f2[cc]] = (int32_t)pos[i];
+        f2[cc]++;
+      }
+  }
+  int32_t* spidx_d = upload<int32_t>(c, spidx, s);
+  int32_t* rowpos_d = upload<int32_t>(c, rowpos, s);
+  double* vals_d = static_cast<double*>(c->alloc(sizeof(double) * mg.ncola));
+  double* probe = vbuf<double>(c, 0);
+  double* aprobe = vbuf<double>(c, 1);
+  double* rsfc = static_cast<double*>(c->alloc(sizeof(double) * c->nverts));
+  DGMI_CHECK(spidx_d && rowpos_d && vals_d && probe && aprobe && rsfc, alloc_failure());
+  for (int cc = 1; cc <= mg.ncolor; ++cc) {
+    DGMI_TRY(launch_probe_mask(c, whichc_d, cc, probe, s));
+    DGMI_TRY(launch_apply<double>(c, c->g64, probe, nullptr, aprobe, s));  // -A R m
+    DGMI_TRY(launch_restrict<double>(c, aprobe, rsfc, s));
+    DGMI_TRY(launch_probe_collect(c, rsfc, spidx_d + off[cc], rowpos_d + off[cc],
+                                  off[cc + 1] - off[cc], vals_d, s));
+  }
+  mg.rar.assign(mg.ncola, 0.0);
+  DGMI_TRY(d2h(mg.rar.data(), vals_d, sizeof(double) * mg.ncola, s));
+  return 0;
+}
+
+// The P1-reduced reference tables of k_rar_elem: SP[j][k] = P^T Sref_jk P (D*D) | MP[f] =
+// P^T M_f P (nface) | CP[f][d] = P^T C_fd P (nface*D), each (D+1)^2, P the P1
+// interpolation at the P3 nodes (nloc, D+1)
+static std::vector<double> rar_refmats(const ElementTables& el) {
+  const int D = el.D, NL = el.nloc, NF = el.nface, NB = D + 1, NN = NL * NL, NB2 = NB * NB;
+  const std::vector<double> r = bdiag_refmats(el);
+  const int nmat = D * D + NF + NF * D;  // the leading matrices of bdiag_refmats, in order
+  std::vector<double> out((size_t)nmat * NB2, 0.0);
+  const double* P = el.p1_prolong.data();
+  for (int t = 0; t < nmat; ++t) {
+    const double* A = r.data() + (size_t)t * NN;
+    for (int a = 0; a < NB; ++a)
+      for (int b = 0; b < NB; ++b) {
+        double s = 0.0;
+        for (int n = 0; n < NL; ++n) {
+          double u = 0.0;
+          for (int m = 0; m < NL; ++m) u += A[n * NL + m] * P[m * NB + b];
+          s += P[n * NB + a] * u;
+        }
+        out[(size_t)t * NB2 + a * NB + b] = s;
+      }
+  }
+  return out;
+}
+
+// RAR assembled from the continuous P1 element matrices (k_rar_elem, k_rar_gather) on the
+// provisional layout pos
+static int rar_assembled(dgmi_ctx* c, const std::vector<int64_t>& pos, hipStream_t s) {
+  MGDev& mg = c->mg;
+  const int64_t nv = c->nverts;
+  std::vector<int32_t> rlo(nv), rhi(nv), colpos(mg.ncola);
+  parallel_chunks(nv, [&](int, int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+      rlo[pos[i]] = (int32_t)mg.fina[i];
+      rhi[pos[i]] = (int32_t)mg.fina[i + 1];
+      for (int64_t k = mg.fina[i]; k < mg.fina[i + 1]; ++k)
+        colpos[k] = (int32_t)pos[mg.cola[k]];
+    }
+  });
+  double* rarm = upload<double>(c, rar_refmats(c->el), s);
+  int32_t* rlo_d = upload<int32_t>(c, rlo, s);
+  int32_t* rhi_d = upload<int32_t>(c, rhi, s);
+  int32_t* colpos_d = upload<int32_t>(c, colpos, s);
+  double* ke = vbuf<double>(c, 1);  // nele*nloc >= nele*(D+1)^2
+  double* vals_d = static_cast<double*>(c->alloc(sizeof(double) * (mg.ncola + 1)));
+  DGMI_CHECK(rarm && rlo_d && rhi_d && colpos_d && ke && vals_d, alloc_failure());
+  DGMI_TRY(launch_rar_assemble(c, rarm, ke, rlo_d, rhi_d, colpos_d, mg.ncola, vals_d, s));
+  mg.rar.assign(mg.ncola + 1, 0.0);
+  DGMI_TRY(d2h(mg.rar.data(), vals_d, sizeof(double) * (mg.ncola + 1), s));
+  DGMI_CHECK(mg.rar[mg.ncola] == 0.0, "element vertex pair without a P1CG entry");
+  mg.rar.resize(mg.ncola);
+  return 0;
+}
+
 int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   DGMI_CHECK(c, "null context");
   hipStream_t s = (hipStream_t)stream;
@@ -1665,15 +1762,20 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
       for (int64_t k = mg.fina[i]; k < mg.fina[i + 1]; ++k)
         DGMI_CHECK(mg.cola[k] >= c->comp_voff[q] && mg.cola[k] < c->comp_voff[q + 1],
                    "components share P1 vertices");
-  // distance-2 colouring, each component on its own (the colouring of the union in
-  // vertex order is the components' colourings side by side)
-  std::vector<int32_t> whichc(c->nverts, 0);
-  std::vector<int> ncq(nc, 0);
-  parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
-    for (int64_t q = q0; q < q1; ++q)
-      ncq[q] = color2_range(mg.fina, mg.cola, c->comp_voff[q], c->comp_voff[q + 1], whichc);
-  }, 1);
-  mg.ncolor = *std::max_element(ncq.begin(), ncq.end());
+  DGMI_CHECK(mg.ncola < INT32_MAX, "P1CG pattern above the 32-bit index range");
+  // distance-2 colouring for the probing, each component on its own (the colouring of the
+  // union in vertex order is the components' colourings side by side)
+  const bool probing = c->rar_probing != 0;
+  std::vector<int32_t> whichc(probing ? c->nverts : 0, 0);
+  mg.ncolor = 0;
+  if (probing) {
+    std::vector<int> ncq(nc, 0);
+    parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
+      for (int64_t q = q0; q < q1; ++q)
+        ncq[q] = color2_range(mg.fina, mg.cola, c->comp_voff[q], c->comp_voff[q + 1], whichc);
+    }, 1);
+    mg.ncolor = *std::max_element(ncq.begin(), ncq.end());
+  }
   // Hilbert order of each component's own vertices (local 1-based positions)
   std::vector<std::vector<int64_t>> lsfc(nc);
   parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
@@ -1717,43 +1819,10 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     for (int64_t i = 0; i < c->comp_voff[q + 1] - c->comp_voff[q]; ++i)
       pos[c->comp_voff[q] + i] = c->comp_voff[q] + lsfc[q][i] - 1;
   DGMI_TRY(build_transfers(pos, c->nverts));
-  std::vector<int32_t> whichc_sfc(c->nverts);
-  for (int64_t i = 0; i < c->nverts; ++i) whichc_sfc[pos[i]] = whichc[i];
-  int32_t* whichc_d = upload<int32_t>(c, whichc_sfc, s);
-  DGMI_CHECK(whichc_d, alloc_failure());
-  // coloured probing of RAR (z02/volume_mf_diff.py:1696-1754): entries grouped by the
-  // colour of their column; each row holds at most one entry per colour
-  std::vector<int64_t> cnt(mg.ncolor + 1, 0);
-  for (int64_t q = 0; q < mg.ncola; ++q) cnt[whichc[mg.cola[q]]]++;
-  std::vector<int64_t> off(mg.ncolor + 2, 0);
-  for (int cc = 1; cc <= mg.ncolor; ++cc) off[cc + 1] = off[cc] + cnt[cc];
-  std::vector<int32_t> spidx(mg.ncola), rowpos(mg.ncola);
-  {
-    std::vector<int64_t> f2(off.begin(), off.end());
-    for (int64_t i = 0; i < c->nverts; ++i)
-      for (int64_t q = mg.fina[i]; q < mg.fina[i + 1]; ++q) {
-        const int cc = whichc[mg.cola[q]];
-        spidx[f2[cc]] = (int32_t)q;
-        rowpos[f2[cc]] = (int32_t)pos[i];
-        f2[cc]++;
-      }
-  }
-  int32_t* spidx_d = upload<int32_t>(c, spidx, s);
-  int32_t* rowpos_d = upload<int32_t>(c, rowpos, s);
-  double* vals_d = static_cast<double*>(c->alloc(sizeof(double) * mg.ncola));
-  double* probe = vbuf<double>(c, 0);
-  double* aprobe = vbuf<double>(c, 1);
-  double* rsfc = static_cast<double*>(c->alloc(sizeof(double) * c->nverts));
-  DGMI_CHECK(spidx_d && rowpos_d && vals_d && probe && aprobe && rsfc, alloc_failure());
-  for (int cc = 1; cc <= mg.ncolor; ++cc) {
-    DGMI_TRY(launch_probe_mask(c, whichc_d, cc, probe, s));
-    DGMI_TRY(launch_apply<double>(c, c->g64, probe, nullptr, aprobe, s));  // -A R m
-    DGMI_TRY(launch_restrict<double>(c, aprobe, rsfc, s));
-    DGMI_TRY(launch_probe_collect(c, rsfc, spidx_d + off[cc], rowpos_d + off[cc],
-                                  off[cc + 1] - off[cc], vals_d, s));
-  }
-  mg.rar.assign(mg.ncola, 0.0);
-  DGMI_TRY(d2h(mg.rar.data(), vals_d, sizeof(double) * mg.ncola, s));
+  if (probing)
+    DGMI_TRY(rar_by_probing(c, whichc, pos, s));
+  else
+    DGMI_TRY(rar_assembled(c, pos, s));
   mark(PH_PROBE);
   // SFC levels of every component (z00-fortran-src/compact_fcode.f90:7-225)
   SfcLayout lay;
@@ -2119,6 +2188,11 @@ int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   if (k == "smoother_half") {
     DGMI_CHECK(value == 0.0 || value == 1.0, "smoother_half must be 0 or 1");
     c->smoother_half = (int)value;
+    return 0;
+  }
+  if (k == "rar_probing") {
+    DGMI_CHECK(value == 0.0 || value == 1.0, "rar_probing must be 0 or 1");
+    c->rar_probing = (int)value;
     return 0;
   }
   if (k == "staged") {

@@ -276,10 +276,17 @@ struct dgmi_ctx {
   // smoother entry read once per block), 0 = k_residual_w / k_smooth_zero
   int staged = 1;
   // FP32 block-staged sweeps read the FP16 copy of the packed smoother (g32.Sh, formed by
-  // dgmi_smoother_setup when set): 3D by default (48^3x6: sweep 188 -> 137 us, 102 -> 107
+  // dgmi_smoother_setup when set): 3D by default (48^3x6: sweep 188 -> 158 us, 102 -> 107
   // iterations, solve 9% faster); 2D off (its smoother is a smaller share of the sweep's
   // bytes, and at high contrast 219 -> 254 iterations)
   int smoother_half = -1;  // -1: by dimension
+  // P1CG operator RAR (read by dgmi_mg_setup): 1 = coloured probing with FP64 operator
+  // applies as the reference does; 0 = assembled from the element matrices of the continuous
+  // P1 hat functions, where the interior-face terms vanish (every one carries a jump), so
+  // RAR[i][j] = sum over the elements holding i and j of the P1-reduced volume and
+  // Dirichlet-face terms, over mult_i mult_j.  Equal to rounding; 0 needs no colouring and
+  // no operator applies.
+  int rar_probing = 0;
   // pre- and post-smoothing sweeps of the FP32 cycles with carried traces (the reference's
   // pre_smooth_its, z02/config.py:112)
   int f32_smooth_its = 3;
@@ -472,6 +479,13 @@ int launch_probe_mask(dgmi_ctx* c, const int32_t* whichc_dev, int color, double*
                       hipStream_t s);
 int launch_probe_collect(dgmi_ctx* c, const double* r_sfc, const int32_t* spidx,
                          const int32_t* rowpos, int64_t nent, double* values, hipStream_t s);
+// RAR assembled (rar_probing 0): ke = the P1-reduced element matrices (rarm = the reduced
+// reference tables), then vals[k] for every CSR slot of the rows by position (slots
+// [rlo[p], rhi[p]) of position p, column positions colpos); vals[nslot] = 1 when an element
+// pair has no slot
+int launch_rar_assemble(dgmi_ctx* c, const double* rarm, double* ke, const int32_t* rlo,
+                        const int32_t* rhi, const int32_t* colpos, int64_t nslot,
+                        double* vals, hipStream_t s);
 // vector kernels (krylov.hip)
 // mode 0: out[0] = x.y ; mode 1: out[0] = x.y, out[1] = sqrt(x.y)
 template <typename T>

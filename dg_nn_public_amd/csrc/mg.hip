@@ -185,6 +185,96 @@ __global__ void k_probe_collect(int64_t nent, const int32_t* __restrict__ spidx,
   if (k < nent) values[spidx[k]] = -r[rowpos[k]];
 }
 
+// RAR assembled from the element matrices of the continuous P1 hat functions.  For a P1CG
+// vector v, P I_fc v is the continuous P1 function of v/mult, and I_cf sums the test
+// functions of every vertex over its elements, so RAR[i][j] = a(phi_i, phi_j) / (mult_i
+// mult_j) with a the SIPG form: its interior-face terms all carry a jump of u or v and
+// vanish, leaving the volume term and the Dirichlet faces' terms.  Per element (one
+// thread per entry (a, b)), with the reference tables reduced to the vertex functions
+// (rarm: SP[j][k] = P^T Sref_jk P | MP[f] = P^T M_f P | CP[f][d] = P^T C_fd P):
+//   ke[a][b] = sum_q G_q (SP_jk + SP_kj)[a][b]
+//            + sum_{Dirichlet f} wf_f (pen_f MP_f[a][b] - sum_d c_fd (CP_fd[a][b] + CP_fd[b][a]))
+// the terms of k_bdiag's element block with the interior faces left out.
+template <int D>
+__global__ __launch_bounds__(256) void k_rar_elem(int nele, const double* __restrict__ rarm,
+                                                  const double* __restrict__ G,
+                                                  const double* __restrict__ cf,
+                                                  const double* __restrict__ wf,
+                                                  const double* __restrict__ pen,
+                                                  const int32_t* __restrict__ fcode,
+                                                  double* __restrict__ ke,
+                                                  double* __restrict__ miss) {
+  constexpr int NB = D + 1, NB2 = NB * NB, NF = D + 1;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid == 0) *miss = 0.0;
+  if (tid >= (int64_t)nele * NB2) return;
+  const int e = (int)(tid / NB2);
+  const int ab = (int)(tid - (int64_t)e * NB2);
+  const int a = ab / NB, b = ab - (ab / NB) * NB;
+  const double* SP = rarm;
+  const double* MP = SP + D * D * NB2;
+  const double* CP = MP + NF * NB2;
+  constexpr int NQ = D == 3 ? 6 : 3;
+  constexpr int jj[6] = {0, 1, 2, 0, 0, 1}, kk[6] = {0, 1, 2, 1, 2, 2};
+  constexpr int jj2[3] = {0, 1, 0}, kk2[3] = {0, 1, 1};
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int j = D == 3 ? jj[q] : jj2[q], k = D == 3 ? kk[q] : kk2[q];
+    double v = SP[(j * D + k) * NB2 + ab];
+    if (j != k) v += SP[(k * D + j) * NB2 + ab];
+    s += G[q * nele + e] * v;
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    if ((fcode[e * NF + f] & 15) != 1) continue;
+    double cs = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      cs += cf[(f * D + d) * nele + e] *
+            (CP[(f * D + d) * NB2 + a * NB + b] + CP[(f * D + d) * NB2 + b * NB + a]);
+    s += wf[f * nele + e] * (pen[f * nele + e] * MP[f * NB2 + ab] - cs);
+  }
+  ke[(size_t)e * NB2 + ab] = s;
+}
+
+// One thread per P1CG row (by position p): its slots [rlo[p], rhi[p]) with column positions
+// colpos are summed over the row vertex's element entries (rptr/rent, in a fixed order, so
+// the result does not depend on scheduling) and scaled by rw_p rw_j.
+template <int D>
+__global__ __launch_bounds__(256) void k_rar_gather(int ncg, const int32_t* __restrict__ rptr,
+                                                    const int32_t* __restrict__ rent,
+                                                    const int32_t* __restrict__ cellpos,
+                                                    const double* __restrict__ rw,
+                                                    const double* __restrict__ ke,
+                                                    const int32_t* __restrict__ rlo,
+                                                    const int32_t* __restrict__ rhi,
+                                                    const int32_t* __restrict__ colpos,
+                                                    double* __restrict__ vals,
+                                                    double* __restrict__ miss) {
+  constexpr int NB = D + 1, NB2 = NB * NB;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ncg) return;
+  const int k0 = rlo[p], k1 = rhi[p];
+  for (int k = k0; k < k1; ++k) vals[k] = 0.0;
+  for (int t = rptr[p]; t < rptr[p + 1]; ++t) {
+    const int q = rent[t];
+    const int e = q / NB, a = q - e * NB;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int jp = cellpos[e * NB + b];
+      int k = k0;
+      while (k < k1 && colpos[k] != jp) ++k;
+      if (k < k1)
+        vals[k] += ke[(size_t)e * NB2 + a * NB + b];
+      else
+        *miss = 1.0;
+    }
+  }
+  const double wp = rw[p];
+  for (int k = k0; k < k1; ++k) vals[k] *= wp * rw[colpos[k]];
+}
+
 // ---------------------------------------------------------------------------------------
 // SFC level kernels.  Jacobi sweep (z02/multigrid_linearelastic.py:25-39):
 //   eout = ein + omega (b - A ein) / diag
@@ -725,6 +815,29 @@ int launch_probe_collect(dgmi_ctx*, const double* r_sfc, const int32_t* spidx,
   if (nent == 0) return 0;
   const dim3 blk(256), grd((unsigned)((nent + 255) / 256));
   hipLaunchKernelGGL(k_probe_collect, grd, blk, 0, s, nent, spidx, rowpos, r_sfc, values);
+  DGMI_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_rar_assemble(dgmi_ctx* c, const double* rarm, double* ke, const int32_t* rlo,
+                        const int32_t* rhi, const int32_t* colpos, int64_t nslot,
+                        double* vals, hipStream_t s) {
+  const int nele = (int)c->nele, nb = c->D + 1;
+  const int ncg = (int)c->mg.cg_nonods;
+  double* miss = vals + nslot;
+  const dim3 blk(256), grd((unsigned)(((int64_t)nele * nb * nb + 255) / 256));
+  const dim3 grd2((unsigned)((ncg + 255) / 256));
+  if (c->D == 3) {
+    hipLaunchKernelGGL(k_rar_elem<3>, grd, blk, 0, s, nele, rarm, c->g64.G, c->g64.cf,
+                       c->g64.wf, c->g64.pen, c->fcode, ke, miss);
+    hipLaunchKernelGGL(k_rar_gather<3>, grd2, blk, 0, s, ncg, c->mg.rptr, c->mg.rent,
+                       c->mg.cellpos, c->mg.rw64, ke, rlo, rhi, colpos, vals, miss);
+  } else {
+    hipLaunchKernelGGL(k_rar_elem<2>, grd, blk, 0, s, nele, rarm, c->g64.G, c->g64.cf,
+                       c->g64.wf, c->g64.pen, c->fcode, ke, miss);
+    hipLaunchKernelGGL(k_rar_gather<2>, grd2, blk, 0, s, ncg, c->mg.rptr, c->mg.rent,
+                       c->mg.cellpos, c->mg.rw64, ke, rlo, rhi, colpos, vals, miss);
+  }
   DGMI_HIP(hipGetLastError());
   return 0;
 }

@@ -441,8 +441,10 @@ class DGDiffusion:
 
     # ---------------------------------------------------------------- multigrid
     def _get_RAR_and_sfc_data_Um(self):
-        """P1CG operator RAR by coloured probing + SFC level operators
-        (z02/volume_mf_diff.py:1642-1693); sets ``sfc_data_Um`` as the reference does."""
+        """P1CG operator RAR + SFC level operators (z02/volume_mf_diff.py:1642-1693); sets
+        ``sfc_data_Um`` as the reference does.  RAR is assembled from the continuous P1
+        element matrices unless ``set_param("rar_probing", 1)`` asks for the reference's
+        coloured probing (z02/volume_mf_diff.py:1696-1754); the two agree to rounding."""
         self._ensure_smoother()
         check(self.lib.dgmi_mg_setup(self.ctx, _stream()), "dgmi_mg_setup")
         info = (C.c_int64 * 128)()

@@ -148,10 +148,12 @@ int dgmi_smooth(dgmi_ctx* ctx, int dtype, void* x, const void* b, void* r_out, v
 int dgmi_smooth_n(dgmi_ctx* ctx, int dtype, void* x, const void* b, int n, void* stream);
 
 /* Multigrid set-up (z02/volume_mf_diff.py:1642-1754, z02/multigrid_linearelastic.py:175-283):
- * P1CG sparsity, distance-2 colouring, RAR by coloured probing with the FP64 apply,
- * SFC ordering, SFC level operators.  Synchronises the stream. */
+ * P1CG sparsity, RAR (assembled from the continuous P1 element matrices by default; with
+ * "rar_probing" 1, distance-2 colouring and coloured probing with the FP64 apply as the
+ * reference), SFC ordering, SFC level operators.  Synchronises the stream. */
 int dgmi_mg_setup(dgmi_ctx* ctx, void* stream);
-/* info[0] = cg_nonods, info[1] = ncola, info[2] = ncolor, info[3] = nlevel,
+/* info[0] = cg_nonods, info[1] = ncola, info[2] = ncolor (0 when RAR was assembled),
+ * info[3] = nlevel,
  * info[4..4+nlevel) = nodes per level */
 int dgmi_mg_info(const dgmi_ctx* ctx, int64_t* info, int cap);
 /* host copies: fina (cg+1), cola (ncola), rar values (ncola), sfc (cg, 1-based) */
@@ -282,6 +284,12 @@ int dgmi_autotune(dgmi_ctx* ctx, int reps, int32_t* choice, double* times_us, vo
  *                     formed by dgmi_smoother_setup when the parameter is on then (set it
  *                     before to use it in 2D); FP64 refinement keeps the final accuracy,
  *                     the iteration count can rise at high contrast.
+ *   "rar_probing"     how dgmi_mg_setup forms RAR: 0 = assembled (default): on continuous
+ *                     P1 functions the interior-face terms of the form vanish, so RAR[i][j]
+ *                     sums the P1-reduced volume and Dirichlet-face element terms over the
+ *                     elements holding both vertices, over mult_i mult_j (no colouring, no
+ *                     operator applies); 1 = coloured probing with the FP64 apply as the
+ *                     reference (z02/volume_mf_diff.py:1696-1754).  Equal to rounding.
  *   "staged"          FP32 single-pass applies, sweeps and the sweep from zero: 1 = the
  *                     block-staged kernels (rows read and written through LDS, face terms
  *                     in node form, volume term through the P2 gradient; default); 0 = the

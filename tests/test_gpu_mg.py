@@ -1,10 +1,12 @@
-"""HIP multigrid (RAR by coloured probing, transfers, SFC V-cycle, preconditioner)
-against the FP64 oracle."""
+"""HIP multigrid (RAR assembled and by coloured probing, transfers, SFC V-cycle,
+preconditioner) against the FP64 oracle."""
 import numpy as np
 import pytest
 import torch
 
-from gpu_common import build, rel, to_dev, to_np
+from gpu_common import DEV, build, rel, to_dev, to_np
+from dg_nn_public_amd import meshgen
+from dg_nn_public_amd.diffusion import DGDiffusion
 from oracle import dg, mg
 
 pytestmark = pytest.mark.gpu
@@ -27,6 +29,44 @@ def test_rar_and_sfc_match_oracle(prob):
     assert np.array_equal(sfc, mgd.levels.sfc)
     assert rel(rar, mgd.rar) < 1e-12
     assert g.nodes_per_level == mgd.levels.nodes_per_level
+
+
+def _mixed_faces(m, seed):
+    """glb_bcface_type with every boundary kind: Dirichlet (0), Neumann (1) and no term (2)
+    drawn per boundary face; interior faces -1."""
+    _, _, bct = dg.topology(m.cells, m.D)
+    rng = np.random.default_rng(seed)
+    b = bct >= 0
+    bct[b] = rng.integers(0, 3, int(b.sum()))
+    return bct
+
+
+@pytest.mark.parametrize("D,n", [(2, 6), (3, 3)])
+def test_rar_assembled_and_probed_match_oracle(D, n):
+    """RAR from the continuous P1 element matrices (the default) and by the reference's
+    coloured probing both equal the oracle's coloured probing (z02/volume_mf_diff.py:
+    1696-1754), on a mesh with every face kind and a 1e4 diffusivity contrast."""
+    m = meshgen.square_mesh(n, seed=7) if D == 2 else meshgen.cube_mesh(n, seed=7)
+    kap = 10.0 ** np.random.default_rng(8).uniform(-2, 2, m.nele)
+    bct = _mixed_faces(m, 9)
+    print(f"[case] D={D} n={n} faces {np.bincount(bct + 1)}", flush=True)
+    sp = dg.make_space(m, kap, bctype=bct)
+    mgd = mg.build_mg(sp, m.verts)
+    out = {}
+    for probing in (0, 1):
+        g = DGDiffusion.from_mesh(m, kap, bctype=bct, device=DEV)
+        g.set_param("rar_probing", probing)
+        g._get_RAR_and_sfc_data_Um()
+        fina, cola, rar, sfc = g.mg_export()
+        assert np.array_equal(fina, mgd.fina) and np.array_equal(cola, mgd.cola)
+        err = rel(rar, mgd.rar)
+        print(f"rar_probing={probing} ncolor={g.ncolor} rel err {err:.2e}")
+        assert err < 1e-12
+        assert g.ncolor == (int(mg.color2(mgd.fina, mgd.cola, m.nverts)[1]) if probing else 0)
+        out[probing] = rar
+        g.close()
+    # the two methods agree with each other to rounding
+    assert rel(out[0], out[1]) < 1e-12
 
 
 def test_restrict_and_prolong_match_oracle(prob):
