@@ -1132,61 +1132,61 @@ __global__ __launch_bounds__(256) void k_smooth_zero(int nele, T* __restrict__ x
 // bdiagK[e][n][m] = sum_{j,k} G_jk Sref_jk[n][m]
 //   + sum_f wf_f (pen_f M_f[n][m] - alpha_f sum_d c_fd (C_fd[n][m] + C_fd[m][n]))
 // alpha = 1/2 on interior faces, 1 on Dirichlet faces (z02/get_bdiag_diag.py:105-284).
-// refmats: Sref (D*D*NL*NL) | M (NF*NL*NL) | C (NF*D*NL*NL), FP64.
+// Written as a sum of NT fixed reference matrices with per-element coefficients,
+//   bdiagK[e] = sum_t coef[e][t] T_t,
+// T: the symmetrised Sref_jk (NQ), then per face M_f and C_fd + C_fd^T (1 + D each), with
+// coef = G_q | wf pen, -alpha wf c_fd (zero on faces without a term).  A work-group takes
+// EB elements: their coefficients go to LDS, and thread nm keeps column nm of the NT
+// tables in registers, so each output costs NT FMAs with broadcast LDS reads and the
+// rows are written contiguously.
+template <int D> struct BdiagDims {
+  static constexpr int NL = D == 3 ? 20 : 10, NN = NL * NL, NF = D + 1;
+  static constexpr int NQ = D == 3 ? 6 : 3, NT = NQ + NF * (1 + D);
+  static constexpr int THREADS = (NN + 63) / 64 * 64, EB = 64;
+};
 template <int D>
-__global__ __launch_bounds__(256) void k_bdiag(int nele, const double* __restrict__ refm,
-                                               const double* __restrict__ G,
-                                               const double* __restrict__ cf,
-                                               const double* __restrict__ wf,
-                                               const double* __restrict__ pen,
-                                               const int32_t* __restrict__ fcode,
-                                               double* __restrict__ bdiag) {
-  using M = Dm<D>;
-  constexpr int NL = M::NL, NN = NL * NL;
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (int64_t)nele * NN) return;
-  const int e = (int)(tid / NN);
-  const int nm = (int)(tid - (int64_t)e * NN);
-  const int n = nm / NL, m = nm - (nm / NL) * NL;
-  const double* Sref = refm;
-  const double* Mf = refm + D * D * NN;
-  const double* Cf = Mf + M::NF * NN;
-  double s = 0.0;
-  if constexpr (D == 3) {
-    const int jj[6] = {0, 1, 2, 0, 0, 1}, kk[6] = {0, 1, 2, 1, 2, 2};
+__global__ __launch_bounds__(BdiagDims<D>::THREADS) void k_bdiag(
+    int nele, const double* __restrict__ tab, const double* __restrict__ G,
+    const double* __restrict__ cf, const double* __restrict__ wf,
+    const double* __restrict__ pen, const int32_t* __restrict__ fcode,
+    double* __restrict__ bdiag) {
+  using B = BdiagDims<D>;
+  constexpr int NN = B::NN, NF = B::NF, NQ = B::NQ, NT = B::NT, EB = B::EB;
+  __shared__ double cs[EB][NT + 1];
+  const int e0 = blockIdx.x * EB;
+  const int t = threadIdx.x;
+  if (t < EB && e0 + t < nele) {
+    const int e = e0 + t;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const double gq = G[q * nele + e];
-      const int j = jj[q], k = kk[q];
-      double v = Sref[(j * D + k) * NN + nm];
-      if (j != k) v += Sref[(k * D + j) * NN + nm];
-      s += gq * v;
-    }
-  } else {
-    const int jj[3] = {0, 1, 0}, kk[3] = {0, 1, 1};
+    for (int q = 0; q < NQ; ++q) cs[t][q] = G[(size_t)q * nele + e];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const double gq = G[q * nele + e];
-      const int j = jj[q], k = kk[q];
-      double v = Sref[(j * D + k) * NN + nm];
-      if (j != k) v += Sref[(k * D + j) * NN + nm];
-      s += gq * v;
+    for (int f = 0; f < NF; ++f) {
+      const int type = fcode[(size_t)e * NF + f] & 15;
+      double* o = &cs[t][NQ + f * (1 + D)];
+      if (type >= 2) {
+#pragma unroll
+        for (int d = 0; d <= D; ++d) o[d] = 0.0;
+        continue;
+      }
+      const double alpha = type == 0 ? 0.5 : 1.0;
+      const double w = wf[(size_t)f * nele + e];
+      o[0] = w * pen[(size_t)f * nele + e];
+#pragma unroll
+      for (int d = 0; d < D; ++d) o[1 + d] = -alpha * w * cf[(size_t)(f * D + d) * nele + e];
     }
   }
-  for (int f = 0; f < M::NF; ++f) {
-    const int type = fcode[e * M::NF + f] & 15;
-    if (type >= 2) continue;
-    const double alpha = type == 0 ? 0.5 : 1.0;
-    double t = pen[f * nele + e] * Mf[f * NN + nm];
-    double cs = 0.0;
+  __syncthreads();
+  if (t >= NN) return;
+  double tv[NT];
 #pragma unroll
-    for (int d = 0; d < D; ++d)
-      cs += cf[(f * D + d) * nele + e] *
-            (Cf[(f * D + d) * NN + n * NL + m] + Cf[(f * D + d) * NN + m * NL + n]);
-    t -= alpha * cs;
-    s += wf[f * nele + e] * t;
+  for (int k = 0; k < NT; ++k) tv[k] = tab[k * NN + t];
+  const int ne = min(EB, nele - e0);
+  for (int j = 0; j < ne; ++j) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) acc = fma(cs[j][k], tv[k], acc);
+    bdiag[(size_t)(e0 + j) * NN + t] = acc;
   }
-  bdiag[(size_t)e * NN + nm] = s;
 }
 
 // Right-hand side (z02/volume_mf_diff.py:195-239 volume, :298-357 Dirichlet faces with the
@@ -1298,67 +1298,208 @@ __global__ __launch_bounds__(256) void k_build_S_jacobi(int nele, const double* 
 }
 
 // omega * B^-1 for every element (the exact block smoother of
-// z05/volume_mf_um_diff_varprop.py:384-392), FP64.  One wave per element: Gauss-Jordan
-// elimination with partial pivoting on [B | I] in LDS.  A singular block sets *fail to
-// 1 + its element index.
+// z05/volume_mf_um_diff_varprop.py:384-392) in FP64, written straight into the packed
+// symmetric smoother copies: P64 (FP64) and S32 (FP32) entry (a, b), a <= b, =
+// (S[a][b] + S[b][a]) / 2 with S = omega B^-1; S16 (bfloat16) when set; H/Hs (FP16 pairs
+// over the element's largest magnitude, which Hs keeps) when set.
+// A wave holds 64 / NL elements, row r of element g in the registers of lane g NL + r:
+// in-place Gauss-Jordan with partial pivoting.  The pivot of column c is the row, among
+// those not yet used, with the largest |a[r][c]| to 15 mantissa bits (lowest row among
+// those); the pivot keys and then the pivot row go through LDS to the element's lanes,
+// and rows are never swapped, so row p_c ends up holding row c of B^-1 with columns
+// permuted: stored[p_c][j] = B^-1[c][p_j].  The blocks come in through LDS with coalesced
+// loads, the rows go back there, and the work-group forms the packed entries and writes
+// EB consecutive elements of every packed vector together.  A singular block sets *fail
+// to 1 + its element index.
+// Orders LDS writes and reads among the lanes of one wave (its LDS operations execute in
+// order; the fences keep the compiler from moving accesses across)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int SDIR_WAVES = 4;
+template <int NL> struct SdirDims {
+  static constexpr int G = 64 / NL;                // elements per wave
+  static constexpr int EB = G * SDIR_WAVES;        // elements per work-group
+  static constexpr int NT = 64 * SDIR_WAVES;       // threads
+  static constexpr int NS = sym_size<NL>(), NP = (NS + 1) / 2;
+  static constexpr int PPT = (NS * EB + NT - 1) / NT;  // packed entries per thread
+  static constexpr int RP = NT / EB;                // threads per element in the max
+};
 template <int NL>
-__global__ __launch_bounds__(256) void k_build_S_direct(int nele, const double* __restrict__ B,
-                                                        double omega, double* __restrict__ S64,
-                                                        int* __restrict__ fail) {
-  constexpr int W2 = 2 * NL + 1;  // [B | I] row, padded
-  __shared__ double a[4][NL][W2];
+__global__ __launch_bounds__(64 * SDIR_WAVES) void k_build_S_direct(
+    int nele, const double* __restrict__ B, double omega, double* __restrict__ P64,
+    float* __restrict__ S32, uint16_t* __restrict__ S16, uint32_t* __restrict__ H,
+    float* __restrict__ Hs, int* __restrict__ fail) {
+  using Q = SdirDims<NL>;
+  constexpr int G = Q::G, EB = Q::EB, NT = Q::NT, NS = Q::NS, NP = Q::NP;
+  constexpr int PPT = Q::PPT, RP = Q::RP;
+  static_assert(NS * EB <= EB * NL * NL, "packed entries fit in the row storage");
+  __shared__ __align__(16) double L[EB * NL * NL];  // blocks/rows [le][r][j]; packed [i][le]
+  constexpr int NK = (NL + 3) / 4 * 4;                // key rows padded to 16 bytes
+  __shared__ __align__(16) uint32_t ck[EB][NK];        // pivot keys of column c
+  __shared__ __align__(16) double pv[EB][NL];          // the pivot row, scaled
+  __shared__ int pr[EB][NL], qi[EB][NL];
+  __shared__ uint8_t ab[NS][2];  // (a, b) of packed entry i
+  __shared__ double red[RP][EB];
+  __shared__ double mx[EB];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int e = blockIdx.x * 4 + w;
-  const bool valid = e < nele;  // every wave runs the same steps (block barriers)
-  const double* Be = B + (size_t)(valid ? e : 0) * NL * NL;
-  for (int k = lane; k < NL * 2 * NL; k += 64) {
-    const int r = k / (2 * NL), cc = k % (2 * NL);
-    a[w][r][cc] = cc < NL ? Be[r * NL + cc] : (cc - NL == r ? 1.0 : 0.0);
+  const int g = lane / NL, r = lane - (lane / NL) * NL;
+  const bool row = g < G;  // lanes past G * NL hold no row
+  const int le = w * G + (row ? g : 0);  // row-less lanes mirror the wave's first element
+  const int e0 = blockIdx.x * EB;
+  const bool act = row && e0 + le < nele;
+  // the work-group's blocks are contiguous in B: coalesced 16-byte loads into LDS, then
+  // each lane takes its row from there
+  const int ne = min(EB, nele - e0);
+  {
+    const double2* src = reinterpret_cast<const double2*>(B + (size_t)e0 * NL * NL);
+    double2* dst = reinterpret_cast<double2*>(L);
+    for (int q = threadIdx.x; q < ne * NL * NL / 2; q += NT) dst[q] = src[q];
   }
   __syncthreads();
-  bool singular = false;
-  for (int c = 0; c < NL; ++c) {
-    // pivot: largest |a[r][c]|, r >= c (lanes r - c), first index on ties
-    double best = lane + c < NL ? fabs(a[w][lane + c][c]) : -1.0;
-    int arg = lane + c;
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(best, off, 64);
-      const int oa = __shfl_xor(arg, off, 64);
-      if (ob > best || (ob == best && oa < arg)) {
-        best = ob;
-        arg = oa;
-      }
-    }
-    const int p = arg;
-    singular |= !(best > 0.0);
-    __syncthreads();
-    if (p != c)
-      for (int cc = lane; cc < 2 * NL; cc += 64) {
-        const double t = a[w][c][cc];
-        a[w][c][cc] = a[w][p][cc];
-        a[w][p][cc] = t;
-      }
-    __syncthreads();
-    const double d = singular ? 0.0 : 1.0 / a[w][c][c];
-    double f[NL];  // multipliers of every row, read before the update
+  double a[NL];
+  {
+    const double2* src = reinterpret_cast<const double2*>(L + (le * NL + r) * NL);
 #pragma unroll
-    for (int r = 0; r < NL; ++r) f[r] = a[w][r][c];
-    __syncthreads();
-    for (int cc = lane; cc < 2 * NL; cc += 64) {
-      const double pv = a[w][c][cc] * d;
-#pragma unroll
-      for (int r = 0; r < NL; ++r)
-        if (r != c) a[w][r][cc] -= f[r] * pv;
-      a[w][c][cc] = pv;
+    for (int m = 0; m < NL / 2; ++m) {
+      const double2 t = src[m];
+      a[2 * m] = act ? t.x : 0.0;
+      a[2 * m + 1] = act ? t.y : 0.0;
     }
-    __syncthreads();
   }
-  if (valid) {
-    if (singular && lane == 0) atomicCAS(fail, 0, 1 + e);
-    for (int k = lane; k < NL * NL; k += 64) {
-      const int n = k / NL, m = k % NL;
-      S64[s_index<double, NL>(n, m, e, nele)] = omega * a[w][n][NL + m];
+  if (row && r < NK - NL) ck[le][NL + r] = 0u;  // key padding
+  for (int i = threadIdx.x; i < NS; i += NT) {
+    int ra = 0, rem = i;
+    while (rem >= NL - ra) {
+      rem -= NL - ra;
+      ++ra;
     }
+    ab[i][0] = (uint8_t)ra;
+    ab[i][1] = (uint8_t)(ra + rem);
+  }
+  bool used = !act;
+  bool singular = false;
+#pragma unroll
+  for (int c = 0; c < NL; ++c) {
+    // pivot key: the upper word of |a[r][c]| (exponent and 20 mantissa bits) with its low
+    // 5 bits replaced by 31 - r, so the largest key is the largest magnitude to 15
+    // mantissa bits, the lowest row among those; 0 for used rows and NaN.  The keys and
+    // the pivot row stay within the element's wave.
+    const double v = fabs(a[c]);
+    const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(v) >> 32);
+    if (row) ck[le][r] = (used || !(v >= 0.0)) ? 0u : ((hi & ~31u) | (uint32_t)(31 - r));
+    wave_lds_sync();
+    uint32_t kmax = 0;
+#pragma unroll
+    for (int k = 0; k < NK / 4; ++k) {
+      const uint4 t = reinterpret_cast<const uint4*>(ck[le])[k];
+      kmax = max(max(kmax, max(t.x, t.y)), max(t.z, t.w));
+    }
+    const bool ok = (kmax >> 5) != 0;  // a nonzero candidate exists
+    const int p = ok ? 31 - (int)(kmax & 31u) : 0;
+    singular = singular || !ok;
+    const bool me = act && r == p;
+    // the pivot lane publishes its row scaled by d = 1 / a[p][c], with d in column c
+    if (me) {
+      const double d = 1.0 / a[c];
+      double2* dst = reinterpret_cast<double2*>(pv[le]);
+#pragma unroll
+      for (int j = 0; j < NL / 2; ++j)
+        dst[j] = make_double2(2 * j == c ? d : a[2 * j] * d, 2 * j + 1 == c ? d : a[2 * j + 1] * d);
+    }
+    if (row && r == 0) pr[le][c] = p;
+    wave_lds_sync();
+    const double f = a[c];
+    // pivot row: a[j] = pv[j]; the others: a[j] - f pv[j] (sc = 0, fx = -1 give pv[j])
+    const double sc = me ? 0.0 : 1.0, fx = me ? -1.0 : f;
+    double pr2[NL];
+#pragma unroll
+    for (int j = 0; j < NL / 2; ++j) {
+      const double2 t = reinterpret_cast<const double2*>(pv[le])[j];
+      pr2[2 * j] = t.x;
+      pr2[2 * j + 1] = t.y;
+    }
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      if (j == c) continue;
+      a[j] = fma(-fx, pr2[j], a[j] * sc);
+    }
+    const double d = ok ? pr2[c] : 0.0;
+    a[c] = me ? d : -f * d;
+    used = used || me;
+    wave_lds_sync();  // every lane has read ck and pv before the next column rewrites them
+  }
+  if (row) {
+    double2* dst = reinterpret_cast<double2*>(L + (le * NL + r) * NL);
+#pragma unroll
+    for (int j = 0; j < NL / 2; ++j) dst[j] = make_double2(a[2 * j], a[2 * j + 1]);
+    qi[le][r] = r;  // a singular block may repeat a pivot row: keep indices in range
+  }
+  if (act && r == 0 && singular) atomicCAS(fail, 0, 1 + e0 + le);
+  __syncthreads();
+  if (row) qi[le][pr[le][r]] = r;
+  __syncthreads();
+  // packed entries (i, element j), B^-1[a][b] = stored[p_a][q_b]
+  double pk[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const int idx = threadIdx.x + k * NT;
+    pk[k] = 0.0;
+    if (idx < NS * EB) {
+      const int i = idx / EB, j = idx - (idx / EB) * EB;
+      const int ra = ab[i][0], rb = ab[i][1];
+      const double xab = L[(j * NL + pr[j][ra]) * NL + qi[j][rb]];
+      const double xba = L[(j * NL + pr[j][rb]) * NL + qi[j][ra]];
+      pk[k] = 0.5 * (omega * xab + omega * xba);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const int idx = threadIdx.x + k * NT;
+    if (idx < NS * EB) L[idx] = pk[k];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < NS * EB; idx += NT) {
+    const int i = idx / EB, j = idx - (idx / EB) * EB;
+    if (j >= ne) continue;
+    const double v = L[idx];
+    const size_t o = (size_t)i * nele + e0 + j;
+    P64[o] = v;
+    S32[o] = (float)v;
+    if (S16) S16[o] = f32_to_bf16((float)v);
+  }
+  if (H) {
+    // largest magnitude per element: RP partial maxima each, then one thread per element
+    if ((int)threadIdx.x < RP * EB) {
+      const int j = threadIdx.x % EB, part = threadIdx.x / EB;
+      double m = 0.0;
+      for (int i = part; i < NS; i += RP) m = fmax(m, fabs(L[i * EB + j]));
+      red[part][j] = m;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < EB) {
+      double m = 0.0;
+#pragma unroll
+      for (int q = 0; q < RP; ++q) m = fmax(m, red[q][threadIdx.x]);
+      mx[threadIdx.x] = m;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < NP * EB; idx += NT) {
+      const int q = idx / EB, j = idx - (idx / EB) * EB;
+      if (j >= ne) continue;
+      const float mf = (float)mx[j];
+      const double inv = mx[j] > 0.0 ? 1.0 / (double)mf : 0.0;
+      const int i0 = 2 * q, i1 = 2 * q + 1;
+      const _Float16 lo = (_Float16)(float)(L[i0 * EB + j] * inv);
+      const _Float16 hi = i1 < NS ? (_Float16)(float)(L[i1 * EB + j] * inv) : (_Float16)0.0f;
+      H[(size_t)q * nele + e0 + j] = (uint32_t)__builtin_bit_cast(uint16_t, lo) |
+                                     ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+    }
+    if ((int)threadIdx.x < ne) Hs[e0 + threadIdx.x] = (float)mx[threadIdx.x];
   }
 }
 
@@ -1381,14 +1522,19 @@ __global__ __launch_bounds__(256) void k_pack_S_sym(int nele, const double* __re
     }
 }
 
-int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, double* S, int* fail,
+int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, bool half, int* fail,
                           hipStream_t s) {
   const int nele = (int)c->nele;
-  const dim3 blk(256), grd((nele + 3) / 4);
+  const int eb = c->D == 3 ? SdirDims<20>::EB : SdirDims<10>::EB;
+  const dim3 blk(64 * SDIR_WAVES), grd((nele + eb - 1) / eb);
+  uint32_t* H = half ? c->g32.Sh : nullptr;
+  float* Hs = half ? c->g32.Shs : nullptr;
   if (c->D == 3)
-    hipLaunchKernelGGL(k_build_S_direct<20>, grd, blk, 0, s, nele, bdiag, omega, S, fail);
+    hipLaunchKernelGGL(k_build_S_direct<20>, grd, blk, 0, s, nele, bdiag, omega, c->g64.S,
+                       c->g32.S, c->g32.S16, H, Hs, fail);
   else
-    hipLaunchKernelGGL(k_build_S_direct<10>, grd, blk, 0, s, nele, bdiag, omega, S, fail);
+    hipLaunchKernelGGL(k_build_S_direct<10>, grd, blk, 0, s, nele, bdiag, omega, c->g64.S,
+                       c->g32.S, c->g32.S16, H, Hs, fail);
   DGMI_HIP(hipGetLastError());
   return 0;
 }
@@ -1822,16 +1968,17 @@ template int launch_l2_error<double>(dgmi_ctx*, const double*, const double*, co
 template int launch_l2_error<float>(dgmi_ctx*, const float*, const float*, const double*,
                                     double*, int, hipStream_t);
 
-int launch_bdiag(dgmi_ctx* c, const double* refm, double* bdiag, hipStream_t s) {
+int launch_bdiag(dgmi_ctx* c, const double* tab, double* bdiag, hipStream_t s) {
   const int nele = (int)c->nele;
-  const int64_t nthreads = (int64_t)nele * c->nloc * c->nloc;
-  const dim3 blk(256), grd((unsigned)((nthreads + 255) / 256));
-  if (c->D == 3)
-    hipLaunchKernelGGL(k_bdiag<3>, grd, blk, 0, s, nele, refm, c->g64.G, c->g64.cf, c->g64.wf,
-                       c->g64.pen, c->fcode, bdiag);
-  else
-    hipLaunchKernelGGL(k_bdiag<2>, grd, blk, 0, s, nele, refm, c->g64.G, c->g64.cf, c->g64.wf,
-                       c->g64.pen, c->fcode, bdiag);
+  if (c->D == 3) {
+    using B = BdiagDims<3>;
+    hipLaunchKernelGGL(k_bdiag<3>, dim3((nele + B::EB - 1) / B::EB), dim3(B::THREADS), 0, s,
+                       nele, tab, c->g64.G, c->g64.cf, c->g64.wf, c->g64.pen, c->fcode, bdiag);
+  } else {
+    using B = BdiagDims<2>;
+    hipLaunchKernelGGL(k_bdiag<2>, dim3((nele + B::EB - 1) / B::EB), dim3(B::THREADS), 0, s,
+                       nele, tab, c->g64.G, c->g64.cf, c->g64.wf, c->g64.pen, c->fcode, bdiag);
+  }
   DGMI_HIP(hipGetLastError());
   return 0;
 }

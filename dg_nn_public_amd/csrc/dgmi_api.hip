@@ -408,9 +408,44 @@ static std::vector<double> bdiag_refmats(const ElementTables& el) {
   return r;
 }
 
+// The NT reference matrices of k_bdiag's table form: the symmetrised volume tables
+// (Sref_jj; Sref_jk + Sref_kj, j < k, in the order 00 11 22 01 02 12 / 00 11 01 of G),
+// then per face M_f and C_fd + C_fd^T (d < D), each NL*NL
+static std::vector<double> bdiag_tables(const ElementTables& el) {
+  const int D = el.D, NL = el.nloc, NF = el.nface, NN = NL * NL;
+  const std::vector<double> r = bdiag_refmats(el);
+  const double* Sref = r.data();
+  const double* M = Sref + (size_t)D * D * NN;
+  const double* C = M + (size_t)NF * NN;
+  const int jj3[6] = {0, 1, 2, 0, 0, 1}, kk3[6] = {0, 1, 2, 1, 2, 2};
+  const int jj2[3] = {0, 1, 0}, kk2[3] = {0, 1, 1};
+  const int NQ = D == 3 ? 6 : 3;
+  std::vector<double> t((size_t)(NQ + NF * (1 + D)) * NN, 0.0);
+  for (int q = 0; q < NQ; ++q) {
+    const int j = D == 3 ? jj3[q] : jj2[q], k = D == 3 ? kk3[q] : kk2[q];
+    for (int nm = 0; nm < NN; ++nm) {
+      double v = Sref[(size_t)(j * D + k) * NN + nm];
+      if (j != k) v += Sref[(size_t)(k * D + j) * NN + nm];
+      t[(size_t)q * NN + nm] = v;
+    }
+  }
+  for (int f = 0; f < NF; ++f) {
+    double* o = t.data() + (size_t)(NQ + f * (1 + D)) * NN;
+    std::copy(M + (size_t)f * NN, M + (size_t)(f + 1) * NN, o);
+    for (int d = 0; d < D; ++d) {
+      const double* Cd = C + (size_t)(f * D + d) * NN;
+      for (int n = 0; n < NL; ++n)
+        for (int m = 0; m < NL; ++m)
+          o[(size_t)(1 + d) * NN + n * NL + m] = Cd[n * NL + m] + Cd[m * NL + n];
+    }
+  }
+  return t;
+}
+
 struct Scratch {
   double* bdiag = nullptr;
   double* refm = nullptr;
+  double* btab = nullptr;  // bdiag_tables
   double* vec64[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   float* vec32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double* cg64[2] = {nullptr, nullptr};
@@ -474,15 +509,15 @@ template <> DevGeom<float>& geom<float>(dgmi_ctx* c) { return c->g32; }
 
 static int compute_bdiag(dgmi_ctx* c, hipStream_t s) {
   Scratch& sc = scratch(c);
-  if (!sc.refm) {
-    sc.refm = upload<double>(c, bdiag_refmats(c->el), s);
-    DGMI_CHECK(sc.refm, alloc_failure());
+  if (!sc.btab) {
+    sc.btab = upload<double>(c, bdiag_tables(c->el), s);
+    DGMI_CHECK(sc.btab, alloc_failure());
   }
   if (!sc.bdiag) {
     sc.bdiag = static_cast<double*>(c->alloc(sizeof(double) * c->nele * c->nloc * c->nloc));
     DGMI_CHECK(sc.bdiag, alloc_failure());
   }
-  return launch_bdiag(c, sc.refm, sc.bdiag, s);
+  return launch_bdiag(c, sc.btab, sc.bdiag, s);
 }
 
 // _um_left_precond: x = M^-1 b, one V-cycle from x = 0 (z02/volume_mf_diff.py:1934-2029)
@@ -1533,8 +1568,7 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
              "unknown smoother variant");
   hipStream_t s = (hipStream_t)stream;
   DGMI_TRY(compute_bdiag(c, s));
-  // built as full FP64 blocks in a work buffer, kept as the packed symmetric form
-  // (NL(NL+1)/2 entries per element) in FP64 and FP32
+  // kept as the packed symmetric form (NL(NL+1)/2 entries per element) in FP64 and FP32
   const int64_t nS = c->nele * c->nloc * c->nloc;
   const int64_t nSp = c->nele * (c->nloc * (c->nloc + 1) / 2);
   if (!c->g64.S) c->g64.S = static_cast<double*>(c->alloc(sizeof(double) * nSp));
@@ -1546,40 +1580,41 @@ int dgmi_smoother_setup(dgmi_ctx* c, int variant, double omega, void* stream) {
     DGMI_CHECK(c->g32.S16, alloc_failure());
   }
   Scratch& sc = scratch(c);
-  double* full = nullptr;
-  if (c->alloc_limit > 0 && c->dev_bytes + sizeof(double) * nS > c->alloc_limit) {
-    set_error(alloc_failure("smoother work buffer: the context's device-byte limit"));
-    return -2;
-  }
-  full = static_cast<double*>(dev_alloc(sizeof(double) * nS));
-  DGMI_CHECK(full, alloc_failure("smoother work buffer"));
-  struct FreeFull {
-    double* p;
-    hipStream_t s;
-    ~FreeFull() {
-      (void)hipStreamSynchronize(s);
-      dev_free(p);
-    }
-  } free_full{full, s};
-  if (variant == DGMI_SMOOTH_JACOBI3) {
-    DGMI_TRY(launch_build_S_jacobi<double>(c, sc.bdiag, omega, full, s));
-  } else {
-    int* fail = static_cast<int*>(c->alloc(sizeof(int)));
-    DGMI_CHECK(fail, alloc_failure());
-    DGMI_HIP(hipMemsetAsync(fail, 0, sizeof(int), s));
-    DGMI_TRY(launch_build_S_direct(c, sc.bdiag, omega, full, fail, s));
-    int hf = 0;
-    DGMI_TRY(d2h(&hf, fail, sizeof(int), s));
-    DGMI_CHECK(hf == 0, "singular diagonal block " + std::to_string(hf - 1));
-  }
-  DGMI_TRY(launch_pack_S_sym(c, full, s));
   if (c->smoother_half < 0) c->smoother_half = c->D == 3 ? 1 : 0;
   if (c->smoother_half) {
     const int64_t npair = c->nele * ((c->nloc * (c->nloc + 1) / 2 + 1) / 2);
     if (!c->g32.Sh) c->g32.Sh = static_cast<uint32_t*>(c->alloc(sizeof(uint32_t) * npair));
     if (!c->g32.Shs) c->g32.Shs = static_cast<float*>(c->alloc(sizeof(float) * c->nele));
     DGMI_CHECK(c->g32.Sh && c->g32.Shs, alloc_failure());
-    DGMI_TRY(launch_pack_S_half(c, s));
+  }
+  if (variant == DGMI_SMOOTH_DIRECT) {
+    // inverted and packed in one kernel
+    int* fail = static_cast<int*>(c->alloc(sizeof(int)));
+    DGMI_CHECK(fail, alloc_failure());
+    DGMI_HIP(hipMemsetAsync(fail, 0, sizeof(int), s));
+    DGMI_TRY(launch_build_S_direct(c, sc.bdiag, omega, c->smoother_half != 0, fail, s));
+    int hf = 0;
+    DGMI_TRY(d2h(&hf, fail, sizeof(int), s));
+    DGMI_CHECK(hf == 0, "singular diagonal block " + std::to_string(hf - 1));
+  } else {
+    // built as full FP64 blocks in a work buffer, then packed
+    if (c->alloc_limit > 0 && c->dev_bytes + sizeof(double) * nS > c->alloc_limit) {
+      set_error(alloc_failure("smoother work buffer: the context's device-byte limit"));
+      return -2;
+    }
+    double* full = static_cast<double*>(dev_alloc(sizeof(double) * nS));
+    DGMI_CHECK(full, alloc_failure("smoother work buffer"));
+    struct FreeFull {
+      double* p;
+      hipStream_t s;
+      ~FreeFull() {
+        (void)hipStreamSynchronize(s);
+        dev_free(p);
+      }
+    } free_full{full, s};
+    DGMI_TRY(launch_build_S_jacobi<double>(c, sc.bdiag, omega, full, s));
+    DGMI_TRY(launch_pack_S_sym(c, full, s));
+    if (c->smoother_half) DGMI_TRY(launch_pack_S_half(c, s));
   }
   c->smoother_variant = variant;
   c->omega = omega;

@@ -395,16 +395,18 @@ int launch_rhs(dgmi_ctx* c, const double* refm, const double* detJ, const double
                double* b, hipStream_t s);
 template <typename T>
 int launch_smooth_zero(dgmi_ctx* c, DevGeom<T>& g, T* x, const T* b, hipStream_t s);
-int launch_bdiag(dgmi_ctx* c, const double* refmats, double* bdiag, hipStream_t s);
+// bdiag from the reference tables in the form of k_bdiag (bdiag_tables on the host)
+int launch_bdiag(dgmi_ctx* c, const double* tab, double* bdiag, hipStream_t s);
 // partial (2 nb doubles): per block of 256 elements the l2^2 part and the max |x_num - x_ana|
 template <typename T>
 int launch_l2_error(dgmi_ctx* c, const T* xn, const T* xa, const double* cn, double* partial,
                     int nb, hipStream_t s);
 template <typename T>
 int launch_build_S_jacobi(dgmi_ctx* c, const double* bdiag, double omega, T* S, hipStream_t s);
-// omega * inverse of every diagonal block into both smoother copies; *fail = 1 + element
-// index of a singular block
-int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, double* S, int* fail,
+// omega * inverse of every diagonal block, written as the packed smoother copies (g64.S,
+// g32.S, g32.S16 when allocated, and with `half` the FP16 copy g32.Sh/Shs); *fail = 1 +
+// element index of a singular block
+int launch_build_S_direct(dgmi_ctx* c, const double* bdiag, double omega, bool half, int* fail,
                           hipStream_t s);
 // FP32 vector-family sweeps carrying traces between them (see apply.hip)
 bool trace_io(const dgmi_ctx* c);

@@ -1044,39 +1044,47 @@ __global__ __launch_bounds__(256) void k_csr_to_dense(const GjMat* __restrict__ 
   for (int32_t q = m.rp[i]; q < m.rp[i + 1]; ++q) row[m.col[q]] += m.val[q];
 }
 
-// P = A_KK^-1 by unblocked Gauss-Jordan in LDS; written to A_KK and to p (GJB x GJB)
-__global__ __launch_bounds__(256) void k_bgj_pivot(const GjMat* __restrict__ mats, int k0,
-                                                   int* __restrict__ fail) {
+// P = A_KK^-1 by unblocked Gauss-Jordan in LDS; written to A_KK and to p (GJB x GJB).
+// Wave q updates columns [q GJB/4, (q+1) GJB/4) of every row, lane r row r, from the
+// scaled pivot row that wave 0 publishes: two barriers and GJB/4 FMAs per lane a step.
+constexpr int GJP_WAVES = 4, GJP_COLS = GJB / GJP_WAVES;
+__global__ __launch_bounds__(64 * GJP_WAVES) void k_bgj_pivot(const GjMat* __restrict__ mats,
+                                                              int k0, int* __restrict__ fail) {
+  static_assert(GJB == 64, "one lane per row");
   const GjMat m = mats[blockIdx.z];
   if (k0 >= m.N) return;
   const int N = m.N;
   double* A = m.a;
   __shared__ double a[GJB][GJB + 1];
-  __shared__ double fcol[GJB], frow[GJB];
-  const int t = threadIdx.x;
-  for (int q = t; q < GJB * GJB; q += 256) a[q / GJB][q % GJB] = A[(size_t)(k0 + q / GJB) * N + k0 + q % GJB];
+  __shared__ double prow[GJB];
+  const int t = threadIdx.x & 63, q = threadIdx.x >> 6, j0 = q * GJP_COLS;
+  for (int x = threadIdx.x; x < GJB * GJB; x += 64 * GJP_WAVES)
+    a[x / GJB][x % GJB] = A[(size_t)(k0 + x / GJB) * N + k0 + x % GJB];
   __syncthreads();
+  bool bad = false;
   for (int k = 0; k < GJB; ++k) {
-    // column and row k before the step, so the step reads nothing it writes
-    if (t < GJB) fcol[t] = a[t][k];
-    else if (t < 2 * GJB) frow[t - GJB] = a[k][t - GJB];
-    __syncthreads();
-    const double piv = fcol[k];
-    if (t == 0 && (!(piv > 0.0) || !isfinite(piv))) fail[m.fail] = 1;
+    const double piv = a[k][k];
+    bad = bad || !(piv > 0.0) || !isfinite(piv);
     const double inv = 1.0 / piv;
-    for (int q = t; q < GJB * GJB; q += 256) {
-      const int i = q / GJB, j = q % GJB;
-      if (i == k && j == k) a[i][j] = inv;
-      else if (i == k) a[i][j] = frow[j] * inv;
-      else if (j == k) a[i][j] = -fcol[i] * inv;
-      else a[i][j] -= fcol[i] * (frow[j] * inv);
-    }
+    const double f = a[t][k];
+    // row k scaled (its column k becomes 1/piv), read by every lane after the barrier
+    if (q == 0) prow[t] = t == k ? inv : a[k][t] * inv;
+    __syncthreads();
+    // row k: prow (sc = 0, fx = -1 give it exactly); the others: a - f prow
+    const double sc = t == k ? 0.0 : 1.0, fx = t == k ? -1.0 : f;
+    double rw[GJP_COLS];
+#pragma unroll
+    for (int j = 0; j < GJP_COLS; ++j) rw[j] = fma(-fx, prow[j0 + j], a[t][j0 + j] * sc);
+#pragma unroll
+    for (int j = 0; j < GJP_COLS; ++j) a[t][j0 + j] = rw[j];
+    if (t != k && k >= j0 && k < j0 + GJP_COLS) a[t][k] = -f * inv;
     __syncthreads();
   }
-  for (int q = t; q < GJB * GJB; q += 256) {
-    const double v = a[q / GJB][q % GJB];
-    A[(size_t)(k0 + q / GJB) * N + k0 + q % GJB] = v;
-    m.p[q] = v;
+  if (threadIdx.x == 0 && bad) fail[m.fail] = 1;
+  for (int x = threadIdx.x; x < GJB * GJB; x += 64 * GJP_WAVES) {
+    const double v = a[x / GJB][x % GJB];
+    A[(size_t)(k0 + x / GJB) * N + k0 + x % GJB] = v;
+    m.p[x] = v;
   }
 }
 
@@ -1267,7 +1275,8 @@ static int invert_levels(dgmi_ctx* c, const std::vector<LevelInverse>& jobs,
                          (const GjMat*)dm);
       for (int kb = 0; kb < nbmax; ++kb) {
         const int k0 = kb * GJB;
-        hipLaunchKernelGGL(k_bgj_pivot, dim3(1, 1, nm), blk, 0, s, (const GjMat*)dm, k0, dfail);
+        hipLaunchKernelGGL(k_bgj_pivot, dim3(1, 1, nm), dim3(64 * GJP_WAVES), 0, s,
+                           (const GjMat*)dm, k0, dfail);
         hipLaunchKernelGGL(k_bgj_row, dim3(nbmax, 1, nm), blk, 0, s, (const GjMat*)dm, k0);
         hipLaunchKernelGGL(k_bgj_rest, dim3(nbmax, nbmax, nm), blk, 0, s, (const GjMat*)dm, k0);
         hipLaunchKernelGGL(k_bgj_col, dim3(nbmax, 1, nm), blk, 0, s, (const GjMat*)dm, k0);
