@@ -74,6 +74,7 @@ SIGNATURES = {
     "dgmi_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32), _DP, _P]),
     "dgmi_dot": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, _P]),
     "dgmi_axpy": (C.c_int, [C.c_int, C.c_int64, C.c_double, _P, _P, _P]),
+    "dgmi_upload": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_void_p), _I64P, _P]),
     "dgmi_dot_noalloc": (C.c_int, [C.c_int, C.c_int64, _P, _P, _P, _P]),
     "dgmi_dot_noalloc_max_n": (C.c_int64, []),
     "dgmi_last_error": (C.c_char_p, []),

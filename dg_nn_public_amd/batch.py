@@ -86,8 +86,9 @@ class DGBatch:
 
     def rhs(self, fs, u_bcs):
         """b = M f + Dirichlet terms of every component (nodal f and u_bc per mesh)."""
-        f = np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1) for a in fs])
-        u = np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1) for a in u_bcs])
+        # each mesh's data goes straight to its offset in the packed device vector
+        f = self.space._upload(fs, "f")
+        u = self.space._upload(u_bcs, "u_bc")
         return self.space._get_rhs(self.space.new_vector(), f, u)
 
     def solve(self, b, rtol=1e-10, inner=torch.float32, m=GMRES_M, max_its=GMRES_ITS, x=None):

@@ -2163,6 +2163,25 @@ int dgmi_autotune(dgmi_ctx* c, int reps, int32_t* choice, double* times_us, void
   return 0;
 }
 
+int dgmi_upload(dgmi_ctx* c, void* dst, int npieces, const void* const* src,
+                const int64_t* bytes, void* stream) {
+  DGMI_CHECK(c && dst && npieces >= 0 && (npieces == 0 || (src && bytes)), "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  char* d = static_cast<char*>(dst);
+  int rc = 0;
+  for (int i = 0; i < npieces && rc == 0; ++i) {
+    if (bytes[i] < 0 || (bytes[i] > 0 && !src[i])) {
+      set_error("upload piece " + std::to_string(i) + ": null pointer or negative size");
+      rc = -2;
+      break;
+    }
+    rc = h2d(c, d, src[i], (size_t)bytes[i], s);
+    d += bytes[i];
+  }
+  const int rr = h2d_release(c);  // waits for the copies, returns the staging buffers
+  return rc ? rc : rr;
+}
+
 int dgmi_set_param(dgmi_ctx* c, const char* name, double value) {
   DGMI_CHECK(c && name, "null argument");
   const std::string k(name);

@@ -718,8 +718,24 @@ class DGDiffusion:
         return x_i, max(p["its"] for p in per)
 
     def _dof_data(self, a, name):
-        return self._vec(torch.as_tensor(a, dtype=torch.float64, device=self.device)
-                         .reshape(-1).contiguous(), name, (torch.float64,))
+        if isinstance(a, torch.Tensor) and a.device.type == "cuda":
+            return self._vec(a.to(torch.float64).reshape(-1).contiguous(), name,
+                             (torch.float64,))
+        return self._upload([a], name)
+
+    def _upload(self, pieces, name):
+        """A new device vector of host pieces (arrays or CPU tensors, taken as float64)
+        laid end to end, copied through the library's pinned staging buffers."""
+        arrs = [np.ascontiguousarray(np.asarray(p, dtype=np.float64)).reshape(-1)
+                for p in pieces]
+        if sum(a.size for a in arrs) != self.ndof:
+            raise ValueError(f"{name} must hold {self.ndof} values")
+        out = torch.empty(self.ndof, dtype=torch.float64, device=self.device)
+        src = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        nb = (C.c_int64 * len(arrs))(*[a.nbytes for a in arrs])
+        check(self.lib.dgmi_upload(self.ctx, _ptr(out), len(arrs), src, nb, _stream()),
+              "dgmi_upload")
+        return out
 
     def _get_rhs(self, x_rhs, f, u_bc, alpha_x_i=None):
         """x_rhs += M f + Dirichlet terms + Neumann terms (+ M alpha_x_i / dt when the

@@ -309,6 +309,15 @@ int dgmi_set_param(dgmi_ctx* ctx, const char* name, double value);
 /* Krylov vector primitives on device vectors of n FP64 or FP32 values. */
 int dgmi_dot(int dtype, int64_t n, const void* x, const void* y, double* out_dev, void* stream);
 int dgmi_axpy(int dtype, int64_t n, double a, const void* x, void* y, void* stream);
+/* Host-to-device copy through the context's pinned staging buffers: the npieces host
+ * pieces src[i] (bytes[i] each) land one after the other from dst, queued on `stream`
+ * (copy-engine DMA from pinned memory, which no kernel queue holds up).  Returns once
+ * every byte has reached the device, so the host pieces may be reused at once.  No
+ * counterpart in the reference, whose tensors torch creates on the device; the host
+ * wrapper uploads right-hand-side data with it (per-request pieces of a packed job go
+ * straight to their offsets, without a host concatenation). */
+int dgmi_upload(dgmi_ctx* ctx, void* dst, int npieces, const void* const* src,
+                const int64_t* bytes, void* stream);
 /* Allocation-free dgmi_dot: out_dev[0] = x . y (FP64 accumulation) with the partial sums
  * in a fixed array of this library in device memory (256 KiB per GPU, held until the
  * process ends) instead of a buffer from hipMalloc.  No counterpart in the reference; it is

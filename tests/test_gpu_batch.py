@@ -97,3 +97,34 @@ def test_batch_solve_equals_alone_and_converges(batch, inner):
             # that ends near the threshold may take one more or one fewer cycle
             assert abs(st[c]["its"] - sa["its"]) <= 3
         assert rel(xs, to_np(xa)) < 1e-9
+
+
+def test_staged_upload_of_pieces(batch):
+    """dgmi_upload puts host pieces (of uneven sizes, an empty one among them, larger
+    than one 16 MB staging buffer in total) end to end on the device, bit for bit, and
+    the packed right-hand side built from per-mesh pieces equals the one from their
+    concatenation."""
+    import ctypes as C
+    from dg_nn_public_amd import _lib
+    pbs, bt, alone = batch
+    g = bt.space
+    rng = np.random.default_rng(31)
+    sizes = [3, 0, 2_500_001, 1_000_000, 7]
+    pieces = [rng.standard_normal(k) for k in sizes]
+    ref = np.concatenate(pieces)
+    out = torch.empty(ref.size, dtype=torch.float64, device=g.device)
+    src = (C.c_void_p * len(pieces))(*[p.ctypes.data for p in pieces])
+    nb = (C.c_int64 * len(pieces))(*[p.nbytes for p in pieces])
+    assert g.lib.dgmi_upload(g.ctx, C.c_void_p(out.data_ptr()), len(pieces), src, nb,
+                             None) == 0
+    assert np.array_equal(to_np(out), ref)
+    nb[2] = -8
+    assert g.lib.dgmi_upload(g.ctx, C.c_void_p(out.data_ptr()), len(pieces), src, nb,
+                             None) != 0
+    assert b"negative" in _lib.load().dgmi_last_error()
+    f = [p.f for p in pbs]
+    u = [p.u_bc for p in pbs]
+    b1 = bt.rhs(f, u)
+    b2 = g._get_rhs(g.new_vector(), np.concatenate([np.ravel(a) for a in f]),
+                    np.concatenate([np.ravel(a) for a in u]))
+    assert torch.equal(b1, b2)
