@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="serve",
-                    choices=["serve", "solve", "precision", "batch", "sweep"])
+                    choices=["serve", "solve", "precision", "batch", "sweep", "setup"])
     # serving
     ap.add_argument("--nreq", type=int, default=48, help="requests per stream")
     ap.add_argument("--rate", type=float, default=500.0, help="arrival rate of a stream, req/s")
@@ -139,7 +139,7 @@ def main():
                                    "supported (multi-GPU sharding is design-only)"}))
         sys.exit(2)
     {"serve": run_headline, "solve": run_solve, "precision": run_precision,
-     "batch": run_batch, "sweep": run_serve}[args.workload](args)
+     "batch": run_batch, "sweep": run_serve, "setup": run_setup}[args.workload](args)
 
 
 # ----------------------------------------------------------------------------------------
@@ -821,6 +821,43 @@ def _batch_requests(nsolves, emax, seed=0):
         reqs.append(dict(D=D, n=n, mesh_seed=i, kappa_seed=1000 + i,
                          sigma=float(rng.uniform(0.5, 2.0))))
     return reqs
+
+
+def run_setup(args):
+    """Set-up of one large serving job (--large-n^3 x 6 tets, as the serving mix's large
+    jobs): DGBatch creation (context, smoother, multigrid) and the right-hand side,
+    --steps times after --warmup untimed ones; per-phase host times from the library."""
+    import numpy as np
+    import torch
+
+    from dg_nn_public_amd import meshgen
+    from dg_nn_public_amd.batch import DGBatch
+    m = meshgen.cube_mesh(args.large_n, seed=5000)
+    pb = meshgen.sine_problem(m, sigma=1.0, kappa_seed=6000)
+    torch.zeros(1, device="cuda:0")
+    walls, rhs, phases = [], [], []
+    for k in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bt = DGBatch([m], [pb.kappa], graph_vcycle=False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        bt.rhs([pb.f], [pb.u_bc])
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        ph = bt.space.setup_timing()
+        bt.close()
+        log(f"set-up {k}: DGBatch {t1 - t0:.3f}s rhs {t2 - t1:.3f}s "
+            f"{ {q: round(v, 4) for q, v in ph.items()} }")
+        if k >= args.warmup:
+            walls.append(t1 - t0)
+            rhs.append(t2 - t1)
+            phases.append(ph)
+    print(json.dumps({"workload": f"set-up of one {args.large_n}^3x6 tet job ({m.nele} "
+                                  "elements)",
+                      "dgbatch_s": float(np.median(walls)), "rhs_s": float(np.median(rhs)),
+                      "phases_s": {q: float(np.median([p[q] for p in phases]))
+                                   for q in phases[0]}}))
 
 
 def run_batch(args):
