@@ -1207,8 +1207,38 @@ const char* dgmi_last_error(void) { return g_err.c_str(); }
 
 // context creation; the P3 node coordinates come in nparts pieces (piece q holds elements
 // [part_off[q], part_off[q+1]))
+// the P1CG pattern and Hilbert orders of the component vertex ranges voff, on a host thread
+static void start_graph_job(dgmi_ctx* c, std::vector<int64_t> voff) {
+  auto* job = new GraphJob{};
+  job->voff = std::move(voff);
+  job->th = std::thread([c, job] {
+    try {
+      p1cg_sparsity(c->cells.data(), c->nele, c->D, c->nverts, job->fina, job->cola);
+      const int nc = (int)job->voff.size() - 1;
+      job->lsfc.assign(nc, {});
+      parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
+        for (int64_t q = q0; q < q1; ++q) {
+          const int64_t v0 = job->voff[q], nv = job->voff[q + 1] - v0;
+          hilbert_order(c->verts.data() + v0 * c->D, nv, c->D, job->lsfc[q]);
+        }
+      }, 1);
+      job->ok = true;
+    } catch (...) {
+      job->ok = false;  // dgmi_mg_setup forms them itself
+    }
+  });
+  c->graph = job;
+}
+static GraphJob* join_graph_job(dgmi_ctx* c) {
+  GraphJob* job = c->graph;
+  c->graph = nullptr;
+  if (job && job->th.joinable()) job->th.join();
+  return job;
+}
+
 static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts,
-                           const int64_t* part_off, int nparts, dgmi_ctx** out) {
+                           const int64_t* part_off, int nparts, dgmi_ctx** out,
+                           const int64_t* voff = nullptr, int nvoff = 0) {
   DGMI_CHECK(d && out, "null argument");
   DGMI_CHECK(d->D == 2 || d->D == 3, "D must be 2 or 3");
   DGMI_CHECK(d->nele > 0 && d->nverts > 0, "empty mesh");
@@ -1238,6 +1268,11 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
       set_error("cell vertex id out of range");
       return -2;
     }
+  {
+    std::vector<int64_t> vo = voff ? std::vector<int64_t>(voff, voff + nvoff + 1)
+                                   : std::vector<int64_t>{0, c->nverts};
+    start_graph_job(c, std::move(vo));
+  }
   double tp = wall_seconds();
   c->topo = build_topology(c->cells.data(), c->nele, c->D);
   c->t_setup[PH_TOPOLOGY] += wall_seconds() - tp;
@@ -1246,6 +1281,7 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
     for (int64_t z = 0; z < c->nele * c->nface; ++z) {
       const int64_t t = d->bctype[z];
       if (t < 0 && c->topo.nbf[z] < 0) {
+        delete join_graph_job(c);
         delete c;
         set_error("face marked interior has no neighbour");
         return -2;
@@ -1254,6 +1290,7 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
     }
   }
   if (d->face_kappa != 0 && d->face_kappa != 1) {
+    delete join_graph_job(c);
     delete c;
     set_error("face_kappa must be 0 or 1");
     return -2;
@@ -1346,7 +1383,7 @@ static int ctx_create_batch_impl(const dgmi_mesh_desc* d, int32_t ncomp, const i
       }
   dgmi_ctx* c = nullptr;
   if (x_parts) {
-    DGMI_TRY(ctx_create_impl(d, x_parts, eoff, ncomp, &c));
+    DGMI_TRY(ctx_create_impl(d, x_parts, eoff, ncomp, &c, voff, ncomp));
   } else {
     DGMI_TRY(dgmi_ctx_create(d, &c));
   }
@@ -1376,6 +1413,7 @@ int dgmi_ctx_create_batch_parts(const dgmi_mesh_desc* d, int32_t ncomp, const in
 
 int dgmi_ctx_destroy(dgmi_ctx* c) {
   if (!c) return 0;
+  delete join_graph_job(c);
   if (c->destroy_sync) (void)hipDeviceSynchronize();
   (void)h2d_release(c);
   if (c->mg.graph64.exec) (void)hipGraphExecDestroy(c->mg.graph64.exec);
@@ -1789,7 +1827,20 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     double* tp;
     ~LayoutMark() { c->t_setup[PH_LAYOUT] += wall_seconds() - *tp; }
   };
-  p1cg_sparsity(c->cells.data(), c->nele, D, c->nverts, mg.fina, mg.cola);
+  // the pattern and the Hilbert orders from the thread started at creation, when it
+  // formed them for these component ranges
+  std::vector<std::vector<int64_t>> lsfc;
+  {
+    GraphJob* job = join_graph_job(c);
+    if (job && job->ok && job->voff == c->comp_voff) {
+      mg.fina = std::move(job->fina);
+      mg.cola = std::move(job->cola);
+      lsfc = std::move(job->lsfc);
+    } else {
+      p1cg_sparsity(c->cells.data(), c->nele, D, c->nverts, mg.fina, mg.cola);
+    }
+    delete job;
+  }
   mg.ncola = (int64_t)mg.cola.size();
   // components own contiguous vertex ranges and share no P1CG coupling
   for (int q = 0; q < nc; ++q)
@@ -1812,13 +1863,15 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
     mg.ncolor = *std::max_element(ncq.begin(), ncq.end());
   }
   // Hilbert order of each component's own vertices (local 1-based positions)
-  std::vector<std::vector<int64_t>> lsfc(nc);
-  parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
-    for (int64_t q = q0; q < q1; ++q) {
-      const int64_t v0 = c->comp_voff[q], nv = c->comp_voff[q + 1] - v0;
-      hilbert_order(c->verts.data() + v0 * D, nv, D, lsfc[q]);
-    }
-  }, 1);
+  if ((int)lsfc.size() != nc) {
+    lsfc.assign(nc, {});
+    parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
+      for (int64_t q = q0; q < q1; ++q) {
+        const int64_t v0 = c->comp_voff[q], nv = c->comp_voff[q + 1] - v0;
+        hilbert_order(c->verts.data() + v0 * D, nv, D, lsfc[q]);
+      }
+    }, 1);
+  }
   mark(PH_GRAPH);
   // restriction lists for a layout: SFC position of every vertex (0-based)
   auto build_transfers = [&](const std::vector<int64_t>& pos, int64_t ncg) -> int {

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "setup_host.h"
@@ -240,12 +241,26 @@ int expand_active(const SegDev& sg, int64_t nele, const int32_t* flag, int32_t* 
 
 }  // namespace dgmi
 
+namespace dgmi {
+// The P1CG pattern and every component's Hilbert order need only the cells and the
+// vertices: a host thread started at context creation forms them while the rest of the
+// set-up (topology, geometry, smoother) runs, and dgmi_mg_setup takes them.
+struct GraphJob {
+  std::thread th;
+  std::vector<int64_t> voff;  // the component vertex ranges the orders are for
+  std::vector<int64_t> fina, cola;
+  std::vector<std::vector<int64_t>> lsfc;  // local 1-based SFC positions per component
+  bool ok = false;
+};
+}  // namespace dgmi
+
 struct dgmi_ctx {
   int D = 0, nloc = 0, ngi = 0, sngi = 0, nface = 0, ng = 0;
   int64_t nele = 0, nverts = 0, ndof = 0;
   double eta_e = 36.0;
   dgmi::ElementTables el;
   dgmi::Topology topo;
+  dgmi::GraphJob* graph = nullptr;  // joined by dgmi_mg_setup or dgmi_ctx_destroy
   std::vector<int64_t> cells;
   std::vector<double> verts, kappa;
   int32_t* nbface = nullptr;
