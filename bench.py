@@ -82,14 +82,14 @@ def sweep_bytes_per_elem(D, nloc, nface, sngi, single_pass=True, half=False):
 
 def sweep_flops_per_elem(D, nloc, nface, sngi):
     """Flops of one such sweep per element as the kernel performs them: per face the
-    neighbour's normal derivative at the face's nk quadratic nodes, the flux in node form
-    (face mass and mixed face matrices over the nfn face nodes) and its lift, then the
-    packed smoother product."""
-    nfn, nk = (10, 6) if D == 3 else (4, 3)
+    neighbour's normal derivative at the face's nk quadratic nodes and the flux in node
+    form (face mass and mixed face matrices over the nfn face nodes); the faces' lifts
+    summed at the np2 P2 nodes and applied once; then the packed smoother product."""
+    nfn, nk, np2 = (10, 6, 10) if D == 3 else (4, 3, 6)
     dk = nk * (D * nloc + D)
     nodes = nfn * nfn + 2 * nfn * nk
-    lift = nk * D + nloc * nk * D
-    return 2 * nface * (dk + nodes + lift) + 2 * nloc * nloc
+    lift = nface * nk * D + nloc * np2 * D
+    return 2 * (nface * (dk + nodes) + lift) + 2 * nloc * nloc
 
 
 def log(*a):

@@ -682,7 +682,18 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
       } else {
         fl.set_nodes(f, fc, xrf, dkn, -pe * wfe, 0.5f * wfe, 0.5f * wfe);
       }
-      fl.finish(f, fc, c, acc);
+      if constexpr (APPLY) {
+        fl.finish(f, fc, c, acc);
+      } else {
+        // the sweep lifts all faces at once (step 2); this face's Q and c go to the
+        // slots of red[w] at the nodes off face w, which the residual never reads
+        using O = FaceOff<D>;
+        static_assert(O::NO >= F::NK + D, "room for Q and c off the face");
+#pragma unroll
+        for (int k2 = 0; k2 < F::NK; ++k2) red[w][O::tab.off[w][k2]][lane] = fl.Q[k2];
+#pragma unroll
+        for (int d = 0; d < D; ++d) red[w][O::tab.off[w][F::NK + d]][lane] = c[d];
+      }
     }
     if constexpr (APPLY) {
       // ---- 1c. volume term, second half: h_w = sum_e G_we (VM g_e), r += VD_w^T h_w
@@ -703,11 +714,58 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
       }
     }
     // (red[w] is next written by this lane only, after its reads above: program order)
+    if constexpr (APPLY) {
 #pragma unroll
-    for (int n = 0; n < NL; ++n) red[w][n][lane] = acc[n];
+      for (int n = 0; n < NL; ++n) red[w][n][lane] = acc[n];
 #pragma unroll
-    for (int i = 0; i < F::NFN; ++i) red[w][fnodes[w * F::NFN + i]][lane] += fl.accf[i];
+      for (int i = 0; i < F::NFN; ++i) red[w][fnodes[w * F::NFN + i]][lane] += fl.accf[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < F::NFN; ++i) red[w][fnodes[w * F::NFN + i]][lane] = fl.accf[i];
+    }
     __syncthreads();
+    if constexpr (SWEEP) {
+      // ---- 1c. (sweeps) the faces' lifts summed at the P2 nodes,
+      //   W[p][d] = sum_{f, k: node k of face f is p} c_fd Q_fk,
+      // then for this wave's rows r_n = b_n - sum_{faces f on n} accf_f(n)
+      //   - sum_{p,d} W[p][d] dphi_n/dxi_d(p)  (one lift instead of one per face)
+      using O = FaceOff<D>;
+      float Wp[F::NP2][D];
+#pragma unroll
+      for (int i = 0; i < F::NP2; ++i)
+#pragma unroll
+        for (int d = 0; d < D; ++d) Wp[i][d] = 0.0f;
+#pragma unroll
+      for (int ff = 0; ff < NF; ++ff) {
+        float cc[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) cc[d] = red[ff][O::tab.off[ff][F::NK + d]][lane];
+#pragma unroll
+        for (int k2 = 0; k2 < F::NK; ++k2) {
+          const float q = red[ff][O::tab.off[ff][k2]][lane];
+#pragma unroll
+          for (int d = 0; d < D; ++d) Wp[FaceP2<D>::p[ff][k2]][d] += cc[d] * q;
+        }
+      }
+      const float* vd = fc + F::VD;
+#pragma unroll
+      for (int k = 0; k < NPW; ++k) {
+        const int n = w * NPW + k;
+        if (NL % NF == 0 || n < NL) {
+          float s = bs[n][lane];
+          const int mk = O::tab.mask[n];
+#pragma unroll
+          for (int ff = 0; ff < NF; ++ff)
+            if (mk & (1 << ff)) s -= red[ff][n][lane];
+#pragma unroll
+          for (int i = 0; i < F::NP2; ++i)
+#pragma unroll
+            for (int d = 0; d < D; ++d) s -= Wp[i][d] * vd[(d * F::NP2 + i) * NL + n];
+          bs[n][lane] = s;  // this wave's rows only
+        }
+      }
+      __syncthreads();
+    }
   }
   if constexpr (APPLY) {
     // ---- 2. rows of this wave: r = -A x, b - A x or A x
@@ -728,11 +786,7 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
     float r[NL], y[NL];
 #pragma unroll
     for (int n = 0; n < NL; ++n) {
-      float s = bs[n][lane];
-      if constexpr (!ZERO) {
-#pragma unroll
-        for (int ww = 0; ww < NF; ++ww) s -= red[ww][n][lane];
-      }
+      const float s = bs[n][lane];  // b, or b - A_off x from step 1c
       r[n] = s;
       y[n] = 0.0f;
     }

@@ -1318,6 +1318,21 @@ static int ctx_create_impl(const dgmi_mesh_desc* d, const double* const* x_parts
     for (int i = 0; i < c->el.snloc; ++i)
       fn_ok = fn_ok && c->el.fnodes[f * c->el.snloc + i] ==
                            (c->D == 3 ? FaceNodes<3>::v[f][i] : FaceNodes<2>::v[f][i]);
+  // the sweeps lift every face at the P2 nodes: each face's gradient table at its
+  // quadratic node k must be the volume table at P2 node FaceP2<D>::p[f][k]
+  if (fn_ok && (int)c->el.fgk.size() == c->nface * c->el.nfk * c->D * c->nloc &&
+      (int)c->el.vd.size() == c->D * c->el.np2 * c->nloc) {
+    for (int f = 0; f < c->nface; ++f)
+      for (int k = 0; k < c->el.nfk; ++k) {
+        const int p = c->D == 3 ? FaceP2<3>::p[f][k] : FaceP2<2>::p[f][k];
+        for (int d = 0; d < c->D; ++d)
+          for (int n = 0; n < c->nloc; ++n)
+            fn_ok = fn_ok && c->el.fgk[((f * c->el.nfk + k) * c->D + d) * c->nloc + n] ==
+                                 c->el.vd[(d * c->el.np2 + p) * c->nloc + n];
+      }
+  } else {
+    fn_ok = false;
+  }
   if (!fn_ok || !(c->el.face_interp_err < 1e-12) || !(c->el.vol_factor_err < 1e-12) ||
       c->el.snloc != (c->D == 3 ? FaceDims<3>::NFN : FaceDims<2>::NFN) ||
       c->el.nfk != (c->D == 3 ? FaceDims<3>::NK : FaceDims<2>::NK) ||

@@ -48,6 +48,45 @@ template <> struct FaceNodes<2> {
   static constexpr int v[3][4] = {{0, 1, 3, 4}, {1, 2, 5, 6}, {0, 2, 7, 8}};
 };
 
+// The P2 node (vertices, then edge midpoints (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) in 3D,
+// (0,1) (0,2) (1,2) in 2D) at quadratic node k of face f: the face-gradient table GK[f][k]
+// equals the volume table VD at that node (the host checks it), so the lifts of all faces
+// can be summed at the P2 nodes and applied once.
+template <int D> struct FaceP2;
+template <> struct FaceP2<3> {
+  static constexpr int p[4][6] = {{2, 1, 3, 7, 8, 9}, {0, 2, 3, 5, 9, 6}, {1, 0, 3, 4, 6, 8},
+                                  {0, 1, 2, 4, 7, 5}};
+};
+template <> struct FaceP2<2> {
+  static constexpr int p[3][3] = {{0, 1, 3}, {1, 2, 5}, {2, 0, 4}};
+};
+
+// The nodes off face f (ascending), and per node the faces it lies on (bit f)
+template <int D> struct FaceOff {
+  static constexpr int NL = D == 3 ? 20 : 10, NF = D + 1, NFN = D == 3 ? 10 : 4;
+  static constexpr int NO = NL - NFN;
+  struct Tab {
+    int off[NF][NO];
+    int mask[NL];
+  };
+  static constexpr Tab make() {
+    Tab t{};
+    for (int n = 0; n < NL; ++n) t.mask[n] = 0;
+    for (int f = 0; f < NF; ++f) {
+      bool on[NL] = {};
+      for (int i = 0; i < NFN; ++i) {
+        on[FaceNodes<D>::v[f][i]] = true;
+        t.mask[FaceNodes<D>::v[f][i]] |= 1 << f;
+      }
+      int j = 0;
+      for (int n = 0; n < NL; ++n)
+        if (!on[n]) t.off[f][j++] = n;
+    }
+    return t;
+  }
+  static constexpr Tab tab = make();
+};
+
 // c . grad(u) at the NK quadratic nodes of face f (c = J^-T n_f), from the row xv
 template <typename T, int D, typename TT>
 __device__ __forceinline__ void face_dk(int f, const TT* __restrict__ fc, const T* xv,
