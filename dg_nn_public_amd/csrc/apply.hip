@@ -682,11 +682,10 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
       } else {
         fl.set_nodes(f, fc, xrf, dkn, -pe * wfe, 0.5f * wfe, 0.5f * wfe);
       }
-      if constexpr (APPLY) {
-        fl.finish(f, fc, c, acc);
-      } else {
-        // the sweep lifts all faces at once (step 2); this face's Q and c go to the
-        // slots of red[w] at the nodes off face w, which the residual never reads
+      {
+        // the faces' lifts are summed at the P2 nodes and applied once (the applies with
+        // the volume term in step 1c, the sweeps in their own step 1c): this face's Q and
+        // c go to the slots of red[w] at the nodes off face w, which the rows never read
         using O = FaceOff<D>;
         static_assert(O::NO >= F::NK + D, "room for Q and c off the face");
 #pragma unroll
@@ -696,22 +695,36 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
       }
     }
     if constexpr (APPLY) {
-      // ---- 1c. volume term, second half: h_w = sum_e G_we (VM g_e), r += VD_w^T h_w
-      __syncthreads();  // vtmp of every direction
+      // ---- 1c. volume term, second half, with the faces' lifts in direction w:
+      //   h_w = sum_e G_we (VM g_e) + W[.][w],  W[p][w] = sum_{f, k: node k of face f is p}
+      //   c_fw Q_fk,  r += VD_w^T h_w
+      __syncthreads();  // vtmp of every direction, every face's Q and c
       if (w < D) {
+        using O = FaceOff<D>;
         const float* vd = fc + F::VD + w * F::NP2 * NL;
         float gr[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) gr[d] = G[(w == d ? w : D - 1 + w + d) * nele + ec];
+        float hw[F::NP2];
+#pragma unroll
+        for (int i = 0; i < F::NP2; ++i) hw[i] = 0.0f;
+#pragma unroll
+        for (int ff = 0; ff < NF; ++ff) {
+          const float cw = red[ff][O::tab.off[ff][F::NK + w]][lane];
+#pragma unroll
+          for (int k2 = 0; k2 < F::NK; ++k2)
+            hw[FaceP2<D>::p[ff][k2]] += cw * red[ff][O::tab.off[ff][k2]][lane];
+        }
 #pragma unroll
         for (int i = 0; i < F::NP2; ++i) {
-          float h = 0.0f;
+          float h = hw[i];
 #pragma unroll
           for (int d = 0; d < D; ++d) h += gr[d] * vtmp[d][i][lane];
 #pragma unroll
           for (int n = 0; n < NL; ++n) acc[n] += vd[i * NL + n] * h;
         }
       }
+      __syncthreads();  // every wave has read the parked Q and c before red[w] is rewritten
     }
     // (red[w] is next written by this lane only, after its reads above: program order)
     if constexpr (APPLY) {
