@@ -505,6 +505,24 @@ __device__ __forceinline__ void sym_share_h(const uint32_t* __restrict__ H, int 
   sym_part_h<NL, P0>(H, nele, e, r, y, std::make_integer_sequence<int, P1 - P0>{});
 }
 
+// The same from the wave's words already in registers (hw[j] = pair P0 + j)
+template <int NL, int PAIR>
+__device__ __forceinline__ void sym_pair_hw(uint32_t wv, const float* r, float* y) {
+  sym_term_s<NL, 2 * PAIR>(half_lo(wv), r, y);
+  if constexpr (2 * PAIR + 1 < sym_size<NL>()) sym_term_s<NL, 2 * PAIR + 1>(half_hi(wv), r, y);
+}
+template <int NL, int P0, int... J>
+__device__ __forceinline__ void sym_part_hw(const uint32_t* hw, const float* r, float* y,
+                                            std::integer_sequence<int, J...>) {
+  (sym_pair_hw<NL, P0 + J>(hw[J], r, y), ...);
+}
+template <int NL, int NF, int W>
+__device__ __forceinline__ void sym_share_hw(const uint32_t* hw, const float* r, float* y) {
+  constexpr int NP = (sym_size<NL>() + 1) / 2;
+  constexpr int P0 = W * NP / NF, P1 = (W + 1) * NP / NF;
+  sym_part_hw<NL, P0>(hw, r, y, std::make_integer_sequence<int, P1 - P0>{});
+}
+
 template <int D, int MODE, bool HALF = false>
 __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void k_staged(
     int nele, const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ out,
@@ -584,6 +602,15 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
       for (int i = tid; i < NPERM * NL; i += NT) ptab[i] = nperm[i];
   }
   __syncthreads();
+  // (FP16 smoother) this wave's words of the packed smoother, loaded before the face work
+  // so that their latency hides behind it
+  constexpr int NPH = (sym_size<NL>() + 1) / 2, HW = (NPH + NF - 1) / NF;
+  uint32_t hw[HALF ? HW : 1];
+  if constexpr (HALF) {
+    const int p0 = w * NPH / NF, p1 = (w + 1) * NPH / NF;
+#pragma unroll
+    for (int j = 0; j < HW; ++j) hw[j] = p0 + j < p1 ? Sh[(size_t)(p0 + j) * nele + ec] : 0u;
+  }
   if constexpr (!ZERO) {
     float acc[NL];
 #pragma unroll
@@ -804,10 +831,10 @@ __global__ __launch_bounds__(64 * (D + 1), MODE == 5 || MODE == 6 ? 5 : 4) void 
       y[n] = 0.0f;
     }
     if constexpr (HALF) {
-      if (w == 0) sym_share_h<NL, NF, 0>(Sh, nele, ec, r, y);
-      else if (w == 1) sym_share_h<NL, NF, 1>(Sh, nele, ec, r, y);
-      else if (w == 2) sym_share_h<NL, NF, 2>(Sh, nele, ec, r, y);
-      else if constexpr (NF > 3) sym_share_h<NL, NF, 3>(Sh, nele, ec, r, y);
+      if (w == 0) sym_share_hw<NL, NF, 0>(hw, r, y);
+      else if (w == 1) sym_share_hw<NL, NF, 1>(hw, r, y);
+      else if (w == 2) sym_share_hw<NL, NF, 2>(hw, r, y);
+      else if constexpr (NF > 3) sym_share_hw<NL, NF, 3>(hw, r, y);
       const float sc = Shs[ec];
 #pragma unroll
       for (int n = 0; n < NL; ++n) y[n] *= sc;
