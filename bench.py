@@ -45,7 +45,7 @@ METRIC = "solves/s + p50/p99 latency at rtol 1e-10; DG apply GDoF/s, mixed vs FP
 # calibrated HBM bytes per launch of the FP32 single-pass smoothing sweep with the FP16
 # smoother copy (k_staged<3,5,true>, the 3D default): FETCH_SIZE x 2 + WRITE_SIZE from two rocprofv3
 # --pmc passes (profiles/), by element count
-PMC_SWEEP = {663552: 563.1e6}
+PMC_SWEEP = {663552: 577.1e6}
 
 
 def flops_per_elem(D, nloc, ngi, sngi, nface):
