@@ -1216,10 +1216,18 @@ static void start_graph_job(dgmi_ctx* c, std::vector<int64_t> voff) {
       p1cg_sparsity(c->cells.data(), c->nele, c->D, c->nverts, job->fina, job->cola);
       const int nc = (int)job->voff.size() - 1;
       job->lsfc.assign(nc, {});
+      job->pat.assign(nc, {});
       parallel_chunks(nc, [&](int, int64_t q0, int64_t q1) {
         for (int64_t q = q0; q < q1; ++q) {
-          const int64_t v0 = job->voff[q], nv = job->voff[q + 1] - v0;
-          hilbert_order(c->verts.data() + v0 * c->D, nv, c->D, job->lsfc[q]);
+          const int64_t v0 = job->voff[q], v1 = job->voff[q + 1];
+          hilbert_order(c->verts.data() + v0 * c->D, v1 - v0, c->D, job->lsfc[q]);
+          // the component's own pattern, then its SFC level patterns (the values wait for
+          // RAR)
+          const int64_t k0 = job->fina[v0];
+          std::vector<int64_t> fq(v1 - v0 + 1), cq(job->fina[v1] - k0);
+          for (int64_t i = v0; i <= v1; ++i) fq[i - v0] = job->fina[i] - k0;
+          for (int64_t k = k0; k < job->fina[v1]; ++k) cq[k - k0] = job->cola[k] - v0;
+          build_sfc_level_pattern(fq, cq, job->lsfc[q], job->pat[q]);
         }
       }, 1);
       job->ok = true;
@@ -1845,12 +1853,14 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   // the pattern and the Hilbert orders from the thread started at creation, when it
   // formed them for these component ranges
   std::vector<std::vector<int64_t>> lsfc;
+  std::vector<SfcLevelPattern> lpat;  // SFC level patterns from the same thread
   {
     GraphJob* job = join_graph_job(c);
     if (job && job->ok && job->voff == c->comp_voff) {
       mg.fina = std::move(job->fina);
       mg.cola = std::move(job->cola);
       lsfc = std::move(job->lsfc);
+      lpat = std::move(job->pat);
     } else {
       p1cg_sparsity(c->cells.data(), c->nele, D, c->nverts, mg.fina, mg.cola);
     }
@@ -1940,14 +1950,15 @@ int dgmi_mg_setup(dgmi_ctx* c, void* stream) {
   parallel_chunks(nc, [&](int, int64_t qa, int64_t qb) {
   for (int64_t q = qa; q < qb; ++q) {
     const int64_t v0 = c->comp_voff[q], v1 = c->comp_voff[q + 1];
-    std::vector<int64_t> fq(v1 - v0 + 1), cq(mg.fina[v1] - mg.fina[v0]);
-    std::vector<double> aq(cq.size());
-    for (int64_t i = v0; i <= v1; ++i) fq[i - v0] = mg.fina[i] - mg.fina[v0];
-    for (int64_t k = mg.fina[v0]; k < mg.fina[v1]; ++k) {
-      cq[k - mg.fina[v0]] = mg.cola[k] - v0;
-      aq[k - mg.fina[v0]] = mg.rar[k];
+    std::vector<double> aq(mg.rar.begin() + mg.fina[v0], mg.rar.begin() + mg.fina[v1]);
+    if ((int)lpat.size() == nc) {
+      fill_sfc_levels(lpat[q], aq, lay.lh[q]);  // the same sums, patterns formed earlier
+    } else {
+      std::vector<int64_t> fq(v1 - v0 + 1), cq(mg.fina[v1] - mg.fina[v0]);
+      for (int64_t i = v0; i <= v1; ++i) fq[i - v0] = mg.fina[i] - mg.fina[v0];
+      for (int64_t k = mg.fina[v0]; k < mg.fina[v1]; ++k) cq[k - mg.fina[v0]] = mg.cola[k] - v0;
+      build_sfc_levels(fq, cq, aq, lsfc[q], lay.lh[q]);
     }
-    build_sfc_levels(fq, cq, aq, lsfc[q], lay.lh[q]);
     const SfcLevelsHost& h = lay.lh[q];
     std::vector<int64_t> nnz(h.nlevel);
     for (int l = 0; l < h.nlevel; ++l) nnz[l] = (int64_t)h.col[l].size();

@@ -250,6 +250,7 @@ struct GraphJob {
   std::vector<int64_t> voff;  // the component vertex ranges the orders are for
   std::vector<int64_t> fina, cola;
   std::vector<std::vector<int64_t>> lsfc;  // local 1-based SFC positions per component
+  std::vector<SfcLevelPattern> pat;        // each component's SFC level patterns
   bool ok = false;
 };
 }  // namespace dgmi

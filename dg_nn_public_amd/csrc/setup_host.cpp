@@ -587,104 +587,148 @@ int nlevel_sfc(int64_t n) {
   return lev;
 }
 
-void build_sfc_levels(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
-                      const std::vector<double>& rar, const std::vector<int64_t>& sfc,
-                      SfcLevelsHost& lv) {
-  // Level 0 is RAR renumbered into SFC order, each row sorted by column.  Row I of level
-  // l+1 gathers rows 2I and 2I+1 of level l with columns c -> c >> 1 (positions p and q of
-  // one level are summed into p >> 1, q >> 1: the aggregation of compact_fcode.f90:7-225);
-  // entries with one column are summed in the order: row 2I by column, then row 2I+1 by
-  // column.  Rows are independent: each pass counts, then fills, in parallel.
+void build_sfc_level_pattern(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
+                             const std::vector<int64_t>& sfc, SfcLevelPattern& pat) {
+  // Level 0 is the pattern renumbered into SFC order, each row sorted by column.  Row I of
+  // level l+1 gathers rows 2I and 2I+1 of level l with columns c -> c >> 1 (positions p
+  // and q of one level are summed into p >> 1, q >> 1: the aggregation of
+  // compact_fcode.f90:7-225); entries with one column are summed in the order: row 2I by
+  // column, then row 2I+1 by column.  Rows are independent: each pass counts, then fills,
+  // in parallel.  Only the sources of each entry are recorded here.
   const int64_t n0 = (int64_t)fina.size() - 1;
-  lv.nlevel = nlevel_sfc(n0);
-  lv.n.assign(lv.nlevel, 0);
-  lv.rowptr.assign(lv.nlevel, {});
-  lv.col.assign(lv.nlevel, {});
-  lv.val.assign(lv.nlevel, {});
-  lv.diag.assign(lv.nlevel, {});
-  auto finish = [&](int l) {
-    const int64_t n = lv.n[l];
-    const std::vector<int32_t>& rp = lv.rowptr[l];
-    lv.diag[l].assign(n, 0.0);
-    parallel_chunks(n, [&](int, int64_t a, int64_t b) {
-      for (int64_t i = a; i < b; ++i)
-        for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
-          if (lv.col[l][k] == i) lv.diag[l][i] = lv.val[l][k];
-    });
-  };
+  pat.nlevel = nlevel_sfc(n0);
+  pat.n.assign(pat.nlevel, 0);
+  pat.rowptr.assign(pat.nlevel, {});
+  pat.col.assign(pat.nlevel, {});
+  pat.sptr.assign(pat.nlevel, {});
+  pat.src.assign(pat.nlevel, {});
   // ---- level 0
   {
     std::vector<int64_t> node(n0);  // node at each SFC position
     for (int64_t i = 0; i < n0; ++i) node[sfc[i] - 1] = i;
-    std::vector<int32_t>& rp = lv.rowptr[0];
+    std::vector<int32_t>& rp = pat.rowptr[0];
     rp.assign(n0 + 1, 0);
     for (int64_t p = 0; p < n0; ++p) rp[p + 1] = (int32_t)(fina[node[p] + 1] - fina[node[p]]);
     for (int64_t p = 0; p < n0; ++p) rp[p + 1] += rp[p];
-    lv.col[0].resize(rp[n0]);
-    lv.val[0].resize(rp[n0]);
+    pat.col[0].resize(rp[n0]);
+    pat.perm0.resize(rp[n0]);
     parallel_chunks(n0, [&](int, int64_t a, int64_t b) {
-      std::vector<std::pair<int32_t, double>> row;
+      std::vector<std::pair<int32_t, int32_t>> row;
       for (int64_t p = a; p < b; ++p) {
         const int64_t i = node[p];
         row.clear();
         for (int64_t c = fina[i]; c < fina[i + 1]; ++c)
-          row.push_back({(int32_t)(sfc[cola[c]] - 1), rar[c]});
+          row.push_back({(int32_t)(sfc[cola[c]] - 1), (int32_t)c});
         std::stable_sort(row.begin(), row.end(),
                          [](const auto& x, const auto& y) { return x.first < y.first; });
         for (size_t k = 0; k < row.size(); ++k) {
-          lv.col[0][rp[p] + k] = row[k].first;
-          lv.val[0][rp[p] + k] = row[k].second;
+          pat.col[0][rp[p] + k] = row[k].first;
+          pat.perm0[rp[p] + k] = row[k].second;
         }
       }
     });
-    lv.n[0] = n0;
-    finish(0);
+    pat.n[0] = n0;
   }
   // ---- coarser levels
-  for (int l = 1; l < lv.nlevel; ++l) {
-    const int64_t nf = lv.n[l - 1], nc = (nf - 1) / 2 + 1;
-    const std::vector<int32_t>& frp = lv.rowptr[l - 1];
-    const std::vector<int32_t>& fcol = lv.col[l - 1];
-    const std::vector<double>& fval = lv.val[l - 1];
-    std::vector<int32_t>& rp = lv.rowptr[l];
+  for (int l = 1; l < pat.nlevel; ++l) {
+    const int64_t nf = pat.n[l - 1], nc = (nf - 1) / 2 + 1;
+    const std::vector<int32_t>& frp = pat.rowptr[l - 1];
+    const std::vector<int32_t>& fcol = pat.col[l - 1];
+    std::vector<int32_t>& rp = pat.rowptr[l];
     rp.assign(nc + 1, 0);
-    // the coarse row of the pair as (column, value), merged; count then fill
-    auto merge = [&](int64_t I, std::vector<std::pair<int32_t, double>>& row) {
+    // the coarse row of the pair as (column, source entry), sorted by column (stable)
+    auto merge = [&](int64_t I, std::vector<std::pair<int32_t, int32_t>>& row) {
       row.clear();
       for (int64_t i = 2 * I; i < std::min(2 * I + 2, nf); ++i)
-        for (int32_t k = frp[i]; k < frp[i + 1]; ++k) row.push_back({fcol[k] >> 1, fval[k]});
+        for (int32_t k = frp[i]; k < frp[i + 1]; ++k) row.push_back({fcol[k] >> 1, k});
       std::stable_sort(row.begin(), row.end(),
                        [](const auto& x, const auto& y) { return x.first < y.first; });
-      size_t w = 0;
-      for (size_t k = 0; k < row.size(); ++k) {
-        if (w > 0 && row[w - 1].first == row[k].first) row[w - 1].second += row[k].second;
-        else row[w++] = row[k];
-      }
-      row.resize(w);
     };
+    auto distinct = [](const std::vector<std::pair<int32_t, int32_t>>& row) {
+      int32_t m = 0;
+      for (size_t k = 0; k < row.size(); ++k)
+        if (k == 0 || row[k].first != row[k - 1].first) ++m;
+      return m;
+    };
+    std::vector<int32_t> scount(nc + 1, 0);  // source entries per coarse row
     parallel_chunks(nc, [&](int, int64_t a, int64_t b) {
-      std::vector<std::pair<int32_t, double>> row;
+      std::vector<std::pair<int32_t, int32_t>> row;
       for (int64_t I = a; I < b; ++I) {
         merge(I, row);
-        rp[I + 1] = (int32_t)row.size();
+        rp[I + 1] = distinct(row);
+        scount[I + 1] = (int32_t)row.size();
       }
     });
-    for (int64_t I = 0; I < nc; ++I) rp[I + 1] += rp[I];
-    lv.col[l].resize(rp[nc]);
-    lv.val[l].resize(rp[nc]);
+    for (int64_t I = 0; I < nc; ++I) {
+      rp[I + 1] += rp[I];
+      scount[I + 1] += scount[I];
+    }
+    pat.col[l].resize(rp[nc]);
+    pat.sptr[l].resize((size_t)rp[nc] + 1);
+    pat.src[l].resize(scount[nc]);
+    pat.sptr[l][rp[nc]] = scount[nc];
     parallel_chunks(nc, [&](int, int64_t a, int64_t b) {
-      std::vector<std::pair<int32_t, double>> row;
+      std::vector<std::pair<int32_t, int32_t>> row;
       for (int64_t I = a; I < b; ++I) {
         merge(I, row);
+        int32_t w = rp[I] - 1, s = scount[I];
         for (size_t k = 0; k < row.size(); ++k) {
-          lv.col[l][rp[I] + k] = row[k].first;
-          lv.val[l][rp[I] + k] = row[k].second;
+          if (k == 0 || row[k].first != row[k - 1].first) {
+            ++w;
+            pat.col[l][w] = row[k].first;
+            pat.sptr[l][w] = s;
+          }
+          pat.src[l][s++] = row[k].second;
         }
       }
     });
-    lv.n[l] = nc;
-    finish(l);
+    pat.n[l] = nc;
   }
+}
+
+void fill_sfc_levels(SfcLevelPattern& pat, const std::vector<double>& rar, SfcLevelsHost& lv) {
+  lv.nlevel = pat.nlevel;
+  lv.n = pat.n;
+  lv.rowptr = std::move(pat.rowptr);
+  lv.col = std::move(pat.col);
+  lv.val.assign(lv.nlevel, {});
+  lv.diag.assign(lv.nlevel, {});
+  for (int l = 0; l < lv.nlevel; ++l) {
+    const int64_t n = lv.n[l];
+    const std::vector<int32_t>& rp = lv.rowptr[l];
+    std::vector<double>& v = lv.val[l];
+    v.resize(rp[n]);
+    if (l == 0) {
+      parallel_chunks((int64_t)v.size(), [&](int, int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; ++k) v[k] = rar[pat.perm0[k]];
+      }, 1 << 16);
+    } else {
+      const std::vector<double>& fv = lv.val[l - 1];
+      const std::vector<int32_t>& sp = pat.sptr[l];
+      const std::vector<int32_t>& sr = pat.src[l];
+      parallel_chunks((int64_t)v.size(), [&](int, int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; ++k) {
+          double s = fv[sr[sp[k]]];
+          for (int32_t j = sp[k] + 1; j < sp[k + 1]; ++j) s += fv[sr[j]];
+          v[k] = s;
+        }
+      }, 1 << 16);
+    }
+    lv.diag[l].assign(n, 0.0);
+    parallel_chunks(n, [&](int, int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+          if (lv.col[l][k] == i) lv.diag[l][i] = v[k];
+    });
+  }
+}
+
+void build_sfc_levels(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
+                      const std::vector<double>& rar, const std::vector<int64_t>& sfc,
+                      SfcLevelsHost& lv) {
+  SfcLevelPattern pat;
+  build_sfc_level_pattern(fina, cola, sfc, pat);
+  fill_sfc_levels(pat, rar, lv);
 }
 
 }  // namespace dgmi

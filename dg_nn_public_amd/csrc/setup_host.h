@@ -113,5 +113,20 @@ struct SfcLevelsHost {
 void build_sfc_levels(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
                       const std::vector<double>& rar, const std::vector<int64_t>& sfc,
                       SfcLevelsHost& lv);
+// The same in two steps: the level patterns need only the P1CG pattern and the SFC order
+// (so they can be formed before RAR is known), and record which entries each entry sums;
+// filling the values from RAR then repeats build_sfc_levels' sums in its order.
+struct SfcLevelPattern {
+  int nlevel = 0;
+  std::vector<int64_t> n;
+  std::vector<std::vector<int32_t>> rowptr, col;
+  std::vector<int32_t> perm0;  // level-0 entry k takes the input CSR entry perm0[k]
+  // level l >= 1: entry k sums the level l-1 entries src[l][sptr[l][k] .. sptr[l][k+1])
+  std::vector<std::vector<int32_t>> sptr, src;
+};
+void build_sfc_level_pattern(const std::vector<int64_t>& fina, const std::vector<int64_t>& cola,
+                             const std::vector<int64_t>& sfc, SfcLevelPattern& pat);
+// moves the pattern's rowptr and col into lv
+void fill_sfc_levels(SfcLevelPattern& pat, const std::vector<double>& rar, SfcLevelsHost& lv);
 
 }  // namespace dgmi
