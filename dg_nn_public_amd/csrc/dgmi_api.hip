@@ -1408,7 +1408,9 @@ static int ctx_create_batch_impl(const dgmi_mesh_desc* d, int32_t ncomp, const i
   if (x_parts) {
     DGMI_TRY(ctx_create_impl(d, x_parts, eoff, ncomp, &c, voff, ncomp));
   } else {
-    DGMI_TRY(dgmi_ctx_create(d, &c));
+    const int64_t off[2] = {0, d->nele};
+    const double* parts[1] = {d->x_ref_in};
+    DGMI_TRY(ctx_create_impl(d, parts, off, 1, &c, voff, ncomp));
   }
   c->ncomp = ncomp;
   c->comp_eoff.assign(eoff, eoff + ncomp + 1);
